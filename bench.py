@@ -1,0 +1,165 @@
+"""Benchmark: grid-point RK3 substeps per second of the plain-FFT RK3 step
+on one MI355X (FP64 plain by default; --precision mixed for FP32 FFTs).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                  [--precision fp64|mixed] [--no-cpu]
+
+Prints ONE JSON line.  value = grid points (N1*N2*ny summed over channels)
+x RK3 substeps / wall second, with the fields resident in HBM, timed between
+two device synchronisations around K graph-replayed steps.  `roofline` is for
+the dominant kernel, from its HIP-event duration measured here and the
+algorithmic bytes that kernel must move.  `cpu_baseline` times the CPU
+oracle (a from-scratch FP64 port of the reference algorithm) on a bounded
+sample of the same workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-pt RK3 substeps/s, FP64 plain vs mixed/batched/closure; HBM % of peak"
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["k_zinv", "k_xprod", "k_zfwd", "k_rhs", "k_solve"]
+
+
+def algorithmic_bytes(cfg, precision):
+    """Bytes each kernel of one substep must read and write at least once
+    (intermediates of the nonlinear pipeline at their stored precision)."""
+    N1, N2, ny = cfg["N1"], cfg["N2"], cfg["ny"]
+    Ng2 = cfg["Ngal2"]
+    nxh = N1 // 2 + 1
+    ncol = nxh * N2
+    nyu, nyv = ny + 1, ny
+    nplu, nplv = nyu - 2, nyv - 2
+    row = ncol * 16
+    es = 8 if precision == "mixed" else 16
+    plane = Ng2 * nxh * es
+    hz = (nplu + nplv) * 3 * plane
+    gx = (nplu * 4 + nplv * 2) * plane
+    zinv = (2 * nyu + nyv) * row + hz
+    xprod = hz + gx
+    zfwd = gx + (3 * nplu + 3 * nplv) * row
+    rhs = ((2 * nyu + nyv + (nyu - 2)) + (2 * nplu + nplv) * 2 + (2 * nyv + nplu)) * row \
+        + (2 * nyu + nyv + 2 * nplu + nplv) * row
+    solve = ((2 * nyu + nyv) + (nyu - 2)) * row + ((2 * nyu + nyv) + 2 * (nyu - 2)) * row
+    return dict(k_zinv=zinv, k_xprod=xprod, k_zfwd=zfwd, k_rhs=rhs, k_solve=solve)
+
+
+def baseline_model_bytes(cfg, precision):
+    """B_sub = 78 S + 57 F of the roofline model the metric is quoted with."""
+    N1, N2, ny = cfg["N1"], cfg["N2"], cfg["ny"]
+    S = (N1 // 2 + 1) * N2 * ny * 16
+    F = (cfg["Ngal1"] + 2) * cfg["Ngal2"] * ny * (4 if precision == "mixed" else 8)
+    return 78 * S + 57 * F
+
+
+def cpu_baseline(cfg, seconds=12.0, max_sub=60):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import init_fields, oracle_for
+    from oracle import pyoracle as po
+    from rnl_multiblock_nofft_amd import Solver  # noqa: F401
+
+    cores = os.cpu_count() or 1
+    cores = min(cores, 16)
+    po.set_threads(cores)
+    o = oracle_for(cfg)
+    g = o.grid()
+    f = init_fields(cfg, g, seed=1234)
+    for name in ("u1", "u2", "u3", "p"):
+        o.set(name, f[name])
+    o.init_flowrate()
+    o.substep(1)  # warm plans / threads
+    n = 0
+    t0 = time.perf_counter()
+    while n < max_sub:
+        o.substep(n % 3 + 1)
+        n += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    gp = cfg["N1"] * cfg["N2"] * cfg["ny"]
+    return dict(value=gp * n / dt, unit="grid-pt RK3 substeps/s", cores=cores, kind="port",
+                sample="%d RK3 substeps of one %dx%dx%d channel (FP64 CPU oracle, OpenMP, %d threads), %.1f s"
+                       % (n, cfg["N1"], cfg["N2"], cfg["ny"], cores, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="ridges_64x64x153")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-substeps", type=int, default=30)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs one channel ensemble on one GPU; --gpus %d is not "
+              "supported (multi-GPU sharding is described in DESIGN.md, not implemented)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from rnl_multiblock_nofft_amd import Solver, configs, initial
+
+    torch.cuda.set_device(0)
+    cfgs = configs.WORKLOADS[args.workload]()
+    s = Solver(cfgs, precision=args.precision)
+    for q, c in enumerate(cfgs):
+        kmax = min(8, c["N1"] // 2 - 1, c["N2"] // 2 - 1)
+        f = initial.perturbed_poiseuille(c, s.grid(q), 1234 + q, kmax=kmax)
+        for name in ("u1", "u2", "u3", "p"):
+            s.upload(q, name, f[name])
+    s.init_flowrate()
+    s.step(args.warmup)
+    s.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    gp = sum(c["N1"] * c["N2"] * c["ny"] for c in cfgs)
+    value = gp * 3 * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+
+    # live per-kernel durations (HIP events on the library's stream)
+    nsub = max(3, args.kernel_substeps)
+    ms = s.time_kernels(nsub) / nsub
+    alg = {k: 0 for k in KERNELS}
+    for c in cfgs:
+        for k, v in algorithmic_bytes(c, args.precision).items():
+            alg[k] += v
+    dom = KERNELS[int(max(range(5), key=lambda q: ms[q]))]
+    dom_ms = float(ms[KERNELS.index(dom)])
+    achieved = alg[dom] / (dom_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel=dom,
+                    kernel_ms=round(dom_ms, 5), alg_bytes=alg[dom],
+                    kernels_ms={k: round(float(ms[q]), 5) for q, k in enumerate(KERNELS)})
+    bmodel = sum(baseline_model_bytes(c, args.precision) for c in cfgs)
+    hbm_pct = 100.0 * bmodel * (3 * args.steps / elapsed) / (HBM_PEAK_GBS * 1e9)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(max(cfgs, key=lambda c: c["N1"] * c["N2"] * c["ny"]))
+    rec = dict(metric=METRIC, value=round(value, 1), unit="grid-pt RK3 substeps/s", n_gpus=1,
+               steps=args.steps, warmup=args.warmup, ms_per_step=round(ms_per_step, 5),
+               higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="f64" if args.precision == "fp64" else "f32-fft/f64-solve", data="synthetic",
+               config=dict(workload=args.workload, channels=len(cfgs),
+                           grids=sorted({"%dx%dx%d" % (c["N1"], c["N2"], c["ny"]) for c in cfgs}),
+                           precision=args.precision, walls="smooth", dt=cfgs[0]["dt"],
+                           grid_points=gp, hbm_pct_of_peak_model=round(hbm_pct, 3)),
+               roofline=roofline, cpu_baseline=cpu)
+    print(json.dumps(rec), flush=True)
+    s.close()
+
+
+if __name__ == "__main__":
+    main()

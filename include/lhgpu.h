@@ -1,0 +1,152 @@
+/*
+ * lhgpu.h -- C-ABI of liblhgpu.so, the MI355X (gfx950) implementation of the
+ * plain-FFT RK3 time step of LittleHarsh (channel-flow DNS).
+ *
+ * Plain pointers and sizes only.  Host arrays are passed in the reference's
+ * Fortran layout `complex(8) f(jlim1:jlim2, ncol)` (y contiguous, then the
+ * spectral column), which is what the Fortran shims hand over with c_loc.
+ * Column c holds x-mode i = c % (N1/2+1) and z-index k = c / (N1/2+1), the
+ * order proc_lims_columns builds for equal band resolutions
+ * (start.f90:1574-1600).  Row ranges:
+ *   LH_U1, LH_U3:   j = 0..ny     (ny+1 rows incl. ghost cells)
+ *   LH_U2:          j = 0..ny-1   (v faces incl. walls)
+ *   LH_P:           j = 1..ny-1   (pressure centres)
+ * where ny is the number of v points wall to wall (input.in `ny`).
+ *
+ * Every entry point returns 0 on success and a negative LH_E* code on error;
+ * lh_last_error() gives a message.  A context is single-threaded: calls on
+ * one context must not overlap.
+ */
+#ifndef LHGPU_H
+#define LHGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LH_OK 0
+#define LH_EINVAL (-1)   /* invalid argument / shape */
+#define LH_EHIP (-2)     /* HIP runtime error */
+#define LH_ENOMEM (-3)   /* device allocation failed */
+#define LH_ESTATE (-4)   /* call not valid in the context's state */
+
+typedef struct lh_config {
+  int N1, N2;          /* Fourier modes in x, z (N = 2*(Ngal/3)); even */
+  int Ngal1, Ngal2;    /* physical (3/2-padded) grid; 3*2^k or 2^k */
+  int ny;              /* v points wall to wall (input.in `ny`) */
+  double alp, bet;     /* Lx = 2pi/alp, Lz = 2pi/bet */
+  double Re;           /* 1/nu as the reference uses it (C1 = aRK/Re) */
+  int grid_kind;       /* 0: polynomial y = a(j-q)+b(j-q)^p (ygrid); 1: tanh */
+  double dyq;          /* polynomial grid: dymax/dymin */
+  int ppp;             /* polynomial grid exponent */
+  double gamma;        /* tanh grid: y = tanh(gamma(2j/(ny-1)-1))/tanh(gamma) */
+  double bslpu1, bslpu3; /* Navier slip lengths of the ghost update */
+  int flag_ctpress;    /* 0: constant flow rate; 1: constant pressure gradient */
+  double mpgx;         /* mean pressure gradient (initial / prescribed) */
+  double dt;           /* fixed time step */
+  int precision;       /* LH_PREC_FP64 or LH_PREC_MIXED */
+} lh_config;
+
+#define LH_PREC_FP64 0   /* everything in FP64 */
+#define LH_PREC_MIXED 1  /* FFTs/products in FP32 (FP64 twiddles), solves FP64 */
+
+/* field ids for upload/download */
+#define LH_U1 0
+#define LH_U2 1
+#define LH_U3 2
+#define LH_P 3
+/* internal arrays, downloadable for diagnostics (rows as in the oracle):
+ * LH_NU1/LH_NU3 rows 1..ny-1, LH_NU2 rows 1..ny-2, LH_DIV/LH_PSI rows 1..ny-1 */
+#define LH_NU1 4
+#define LH_NU2 5
+#define LH_NU3 6
+#define LH_R1 7    /* rhs u1 + dt*du1, rows 0..ny (ghost rows 0) */
+#define LH_R2 8    /* rhs u2 + dt*du2, rows 0..ny-1 */
+#define LH_R3 9
+#define LH_PSI 10
+#define LH_DIV 11
+
+typedef struct lh_ctx lh_ctx;
+
+const char *lh_last_error(void);
+
+/* Create `nchan` channels (ensemble members) in one context.  cfgs[c] may
+ * differ in every field except `precision`.  Replaces start()'s setup
+ * (start.f90:36-691): y-grid (ygrid :693), wavenumbers (def_k :1079), RK3
+ * coefficients (:419-433), grid weights (:537-559) and the Poisson operator
+ * (LU_buildP, tridLU_3D.f90:333).  Fields start at zero. */
+int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out);
+int lh_ctx_destroy(lh_ctx *ctx);
+
+/* Replace channel `ch`'s y-grid (yu[0..ny], yv[0..ny-1], dthdyu[0..ny],
+ * dthdyv[0..ny-1]) as the Fortran host's ygrid computed it; derived
+ * coefficients (dyu2i, dyv2i, grid weights) are rebuilt. */
+int lh_set_grid(lh_ctx *ctx, int ch, const double *yu, const double *yv,
+                const double *dthdyu, const double *dthdyv);
+int lh_get_grid(lh_ctx *ctx, int ch, double *yu, double *yv, double *dthdyu,
+                double *dthdyv);
+
+/* Host <-> device copies of one field of channel `ch`, reference layout,
+ * re/im interleaved.  The device stores fields mode-major (each y row holds
+ * all columns contiguously); the on-device LDS-tiled transpose replaces the
+ * plane/column exchanges modes_to_planes_UVP / planes_to_modes_UVP
+ * (FOU3D.f90:1000-1098, 2295-2442). */
+int lh_field_upload(lh_ctx *ctx, int ch, int field, const double *reim);
+int lh_field_download(lh_ctx *ctx, int ch, int field, double *reim);
+
+/* scalars: out[0..7] = t, dt, mpgx, QxT, Qx, dtv, 0, iter */
+int lh_get_scalars(lh_ctx *ctx, int ch, double *out8);
+int lh_set_scalars(lh_ctx *ctx, int ch, double t, double mpgx, double QxT, int iter);
+/* QxT := flow rate of the current u1 mean mode (getini, start.f90:2166-2171;
+ * flowrateIm littleharsh.f90:435) */
+int lh_init_flowrate(lh_ctx *ctx);
+
+/* One RK3 substep kRK in 1..3 for every channel (littleharsh.f90:176-215):
+ * RHS0_u1/u2/u3, nonlinear (FOU3D.f90:41, with ops_in_planes2's plain FFT
+ * products FOU3D.f90:2090-2251), solveU_W, solveV, meanflow_ctU, solveP,
+ * v_corr. */
+int lh_substep(lh_ctx *ctx, int kRK);
+/* nsteps full RK3 steps (main loop littleharsh.f90:172-228), replayed from a
+ * captured HIP graph after the first step. */
+int lh_step(lh_ctx *ctx, int nsteps);
+int lh_sync(lh_ctx *ctx);
+/* Measurement: run `nsub` substeps (kRK cycling 1,2,3 from kRK = 1) with the
+ * five kernels of a substep launched one by one between HIP events on the
+ * context's stream; ms_out[0..4] receive the summed milliseconds of the
+ * z-inverse, x-product, z-forward, right-hand-side and column-solve kernels.
+ * Advances the state like lh_substep.  No counterpart in the reference. */
+int lh_time_kernels(lh_ctx *ctx, int nsub, double *ms_out5);
+
+/* Phases of a substep, for diagnostics and per-phase parity checks:
+ * LH_PHASE_NONLINEAR: interp_u/v + modes_to_planes + ops_in_planes2 +
+ *                     planes_to_modes (x/z part of N and the y-fluxes)
+ * LH_PHASE_RHS:       der_y of the fluxes, RHS0_*, du update, rhs = u+dt*du
+ * LH_PHASE_SOLVE:     solveU_W, solveV, meanflow_ctU, solveP, v_corr */
+#define LH_PHASE_NONLINEAR 1
+#define LH_PHASE_RHS 2
+#define LH_PHASE_SOLVE 3
+int lh_run_phase(lh_ctx *ctx, int phase, int kRK);
+
+/* Stand-alone batched transforms on device planes, replacing four_to_phys_du
+ * (FOU3D.f90:766) and phys_to_four_du (FOU3D.f90:2253) with their FFTPACK
+ * conventions (cft_buff.f:1-65, rft_buff.f:24-90).  planes: nplanes device
+ * planes of (Ngal1+2) x Ngal2 doubles, Fortran order, transformed in place. */
+int lh_four_to_phys_planes(int N1, int Ngal1, int Ngal2, double *d_planes,
+                           int nplanes, void *stream);
+int lh_phys_to_four_planes(int N1, int Ngal1, int Ngal2, double *d_planes,
+                           int nplanes, void *stream);
+
+/* Stand-alone batched tridiagonal solves on device data in the reference
+ * layout x(jlo:jhi, ncol) (re/im interleaved), replacing LUsolU_W / LUsolV
+ * (tridLU_3D.f90:55,135) for one RK3 substep of channel `ch`: solves
+ * (I - beta(D2 + k^2)) x = rhs with the reference's wall rows. */
+int lh_lusol_uw(lh_ctx *ctx, int ch, int kRK, const double *d_rhsu,
+                const double *d_rhsw, double *d_u, double *d_w);
+int lh_lusol_v(lh_ctx *ctx, int ch, int kRK, const double *d_rhsv, double *d_v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

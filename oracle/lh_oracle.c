@@ -1,0 +1,1000 @@
+/*
+ * lh_oracle.c -- CPU FP64 restatement of the plain-FFT RK3 step of LittleHarsh
+ * (bananapancake12/RNL_MULTIBLOCK_NOFFT), for one channel whose bands share
+ * one x/z resolution, run as a single domain (the reference's np=1 view).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Each function cites the reference routine it restates.  The nonlinear term
+ * is the dormant plain path `ops_in_planes2` (FOU3D.f90:2090-2251), which the
+ * reference would run if FOU3D.f90:245 called it instead of the triad list.
+ * Deviations, each deliberate and tested:
+ *   - the flux planes uv/wv at the two wall faces and vv at the two ghost
+ *     centres are zero (the active path zeroes them, FOU3D.f90:1752,1891-1894;
+ *     the plain path never writes them);
+ *   - dtc_calc takes the max over the transformed physical planes, without
+ *     the one-plane shift the reference's explicit-shape dummy causes
+ *     (FOU3D.f90:708-716 vs start.f90:567-569);
+ *   - optional fixed dt (the reference only has the adaptive dt).
+ */
+#include "lh_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+/* ------------------------------------------------------------------------ */
+/* Mixed-radix complex FFT (own implementation: recursive decimation in time,
+ * radices 4, 2, 3, 5).  Sign convention as FFTPACK: forward e^{-i}, backward
+ * e^{+i}, both unscaled (cfftf/cfftb, cft_buff.f:186-279,631-700).          */
+
+#define MAXFAC 32
+typedef struct {
+  int n, nf;
+  int fac[MAXFAC];
+  cplx *w; /* w[m] = exp(-2 pi i m / n) */
+} fftplan;
+
+static void plan_init(fftplan *p, int n) {
+  p->n = n;
+  p->nf = 0;
+  int r = n;
+  const int tryr[4] = {4, 2, 3, 5};
+  for (int t = 0; t < 4 && r > 1; t++) {
+    while (r % tryr[t] == 0 && !(tryr[t] == 4 && r % 4 != 0)) {
+      p->fac[p->nf++] = tryr[t];
+      r /= tryr[t];
+    }
+  }
+  if (r != 1) { /* generic leftover factor (not used by this solver) */
+    p->fac[p->nf++] = r;
+  }
+  p->w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  for (int m = 0; m < n; m++) {
+    double a = -2.0 * M_PI * (double)m / (double)n;
+    p->w[m] = cos(a) + I * sin(a);
+  }
+}
+static void plan_free(fftplan *p) { free(p->w); }
+
+static void fft_rec(const cplx *in, cplx *out, int n, int sin_, const int *fac,
+                    const fftplan *pl, int wstr, int sign) {
+  if (n == 1) {
+    out[0] = in[0];
+    return;
+  }
+  int p = fac[0], m = n / p;
+  for (int q = 0; q < p; q++)
+    fft_rec(in + (size_t)q * sin_, out + (size_t)q * m, m, sin_ * p, fac + 1, pl,
+            wstr * p, sign);
+  const int N = pl->n;
+  cplx t[MAXFAC > 64 ? MAXFAC : 64];
+  for (int k = 0; k < m; k++) {
+    for (int q = 0; q < p; q++) {
+      cplx w = pl->w[((size_t)q * k * wstr) % N];
+      if (sign > 0) w = conj(w);
+      t[q] = out[(size_t)q * m + k] * w;
+    }
+    for (int s2 = 0; s2 < p; s2++) {
+      cplx acc = 0;
+      for (int q = 0; q < p; q++) {
+        cplx w = pl->w[((size_t)q * s2 * (N / p)) % N];
+        if (sign > 0) w = conj(w);
+        acc += t[q] * w;
+      }
+      out[(size_t)s2 * m + k] = acc;
+    }
+  }
+}
+
+/* plan cache (small n only) */
+#define PLANCACHE 16
+static fftplan g_plans[PLANCACHE];
+static int g_nplans = 0;
+static const fftplan *get_plan(int n) {
+  static __thread const fftplan *last = NULL;
+  if (last && last->n == n) return last;
+  const fftplan *r = NULL;
+#pragma omp critical(lho_plan)
+  {
+    for (int i = 0; i < g_nplans; i++)
+      if (g_plans[i].n == n) r = &g_plans[i];
+    if (!r && g_nplans < PLANCACHE) {
+      plan_init(&g_plans[g_nplans], n);
+      r = &g_plans[g_nplans++];
+    }
+  }
+  last = r;
+  return r;
+}
+
+/* x (n complex, unit stride) -> transform in place; sign -1 forward, +1 back */
+static void cfft(cplx *x, int n, int sign) {
+  const fftplan *pl = get_plan(n);
+  cplx buf[1024];
+  cplx *tmp = n <= 1024 ? buf : (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  fft_rec(x, tmp, n, 1, pl->fac, pl, 1, sign);
+  memcpy(x, tmp, sizeof(cplx) * (size_t)n);
+  if (tmp != buf) free(tmp);
+}
+
+/* cft(c,ise,isa,m,iopt,aux), cft_buff.f:1-65: m complex vectors of length n,
+ * element stride ise and vector stride isa (both in reals); forward (iopt<0)
+ * scaled by 1/n, inverse unscaled. */
+void lho_cft(double *c, int n, int ise, int isa, int m, int iopt) {
+  cplx buf[1024];
+  cplx *v = n <= 1024 ? buf : (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  for (int k = 0; k < m; k++) {
+    double *base = c + (size_t)k * isa;
+    for (int j = 0; j < n; j++) v[j] = base[(size_t)j * ise] + I * base[(size_t)j * ise + 1];
+    cfft(v, n, iopt < 0 ? -1 : +1);
+    double s = iopt < 0 ? 1.0 / n : 1.0;
+    for (int j = 0; j < n; j++) {
+      base[(size_t)j * ise] = creal(v[j]) * s;
+      base[(size_t)j * ise + 1] = cimag(v[j]) * s;
+    }
+  }
+  if (v != buf) free(v);
+}
+
+/* rft(c,isa,m,iopt,aux), rft_buff.f:24-90: m real vectors of length n stored
+ * with stride isa >= n+2.  Fourier side: cc(k) = c(2k+1) + i c(2k+2), k=0..n/2.
+ * Forward: cc = DFT(x)/n with imag(cc(0)) = imag(cc(n/2)) = 0.  Inverse:
+ * x_j = sum over the Hermitian extension of cc, with imag(cc(0)) and
+ * imag(cc(n/2)) ignored; c(n+1), c(n+2) are left as they were. */
+void lho_rft(double *c, int n, int isa, int m, int iopt) {
+  cplx buf[1024];
+  cplx *v = n <= 1024 ? buf : (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  for (int k = 0; k < m; k++) {
+    double *x = c + (size_t)k * isa;
+    if (iopt < 0) {
+      for (int j = 0; j < n; j++) v[j] = x[j];
+      cfft(v, n, -1);
+      double s = 1.0 / n;
+      x[0] = creal(v[0]) * s;
+      x[1] = 0.0;
+      for (int q = 1; q < n / 2; q++) {
+        x[2 * q] = creal(v[q]) * s;
+        x[2 * q + 1] = cimag(v[q]) * s;
+      }
+      x[n] = creal(v[n / 2]) * s;
+      x[n + 1] = 0.0;
+    } else {
+      v[0] = x[0];
+      for (int q = 1; q < n / 2; q++) {
+        v[q] = x[2 * q] + I * x[2 * q + 1];
+        v[n - q] = conj(v[q]);
+      }
+      v[n / 2] = x[n];
+      cfft(v, n, +1);
+      for (int j = 0; j < n; j++) x[j] = creal(v[j]);
+    }
+  }
+  if (v != buf) free(v);
+}
+
+/* ------------------------------------------------------------------------ */
+
+struct lho_state {
+  lho_cfg c;
+  int nyu, nyv;     /* rows of u grid (ny+1) and v grid (ny) */
+  int nxh, ncol;    /* N1/2+1, nxh*N2 */
+  int igal, kgal;   /* Ngal1+2, Ngal2 */
+  double *yu, *yv, *dthdyu, *dthdyv, *dyu2i, *dyv2i; /* dyu2i[3*j+m] */
+  double gw1, gw2, gwbc1, gwbc3, gwi1, gwi2;
+  double ddthetavi;
+  double *kx, *kz, *k2x, *k2z; /* kx[i]: imag part of k1F_x; kz[k] of k1F_z */
+  double aRK[3], bRK[3], gRK[3], cRK[3], dRK[3];
+  double t, dt, dti, mpgx, QxT, Qx, dtv, dtc;
+  int iter;
+  /* fields: f[col*rows + j] with rows = nyu (u and p like) or nyv (v like) */
+  cplx *u1, *u2, *u3, *p, *Nu1, *Nu2, *Nu3, *du1, *du2, *du3, *psi, *div;
+  cplx *u1i, *u2i, *u3i, *uvf, *wvf, *vvc;
+  double *dg; /* prefactored Poisson LU, dg[(col*nyu + j)*3 + m] */
+};
+
+static double ipow(double x, int n) {
+  double r = 1.0;
+  for (int i = 0; i < n; i++) r *= x;
+  return r;
+}
+
+int lho_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+/* y(theta) and dy/dtheta of the wall-normal mapping */
+static void ymap(const lho_cfg *c, double aaa, double bbb, double qqq, double th,
+                 double *y, double *dthdy) {
+  if (c->grid_kind == 1) {
+    double g = c->gamma, tg = tanh(g);
+    double s = g * (th / qqq - 1.0);
+    double ts = tanh(s);
+    *y = ts / tg;
+    *dthdy = 1.0 / ((g / (qqq * tg)) * (1.0 - ts * ts));
+  } else {
+    int p = c->ppp;
+    double x = th - qqq;
+    *y = aaa * x + bbb * ipow(x, p);
+    *dthdy = 1.0 / (aaa + p * bbb * ipow(x, p - 1));
+  }
+}
+
+/* derived quantities once yu,yv,dthdyu,dthdyv are known (start.f90:537-559,
+ * :943-987 second-derivative coefficients, :456 dtv) */
+static void finish_grid(lho_state *s) {
+  int nyu = s->nyu, nyv = s->nyv, nn = nyv - 1; /* nn: last v index */
+  double *yu = s->yu, *yv = s->yv;
+  double *a = s->dyu2i, *b = s->dyv2i;
+  /* j = 0 */
+  a[0] = 0.0;
+  a[1] = -2.0 / ((yu[1] - yu[0]) * (yu[1] - yu[0]));
+  a[2] = 1.0 / ((yu[1] - yu[0]) * (yu[1] - yu[0]));
+  b[0] = 0.0;
+  b[1] = -2.0 / ((yv[1] - yv[0]) * (yv[1] - yv[0]));
+  b[2] = 1.0 / ((yv[1] - yv[0]) * (yv[1] - yv[0]));
+  for (int j = 1; j <= nn - 1; j++) {
+    a[3 * j + 0] = 2.0 / ((yu[j] - yu[j - 1]) * (yu[j + 1] - yu[j - 1]));
+    a[3 * j + 1] = -2.0 / ((yu[j + 1] - yu[j]) * (yu[j] - yu[j - 1]));
+    a[3 * j + 2] = 2.0 / ((yu[j + 1] - yu[j]) * (yu[j + 1] - yu[j - 1]));
+    b[3 * j + 0] = 2.0 / ((yv[j] - yv[j - 1]) * (yv[j + 1] - yv[j - 1]));
+    b[3 * j + 1] = -2.0 / ((yv[j + 1] - yv[j]) * (yv[j] - yv[j - 1]));
+    b[3 * j + 2] = 2.0 / ((yv[j + 1] - yv[j]) * (yv[j + 1] - yv[j - 1]));
+  }
+  {
+    int j = nn;
+    a[3 * j + 0] = 2.0 / ((yu[j] - yu[j - 1]) * (yu[j + 1] - yu[j - 1]));
+    a[3 * j + 1] = -2.0 / ((yu[j + 1] - yu[j]) * (yu[j] - yu[j - 1]));
+    a[3 * j + 2] = 2.0 / ((yu[j + 1] - yu[j]) * (yu[j + 1] - yu[j - 1]));
+    double d = (yv[j] - yv[j - 1]) * (yv[j] - yv[j - 1]);
+    b[3 * j + 0] = 1.0 / d;
+    b[3 * j + 1] = -2.0 / d;
+    b[3 * j + 2] = 0.0;
+  }
+  {
+    int j = nn + 1;
+    double d = (yu[j] - yu[j - 1]) * (yu[j] - yu[j - 1]);
+    a[3 * j + 0] = 1.0 / d;
+    a[3 * j + 1] = -2.0 / d;
+    a[3 * j + 2] = 0.0;
+  }
+  int ju = nyu - 1, jv = nyv - 1;
+  s->gw1 = -(yv[0] - yu[0]) / (yv[0] - yu[1]);
+  s->gw2 = (yu[ju] - yv[jv]) / (yv[jv] - yu[ju - 1]);
+  double ddv = s->ddthetavi;
+  s->gwbc1 = ((yu[0] - yv[0]) + s->c.bslpu1 * ddv * s->dthdyv[0] * (yu[1] - yu[0])) /
+             ((yu[1] - yv[0]) + s->c.bslpu1 * ddv * s->dthdyv[0] * (yu[1] - yu[0]));
+  s->gwbc3 = ((yu[0] - yv[0]) + s->c.bslpu3 * ddv * s->dthdyv[0] * (yu[1] - yu[0])) /
+             ((yu[1] - yv[0]) + s->c.bslpu3 * ddv * s->dthdyv[0] * (yu[1] - yu[0]));
+  s->gwi1 = (yu[0] - yu[1]) / (yv[0] - yu[1]);
+  s->gwi2 = (yu[ju] - yu[ju - 1]) / (yv[jv] - yu[ju - 1]);
+  s->dtv = s->c.Re / (-s->k2x[s->c.N1 / 2] - s->k2z[s->c.N2 / 2 - 1] +
+                      4.0 / ((yv[1] - yv[0]) * (yv[1] - yv[0])));
+}
+
+static void build_dg(lho_state *s);
+
+lho_state *lho_create(const lho_cfg *cfg) {
+  lho_state *s = (lho_state *)calloc(1, sizeof(lho_state));
+  s->c = *cfg;
+  int ny = cfg->ny;
+  s->nyv = ny;
+  s->nyu = ny + 1;
+  s->nxh = cfg->N1 / 2 + 1;
+  s->ncol = s->nxh * cfg->N2;
+  s->igal = cfg->Ngal1 + 2;
+  s->kgal = cfg->Ngal2;
+  s->ddthetavi = 1.0;
+  s->yu = calloc(s->nyu, sizeof(double));
+  s->yv = calloc(s->nyv, sizeof(double));
+  s->dthdyu = calloc(s->nyu, sizeof(double));
+  s->dthdyv = calloc(s->nyv, sizeof(double));
+  s->dyu2i = calloc(3 * (size_t)s->nyu, sizeof(double));
+  s->dyv2i = calloc(3 * (size_t)s->nyv, sizeof(double));
+  /* def_k, start.f90:1079-1136 */
+  s->kx = calloc(s->nxh, sizeof(double));
+  s->k2x = calloc(s->nxh, sizeof(double));
+  s->kz = calloc(cfg->N2, sizeof(double));
+  s->k2z = calloc(cfg->N2, sizeof(double));
+  for (int i = 0; i < s->nxh; i++) {
+    s->kx[i] = cfg->alp * i;
+    double t = cfg->alp * i;
+    s->k2x[i] = -(t * t);
+  }
+  for (int k = 1; k <= cfg->N2 / 2; k++) {
+    double t = cfg->bet * (k - 1);
+    s->kz[k - 1] = t;
+    s->k2z[k - 1] = -(t * t);
+  }
+  for (int k = cfg->N2 / 2 + 1; k <= cfg->N2; k++) {
+    double t = cfg->bet * (cfg->N2 - k + 1);
+    s->kz[k - 1] = -t;
+    s->k2z[k - 1] = -(t * t);
+  }
+  /* RK3 coefficients (Le & Moin), start.f90:419-433 */
+  const double a[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0};
+  const double g[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+  const double cc[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+  const double d[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
+  for (int q = 0; q < 3; q++) {
+    s->aRK[q] = a[q];
+    s->bRK[q] = a[q];
+    s->gRK[q] = g[q];
+    s->cRK[q] = cc[q];
+    s->dRK[q] = d[q];
+  }
+  /* ygrid, start.f90:693-1077 ('Wall' grid, no immersed region) */
+  int nn = ny - 1;
+  double qqq = nn / 2.0, aaa = 0, bbb = 0;
+  if (cfg->grid_kind == 0) {
+    int p = cfg->ppp;
+    double dyq = cfg->dyq;
+    bbb = (dyq - 1.0) /
+          (qqq * (1.0 - ipow(qqq, p - 1) +
+                  dyq * ((1.0 - qqq) * ipow(qqq, p - 1) - ipow(1.0 - qqq, p))));
+    aaa = (1.0 - bbb * ipow(qqq, p)) / qqq;
+  }
+  ymap(cfg, aaa, bbb, qqq, -0.5, &s->yu[0], &s->dthdyu[0]);
+  for (int j = 0; j <= nn; j++) {
+    ymap(cfg, aaa, bbb, qqq, (double)j, &s->yv[j], &s->dthdyv[j]);
+    ymap(cfg, aaa, bbb, qqq, j + 0.5, &s->yu[j + 1], &s->dthdyu[j + 1]);
+  }
+  finish_grid(s);
+
+  size_t nu = (size_t)s->ncol * s->nyu, nv = (size_t)s->ncol * s->nyv;
+  s->u1 = calloc(nu, sizeof(cplx));
+  s->u3 = calloc(nu, sizeof(cplx));
+  s->u2 = calloc(nv, sizeof(cplx));
+  s->p = calloc(nu, sizeof(cplx));
+  s->Nu1 = calloc(nu, sizeof(cplx));
+  s->Nu3 = calloc(nu, sizeof(cplx));
+  s->Nu2 = calloc(nv, sizeof(cplx));
+  s->du1 = calloc(nu, sizeof(cplx));
+  s->du3 = calloc(nu, sizeof(cplx));
+  s->du2 = calloc(nv, sizeof(cplx));
+  s->psi = calloc(nu, sizeof(cplx));
+  s->div = calloc(nu, sizeof(cplx));
+  s->u1i = calloc(nv, sizeof(cplx));
+  s->u3i = calloc(nv, sizeof(cplx));
+  s->u2i = calloc(nu, sizeof(cplx));
+  s->uvf = calloc(nv, sizeof(cplx));
+  s->wvf = calloc(nv, sizeof(cplx));
+  s->vvc = calloc(nu, sizeof(cplx));
+  s->dg = calloc(3 * nu, sizeof(double));
+  build_dg(s);
+  s->mpgx = cfg->mpgx;
+  s->dt = cfg->dt;
+  s->dti = cfg->dt > 0 ? 1.0 / cfg->dt : 0.0;
+  return s;
+}
+
+void lho_destroy(lho_state *s) {
+  if (!s) return;
+  void *ptrs[] = {s->yu, s->yv, s->dthdyu, s->dthdyv, s->dyu2i, s->dyv2i, s->kx, s->kz,
+                  s->k2x, s->k2z, s->u1, s->u2, s->u3, s->p, s->Nu1, s->Nu2, s->Nu3,
+                  s->du1, s->du2, s->du3, s->psi, s->div, s->u1i, s->u2i, s->u3i,
+                  s->uvf, s->wvf, s->vvc, s->dg};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+  free(s);
+}
+
+int lho_nrows_u(const lho_state *s) { return s->nyu; }
+int lho_ncol(const lho_state *s) { return s->ncol; }
+
+void lho_get_grid(const lho_state *s, double *yu, double *yv, double *dthdyu,
+                  double *dthdyv, double *dyu2i, double *dyv2i) {
+  if (yu) memcpy(yu, s->yu, sizeof(double) * s->nyu);
+  if (yv) memcpy(yv, s->yv, sizeof(double) * s->nyv);
+  if (dthdyu) memcpy(dthdyu, s->dthdyu, sizeof(double) * s->nyu);
+  if (dthdyv) memcpy(dthdyv, s->dthdyv, sizeof(double) * s->nyv);
+  if (dyu2i) memcpy(dyu2i, s->dyu2i, sizeof(double) * 3 * s->nyu);
+  if (dyv2i) memcpy(dyv2i, s->dyv2i, sizeof(double) * 3 * s->nyv);
+}
+
+void lho_set_grid(lho_state *s, const double *yu, const double *yv, const double *dthdyu,
+                  const double *dthdyv) {
+  memcpy(s->yu, yu, sizeof(double) * s->nyu);
+  memcpy(s->yv, yv, sizeof(double) * s->nyv);
+  memcpy(s->dthdyu, dthdyu, sizeof(double) * s->nyu);
+  memcpy(s->dthdyv, dthdyv, sizeof(double) * s->nyv);
+  finish_grid(s);
+  build_dg(s);
+}
+
+void lho_get_scalars(const lho_state *s, double *o) {
+  o[0] = s->t;
+  o[1] = s->dt;
+  o[2] = s->mpgx;
+  o[3] = s->QxT;
+  o[4] = s->Qx;
+  o[5] = s->dtv;
+  o[6] = s->dtc;
+  o[7] = s->iter;
+}
+
+void lho_set_scalars(lho_state *s, double t, double mpgx, double QxT, int iter) {
+  s->t = t;
+  s->mpgx = mpgx;
+  s->QxT = QxT;
+  s->iter = iter;
+}
+
+static cplx *field_ptr(const lho_state *s, int fid, int *rows) {
+  switch (fid) {
+    case LHO_U1: *rows = s->nyu; return s->u1;
+    case LHO_U2: *rows = s->nyv; return s->u2;
+    case LHO_U3: *rows = s->nyu; return s->u3;
+    case LHO_P: *rows = s->nyu; return s->p;
+    case LHO_NU1: *rows = s->nyu; return s->Nu1;
+    case LHO_NU2: *rows = s->nyv; return s->Nu2;
+    case LHO_NU3: *rows = s->nyu; return s->Nu3;
+    case LHO_DU1: *rows = s->nyu; return s->du1;
+    case LHO_DU2: *rows = s->nyv; return s->du2;
+    case LHO_DU3: *rows = s->nyu; return s->du3;
+    case LHO_PSI: *rows = s->nyu; return s->psi;
+    case LHO_DIV: *rows = s->nyu; return s->div;
+  }
+  *rows = 0;
+  return NULL;
+}
+
+int lho_field_rows(const lho_state *s, int fid, int *jlo, int *jhi) {
+  switch (fid) {
+    case LHO_U1: case LHO_U3: case LHO_DU1: case LHO_DU3:
+      *jlo = 0; *jhi = s->nyu - 1; break;
+    case LHO_U2: case LHO_DU2:
+      *jlo = 0; *jhi = s->nyv - 1; break;
+    case LHO_P: case LHO_PSI: case LHO_DIV: case LHO_NU1: case LHO_NU3:
+      *jlo = 1; *jhi = s->nyu - 2; break;
+    case LHO_NU2:
+      *jlo = 1; *jhi = s->nyv - 2; break;
+    default: return -1;
+  }
+  return *jhi - *jlo + 1;
+}
+
+void lho_set_field(lho_state *s, int fid, const double *reim) {
+  int rows, jlo, jhi;
+  cplx *f = field_ptr(s, fid, &rows);
+  int n = lho_field_rows(s, fid, &jlo, &jhi);
+  for (int c = 0; c < s->ncol; c++)
+    for (int j = jlo; j <= jhi; j++) {
+      size_t q = (size_t)c * n + (j - jlo);
+      f[(size_t)c * rows + j] = reim[2 * q] + I * reim[2 * q + 1];
+    }
+}
+
+void lho_get_field(const lho_state *s, int fid, double *reim) {
+  int rows, jlo, jhi;
+  cplx *f = field_ptr(s, fid, &rows);
+  int n = lho_field_rows(s, fid, &jlo, &jhi);
+  for (int c = 0; c < s->ncol; c++)
+    for (int j = jlo; j <= jhi; j++) {
+      size_t q = (size_t)c * n + (j - jlo);
+      reim[2 * q] = creal(f[(size_t)c * rows + j]);
+      reim[2 * q + 1] = cimag(f[(size_t)c * rows + j]);
+    }
+}
+
+/* flowrateIm / flowrateRe, littleharsh.f90:435-474 */
+static double flowrate_c(const lho_state *s, const cplx *u) {
+  double Q = 0.0;
+  for (int j = 1; j <= s->nyu - 2; j++) Q = Q + (s->yv[j] - s->yv[j - 1]) * creal(u[j]);
+  return Q;
+}
+static double flowrate_r(const lho_state *s, const double *u) {
+  double Q = 0.0;
+  for (int j = 1; j <= s->nyu - 2; j++) Q = Q + (s->yv[j] - s->yv[j - 1]) * u[j];
+  return Q;
+}
+
+void lho_init_flowrate(lho_state *s) {
+  s->Qx = flowrate_c(s, s->u1);
+  if (s->c.flag_ctpress != 1) s->QxT = s->Qx;
+}
+
+/* ------------------------------------------------------------------------ */
+/* RHS0_u1/u2/u3 + laplacian_U/V, littleharsh.f90:274-340,635-788          */
+void lho_rhs0(lho_state *s, int kRK) {
+  int q = kRK - 1;
+  double C1 = s->aRK[q] / s->c.Re, C2 = -s->cRK[q];
+  int nyu = s->nyu, nyv = s->nyv;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < s->ncol; c++) {
+    int i = c % s->nxh, k = c / s->nxh;
+    cplx C3x = -s->dRK[q] * (I * s->kx[i]);
+    cplx C3z = -s->dRK[q] * (I * s->kz[k]);
+    const cplx *u1 = s->u1 + (size_t)c * nyu, *u3 = s->u3 + (size_t)c * nyu;
+    const cplx *u2 = s->u2 + (size_t)c * nyv, *p = s->p + (size_t)c * nyu;
+    cplx *d1 = s->du1 + (size_t)c * nyu, *d3 = s->du3 + (size_t)c * nyu;
+    cplx *d2 = s->du2 + (size_t)c * nyv;
+    const cplx *N1 = s->Nu1 + (size_t)c * nyu, *N3 = s->Nu3 + (size_t)c * nyu;
+    const cplx *N2 = s->Nu2 + (size_t)c * nyv;
+    double k2x = s->k2x[i], k2z = s->k2z[k];
+    for (int j = 1; j <= nyu - 2; j++) {
+      const double *a = s->dyu2i + 3 * j;
+      cplx L1 = a[0] * u1[j - 1] + (k2x + k2z + a[1]) * u1[j] + a[2] * u1[j + 1];
+      cplx L3 = a[0] * u3[j - 1] + (k2x + k2z + a[1]) * u3[j] + a[2] * u3[j + 1];
+      d1[j] = C1 * L1 + C2 * N1[j] + C3x * p[j];
+      d3[j] = C1 * L3 + C2 * N3[j] + C3z * p[j];
+    }
+    for (int j = 1; j <= nyv - 2; j++) {
+      const double *a = s->dyv2i + 3 * j;
+      cplx L2 = a[0] * u2[j - 1] + (k2x + k2z + a[1]) * u2[j] + a[2] * u2[j + 1];
+      double C3 = -s->dRK[q] * s->ddthetavi * s->dthdyv[j];
+      d2[j] = C1 * L2 + C2 * N2[j] + C3 * (p[j + 1] - p[j]);
+    }
+  }
+  /* mean pressure gradient on mode (0,0): littleharsh.f90:670-676 */
+  double Cm = -s->dRK[q] * s->mpgx;
+  for (int j = 1; j <= nyu - 2; j++) s->du1[j] = s->du1[j] + Cm;
+}
+
+/* ------------------------------------------------------------------------ */
+/* plane helpers (FOU3D.f90)                                               */
+
+/* modes_to_planes_UVP (FOU3D.f90:1000-1098) for one plane: column (i,k) ->
+ * P(2i+1, krow), P(2i+2, krow), krow = k (k<=N2/2) else k+Ngal2-N2 (1-based) */
+static void modes_to_plane(const lho_state *s, const cplx *f, int rows, int j, double *P) {
+  memset(P, 0, sizeof(double) * (size_t)s->igal * s->kgal);
+  int N2 = s->c.N2, dk = N2 - s->c.Ngal2;
+  for (int c = 0; c < s->ncol; c++) {
+    int i = c % s->nxh, k = c / s->nxh + 1;
+    int krow = k <= N2 / 2 ? k : k - dk;
+    cplx v = f[(size_t)c * rows + j];
+    P[(size_t)(krow - 1) * s->igal + 2 * i] = creal(v);
+    P[(size_t)(krow - 1) * s->igal + 2 * i + 1] = cimag(v);
+  }
+}
+static void plane_to_modes(const lho_state *s, const double *P, cplx *f, int rows, int j) {
+  int N2 = s->c.N2, dk = N2 - s->c.Ngal2;
+  for (int c = 0; c < s->ncol; c++) {
+    int i = c % s->nxh, k = c / s->nxh + 1;
+    int krow = k <= N2 / 2 ? k : k - dk;
+    f[(size_t)c * rows + j] = P[(size_t)(krow - 1) * s->igal + 2 * i] +
+                              I * P[(size_t)(krow - 1) * s->igal + 2 * i + 1];
+  }
+}
+
+/* four_to_phys_du, FOU3D.f90:766-784 */
+void lho_four_to_phys_plane(const lho_state *s, double *P) {
+  int igal = s->igal, kgal = s->kgal;
+  for (int ii = 0; ii < igal; ii++) P[(size_t)(kgal / 2) * igal + ii] = 0.0;
+  lho_cft(P, s->c.Ngal2, igal, 2, (s->c.N1 + 2) / 2, +1);
+  lho_rft(P, s->c.Ngal1, igal, kgal, +1);
+}
+/* phys_to_four_du, FOU3D.f90:2253-2272 */
+void lho_phys_to_four_plane(const lho_state *s, double *P) {
+  int igal = s->igal, kgal = s->kgal;
+  lho_rft(P, s->c.Ngal1, igal, kgal, -1);
+  lho_cft(P, s->c.Ngal2, igal, 2, (s->c.N1 + 2) / 2, -1);
+  for (int ii = 0; ii < igal; ii++) P[(size_t)(kgal / 2) * igal + ii] = 0.0;
+}
+
+/* der_x + der_z and their sum (FOU3D.f90:476-527,552-593, 2160-2165):
+ * out = ikx*A + ikz*B on the retained modes, zero elsewhere */
+static void derxz_sum(const lho_state *s, const double *A, const double *B, double *out) {
+  int igal = s->igal, kgal = s->kgal, N1 = s->c.N1, N2 = s->c.N2;
+  int Ngal1 = s->c.Ngal1;
+  int dk2 = N2 - kgal;
+  for (int k = 1; k <= kgal; k++) {
+    int live = (k <= N2 / 2) || (k >= kgal - N2 / 2 + 2);
+    int kzi = k <= N2 / 2 ? k : k + dk2; /* 1-based index into k1F_z */
+    for (int i = 0; i <= Ngal1 / 2; i++) {
+      size_t o = (size_t)(k - 1) * igal + 2 * i;
+      if (live && i <= N1 / 2) {
+        cplx a = A[o] + I * A[o + 1], b = B[o] + I * B[o + 1];
+        cplx dx = (I * s->kx[i]) * a;
+        cplx dz = (I * s->kz[kzi - 1]) * b;
+        cplx r = dx + dz;
+        out[o] = creal(r);
+        out[o + 1] = cimag(r);
+      } else {
+        out[o] = 0.0;
+        out[o + 1] = 0.0;
+      }
+    }
+  }
+}
+
+/* interp_u / interp_v, FOU3D.f90:427-474 */
+static void interp_all(lho_state *s) {
+  int nyu = s->nyu, nyv = s->nyv;
+  const double *yu = s->yu, *yv = s->yv;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < s->ncol; c++) {
+    const cplx *u1 = s->u1 + (size_t)c * nyu, *u3 = s->u3 + (size_t)c * nyu;
+    const cplx *u2 = s->u2 + (size_t)c * nyv;
+    cplx *o1 = s->u1i + (size_t)c * nyv, *o3 = s->u3i + (size_t)c * nyv;
+    cplx *o2 = s->u2i + (size_t)c * nyu;
+    for (int j = 0; j <= nyv - 1; j++) {
+      o1[j] = ((yv[j] - yu[j]) * u1[j + 1] + (yu[j + 1] - yv[j]) * u1[j]) / (yu[j + 1] - yu[j]);
+      o3[j] = ((yv[j] - yu[j]) * u3[j + 1] + (yu[j + 1] - yv[j]) * u3[j]) / (yu[j + 1] - yu[j]);
+    }
+    for (int j = 1; j <= nyu - 2; j++)
+      o2[j] = ((yu[j] - yv[j - 1]) * u2[j] + (yv[j] - yu[j]) * u2[j - 1]) / (yv[j] - yv[j - 1]);
+    o2[0] = s->gwi1 * (u2[0] - o2[1]) + o2[1];
+    o2[nyu - 1] = s->gwi2 * (u2[nyv - 1] - o2[nyu - 2]) + o2[nyu - 2];
+  }
+}
+
+/* nonlinear (FOU3D.f90:41-410) with ops_in_planes2 (FOU3D.f90:2090-2251) */
+void lho_nonlinear(lho_state *s, int kRK) {
+  int nyu = s->nyu, nyv = s->nyv, igal = s->igal, kgal = s->kgal;
+  int Ngal1 = s->c.Ngal1, Ngal2 = s->c.Ngal2;
+  size_t psz = (size_t)igal * kgal;
+  interp_all(s);
+  double m1 = 0.0, m2 = 1e-6, m3 = 0.0; /* dtc_calc maxima */
+  /* u-planes: uu, uw, vv, ww -> Nu1, Nu3 (PL part) and vv */
+#pragma omp parallel reduction(max : m1, m3)
+  {
+    double *P1 = malloc(sizeof(double) * psz), *P2 = malloc(sizeof(double) * psz);
+    double *P3 = malloc(sizeof(double) * psz), *Q = malloc(sizeof(double) * psz);
+    double *R = malloc(sizeof(double) * psz);
+#pragma omp for schedule(dynamic)
+    for (int j = 1; j <= nyu - 2; j++) {
+      modes_to_plane(s, s->u1, nyu, j, P1);
+      modes_to_plane(s, s->u2i, nyu, j, P2);
+      modes_to_plane(s, s->u3, nyu, j, P3);
+      lho_four_to_phys_plane(s, P1);
+      lho_four_to_phys_plane(s, P2);
+      lho_four_to_phys_plane(s, P3);
+      for (int k = 0; k < Ngal2; k++)
+        for (int i = 0; i < Ngal1; i++) {
+          size_t o = (size_t)k * igal + i;
+          double a = P1[o], b = P2[o], c = P3[o];
+          if (fabs(a) > m1) m1 = fabs(a);
+          if (fabs(c) > m3) m3 = fabs(c);
+          Q[o] = a * c;  /* uw */
+          P1[o] = a * a; /* uu */
+          P2[o] = b * b; /* vv */
+          P3[o] = c * c; /* ww */
+        }
+      lho_phys_to_four_plane(s, P1);
+      lho_phys_to_four_plane(s, Q);
+      lho_phys_to_four_plane(s, P2);
+      lho_phys_to_four_plane(s, P3);
+      derxz_sum(s, P1, Q, R);
+      plane_to_modes(s, R, s->Nu1, nyu, j);
+      derxz_sum(s, Q, P3, R);
+      plane_to_modes(s, R, s->Nu3, nyu, j);
+      plane_to_modes(s, P2, s->vvc, nyu, j);
+    }
+    free(P1); free(P2); free(P3); free(Q); free(R);
+  }
+  /* v-planes: uv, wv -> Nu2 (PL part), uv, wv */
+#pragma omp parallel reduction(max : m2)
+  {
+    double *P1 = malloc(sizeof(double) * psz), *P2 = malloc(sizeof(double) * psz);
+    double *P3 = malloc(sizeof(double) * psz), *R = malloc(sizeof(double) * psz);
+#pragma omp for schedule(dynamic)
+    for (int j = 1; j <= nyv - 2; j++) {
+      modes_to_plane(s, s->u1i, nyv, j, P1);
+      modes_to_plane(s, s->u2, nyv, j, P2);
+      modes_to_plane(s, s->u3i, nyv, j, P3);
+      lho_four_to_phys_plane(s, P1);
+      lho_four_to_phys_plane(s, P2);
+      lho_four_to_phys_plane(s, P3);
+      for (int k = 0; k < Ngal2; k++)
+        for (int i = 0; i < Ngal1; i++) {
+          size_t o = (size_t)k * igal + i;
+          double v = P2[o];
+          double av = fabs(v) * s->dthdyv[j];
+          if (av > m2) m2 = av;
+          P1[o] = P1[o] * v; /* uv */
+          P3[o] = P3[o] * v; /* wv */
+        }
+      lho_phys_to_four_plane(s, P1);
+      lho_phys_to_four_plane(s, P3);
+      derxz_sum(s, P1, P3, R);
+      plane_to_modes(s, R, s->Nu2, nyv, j);
+      plane_to_modes(s, P1, s->uvf, nyv, j);
+      plane_to_modes(s, P3, s->wvf, nyv, j);
+    }
+    free(P1); free(P2); free(P3); free(R);
+  }
+  /* wall faces / ghost centres of the flux planes are zero */
+  for (int c = 0; c < s->ncol; c++) {
+    s->uvf[(size_t)c * nyv] = 0;
+    s->uvf[(size_t)c * nyv + nyv - 1] = 0;
+    s->wvf[(size_t)c * nyv] = 0;
+    s->wvf[(size_t)c * nyv + nyv - 1] = 0;
+    s->vvc[(size_t)c * nyu] = 0;
+    s->vvc[(size_t)c * nyu + nyu - 1] = 0;
+  }
+  /* y-derivatives of the fluxes, FOU3D.f90:266-283 (der_yu_h, der_yv_h) */
+  double ddv = s->ddthetavi;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < s->ncol; c++) {
+    cplx *N1 = s->Nu1 + (size_t)c * nyu, *N3 = s->Nu3 + (size_t)c * nyu;
+    cplx *N2 = s->Nu2 + (size_t)c * nyv;
+    const cplx *uv = s->uvf + (size_t)c * nyv, *wv = s->wvf + (size_t)c * nyv;
+    const cplx *vv = s->vvc + (size_t)c * nyu;
+    for (int j = 1; j <= nyu - 2; j++) {
+      N1[j] = N1[j] + (uv[j] - uv[j - 1]) * s->dthdyu[j] * ddv;
+      N3[j] = N3[j] + (wv[j] - wv[j - 1]) * s->dthdyu[j] * ddv;
+    }
+    for (int j = 1; j <= nyv - 2; j++) N2[j] = N2[j] + (vv[j + 1] - vv[j]) * s->dthdyv[j] * ddv;
+  }
+  /* time step, FOU3D.f90:288-297 and dtc_calc FOU3D.f90:699-736 */
+  if (kRK == 1) {
+    int N1m = s->c.N1, N2m = s->c.N2;
+    s->dtc = fmin(fmin(1.0 / (s->c.alp * (N1m / 2) * m1), 1.0 / (s->c.bet * (N2m / 2) * m3)),
+                  1.0 / (m2 * s->ddthetavi));
+    if (!s->c.fixed_dt) s->dt = fmin(2.5 * s->dtv, s->c.CFL * s->dtc);
+    s->dti = 1.0 / s->dt;
+    s->t = s->t + s->dt;
+  }
+  /* du += -gRK*N, FOU3D.f90:398-410 */
+  double C1 = -s->gRK[kRK - 1];
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < s->ncol; c++) {
+    for (int j = 1; j <= nyu - 2; j++) {
+      s->du1[(size_t)c * nyu + j] += C1 * s->Nu1[(size_t)c * nyu + j];
+      s->du3[(size_t)c * nyu + j] += C1 * s->Nu3[(size_t)c * nyu + j];
+    }
+    for (int j = 1; j <= nyv - 2; j++) s->du2[(size_t)c * nyv + j] += C1 * s->Nu2[(size_t)c * nyv + j];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* LU_dec (tridLU_3D.f90:437-474): a[3*j+0] lower, [1] diag, [2] upper      */
+static void lu_dec(double *a, int n) {
+  a[1] = 1.0 / a[1];
+  for (int j = 1; j < n; j++) {
+    a[3 * (j - 1) + 2] = a[3 * (j - 1) + 2] * a[3 * (j - 1) + 1];
+    a[3 * j + 1] = 1.0 / (a[3 * j + 1] - a[3 * j + 0] * a[3 * (j - 1) + 2]);
+  }
+}
+static void lu_sol_c(const double *a, const cplx *rhs, cplx *x, int n) {
+  x[0] = rhs[0] * a[1];
+  for (int j = 1; j < n; j++) x[j] = (rhs[j] - a[3 * j] * x[j - 1]) * a[3 * j + 1];
+  for (int j = n - 2; j >= 0; j--) x[j] = x[j] - a[3 * j + 2] * x[j + 1];
+}
+
+/* solveU_W + LUsolU_W + LU_buildU, littleharsh.f90:791-828,
+ * tridLU_3D.f90:55-132,257-291 */
+void lho_solve_uw(lho_state *s, int kRK) {
+  int nyu = s->nyu;
+  double beta = s->bRK[kRK - 1] * s->dt / s->c.Re;
+#pragma omp parallel
+  {
+    double *a = malloc(sizeof(double) * 3 * nyu);
+#pragma omp for schedule(static)
+    for (int c = 0; c < s->ncol; c++) {
+      int i = c % s->nxh, k = c / s->nxh;
+      cplx *u = s->u1 + (size_t)c * nyu, *w = s->u3 + (size_t)c * nyu;
+      cplx *du = s->du1 + (size_t)c * nyu, *dw = s->du3 + (size_t)c * nyu;
+      du[0] = 0;
+      dw[0] = 0;
+      for (int j = 1; j <= nyu - 2; j++) {
+        du[j] = u[j] + s->dt * du[j];
+        dw[j] = w[j] + s->dt * dw[j];
+      }
+      du[nyu - 1] = 0;
+      dw[nyu - 1] = 0;
+      double k2x = s->k2x[i], k2z = s->k2z[k];
+      a[0] = 0.0;
+      a[1] = 1.0;
+      a[2] = s->gw1;
+      for (int j = 1; j <= nyu - 2; j++) {
+        const double *d = s->dyu2i + 3 * j;
+        a[3 * j] = -beta * d[0];
+        a[3 * j + 1] = 1.0 - beta * (d[1] + k2x + k2z);
+        a[3 * j + 2] = -beta * d[2];
+      }
+      a[3 * (nyu - 1)] = s->gw2;
+      a[3 * (nyu - 1) + 1] = 1.0;
+      a[3 * (nyu - 1) + 2] = 0.0;
+      lu_dec(a, nyu);
+      lu_sol_c(a, du, u, nyu);
+      lu_sol_c(a, dw, w, nyu);
+    }
+    free(a);
+  }
+}
+
+/* solveV + LUsolV + LU_buildV, littleharsh.f90:877-907, tridLU_3D.f90:135-189,294-329 */
+void lho_solve_v(lho_state *s, int kRK) {
+  int nyv = s->nyv;
+  double beta = s->bRK[kRK - 1] * s->dt / s->c.Re;
+#pragma omp parallel
+  {
+    double *a = malloc(sizeof(double) * 3 * nyv);
+#pragma omp for schedule(static)
+    for (int c = 0; c < s->ncol; c++) {
+      int i = c % s->nxh, k = c / s->nxh;
+      cplx *v = s->u2 + (size_t)c * nyv, *dv = s->du2 + (size_t)c * nyv;
+      dv[0] = 0;
+      for (int j = 1; j <= nyv - 2; j++) dv[j] = v[j] + s->dt * dv[j];
+      dv[nyv - 1] = 0;
+      double k2x = s->k2x[i], k2z = s->k2z[k];
+      a[0] = 0.0;
+      a[1] = 1.0;
+      a[2] = 0.0;
+      for (int j = 1; j <= nyv - 2; j++) {
+        const double *d = s->dyv2i + 3 * j;
+        a[3 * j] = -beta * d[0];
+        a[3 * j + 1] = 1.0 - beta * (d[1] + k2x + k2z);
+        a[3 * j + 2] = -beta * d[2];
+      }
+      a[3 * (nyv - 1)] = 0.0;
+      a[3 * (nyv - 1) + 1] = 1.0;
+      a[3 * (nyv - 1) + 2] = 0.0;
+      lu_dec(a, nyv);
+      lu_sol_c(a, dv, v, nyv);
+    }
+    free(a);
+  }
+}
+
+/* LUsol0 + LU_build0 + LU_dec0, tridLU_3D.f90:228-254,409-435,520-537 */
+void lho_lusol0(const lho_state *s, int kRK, double *x) {
+  int n = s->nyu;
+  double beta = s->bRK[kRK - 1] * s->dt / s->c.Re;
+  double *a = malloc(sizeof(double) * 3 * n);
+  a[0] = 0.0;
+  a[1] = 1.0;
+  a[2] = s->gw1;
+  for (int j = 1; j <= n - 2; j++) {
+    const double *d = s->dyu2i + 3 * j;
+    a[3 * j] = -beta * d[0];
+    a[3 * j + 1] = 1.0 - beta * d[1];
+    a[3 * j + 2] = -beta * d[2];
+  }
+  a[3 * (n - 1)] = s->gw2;
+  a[3 * (n - 1) + 1] = 1.0;
+  a[3 * (n - 1) + 2] = 0.0;
+  lu_dec(a, n);
+  x[0] = x[0] * a[1];
+  for (int j = 1; j < n; j++) x[j] = (x[j] - a[3 * j] * x[j - 1]) * a[3 * j + 1];
+  for (int j = n - 2; j >= 0; j--) x[j] = x[j] - a[3 * j + 2] * x[j + 1];
+  free(a);
+}
+
+/* meanflow_ctU + flowrate_corr, littleharsh.f90:476-509,547-562 */
+void lho_meanflow_ctU(lho_state *s, int kRK) {
+  int n = s->nyu;
+  double *u11 = malloc(sizeof(double) * n);
+  double bdt = -s->dRK[kRK - 1] * s->dt;
+  u11[0] = 0.0;
+  for (int j = 1; j <= n - 2; j++) u11[j] = bdt;
+  u11[n - 1] = 0.0;
+  lho_lusol0(s, kRK, u11);
+  double Qx1 = flowrate_r(s, u11);
+  double Qx0 = flowrate_c(s, s->u1);
+  double g = (s->QxT - Qx0) / Qx1;
+  s->mpgx = s->mpgx + g;
+  for (int j = 0; j <= n - 1; j++) s->u1[j] = s->u1[j] + g * u11[j];
+  free(u11);
+}
+
+/* divergence, littleharsh.f90:237-272: p rows 1..ny-1 */
+void lho_divergence(lho_state *s) {
+  int nyu = s->nyu, nyv = s->nyv;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < s->ncol; c++) {
+    int i = c % s->nxh, k = c / s->nxh;
+    cplx kx = I * s->kx[i], kz = I * s->kz[k];
+    const cplx *u1 = s->u1 + (size_t)c * nyu, *u3 = s->u3 + (size_t)c * nyu;
+    const cplx *u2 = s->u2 + (size_t)c * nyv;
+    cplx *d = s->div + (size_t)c * nyu;
+    for (int j = 1; j <= nyu - 2; j++)
+      d[j] = kx * u1[j] + kz * u3[j] + s->ddthetavi * s->dthdyu[j] * (u2[j] - u2[j - 1]);
+  }
+}
+
+/* LU_buildP + LU_decP, tridLU_3D.f90:333-407,477-518 (p rows 1..ny-1) */
+static void build_dg(lho_state *s) {
+  int ns = 1, ne = s->nyu - 2, nyu = s->nyu;
+  double ddv = s->ddthetavi;
+  double *Db = calloc(nyu + 1, sizeof(double)), *Gb = calloc(nyu + 1, sizeof(double));
+  double *Dt = calloc(nyu + 1, sizeof(double)), *Gt = calloc(nyu + 1, sizeof(double));
+  for (int j = ns; j <= ne; j++) {
+    Db[j - 1] = -ddv * s->dthdyu[j];
+    Gb[j] = -ddv * s->dthdyv[j];
+    Dt[j] = ddv * s->dthdyu[j];
+    Gt[j + 1] = ddv * s->dthdyv[j];
+  }
+  for (int c = 0; c < s->ncol; c++) {
+    int i = c % s->nxh, k = c / s->nxh;
+    double k2x = s->k2x[i], k2z = s->k2z[k];
+    double *a = s->dg + (size_t)c * nyu * 3; /* rows indexed by j directly */
+    memset(a, 0, sizeof(double) * 3 * nyu);
+    a[3 * ns + 1] = (Dt[ns] * Gb[ns]) + k2x + k2z;
+    a[3 * ns + 2] = (Dt[ns] * Gt[ns + 1]);
+    a[3 * ne + 1] = (Db[ne - 1] * Gt[ne]) + k2x + k2z;
+    a[3 * ne + 0] = (Db[ne - 1] * Gb[ne - 1]);
+    for (int j = ns + 1; j <= ne - 1; j++) {
+      a[3 * j + 1] = (Db[j - 1] * Gt[j]) + (Dt[j] * Gb[j]) + k2x + k2z;
+      a[3 * j + 0] = (Db[j - 1] * Gb[j - 1]);
+      a[3 * j + 2] = (Dt[j] * Gt[j + 1]);
+    }
+    if (c == 0) { /* pin of the mean-mode pressure, tridLU_3D.f90:384-390 */
+      a[3 * ns + 0] = 0.0;
+      a[3 * ns + 1] = 1.0 / ((s->yu[2] - s->yu[1]) * (s->yu[2] - s->yu[1]));
+      a[3 * ns + 2] = 0.0;
+    }
+    lu_dec(a + 3 * ns, ne - ns + 1);
+  }
+  free(Db); free(Gb); free(Dt); free(Gt);
+}
+
+/* solveP + LUsolP, littleharsh.f90:581-633, tridLU_3D.f90:192-226 */
+void lho_solve_p(lho_state *s, int kRK) {
+  int nyu = s->nyu, ns = 1, ne = nyu - 2;
+  double dtirk = s->dti / (s->aRK[kRK - 1] + s->bRK[kRK - 1]);
+  lho_divergence(s);
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < s->ncol; c++) {
+    cplx *ps = s->psi + (size_t)c * nyu;
+    const cplx *d = s->div + (size_t)c * nyu;
+    for (int j = ns; j <= ne; j++) ps[j] = dtirk * d[j];
+    if (c == 0) ps[ns] = 0.0;
+    const double *a = s->dg + (size_t)c * nyu * 3;
+    ps[ns] = ps[ns] * a[3 * ns + 1];
+    for (int j = ns + 1; j <= ne; j++) ps[j] = (ps[j] - a[3 * j] * ps[j - 1]) * a[3 * j + 1];
+    for (int j = ne - 1; j >= ns; j--) ps[j] = ps[j] - a[3 * j + 2] * ps[j + 1];
+    cplx *p = s->p + (size_t)c * nyu;
+    for (int j = ns; j <= ne; j++) p[j] = p[j] + ps[j];
+  }
+}
+
+/* v_corr, littleharsh.f90:956-1052 */
+void lho_v_corr(lho_state *s, int kRK) {
+  int nyu = s->nyu, nyv = s->nyv;
+  double dtrk = s->dt * (s->aRK[kRK - 1] + s->bRK[kRK - 1]);
+  double dtrk_u2 = dtrk * s->ddthetavi;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < s->ncol; c++) {
+    int i = c % s->nxh, k = c / s->nxh;
+    cplx kx = dtrk * (I * s->kx[i]), kz = dtrk * (I * s->kz[k]);
+    cplx *u1 = s->u1 + (size_t)c * nyu, *u3 = s->u3 + (size_t)c * nyu;
+    cplx *u2 = s->u2 + (size_t)c * nyv;
+    const cplx *ps = s->psi + (size_t)c * nyu;
+    for (int j = 1; j <= nyu - 2; j++) u1[j] = u1[j] - kx * ps[j];
+    u1[0] = s->gwbc1 * u1[1];
+    u1[nyu - 1] = s->gwbc1 * u1[nyu - 2];
+    for (int j = 1; j <= nyv - 2; j++)
+      u2[j] = u2[j] - dtrk_u2 * s->dthdyv[j] * (ps[j + 1] - ps[j]);
+    u2[0] = 0;
+    u2[nyv - 1] = 0;
+    for (int j = 1; j <= nyu - 2; j++) u3[j] = u3[j] - kz * ps[j];
+    u3[0] = s->gwbc3 * u3[1];
+    u3[nyu - 1] = s->gwbc3 * u3[nyu - 2];
+  }
+  lho_divergence(s);
+}
+
+/* one RK3 substep, littleharsh.f90:176-215 */
+void lho_substep(lho_state *s, int kRK) {
+  lho_rhs0(s, kRK);
+  lho_nonlinear(s, kRK);
+  lho_solve_uw(s, kRK);
+  lho_solve_v(s, kRK);
+  if (s->c.flag_ctpress == 0) lho_meanflow_ctU(s, kRK);
+  lho_solve_p(s, kRK);
+  lho_v_corr(s, kRK);
+}
+
+/* main loop body, littleharsh.f90:172-228 */
+void lho_step(lho_state *s, int nsteps) {
+  for (int n = 0; n < nsteps; n++) {
+    for (int kRK = 1; kRK <= 3; kRK++) lho_substep(s, kRK);
+    if (s->c.flag_ctpress == 1) s->Qx = flowrate_c(s, s->u1);
+    s->iter = s->iter + 1;
+  }
+}

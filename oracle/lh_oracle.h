@@ -1,0 +1,102 @@
+/*
+ * lh_oracle.h -- CPU FP64 oracle for the plain-FFT RK3 step of LittleHarsh.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (liblhgpu.so) never links, loads or calls it.
+ *
+ * Field storage follows the reference's Fortran allocation of a band array
+ * (`complex(8) f(jlim1:jlim2, ncol)`, littleharsh.f90:119-130): y is the
+ * contiguous index, then the spectral column.  Column c (0-based) holds
+ * x-mode i = c % (N1/2+1) and z-index k = c / (N1/2+1) (k = 0..N2-1, the
+ * reference's columns_k - 1), the order proc_lims_columns builds for the
+ * long pencils (start.f90:1574-1600).
+ *
+ * Row ranges (j is the reference's row index):
+ *   u-grid (u1,u3,du1,du3):   j = 0 .. ny      (ny+1 rows incl. ghosts)
+ *   v-grid (u2,du2):          j = 0 .. ny-1    (ny rows incl. walls)
+ *   p-grid (p,psi,div):       j = 1 .. ny-1    (ny-1 rows)
+ *   Nu1,Nu3:                  j = 1 .. ny-1
+ *   Nu2:                      j = 1 .. ny-2
+ * where ny is the number of v points wall to wall (input.in `ny`).
+ */
+#ifndef LH_ORACLE_H
+#define LH_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lho_cfg {
+  int N1, N2;          /* Fourier modes in x and z (N = 2*(Ngal/3)) */
+  int Ngal1, Ngal2;    /* 3/2-padded physical grid */
+  int ny;              /* v points wall to wall */
+  double alp, bet;     /* Lx = 2pi/alp, Lz = 2pi/bet */
+  double Re;           /* 1/nu, as the reference uses it */
+  int grid_kind;       /* 0: polynomial y = a(j-q)+b(j-q)^p ; 1: tanh(gamma) */
+  double dyq;          /* polynomial grid: dymax/dymin */
+  int ppp;             /* polynomial grid exponent */
+  double gamma;        /* tanh grid stretching factor */
+  double bslpu1, bslpu3; /* Navier slip lengths used by the ghost update */
+  int flag_ctpress;    /* 0: constant flow rate, 1: constant pressure gradient */
+  double mpgx;         /* mean pressure gradient (initial / prescribed) */
+  int fixed_dt;        /* 1: dt fixed; 0: dt = min(2.5 dtv, CFL dtc) at kRK=1 */
+  double dt;
+  double CFL;
+} lho_cfg;
+
+typedef struct lho_state lho_state;
+
+lho_state *lho_create(const lho_cfg *cfg);
+void lho_destroy(lho_state *s);
+int lho_set_threads(int n);
+
+/* grid / wavenumber accessors (copies); arrays sized as documented */
+int lho_nrows_u(const lho_state *s);   /* ny+1 */
+int lho_ncol(const lho_state *s);      /* (N1/2+1)*N2 */
+void lho_get_grid(const lho_state *s, double *yu, double *yv, double *dthdyu,
+                  double *dthdyv, double *dyu2i, double *dyv2i);
+/* dyu2i/dyv2i are 3 x rows, Fortran order dyu2i(1:3, j) */
+void lho_set_grid(lho_state *s, const double *yu, const double *yv,
+                  const double *dthdyu, const double *dthdyv);
+void lho_get_scalars(const lho_state *s, double *out8);
+/* out8 = {t, dt, mpgx, QxT, Qx, dtv, dtc, iter} */
+void lho_set_scalars(lho_state *s, double t, double mpgx, double QxT, int iter);
+
+/* field ids */
+enum { LHO_U1 = 0, LHO_U2, LHO_U3, LHO_P, LHO_NU1, LHO_NU2, LHO_NU3,
+       LHO_DU1, LHO_DU2, LHO_DU3, LHO_PSI, LHO_DIV, LHO_NFIELD };
+/* copy a field in/out in the reference allocation layout (re,im interleaved) */
+int lho_field_rows(const lho_state *s, int fid, int *jlo, int *jhi);
+void lho_set_field(lho_state *s, int fid, const double *reim);
+void lho_get_field(const lho_state *s, int fid, double *reim);
+/* QxT from the current u1 mean mode (getini, start.f90:2166-2171) */
+void lho_init_flowrate(lho_state *s);
+
+/* RK3 pieces, littleharsh.f90:176-215 */
+void lho_rhs0(lho_state *s, int kRK);
+void lho_nonlinear(lho_state *s, int kRK);
+void lho_solve_uw(lho_state *s, int kRK);
+void lho_solve_v(lho_state *s, int kRK);
+void lho_meanflow_ctU(lho_state *s, int kRK);
+void lho_solve_p(lho_state *s, int kRK);
+void lho_v_corr(lho_state *s, int kRK);
+void lho_substep(lho_state *s, int kRK);
+void lho_step(lho_state *s, int nsteps);
+void lho_divergence(lho_state *s);
+
+/* Plane transforms on one (Ngal1+2) x Ngal2 plane, Fortran layout, in place
+ * (four_to_phys_du FOU3D.f90:766, phys_to_four_du FOU3D.f90:2253). */
+void lho_four_to_phys_plane(const lho_state *s, double *plane);
+void lho_phys_to_four_plane(const lho_state *s, double *plane);
+/* FFTPACK-convention wrappers (cft_buff.f:1-65, rft_buff.f:24-90) */
+void lho_cft(double *c, int n, int ise, int isa, int m, int iopt);
+void lho_rft(double *c, int n, int isa, int m, int iopt);
+
+/* Batched tridiagonal helpers on the reference's matrices */
+void lho_lusol0(const lho_state *s, int kRK, double *x); /* x(0:ny), LUsol0 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,205 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/liblhoracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+
+Fields cross this binding in the reference's allocation layout
+(`complex(8) f(jlim1:jlim2, ncol)`, y contiguous) as complex128 numpy arrays
+of shape (ncol, nrows).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liblhoracle.so")
+
+U1, U2, U3, P, NU1, NU2, NU3, DU1, DU2, DU3, PSI, DIV = range(12)
+FIELD_IDS = {"u1": U1, "u2": U2, "u3": U3, "p": P, "Nu1": NU1, "Nu2": NU2,
+             "Nu3": NU3, "du1": DU1, "du2": DU2, "du3": DU3, "psi": PSI, "div": DIV}
+
+
+class Cfg(ctypes.Structure):
+    _fields_ = [("N1", ctypes.c_int), ("N2", ctypes.c_int),
+                ("Ngal1", ctypes.c_int), ("Ngal2", ctypes.c_int),
+                ("ny", ctypes.c_int),
+                ("alp", ctypes.c_double), ("bet", ctypes.c_double),
+                ("Re", ctypes.c_double),
+                ("grid_kind", ctypes.c_int), ("dyq", ctypes.c_double),
+                ("ppp", ctypes.c_int), ("gamma", ctypes.c_double),
+                ("bslpu1", ctypes.c_double), ("bslpu3", ctypes.c_double),
+                ("flag_ctpress", ctypes.c_int), ("mpgx", ctypes.c_double),
+                ("fixed_dt", ctypes.c_int), ("dt", ctypes.c_double),
+                ("CFL", ctypes.c_double)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        vp = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.lho_create.restype = vp
+        L.lho_create.argtypes = [ctypes.POINTER(Cfg)]
+        L.lho_destroy.argtypes = [vp]
+        L.lho_set_threads.argtypes = [ctypes.c_int]
+        L.lho_set_threads.restype = ctypes.c_int
+        L.lho_nrows_u.argtypes = [vp]
+        L.lho_ncol.argtypes = [vp]
+        L.lho_get_grid.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.lho_set_grid.argtypes = [vp, dp, dp, dp, dp]
+        L.lho_get_scalars.argtypes = [vp, dp]
+        L.lho_set_scalars.argtypes = [vp, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_int]
+        L.lho_field_rows.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int)]
+        L.lho_set_field.argtypes = [vp, ctypes.c_int, dp]
+        L.lho_get_field.argtypes = [vp, ctypes.c_int, dp]
+        for name in ("lho_rhs0", "lho_nonlinear", "lho_solve_uw", "lho_solve_v",
+                     "lho_meanflow_ctU", "lho_solve_p", "lho_v_corr", "lho_substep",
+                     "lho_step"):
+            getattr(L, name).argtypes = [vp, ctypes.c_int]
+        L.lho_init_flowrate.argtypes = [vp]
+        L.lho_divergence.argtypes = [vp]
+        L.lho_four_to_phys_plane.argtypes = [vp, dp]
+        L.lho_phys_to_four_plane.argtypes = [vp, dp]
+        L.lho_cft.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int]
+        L.lho_rft.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.lho_lusol0.argtypes = [vp, ctypes.c_int, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def cft(c, n, ise, isa, m, iopt):
+    """FFTPACK cft semantics on a float64 array (modified in place)."""
+    assert c.dtype == np.float64 and c.flags.c_contiguous
+    lib().lho_cft(_dp(c), n, ise, isa, m, iopt)
+
+
+def rft(c, n, isa, m, iopt):
+    assert c.dtype == np.float64 and c.flags.c_contiguous
+    lib().lho_rft(_dp(c), n, isa, m, iopt)
+
+
+def set_threads(n):
+    return lib().lho_set_threads(int(n))
+
+
+class Oracle:
+    """One channel on the CPU.  `cfg` is a dict of lho_cfg fields."""
+
+    def __init__(self, **cfg):
+        c = Cfg()
+        defaults = dict(grid_kind=0, dyq=10.0, ppp=5, gamma=2.0, bslpu1=0.0,
+                        bslpu3=0.0, flag_ctpress=0, mpgx=0.0, fixed_dt=1,
+                        dt=1e-3, CFL=0.7, alp=1.0, bet=2.0)
+        defaults.update(cfg)
+        if "Ngal1" not in defaults:
+            defaults["Ngal1"] = 3 * defaults["N1"] // 2
+        if "Ngal2" not in defaults:
+            defaults["Ngal2"] = 3 * defaults["N2"] // 2
+        for k, v in defaults.items():
+            setattr(c, k, v)
+        self.cfg = defaults
+        self._c = c
+        self.h = lib().lho_create(ctypes.byref(c))
+        self.ny = c.ny
+        self.nyu = c.ny + 1
+        self.ncol = (c.N1 // 2 + 1) * c.N2
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().lho_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    # grid
+    def grid(self):
+        nyu, nyv = self.nyu, self.ny
+        yu = np.zeros(nyu); yv = np.zeros(nyv)
+        du = np.zeros(nyu); dv = np.zeros(nyv)
+        a = np.zeros(3 * nyu); b = np.zeros(3 * nyv)
+        lib().lho_get_grid(self.h, _dp(yu), _dp(yv), _dp(du), _dp(dv), _dp(a), _dp(b))
+        return dict(yu=yu, yv=yv, dthdyu=du, dthdyv=dv,
+                    dyu2i=a.reshape(nyu, 3), dyv2i=b.reshape(nyv, 3))
+
+    def set_grid(self, yu, yv, dthdyu, dthdyv):
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (yu, yv, dthdyu, dthdyv)]
+        lib().lho_set_grid(self.h, *[_dp(x) for x in arrs])
+
+    def scalars(self):
+        o = np.zeros(8)
+        lib().lho_get_scalars(self.h, _dp(o))
+        return dict(t=o[0], dt=o[1], mpgx=o[2], QxT=o[3], Qx=o[4], dtv=o[5], dtc=o[6],
+                    iter=int(o[7]))
+
+    def set_scalars(self, t=0.0, mpgx=None, QxT=None, iter=0):
+        s = self.scalars()
+        lib().lho_set_scalars(self.h, t, s["mpgx"] if mpgx is None else mpgx,
+                              s["QxT"] if QxT is None else QxT, iter)
+
+    def rows(self, name):
+        lo = ctypes.c_int(); hi = ctypes.c_int()
+        lib().lho_field_rows(self.h, FIELD_IDS[name], ctypes.byref(lo), ctypes.byref(hi))
+        return lo.value, hi.value
+
+    def get(self, name):
+        lo, hi = self.rows(name)
+        out = np.zeros((self.ncol, hi - lo + 1), dtype=np.complex128)
+        lib().lho_get_field(self.h, FIELD_IDS[name], _dp(out.view(np.float64)))
+        return out
+
+    def set(self, name, arr):
+        lo, hi = self.rows(name)
+        a = np.ascontiguousarray(arr, dtype=np.complex128)
+        assert a.shape == (self.ncol, hi - lo + 1), (a.shape, self.ncol, hi - lo + 1)
+        lib().lho_set_field(self.h, FIELD_IDS[name], _dp(a.view(np.float64)))
+
+    def init_flowrate(self):
+        lib().lho_init_flowrate(self.h)
+
+    def substep(self, kRK):
+        lib().lho_substep(self.h, kRK)
+
+    def step(self, n=1):
+        lib().lho_step(self.h, n)
+
+    def phase(self, name, kRK):
+        getattr(lib(), "lho_" + name)(self.h, kRK)
+
+    def divergence(self):
+        lib().lho_divergence(self.h)
+        return self.get("div")
+
+    def four_to_phys(self, plane):
+        p = np.ascontiguousarray(plane, dtype=np.float64).copy()
+        lib().lho_four_to_phys_plane(self.h, _dp(p))
+        return p
+
+    def phys_to_four(self, plane):
+        p = np.ascontiguousarray(plane, dtype=np.float64).copy()
+        lib().lho_phys_to_four_plane(self.h, _dp(p))
+        return p
+
+    def lusol0(self, kRK, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        lib().lho_lusol0(self.h, kRK, _dp(x))
+        return x

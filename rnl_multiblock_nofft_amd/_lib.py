@@ -1,0 +1,85 @@
+"""ctypes binding of liblhgpu.so (the C-ABI declared in include/lhgpu.h).
+
+The HIP library is the only compute path: importing a product entry point
+without a built liblhgpu.so raises, there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblhgpu.so")
+
+LH_U1, LH_U2, LH_U3, LH_P = 0, 1, 2, 3
+LH_NU1, LH_NU2, LH_NU3 = 4, 5, 6
+LH_R1, LH_R2, LH_R3 = 7, 8, 9
+LH_PSI, LH_DIV = 10, 11
+FIELDS = {"u1": LH_U1, "u2": LH_U2, "u3": LH_U3, "p": LH_P, "Nu1": LH_NU1,
+          "Nu2": LH_NU2, "Nu3": LH_NU3, "r1": LH_R1, "r2": LH_R2, "r3": LH_R3,
+          "psi": LH_PSI, "div": LH_DIV}
+PHASE_NONLINEAR, PHASE_RHS, PHASE_SOLVE = 1, 2, 3
+PREC_FP64, PREC_MIXED = 0, 1
+
+# every symbol include/lhgpu.h declares
+EXPORTS = ["lh_last_error", "lh_ctx_create", "lh_ctx_destroy", "lh_set_grid", "lh_get_grid",
+           "lh_field_upload", "lh_field_download", "lh_get_scalars", "lh_set_scalars",
+           "lh_init_flowrate", "lh_substep", "lh_step", "lh_sync", "lh_time_kernels", "lh_run_phase",
+           "lh_four_to_phys_planes", "lh_phys_to_four_planes", "lh_lusol_uw", "lh_lusol_v"]
+
+
+class LhConfig(ctypes.Structure):
+    _fields_ = [("N1", ctypes.c_int), ("N2", ctypes.c_int),
+                ("Ngal1", ctypes.c_int), ("Ngal2", ctypes.c_int), ("ny", ctypes.c_int),
+                ("alp", ctypes.c_double), ("bet", ctypes.c_double), ("Re", ctypes.c_double),
+                ("grid_kind", ctypes.c_int), ("dyq", ctypes.c_double), ("ppp", ctypes.c_int),
+                ("gamma", ctypes.c_double), ("bslpu1", ctypes.c_double),
+                ("bslpu3", ctypes.c_double), ("flag_ctpress", ctypes.c_int),
+                ("mpgx", ctypes.c_double), ("dt", ctypes.c_double), ("precision", ctypes.c_int)]
+
+
+class LhError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load liblhgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LhError("liblhgpu.so is not built (run __graft_entry__.build() or "
+                      "python rnl_multiblock_nofft_amd/build.py)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    L.lh_last_error.restype = ctypes.c_char_p
+    L.lh_ctx_create.argtypes = [ctypes.POINTER(LhConfig), ip, ctypes.POINTER(vp)]
+    L.lh_ctx_destroy.argtypes = [vp]
+    L.lh_set_grid.argtypes = [vp, ip, dp, dp, dp, dp]
+    L.lh_get_grid.argtypes = [vp, ip, dp, dp, dp, dp]
+    L.lh_field_upload.argtypes = [vp, ip, ip, dp]
+    L.lh_field_download.argtypes = [vp, ip, ip, dp]
+    L.lh_get_scalars.argtypes = [vp, ip, dp]
+    L.lh_set_scalars.argtypes = [vp, ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, ip]
+    L.lh_init_flowrate.argtypes = [vp]
+    L.lh_substep.argtypes = [vp, ip]
+    L.lh_step.argtypes = [vp, ip]
+    L.lh_sync.argtypes = [vp]
+    L.lh_time_kernels.argtypes = [vp, ip, dp]
+    L.lh_run_phase.argtypes = [vp, ip, ip]
+    L.lh_four_to_phys_planes.argtypes = [ip, ip, ip, vp, ip, vp]
+    L.lh_phys_to_four_planes.argtypes = [ip, ip, ip, vp, ip, vp]
+    L.lh_lusol_uw.argtypes = [vp, ip, ip, vp, vp, vp, vp]
+    L.lh_lusol_v.argtypes = [vp, ip, ip, vp, vp]
+    for name in EXPORTS:
+        if name != "lh_last_error":
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise LhError("liblhgpu error %d: %s" % (rc, load().lh_last_error().decode()))
+    return rc

@@ -1,0 +1,721 @@
+// lh_api.hip -- host side of liblhgpu: setup that start() performs in the
+// reference (ygrid, def_k, grid weights), device allocation, field
+// transfers, the substep/step drivers (HIP-graph replayed) and the C-ABI
+// declared in include/lhgpu.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lhgpu.h"
+#include "lh_common.h"
+#include "lh_kernels.h"
+
+using namespace lh;
+
+static thread_local std::string g_err;
+static int set_err(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int set_err(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return set_err(LH_EHIP, "%s: %s", #x, hipGetErrorString(e_));    \
+  } while (0)
+
+extern "C" const char *lh_last_error(void) { return g_err.c_str(); }
+
+namespace {
+
+// host copy of one channel's setup
+struct HostChan {
+  lh_config cfg;
+  int nyu, nyv, nxh, ncol, ldc, nslot;
+  std::vector<double> yu, yv, dthdyu, dthdyv, dyu2i, dyv2i, kx, kz, k2x, k2z;
+  std::vector<double2> twz, twx;
+  double gw1, gw2, gwbc1, gwbc3, gwi1, gwi2, dtv;
+  // device block
+  void *dmem = nullptr;
+  size_t dbytes = 0;
+  Chan dev;  // pointers into dmem
+  double t = 0.0;
+  int iter = 0;
+};
+
+double ipow(double x, int n) {
+  double r = 1.0;
+  for (int i = 0; i < n; i++) r *= x;
+  return r;
+}
+
+// y(theta), dtheta/dy of the wall-normal mapping (ygrid, start.f90:693-1077;
+// 'Wall' grid without immersed region).  grid_kind 1 is a tanh stretching.
+void ymap(const lh_config &c, double aaa, double bbb, double qqq, double th, double *y, double *dthdy) {
+  if (c.grid_kind == 1) {
+    double g = c.gamma, tg = tanh(g);
+    double s = g * (th / qqq - 1.0);
+    double ts = tanh(s);
+    *y = ts / tg;
+    *dthdy = 1.0 / ((g / (qqq * tg)) * (1.0 - ts * ts));
+  } else {
+    int p = c.ppp;
+    double x = th - qqq;
+    *y = aaa * x + bbb * ipow(x, p);
+    *dthdy = 1.0 / (aaa + p * bbb * ipow(x, p - 1));
+  }
+}
+
+// dyu2i/dyv2i (start.f90:943-987), grid weights (start.f90:537-559), dtv
+void derive_grid(HostChan &h) {
+  const int nn = h.nyv - 1;
+  const std::vector<double> &yu = h.yu, &yv = h.yv;
+  std::vector<double> &a = h.dyu2i, &b = h.dyv2i;
+  a.assign(3 * h.nyu, 0.0);
+  b.assign(3 * h.nyv, 0.0);
+  a[1] = -2.0 / ((yu[1] - yu[0]) * (yu[1] - yu[0]));
+  a[2] = 1.0 / ((yu[1] - yu[0]) * (yu[1] - yu[0]));
+  b[1] = -2.0 / ((yv[1] - yv[0]) * (yv[1] - yv[0]));
+  b[2] = 1.0 / ((yv[1] - yv[0]) * (yv[1] - yv[0]));
+  for (int j = 1; j <= nn - 1; j++) {
+    a[3 * j + 0] = 2.0 / ((yu[j] - yu[j - 1]) * (yu[j + 1] - yu[j - 1]));
+    a[3 * j + 1] = -2.0 / ((yu[j + 1] - yu[j]) * (yu[j] - yu[j - 1]));
+    a[3 * j + 2] = 2.0 / ((yu[j + 1] - yu[j]) * (yu[j + 1] - yu[j - 1]));
+    b[3 * j + 0] = 2.0 / ((yv[j] - yv[j - 1]) * (yv[j + 1] - yv[j - 1]));
+    b[3 * j + 1] = -2.0 / ((yv[j + 1] - yv[j]) * (yv[j] - yv[j - 1]));
+    b[3 * j + 2] = 2.0 / ((yv[j + 1] - yv[j]) * (yv[j + 1] - yv[j - 1]));
+  }
+  {
+    int j = nn;
+    a[3 * j + 0] = 2.0 / ((yu[j] - yu[j - 1]) * (yu[j + 1] - yu[j - 1]));
+    a[3 * j + 1] = -2.0 / ((yu[j + 1] - yu[j]) * (yu[j] - yu[j - 1]));
+    a[3 * j + 2] = 2.0 / ((yu[j + 1] - yu[j]) * (yu[j + 1] - yu[j - 1]));
+    double d = (yv[j] - yv[j - 1]) * (yv[j] - yv[j - 1]);
+    b[3 * j + 0] = 1.0 / d;
+    b[3 * j + 1] = -2.0 / d;
+  }
+  {
+    int j = nn + 1;
+    double d = (yu[j] - yu[j - 1]) * (yu[j] - yu[j - 1]);
+    a[3 * j + 0] = 1.0 / d;
+    a[3 * j + 1] = -2.0 / d;
+  }
+  const int ju = h.nyu - 1, jv = h.nyv - 1;
+  const double ddv = 1.0;
+  h.gw1 = -(yv[0] - yu[0]) / (yv[0] - yu[1]);
+  h.gw2 = (yu[ju] - yv[jv]) / (yv[jv] - yu[ju - 1]);
+  const double b1 = h.cfg.bslpu1, b3 = h.cfg.bslpu3, dv0 = h.dthdyv[0];
+  h.gwbc1 = ((yu[0] - yv[0]) + b1 * ddv * dv0 * (yu[1] - yu[0])) / ((yu[1] - yv[0]) + b1 * ddv * dv0 * (yu[1] - yu[0]));
+  h.gwbc3 = ((yu[0] - yv[0]) + b3 * ddv * dv0 * (yu[1] - yu[0])) / ((yu[1] - yv[0]) + b3 * ddv * dv0 * (yu[1] - yu[0]));
+  h.gwi1 = (yu[0] - yu[1]) / (yv[0] - yu[1]);
+  h.gwi2 = (yu[ju] - yu[ju - 1]) / (yv[jv] - yu[ju - 1]);
+  h.dtv = h.cfg.Re / (-h.k2x[h.cfg.N1 / 2] - h.k2z[h.cfg.N2 / 2 - 1] + 4.0 / ((yv[1] - yv[0]) * (yv[1] - yv[0])));
+}
+
+void twiddles(std::vector<double2> &tw, int n) {
+  tw.resize(n);
+  for (int m = 0; m < n; m++) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+    tw[m] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+}
+
+int valid_len(int n) {
+  if (n < 2) return 0;
+  while (n % 3 == 0) n /= 3;
+  while (n % 2 == 0) n /= 2;
+  return n == 1;
+}
+
+int setup_host(HostChan &h, const lh_config &c) {
+  if (c.N1 < 2 || c.N2 < 2 || (c.N1 & 1) || (c.N2 & 1))
+    return set_err(LH_EINVAL, "N1, N2 must be even and >= 2 (got %d, %d)", c.N1, c.N2);
+  if (c.Ngal1 < c.N1 + 2 || c.Ngal2 < c.N2 + 2 || (c.Ngal1 & 1) || (c.Ngal2 & 1))
+    return set_err(LH_EINVAL, "Ngal must be even and exceed N (got %d,%d for %d,%d)", c.Ngal1, c.Ngal2, c.N1, c.N2);
+  if (2 * c.Ngal1 < 3 * c.N1 || 2 * c.Ngal2 < 3 * c.N2)
+    return set_err(LH_EINVAL, "Ngal below the 3/2 rule (%d,%d for %d,%d)", c.Ngal1, c.Ngal2, c.N1, c.N2);
+  if (!valid_len(c.Ngal1) || !valid_len(c.Ngal2))
+    return set_err(LH_EINVAL, "Ngal must be 2^a 3^b (got %d, %d)", c.Ngal1, c.Ngal2);
+  if (c.ny < 5) return set_err(LH_EINVAL, "ny must be >= 5 (got %d)", c.ny);
+  if (!(c.dt > 0.0)) return set_err(LH_EINVAL, "dt must be positive");
+  if (!(c.Re > 0.0)) return set_err(LH_EINVAL, "Re must be positive");
+  if (c.grid_kind == 0 && (c.ppp < 2 || !(c.dyq > 0))) return set_err(LH_EINVAL, "bad polynomial grid");
+  if (c.grid_kind == 1 && !(c.gamma > 0)) return set_err(LH_EINVAL, "bad tanh grid");
+  if (c.grid_kind != 0 && c.grid_kind != 1) return set_err(LH_EINVAL, "grid_kind must be 0 or 1");
+  h.cfg = c;
+  h.nyv = c.ny;
+  h.nyu = c.ny + 1;
+  h.nxh = c.N1 / 2 + 1;
+  h.ncol = h.nxh * c.N2;
+  h.ldc = (h.ncol + 63) / 64 * 64;
+  h.nslot = (h.nyu - 2) + (h.nyv - 2);
+  // def_k, start.f90:1079-1136
+  h.kx.resize(h.nxh);
+  h.k2x.resize(h.nxh);
+  h.kz.resize(c.N2);
+  h.k2z.resize(c.N2);
+  for (int i = 0; i < h.nxh; i++) {
+    double t = c.alp * i;
+    h.kx[i] = t;
+    h.k2x[i] = -(t * t);
+  }
+  for (int k = 1; k <= c.N2 / 2; k++) {
+    double t = c.bet * (k - 1);
+    h.kz[k - 1] = t;
+    h.k2z[k - 1] = -(t * t);
+  }
+  for (int k = c.N2 / 2 + 1; k <= c.N2; k++) {
+    double t = c.bet * (c.N2 - k + 1);
+    h.kz[k - 1] = -t;
+    h.k2z[k - 1] = -(t * t);
+  }
+  // ygrid
+  const int nn = c.ny - 1;
+  double qqq = nn / 2.0, aaa = 0.0, bbb = 0.0;
+  if (c.grid_kind == 0) {
+    int p = c.ppp;
+    double dyq = c.dyq;
+    bbb = (dyq - 1.0) / (qqq * (1.0 - ipow(qqq, p - 1) + dyq * ((1.0 - qqq) * ipow(qqq, p - 1) - ipow(1.0 - qqq, p))));
+    aaa = (1.0 - bbb * ipow(qqq, p)) / qqq;
+  }
+  h.yu.assign(h.nyu, 0.0);
+  h.yv.assign(h.nyv, 0.0);
+  h.dthdyu.assign(h.nyu, 0.0);
+  h.dthdyv.assign(h.nyv, 0.0);
+  ymap(c, aaa, bbb, qqq, -0.5, &h.yu[0], &h.dthdyu[0]);
+  for (int j = 0; j <= nn; j++) {
+    ymap(c, aaa, bbb, qqq, (double)j, &h.yv[j], &h.dthdyv[j]);
+    ymap(c, aaa, bbb, qqq, j + 0.5, &h.yu[j + 1], &h.dthdyu[j + 1]);
+  }
+  derive_grid(h);
+  twiddles(h.twz, c.Ngal2);
+  twiddles(h.twx, c.Ngal1);
+  return LH_OK;
+}
+
+template <typename T> size_t al(size_t n) { return (n * sizeof(T) + 255) / 256 * 256; }
+
+}  // namespace
+
+struct lh_ctx {
+  std::vector<HostChan> ch;
+  bool mixed = false;
+  hipStream_t stream = nullptr;
+  Chan *d_chans = nullptr;
+  ShapeMax shape;
+  double2 *staging = nullptr;
+  size_t staging_elems = 0;
+  hipGraphExec_t graph = nullptr;
+  int device = 0;
+};
+
+namespace {
+
+int upload_chan_constants(lh_ctx *x, HostChan &h) {
+  Chan &d = h.dev;
+  HIPCHK(hipMemcpy((void *)d.yu, h.yu.data(), h.nyu * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.yv, h.yv.data(), h.nyv * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.dthdyu, h.dthdyu.data(), h.nyu * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.dthdyv, h.dthdyv.data(), h.nyv * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.dyu2i, h.dyu2i.data(), 3 * h.nyu * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.dyv2i, h.dyv2i.data(), 3 * h.nyv * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.kx, h.kx.data(), h.nxh * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.k2x, h.k2x.data(), h.nxh * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.kz, h.kz.data(), h.cfg.N2 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.k2z, h.k2z.data(), h.cfg.N2 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.twz, h.twz.data(), h.cfg.Ngal2 * sizeof(double2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.twx, h.twx.data(), h.cfg.Ngal1 * sizeof(double2), hipMemcpyHostToDevice));
+  d.gw1 = h.gw1;
+  d.gw2 = h.gw2;
+  d.gwbc1 = h.gwbc1;
+  d.gwbc3 = h.gwbc3;
+  d.gwi1 = h.gwi1;
+  d.gwi2 = h.gwi2;
+  (void)x;
+  return LH_OK;
+}
+
+int alloc_chan(lh_ctx *x, HostChan &h) {
+  const lh_config &c = h.cfg;
+  const size_t fld = (size_t)h.nyu * h.ldc;  // complex elements per field
+  const size_t nfield = 18;
+  const size_t ninter = (size_t)h.nslot * c.Ngal2 * h.nxh;
+  size_t bytes = 0;
+  bytes += nfield * al<double2>(fld);
+  bytes += al<double>(fld);  // scr
+  bytes += al<double>(h.nyu) * 4 + al<double>(h.nyv) * 2 + al<double>(3 * h.nyu) + al<double>(3 * h.nyv);
+  bytes += al<double>(h.nxh) * 2 + al<double>(c.N2) * 2 + al<double2>(c.Ngal2) + al<double2>(c.Ngal1);
+  bytes += al<double>(h.nyu) + al<double>(SC_NSCAL);
+  if (x->mixed)
+    bytes += al<float2>(3 * ninter) + al<float2>(4 * ninter);
+  else
+    bytes += al<double2>(3 * ninter) + al<double2>(4 * ninter);
+  hipError_t e = hipMalloc(&h.dmem, bytes);
+  if (e != hipSuccess) return set_err(LH_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  h.dbytes = bytes;
+  HIPCHK(hipMemset(h.dmem, 0, bytes));
+  char *p = (char *)h.dmem;
+  auto take = [&](size_t b) {
+    char *q = p;
+    p += b;
+    return (void *)q;
+  };
+  Chan &d = h.dev;
+  memset(&d, 0, sizeof d);
+  double2 **fl[18] = {&d.u1, &d.u2, &d.u3, &d.p, &d.N1o, &d.N2o, &d.N3o, &d.r1, &d.r2,
+                      &d.r3, &d.nl1, &d.nl2, &d.nl3, &d.fuv, &d.fwv, &d.fvv, &d.psi, &d.div};
+  for (int q = 0; q < 18; q++) *fl[q] = (double2 *)take(al<double2>(fld));
+  d.scr = (double *)take(al<double>(fld));
+  d.yu = (double *)take(al<double>(h.nyu));
+  d.dthdyu = (double *)take(al<double>(h.nyu));
+  d.yv = (double *)take(al<double>(h.nyv));
+  d.dthdyv = (double *)take(al<double>(h.nyv));
+  d.dyu2i = (double *)take(al<double>(3 * h.nyu));
+  d.dyv2i = (double *)take(al<double>(3 * h.nyv));
+  d.u11 = (double *)take(al<double>(h.nyu));
+  (void)take(al<double>(h.nyu));
+  (void)take(al<double>(h.nyu));
+  d.kx = (double *)take(al<double>(h.nxh));
+  d.k2x = (double *)take(al<double>(h.nxh));
+  d.kz = (double *)take(al<double>(c.N2));
+  d.k2z = (double *)take(al<double>(c.N2));
+  d.twz = (double2 *)take(al<double2>(c.Ngal2));
+  d.twx = (double2 *)take(al<double2>(c.Ngal1));
+  (void)take(al<double>(h.nyu));
+  d.scal = (double *)take(al<double>(SC_NSCAL));
+  if (x->mixed) {
+    d.hzf = (float2 *)take(al<float2>(3 * ninter));
+    d.gxf = (float2 *)take(al<float2>(4 * ninter));
+  } else {
+    d.hz = (double2 *)take(al<double2>(3 * ninter));
+    d.gx = (double2 *)take(al<double2>(4 * ninter));
+  }
+  if ((size_t)(p - (char *)h.dmem) > bytes) return set_err(LH_EINVAL, "internal: allocation overrun");
+  d.N1 = c.N1;
+  d.N2 = c.N2;
+  d.Ng1 = c.Ngal1;
+  d.Ng2 = c.Ngal2;
+  d.ny = c.ny;
+  d.nyu = h.nyu;
+  d.nyv = h.nyv;
+  d.nxh = h.nxh;
+  d.ncol = h.ncol;
+  d.ldc = h.ldc;
+  d.nplu = h.nyu - 2;
+  d.nplv = h.nyv - 2;
+  d.nslot = h.nslot;
+  d.flag_ctpress = c.flag_ctpress;
+  d.Re = c.Re;
+  d.alp = c.alp;
+  d.bet = c.bet;
+  int rc = upload_chan_constants(x, h);
+  if (rc) return rc;
+  double sc[SC_NSCAL] = {0};
+  sc[SC_MPGX] = c.mpgx;
+  sc[SC_DT] = c.dt;
+  HIPCHK(hipMemcpy(d.scal, sc, sizeof sc, hipMemcpyHostToDevice));
+  return LH_OK;
+}
+
+int push_descriptors(lh_ctx *x) {
+  std::vector<Chan> v;
+  for (auto &h : x->ch) v.push_back(h.dev);
+  HIPCHK(hipMemcpy(x->d_chans, v.data(), v.size() * sizeof(Chan), hipMemcpyHostToDevice));
+  return LH_OK;
+}
+
+// rows of a field in the reference layout, device array, jlo
+int field_map(HostChan &h, int field, double2 **arr, int *jlo, int *nrows) {
+  Chan &d = h.dev;
+  switch (field) {
+    case LH_U1: *arr = d.u1; *jlo = 0; *nrows = h.nyu; return 0;
+    case LH_U2: *arr = d.u2; *jlo = 0; *nrows = h.nyv; return 0;
+    case LH_U3: *arr = d.u3; *jlo = 0; *nrows = h.nyu; return 0;
+    case LH_P: *arr = d.p; *jlo = 1; *nrows = h.nyu - 2; return 0;
+    case LH_NU1: *arr = d.N1o; *jlo = 1; *nrows = h.nyu - 2; return 0;
+    case LH_NU2: *arr = d.N2o; *jlo = 1; *nrows = h.nyv - 2; return 0;
+    case LH_NU3: *arr = d.N3o; *jlo = 1; *nrows = h.nyu - 2; return 0;
+    case LH_R1: *arr = d.r1; *jlo = 0; *nrows = h.nyu; return 0;
+    case LH_R2: *arr = d.r2; *jlo = 0; *nrows = h.nyv; return 0;
+    case LH_R3: *arr = d.r3; *jlo = 0; *nrows = h.nyu; return 0;
+    case LH_PSI: *arr = d.psi; *jlo = 1; *nrows = h.nyu - 2; return 0;
+    case LH_DIV: *arr = d.div; *jlo = 1; *nrows = h.nyu - 2; return 0;
+  }
+  return -1;
+}
+
+int check_ctx(lh_ctx *x) {
+  if (!x) return set_err(LH_EINVAL, "null context");
+  HIPCHK(hipSetDevice(x->device));
+  return LH_OK;
+}
+
+int run_substep(lh_ctx *x, int kRK) {
+  HIPCHK(launch_nonlinear(x->d_chans, x->shape, x->mixed, x->stream));
+  HIPCHK(launch_rhs(x->d_chans, x->shape, kRK, x->stream));
+  HIPCHK(launch_solve(x->d_chans, x->shape, kRK, x->stream));
+  return LH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
+  if (!cfgs || nchan < 1 || !out) return set_err(LH_EINVAL, "lh_ctx_create: bad arguments");
+  *out = nullptr;
+  for (int c = 1; c < nchan; c++)
+    if (cfgs[c].precision != cfgs[0].precision)
+      return set_err(LH_EINVAL, "all channels of a context share one precision mode");
+  if (cfgs[0].precision != LH_PREC_FP64 && cfgs[0].precision != LH_PREC_MIXED)
+    return set_err(LH_EINVAL, "unknown precision mode %d", cfgs[0].precision);
+  lh_ctx *x = new lh_ctx();
+  x->mixed = cfgs[0].precision == LH_PREC_MIXED;
+  hipError_t e = hipGetDevice(&x->device);
+  if (e != hipSuccess) {
+    delete x;
+    return set_err(LH_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
+  }
+  x->ch.resize(nchan);
+  int rc = LH_OK;
+  size_t stage = 0;
+  for (int c = 0; c < nchan && rc == LH_OK; c++) {
+    rc = setup_host(x->ch[c], cfgs[c]);
+    if (rc == LH_OK) {
+      shape_add(x->shape, cfgs[c].N1, cfgs[c].N2, cfgs[c].Ngal1, cfgs[c].Ngal2, cfgs[c].ny);
+      size_t s = (size_t)x->ch[c].ncol * x->ch[c].nyu;
+      if (s > stage) stage = s;
+    }
+  }
+  if (rc == LH_OK) {
+    e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) rc = set_err(LH_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  for (int c = 0; c < nchan && rc == LH_OK; c++) rc = alloc_chan(x, x->ch[c]);
+  if (rc == LH_OK) {
+    e = hipMalloc(&x->d_chans, nchan * sizeof(Chan));
+    if (e != hipSuccess) rc = set_err(LH_ENOMEM, "hipMalloc(descriptors): %s", hipGetErrorString(e));
+  }
+  if (rc == LH_OK) {
+    e = hipMalloc(&x->staging, stage * sizeof(double2));
+    if (e != hipSuccess) rc = set_err(LH_ENOMEM, "hipMalloc(staging): %s", hipGetErrorString(e));
+    x->staging_elems = stage;
+  }
+  if (rc == LH_OK) rc = push_descriptors(x);
+  if (rc != LH_OK) {
+    std::string keep = g_err;
+    lh_ctx_destroy(x);
+    g_err = keep;
+    return rc;
+  }
+  *out = x;
+  return LH_OK;
+}
+
+int lh_ctx_destroy(lh_ctx *x) {
+  if (!x) return LH_OK;
+  (void)hipSetDevice(x->device);
+  if (x->stream) (void)hipStreamSynchronize(x->stream);
+  if (x->graph) (void)hipGraphExecDestroy(x->graph);
+  for (auto &h : x->ch)
+    if (h.dmem) (void)hipFree(h.dmem);
+  if (x->d_chans) (void)hipFree(x->d_chans);
+  if (x->staging) (void)hipFree(x->staging);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+  delete x;
+  return LH_OK;
+}
+
+int lh_set_grid(lh_ctx *x, int ch, const double *yu, const double *yv, const double *dthdyu,
+                const double *dthdyv) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || !yu || !yv || !dthdyu || !dthdyv)
+    return set_err(LH_EINVAL, "lh_set_grid: bad arguments");
+  HostChan &h = x->ch[ch];
+  h.yu.assign(yu, yu + h.nyu);
+  h.yv.assign(yv, yv + h.nyv);
+  h.dthdyu.assign(dthdyu, dthdyu + h.nyu);
+  h.dthdyv.assign(dthdyv, dthdyv + h.nyv);
+  derive_grid(h);
+  HIPCHK(hipStreamSynchronize(x->stream));
+  rc = upload_chan_constants(x, h);
+  if (rc) return rc;
+  return push_descriptors(x);
+}
+
+int lh_get_grid(lh_ctx *x, int ch, double *yu, double *yv, double *dthdyu, double *dthdyv) {
+  if (!x || ch < 0 || ch >= (int)x->ch.size()) return set_err(LH_EINVAL, "lh_get_grid: bad arguments");
+  HostChan &h = x->ch[ch];
+  if (yu) memcpy(yu, h.yu.data(), h.nyu * sizeof(double));
+  if (yv) memcpy(yv, h.yv.data(), h.nyv * sizeof(double));
+  if (dthdyu) memcpy(dthdyu, h.dthdyu.data(), h.nyu * sizeof(double));
+  if (dthdyv) memcpy(dthdyv, h.dthdyv.data(), h.nyv * sizeof(double));
+  return LH_OK;
+}
+
+int lh_field_upload(lh_ctx *x, int ch, int field, const double *reim) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || !reim) return set_err(LH_EINVAL, "lh_field_upload: bad arguments");
+  HostChan &h = x->ch[ch];
+  double2 *arr;
+  int jlo, nrows;
+  if (field_map(h, field, &arr, &jlo, &nrows)) return set_err(LH_EINVAL, "lh_field_upload: bad field %d", field);
+  size_t n = (size_t)h.ncol * nrows;
+  HIPCHK(hipMemcpyAsync(x->staging, reim, n * sizeof(double2), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(launch_to_rows(x->staging, arr, h.ncol, nrows, jlo, h.ldc, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+int lh_field_download(lh_ctx *x, int ch, int field, double *reim) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || !reim) return set_err(LH_EINVAL, "lh_field_download: bad arguments");
+  HostChan &h = x->ch[ch];
+  double2 *arr;
+  int jlo, nrows;
+  if (field_map(h, field, &arr, &jlo, &nrows)) return set_err(LH_EINVAL, "lh_field_download: bad field %d", field);
+  size_t n = (size_t)h.ncol * nrows;
+  HIPCHK(launch_from_rows(arr, x->staging, h.ncol, nrows, jlo, h.ldc, x->stream));
+  HIPCHK(hipMemcpyAsync(reim, x->staging, n * sizeof(double2), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+int lh_get_scalars(lh_ctx *x, int ch, double *o) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || !o) return set_err(LH_EINVAL, "lh_get_scalars: bad arguments");
+  HostChan &h = x->ch[ch];
+  double sc[SC_NSCAL];
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipMemcpy(sc, h.dev.scal, sizeof sc, hipMemcpyDeviceToHost));
+  o[0] = h.t;
+  o[1] = sc[SC_DT];
+  o[2] = sc[SC_MPGX];
+  o[3] = sc[SC_QXT];
+  o[4] = sc[SC_QX];
+  o[5] = h.dtv;
+  o[6] = 0.0;
+  o[7] = h.iter;
+  return LH_OK;
+}
+
+int lh_set_scalars(lh_ctx *x, int ch, double t, double mpgx, double QxT, int iter) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size()) return set_err(LH_EINVAL, "lh_set_scalars: bad channel");
+  HostChan &h = x->ch[ch];
+  double sc[SC_NSCAL];
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipMemcpy(sc, h.dev.scal, sizeof sc, hipMemcpyDeviceToHost));
+  sc[SC_MPGX] = mpgx;
+  sc[SC_QXT] = QxT;
+  HIPCHK(hipMemcpy(h.dev.scal, sc, sizeof sc, hipMemcpyHostToDevice));
+  h.t = t;
+  h.iter = iter;
+  return LH_OK;
+}
+
+int lh_init_flowrate(lh_ctx *x) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  HIPCHK(launch_flowrate(x->d_chans, (int)x->ch.size(), 1, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+int lh_run_phase(lh_ctx *x, int phase, int kRK) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (kRK < 1 || kRK > 3) return set_err(LH_EINVAL, "kRK must be 1..3");
+  switch (phase) {
+    case LH_PHASE_NONLINEAR: HIPCHK(launch_nonlinear(x->d_chans, x->shape, x->mixed, x->stream)); break;
+    case LH_PHASE_RHS: HIPCHK(launch_rhs(x->d_chans, x->shape, kRK, x->stream)); break;
+    case LH_PHASE_SOLVE: HIPCHK(launch_solve(x->d_chans, x->shape, kRK, x->stream)); break;
+    default: return set_err(LH_EINVAL, "unknown phase %d", phase);
+  }
+  if (kRK == 1 && phase == LH_PHASE_NONLINEAR)
+    for (auto &h : x->ch) h.t += h.cfg.dt;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+int lh_substep(lh_ctx *x, int kRK) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (kRK < 1 || kRK > 3) return set_err(LH_EINVAL, "kRK must be 1..3");
+  rc = run_substep(x, kRK);
+  if (rc) return rc;
+  if (kRK == 1)
+    for (auto &h : x->ch) h.t += h.cfg.dt;
+  if (kRK == 3)
+    for (auto &h : x->ch) h.iter += 1;
+  return LH_OK;
+}
+
+int lh_step(lh_ctx *x, int nsteps) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (nsteps < 0) return set_err(LH_EINVAL, "nsteps must be >= 0");
+  if (nsteps == 0) return LH_OK;
+  if (!x->graph) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+    int r = LH_OK;
+    for (int k = 1; k <= 3 && r == LH_OK; k++) r = run_substep(x, k);
+    hipError_t e = hipStreamEndCapture(x->stream, &g);
+    if (r) return r;
+    if (e != hipSuccess) return set_err(LH_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&x->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      x->graph = nullptr;
+      return set_err(LH_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+  }
+  for (int n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(x->graph, x->stream));
+  HIPCHK(hipGetLastError());
+  for (auto &h : x->ch) {
+    h.t += nsteps * h.cfg.dt;
+    h.iter += nsteps;
+  }
+  return LH_OK;
+}
+
+int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (nsub < 1 || !ms) return set_err(LH_EINVAL, "lh_time_kernels: bad arguments");
+  hipEvent_t ev[6];
+  for (int q = 0; q < 6; q++) HIPCHK(hipEventCreate(&ev[q]));
+  for (int q = 0; q < 5; q++) ms[q] = 0.0;
+  int kRK = 1;
+  for (int n = 0; n < nsub && rc == LH_OK; n++) {
+    HIPCHK(hipEventRecord(ev[0], x->stream));
+    for (int st = 0; st < 3; st++) {
+      HIPCHK(launch_nl_stage(x->d_chans, x->shape, x->mixed, st, x->stream));
+      HIPCHK(hipEventRecord(ev[st + 1], x->stream));
+    }
+    HIPCHK(launch_rhs(x->d_chans, x->shape, kRK, x->stream));
+    HIPCHK(hipEventRecord(ev[4], x->stream));
+    HIPCHK(launch_solve(x->d_chans, x->shape, kRK, x->stream));
+    HIPCHK(hipEventRecord(ev[5], x->stream));
+    HIPCHK(hipEventSynchronize(ev[5]));
+    for (int q = 0; q < 5; q++) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+      ms[q] += t;
+    }
+    if (kRK == 1)
+      for (auto &h : x->ch) h.t += h.cfg.dt;
+    if (kRK == 3)
+      for (auto &h : x->ch) h.iter += 1;
+    kRK = kRK % 3 + 1;
+  }
+  for (int q = 0; q < 6; q++) (void)hipEventDestroy(ev[q]);
+  return rc;
+}
+
+int lh_sync(lh_ctx *x) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+static std::vector<double2> &tw_cache(int n) {
+  static std::vector<std::vector<double2>> store;
+  for (auto &v : store)
+    if ((int)v.size() == n) return v;
+  store.emplace_back();
+  twiddles(store.back(), n);
+  return store.back();
+}
+
+static int plane_args(int N1, int Ngal1, int Ngal2, double *d, int np) {
+  if (!d || np < 1 || N1 < 2 || (N1 & 1) || Ngal1 < N1 + 2 || !valid_len(Ngal1) || !valid_len(Ngal2))
+    return set_err(LH_EINVAL, "plane transform: bad arguments");
+  return LH_OK;
+}
+
+static int plane_tw(int Ngal1, int Ngal2, double2 **dz, double2 **dx) {
+  static std::vector<std::pair<int, double2 *>> dev;
+  int dv = 0;
+  (void)hipGetDevice(&dv);
+  auto find = [&](int n) -> double2 * {
+    int key = n * 64 + dv;
+    for (auto &p : dev)
+      if (p.first == key) return p.second;
+    std::vector<double2> &h = tw_cache(n);
+    double2 *d = nullptr;
+    if (hipMalloc(&d, n * sizeof(double2)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    dev.push_back({key, d});
+    return d;
+  };
+  *dz = find(Ngal2);
+  *dx = find(Ngal1);
+  if (!*dz || !*dx) return set_err(LH_ENOMEM, "plane transform: twiddle table allocation failed");
+  return LH_OK;
+}
+
+int lh_four_to_phys_planes(int N1, int Ngal1, int Ngal2, double *d, int np, void *stream) {
+  int rc = plane_args(N1, Ngal1, Ngal2, d, np);
+  if (rc) return rc;
+  double2 *tz, *tx;
+  rc = plane_tw(Ngal1, Ngal2, &tz, &tx);
+  if (rc) return rc;
+  HIPCHK(launch_plane_f2p(N1, Ngal1, Ngal2, d, np, tz, tx, (hipStream_t)stream));
+  return LH_OK;
+}
+
+int lh_phys_to_four_planes(int N1, int Ngal1, int Ngal2, double *d, int np, void *stream) {
+  int rc = plane_args(N1, Ngal1, Ngal2, d, np);
+  if (rc) return rc;
+  double2 *tz, *tx;
+  rc = plane_tw(Ngal1, Ngal2, &tz, &tx);
+  if (rc) return rc;
+  HIPCHK(launch_plane_p2f(N1, Ngal1, Ngal2, d, np, tz, tx, (hipStream_t)stream));
+  return LH_OK;
+}
+
+int lh_lusol_uw(lh_ctx *x, int ch, int kRK, const double *ru, const double *rw, double *u, double *w) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || kRK < 1 || kRK > 3 || !ru || !rw || !u || !w)
+    return set_err(LH_EINVAL, "lh_lusol_uw: bad arguments");
+  HostChan &h = x->ch[ch];
+  const Chan *dc = x->d_chans + ch;
+  HIPCHK(launch_lusol_ref2(dc, kRK, 0, (const double2 *)ru, (const double2 *)rw, (double2 *)u, (double2 *)w,
+                           h.dev.scr, h.ncol, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+int lh_lusol_v(lh_ctx *x, int ch, int kRK, const double *rv, double *v) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || kRK < 1 || kRK > 3 || !rv || !v)
+    return set_err(LH_EINVAL, "lh_lusol_v: bad arguments");
+  HostChan &h = x->ch[ch];
+  const Chan *dc = x->d_chans + ch;
+  HIPCHK(launch_lusol_ref2(dc, kRK, 1, (const double2 *)rv, nullptr, (double2 *)v, nullptr, h.dev.scr, h.ncol,
+                           x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+}  // extern "C"

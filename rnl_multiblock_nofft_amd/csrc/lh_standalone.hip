@@ -1,0 +1,211 @@
+// lh_standalone.hip -- stand-alone drop-in kernels on reference-layout data:
+//   * plane transforms four_to_phys_du / phys_to_four_du (FOU3D.f90:766-784,
+//     2253-2272) on (Ngal1+2) x Ngal2 Fortran-order planes, FFTPACK
+//     conventions (cft_buff.f:1-65, rft_buff.f:24-90);
+//   * the Helmholtz solves of LUsolU_W / LUsolV (tridLU_3D.f90:55-189) on
+//     x(jlo:jhi, ncol) arrays.
+#include "lh_common.h"
+#include "lh_fft.h"
+#include "lh_kernels.h"
+
+namespace lh {
+
+static __host__ __device__ int plane_tile(int n) {
+  int t = 1024 / n;
+  return t < 1 ? 1 : t;
+}
+
+// z transform of the retained x-modes i = 0..N1/2 of every plane.
+// dir = +1: inverse (zero row Ngal2/2 first); dir = -1: forward, scaled, then
+// zero row Ngal2/2 of the whole plane.
+__global__ void __launch_bounds__(256) k_plane_z(double *planes, int N1, int Ng1, int Ng2, int dir,
+                                                 const double2 *twz) {
+  const int igal = Ng1 + 2, nv = N1 / 2 + 1;
+  double *P = planes + (size_t)blockIdx.y * igal * Ng2;
+  const int TI = plane_tile(Ng2);
+  const int i0 = blockIdx.x * TI;
+  extern __shared__ double2 smem_d[];
+  if (i0 < nv) {
+    const int ni = min(TI, nv - i0);
+    double2 *A = smem_d, *B = smem_d + TI * Ng2;
+    for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) {
+      int z = t / ni, ii = t - z * ni;
+      double2 v = make_double2(P[(size_t)z * igal + 2 * (i0 + ii)], P[(size_t)z * igal + 2 * (i0 + ii) + 1]);
+      if (dir > 0 && z == Ng2 / 2) v = make_double2(0, 0);
+      A[ii * Ng2 + z] = v;
+    }
+    __syncthreads();
+    double2 *R = fft_lds<double2>(A, B, Ng2, ni, Ng2, dir, twz);
+    const double s = dir < 0 ? 1.0 / Ng2 : 1.0;
+    for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) {
+      int z = t / ni, ii = t - z * ni;
+      double2 v = R[ii * Ng2 + z];
+      if (dir < 0 && z == Ng2 / 2) v = make_double2(0, 0);
+      P[(size_t)z * igal + 2 * (i0 + ii)] = v.x * s;
+      P[(size_t)z * igal + 2 * (i0 + ii) + 1] = v.y * s;
+    }
+  }
+  // entries not transformed: row Ngal2/2 is zeroed (four_to_phys zeroes it
+  // before, phys_to_four after the transforms)
+  if (blockIdx.x == 0) {
+    __syncthreads();
+    for (int ii = 2 * nv + threadIdx.x; ii < igal; ii += blockDim.x) P[(size_t)(Ng2 / 2) * igal + ii] = 0.0;
+  }
+}
+
+// x transform (real) of every z-row; two rows share one complex FFT.
+__global__ void __launch_bounds__(256) k_plane_x(double *planes, int Ng1, int Ng2, int dir,
+                                                 const double2 *twx) {
+  const int igal = Ng1 + 2, h = Ng1 / 2;
+  double *P = planes + (size_t)blockIdx.y * igal * Ng2;
+  const int TZ = plane_tile(Ng1);  // pairs of rows per block
+  const int z0 = blockIdx.x * 2 * TZ;
+  if (z0 >= Ng2) return;
+  const int nzr = min(2 * TZ, Ng2 - z0);  // rows in this block
+  const int npair = (nzr + 1) / 2;
+  extern __shared__ double2 smem_d[];
+  double2 *A = smem_d, *B = smem_d + TZ * Ng1;
+  if (dir > 0) {
+    for (int t = threadIdx.x; t < npair * Ng1; t += blockDim.x) {
+      int q = t / Ng1, m = t - q * Ng1;
+      int za = z0 + 2 * q, zb = za + 1;
+      double2 a = make_double2(0, 0), b = make_double2(0, 0);
+      int mm = m <= h ? m : Ng1 - m;
+      double sg = m <= h ? 1.0 : -1.0;
+      const double *ra = P + (size_t)za * igal;
+      a = make_double2(ra[2 * mm], sg * ra[2 * mm + 1]);
+      if (mm == 0 || mm == h) a.y = 0.0;
+      if (zb < z0 + nzr) {
+        const double *rb = P + (size_t)zb * igal;
+        b = make_double2(rb[2 * mm], sg * rb[2 * mm + 1]);
+        if (mm == 0 || mm == h) b.y = 0.0;
+      }
+      A[q * Ng1 + m] = make_double2(a.x - b.y, a.y + b.x);
+    }
+    __syncthreads();
+    double2 *R = fft_lds<double2>(A, B, Ng1, npair, Ng1, +1, twx);
+    // keep the packed cc(n/2) entries where the reference leaves them
+    for (int t = threadIdx.x; t < npair * Ng1; t += blockDim.x) {
+      int q = t / Ng1, n = t - q * Ng1;
+      int za = z0 + 2 * q, zb = za + 1;
+      double2 v = R[q * Ng1 + n];
+      P[(size_t)za * igal + n] = v.x;
+      if (zb < z0 + nzr) P[(size_t)zb * igal + n] = v.y;
+    }
+  } else {
+    for (int t = threadIdx.x; t < npair * Ng1; t += blockDim.x) {
+      int q = t / Ng1, n = t - q * Ng1;
+      int za = z0 + 2 * q, zb = za + 1;
+      double a = P[(size_t)za * igal + n];
+      double b = zb < z0 + nzr ? P[(size_t)zb * igal + n] : 0.0;
+      A[q * Ng1 + n] = make_double2(a, b);
+    }
+    __syncthreads();
+    double2 *R = fft_lds<double2>(A, B, Ng1, npair, Ng1, -1, twx);
+    const double inv = 1.0 / Ng1;
+    for (int t = threadIdx.x; t < npair * (h + 1); t += blockDim.x) {
+      int q = t / (h + 1), m = t - q * (h + 1);
+      int za = z0 + 2 * q, zb = za + 1;
+      double2 Ym = R[q * Ng1 + m], Yc = R[q * Ng1 + (m == 0 ? 0 : Ng1 - m)];
+      double2 Pa, Pb;
+      if (m == 0 || m == h) {
+        Pa = make_double2(0.5 * (Ym.x + Yc.x) * inv, 0.0);
+        Pb = make_double2(0.5 * (Ym.y + Yc.y) * inv, 0.0);
+      } else {
+        Pa = make_double2(0.5 * (Ym.x + Yc.x) * inv, 0.5 * (Ym.y - Yc.y) * inv);
+        Pb = make_double2(0.5 * (Ym.y + Yc.y) * inv, -0.5 * (Ym.x - Yc.x) * inv);
+      }
+      P[(size_t)za * igal + 2 * m] = Pa.x;
+      P[(size_t)za * igal + 2 * m + 1] = Pa.y;
+      if (zb < z0 + nzr) {
+        P[(size_t)zb * igal + 2 * m] = Pb.x;
+        P[(size_t)zb * igal + 2 * m + 1] = Pb.y;
+      }
+    }
+  }
+}
+
+hipError_t launch_plane_f2p(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
+                            const double2 *twx, hipStream_t s) {
+  int nv = N1 / 2 + 1, TI = plane_tile(Ng2), TZ = plane_tile(Ng1);
+  dim3 gz((nv + TI - 1) / TI, nplanes), gx((Ng2 + 2 * TZ - 1) / (2 * TZ), nplanes);
+  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), 2 * TI * Ng2 * sizeof(double2), s, planes, N1, Ng1, Ng2, +1,
+                     twz);
+  hipLaunchKernelGGL(k_plane_x, gx, dim3(256), 2 * TZ * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, +1, twx);
+  return hipGetLastError();
+}
+
+hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
+                            const double2 *twx, hipStream_t s) {
+  int nv = N1 / 2 + 1, TI = plane_tile(Ng2), TZ = plane_tile(Ng1);
+  dim3 gz((nv + TI - 1) / TI, nplanes), gx((Ng2 + 2 * TZ - 1) / (2 * TZ), nplanes);
+  hipLaunchKernelGGL(k_plane_x, gx, dim3(256), 2 * TZ * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, -1, twx);
+  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), 2 * TI * Ng2 * sizeof(double2), s, planes, N1, Ng1, Ng2, -1,
+                     twz);
+  return hipGetLastError();
+}
+
+// LUsolU_W (vgrid = 0, two right-hand sides) / LUsolV (vgrid = 1, one)
+__global__ void __launch_bounds__(64) k_lusol_ref(const Chan *chp, int kRK, int vgrid, const double2 *ra,
+                                                  const double2 *rb, double2 *xa, double2 *xb, double *scr,
+                                                  int ncol) {
+  const Chan &ch = *chp;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int i = c % ch.nxh, k = c / ch.nxh;
+  const int n = vgrid ? ch.nyv : ch.nyu;
+  const double *dy2 = vgrid ? ch.dyv2i : ch.dyu2i;
+  const double beta = rk_b(kRK) * ch.scal[SC_DT] / ch.Re;
+  const double k2x = ch.k2x[i], k2z = ch.k2z[k];
+  const size_t base = (size_t)c * n;
+  double *sc = scr + base;
+  double cprev = vgrid ? 0.0 : ch.gw1;
+  double2 ya = ra[base], yb = rb ? rb[base] : make_double2(0, 0);
+  xa[base] = ya;
+  if (rb) xb[base] = yb;
+  sc[0] = cprev;
+  for (int j = 1; j < n; j++) {
+    double a1, a2, a3;
+    if (j <= n - 2) {
+      a1 = -beta * dy2[3 * j];
+      a2 = 1.0 - beta * ((dy2[3 * j + 1] + k2x) + k2z);
+      a3 = -beta * dy2[3 * j + 2];
+    } else {
+      a1 = vgrid ? 0.0 : ch.gw2;
+      a2 = 1.0;
+      a3 = 0.0;
+    }
+    double d = 1.0 / (a2 - a1 * cprev);
+    double2 r = ra[base + j];
+    ya = make_double2((r.x - a1 * ya.x) * d, (r.y - a1 * ya.y) * d);
+    xa[base + j] = ya;
+    if (rb) {
+      double2 q = rb[base + j];
+      yb = make_double2((q.x - a1 * yb.x) * d, (q.y - a1 * yb.y) * d);
+      xb[base + j] = yb;
+    }
+    cprev = a3 * d;
+    sc[j] = cprev;
+  }
+  for (int j = n - 2; j >= 0; j--) {
+    double cj = sc[j];
+    double2 x = xa[base + j];
+    ya = make_double2(x.x - cj * ya.x, x.y - cj * ya.y);
+    xa[base + j] = ya;
+    if (rb) {
+      double2 w = xb[base + j];
+      yb = make_double2(w.x - cj * yb.x, w.y - cj * yb.y);
+      xb[base + j] = yb;
+    }
+  }
+}
+
+hipError_t launch_lusol_ref2(const Chan *d_chan, int kRK, int vgrid, const double2 *rhs_a,
+                             const double2 *rhs_b, double2 *x_a, double2 *x_b, double *scr, int ncol,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_lusol_ref, dim3((ncol + 63) / 64), dim3(64), 0, s, d_chan, kRK, vgrid, rhs_a, rhs_b,
+                     x_a, x_b, scr, ncol);
+  return hipGetLastError();
+}
+
+}  // namespace lh

@@ -1,0 +1,128 @@
+"""Host-side driver of liblhgpu: an ensemble of channels on one GPU.
+
+Mirrors the reference's main program (littleharsh.f90:58-230): setup
+(`start`), initial fields (`getini`), the RK3 loop, and field access in the
+reference's column layout.  All compute goes through the C-ABI of
+liblhgpu.so; there is no CPU path.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import LhConfig, check
+
+
+def _rows(cfg, name):
+    ny = cfg["ny"]
+    return {"u1": ny + 1, "u3": ny + 1, "u2": ny, "p": ny - 1, "Nu1": ny - 1, "Nu3": ny - 1,
+            "Nu2": ny - 2, "r1": ny + 1, "r3": ny + 1, "r2": ny, "psi": ny - 1,
+            "div": ny - 1}[name]
+
+
+def _ncol(cfg):
+    return (cfg["N1"] // 2 + 1) * cfg["N2"]
+
+
+class Solver:
+    """`cfgs`: list of channel dicts (see configs.channel).  precision:
+    "fp64" (plain) or "mixed"."""
+
+    def __init__(self, cfgs, precision="fp64"):
+        L = _lib.load()
+        self.cfgs = [dict(c) for c in cfgs]
+        n = len(cfgs)
+        arr = (LhConfig * n)()
+        prec = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
+        self.precision = precision
+        for q, c in enumerate(cfgs):
+            s = arr[q]
+            for f, _ in LhConfig._fields_:
+                if f == "precision":
+                    s.precision = prec
+                elif f in c:
+                    setattr(s, f, c[f])
+        h = ctypes.c_void_p()
+        check(L.lh_ctx_create(arr, n, ctypes.byref(h)))
+        self.h = h
+        self.L = L
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nchan(self):
+        return len(self.cfgs)
+
+    def grid(self, ch=0):
+        c = self.cfgs[ch]
+        ny = c["ny"]
+        yu, yv = np.zeros(ny + 1), np.zeros(ny)
+        du, dv = np.zeros(ny + 1), np.zeros(ny)
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        check(self.L.lh_get_grid(self.h, ch, dp(yu), dp(yv), dp(du), dp(dv)))
+        return dict(yu=yu, yv=yv, dthdyu=du, dthdyv=dv)
+
+    def set_grid(self, ch, yu, yv, dthdyu, dthdyv):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (yu, yv, dthdyu, dthdyv)]
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        check(self.L.lh_set_grid(self.h, ch, *[dp(a) for a in arrs]))
+
+    def upload(self, ch, name, arr):
+        c = self.cfgs[ch]
+        a = np.ascontiguousarray(arr, np.complex128)
+        if a.shape != (_ncol(c), _rows(c, name)):
+            raise ValueError("field %s: shape %s, expected %s" % (name, a.shape, (_ncol(c), _rows(c, name))))
+        check(self.L.lh_field_upload(self.h, ch, _lib.FIELDS[name],
+                                     a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def download(self, ch, name):
+        c = self.cfgs[ch]
+        out = np.zeros((_ncol(c), _rows(c, name)), np.complex128)
+        check(self.L.lh_field_download(self.h, ch, _lib.FIELDS[name],
+                                       out.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def set_fields(self, ch, fields):
+        for k, v in fields.items():
+            self.upload(ch, k, v)
+
+    def scalars(self, ch=0):
+        o = np.zeros(8)
+        check(self.L.lh_get_scalars(self.h, ch, o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return dict(t=o[0], dt=o[1], mpgx=o[2], QxT=o[3], Qx=o[4], dtv=o[5], iter=int(o[7]))
+
+    def set_scalars(self, ch, t=0.0, mpgx=None, QxT=None, iter=0):
+        s = self.scalars(ch)
+        check(self.L.lh_set_scalars(self.h, ch, float(t), float(s["mpgx"] if mpgx is None else mpgx),
+                                    float(s["QxT"] if QxT is None else QxT), int(iter)))
+
+    def init_flowrate(self):
+        check(self.L.lh_init_flowrate(self.h))
+
+    def substep(self, kRK):
+        check(self.L.lh_substep(self.h, kRK))
+
+    def step(self, n=1):
+        check(self.L.lh_step(self.h, n))
+
+    def phase(self, phase, kRK):
+        check(self.L.lh_run_phase(self.h, phase, kRK))
+
+    def time_kernels(self, nsub):
+        """HIP-event times (ms, summed over nsub substeps) of the five kernels
+        [zinv, xprod, zfwd, rhs, solve]."""
+        ms = np.zeros(5)
+        check(self.L.lh_time_kernels(self.h, int(nsub), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return ms
+
+    def sync(self):
+        check(self.L.lh_sync(self.h))

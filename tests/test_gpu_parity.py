@@ -1,0 +1,146 @@
+"""GPU parity: liblhgpu (through the C-ABI) against the FP64 CPU oracle on the
+same seeded fields.  Bar: relative L2 <= 1e-12 per field after one RK3 step
+in FP64 (and per phase); mixed precision <= 1e-5."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import init_fields, oracle_for, relerr, small_cfg
+from rnl_multiblock_nofft_amd import Solver, _lib, configs
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_pair(cfg, seed=1234, precision="fp64"):
+    s = Solver([cfg], precision=precision)
+    o = oracle_for(cfg)
+    g = s.grid(0)
+    og = o.grid()
+    for k in ("yu", "yv", "dthdyu", "dthdyv"):
+        assert np.array_equal(g[k], og[k]), k
+    f = init_fields(cfg, g, seed)
+    for name in ("u1", "u2", "u3", "p"):
+        s.upload(0, name, f[name])
+        o.set(name, f[name])
+    s.init_flowrate()
+    o.init_flowrate()
+    assert s.scalars(0)["QxT"] == o.scalars()["QxT"]
+    return s, o, f
+
+
+CASES = {
+    "N8_ny17": small_cfg(8, 17),
+    "N16_ny33_poly_ctP": small_cfg(16, 33, grid_kind=0, flag_ctpress=1, mpgx=-2.0 / 4200.0),
+    "N32_ny153": configs.smooth_32()[0],
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_phases_kRK(name):
+    cfg = CASES[name]
+    say("case", name)
+    s, o, f = make_pair(cfg)
+    dt = cfg["dt"]
+    for kRK in (1, 2, 3):
+        s.phase(_lib.PHASE_NONLINEAR, kRK)
+        s.phase(_lib.PHASE_RHS, kRK)
+        o.phase("rhs0", kRK)
+        o.phase("nonlinear", kRK)
+        for nm in ("Nu1", "Nu2", "Nu3"):
+            e = relerr(s.download(0, nm), o.get(nm))
+            say(name, kRK, nm, e)
+            assert e <= TOL, (nm, e)
+        for nm, un, dn in (("r1", "u1", "du1"), ("r2", "u2", "du2"), ("r3", "u3", "du3")):
+            ref = o.get(un) + dt * o.get(dn)
+            ref[:, 0] = 0
+            ref[:, -1] = 0
+            e = relerr(s.download(0, nm), ref)
+            say(name, kRK, nm, e)
+            assert e <= TOL, (nm, e)
+        s.phase(_lib.PHASE_SOLVE, kRK)
+        o.phase("solve_uw", kRK)
+        o.phase("solve_v", kRK)
+        if cfg["flag_ctpress"] == 0:
+            o.phase("meanflow_ctU", kRK)
+        o.phase("solve_p", kRK)
+        o.phase("v_corr", kRK)
+        for nm in ("u1", "u2", "u3", "p"):
+            e = relerr(s.download(0, nm), o.get(nm))
+            say(name, kRK, nm, e)
+            assert e <= TOL, (nm, e)
+        d = s.download(0, "div")
+        assert np.abs(d).max() < 1e-10 * max(1.0, np.abs(s.download(0, "u1")).max())
+        assert abs(s.scalars(0)["mpgx"] - o.scalars()["mpgx"]) <= 1e-12 * max(1e-300, abs(o.scalars()["mpgx"])) + 1e-18
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_one_step_graph(name):
+    cfg = CASES[name]
+    say("case", name)
+    s, o, f = make_pair(cfg, seed=99)
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say(name, nm, e)
+        assert e <= TOL, (nm, e)
+    # a second and third step replay the captured graph
+    s.step(2)
+    o.step(2)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say(name, "3 steps", nm, e)
+        assert e <= 1e-11, (nm, e)
+
+
+def test_ensemble_uneven():
+    cfgs = [small_cfg(8, 17), small_cfg(16, 33), small_cfg(8, 17, grid_kind=0), small_cfg(16, 25)]
+    say("ensemble", len(cfgs))
+    s = Solver(cfgs)
+    oracles = []
+    for q, c in enumerate(cfgs):
+        o = oracle_for(c)
+        f = init_fields(c, s.grid(q), seed=1234 + q)
+        for name in ("u1", "u2", "u3", "p"):
+            s.upload(q, name, f[name])
+            o.set(name, f[name])
+        o.init_flowrate()
+        oracles.append(o)
+    s.init_flowrate()
+    s.step(2)
+    for q, o in enumerate(oracles):
+        o.step(2)
+        for nm in ("u1", "u2", "u3", "p"):
+            e = relerr(s.download(q, nm), o.get(nm))
+            say("chan", q, nm, e)
+            assert e <= 1e-11, (q, nm, e)
+
+
+def test_mixed_precision_error():
+    cfg = configs.smooth_32()[0]
+    say("mixed")
+    s, o, f = make_pair(cfg, precision="mixed")
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("mixed", nm, e)
+        assert e <= 1e-5, (nm, e)
+
+
+def test_bad_config_raises():
+    cfg = small_cfg(8, 17)
+    cfg["Ngal1"] = 10  # below the 3/2 rule
+    with pytest.raises(_lib.LhError):
+        Solver([cfg])
+    cfg = small_cfg(8, 17, dt=0.0)
+    with pytest.raises(_lib.LhError):
+        Solver([cfg])
