@@ -765,6 +765,52 @@ static void lu_sol_c(const double *a, const cplx *rhs, cplx *x, int n) {
   for (int j = n - 2; j >= 0; j--) x[j] = x[j] - a[3 * j + 2] * x[j + 1];
 }
 
+/* matrices of LU_buildU / LU_buildV (tridLU_3D.f90:257-329) */
+static void build_helm(const lho_state *s, int kRK, int c, int vgrid, double *a) {
+  int n = vgrid ? s->nyv : s->nyu;
+  const double *dy = vgrid ? s->dyv2i : s->dyu2i;
+  double beta = s->bRK[kRK - 1] * s->dt / s->c.Re;
+  int i = c % s->nxh, k = c / s->nxh;
+  double k2x = s->k2x[i], k2z = s->k2z[k];
+  a[0] = 0.0;
+  a[1] = 1.0;
+  a[2] = vgrid ? 0.0 : s->gw1;
+  for (int j = 1; j <= n - 2; j++) {
+    const double *d = dy + 3 * j;
+    a[3 * j] = -beta * d[0];
+    a[3 * j + 1] = 1.0 - beta * (d[1] + k2x + k2z);
+    a[3 * j + 2] = -beta * d[2];
+  }
+  a[3 * (n - 1)] = vgrid ? 0.0 : s->gw2;
+  a[3 * (n - 1) + 1] = 1.0;
+  a[3 * (n - 1) + 2] = 0.0;
+}
+
+/* LUsolU_W / LUsolV on arbitrary right-hand sides in the reference layout
+ * (ncol x rows, re/im interleaved); vgrid selects LUsolV */
+void lho_lusol_helm(const lho_state *s, int kRK, int vgrid, const double *rhs, double *x) {
+  int n = vgrid ? s->nyv : s->nyu;
+  double *a = malloc(sizeof(double) * 3 * n);
+  for (int c = 0; c < s->ncol; c++) {
+    build_helm(s, kRK, c, vgrid, a);
+    lu_dec(a, n);
+    lu_sol_c(a, (const cplx *)rhs + (size_t)c * n, (cplx *)x + (size_t)c * n, n);
+  }
+  free(a);
+}
+
+/* LUsolP with the prefactored LU_buildP operator on p rows 1..ny-1 */
+void lho_lusolp(const lho_state *s, double *reim) {
+  int nyu = s->nyu, ns = 1, ne = nyu - 2, nr = ne - ns + 1;
+  for (int c = 0; c < s->ncol; c++) {
+    cplx *ps = (cplx *)reim + (size_t)c * nr - ns;
+    const double *a = s->dg + (size_t)c * nyu * 3;
+    ps[ns] = ps[ns] * a[3 * ns + 1];
+    for (int j = ns + 1; j <= ne; j++) ps[j] = (ps[j] - a[3 * j] * ps[j - 1]) * a[3 * j + 1];
+    for (int j = ne - 1; j >= ns; j--) ps[j] = ps[j] - a[3 * j + 2] * ps[j + 1];
+  }
+}
+
 /* solveU_W + LUsolU_W + LU_buildU, littleharsh.f90:791-828,
  * tridLU_3D.f90:55-132,257-291 */
 void lho_solve_uw(lho_state *s, int kRK) {

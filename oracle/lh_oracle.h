@@ -93,7 +93,10 @@ void lho_phys_to_four_plane(const lho_state *s, double *plane);
 void lho_cft(double *c, int n, int ise, int isa, int m, int iopt);
 void lho_rft(double *c, int n, int isa, int m, int iopt);
 
-/* Batched tridiagonal helpers on the reference's matrices */
+/* Batched tridiagonal helpers on the reference's matrices (reference layout,
+ * ncol x rows complex, re/im interleaved) */
+void lho_lusol_helm(const lho_state *s, int kRK, int vgrid, const double *rhs, double *x);
+void lho_lusolp(const lho_state *s, double *reim);
 void lho_lusol0(const lho_state *s, int kRK, double *x); /* x(0:ny), LUsol0 */
 
 #ifdef __cplusplus

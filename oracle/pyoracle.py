@@ -78,6 +78,8 @@ def lib():
                               ctypes.c_int, ctypes.c_int]
         L.lho_rft.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.lho_lusol0.argtypes = [vp, ctypes.c_int, dp]
+        L.lho_lusol_helm.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.lho_lusolp.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -198,6 +200,17 @@ class Oracle:
         p = np.ascontiguousarray(plane, dtype=np.float64).copy()
         lib().lho_phys_to_four_plane(self.h, _dp(p))
         return p
+
+    def lusol_helm(self, kRK, vgrid, rhs):
+        r = np.ascontiguousarray(rhs, dtype=np.complex128)
+        x = np.zeros_like(r)
+        lib().lho_lusol_helm(self.h, kRK, int(vgrid), _dp(r.view(np.float64)), _dp(x.view(np.float64)))
+        return x
+
+    def lusolp(self, rhs):
+        x = np.ascontiguousarray(rhs, dtype=np.complex128).copy()
+        lib().lho_lusolp(self.h, _dp(x.view(np.float64)))
+        return x
 
     def lusol0(self, kRK, x):
         x = np.ascontiguousarray(x, dtype=np.float64).copy()

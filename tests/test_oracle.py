@@ -1,0 +1,274 @@
+"""Pins the CPU oracle (the checker) against the reference's own code and
+data and against closed forms.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import oracle_for, relerr, small_cfg
+from oracle import pyoracle as po
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REF = "/root/reference"
+
+
+# --- FFTPACK conventions: DFT brute force (cft_buff.f:1-65, rft_buff.f:24-90)
+@pytest.mark.parametrize("n", [6, 12, 24, 48, 96, 192, 384, 64, 128])
+def test_cft_rft_match_dft(n):
+    rng = np.random.default_rng(n)
+    z = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    jk = np.outer(np.arange(n), np.arange(n))
+    Wf = np.exp(-2j * np.pi * jk / n)
+    c = np.empty(2 * n)
+    c[0::2], c[1::2] = z.real, z.imag
+    f = c.copy()
+    po.cft(f, n, 2, 2 * n, 1, -1)               # forward, scaled by 1/n
+    assert np.allclose(f[0::2] + 1j * f[1::2], Wf @ z / n, atol=1e-13, rtol=0)
+    b = c.copy()
+    po.cft(b, n, 2, 2 * n, 1, +1)               # inverse, unscaled, e^{+i}
+    assert np.allclose(b[0::2] + 1j * b[1::2], np.conj(Wf) @ z, atol=1e-12 * n, rtol=0)
+    x = rng.standard_normal(n)
+    r = np.zeros(n + 2)
+    r[:n] = x
+    po.rft(r, n, n + 2, 1, -1)
+    cc = r[0::2] + 1j * r[1::2]
+    X = (Wf @ x / n)[: n // 2 + 1]
+    assert np.allclose(cc, X, atol=1e-14)
+    assert r[1] == 0.0 and r[n + 1] == 0.0       # imag(cc(0)), imag(cc(n/2)) zeroed
+    # inverse: x_j = sum of the Hermitian extension, imag(cc(0)), imag(cc(n/2)) ignored
+    cc2 = X.copy()
+    cc2[0] += 5j
+    cc2[-1] += 3j
+    r2 = np.zeros(n + 2)
+    r2[0::2], r2[1::2] = cc2.real, cc2.imag
+    po.rft(r2, n, n + 2, 1, +1)
+    assert np.allclose(r2[:n], x, atol=1e-13)
+
+
+def test_plane_transforms_match_reference_fftpack():
+    """four_to_phys_du / phys_to_four_du against the reference's FFTPACK
+    (compiled unmodified from cft_buff.f / rft_buff.f), golden vectors."""
+    g = np.load(os.path.join(GOLD, "fftpack_planes.npz"))
+    for N in (8, 32):
+        Ng = 3 * N // 2
+        o = po.Oracle(N1=N, N2=N, ny=9, Re=100.0)
+        m = np.zeros((Ng, Ng + 2), bool)
+        m[:, :Ng] = True
+        a = o.four_to_phys(g["f2p_in_%d" % N])
+        b = g["f2p_out_%d" % N]
+        assert np.abs(a - b)[m.ravel()].max() <= 1e-14 * np.abs(b).max()
+        m2 = np.zeros((Ng, Ng + 2), bool)
+        m2[:, : N + 2] = True
+        a = o.phys_to_four(g["p2f_in_%d" % N])
+        b = g["p2f_out_%d" % N]
+        assert np.abs(a - b)[m2.ravel()].max() <= 1e-14 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("tag,cfg", [
+    ("poly", dict(N1=4, N2=4, ny=17, Re=3250.0, grid_kind=0, dyq=10.0, ppp=5, dt=2e-3)),
+    ("tanh", dict(N1=4, N2=4, ny=21, Re=4200.0, grid_kind=1, gamma=2.0, dt=1e-3))])
+def test_solves_match_reference_tridlu(tag, cfg):
+    """LUsolU_W / LUsolV / LUsolP / LUsol0 against the reference's own
+    tridLU_3D.f90 (compiled unmodified), golden vectors."""
+    g = np.load(os.path.join(GOLD, "tridlu.npz"))
+    o = po.Oracle(**cfg)
+    og = o.grid()
+    for k in ("yu", "yv", "dthdyu", "dthdyv"):
+        assert np.array_equal(og[k], g[tag + "_" + k])
+    for kRK in (1, 2, 3):
+        u = o.lusol_helm(kRK, 0, g["%s_k%d_ru" % (tag, kRK)])
+        w = o.lusol_helm(kRK, 0, g["%s_k%d_rw" % (tag, kRK)])
+        v = o.lusol_helm(kRK, 1, g["%s_k%d_rv" % (tag, kRK)])
+        assert relerr(u, g["%s_k%d_u" % (tag, kRK)]) <= 1e-15
+        assert relerr(w, g["%s_k%d_w" % (tag, kRK)]) <= 1e-15
+        assert relerr(v, g["%s_k%d_v" % (tag, kRK)]) <= 1e-15
+        x0 = o.lusol0(kRK, g["%s_k%d_r0" % (tag, kRK)])
+        assert relerr(x0, g["%s_k%d_x0" % (tag, kRK)]) <= 1e-15
+    xp = o.lusolp(g[tag + "_rp"])
+    assert relerr(xp, g[tag + "_xp"]) <= 1e-15
+
+
+def test_grid_and_scalars_match_reference_case():
+    """ygrid (start.f90:693-1077), dtv (start.f90:456) and flowrateIm of the
+    reference's own 32x32x153 case against its fixture header and history."""
+    g = np.load(os.path.join(GOLD, "reference_case.npz"))
+    o = po.Oracle(N1=32, N2=32, ny=153, Re=3250.0, grid_kind=0, dyq=10.0, ppp=5,
+                  flag_ctpress=1, mpgx=-3.0675e-3)
+    og = o.grid()
+    for k in ("yu", "yv", "dthdyu", "dthdyv"):
+        assert np.array_equal(og[k], g[k]), k
+    h0 = g["hist_iter0"]  # t, dtv, dtc, dt, err, Qx, mpgx, Umax, Uslip
+    assert o.scalars()["dtv"] == h0[1]
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "output")), reason="reference fixtures absent")
+def test_one_step_close_to_reference_snapshot():
+    """One step from the reference's input fixture against its iter-1
+    snapshot.  The snapshot comes from the triad-list path, whose treatment of
+    the non-Hermitian x-mean modes and the x-Nyquist modes of this input
+    differs from the FFT path's; the bound is therefore loose.  It catches any
+    indexing error, which would be O(1)."""
+    from rnl_multiblock_nofft_amd import fieldio
+
+    inp = {f: fieldio.read_field(os.path.join(REF, "input", "%s_0032x0032x0153_t00000.dat" % f))
+           for f in ("u1", "u2", "u3", "p")}
+    out = {f: fieldio.read_field(os.path.join(REF, "output", "%s_0032x0032x0153_t00052.dat" % f))
+           for f in ("u1", "u2", "u3", "p")}
+    o = po.Oracle(N1=32, N2=32, ny=153, Re=3250.0, grid_kind=0, dyq=10.0, ppp=5,
+                  flag_ctpress=1, mpgx=-3.0675e-3, fixed_dt=0, CFL=0.7)
+    for f in inp:
+        o.set(f, inp[f]["field"])
+    o.init_flowrate()
+    assert o.scalars()["Qx"] == pytest.approx(1.6539730978573473, rel=1e-15)
+    o.step(1)
+    s = o.scalars()
+    assert s["dt"] == out["u1"]["t"]                     # 2.5 dtv, exactly
+    bounds = {"u1": 2e-3, "u2": 1e-2, "u3": 2e-2, "p": 0.5}
+    for f in inp:
+        assert relerr(o.get(f), out[f]["field"]) < bounds[f], f
+    u1 = o.get("u1")
+    g = o.grid()
+    Q = sum((g["yv"][j] - g["yv"][j - 1]) * u1[0, j].real for j in range(1, 153))
+    assert Q == pytest.approx(1.6539375434319405, rel=1e-7)
+
+
+def test_poiseuille_is_discrete_steady_state():
+    """With N=0 and a constant pressure gradient, the discrete Poiseuille
+    profile (D2 u = Re mpgx with the no-slip ghost rows) is a fixed point of
+    every RK3 substep (d_k = 2 b_k for these coefficients).  A wrong sign or
+    coefficient on the mpgx, viscous or solve terms breaks it."""
+    from scipy.linalg import solve_banded
+
+    cfg = small_cfg(8, 33, grid_kind=1, flag_ctpress=1, mpgx=-2.0 / 4200.0, dt=2e-3)
+    o = oracle_for(cfg)
+    g = o.grid()
+    ny = cfg["ny"]
+    nyu = ny + 1
+    yu, yv = g["yu"], g["yv"]
+    D = g["dyu2i"]
+    gw = (yu[0] - yv[0]) / (yu[1] - yv[0])
+    # assemble (dyu2i rows 1..ny-1) with u0 = gw u1, u_n = gw u_{n-1}
+    n = nyu - 2
+    ab = np.zeros((3, n))
+    rhs = np.full(n, cfg["Re"] * cfg["mpgx"])
+    for r in range(n):
+        j = r + 1
+        ab[1, r] = D[j, 1] + (gw * D[j, 0] if r == 0 else 0) + (gw * D[j, 2] if r == n - 1 else 0)
+        if r > 0:
+            ab[2, r - 1] = D[j, 0]
+        if r < n - 1:
+            ab[0, r + 1] = D[j, 2]
+    ui = solve_banded((1, 1), ab, rhs)
+    u = np.zeros(nyu)
+    u[1:-1] = ui
+    u[0] = gw * u[1]
+    u[-1] = gw * u[-2]
+    U1 = np.zeros((o.ncol, nyu), complex)
+    U1[0] = u
+    o.set("u1", U1)
+    o.init_flowrate()
+    o.step(2)
+    assert np.abs(o.get("u1")[0] - u).max() <= 1e-12 * np.abs(u).max()
+    assert np.abs(o.get("u2")).max() <= 1e-15
+
+
+def test_constant_flow_rate_and_divergence():
+    """flowrate_corr keeps Qx = QxT to round-off (littleharsh.f90:476-509) and
+    the projection leaves the discrete divergence at round-off."""
+    from helpers import init_fields
+
+    cfg = small_cfg(16, 33)
+    o = oracle_for(cfg)
+    f = init_fields(cfg, o.grid(), seed=5)
+    for k, v in f.items():
+        o.set(k, v)
+    o.init_flowrate()
+    QxT = o.scalars()["QxT"]
+    o.step(3)
+    g = o.grid()
+    u1 = o.get("u1")
+    Q = sum((g["yv"][j] - g["yv"][j - 1]) * u1[0, j].real for j in range(1, cfg["ny"]))
+    assert abs(Q - QxT) <= 1e-14 * abs(QxT)
+    assert np.abs(o.divergence()).max() <= 1e-12 * np.abs(u1).max()
+
+
+def _brute_nonlinear_plane(cu, cv, cw, N1, N2, Ng1, Ng2, alp, bet, vplane):
+    """The plain-path nonlinear term of one plane from its definition by
+    explicit double sums: synthesise u on the Ng1 x Ng2 grid from the retained
+    half spectrum (x-mean taken as its real part, FFTPACK's inverse rft),
+    multiply, project with the forward DFT / (Ng1 Ng2), keep i <= N1/2 and the
+    retained kz, apply ikx / ikz with kz = -N2/2 zeroed."""
+    nxh = N1 // 2 + 1
+    ks = np.array([k if k < N2 // 2 else k - N2 for k in range(N2)])
+    x = np.arange(Ng1) / Ng1
+    z = np.arange(Ng2) / Ng2
+
+    def synth(c):
+        c = c.reshape(N2, nxh)
+        Ez = np.exp(2j * np.pi * np.outer(z, ks))          # (Ng2, N2)
+        cc = Ez @ c                                          # (Ng2, nxh)
+        cc[:, 0] = cc[:, 0].real
+        Ex = np.exp(2j * np.pi * np.outer(np.arange(1, nxh), np.arange(Ng1)) / Ng1)
+        return cc[:, 0:1].real + 2 * (cc[:, 1:] @ Ex).real   # (Ng2, Ng1)
+
+    def proj(q):
+        Ex = np.exp(-2j * np.pi * np.outer(np.arange(Ng1), np.arange(nxh)) / Ng1)
+        Ez = np.exp(-2j * np.pi * np.outer(ks, np.arange(Ng2)) / Ng2)
+        return (Ez @ q @ Ex / (Ng1 * Ng2)).reshape(-1)     # (N2*nxh)
+
+    U, V, W = synth(cu), synth(cv), synth(cw)
+    kx = alp * (np.arange(N2 * nxh) % nxh)
+    kz = bet * ks[np.arange(N2 * nxh) // nxh].astype(float)
+    live = ks[np.arange(N2 * nxh) // nxh] != -(N2 // 2)
+    if vplane:  # U = u interpolated to the face, V = v, W = w interpolated
+        UV, WV = proj(U * V), proj(W * V)
+        return dict(N2=np.where(live, 1j * kx * UV + 1j * kz * WV, 0), uv=UV, wv=WV)
+    UU, UW, VV, WW = proj(U * U), proj(U * W), proj(V * V), proj(W * W)
+    return dict(N1=np.where(live, 1j * kx * UU + 1j * kz * UW, 0),
+                N3=np.where(live, 1j * kx * UW + 1j * kz * WW, 0), vv=VV)
+
+
+def test_nonlinear_term_matches_brute_force_convolution():
+    """Nu of the oracle (interp + planes + FFTPACK-convention transforms +
+    dealiasing + d/dy of the fluxes) against explicit O(N^4) sums."""
+    from helpers import init_fields
+
+    cfg = small_cfg(8, 9)
+    N1, N2, Ng1, Ng2 = 8, 8, 12, 12
+    o = oracle_for(cfg)
+    g = o.grid()
+    f = init_fields(cfg, g, seed=3)
+    rng = np.random.default_rng(11)
+    # break Hermitian symmetry of the x-mean modes and fill the Nyquist modes
+    # too: the definition above fixes how the FFT path treats them
+    for k in ("u1", "u2", "u3"):
+        f[k] = f[k] + 0.01 * (rng.standard_normal(f[k].shape) + 1j * rng.standard_normal(f[k].shape))
+    for k, v in f.items():
+        o.set(k, v)
+    o.phase("nonlinear", 1)  # du is zero, so du = -gRK * Nu afterwards
+    yu, yv, du, dv = g["yu"], g["yv"], g["dthdyu"], g["dthdyv"]
+    ny, nyu = cfg["ny"], cfg["ny"] + 1
+    u1, u2, u3 = f["u1"], f["u2"], f["u3"]
+    u1i = ((yv - yu[:-1]) * u1[:, 1:] + (yu[1:] - yv) * u1[:, :-1]) / (yu[1:] - yu[:-1])
+    u3i = ((yv - yu[:-1]) * u3[:, 1:] + (yu[1:] - yv) * u3[:, :-1]) / (yu[1:] - yu[:-1])
+    u2i = np.zeros_like(u1)
+    for j in range(1, nyu - 1):
+        u2i[:, j] = ((yu[j] - yv[j - 1]) * u2[:, j] + (yv[j] - yu[j]) * u2[:, j - 1]) / (yv[j] - yv[j - 1])
+    N1x = np.zeros_like(u1)
+    N3x = np.zeros_like(u1)
+    vv = np.zeros_like(u1)
+    N2x = np.zeros_like(u2)
+    uv = np.zeros_like(u2)
+    wv = np.zeros_like(u2)
+    for j in range(1, nyu - 1):
+        r = _brute_nonlinear_plane(u1[:, j], u2i[:, j], u3[:, j], N1, N2, Ng1, Ng2, 1.0, 2.0, False)
+        N1x[:, j], N3x[:, j], vv[:, j] = r["N1"], r["N3"], r["vv"]
+    for j in range(1, ny - 1):
+        r = _brute_nonlinear_plane(u1i[:, j], u2[:, j], u3i[:, j], N1, N2, Ng1, Ng2, 1.0, 2.0, True)
+        N2x[:, j], uv[:, j], wv[:, j] = r["N2"], r["uv"], r["wv"]
+    Nu1 = N1x[:, 1:-1] + (uv[:, 1:] - uv[:, :-1]) * du[1:-1]
+    Nu3 = N3x[:, 1:-1] + (wv[:, 1:] - wv[:, :-1]) * du[1:-1]
+    Nu2 = N2x[:, 1:-1] + (vv[:, 2:-1] - vv[:, 1:-2]) * dv[1:-1]
+    assert relerr(o.get("Nu1"), Nu1) <= 1e-12
+    assert relerr(o.get("Nu2"), Nu2) <= 1e-12
+    assert relerr(o.get("Nu3"), Nu3) <= 1e-12
