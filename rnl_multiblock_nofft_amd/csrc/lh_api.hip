@@ -248,59 +248,50 @@ int upload_chan_constants(lh_ctx *x, HostChan &h) {
 int alloc_chan(lh_ctx *x, HostChan &h) {
   const lh_config &c = h.cfg;
   const size_t fld = (size_t)h.nyu * h.ldc;  // complex elements per field
-  const size_t nfield = 18;
   const size_t ninter = (size_t)h.nslot * c.Ngal2 * h.nxh;
-  size_t bytes = 0;
-  bytes += nfield * al<double2>(fld);
-  bytes += al<double>(fld);  // scr
-  bytes += al<double>(h.nyu) * 4 + al<double>(h.nyv) * 2 + al<double>(3 * h.nyu) + al<double>(3 * h.nyv);
-  bytes += al<double>(h.nxh) * 2 + al<double>(c.N2) * 2 + al<double2>(c.Ngal2) + al<double2>(c.Ngal1);
-  bytes += al<double>(h.nyu) + al<double>(SC_NSCAL);
-  if (x->mixed)
-    bytes += al<float2>(3 * ninter) + al<float2>(4 * ninter);
-  else
-    bytes += al<double2>(3 * ninter) + al<double2>(4 * ninter);
+  Chan &d = h.dev;
+  memset(&d, 0, sizeof d);
+  // the same sequence sizes the block (base == nullptr) and carves it up
+  auto layout = [&](char *base) -> size_t {
+    size_t off = 0;
+    auto take = [&](size_t b) -> void * {
+      void *q = base ? (void *)(base + off) : nullptr;
+      off += b;
+      return q;
+    };
+    double2 **fl[18] = {&d.u1, &d.u2, &d.u3, &d.p, &d.N1o, &d.N2o, &d.N3o, &d.r1, &d.r2,
+                        &d.r3, &d.nl1, &d.nl2, &d.nl3, &d.fuv, &d.fwv, &d.fvv, &d.psi, &d.div};
+    for (int q = 0; q < 18; q++) *fl[q] = (double2 *)take(al<double2>(fld));
+    d.scr = (double *)take(al<double>(2 * fld));
+    d.yu = (double *)take(al<double>(h.nyu));
+    d.dthdyu = (double *)take(al<double>(h.nyu));
+    d.yv = (double *)take(al<double>(h.nyv));
+    d.dthdyv = (double *)take(al<double>(h.nyv));
+    d.dyu2i = (double *)take(al<double>(3 * h.nyu));
+    d.dyv2i = (double *)take(al<double>(3 * h.nyv));
+    d.u11 = (double *)take(al<double>(4 * h.nyu));
+    d.kx = (double *)take(al<double>(h.nxh));
+    d.k2x = (double *)take(al<double>(h.nxh));
+    d.kz = (double *)take(al<double>(c.N2));
+    d.k2z = (double *)take(al<double>(c.N2));
+    d.twz = (double2 *)take(al<double2>(c.Ngal2));
+    d.twx = (double2 *)take(al<double2>(c.Ngal1));
+    d.scal = (double *)take(al<double>(SC_NSCAL));
+    if (x->mixed) {
+      d.hzf = (float2 *)take(al<float2>(3 * ninter));
+      d.gxf = (float2 *)take(al<float2>(4 * ninter));
+    } else {
+      d.hz = (double2 *)take(al<double2>(3 * ninter));
+      d.gx = (double2 *)take(al<double2>(4 * ninter));
+    }
+    return off;
+  };
+  size_t bytes = layout(nullptr);
   hipError_t e = hipMalloc(&h.dmem, bytes);
   if (e != hipSuccess) return set_err(LH_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
   h.dbytes = bytes;
   HIPCHK(hipMemset(h.dmem, 0, bytes));
-  char *p = (char *)h.dmem;
-  auto take = [&](size_t b) {
-    char *q = p;
-    p += b;
-    return (void *)q;
-  };
-  Chan &d = h.dev;
-  memset(&d, 0, sizeof d);
-  double2 **fl[18] = {&d.u1, &d.u2, &d.u3, &d.p, &d.N1o, &d.N2o, &d.N3o, &d.r1, &d.r2,
-                      &d.r3, &d.nl1, &d.nl2, &d.nl3, &d.fuv, &d.fwv, &d.fvv, &d.psi, &d.div};
-  for (int q = 0; q < 18; q++) *fl[q] = (double2 *)take(al<double2>(fld));
-  d.scr = (double *)take(al<double>(fld));
-  d.yu = (double *)take(al<double>(h.nyu));
-  d.dthdyu = (double *)take(al<double>(h.nyu));
-  d.yv = (double *)take(al<double>(h.nyv));
-  d.dthdyv = (double *)take(al<double>(h.nyv));
-  d.dyu2i = (double *)take(al<double>(3 * h.nyu));
-  d.dyv2i = (double *)take(al<double>(3 * h.nyv));
-  d.u11 = (double *)take(al<double>(h.nyu));
-  (void)take(al<double>(h.nyu));
-  (void)take(al<double>(h.nyu));
-  d.kx = (double *)take(al<double>(h.nxh));
-  d.k2x = (double *)take(al<double>(h.nxh));
-  d.kz = (double *)take(al<double>(c.N2));
-  d.k2z = (double *)take(al<double>(c.N2));
-  d.twz = (double2 *)take(al<double2>(c.Ngal2));
-  d.twx = (double2 *)take(al<double2>(c.Ngal1));
-  (void)take(al<double>(h.nyu));
-  d.scal = (double *)take(al<double>(SC_NSCAL));
-  if (x->mixed) {
-    d.hzf = (float2 *)take(al<float2>(3 * ninter));
-    d.gxf = (float2 *)take(al<float2>(4 * ninter));
-  } else {
-    d.hz = (double2 *)take(al<double2>(3 * ninter));
-    d.gx = (double2 *)take(al<double2>(4 * ninter));
-  }
-  if ((size_t)(p - (char *)h.dmem) > bytes) return set_err(LH_EINVAL, "internal: allocation overrun");
+  layout((char *)h.dmem);
   d.N1 = c.N1;
   d.N2 = c.N2;
   d.Ng1 = c.Ngal1;
