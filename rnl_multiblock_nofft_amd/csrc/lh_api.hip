@@ -45,10 +45,7 @@ struct HostChan {
   std::vector<double> yu, yv, dthdyu, dthdyv, dyu2i, dyv2i, kx, kz, k2x, k2z;
   std::vector<double2> twz, twx;
   double gw1, gw2, gwbc1, gwbc3, gwi1, gwi2, dtv;
-  // device block
-  void *dmem = nullptr;
-  size_t dbytes = 0;
-  Chan dev;  // pointers into dmem
+  Chan dev;  // offsets into the context arena
   double t = 0.0;
   int iter = 0;
 };
@@ -210,6 +207,8 @@ template <typename T> size_t al(size_t n) { return (n * sizeof(T) + 255) / 256 *
 struct lh_ctx {
   std::vector<HostChan> ch;
   bool mixed = false;
+  char *arena = nullptr;  // every channel's device arrays
+  size_t arena_bytes = 0;
   hipStream_t stream = nullptr;
   Chan *d_chans = nullptr;
   ShapeMax shape;
@@ -222,7 +221,8 @@ struct lh_ctx {
 namespace {
 
 int upload_chan_constants(lh_ctx *x, HostChan &h) {
-  Chan &d = h.dev;
+  Chan &dc = h.dev;
+  View d(x->arena, dc);
   HIPCHK(hipMemcpy((void *)d.yu, h.yu.data(), h.nyu * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy((void *)d.yv, h.yv.data(), h.nyv * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy((void *)d.dthdyu, h.dthdyu.data(), h.nyu * sizeof(double), hipMemcpyHostToDevice));
@@ -235,63 +235,93 @@ int upload_chan_constants(lh_ctx *x, HostChan &h) {
   HIPCHK(hipMemcpy((void *)d.k2z, h.k2z.data(), h.cfg.N2 * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy((void *)d.twz, h.twz.data(), h.cfg.Ngal2 * sizeof(double2), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy((void *)d.twx, h.twx.data(), h.cfg.Ngal1 * sizeof(double2), hipMemcpyHostToDevice));
-  d.gw1 = h.gw1;
-  d.gw2 = h.gw2;
-  d.gwbc1 = h.gwbc1;
-  d.gwbc3 = h.gwbc3;
-  d.gwi1 = h.gwi1;
-  d.gwi2 = h.gwi2;
-  (void)x;
+  // unit-forcing solutions of the mean mode for each substep (flowrate_corr's
+  // u11 = LUsol0(-dRK dt), littleharsh.f90:484-494) and their flow rates
+  {
+    const int n = h.nyu;
+    std::vector<double> u11(3 * n), a(3 * n);
+    double qx1[3];
+    for (int kRK = 1; kRK <= 3; kRK++) {
+      const double beta = rk_b(kRK) * h.cfg.dt / h.cfg.Re, bdt = -rk_d(kRK) * h.cfg.dt;
+      double *x = u11.data() + (kRK - 1) * n;
+      a[0] = 0.0;
+      a[1] = 1.0;
+      a[2] = h.gw1;
+      for (int j = 1; j <= n - 2; j++) {
+        a[3 * j] = -beta * h.dyu2i[3 * j];
+        a[3 * j + 1] = 1.0 - beta * h.dyu2i[3 * j + 1];
+        a[3 * j + 2] = -beta * h.dyu2i[3 * j + 2];
+      }
+      a[3 * (n - 1)] = h.gw2;
+      a[3 * (n - 1) + 1] = 1.0;
+      a[3 * (n - 1) + 2] = 0.0;
+      a[1] = 1.0 / a[1];
+      for (int j = 1; j < n; j++) {
+        a[3 * (j - 1) + 2] = a[3 * (j - 1) + 2] * a[3 * (j - 1) + 1];
+        a[3 * j + 1] = 1.0 / (a[3 * j + 1] - a[3 * j] * a[3 * (j - 1) + 2]);
+      }
+      for (int j = 0; j < n; j++) x[j] = (j == 0 || j == n - 1) ? 0.0 : bdt;
+      x[0] = x[0] * a[1];
+      for (int j = 1; j < n; j++) x[j] = (x[j] - a[3 * j] * x[j - 1]) * a[3 * j + 1];
+      for (int j = n - 2; j >= 0; j--) x[j] = x[j] - a[3 * j + 2] * x[j + 1];
+      double Q = 0.0;
+      for (int j = 1; j <= n - 2; j++) Q = Q + (h.yv[j] - h.yv[j - 1]) * x[j];
+      qx1[kRK - 1] = Q;
+    }
+    HIPCHK(hipMemcpy((void *)d.u11, u11.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void *)(d.scal + SC_QX1), qx1, sizeof qx1, hipMemcpyHostToDevice));
+  }
+  dc.gw1 = h.gw1;
+  dc.gw2 = h.gw2;
+  dc.gwbc1 = h.gwbc1;
+  dc.gwbc3 = h.gwbc3;
+  dc.gwi1 = h.gwi1;
+  dc.gwi2 = h.gwi2;
   return LH_OK;
 }
 
-int alloc_chan(lh_ctx *x, HostChan &h) {
+// assigns h.dev's offsets from `base`, returns the end offset
+size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
   const lh_config &c = h.cfg;
   const size_t fld = (size_t)h.nyu * h.ldc;  // complex elements per field
   const size_t ninter = (size_t)h.nslot * c.Ngal2 * h.nxh;
   Chan &d = h.dev;
   memset(&d, 0, sizeof d);
-  // the same sequence sizes the block (base == nullptr) and carves it up
-  auto layout = [&](char *base) -> size_t {
-    size_t off = 0;
-    auto take = [&](size_t b) -> void * {
-      void *q = base ? (void *)(base + off) : nullptr;
-      off += b;
-      return q;
-    };
-    double2 **fl[18] = {&d.u1, &d.u2, &d.u3, &d.p, &d.N1o, &d.N2o, &d.N3o, &d.r1, &d.r2,
-                        &d.r3, &d.nl1, &d.nl2, &d.nl3, &d.fuv, &d.fwv, &d.fvv, &d.psi, &d.div};
-    for (int q = 0; q < 18; q++) *fl[q] = (double2 *)take(al<double2>(fld));
-    d.scr = (double *)take(al<double>(2 * fld));
-    d.yu = (double *)take(al<double>(h.nyu));
-    d.dthdyu = (double *)take(al<double>(h.nyu));
-    d.yv = (double *)take(al<double>(h.nyv));
-    d.dthdyv = (double *)take(al<double>(h.nyv));
-    d.dyu2i = (double *)take(al<double>(3 * h.nyu));
-    d.dyv2i = (double *)take(al<double>(3 * h.nyv));
-    d.u11 = (double *)take(al<double>(4 * h.nyu));
-    d.kx = (double *)take(al<double>(h.nxh));
-    d.k2x = (double *)take(al<double>(h.nxh));
-    d.kz = (double *)take(al<double>(c.N2));
-    d.k2z = (double *)take(al<double>(c.N2));
-    d.twz = (double2 *)take(al<double2>(c.Ngal2));
-    d.twx = (double2 *)take(al<double2>(c.Ngal1));
-    d.scal = (double *)take(al<double>(SC_NSCAL));
-    if (x->mixed) {
-      d.hzf = (float2 *)take(al<float2>(3 * ninter));
-      d.gxf = (float2 *)take(al<float2>(4 * ninter));
-    } else {
-      d.hz = (double2 *)take(al<double2>(3 * ninter));
-      d.gx = (double2 *)take(al<double2>(4 * ninter));
-    }
-    return off;
+  size_t off = base;
+  auto take = [&](size_t b) -> size_t {
+    size_t q = off;
+    off += b;
+    return q;
   };
-  size_t bytes = layout(nullptr);
-  hipError_t e = hipMalloc(&h.dmem, bytes);
-  if (e != hipSuccess) return set_err(LH_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-  h.dbytes = bytes;
-  HIPCHK(hipMemset(h.dmem, 0, bytes));
-  layout((char *)h.dmem);
+  size_t *fl[18] = {&d.o_u1, &d.o_u2, &d.o_u3, &d.o_p, &d.o_N1o, &d.o_N2o, &d.o_N3o, &d.o_r1, &d.o_r2,
+                    &d.o_r3, &d.o_nl1, &d.o_nl2, &d.o_nl3, &d.o_fuv, &d.o_fwv, &d.o_fvv, &d.o_psi, &d.o_div};
+  for (int q = 0; q < 18; q++) *fl[q] = take(al<double2>(fld));
+  d.o_scr = take(al<double>(2 * fld));
+  d.o_yu = take(al<double>(h.nyu));
+  d.o_dthdyu = take(al<double>(h.nyu));
+  d.o_yv = take(al<double>(h.nyv));
+  d.o_dthdyv = take(al<double>(h.nyv));
+  d.o_dyu2i = take(al<double>(3 * h.nyu));
+  d.o_dyv2i = take(al<double>(3 * h.nyv));
+  d.o_u11 = take(al<double>(3 * h.nyu));
+  d.o_kx = take(al<double>(h.nxh));
+  d.o_k2x = take(al<double>(h.nxh));
+  d.o_kz = take(al<double>(c.N2));
+  d.o_k2z = take(al<double>(c.N2));
+  d.o_twz = take(al<double2>(c.Ngal2));
+  d.o_twx = take(al<double2>(c.Ngal1));
+  d.o_scal = take(al<double>(SC_NSCAL));
+  if (x->mixed) {
+    d.o_hzf = take(al<float2>(3 * ninter));
+    d.o_gxf = take(al<float2>(4 * ninter));
+    d.o_hz = d.o_hzf;
+    d.o_gx = d.o_gxf;
+  } else {
+    d.o_hz = take(al<double2>(3 * ninter));
+    d.o_gx = take(al<double2>(4 * ninter));
+    d.o_hzf = d.o_hz;
+    d.o_gxf = d.o_gx;
+  }
   d.N1 = c.N1;
   d.N2 = c.N2;
   d.Ng1 = c.Ngal1;
@@ -309,13 +339,16 @@ int alloc_chan(lh_ctx *x, HostChan &h) {
   d.Re = c.Re;
   d.alp = c.alp;
   d.bet = c.bet;
-  int rc = upload_chan_constants(x, h);
-  if (rc) return rc;
+  return off;
+}
+
+int init_chan(lh_ctx *x, HostChan &h) {
+  View d(x->arena, h.dev);
   double sc[SC_NSCAL] = {0};
-  sc[SC_MPGX] = c.mpgx;
-  sc[SC_DT] = c.dt;
+  sc[SC_MPGX] = h.cfg.mpgx;
+  sc[SC_DT] = h.cfg.dt;
   HIPCHK(hipMemcpy(d.scal, sc, sizeof sc, hipMemcpyHostToDevice));
-  return LH_OK;
+  return upload_chan_constants(x, h);
 }
 
 int push_descriptors(lh_ctx *x) {
@@ -326,8 +359,8 @@ int push_descriptors(lh_ctx *x) {
 }
 
 // rows of a field in the reference layout, device array, jlo
-int field_map(HostChan &h, int field, double2 **arr, int *jlo, int *nrows) {
-  Chan &d = h.dev;
+int field_map(lh_ctx *x, HostChan &h, int field, double2 **arr, int *jlo, int *nrows) {
+  View d(x->arena, h.dev);
   switch (field) {
     case LH_U1: *arr = d.u1; *jlo = 0; *nrows = h.nyu; return 0;
     case LH_U2: *arr = d.u2; *jlo = 0; *nrows = h.nyv; return 0;
@@ -352,9 +385,9 @@ int check_ctx(lh_ctx *x) {
 }
 
 int run_substep(lh_ctx *x, int kRK) {
-  HIPCHK(launch_nonlinear(x->d_chans, x->shape, x->mixed, x->stream));
-  HIPCHK(launch_rhs(x->d_chans, x->shape, kRK, x->stream));
-  HIPCHK(launch_solve(x->d_chans, x->shape, kRK, x->stream));
+  HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream));
+  HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
+  HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
   return LH_OK;
 }
 
@@ -392,7 +425,18 @@ int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
     e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
     if (e != hipSuccess) rc = set_err(LH_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
-  for (int c = 0; c < nchan && rc == LH_OK; c++) rc = alloc_chan(x, x->ch[c]);
+  if (rc == LH_OK) {
+    size_t off = 0;
+    for (int c = 0; c < nchan; c++) off = layout_chan(x, x->ch[c], off);
+    e = hipMalloc((void **)&x->arena, off);
+    if (e != hipSuccess) rc = set_err(LH_ENOMEM, "hipMalloc(%zu): %s", off, hipGetErrorString(e));
+    x->arena_bytes = off;
+  }
+  if (rc == LH_OK) {
+    e = hipMemset(x->arena, 0, x->arena_bytes);
+    if (e != hipSuccess) rc = set_err(LH_EHIP, "hipMemset: %s", hipGetErrorString(e));
+  }
+  for (int c = 0; c < nchan && rc == LH_OK; c++) rc = init_chan(x, x->ch[c]);
   if (rc == LH_OK) {
     e = hipMalloc(&x->d_chans, nchan * sizeof(Chan));
     if (e != hipSuccess) rc = set_err(LH_ENOMEM, "hipMalloc(descriptors): %s", hipGetErrorString(e));
@@ -418,8 +462,7 @@ int lh_ctx_destroy(lh_ctx *x) {
   (void)hipSetDevice(x->device);
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   if (x->graph) (void)hipGraphExecDestroy(x->graph);
-  for (auto &h : x->ch)
-    if (h.dmem) (void)hipFree(h.dmem);
+  if (x->arena) (void)hipFree(x->arena);
   if (x->d_chans) (void)hipFree(x->d_chans);
   if (x->staging) (void)hipFree(x->staging);
   if (x->stream) (void)hipStreamDestroy(x->stream);
@@ -462,7 +505,7 @@ int lh_field_upload(lh_ctx *x, int ch, int field, const double *reim) {
   HostChan &h = x->ch[ch];
   double2 *arr;
   int jlo, nrows;
-  if (field_map(h, field, &arr, &jlo, &nrows)) return set_err(LH_EINVAL, "lh_field_upload: bad field %d", field);
+  if (field_map(x, h, field, &arr, &jlo, &nrows)) return set_err(LH_EINVAL, "lh_field_upload: bad field %d", field);
   size_t n = (size_t)h.ncol * nrows;
   HIPCHK(hipMemcpyAsync(x->staging, reim, n * sizeof(double2), hipMemcpyHostToDevice, x->stream));
   HIPCHK(launch_to_rows(x->staging, arr, h.ncol, nrows, jlo, h.ldc, x->stream));
@@ -477,7 +520,7 @@ int lh_field_download(lh_ctx *x, int ch, int field, double *reim) {
   HostChan &h = x->ch[ch];
   double2 *arr;
   int jlo, nrows;
-  if (field_map(h, field, &arr, &jlo, &nrows)) return set_err(LH_EINVAL, "lh_field_download: bad field %d", field);
+  if (field_map(x, h, field, &arr, &jlo, &nrows)) return set_err(LH_EINVAL, "lh_field_download: bad field %d", field);
   size_t n = (size_t)h.ncol * nrows;
   HIPCHK(launch_from_rows(arr, x->staging, h.ncol, nrows, jlo, h.ldc, x->stream));
   HIPCHK(hipMemcpyAsync(reim, x->staging, n * sizeof(double2), hipMemcpyDeviceToHost, x->stream));
@@ -492,7 +535,7 @@ int lh_get_scalars(lh_ctx *x, int ch, double *o) {
   HostChan &h = x->ch[ch];
   double sc[SC_NSCAL];
   HIPCHK(hipStreamSynchronize(x->stream));
-  HIPCHK(hipMemcpy(sc, h.dev.scal, sizeof sc, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(sc, View(x->arena, h.dev).scal, sizeof sc, hipMemcpyDeviceToHost));
   o[0] = h.t;
   o[1] = sc[SC_DT];
   o[2] = sc[SC_MPGX];
@@ -511,10 +554,10 @@ int lh_set_scalars(lh_ctx *x, int ch, double t, double mpgx, double QxT, int ite
   HostChan &h = x->ch[ch];
   double sc[SC_NSCAL];
   HIPCHK(hipStreamSynchronize(x->stream));
-  HIPCHK(hipMemcpy(sc, h.dev.scal, sizeof sc, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(sc, View(x->arena, h.dev).scal, sizeof sc, hipMemcpyDeviceToHost));
   sc[SC_MPGX] = mpgx;
   sc[SC_QXT] = QxT;
-  HIPCHK(hipMemcpy(h.dev.scal, sc, sizeof sc, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(View(x->arena, h.dev).scal, sc, sizeof sc, hipMemcpyHostToDevice));
   h.t = t;
   h.iter = iter;
   return LH_OK;
@@ -523,7 +566,7 @@ int lh_set_scalars(lh_ctx *x, int ch, double t, double mpgx, double QxT, int ite
 int lh_init_flowrate(lh_ctx *x) {
   int rc = check_ctx(x);
   if (rc) return rc;
-  HIPCHK(launch_flowrate(x->d_chans, (int)x->ch.size(), 1, x->stream));
+  HIPCHK(launch_flowrate(x->arena, x->d_chans, (int)x->ch.size(), 1, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   return LH_OK;
 }
@@ -533,9 +576,9 @@ int lh_run_phase(lh_ctx *x, int phase, int kRK) {
   if (rc) return rc;
   if (kRK < 1 || kRK > 3) return set_err(LH_EINVAL, "kRK must be 1..3");
   switch (phase) {
-    case LH_PHASE_NONLINEAR: HIPCHK(launch_nonlinear(x->d_chans, x->shape, x->mixed, x->stream)); break;
-    case LH_PHASE_RHS: HIPCHK(launch_rhs(x->d_chans, x->shape, kRK, x->stream)); break;
-    case LH_PHASE_SOLVE: HIPCHK(launch_solve(x->d_chans, x->shape, kRK, x->stream)); break;
+    case LH_PHASE_NONLINEAR: HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream)); break;
+    case LH_PHASE_RHS: HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream)); break;
+    case LH_PHASE_SOLVE: HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream)); break;
     default: return set_err(LH_EINVAL, "unknown phase %d", phase);
   }
   if (kRK == 1 && phase == LH_PHASE_NONLINEAR)
@@ -597,12 +640,12 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
   for (int n = 0; n < nsub && rc == LH_OK; n++) {
     HIPCHK(hipEventRecord(ev[0], x->stream));
     for (int st = 0; st < 3; st++) {
-      HIPCHK(launch_nl_stage(x->d_chans, x->shape, x->mixed, st, x->stream));
+      HIPCHK(launch_nl_stage(x->arena, x->d_chans, x->shape, x->mixed, st, x->stream));
       HIPCHK(hipEventRecord(ev[st + 1], x->stream));
     }
-    HIPCHK(launch_rhs(x->d_chans, x->shape, kRK, x->stream));
+    HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
     HIPCHK(hipEventRecord(ev[4], x->stream));
-    HIPCHK(launch_solve(x->d_chans, x->shape, kRK, x->stream));
+    HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
     HIPCHK(hipEventRecord(ev[5], x->stream));
     HIPCHK(hipEventSynchronize(ev[5]));
     for (int q = 0; q < 5; q++) {
@@ -690,8 +733,8 @@ int lh_lusol_uw(lh_ctx *x, int ch, int kRK, const double *ru, const double *rw, 
     return set_err(LH_EINVAL, "lh_lusol_uw: bad arguments");
   HostChan &h = x->ch[ch];
   const Chan *dc = x->d_chans + ch;
-  HIPCHK(launch_lusol_ref2(dc, kRK, 0, (const double2 *)ru, (const double2 *)rw, (double2 *)u, (double2 *)w,
-                           h.dev.scr, h.ncol, x->stream));
+  HIPCHK(launch_lusol_ref2(x->arena, dc, kRK, 0, (const double2 *)ru, (const double2 *)rw, (double2 *)u, (double2 *)w,
+                           View(x->arena, h.dev).scr, h.ncol, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   return LH_OK;
 }
@@ -703,7 +746,7 @@ int lh_lusol_v(lh_ctx *x, int ch, int kRK, const double *rv, double *v) {
     return set_err(LH_EINVAL, "lh_lusol_v: bad arguments");
   HostChan &h = x->ch[ch];
   const Chan *dc = x->d_chans + ch;
-  HIPCHK(launch_lusol_ref2(dc, kRK, 1, (const double2 *)rv, nullptr, (double2 *)v, nullptr, h.dev.scr, h.ncol,
+  HIPCHK(launch_lusol_ref2(x->arena, dc, kRK, 1, (const double2 *)rv, nullptr, (double2 *)v, nullptr, View(x->arena, h.dev).scr, h.ncol,
                            x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   return LH_OK;

@@ -25,8 +25,12 @@ __host__ __device__ inline double rk_c(int k) { return k == 1 ? 0.0 : (k == 2 ? 
 __host__ __device__ inline double rk_d(int k) { return k == 1 ? 8.0 / 15.0 : (k == 2 ? 2.0 / 15.0 : 1.0 / 3.0); }
 
 // per-channel device scalars (in HBM so that graph replays see updates)
-enum { SC_MPGX = 0, SC_QXT, SC_QX, SC_DT, SC_T, SC_NSCAL = 8 };
+enum { SC_MPGX = 0, SC_QXT, SC_QX, SC_DT, SC_T, SC_QX1, SC_NSCAL = 8 };  // SC_QX1..+2: per kRK
 
+// Per-channel descriptor.  Every device array lives in one context-wide
+// arena; the descriptor holds byte offsets, and kernels derive typed pointers
+// from the arena base they receive as an argument (so that the compiler sees
+// global-memory pointers and emits global_load/store, not flat accesses).
 struct Chan {
   int N1, N2, Ng1, Ng2;
   int ny, nyu, nyv, nxh, ncol, ldc;
@@ -34,22 +38,43 @@ struct Chan {
   int flag_ctpress;
   double Re, alp, bet;
   double gw1, gw2, gwbc1, gwbc3, gwi1, gwi2;
-  // grid / wavenumbers
-  const double *yu, *yv, *dthdyu, *dthdyv, *dyu2i, *dyv2i;  // dyu2i[3*j+m]
-  const double *kx, *kz, *k2x, *k2z;                        // [nxh], [N2]
-  const double2 *twz, *twx;  // exp(-2 pi i m / Ng2), exp(-2 pi i m / Ng1)
-  // state
-  double2 *u1, *u2, *u3, *p;
-  double2 *N1o, *N2o, *N3o;  // nonlinear terms of the previous substep
-  double2 *r1, *r2, *r3;     // right-hand sides of the Helmholtz solves
-  double2 *nl1, *nl2, *nl3;  // x/z part of the new nonlinear terms
-  double2 *fuv, *fwv, *fvv;  // y-fluxes uv, wv (v rows) and vv (u rows)
-  double2 *psi, *div;
-  double *scr;               // Thomas scratch [nyu][ldc]
-  double *u11;               // unit-forcing mean-mode solution [nyu]
-  double2 *hz, *gx;          // nonlinear pipeline intermediates (FP64 mode)
-  float2 *hzf, *gxf;         // same, FP32 (mixed mode)
-  double *scal;              // SC_* scalars
+  // grid / wavenumbers: yu,yv,dthdyu,dthdyv [rows], dyu2i/dyv2i [3*j+m],
+  // kx,k2x [nxh], kz,k2z [N2], twz/twx exp(-2 pi i m/Ngal2), exp(-2 pi i m/Ngal1)
+  size_t o_yu, o_yv, o_dthdyu, o_dthdyv, o_dyu2i, o_dyv2i, o_kx, o_kz, o_k2x, o_k2z, o_twz, o_twx;
+  // state: fields [row][ldc] double2; N*o nonlinear terms of the previous
+  // substep; r* Helmholtz right-hand sides; nl* x/z part of the new nonlinear
+  // terms; fuv,fwv (v rows) and fvv (u rows) the y-fluxes
+  size_t o_u1, o_u2, o_u3, o_p, o_N1o, o_N2o, o_N3o, o_r1, o_r2, o_r3, o_nl1, o_nl2, o_nl3;
+  size_t o_fuv, o_fwv, o_fvv, o_psi, o_div;
+  // scr: Thomas scratch [2][nyu][ldc]; u11: mean-mode unit-forcing solutions
+  // [3][nyu]; hz/gx (or hzf/gxf in FP32): nonlinear pipeline intermediates;
+  // scal: SC_* scalars
+  size_t o_scr, o_u11, o_hz, o_gx, o_hzf, o_gxf, o_scal;
+};
+
+// typed device pointers of one channel, derived from the arena base
+struct View {
+  const double *yu, *yv, *dthdyu, *dthdyv, *dyu2i, *dyv2i, *kx, *kz, *k2x, *k2z;
+  const double2 *twz, *twx;
+  double2 *u1, *u2, *u3, *p, *N1o, *N2o, *N3o, *r1, *r2, *r3, *nl1, *nl2, *nl3, *fuv, *fwv, *fvv, *psi, *div;
+  double *scr, *u11, *scal;
+  double2 *hz, *gx;
+  float2 *hzf, *gxf;
+  __host__ __device__ View(char *A, const Chan &c)
+      : yu((const double *)(A + c.o_yu)), yv((const double *)(A + c.o_yv)),
+        dthdyu((const double *)(A + c.o_dthdyu)), dthdyv((const double *)(A + c.o_dthdyv)),
+        dyu2i((const double *)(A + c.o_dyu2i)), dyv2i((const double *)(A + c.o_dyv2i)),
+        kx((const double *)(A + c.o_kx)), kz((const double *)(A + c.o_kz)), k2x((const double *)(A + c.o_k2x)),
+        k2z((const double *)(A + c.o_k2z)), twz((const double2 *)(A + c.o_twz)),
+        twx((const double2 *)(A + c.o_twx)), u1((double2 *)(A + c.o_u1)), u2((double2 *)(A + c.o_u2)),
+        u3((double2 *)(A + c.o_u3)), p((double2 *)(A + c.o_p)), N1o((double2 *)(A + c.o_N1o)),
+        N2o((double2 *)(A + c.o_N2o)), N3o((double2 *)(A + c.o_N3o)), r1((double2 *)(A + c.o_r1)),
+        r2((double2 *)(A + c.o_r2)), r3((double2 *)(A + c.o_r3)), nl1((double2 *)(A + c.o_nl1)),
+        nl2((double2 *)(A + c.o_nl2)), nl3((double2 *)(A + c.o_nl3)), fuv((double2 *)(A + c.o_fuv)),
+        fwv((double2 *)(A + c.o_fwv)), fvv((double2 *)(A + c.o_fvv)), psi((double2 *)(A + c.o_psi)),
+        div((double2 *)(A + c.o_div)), scr((double *)(A + c.o_scr)), u11((double *)(A + c.o_u11)),
+        scal((double *)(A + c.o_scal)), hz((double2 *)(A + c.o_hz)), gx((double2 *)(A + c.o_gx)),
+        hzf((float2 *)(A + c.o_hzf)), gxf((float2 *)(A + c.o_gxf)) {}
 };
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }

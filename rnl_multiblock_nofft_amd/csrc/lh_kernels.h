@@ -13,11 +13,11 @@ struct ShapeMax {
 };
 
 void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny);
-hipError_t launch_nl_stage(const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s);
-hipError_t launch_nonlinear(const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s);
-hipError_t launch_rhs(const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
-hipError_t launch_solve(const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
-hipError_t launch_flowrate(const Chan *d_chans, int nchan, int set_target, hipStream_t s);
+hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s);
+hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s);
+hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
+hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
+hipError_t launch_flowrate(char *arena, const Chan *d_chans, int nchan, int set_target, hipStream_t s);
 hipError_t launch_to_rows(const double2 *src, double2 *dst, int ncol, int nrows, int jlo, int ldc,
                           hipStream_t s);
 hipError_t launch_from_rows(const double2 *src, double2 *dst, int ncol, int nrows, int jlo, int ldc,
@@ -28,7 +28,7 @@ hipError_t launch_plane_f2p(int N1, int Ng1, int Ng2, double *planes, int nplane
 hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
                             const double2 *twx, hipStream_t s);
 // stand-alone Helmholtz solves on reference-layout device data
-hipError_t launch_lusol_ref2(const Chan *d_chan, int kRK, int vgrid, const double2 *rhs_a,
+hipError_t launch_lusol_ref2(char *arena, const Chan *d_chan, int kRK, int vgrid, const double2 *rhs_a,
                              const double2 *rhs_b, double2 *x_a, double2 *x_b, double *scr, int ncol,
                              hipStream_t s);
 

@@ -146,17 +146,18 @@ hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplane
 }
 
 // LUsolU_W (vgrid = 0, two right-hand sides) / LUsolV (vgrid = 1, one)
-__global__ void __launch_bounds__(64) k_lusol_ref(const Chan *chp, int kRK, int vgrid, const double2 *ra,
+__global__ void __launch_bounds__(64) k_lusol_ref(char *arena, const Chan *chp, int kRK, int vgrid, const double2 *ra,
                                                   const double2 *rb, double2 *xa, double2 *xb, double *scr,
                                                   int ncol) {
-  const Chan &ch = *chp;
+  const Chan ch = *chp;
+  const View v(arena, ch);
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncol) return;
   const int i = c % ch.nxh, k = c / ch.nxh;
   const int n = vgrid ? ch.nyv : ch.nyu;
-  const double *dy2 = vgrid ? ch.dyv2i : ch.dyu2i;
-  const double beta = rk_b(kRK) * ch.scal[SC_DT] / ch.Re;
-  const double k2x = ch.k2x[i], k2z = ch.k2z[k];
+  const double *dy2 = vgrid ? v.dyv2i : v.dyu2i;
+  const double beta = rk_b(kRK) * v.scal[SC_DT] / ch.Re;
+  const double k2x = v.k2x[i], k2z = v.k2z[k];
   const size_t base = (size_t)c * n;
   double *sc = scr + base;
   double cprev = vgrid ? 0.0 : ch.gw1;
@@ -200,10 +201,10 @@ __global__ void __launch_bounds__(64) k_lusol_ref(const Chan *chp, int kRK, int 
   }
 }
 
-hipError_t launch_lusol_ref2(const Chan *d_chan, int kRK, int vgrid, const double2 *rhs_a,
+hipError_t launch_lusol_ref2(char *arena, const Chan *d_chan, int kRK, int vgrid, const double2 *rhs_a,
                              const double2 *rhs_b, double2 *x_a, double2 *x_b, double *scr, int ncol,
                              hipStream_t s) {
-  hipLaunchKernelGGL(k_lusol_ref, dim3((ncol + 63) / 64), dim3(64), 0, s, d_chan, kRK, vgrid, rhs_a, rhs_b,
+  hipLaunchKernelGGL(k_lusol_ref, dim3((ncol + 63) / 64), dim3(64), 0, s, arena, d_chan, kRK, vgrid, rhs_a, rhs_b,
                      x_a, x_b, scr, ncol);
   return hipGetLastError();
 }
