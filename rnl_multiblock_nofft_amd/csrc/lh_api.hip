@@ -283,7 +283,7 @@ int upload_chan_constants(lh_ctx *x, HostChan &h) {
 // assigns h.dev's offsets from `base`, returns the end offset
 size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
   const lh_config &c = h.cfg;
-  const size_t fld = (size_t)h.nyu * h.ldc;  // complex elements per field
+  const size_t fld = (size_t)(h.nyu + PADROWS) * h.ldc;  // complex elements per field (with padding rows)
   const size_t ninter = (size_t)h.nslot * c.Ngal2 * h.nxh;
   Chan &d = h.dev;
   memset(&d, 0, sizeof d);
@@ -311,6 +311,7 @@ size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
   d.o_twz = take(al<double2>(c.Ngal2));
   d.o_twx = take(al<double2>(c.Ngal1));
   d.o_scal = take(al<double>(SC_NSCAL));
+  d.o_fac = take(al<double>(14 * fld));
   if (x->mixed) {
     d.o_hzf = take(al<float2>(3 * ninter));
     d.o_gxf = take(al<float2>(4 * ninter));
@@ -447,6 +448,11 @@ int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
     x->staging_elems = stage;
   }
   if (rc == LH_OK) rc = push_descriptors(x);
+  if (rc == LH_OK) {
+    e = launch_factors(x->arena, x->d_chans, x->shape, x->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+    if (e != hipSuccess) rc = set_err(LH_EHIP, "factor set-up: %s", hipGetErrorString(e));
+  }
   if (rc != LH_OK) {
     std::string keep = g_err;
     lh_ctx_destroy(x);
@@ -485,7 +491,11 @@ int lh_set_grid(lh_ctx *x, int ch, const double *yu, const double *yv, const dou
   HIPCHK(hipStreamSynchronize(x->stream));
   rc = upload_chan_constants(x, h);
   if (rc) return rc;
-  return push_descriptors(x);
+  rc = push_descriptors(x);
+  if (rc) return rc;
+  HIPCHK(launch_factors(x->arena, x->d_chans, x->shape, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
 }
 
 int lh_get_grid(lh_ctx *x, int ch, double *yu, double *yv, double *dthdyu, double *dthdyv) {

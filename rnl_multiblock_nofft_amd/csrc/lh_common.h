@@ -17,6 +17,10 @@
 
 namespace lh {
 
+// spare rows after every field (and factor table) for branch-free blocked
+// sweeps of the column solve (k_solve reads up to 3*SB rows past an end)
+constexpr int PADROWS = 32;
+
 // RK3 coefficients (Le & Moin 1991), start.f90:419-433
 __host__ __device__ inline double rk_a(int k) { return k == 1 ? 4.0 / 15.0 : (k == 2 ? 1.0 / 15.0 : 1.0 / 6.0); }
 __host__ __device__ inline double rk_b(int k) { return rk_a(k); }
@@ -50,6 +54,11 @@ struct Chan {
   // [3][nyu]; hz/gx (or hzf/gxf in FP32): nonlinear pipeline intermediates;
   // scal: SC_* scalars
   size_t o_scr, o_u11, o_hz, o_gx, o_hzf, o_gxf, o_scal;
+  // Thomas factors of the fixed-dt operators, [row][ldc] doubles each:
+  // o_fac + ((kRK-1)*4 + m) * nyu*ldc*8 for m = 0 dU, 1 cU, 2 dV, 3 cV
+  // (U/W and V Helmholtz, LU_dec order), then o_fac + 12*nyu*ldc*8 dP and
+  // +13 eP: the Poisson operator eliminated from the top row down
+  size_t o_fac;
 };
 
 // typed device pointers of one channel, derived from the arena base
@@ -60,6 +69,7 @@ struct View {
   double *scr, *u11, *scal;
   double2 *hz, *gx;
   float2 *hzf, *gxf;
+  double *fac;
   __host__ __device__ View(char *A, const Chan &c)
       : yu((const double *)(A + c.o_yu)), yv((const double *)(A + c.o_yv)),
         dthdyu((const double *)(A + c.o_dthdyu)), dthdyv((const double *)(A + c.o_dthdyv)),
@@ -74,7 +84,7 @@ struct View {
         fwv((double2 *)(A + c.o_fwv)), fvv((double2 *)(A + c.o_fvv)), psi((double2 *)(A + c.o_psi)),
         div((double2 *)(A + c.o_div)), scr((double *)(A + c.o_scr)), u11((double *)(A + c.o_u11)),
         scal((double *)(A + c.o_scal)), hz((double2 *)(A + c.o_hz)), gx((double2 *)(A + c.o_gx)),
-        hzf((float2 *)(A + c.o_hzf)), gxf((float2 *)(A + c.o_gxf)) {}
+        hzf((float2 *)(A + c.o_hzf)), gxf((float2 *)(A + c.o_gxf)), fac((double *)(A + c.o_fac)) {}
 };
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }

@@ -16,6 +16,7 @@ void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny);
 hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s);
 hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s);
 hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
+hipError_t launch_factors(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
 hipError_t launch_flowrate(char *arena, const Chan *d_chans, int nchan, int set_target, hipStream_t s);
 hipError_t launch_to_rows(const double2 *src, double2 *dst, int ncol, int nrows, int jlo, int ldc,

@@ -339,345 +339,390 @@ __global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int
 }
 
 // ------------------------------------------------------------------------
-// K5: column solves.  One thread per spectral column, three sweeps in y:
-//   sweep 1 (up):   forward elimination of the U/W and V Helmholtz systems
-//                   (LU_buildU/LU_buildV + LU_dec + forward substitution)
+// Factor set-up (once per grid / dt): for every column the LU factors of
+// LU_buildU/LU_buildV + LU_dec for the three substeps (tridLU_3D.f90:257-329,
+// 437-474) and of LU_buildP (tridLU_3D.f90:333-407) eliminated from the top
+// row down.  The operators depend on the grid, k^2 and bRK(kRK) dt/Re only.
+__global__ void __launch_bounds__(64) k_factors(char *arena, const Chan *chans) {
+  const Chan ch = chans[blockIdx.z];
+  const View v(arena, ch);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ch.ncol) return;
+  const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu, nyv = ch.nyv;
+  const size_t plane = (size_t)(nyu + PADROWS) * ldc;
+  const int i = c % nxh, k = c / nxh;
+  const double k2x = v.k2x[i], k2z = v.k2z[k];
+  const double dt = v.scal[SC_DT];
+  for (int kRK = 1; kRK <= 3; kRK++) {
+    const double beta = rk_b(kRK) * dt / ch.Re;
+    double *dU = v.fac + (size_t)((kRK - 1) * 4 + 0) * plane, *cUt = dU + plane;
+    double *dV = dU + 2 * plane, *cVt = dU + 3 * plane;
+    double cU = ch.gw1, cV = 0.0;  // a(3,0)*a(2,0) with a(2,0) = 1
+    dU[c] = 1.0;
+    cUt[c] = cU;
+    dV[c] = 1.0;
+    cVt[c] = cV;
+    for (int j = 1; j < nyu; j++) {
+      double a1, a2, a3;
+      if (j <= nyu - 2) {
+        a1 = -beta * v.dyu2i[3 * j];
+        a2 = 1.0 - beta * ((v.dyu2i[3 * j + 1] + k2x) + k2z);
+        a3 = -beta * v.dyu2i[3 * j + 2];
+      } else {
+        a1 = ch.gw2;
+        a2 = 1.0;
+        a3 = 0.0;
+      }
+      double d = 1.0 / (a2 - a1 * cU);
+      cU = a3 * d;
+      dU[(size_t)j * ldc + c] = d;
+      cUt[(size_t)j * ldc + c] = cU;
+      if (j < nyv) {
+        if (j <= nyv - 2) {
+          a1 = -beta * v.dyv2i[3 * j];
+          a2 = 1.0 - beta * ((v.dyv2i[3 * j + 1] + k2x) + k2z);
+          a3 = -beta * v.dyv2i[3 * j + 2];
+        } else {
+          a1 = 0.0;
+          a2 = 1.0;
+          a3 = 0.0;
+        }
+        d = 1.0 / (a2 - a1 * cV);
+        cV = a3 * d;
+        dV[(size_t)j * ldc + c] = d;
+        cVt[(size_t)j * ldc + c] = cV;
+      }
+    }
+  }
+  double *dP = v.fac + (size_t)12 * plane, *eP = dP + plane;
+  const int ns = 1, ne = nyu - 2;
+  double ep = 0.0;
+  for (int q = ne; q >= ns; q--) {
+    const double du_j = v.dthdyu[q];
+    double a1, a2, a3;
+    if (c == 0 && q == ns) {  // mean-mode pressure pin, tridLU_3D.f90:384-390
+      double h = v.yu[2] - v.yu[1];
+      a1 = 0.0;
+      a2 = 1.0 / (h * h);
+      a3 = 0.0;
+    } else if (q == ns) {
+      a1 = 0.0;
+      a2 = ((du_j * (-v.dthdyv[q])) + k2x) + k2z;
+      a3 = du_j * v.dthdyv[q];
+    } else if (q == ne) {
+      a1 = (-du_j) * (-v.dthdyv[q - 1]);
+      a2 = (((-du_j) * v.dthdyv[q - 1]) + k2x) + k2z;
+      a3 = 0.0;
+    } else {
+      a1 = (-du_j) * (-v.dthdyv[q - 1]);
+      a2 = ((((-du_j) * v.dthdyv[q - 1]) + (du_j * (-v.dthdyv[q]))) + k2x) + k2z;
+      a3 = du_j * v.dthdyv[q];
+    }
+    double d = q == ne ? 1.0 / a2 : 1.0 / (a2 - a3 * ep);
+    ep = a1 * d;
+    dP[(size_t)q * ldc + c] = d;
+    eP[(size_t)q * ldc + c] = ep;
+  }
+}
+
+// ------------------------------------------------------------------------
+// K5: column solves.  One thread per spectral column, three sweeps in y,
+// using the precomputed factors (so each row's recurrence is one or two
+// FMAs):
+//   sweep 1 (up):   forward substitution of the U/W and V Helmholtz systems
 //   sweep 2 (down): back substitution of U/W/V, the divergence of the
-//                   result, and the elimination of the Poisson system
-//                   (LU_buildP operator) from the top row down
+//                   result, and the forward substitution of the Poisson
+//                   system from the top row down
 //   sweep 3 (up):   Poisson back substitution, p += psi, v_corr, the
 //                   flow-rate correction of mode (0,0) and the divergence
-// The recurrences are sequential (Thomas).  Latency is hidden by (a) staging
-// the channel's grid coefficients in LDS once per block, and (b) reading the
-// field rows in blocks of SB, the loads of block b+1 issued before the
-// stores of block b (on gfx9-class GPUs one counter tracks loads and stores
-// in order, so a load issued after a store also waits for that store).
+// Field rows are read in blocks of SB, the loads of block b+1 issued before
+// the stores of block b (on gfx9-class GPUs one counter tracks loads and
+// stores in order, so a load issued after a store also waits for it).  The
+// grid coefficients sit in LDS.
 #define SB 8
 
-// LDS copy of the grid coefficients of one channel
 struct SolveLDS {
-  double *cu0, *cu1, *cu2, *cv0, *cv1, *cv2, *du, *dv, *yv;
+  double *cu0, *cv0, *du, *dv, *yv;
 };
 __device__ SolveLDS stage_coeffs(double *sm, const View &v, int nyu, int nyv) {
   SolveLDS L;
   L.cu0 = sm;
-  L.cu1 = L.cu0 + nyu;
-  L.cu2 = L.cu1 + nyu;
-  L.cv0 = L.cu2 + nyu;
-  L.cv1 = L.cv0 + nyv;
-  L.cv2 = L.cv1 + nyv;
-  L.du = L.cv2 + nyv;
-  L.dv = L.du + nyu;
+  L.du = L.cu0 + nyu;
+  L.cv0 = L.du + nyu;
+  L.dv = L.cv0 + nyv;
   L.yv = L.dv + nyv;
   for (int j = threadIdx.x; j < nyu; j += blockDim.x) {
     L.cu0[j] = v.dyu2i[3 * j];
-    L.cu1[j] = v.dyu2i[3 * j + 1];
-    L.cu2[j] = v.dyu2i[3 * j + 2];
     L.du[j] = v.dthdyu[j];
   }
   for (int j = threadIdx.x; j < nyv; j += blockDim.x) {
     L.cv0[j] = v.dyv2i[3 * j];
-    L.cv1[j] = v.dyv2i[3 * j + 1];
-    L.cv2[j] = v.dyv2i[3 * j + 2];
     L.dv[j] = v.dthdyv[j];
     L.yv[j] = v.yv[j];
   }
   __syncthreads();
   return L;
 }
-__host__ __device__ inline size_t solve_lds_doubles(int nyu) { return 4 * (size_t)nyu + 5 * (size_t)(nyu - 1); }
+__host__ __device__ inline size_t solve_lds_doubles(int nyu) { return 2 * (size_t)nyu + 3 * (size_t)(nyu - 1); }
 
-struct Blk1 { double2 a[SB], b[SB], c[SB]; };
-struct Blk2 { double2 a[SB], b[SB], c[SB]; double e[SB], f[SB]; };
-struct Blk3 { double2 a[SB], b[SB], c[SB], d[SB], g[SB]; double e[SB]; };
+// Row access through buffer resources: the array base lives in a scalar
+// resource descriptor, the row offset in a scalar register and the lane's
+// column in a vector register, so no vector instruction computes addresses.
+typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i32x4;
+typedef __attribute__((__vector_size__(2 * sizeof(int)))) int i32x2;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mkbuf(const void *p, size_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)(bytes < 0x7fffffffu ? bytes : 0x7fffffffu),
+                                           0x00020000);
+}
+#define RB16(j) ((unsigned)(j) * rowb16)
+#define RB8(j) ((unsigned)(j) * rowb8)
+#define LDC(buf, j) __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(buf, oc, RB16(j), 0))
+#define LDR(buf, j) __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(buf, orl, RB8(j), 0))
+#define STC(buf, j, x) __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i32x4, (double2)(x)), buf, os, RB16(j), 0)
 
 __global__ void __launch_bounds__(64) k_solve(char *arena, const Chan *chans, int kRK) {
-  extern __shared__ double smem_s[];
   const Chan ch = chans[blockIdx.z];
   const View v(arena, ch);
   const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu, nyv = ch.nyv;
   if ((int)(blockIdx.x * blockDim.x) >= ch.ncol) return;
-  const SolveLDS L = stage_coeffs(smem_s, v, nyu, nyv);
   const int c0 = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = c0 < ch.ncol;
-  const int c = live ? c0 : ch.ncol - 1;  // idle lanes shadow a real column, never store
+  const int c = live ? c0 : ch.ncol - 1;  // idle lanes shadow a real column
+  const unsigned oc = (unsigned)c * 16u, orl = (unsigned)c * 8u;
+  const unsigned os = (unsigned)(live ? c : ch.ncol) * 16u;  // idle lanes store into column padding
   const int i = c % nxh, k = c / nxh;
   const double dt = v.scal[SC_DT];
   const double beta = rk_b(kRK) * dt / ch.Re;
-  const double k2x = v.k2x[i], k2z = v.k2z[k];
   const double kxi = v.kx[i], kzk = v.kz[k];
-  double *__restrict__ scU = v.scr;
-  double *__restrict__ scV = v.scr + (size_t)nyu * ldc;
-  double2 *__restrict__ u1 = v.u1;
-  double2 *__restrict__ u2 = v.u2;
-  double2 *__restrict__ u3 = v.u3;
-  const double2 *__restrict__ r1 = v.r1;
-  const double2 *__restrict__ r2 = v.r2;
-  const double2 *__restrict__ r3 = v.r3;
-  double2 *__restrict__ psi = v.psi;
-  double2 *__restrict__ pp = v.p;
-  double2 *__restrict__ dvg = v.div;
+  const size_t plane = (size_t)(nyu + PADROWS) * ldc;
+  const unsigned rowb16 = (unsigned)ldc * 16u, rowb8 = (unsigned)ldc * 8u;
+  const size_t fb16 = plane * 16, fb8 = plane * 8;
+  const double *f0 = v.fac + (size_t)((kRK - 1) * 4) * plane;
+  const auto dU = mkbuf(f0, fb8), cUt = mkbuf(f0 + plane, fb8), dV = mkbuf(f0 + 2 * plane, fb8),
+             cVt = mkbuf(f0 + 3 * plane, fb8);
+  const auto dP = mkbuf(v.fac + (size_t)12 * plane, fb8), eP = mkbuf(v.fac + (size_t)13 * plane, fb8);
+  const double *__restrict__ cu0 = v.dyu2i;  // stride 3
+  const double *__restrict__ cv0 = v.dyv2i;
+  const double *__restrict__ du = v.dthdyu;
+  const double *__restrict__ dvy = v.dthdyv;
+  const double *__restrict__ yv = v.yv;
+  const auto u1 = mkbuf(v.u1, fb16), u2 = mkbuf(v.u2, fb16), u3 = mkbuf(v.u3, fb16);
+  const auto r1 = mkbuf(v.r1, fb16), r2 = mkbuf(v.r2, fb16), r3 = mkbuf(v.r3, fb16);
+  const auto psi = mkbuf(v.psi, fb16), pp = mkbuf(v.p, fb16), dvg = mkbuf(v.div, fb16);
   const int ns = 1, ne = nyu - 2;
-  const double gw2 = ch.gw2;
+  const bool col0 = c0 == 0;
 
-  // ---- sweep 1: forward elimination, U/W rows 0..nyu-1, V rows 0..nyv-1
-  double cU = ch.gw1, cV = 0.0;
-  double2 y1 = r1[c], y3 = r3[c], y2 = r2[c];
-  if (live) {
-    u1[c] = y1;
-    u3[c] = y3;
-    u2[c] = y2;
-    scU[c] = cU;
-    scV[c] = cV;
+  // ---- sweep 1: forward substitution, U/W rows 1..nyu-1, V rows 1..nyv-1
+  double2 y1 = LDC(r1, 0), y3 = LDC(r3, 0), y2 = LDC(r2, 0);  // row 0: a(2,0) = 1
+  STC(u1, 0, y1);
+  STC(u3, 0, y3);
+  STC(u2, 0, y2);
+  auto row1 = [&](int j, double2 R1, double2 R3, double2 R2, double d, double e, double a1, double b1) {
+    y1 = make_double2((R1.x - a1 * y1.x) * d, (R1.y - a1 * y1.y) * d);
+    y3 = make_double2((R3.x - a1 * y3.x) * d, (R3.y - a1 * y3.y) * d);
+    y2 = make_double2((R2.x - b1 * y2.x) * e, (R2.y - b1 * y2.y) * e);
+    STC(u1, j, y1);
+    STC(u3, j, y3);
+    STC(u2, j, y2);
+  };
+  int j = 1;
+  const int jend1 = nyv - 2;  // last row where both systems are interior
+  for (; j + SB - 1 <= jend1; j += SB) {
+    double2 R1[SB], R3[SB], R2[SB];
+    double D[SB], E[SB];
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      R1[q] = LDC(r1, j + q);
+      R3[q] = LDC(r3, j + q);
+      R2[q] = LDC(r2, j + q);
+      D[q] = LDR(dU, j + q);
+      E[q] = LDR(dV, j + q);
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      row1(j + q, R1[q], R3[q], R2[q], D[q], E[q], -beta * cu0[3 * (j + q)], -beta * cv0[3 * (j + q)]);
   }
-  auto load1 = [&](Blk1 &B, int jb) {
-#pragma unroll
-    for (int q = 0; q < SB; q++) {
-      int j = min(jb + q, nyu - 1);
-      size_t o = (size_t)j * ldc + c;
-      B.a[q] = r1[o];
-      B.b[q] = r3[o];
-      B.c[q] = r2[(size_t)min(j, nyv - 1) * ldc + c];
-    }
-  };
-  auto comp1 = [&](const Blk1 &B, int jb) {
-#pragma unroll
-    for (int q = 0; q < SB; q++) {
-      int j = jb + q;
-      if (j < nyu) {
-        size_t o = (size_t)j * ldc + c;
-        double a1, a2, a3;
-        if (j <= nyu - 2) {
-          a1 = -beta * L.cu0[j];
-          a2 = 1.0 - beta * ((L.cu1[j] + k2x) + k2z);
-          a3 = -beta * L.cu2[j];
-        } else {
-          a1 = gw2;
-          a2 = 1.0;
-          a3 = 0.0;
-        }
-        double d = 1.0 / (a2 - a1 * cU);
-        y1 = make_double2((B.a[q].x - a1 * y1.x) * d, (B.a[q].y - a1 * y1.y) * d);
-        y3 = make_double2((B.b[q].x - a1 * y3.x) * d, (B.b[q].y - a1 * y3.y) * d);
-        cU = a3 * d;
-        if (live) {
-          u1[o] = y1;
-          u3[o] = y3;
-          scU[o] = cU;
-        }
-        if (j < nyv) {
-          if (j <= nyv - 2) {
-            a1 = -beta * L.cv0[j];
-            a2 = 1.0 - beta * ((L.cv1[j] + k2x) + k2z);
-            a3 = -beta * L.cv2[j];
-          } else {
-            a1 = 0.0;
-            a2 = 1.0;
-            a3 = 0.0;
-          }
-          d = 1.0 / (a2 - a1 * cV);
-          y2 = make_double2((B.c[q].x - a1 * y2.x) * d, (B.c[q].y - a1 * y2.y) * d);
-          cV = a3 * d;
-          if (live) {
-            u2[o] = y2;
-            scV[o] = cV;
-          }
-        }
-      }
-    }
-  };
+  for (; j <= jend1; j++)
+    row1(j, LDC(r1, j), LDC(r3, j), LDC(r2, j), LDR(dU, j), LDR(dV, j), -beta * cu0[3 * j], -beta * cv0[3 * j]);
   {
-    Blk1 A, B;
-    load1(A, 1);
-    for (int jb = 1; jb < nyu; jb += 2 * SB) {
-      load1(B, jb + SB);
-      comp1(A, jb);
-      load1(A, jb + 2 * SB);
-      comp1(B, jb + SB);
-    }
+    // row nyv-1: V wall row (a1 = 0), U/W interior row nyu-2
+    j = nyv - 1;
+    row1(j, LDC(r1, j), LDC(r3, j), LDC(r2, j), LDR(dU, j), LDR(dV, j), -beta * cu0[3 * j], 0.0);
+    // row nyu-1: U/W wall row (a1 = gw2)
+    j = nyu - 1;
+    const double d = LDR(dU, j), a1 = ch.gw2;
+    const double2 R1 = LDC(r1, j), R3 = LDC(r3, j);
+    y1 = make_double2((R1.x - a1 * y1.x) * d, (R1.y - a1 * y1.y) * d);
+    y3 = make_double2((R3.x - a1 * y3.x) * d, (R3.y - a1 * y3.y) * d);
+    STC(u1, j, y1);
+    STC(u3, j, y3);
   }
 
-  // ---- sweep 2: back substitution (rows nyu-2..0), divergence, Poisson
-  //      elimination from row ne down to ns
+  // ---- sweep 2: back substitution (rows nyu-2..0), divergence of pressure
+  //      row j+1, Poisson forward substitution from row ne down to ns
   const double dtirk = (1.0 / dt) / (rk_a(kRK) + rk_b(kRK));
-  const double yu21 = v.yu[2] - v.yu[1];
   double2 x1 = y1, x3 = y3;  // row nyu-1 (final)
   double2 x2 = y2;           // row nyv-1 = nyu-2 (final)
-  double ep = 0.0;
   double2 yp = make_double2(0, 0);
   double Qx0 = 0.0;
-  auto load2 = [&](Blk2 &B, int jb) {
-#pragma unroll
-    for (int q = 0; q < SB; q++) {
-      int j = max(jb - q, 0);
-      size_t o = (size_t)j * ldc + c;
-      B.a[q] = u1[o];
-      B.b[q] = u3[o];
-      B.e[q] = scU[o];
-      size_t ov = (size_t)min(j, nyv - 2) * ldc + c;
-      B.c[q] = u2[ov];
-      B.f[q] = scV[ov];
-    }
-  };
-  auto comp2 = [&](const Blk2 &B, int jb) {
-#pragma unroll
-    for (int q = 0; q < SB; q++) {
-      int j = jb - q;
-      if (j >= 0) {
-        size_t o = (size_t)j * ldc + c;
-        double2 n1 = make_double2(B.a[q].x - B.e[q] * x1.x, B.a[q].y - B.e[q] * x1.y);
-        double2 n3 = make_double2(B.b[q].x - B.e[q] * x3.x, B.b[q].y - B.e[q] * x3.y);
-        double2 n2 = x2;
-        if (j <= nyv - 2) n2 = make_double2(B.c[q].x - B.f[q] * x2.x, B.c[q].y - B.f[q] * x2.y);
-        if (live) {
-          u1[o] = n1;
-          u3[o] = n3;
-          if (j <= nyv - 2) u2[o] = n2;
-        }
-        const int qr = j + 1;  // pressure row whose divergence is now known
-        if (qr >= ns && qr <= ne) {
-          double2 a = cimul(kxi, x1), b = cimul(kzk, x3);
-          double w = L.du[qr];
-          double2 dv = make_double2((a.x + b.x) + w * (x2.x - n2.x), (a.y + b.y) + w * (x2.y - n2.y));
-          double2 rhs = make_double2(dtirk * dv.x, dtirk * dv.y);
-          double a1, a2, a3;
-          const double du_j = w;
-          if (c == 0 && qr == ns) {  // mean-mode pressure pin, tridLU_3D.f90:384-390
-            rhs = make_double2(0, 0);
-            a1 = 0.0;
-            a2 = 1.0 / (yu21 * yu21);
-            a3 = 0.0;
-          } else if (qr == ns) {
-            a1 = 0.0;
-            a2 = ((du_j * (-L.dv[qr])) + k2x) + k2z;
-            a3 = du_j * L.dv[qr];
-          } else if (qr == ne) {
-            a1 = (-du_j) * (-L.dv[qr - 1]);
-            a2 = (((-du_j) * L.dv[qr - 1]) + k2x) + k2z;
-            a3 = 0.0;
-          } else {
-            a1 = (-du_j) * (-L.dv[qr - 1]);
-            a2 = ((((-du_j) * L.dv[qr - 1]) + (du_j * (-L.dv[qr]))) + k2x) + k2z;
-            a3 = du_j * L.dv[qr];
-          }
-          if (c == 0) Qx0 = Qx0 + (L.yv[qr] - L.yv[qr - 1]) * x1.x;
-          double d = qr == ne ? 1.0 / a2 : 1.0 / (a2 - a3 * ep);
-          yp = make_double2((rhs.x - a3 * yp.x) * d, (rhs.y - a3 * yp.y) * d);
-          ep = a1 * d;
-          if (live) {
-            size_t oq = (size_t)qr * ldc + c;
-            psi[oq] = yp;
-            scU[oq] = ep;  // row qr of scU was read one row earlier
-          }
-        }
-        x1 = n1;
-        x3 = n3;
-        x2 = n2;
-      }
-    }
-  };
   {
-    Blk2 A, B;
-    load2(A, nyu - 2);
-    for (int jb = nyu - 2; jb >= 0; jb -= 2 * SB) {
-      load2(B, jb - SB);
-      comp2(A, jb);
-      load2(A, jb - 2 * SB);
-      comp2(B, jb - SB);
+    // row nyu-2: u1, u3 only (v row nyv-1 is final), no pressure row
+    j = nyu - 2;
+    const double cj = LDR(cUt, j);
+    const double2 X1 = LDC(u1, j), X3 = LDC(u3, j);
+    x1 = make_double2(X1.x - cj * x1.x, X1.y - cj * x1.y);
+    x3 = make_double2(X3.x - cj * x3.x, X3.y - cj * x3.y);
+    STC(u1, j, x1);
+    STC(u3, j, x3);
+  }
+  // row j (0 <= j <= nyu-3) with pressure row qr = j+1; a3 is zero on row
+  // ne and on the mean mode's pinned row (the caller passes dvq = 0)
+  auto row2 = [&](int j, double2 X1, double2 X3, double2 X2, double cj, double cv, double d, double w,
+                  double dvq, double dyv) {
+    double2 n1 = make_double2(X1.x - cj * x1.x, X1.y - cj * x1.y);
+    double2 n3 = make_double2(X3.x - cj * x3.x, X3.y - cj * x3.y);
+    double2 n2 = make_double2(X2.x - cv * x2.x, X2.y - cv * x2.y);
+    STC(u1, j, n1);
+    STC(u3, j, n3);
+    STC(u2, j, n2);
+    double2 a = cimul(kxi, x1), b = cimul(kzk, x3);
+    double2 dv = make_double2((a.x + b.x) + w * (x2.x - n2.x), (a.y + b.y) + w * (x2.y - n2.y));
+    double2 rhs = make_double2(dtirk * dv.x, dtirk * dv.y);
+    double a3 = w * dvq;
+    if (col0) Qx0 = Qx0 + dyv * x1.x;
+    yp = make_double2((rhs.x - a3 * yp.x) * d, (rhs.y - a3 * yp.y) * d);
+    STC(psi, j + 1, yp);
+    x1 = n1;
+    x3 = n3;
+    x2 = n2;
+  };
+  j = nyu - 3;  // pressure row ne: a3 = 0
+  row2(j, LDC(u1, j), LDC(u3, j), LDC(u2, j), LDR(cUt, j), LDR(cVt, j), LDR(dP, j + 1), du[j + 1], 0.0,
+       yv[j + 1] - yv[j]);
+  j = nyu - 4;
+  for (; j - SB + 1 >= 1; j -= SB) {
+    double2 X1[SB], X3[SB], X2[SB];
+    double C[SB], CV[SB], Dp[SB];
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      X1[q] = LDC(u1, j - q);
+      X3[q] = LDC(u3, j - q);
+      X2[q] = LDC(u2, j - q);
+      C[q] = LDR(cUt, j - q);
+      CV[q] = LDR(cVt, j - q);
+      Dp[q] = LDR(dP, j - q + 1);
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int jj = j - q;
+      row2(jj, X1[q], X3[q], X2[q], C[q], CV[q], Dp[q], du[jj + 1], dvy[jj + 1], yv[jj + 1] - yv[jj]);
+    }
+  }
+  for (; j >= 1; j--)
+    row2(j, LDC(u1, j), LDC(u3, j), LDC(u2, j), LDR(cUt, j), LDR(cVt, j), LDR(dP, j + 1), du[j + 1], dvy[j + 1],
+         yv[j + 1] - yv[j]);
+  {
+    // j = 0: pressure row ns = 1; the mean mode's row is pinned (rhs = 0)
+    const double d = LDR(dP, 1);
+    const double2 yp_before = yp;
+    row2(0, LDC(u1, 0), LDC(u3, 0), LDC(u2, 0), LDR(cUt, 0), LDR(cVt, 0), d, du[1], col0 ? 0.0 : dvy[1],
+         yv[1] - yv[0]);
+    if (col0) {
+      yp = make_double2((0.0 - 0.0 * yp_before.x) * d, (0.0 - 0.0 * yp_before.y) * d);
+      STC(psi, 1, yp);
     }
   }
   // flow-rate correction of mode (0,0): flowrate_corr, littleharsh.f90:476-509
   double g = 0.0;
-  if (c0 == 0 && ch.flag_ctpress == 0) {
+  if (col0 && ch.flag_ctpress == 0) {
     g = (v.scal[SC_QXT] - Qx0) / v.scal[SC_QX1 + kRK - 1];
     v.scal[SC_MPGX] = v.scal[SC_MPGX] + g;
   }
-  const double *u11 = v.u11 + (size_t)(kRK - 1) * nyu;
+  const double *__restrict__ u11 = v.u11 + (size_t)(kRK - 1) * nyu;
 
-  // ---- sweep 3: Poisson back substitution (rows ns..ne), p += psi, v_corr
+  // ---- sweep 3: Poisson back substitution (rows ns..ne), p += psi, v_corr,
+  //      divergence of pressure row j-1
   const double dtrk = dt * (rk_a(kRK) + rk_b(kRK));
   const double dkx = dtrk * kxi, dkz = dtrk * kzk;
-  double2 pprev = make_double2(0, 0);                          // psi(j-1)
-  double2 u1p = make_double2(0, 0), u3p = make_double2(0, 0);  // corrected u at j-1
-  double2 vpp = make_double2(0, 0);                            // corrected v at j-2 (v(0) = 0)
-  double2 u1first = make_double2(0, 0), u3first = make_double2(0, 0);
-  auto load3 = [&](Blk3 &B, int jb) {
-#pragma unroll
-    for (int q = 0; q < SB; q++) {
-      int j = min(jb + q, ne);
-      size_t o = (size_t)j * ldc + c;
-      B.a[q] = psi[o];
-      B.e[q] = scU[o];
-      B.b[q] = pp[o];
-      B.c[q] = u1[o];
-      B.d[q] = u3[o];
-      B.g[q] = u2[o - ldc];
-    }
-  };
-  auto comp3 = [&](const Blk3 &B, int jb) {
-#pragma unroll
-    for (int q = 0; q < SB; q++) {
-      int j = jb + q;
-      if (j <= ne) {
-        size_t o = (size_t)j * ldc + c;
-        double2 ps = j == ns ? B.a[q] : make_double2(B.a[q].x - B.e[q] * pprev.x, B.a[q].y - B.e[q] * pprev.y);
-        double2 a = cimul(dkx, ps), b = cimul(dkz, ps);
-        double2 n1 = make_double2(B.c[q].x - a.x, B.c[q].y - a.y);
-        double2 n3 = make_double2(B.d[q].x - b.x, B.d[q].y - b.y);
-        if (c0 == 0) n1.x = n1.x + g * u11[j];
-        if (live) {
-          psi[o] = ps;
-          pp[o] = make_double2(B.b[q].x + ps.x, B.b[q].y + ps.y);
-          u1[o] = n1;
-          u3[o] = n3;
-        }
-        if (j == ns) {
-          u1first = n1;
-          u3first = n3;
-        }
-        if (j - 1 >= 1) {
-          // v row j-1 and the divergence of pressure row j-1
-          double sv = dtrk * L.dv[j - 1];
-          double2 vv = make_double2(B.g[q].x - sv * (ps.x - pprev.x), B.g[q].y - sv * (ps.y - pprev.y));
-          double2 da = cimul(kxi, u1p), db = cimul(kzk, u3p);
-          double w = L.du[j - 1];
-          double2 dd = make_double2((da.x + db.x) + w * (vv.x - vpp.x), (da.y + db.y) + w * (vv.y - vpp.y));
-          if (live) {
-            size_t ov = (size_t)(j - 1) * ldc + c;
-            u2[ov] = vv;
-            dvg[ov] = dd;
-          }
-          vpp = vv;
-        }
-        pprev = ps;
-        u1p = n1;
-        u3p = n3;
-      }
-    }
-  };
+  double2 pprev, u1p, u3p, vpp = make_double2(0, 0);  // v(0) = 0
+  double2 u1first, u3first;
   {
-    Blk3 A, B;
-    load3(A, ns);
-    for (int jb = ns; jb <= ne; jb += 2 * SB) {
-      load3(B, jb + SB);
-      comp3(A, jb);
-      load3(A, jb + 2 * SB);
-      comp3(B, jb + SB);
-    }
+    // row ns: psi(1) = y(1)
+    pprev = LDC(psi, 1);
+    double2 a = cimul(dkx, pprev), b = cimul(dkz, pprev);
+    double2 U1 = LDC(u1, 1), U3 = LDC(u3, 1), P = LDC(pp, 1);
+    u1p = make_double2(U1.x - a.x, U1.y - a.y);
+    u3p = make_double2(U3.x - b.x, U3.y - b.y);
+    if (col0) u1p.x = u1p.x + g * u11[1];
+    STC(pp, 1, make_double2(P.x + pprev.x, P.y + pprev.y));
+    STC(u1, 1, u1p);
+    STC(u3, 1, u3p);
+    u1first = u1p;
+    u3first = u3p;
   }
+  auto row3 = [&](int j, double2 Y, double e, double2 P, double2 U1, double2 U3, double2 V) {
+    double2 ps = make_double2(Y.x - e * pprev.x, Y.y - e * pprev.y);
+    double2 a = cimul(dkx, ps), b = cimul(dkz, ps);
+    double2 n1 = make_double2(U1.x - a.x, U1.y - a.y);
+    double2 n3 = make_double2(U3.x - b.x, U3.y - b.y);
+    if (col0) n1.x = n1.x + g * u11[j];
+    STC(psi, j, ps);
+    STC(pp, j, make_double2(P.x + ps.x, P.y + ps.y));
+    STC(u1, j, n1);
+    STC(u3, j, n3);
+    // v row j-1 and the divergence of pressure row j-1
+    const double sv = dtrk * dvy[j - 1], w = du[j - 1];
+    double2 vv = make_double2(V.x - sv * (ps.x - pprev.x), V.y - sv * (ps.y - pprev.y));
+    double2 da = cimul(kxi, u1p), db = cimul(kzk, u3p);
+    double2 dd = make_double2((da.x + db.x) + w * (vv.x - vpp.x), (da.y + db.y) + w * (vv.y - vpp.y));
+    STC(u2, j - 1, vv);
+    STC(dvg, j - 1, dd);
+    vpp = vv;
+    pprev = ps;
+    u1p = n1;
+    u3p = n3;
+  };
+  j = 2;
+  for (; j + SB - 1 <= ne; j += SB) {
+    double2 Y[SB], P[SB], U1[SB], U3[SB], V[SB];
+    double E[SB];
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      Y[q] = LDC(psi, j + q);
+      E[q] = LDR(eP, j + q);
+      P[q] = LDC(pp, j + q);
+      U1[q] = LDC(u1, j + q);
+      U3[q] = LDC(u3, j + q);
+      V[q] = LDC(u2, j + q - 1);
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++) row3(j + q, Y[q], E[q], P[q], U1[q], U3[q], V[q]);
+  }
+  for (; j <= ne; j++) row3(j, LDC(psi, j), LDR(eP, j), LDC(pp, j), LDC(u1, j), LDC(u3, j), LDC(u2, j - 1));
   if (live) {
     // divergence of row ne: v(ne) is the upper wall
     double2 da = cimul(kxi, u1p), db = cimul(kzk, u3p);
-    double w = L.du[ne];
-    dvg[(size_t)ne * ldc + c] = make_double2((da.x + db.x) + w * (0.0 - vpp.x), (da.y + db.y) + w * (0.0 - vpp.y));
+    double w = du[ne];
+    STC(dvg, ne, make_double2((da.x + db.x) + w * (0.0 - vpp.x), (da.y + db.y) + w * (0.0 - vpp.y)));
     // ghost cells and walls (v_corr, littleharsh.f90:992-1044)
-    u1[c] = cscale(ch.gwbc1, u1first);
-    u1[(size_t)(nyu - 1) * ldc + c] = cscale(ch.gwbc1, u1p);
-    u3[c] = cscale(ch.gwbc3, u3first);
-    u3[(size_t)(nyu - 1) * ldc + c] = cscale(ch.gwbc3, u3p);
-    u2[c] = make_double2(0, 0);
-    u2[(size_t)(nyv - 1) * ldc + c] = make_double2(0, 0);
+    STC(u1, 0, cscale(ch.gwbc1, u1first));
+    STC(u1, nyu - 1, cscale(ch.gwbc1, u1p));
+    STC(u3, 0, cscale(ch.gwbc3, u3first));
+    STC(u3, nyu - 1, cscale(ch.gwbc3, u3p));
+    STC(u2, 0, make_double2(0, 0));
+    STC(u2, nyv - 1, make_double2(0, 0));
   }
+}
+#undef RB16
+#undef RB8
+#undef LDC
+#undef LDR
+#undef STC
+
+hipError_t launch_factors(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s) {
+  dim3 g((m.ncol + 63) / 64, 1, m.nchan);
+  hipLaunchKernelGGL(k_factors, g, dim3(64), 0, s, arena, d_chans);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------
@@ -790,7 +835,7 @@ hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int k
 
 hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s) {
   dim3 g((m.ncol + 63) / 64, 1, m.nchan);
-  hipLaunchKernelGGL(k_solve, g, dim3(64), solve_lds_doubles(m.nyu) * sizeof(double), s, arena, d_chans, kRK);
+  hipLaunchKernelGGL(k_solve, g, dim3(64), 0, s, arena, d_chans, kRK);
   return hipGetLastError();
 }
 
