@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-substeps", type=int, default=30)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB, 5 times (calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters against known traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one channel ensemble on one GPU; --gpus %d is not "
@@ -108,6 +111,15 @@ def main():
     from rnl_multiblock_nofft_amd import Solver, configs, initial
 
     torch.cuda.set_device(0)
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_probe_bytes=n * 8, launches=5)), flush=True)
+        return
     cfgs = configs.WORKLOADS[args.workload]()
     s = Solver(cfgs, precision=args.precision)
     for q, c in enumerate(cfgs):

@@ -20,6 +20,7 @@
 #ifndef LHGPU_H
 #define LHGPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -145,6 +146,15 @@ int lh_phys_to_four_planes(int N1, int Ngal1, int Ngal2, double *d_planes,
 int lh_lusol_uw(lh_ctx *ctx, int ch, int kRK, const double *d_rhsu,
                 const double *d_rhsw, double *d_u, double *d_w);
 int lh_lusol_v(lh_ctx *ctx, int ch, int kRK, const double *d_rhsv, double *d_v);
+
+/* Device memory for the device-pointer entry points above, for hosts without
+ * HIP bindings of their own (the Fortran shims stage the reference's host
+ * arrays through these).  No counterpart in the reference, whose arrays are
+ * host memory.  Copies are synchronous. */
+int lh_dev_alloc(size_t bytes, void **out);
+int lh_dev_free(void *p);
+int lh_copy_to_device(void *d, const void *h, size_t bytes);
+int lh_copy_to_host(void *h, const void *d, size_t bytes);
 
 #ifdef __cplusplus
 }

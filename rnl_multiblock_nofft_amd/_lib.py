@@ -23,7 +23,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTS = ["lh_last_error", "lh_ctx_create", "lh_ctx_destroy", "lh_set_grid", "lh_get_grid",
            "lh_field_upload", "lh_field_download", "lh_get_scalars", "lh_set_scalars",
            "lh_init_flowrate", "lh_substep", "lh_step", "lh_sync", "lh_time_kernels", "lh_run_phase",
-           "lh_four_to_phys_planes", "lh_phys_to_four_planes", "lh_lusol_uw", "lh_lusol_v"]
+           "lh_four_to_phys_planes", "lh_phys_to_four_planes", "lh_lusol_uw", "lh_lusol_v",
+           "lh_dev_alloc", "lh_dev_free", "lh_copy_to_device", "lh_copy_to_host"]
 
 
 class LhConfig(ctypes.Structure):
@@ -72,6 +73,10 @@ def load():
     L.lh_phys_to_four_planes.argtypes = [ip, ip, ip, vp, ip, vp]
     L.lh_lusol_uw.argtypes = [vp, ip, ip, vp, vp, vp, vp]
     L.lh_lusol_v.argtypes = [vp, ip, ip, vp, vp]
+    L.lh_dev_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
+    L.lh_dev_free.argtypes = [vp]
+    L.lh_copy_to_device.argtypes = [vp, vp, ctypes.c_size_t]
+    L.lh_copy_to_host.argtypes = [vp, vp, ctypes.c_size_t]
     for name in EXPORTS:
         if name != "lh_last_error":
             getattr(L, name).restype = ctypes.c_int

@@ -762,4 +762,32 @@ int lh_lusol_v(lh_ctx *x, int ch, int kRK, const double *rv, double *v) {
   return LH_OK;
 }
 
+int lh_dev_alloc(size_t bytes, void **out) {
+  if (!out || bytes == 0) return set_err(LH_EINVAL, "lh_dev_alloc: bad arguments");
+  *out = nullptr;
+  if (hipMalloc(out, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return set_err(LH_ENOMEM, "lh_dev_alloc: hipMalloc of %zu bytes failed", bytes);
+  }
+  return LH_OK;
+}
+
+int lh_dev_free(void *p) {
+  if (p) HIPCHK(hipFree(p));
+  return LH_OK;
+}
+
+int lh_copy_to_device(void *d, const void *h, size_t bytes) {
+  if (!d || !h) return set_err(LH_EINVAL, "lh_copy_to_device: null pointer");
+  HIPCHK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+  return LH_OK;
+}
+
+int lh_copy_to_host(void *h, const void *d, size_t bytes) {
+  if (!d || !h) return set_err(LH_EINVAL, "lh_copy_to_host: null pointer");
+  HIPCHK(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
+  return LH_OK;
+}
+
 }  // extern "C"

@@ -1,0 +1,269 @@
+! lh_host.f90 -- Fortran host layer over liblhgpu: the reference's subroutine
+! names and argument meanings, forwarding to the MI355X kernels.
+!
+! Module state mirrors what the reference keeps in `module declaration`
+! (declaration.f90): N(1:2,iband), Ngal(1:2,iband), the substep index kRK,
+! the number of bands.  Here one "band" is one channel of the device
+! context (the reference's bands split one channel in y across MPI ranks;
+! on one GPU a channel is never split, so every band index selects a whole
+! channel and several bands form an ensemble).
+!
+! Routines with the reference's names:
+!   four_to_phys_u, four_to_phys_du, phys_to_four_du   (FOU3D.f90:738-784,2253)
+!   LUsolU_W, LUsolV                                    (tridLU_3D.f90:55,135)
+!   read_in_field, record_out_field                     (start.f90:2444, FOU3D.f90:2856)
+! The LU factor argument `a`, `grid` and `myid` of LUsolU_W/LUsolV are
+! dropped: the factors of the fixed-dt operators live on the device (built
+! once at setup, as LU_buildP is in the reference), and there is one rank.
+module lh_host
+  use iso_c_binding
+  use lhgpu
+  implicit none
+
+  type(c_ptr), save :: lhctx = c_null_ptr
+  integer, save :: nband = 0
+  integer, allocatable, save :: N(:,:), Ngal(:,:), nyb(:)
+  integer, save :: kRK = 1
+
+  ! device staging for the plane and solve shims
+  type(c_ptr), save :: dstage = c_null_ptr
+  integer(c_size_t), save :: dstage_bytes = 0
+
+  ! header of a field file, kept so that record_out writes what read_in read
+  type :: field_header
+    real(8) :: t = 0d0, Re = 0d0, alp = 1d0, bet = 2d0, mpgx = 0d0
+    integer :: nband = 1, iter = 0
+    integer, allocatable :: Nb(:,:)
+    real(8), allocatable :: y(:), dthdy(:)
+    real(8) :: dthetavi = 1d0
+  end type field_header
+
+contains
+
+  ! ---------------------------------------------------------------------
+  ! set-up / tear-down (replaces start's per-band setup, start.f90:36-691)
+  subroutine lh_host_setup(cfgs)
+    type(lh_config), intent(in) :: cfgs(:)
+    integer :: b
+    nband = size(cfgs)
+    allocate(N(2, nband), Ngal(2, nband), nyb(nband))
+    do b = 1, nband
+      N(1, b) = cfgs(b)%N1;  N(2, b) = cfgs(b)%N2
+      Ngal(1, b) = cfgs(b)%Ngal1;  Ngal(2, b) = cfgs(b)%Ngal2
+      nyb(b) = cfgs(b)%ny
+    end do
+    call lh_check(lh_ctx_create(cfgs, int(nband, c_int), lhctx), 'lh_ctx_create')
+  end subroutine lh_host_setup
+
+  subroutine lh_host_finish()
+    if (c_associated(dstage)) call lh_check(lh_dev_free(dstage), 'lh_dev_free')
+    dstage = c_null_ptr
+    dstage_bytes = 0
+    if (c_associated(lhctx)) call lh_check(lh_ctx_destroy(lhctx), 'lh_ctx_destroy')
+    lhctx = c_null_ptr
+    if (allocated(N)) deallocate(N, Ngal, nyb)
+    nband = 0
+  end subroutine lh_host_finish
+
+  ! device staging buffer of at least `bytes`; returns the pointer `off`
+  ! bytes into it
+  function staging(bytes, off) result(p)
+    integer(c_size_t), intent(in) :: bytes, off
+    type(c_ptr) :: p
+    integer(c_intptr_t) :: a
+    if (bytes > dstage_bytes) then
+      if (c_associated(dstage)) call lh_check(lh_dev_free(dstage), 'lh_dev_free')
+      call lh_check(lh_dev_alloc(bytes, dstage), 'lh_dev_alloc')
+      dstage_bytes = bytes
+    end if
+    a = transfer(dstage, a) + int(off, c_intptr_t)
+    p = transfer(a, p)
+  end function staging
+
+  ! ---------------------------------------------------------------------
+  ! FOU3D.f90:766 four_to_phys_du: oddball row Ngal2/2+1 zeroed, inverse z
+  ! transform of the retained x-modes, inverse x transform; in place
+  subroutine four_to_phys_du(du, iband)
+    integer, intent(in) :: iband
+    real(8), intent(inout), target :: du(Ngal(1, iband) + 2, Ngal(2, iband))
+    call plane_transform(du, iband, 1)
+  end subroutine four_to_phys_du
+
+  ! FOU3D.f90:738 four_to_phys_u: the same for three fields
+  subroutine four_to_phys_u(u1, u2, u3, iband)
+    integer, intent(in) :: iband
+    real(8), intent(inout), target :: u1(Ngal(1, iband) + 2, Ngal(2, iband))
+    real(8), intent(inout), target :: u2(Ngal(1, iband) + 2, Ngal(2, iband))
+    real(8), intent(inout), target :: u3(Ngal(1, iband) + 2, Ngal(2, iband))
+    call plane_transform(u1, iband, 1)
+    call plane_transform(u2, iband, 1)
+    call plane_transform(u3, iband, 1)
+  end subroutine four_to_phys_u
+
+  ! FOU3D.f90:2253 phys_to_four_du: forward x and z transforms (1/n each),
+  ! oddball row zeroed; in place
+  subroutine phys_to_four_du(duPL, iband)
+    integer, intent(in) :: iband
+    real(8), intent(inout), target :: duPL(Ngal(1, iband) + 2, Ngal(2, iband))
+    call plane_transform(duPL, iband, -1)
+  end subroutine phys_to_four_du
+
+  subroutine plane_transform(pl, iband, dir)
+    integer, intent(in) :: iband, dir
+    real(8), intent(inout), target :: pl(Ngal(1, iband) + 2, Ngal(2, iband))
+    integer(c_size_t) :: bytes
+    type(c_ptr) :: d
+    bytes = int(size(pl), c_size_t) * 8_c_size_t
+    d = staging(bytes, 0_c_size_t)
+    call lh_check(lh_copy_to_device(d, c_loc(pl), bytes), 'lh_copy_to_device')
+    if (dir > 0) then
+      call lh_check(lh_four_to_phys_planes(int(N(1, iband), c_int), int(Ngal(1, iband), c_int), &
+                    int(Ngal(2, iband), c_int), d, 1_c_int, c_null_ptr), 'lh_four_to_phys_planes')
+    else
+      call lh_check(lh_phys_to_four_planes(int(N(1, iband), c_int), int(Ngal(1, iband), c_int), &
+                    int(Ngal(2, iband), c_int), d, 1_c_int, c_null_ptr), 'lh_phys_to_four_planes')
+    end if
+    call lh_check(lh_copy_to_host(c_loc(pl), d, bytes), 'lh_copy_to_host')
+  end subroutine plane_transform
+
+  ! ---------------------------------------------------------------------
+  ! tridLU_3D.f90:55 LUsolU_W: (I - beta (D2 + k^2)) u = rhsu and the same
+  ! operator for w, for substep kRK, all (kx,kz) columns of band iband.
+  ! Arrays in the reference layout u(0:ny, ncol).
+  subroutine LUsolU_W(u, rhsu, w, rhsw, iband)
+    integer, intent(in) :: iband
+    complex(8), intent(out), target :: u(0:nyb(iband), (N(1, iband) / 2 + 1) * N(2, iband))
+    complex(8), intent(in), target :: rhsu(0:nyb(iband), (N(1, iband) / 2 + 1) * N(2, iband))
+    complex(8), intent(out), target :: w(0:nyb(iband), (N(1, iband) / 2 + 1) * N(2, iband))
+    complex(8), intent(in), target :: rhsw(0:nyb(iband), (N(1, iband) / 2 + 1) * N(2, iband))
+    integer(c_size_t) :: b
+    type(c_ptr) :: d1, d2, d3, d4
+    b = int(size(u), c_size_t) * 16_c_size_t
+    d1 = staging(4 * b, 0_c_size_t)
+    d2 = staging(4 * b, b)
+    d3 = staging(4 * b, 2 * b)
+    d4 = staging(4 * b, 3 * b)
+    call lh_check(lh_copy_to_device(d1, c_loc(rhsu), b), 'lh_copy_to_device')
+    call lh_check(lh_copy_to_device(d2, c_loc(rhsw), b), 'lh_copy_to_device')
+    call lh_check(lh_lusol_uw(lhctx, int(iband - 1, c_int), int(kRK, c_int), d1, d2, d3, d4), 'lh_lusol_uw')
+    call lh_check(lh_copy_to_host(c_loc(u), d3, b), 'lh_copy_to_host')
+    call lh_check(lh_copy_to_host(c_loc(w), d4, b), 'lh_copy_to_host')
+  end subroutine LUsolU_W
+
+  ! tridLU_3D.f90:135 LUsolV: the v-grid operator, rows 0:ny-1 (walls v = 0)
+  subroutine LUsolV(u, rhsu, iband)
+    integer, intent(in) :: iband
+    complex(8), intent(out), target :: u(0:nyb(iband) - 1, (N(1, iband) / 2 + 1) * N(2, iband))
+    complex(8), intent(in), target :: rhsu(0:nyb(iband) - 1, (N(1, iband) / 2 + 1) * N(2, iband))
+    integer(c_size_t) :: b
+    type(c_ptr) :: d1, d2
+    b = int(size(u), c_size_t) * 16_c_size_t
+    d1 = staging(2 * b, 0_c_size_t)
+    d2 = staging(2 * b, b)
+    call lh_check(lh_copy_to_device(d1, c_loc(rhsu), b), 'lh_copy_to_device')
+    call lh_check(lh_lusol_v(lhctx, int(iband - 1, c_int), int(kRK, c_int), d1, d2), 'lh_lusol_v')
+    call lh_check(lh_copy_to_host(c_loc(u), d2, b), 'lh_copy_to_host')
+  end subroutine LUsolV
+
+  ! ---------------------------------------------------------------------
+  ! Field files in the format record_out writes and read_in reads
+  ! (FOU3D.f90:2856-2920, start.f90:2444-2882): unformatted sequential
+  ! records with 4-byte markers.  Record 1: t,Re,alp,bet,mpgx,nband,iter +
+  ! 88 int; record 2: N(4,0:nband+1); record 3: y, dthetavi, dthdy; then one
+  ! record per y row: j, grid, nx, nz, y(j), plane(nx,nz) with plane(2i+1,k),
+  ! plane(2i+2,k) the real/imaginary parts of x-mode i, z-index k.
+  ! Handled with stream access and explicit markers so that no
+  ! compiler-specific record convention is assumed.
+
+  ! read the file into f(jlo:jhi, ncol) (rows as present in the file)
+  subroutine read_in_field(fname, hdr, f, jlo, jhi)
+    character(*), intent(in) :: fname
+    type(field_header), intent(out) :: hdr
+    complex(8), allocatable, intent(out) :: f(:,:)
+    integer, intent(out) :: jlo, jhi
+    integer :: u, ios, m, m2, L, j, grid, nx, nz, nrec, q
+    integer(8) :: pos, fsize
+    real(8) :: yj
+    real(8), allocatable :: pl(:,:)
+    integer, allocatable :: js(:)
+    open(newunit=u, file=fname, access='stream', form='unformatted', status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      write(*, '(a,a)') 'read_in_field: cannot open ', trim(fname)
+      error stop 1
+    end if
+    inquire(unit=u, size=fsize)
+    pos = 1
+    read(u, pos=pos) m
+    read(u) hdr%t, hdr%Re, hdr%alp, hdr%bet, hdr%mpgx, hdr%nband, hdr%iter
+    pos = pos + 8 + m
+    read(u, pos=pos) m
+    allocate(hdr%Nb(4, 0:hdr%nband + 1))
+    read(u) hdr%Nb
+    pos = pos + 8 + m
+    read(u, pos=pos) m
+    L = (m / 8 - 1) / 2
+    allocate(hdr%y(L), hdr%dthdy(L))
+    read(u) hdr%y, hdr%dthetavi, hdr%dthdy
+    pos = pos + 8 + m
+    ! first pass: count rows and find j range
+    nrec = 0
+    jlo = huge(jlo); jhi = -huge(jhi)
+    do while (pos < fsize)
+      read(u, pos=pos) m
+      read(u) j, grid, nx, nz
+      nrec = nrec + 1
+      jlo = min(jlo, j); jhi = max(jhi, j)
+      pos = pos + 8 + m
+    end do
+    if (jhi - jlo + 1 /= nrec) then
+      write(*, '(a)') 'read_in_field: rows are not a contiguous range'
+      error stop 1
+    end if
+    allocate(f(jlo:jhi, (nx / 2) * nz), pl(nx, nz), js(nrec))
+    pos = 1
+    do q = 1, 3
+      read(u, pos=pos) m
+      pos = pos + 8 + m
+    end do
+    do q = 1, nrec
+      read(u, pos=pos) m
+      read(u) j, grid, nx, nz, yj, pl
+      read(u) m2
+      if (m2 /= m) then
+        write(*, '(a)') 'read_in_field: corrupt record marker'
+        error stop 1
+      end if
+      f(j, :) = reshape(cmplx(pl(1::2, :), pl(2::2, :), kind=8), [(nx / 2) * nz])
+      pos = pos + 8 + m
+    end do
+    close(u)
+  end subroutine read_in_field
+
+  subroutine record_out_field(fname, hdr, f, jlo, jhi, grid, nx, nz, yrow)
+    character(*), intent(in) :: fname
+    type(field_header), intent(in) :: hdr
+    integer, intent(in) :: jlo, jhi, grid, nx, nz
+    complex(8), intent(in) :: f(jlo:jhi, (nx / 2) * nz)
+    real(8), intent(in) :: yrow(jlo:jhi)
+    integer :: u, m, j
+    real(8) :: pl(nx, nz)
+    complex(8) :: c(nx / 2, nz)
+    integer, parameter :: zeros(88) = 0
+    open(newunit=u, file=fname, access='stream', form='unformatted', status='replace', action='write')
+    m = 5 * 8 + 2 * 4 + 88 * 4
+    write(u) m, hdr%t, hdr%Re, hdr%alp, hdr%bet, hdr%mpgx, hdr%nband, hdr%iter, zeros, m
+    m = 4 * size(hdr%Nb)
+    write(u) m, hdr%Nb, m
+    m = 8 * (2 * size(hdr%y) + 1)
+    write(u) m, hdr%y, hdr%dthetavi, hdr%dthdy, m
+    do j = jlo, jhi
+      c = reshape(f(j, :), [nx / 2, nz])
+      pl(1::2, :) = real(c, 8)
+      pl(2::2, :) = aimag(c)
+      m = 4 * 4 + 8 + 8 * nx * nz
+      write(u) m, j, grid, nx, nz, yrow(j), pl, m
+    end do
+    close(u)
+  end subroutine record_out_field
+
+end module lh_host

@@ -1,0 +1,199 @@
+! lhgpu.f90 -- ISO_C_BINDING interface to liblhgpu.so (include/lhgpu.h).
+!
+! The thin layer between the Fortran host and the MI355X kernels: one
+! interface block per C entry point, the lh_config record, the field and
+! error constants, and lh_check, which turns a non-zero status into
+! `error stop` with the library's message (the reference stops on errors,
+! e.g. littleharsh.f90:85).
+module lhgpu
+  use iso_c_binding
+  implicit none
+  private :: c_strlen
+
+  integer(c_int), parameter :: LH_OK = 0, LH_EINVAL = -1, LH_EHIP = -2, &
+                               LH_ENOMEM = -3, LH_ESTATE = -4
+  integer(c_int), parameter :: LH_PREC_FP64 = 0, LH_PREC_MIXED = 1
+  integer(c_int), parameter :: LH_U1 = 0, LH_U2 = 1, LH_U3 = 2, LH_P = 3, &
+                               LH_NU1 = 4, LH_NU2 = 5, LH_NU3 = 6, LH_R1 = 7, &
+                               LH_R2 = 8, LH_R3 = 9, LH_PSI = 10, LH_DIV = 11
+  integer(c_int), parameter :: LH_PHASE_NONLINEAR = 1, LH_PHASE_RHS = 2, &
+                               LH_PHASE_SOLVE = 3
+
+  type, bind(C) :: lh_config
+    integer(c_int) :: N1, N2, Ngal1, Ngal2, ny
+    real(c_double) :: alp, bet, Re
+    integer(c_int) :: grid_kind
+    real(c_double) :: dyq
+    integer(c_int) :: ppp
+    real(c_double) :: gamma, bslpu1, bslpu3
+    integer(c_int) :: flag_ctpress
+    real(c_double) :: mpgx, dt
+    integer(c_int) :: precision
+  end type lh_config
+
+  interface
+    function lh_last_error() bind(C, name='lh_last_error') result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+    function lh_ctx_create(cfgs, nchan, ctx) bind(C, name='lh_ctx_create') result(rc)
+      import :: lh_config, c_int, c_ptr
+      type(lh_config), intent(in) :: cfgs(*)
+      integer(c_int), value :: nchan
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: rc
+    end function
+    function lh_ctx_destroy(ctx) bind(C, name='lh_ctx_destroy') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function lh_set_grid(ctx, ch, yu, yv, dthdyu, dthdyv) bind(C, name='lh_set_grid') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch
+      real(c_double), intent(in) :: yu(*), yv(*), dthdyu(*), dthdyv(*)
+      integer(c_int) :: rc
+    end function
+    function lh_get_grid(ctx, ch, yu, yv, dthdyu, dthdyv) bind(C, name='lh_get_grid') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch
+      real(c_double), intent(out) :: yu(*), yv(*), dthdyu(*), dthdyv(*)
+      integer(c_int) :: rc
+    end function
+    function lh_field_upload(ctx, ch, field, reim) bind(C, name='lh_field_upload') result(rc)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch, field
+      complex(c_double_complex), intent(in) :: reim(*)
+      integer(c_int) :: rc
+    end function
+    function lh_field_download(ctx, ch, field, reim) bind(C, name='lh_field_download') result(rc)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch, field
+      complex(c_double_complex), intent(out) :: reim(*)
+      integer(c_int) :: rc
+    end function
+    function lh_get_scalars(ctx, ch, out8) bind(C, name='lh_get_scalars') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch
+      real(c_double), intent(out) :: out8(8)
+      integer(c_int) :: rc
+    end function
+    function lh_set_scalars(ctx, ch, t, mpgx, QxT, iter) bind(C, name='lh_set_scalars') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch, iter
+      real(c_double), value :: t, mpgx, QxT
+      integer(c_int) :: rc
+    end function
+    function lh_init_flowrate(ctx) bind(C, name='lh_init_flowrate') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function lh_substep(ctx, kRK) bind(C, name='lh_substep') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kRK
+      integer(c_int) :: rc
+    end function
+    function lh_step(ctx, nsteps) bind(C, name='lh_step') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+      integer(c_int) :: rc
+    end function
+    function lh_sync(ctx) bind(C, name='lh_sync') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function lh_run_phase(ctx, phase, kRK) bind(C, name='lh_run_phase') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: phase, kRK
+      integer(c_int) :: rc
+    end function
+    function lh_four_to_phys_planes(N1, Ngal1, Ngal2, d_planes, nplanes, stream) &
+        bind(C, name='lh_four_to_phys_planes') result(rc)
+      import :: c_ptr, c_int
+      integer(c_int), value :: N1, Ngal1, Ngal2, nplanes
+      type(c_ptr), value :: d_planes, stream
+      integer(c_int) :: rc
+    end function
+    function lh_phys_to_four_planes(N1, Ngal1, Ngal2, d_planes, nplanes, stream) &
+        bind(C, name='lh_phys_to_four_planes') result(rc)
+      import :: c_ptr, c_int
+      integer(c_int), value :: N1, Ngal1, Ngal2, nplanes
+      type(c_ptr), value :: d_planes, stream
+      integer(c_int) :: rc
+    end function
+    function lh_lusol_uw(ctx, ch, kRK, d_rhsu, d_rhsw, d_u, d_w) bind(C, name='lh_lusol_uw') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, d_rhsu, d_rhsw, d_u, d_w
+      integer(c_int), value :: ch, kRK
+      integer(c_int) :: rc
+    end function
+    function lh_lusol_v(ctx, ch, kRK, d_rhsv, d_v) bind(C, name='lh_lusol_v') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, d_rhsv, d_v
+      integer(c_int), value :: ch, kRK
+      integer(c_int) :: rc
+    end function
+    function lh_dev_alloc(bytes, p) bind(C, name='lh_dev_alloc') result(rc)
+      import :: c_ptr, c_int, c_size_t
+      integer(c_size_t), value :: bytes
+      type(c_ptr), intent(out) :: p
+      integer(c_int) :: rc
+    end function
+    function lh_dev_free(p) bind(C, name='lh_dev_free') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: p
+      integer(c_int) :: rc
+    end function
+    function lh_copy_to_device(d, h, bytes) bind(C, name='lh_copy_to_device') result(rc)
+      import :: c_ptr, c_int, c_size_t
+      type(c_ptr), value :: d, h
+      integer(c_size_t), value :: bytes
+      integer(c_int) :: rc
+    end function
+    function lh_copy_to_host(h, d, bytes) bind(C, name='lh_copy_to_host') result(rc)
+      import :: c_ptr, c_int, c_size_t
+      type(c_ptr), value :: h, d
+      integer(c_size_t), value :: bytes
+      integer(c_int) :: rc
+    end function
+    function c_strlen(s) bind(C, name='strlen') result(n)
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+      integer(c_size_t) :: n
+    end function
+  end interface
+
+contains
+
+  ! stop with the library's message when rc is not LH_OK
+  subroutine lh_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    character(len=:), allocatable :: text
+    type(c_ptr) :: p
+    integer :: n, q
+    if (rc == LH_OK) return
+    p = lh_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, msg, [n])
+    allocate(character(len=n) :: text)
+    do q = 1, n
+      text(q:q) = msg(q)
+    end do
+    write(*, '(a,a,a,i0,a,a)') 'liblhgpu: ', where, ' failed (', rc, '): ', text
+    error stop 1
+  end subroutine lh_check
+
+end module lhgpu
