@@ -128,6 +128,7 @@ def main():
         for name in ("u1", "u2", "u3", "p"):
             s.upload(q, name, f[name])
     s.init_flowrate()
+    tuning = s.autotune(10)
     s.step(args.warmup)
     s.sync()
     torch.cuda.synchronize()
@@ -167,7 +168,8 @@ def main():
                config=dict(workload=args.workload, channels=len(cfgs),
                            grids=sorted({"%dx%dx%d" % (c["N1"], c["N2"], c["ny"]) for c in cfgs}),
                            precision=args.precision, walls="smooth", dt=cfgs[0]["dt"],
-                           grid_points=gp, hbm_pct_of_peak_model=round(hbm_pct, 3)),
+                           grid_points=gp, hbm_pct_of_peak_model=round(hbm_pct, 3),
+                           tuning=tuning),
                roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(rec), flush=True)
     s.close()

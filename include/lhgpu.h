@@ -156,6 +156,19 @@ int lh_dev_free(void *p);
 int lh_copy_to_device(void *d, const void *h, size_t bytes);
 int lh_copy_to_host(void *h, const void *d, size_t bytes);
 
+/* Tuning knobs of the launch shapes (no counterpart in the reference).
+ * LH_TUNE_SOLVE_COLS: spectral columns per workgroup of the column solve;
+ * 0 selects one lane per column streaming rows from HBM, 1..10 the form that
+ * stages a workgroup's columns in LDS (must fit 160 KiB).  Changing a knob
+ * drops the captured step graph. */
+#define LH_TUNE_SOLVE_COLS 1
+int lh_set_tuning(lh_ctx *ctx, int key, int value);
+int lh_get_tuning(lh_ctx *ctx, int key, int *value);
+/* Autotuner: times each feasible setting of every knob on the context's own
+ * channels (`reps` launches each, on the current state, which is saved
+ * before and restored after) and keeps the fastest. */
+int lh_autotune(lh_ctx *ctx, int reps);
+
 #ifdef __cplusplus
 }
 #endif

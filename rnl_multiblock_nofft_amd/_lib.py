@@ -17,6 +17,7 @@ FIELDS = {"u1": LH_U1, "u2": LH_U2, "u3": LH_U3, "p": LH_P, "Nu1": LH_NU1,
           "Nu2": LH_NU2, "Nu3": LH_NU3, "r1": LH_R1, "r2": LH_R2, "r3": LH_R3,
           "psi": LH_PSI, "div": LH_DIV}
 PHASE_NONLINEAR, PHASE_RHS, PHASE_SOLVE = 1, 2, 3
+TUNE_SOLVE_COLS = 1
 PREC_FP64, PREC_MIXED = 0, 1
 
 # every symbol include/lhgpu.h declares
@@ -24,7 +25,8 @@ EXPORTS = ["lh_last_error", "lh_ctx_create", "lh_ctx_destroy", "lh_set_grid", "l
            "lh_field_upload", "lh_field_download", "lh_get_scalars", "lh_set_scalars",
            "lh_init_flowrate", "lh_substep", "lh_step", "lh_sync", "lh_time_kernels", "lh_run_phase",
            "lh_four_to_phys_planes", "lh_phys_to_four_planes", "lh_lusol_uw", "lh_lusol_v",
-           "lh_dev_alloc", "lh_dev_free", "lh_copy_to_device", "lh_copy_to_host"]
+           "lh_dev_alloc", "lh_dev_free", "lh_copy_to_device", "lh_copy_to_host",
+           "lh_set_tuning", "lh_get_tuning", "lh_autotune"]
 
 
 class LhConfig(ctypes.Structure):
@@ -77,6 +79,9 @@ def load():
     L.lh_dev_free.argtypes = [vp]
     L.lh_copy_to_device.argtypes = [vp, vp, ctypes.c_size_t]
     L.lh_copy_to_host.argtypes = [vp, vp, ctypes.c_size_t]
+    L.lh_set_tuning.argtypes = [vp, ip, ip]
+    L.lh_get_tuning.argtypes = [vp, ip, ctypes.POINTER(ip)]
+    L.lh_autotune.argtypes = [vp, ip]
     for name in EXPORTS:
         if name != "lh_last_error":
             getattr(L, name).restype = ctypes.c_int

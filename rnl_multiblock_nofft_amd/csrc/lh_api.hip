@@ -790,4 +790,84 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes) {
   return LH_OK;
 }
 
+// ------------------------------------------------------------------------
+// tuning
+static int solve_cols_ok(const lh_ctx *x, int C) {
+  if (C == 0) return 1;
+  if (C < 1 || C > 10) return 0;
+  return solve_lds_bytes(x->shape.nyu, C) <= 160 * 1024;
+}
+
+int lh_set_tuning(lh_ctx *x, int key, int value) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (key != LH_TUNE_SOLVE_COLS) return set_err(LH_EINVAL, "lh_set_tuning: unknown key %d", key);
+  if (!solve_cols_ok(x, value))
+    return set_err(LH_EINVAL, "lh_set_tuning: %d columns per workgroup do not fit (0, or 1..10 within 160 KiB of LDS)",
+                   value);
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->shape.solve_cols = value;
+  if (x->graph) {
+    (void)hipGraphExecDestroy(x->graph);
+    x->graph = nullptr;
+  }
+  return LH_OK;
+}
+
+int lh_get_tuning(lh_ctx *x, int key, int *value) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (!value) return set_err(LH_EINVAL, "lh_get_tuning: null value");
+  if (key != LH_TUNE_SOLVE_COLS) return set_err(LH_EINVAL, "lh_get_tuning: unknown key %d", key);
+  *value = x->shape.solve_cols;
+  return LH_OK;
+}
+
+int lh_autotune(lh_ctx *x, int reps) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (reps < 1) return set_err(LH_EINVAL, "lh_autotune: reps must be >= 1");
+  HIPCHK(hipStreamSynchronize(x->stream));
+  // the trials run real solves on the current state: keep a copy of the arena
+  char *save = nullptr;
+  if (hipMalloc(&save, x->arena_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_err(LH_ENOMEM, "lh_autotune: no room for a %zu-byte state copy", x->arena_bytes);
+  }
+  HIPCHK(hipMemcpyAsync(save, x->arena, x->arena_bytes, hipMemcpyDeviceToDevice, x->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int cand[] = {0, 1, 2, 3, 4, 5, 6, 8, 10};
+  int best = x->shape.solve_cols;
+  float best_ms = 1e30f;
+  const int keep = x->shape.solve_cols;
+  for (int C : cand) {
+    if (!solve_cols_ok(x, C)) continue;
+    x->shape.solve_cols = C;
+    hipError_t e = launch_solve(x->arena, x->d_chans, x->shape, 1, x->stream);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    HIPCHK(hipEventRecord(e0, x->stream));
+    for (int r = 0; r < reps; r++) HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, 1 + r % 3, x->stream));
+    HIPCHK(hipEventRecord(e1, x->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = C;
+    }
+  }
+  x->shape.solve_cols = keep;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipMemcpyAsync(x->arena, save, x->arena_bytes, hipMemcpyDeviceToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipFree(save));
+  return lh_set_tuning(x, LH_TUNE_SOLVE_COLS, best);
+}
+
 }  // extern "C"

@@ -10,7 +10,10 @@ struct ShapeMax {
   int nslot = 0, ncol = 0, nyu = 0;
   int zinv_tiles = 0, xprod_tiles = 0, zfwd_tiles = 0;
   size_t lds_zinv = 0, lds_xprod = 0, lds_zfwd = 0;  // in complex elements
+  int solve_cols = 0;  // columns per workgroup of the LDS column solve (0: one lane per column)
 };
+int solve_cols_default(int nyu);
+__host__ __device__ size_t solve_lds_bytes(int nyu, int C);
 
 void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny);
 hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s);

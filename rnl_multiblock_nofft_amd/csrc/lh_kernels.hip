@@ -719,6 +719,374 @@ __global__ void __launch_bounds__(64) k_solve(char *arena, const Chan *chans, in
 #undef LDR
 #undef STC
 
+// ------------------------------------------------------------------------
+// K5 (LDS form): column solves with every column of a workgroup staged in
+// LDS.  A workgroup of one wave owns C consecutive columns.  The phases:
+//   L  all lanes load the Helmholtz right-hand sides r1, r3 (u rows), r2
+//      (v rows) and the U, V, Poisson pivots d of this substep into LDS;
+//      no dependency between loads, so they are all in flight at once
+//   H  6 lanes per column (U1, U3, V) x (re, im) run forward and back
+//      substitution of the Helmholtz systems in place (tridLU_3D.f90:55-189
+//      with LU_dec's factors, tridLU_3D.f90:437-474)
+//   R  all lanes: divergence of the solved velocity and the Poisson right-
+//      hand side div/(dt (a+b)) (littleharsh.f90:237-272,581-606)
+//   P  2 lanes per column (re, im): Poisson elimination from the top row
+//      down and back substitution (LUsolP, tridLU_3D.f90:192-226, with the
+//      mean-mode pin, tridLU_3D.f90:384-390); the mean-mode lane also
+//      accumulates the flow rate and applies flowrate_corr
+//      (littleharsh.f90:476-509)
+//   U  all lanes: p += psi and v_corr's projection (littleharsh.f90:956-990)
+//   W  all lanes: divergence, ghost rows and wall rows (littleharsh.f90:
+//      992-1044), stores of u1, u2, u3, p, psi, div
+// Only the recurrences of H and P are serial; the upper diagonals c = a3 d
+// and Poisson multipliers e = a1 d are rebuilt from the pivots with the same
+// products the factor set-up uses, so they need no LDS.
+__host__ __device__ inline size_t solve_lds_col_bytes(int nyu) {
+  const int nyv = nyu - 1;
+  return 16 * (size_t)(3 * nyu + nyv) + 8 * (size_t)(2 * nyu + nyv) + 16;  // + kx, kz
+}
+// per-row coefficients shared by the columns of a workgroup
+constexpr int SOLVE_ROW_ARRAYS = 9;
+__host__ __device__ size_t solve_lds_bytes(int nyu, int C) {
+  return (size_t)C * solve_lds_col_bytes(nyu) + 8 * (size_t)SOLVE_ROW_ARRAYS * nyu;
+}
+#define SB2 8
+#define LB 8
+
+__global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans, int kRK, int C) {
+  const Chan ch = chans[blockIdx.z];
+  const View v(arena, ch);
+  const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu, nyv = ch.nyv, ncol = ch.ncol;
+  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs;
+  // give each XCD a contiguous run of column groups so that neighbouring
+  // groups, which share cache lines, share one L2.  The launch pads
+  // gridDim.x to a multiple of 8, so this is a permutation.
+  const int per = gridDim.x / 8;
+  const int grp = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int c0 = grp * C;
+  if (c0 >= ncol) return;
+  const int lane = threadIdx.x;
+  const size_t plane = (size_t)(nyu + PADROWS) * ldc;
+
+  extern __shared__ double2 smem_d[];
+  double2 *X1 = smem_d, *X3 = X1 + (size_t)nyu * C, *X2 = X3 + (size_t)nyu * C, *Ps = X2 + (size_t)nyv * C;
+  double *DU = reinterpret_cast<double *>(Ps + (size_t)nyu * C), *DV = DU + (size_t)nyu * C, *DP = DV + (size_t)nyv * C;
+  // row coefficients: Helmholtz sub/super-diagonals a1, a3 of U and V (wall
+  // rows included), Poisson a3 = du dthdyv and a1 = du dthdyv(j-1), du, dy
+  double *A1U = DP + (size_t)nyu * C, *A3U = A1U + nyu, *A1V = A3U + nyu, *A3V = A1V + nyu;
+  double *A3P = A3V + nyu, *A1P = A3P + nyu, *DUY = A1P + nyu, *DYV = DUY + nyu, *DVY = DYV + nyu;
+  double *KX = DVY + nyu, *KZ = KX + C;
+
+  const double dt = v.scal[SC_DT];
+  const double beta = rk_b(kRK) * dt / ch.Re;
+  const double dtirk = (1.0 / dt) / (rk_a(kRK) + rk_b(kRK));
+  const double dtrk = dt * (rk_a(kRK) + rk_b(kRK));
+  const double *fU = v.fac + (size_t)((kRK - 1) * 4) * plane, *fV = fU + 2 * plane, *fP = v.fac + (size_t)12 * plane;
+  const int ne = nyu - 2;
+  const int ncl = min(C, ncol - c0);  // live columns of this group
+
+  // ---- L: stage right-hand sides, pivots, row coefficients and the
+  //      columns' wavenumbers (dead columns read column ncol-1); loads are
+  //      issued LB iterations at a time so that they are all in flight
+  {
+    const int nU = nyu * C, nV = nyv * C;
+    for (int t0 = 0; t0 < nU; t0 += 64 * LB) {
+      double2 a[LB], b[LB];
+      double d[LB], e[LB];
+#pragma unroll
+      for (int u = 0; u < LB; u++) {
+        const int t = min(t0 + u * 64 + lane, nU - 1);
+        const int j = t / C, c = min(c0 + (t - j * C), ncol - 1);
+        const size_t o = (size_t)j * ldc + c;
+        a[u] = v.r1[o];
+        b[u] = v.r3[o];
+        d[u] = fU[o];
+        e[u] = fP[o];
+      }
+#pragma unroll
+      for (int u = 0; u < LB; u++) {
+        const int t = t0 + u * 64 + lane;
+        if (t < nU) {
+          X1[t] = a[u];
+          X3[t] = b[u];
+          DU[t] = d[u];
+          DP[t] = e[u];
+        }
+      }
+    }
+    for (int t0 = 0; t0 < nV; t0 += 64 * LB) {
+      double2 a[LB];
+      double d[LB];
+#pragma unroll
+      for (int u = 0; u < LB; u++) {
+        const int t = min(t0 + u * 64 + lane, nV - 1);
+        const int j = t / C, c = min(c0 + (t - j * C), ncol - 1);
+        const size_t o = (size_t)j * ldc + c;
+        a[u] = v.r2[o];
+        d[u] = fV[o];
+      }
+#pragma unroll
+      for (int u = 0; u < LB; u++) {
+        const int t = t0 + u * 64 + lane;
+        if (t < nV) {
+          X2[t] = a[u];
+          DV[t] = d[u];
+        }
+      }
+    }
+  }
+  for (int j0 = 0; j0 < nyu; j0 += 256) {
+    double c0u[4], c2u[4], c0v[4], c2v[4], hu[4], hv[4], hvm[4], yvj[4], yvm[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = min(j0 + u * 64 + lane, nyu - 1), jv = min(j, nyv - 1), jm = max(min(j - 1, nyv - 1), 0);
+      c0u[u] = v.dyu2i[3 * j];
+      c2u[u] = v.dyu2i[3 * j + 2];
+      c0v[u] = v.dyv2i[3 * jv];
+      c2v[u] = v.dyv2i[3 * jv + 2];
+      hu[u] = v.dthdyu[j];
+      hv[u] = v.dthdyv[jv];
+      hvm[u] = v.dthdyv[jm];
+      yvj[u] = v.yv[jv];
+      yvm[u] = v.yv[jm];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = j0 + u * 64 + lane;
+      if (j >= nyu) break;
+      const bool iu = j >= 1 && j <= nyu - 2, iv = j >= 1 && j <= nyv - 2;
+      A1U[j] = iu ? -beta * c0u[u] : ch.gw2;
+      A3U[j] = iu ? -beta * c2u[u] : 0.0;
+      A1V[j] = iv ? -beta * c0v[u] : 0.0;
+      A3V[j] = iv ? -beta * c2v[u] : 0.0;
+      A3P[j] = (j >= 1 && j < ne) ? hu[u] * hv[u] : 0.0;
+      A1P[j] = (j >= 2 && j <= ne) ? (-hu[u]) * (-hvm[u]) : 0.0;
+      DUY[j] = hu[u];
+      DYV[j] = (j >= 1 && j < nyv) ? yvj[u] - yvm[u] : 0.0;
+      DVY[j] = j < nyv ? hv[u] : 0.0;
+    }
+  }
+  if (lane < C) {
+    const int c = min(c0 + lane, ncol - 1);
+    KX[lane] = v.kx[c % nxh];
+    KZ[lane] = v.kz[c / nxh];
+  }
+  __syncthreads();
+
+  // ---- H: Helmholtz forward and back substitution, one real sequence per
+  //      lane, rows read SB2 ahead into registers
+  {
+    const int cl = lane % C, role = lane / C;
+    if (role < 6 && cl < ncl) {
+      const int sys = role >> 1, part = role & 1;
+      double *X = reinterpret_cast<double *>(sys == 0 ? X1 : (sys == 1 ? X3 : X2)) + 2 * cl + part;
+      const double *D = (sys == 2 ? DV : DU) + cl;
+      const double *A1 = sys == 2 ? A1V : A1U, *A3 = sys == 2 ? A3V : A3U;
+      const int n = sys == 2 ? nyv : nyu;
+      const int C2 = 2 * C;
+      double y = X[0];  // row 0: a(2,0) = 1
+      int j = 1;
+      for (; j + SB2 - 1 <= n - 1; j += SB2) {
+        double r[SB2], d[SB2], a[SB2];
+#pragma unroll
+        for (int q = 0; q < SB2; q++) {
+          r[q] = X[(j + q) * C2];
+          d[q] = D[(j + q) * C];
+          a[q] = A1[j + q];
+        }
+#pragma unroll
+        for (int q = 0; q < SB2; q++) {
+          y = (r[q] - a[q] * y) * d[q];
+          X[(j + q) * C2] = y;
+        }
+      }
+      for (; j <= n - 1; j++) {
+        y = (X[j * C2] - A1[j] * y) * D[j * C];
+        X[j * C2] = y;
+      }
+      // back substitution; c_j = a3_j d_j, and c_0 = gw1 (U) or 0 (V)
+      double x = y;
+      j = n - 2;
+      for (; j - SB2 + 1 >= 1; j -= SB2) {
+        double r[SB2], d[SB2], a[SB2];
+#pragma unroll
+        for (int q = 0; q < SB2; q++) {
+          r[q] = X[(j - q) * C2];
+          d[q] = D[(j - q) * C];
+          a[q] = A3[j - q];
+        }
+#pragma unroll
+        for (int q = 0; q < SB2; q++) {
+          x = r[q] - (a[q] * d[q]) * x;
+          X[(j - q) * C2] = x;
+        }
+      }
+      for (; j >= 1; j--) {
+        x = X[j * C2] - (A3[j] * D[j * C]) * x;
+        X[j * C2] = x;
+      }
+      X[0] = X[0] - (sys == 2 ? 0.0 : ch.gw1) * x;
+    }
+  }
+  __syncthreads();
+
+  // ---- R: Poisson right-hand side on pressure rows 1..ne
+  for (int t = lane; t < ne * C; t += 64) {
+    const int q = 1 + t / C, cl = t % C;
+    const double2 x1 = X1[q * C + cl], x3 = X3[q * C + cl], x2 = X2[q * C + cl], n2 = X2[(q - 1) * C + cl];
+    const double2 a = cimul(KX[cl], x1), b = cimul(KZ[cl], x3);
+    const double w = DUY[q];
+    const double2 dv = make_double2((a.x + b.x) + w * (x2.x - n2.x), (a.y + b.y) + w * (x2.y - n2.y));
+    Ps[q * C + cl] = make_double2(dtirk * dv.x, dtirk * dv.y);
+  }
+  __syncthreads();
+
+  // ---- P: flow rate of the mean mode (its workgroup sums over all lanes)
+  //      and flowrate_corr's g; Poisson elimination (top down) and back
+  //      substitution
+  __shared__ double g_sh;
+  if (c0 == 0) {
+    const double *x1r = reinterpret_cast<const double *>(X1);  // column 0, real part
+    double part = 0.0;
+    for (int r = 1 + lane; r <= ne; r += 64) part = part + DYV[r] * x1r[r * 2 * C];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
+    if (lane == 0) {
+      double g = 0.0;
+      if (ch.flag_ctpress == 0) {
+        g = (v.scal[SC_QXT] - part) / v.scal[SC_QX1 + kRK - 1];
+        v.scal[SC_MPGX] = v.scal[SC_MPGX] + g;
+      }
+      g_sh = g;
+    }
+  }
+  {
+    const int cl = lane % C, role = lane / C;
+    const bool col0 = c0 + cl == 0;
+    if (role < 2 && cl < ncl) {
+      double *Y = reinterpret_cast<double *>(Ps) + 2 * cl + role;
+      const double *D = DP + cl;
+      const int C2 = 2 * C;
+      double yp = 0.0;
+      int q = ne;
+      for (; q - SB2 + 1 >= 2; q -= SB2) {
+        double r[SB2], d[SB2], a[SB2];
+#pragma unroll
+        for (int u = 0; u < SB2; u++) {
+          r[u] = Y[(q - u) * C2];
+          d[u] = D[(q - u) * C];
+          a[u] = A3P[q - u];
+        }
+#pragma unroll
+        for (int u = 0; u < SB2; u++) {
+          yp = (r[u] - a[u] * yp) * d[u];
+          Y[(q - u) * C2] = yp;
+        }
+      }
+      for (; q >= 2; q--) {
+        yp = (Y[q * C2] - A3P[q] * yp) * D[q * C];
+        Y[q * C2] = yp;
+      }
+      {  // row ns = 1; the mean mode's row is pinned (a3 = 0, rhs = 0)
+        const double d = D[C];
+        const double yb = yp;
+        yp = col0 ? (0.0 - 0.0 * yb) * d : (Y[C2] - A3P[1] * yp) * d;
+        Y[C2] = yp;
+      }
+      double ps = yp;
+      int j = 2;
+      for (; j + SB2 - 1 <= ne; j += SB2) {
+        double r[SB2], d[SB2], a[SB2];
+#pragma unroll
+        for (int u = 0; u < SB2; u++) {
+          r[u] = Y[(j + u) * C2];
+          d[u] = D[(j + u) * C];
+          a[u] = A1P[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < SB2; u++) {
+          ps = r[u] - (a[u] * d[u]) * ps;
+          Y[(j + u) * C2] = ps;
+        }
+      }
+      for (; j <= ne; j++) {
+        ps = Y[j * C2] - (A1P[j] * D[j * C]) * ps;
+        Y[j * C2] = ps;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- U: p += psi; u1, u3 -= dt(a+b) grad_xz psi (+ flow-rate correction);
+  //      v -= dt(a+b) D_y psi
+  const double *__restrict__ u11 = v.u11 + (size_t)(kRK - 1) * nyu;
+  const bool has0 = c0 == 0;
+  const double g = has0 ? g_sh : 0.0;
+  for (int t0 = 0; t0 < ne * C; t0 += 64 * LB) {
+    double2 P[LB];
+#pragma unroll
+    for (int u = 0; u < LB; u++) {
+      const int t = min(t0 + u * 64 + lane, ne * C - 1);
+      const int q = 1 + t / C, c = min(c0 + t % C, ncol - 1);
+      P[u] = v.p[(size_t)q * ldc + c];
+    }
+#pragma unroll
+    for (int u = 0; u < LB; u++) {
+      const int t = t0 + u * 64 + lane;
+      if (t >= ne * C) break;
+      const int q = 1 + t / C, cl = t % C, c = c0 + cl;
+      const double2 ps = Ps[q * C + cl];
+      const double dkx = dtrk * KX[cl], dkz = dtrk * KZ[cl];
+      const double2 a = cimul(dkx, ps), b = cimul(dkz, ps);
+      const double2 U1 = X1[q * C + cl], U3 = X3[q * C + cl];
+      double2 n1 = make_double2(U1.x - a.x, U1.y - a.y);
+      const double2 n3 = make_double2(U3.x - b.x, U3.y - b.y);
+      if (c == 0) n1.x = n1.x + g * u11[q];
+      X1[q * C + cl] = n1;
+      X3[q * C + cl] = n3;
+      if (q <= ne - 1) {
+        const double2 pn = Ps[(q + 1) * C + cl], V = X2[q * C + cl];
+        const double sv = dtrk * DVY[q];
+        X2[q * C + cl] = make_double2(V.x - sv * (pn.x - ps.x), V.y - sv * (pn.y - ps.y));
+      }
+      if (c < ncol) {
+        const size_t o = (size_t)q * ldc + c;
+        v.p[o] = make_double2(P[u].x + ps.x, P[u].y + ps.y);
+        v.psi[o] = ps;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- W: divergence, ghost and wall rows, stores
+  for (int t = lane; t < nyu * C; t += 64) {
+    const int j = t / C, cl = t % C, c = c0 + cl;
+    if (c >= ncol) continue;
+    const size_t o = (size_t)j * ldc + c;
+    double2 n1, n3;
+    if (j == 0) {
+      n1 = cscale(ch.gwbc1, X1[1 * C + cl]);
+      n3 = cscale(ch.gwbc3, X3[1 * C + cl]);
+    } else if (j == nyu - 1) {
+      n1 = cscale(ch.gwbc1, X1[ne * C + cl]);
+      n3 = cscale(ch.gwbc3, X3[ne * C + cl]);
+    } else {
+      n1 = X1[j * C + cl];
+      n3 = X3[j * C + cl];
+      const double2 da = cimul(KX[cl], n1), db = cimul(KZ[cl], n3);
+      const double2 vq = j == ne ? make_double2(0, 0) : X2[j * C + cl];
+      const double2 vm = j == 1 ? make_double2(0, 0) : X2[(j - 1) * C + cl];
+      const double w = DUY[j];
+      v.div[o] = make_double2((da.x + db.x) + w * (vq.x - vm.x), (da.y + db.y) + w * (vq.y - vm.y));
+    }
+    v.u1[o] = n1;
+    v.u3[o] = n3;
+    if (j < nyv) v.u2[o] = (j == 0 || j == nyv - 1) ? make_double2(0, 0) : X2[j * C + cl];
+  }
+}
+#undef SB2
+#undef LB
+
 hipError_t launch_factors(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s) {
   dim3 g((m.ncol + 63) / 64, 1, m.nchan);
   hipLaunchKernelGGL(k_factors, g, dim3(64), 0, s, arena, d_chans);
@@ -821,6 +1189,7 @@ void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny) {
   mx(m.zfwd_tiles, (nxh + tile_zfwd(Ng2, nxh) - 1) / tile_zfwd(Ng2, nxh));
   mx(m.ncol, ncol);
   mx(m.nyu, ny + 1);
+  m.solve_cols = solve_cols_default(m.nyu);
   mxs(m.lds_zinv, lds_zinv(Ng2, nxh, 1));
   mxs(m.lds_xprod, lds_xprod(Ng1, Ng2, 1));
   mxs(m.lds_zfwd, lds_zfwd(Ng2, nxh, 1));
@@ -834,9 +1203,26 @@ hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int k
 }
 
 hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s) {
-  dim3 g((m.ncol + 63) / 64, 1, m.nchan);
-  hipLaunchKernelGGL(k_solve, g, dim3(64), 0, s, arena, d_chans, kRK);
+  const int C = m.solve_cols;
+  if (C <= 0) {
+    dim3 g((m.ncol + 63) / 64, 1, m.nchan);
+    hipLaunchKernelGGL(k_solve, g, dim3(64), 0, s, arena, d_chans, kRK);
+  } else {
+    const int ngrp = (m.ncol + C - 1) / C;
+    dim3 g((ngrp + 7) / 8 * 8, 1, m.nchan);
+    hipLaunchKernelGGL(k_solve_lds, g, dim3(64), solve_lds_bytes(m.nyu, C), s, arena, d_chans, kRK, C);
+  }
   return hipGetLastError();
+}
+
+// columns per workgroup of the LDS column solve: the most that keep two
+// workgroups' staging within 128 KiB of LDS, at most 10 (6 lanes each)
+int solve_cols_default(int nyu) {
+  size_t b = solve_lds_col_bytes(nyu);
+  int C = (int)((64 * 1024 - 8 * SOLVE_ROW_ARRAYS * (size_t)nyu) / b);
+  if (C > 10) C = 10;
+  if (C < 1 && 1 * b <= 160 * 1024) C = 1;
+  return C;
 }
 
 hipError_t launch_flowrate(char *arena, const Chan *d_chans, int nchan, int set_target, hipStream_t s) {

@@ -18,6 +18,7 @@ module lhgpu
                                LH_R2 = 8, LH_R3 = 9, LH_PSI = 10, LH_DIV = 11
   integer(c_int), parameter :: LH_PHASE_NONLINEAR = 1, LH_PHASE_RHS = 2, &
                                LH_PHASE_SOLVE = 3
+  integer(c_int), parameter :: LH_TUNE_SOLVE_COLS = 1
 
   type, bind(C) :: lh_config
     integer(c_int) :: N1, N2, Ngal1, Ngal2, ny
@@ -112,6 +113,13 @@ module lhgpu
       type(c_ptr), value :: ctx
       integer(c_int) :: rc
     end function
+    function lh_time_kernels(ctx, nsub, ms_out5) bind(C, name='lh_time_kernels') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsub
+      real(c_double), intent(out) :: ms_out5(5)
+      integer(c_int) :: rc
+    end function
     function lh_run_phase(ctx, phase, kRK) bind(C, name='lh_run_phase') result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx
@@ -165,6 +173,25 @@ module lhgpu
       import :: c_ptr, c_int, c_size_t
       type(c_ptr), value :: h, d
       integer(c_size_t), value :: bytes
+      integer(c_int) :: rc
+    end function
+    function lh_set_tuning(ctx, key, val) bind(C, name='lh_set_tuning') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: key, val
+      integer(c_int) :: rc
+    end function
+    function lh_get_tuning(ctx, key, val) bind(C, name='lh_get_tuning') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: key
+      integer(c_int), intent(out) :: val
+      integer(c_int) :: rc
+    end function
+    function lh_autotune(ctx, reps) bind(C, name='lh_autotune') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: reps
       integer(c_int) :: rc
     end function
     function c_strlen(s) bind(C, name='strlen') result(n)

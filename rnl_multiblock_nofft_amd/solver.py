@@ -124,5 +124,19 @@ class Solver:
         check(self.L.lh_time_kernels(self.h, int(nsub), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return ms
 
+    def set_tuning(self, key, value):
+        check(self.L.lh_set_tuning(self.h, int(key), int(value)))
+
+    def get_tuning(self, key):
+        v = ctypes.c_int()
+        check(self.L.lh_get_tuning(self.h, int(key), ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, reps=10):
+        """Time every feasible launch shape on this context's channels and
+        keep the fastest; the state is left unchanged."""
+        check(self.L.lh_autotune(self.h, int(reps)))
+        return {"solve_cols": self.get_tuning(_lib.TUNE_SOLVE_COLS)}
+
     def sync(self):
         check(self.L.lh_sync(self.h))

@@ -79,6 +79,11 @@ def test_phases_kRK(name):
             # predicted velocity is a near-cancellation, so mid-step p carries
             # ~10x the velocity round-off.  The one-step bar below stays 1e-12.
             assert e <= (5 * TOL if nm == "p" else TOL), (nm, e)
+        # psi is the small pressure correction (a near-cancellation): its
+        # error is measured against the size of p
+        e = np.linalg.norm(s.download(0, "psi") - o.get("psi")) / np.linalg.norm(o.get("p"))
+        say(name, kRK, "psi/|p|", e)
+        assert e <= 5 * TOL, ("psi", e)
         d = s.download(0, "div")
         assert np.abs(d).max() < 1e-10 * max(1.0, np.abs(s.download(0, "u1")).max())
         # mpgx accumulates g = (QxT - Qx0)/Qx1, a difference of two flow rates
@@ -104,6 +109,39 @@ def test_one_step_graph(name):
         e = relerr(s.download(0, nm), o.get(nm))
         say(name, "3 steps", nm, e)
         assert e <= 1e-11, (nm, e)
+
+
+@pytest.mark.parametrize("cols", [0, 1, 3, 10])
+def test_column_solve_layouts(cols):
+    """Both column-solve forms (one lane per column from HBM, or C columns
+    per workgroup staged in LDS) give the oracle's step."""
+    cfg = CASES["N16_ny33_poly_ctP"] if cols == 10 else CASES["N32_ny153"]
+    say("solve layout", cols)
+    s, o, f = make_pair(cfg, seed=7)
+    s.set_tuning(_lib.TUNE_SOLVE_COLS, cols)
+    assert s.get_tuning(_lib.TUNE_SOLVE_COLS) == cols
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("cols", cols, nm, e)
+        assert e <= TOL, (nm, e)
+
+
+def test_autotune_keeps_state():
+    cfg = CASES["N32_ny153"]
+    s, o, f = make_pair(cfg, seed=8)
+    before = {nm: s.download(0, nm) for nm in ("u1", "u2", "u3", "p")}
+    sc = s.scalars(0)
+    tuned = s.autotune(3)
+    say("autotune", tuned)
+    for nm, a in before.items():
+        assert np.array_equal(s.download(0, nm), a), nm
+    assert s.scalars(0)["mpgx"] == sc["mpgx"]
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        assert relerr(s.download(0, nm), o.get(nm)) <= TOL, nm
 
 
 def test_ensemble_uneven():
