@@ -159,9 +159,15 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
 /* Tuning knobs of the launch shapes (no counterpart in the reference).
  * LH_TUNE_SOLVE_COLS: spectral columns per workgroup of the column solve;
  * 0 selects one lane per column streaming rows from HBM, 1..10 the form that
- * stages a workgroup's columns in LDS (must fit 160 KiB).  Changing a knob
- * drops the captured step graph. */
+ * stages a workgroup's columns in LDS (must fit 160 KiB).
+ * LH_TUNE_{ZINV,XPROD,ZFWD}_POINTS: complex points per field that one
+ * workgroup of the z-inverse, x-product and z-forward stages transforms
+ * (whole sequences; the LDS they need must fit 160 KiB).
+ * Changing a knob drops the captured step graph. */
 #define LH_TUNE_SOLVE_COLS 1
+#define LH_TUNE_ZINV_POINTS 2
+#define LH_TUNE_XPROD_POINTS 3
+#define LH_TUNE_ZFWD_POINTS 4
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

@@ -2,6 +2,7 @@
 // reference (ygrid, def_k, grid weights), device allocation, field
 // transfers, the substep/step drivers (HIP-graph replayed) and the C-ABI
 // declared in include/lhgpu.h.
+#include <algorithm>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -798,19 +799,51 @@ static int solve_cols_ok(const lh_ctx *x, int C) {
   return solve_lds_bytes(x->shape.nyu, C) <= 160 * 1024;
 }
 
-int lh_set_tuning(lh_ctx *x, int key, int value) {
-  int rc = check_ctx(x);
-  if (rc) return rc;
-  if (key != LH_TUNE_SOLVE_COLS) return set_err(LH_EINVAL, "lh_set_tuning: unknown key %d", key);
-  if (!solve_cols_ok(x, value))
-    return set_err(LH_EINVAL, "lh_set_tuning: %d columns per workgroup do not fit (0, or 1..10 within 160 KiB of LDS)",
-                   value);
-  HIPCHK(hipStreamSynchronize(x->stream));
-  x->shape.solve_cols = value;
+static int *stage_knob(ShapeMax &m, int key) {
+  switch (key) {
+    case LH_TUNE_ZINV_POINTS: return &m.e_zinv;
+    case LH_TUNE_XPROD_POINTS: return &m.e_xprod;
+    case LH_TUNE_ZFWD_POINTS: return &m.e_zfwd;
+  }
+  return nullptr;
+}
+
+// LDS bytes of the largest nonlinear stage with the shape's targets
+static size_t stage_lds_bytes(const lh_ctx *x, const ShapeMax &m) {
+  size_t es = x->mixed ? sizeof(float2) : sizeof(double2);
+  return std::max(m.lds_zinv, std::max(m.lds_xprod, m.lds_zfwd)) * es;
+}
+
+static void drop_graph(lh_ctx *x) {
   if (x->graph) {
     (void)hipGraphExecDestroy(x->graph);
     x->graph = nullptr;
   }
+}
+
+int lh_set_tuning(lh_ctx *x, int key, int value) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (key == LH_TUNE_SOLVE_COLS) {
+    if (!solve_cols_ok(x, value))
+      return set_err(LH_EINVAL,
+                     "lh_set_tuning: %d columns per workgroup do not fit (0, or 1..10 within 160 KiB of LDS)", value);
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->shape.solve_cols = value;
+    drop_graph(x);
+    return LH_OK;
+  }
+  ShapeMax m = x->shape;
+  int *k = stage_knob(m, key);
+  if (!k) return set_err(LH_EINVAL, "lh_set_tuning: unknown key %d", key);
+  if (value < 1) return set_err(LH_EINVAL, "lh_set_tuning: points per workgroup must be positive");
+  *k = value;
+  shape_tiles(m);
+  if (stage_lds_bytes(x, m) > 160 * 1024)
+    return set_err(LH_EINVAL, "lh_set_tuning: %d points per workgroup need more than 160 KiB of LDS", value);
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->shape = m;
+  drop_graph(x);
   return LH_OK;
 }
 
@@ -818,17 +851,45 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
   int rc = check_ctx(x);
   if (rc) return rc;
   if (!value) return set_err(LH_EINVAL, "lh_get_tuning: null value");
-  if (key != LH_TUNE_SOLVE_COLS) return set_err(LH_EINVAL, "lh_get_tuning: unknown key %d", key);
-  *value = x->shape.solve_cols;
+  if (key == LH_TUNE_SOLVE_COLS) {
+    *value = x->shape.solve_cols;
+    return LH_OK;
+  }
+  int *k = stage_knob(x->shape, key);
+  if (!k) return set_err(LH_EINVAL, "lh_get_tuning: unknown key %d", key);
+  *value = *k;
   return LH_OK;
 }
+
+}  // extern "C"
+
+// time `reps` launches of what `launch` issues; returns ms or a negative
+// value when the launch is rejected
+template <typename F> static float time_launches(lh_ctx *x, hipEvent_t e0, hipEvent_t e1, int reps, F launch) {
+  if (launch(0) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1.f;
+  }
+  if (hipEventRecord(e0, x->stream) != hipSuccess) return -1.f;
+  for (int r = 0; r < reps; r++)
+    if (launch(r) != hipSuccess) {
+      (void)hipGetLastError();
+      return -1.f;
+    }
+  if (hipEventRecord(e1, x->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.f;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.f;
+  return ms;
+}
+
+extern "C" {
 
 int lh_autotune(lh_ctx *x, int reps) {
   int rc = check_ctx(x);
   if (rc) return rc;
   if (reps < 1) return set_err(LH_EINVAL, "lh_autotune: reps must be >= 1");
   HIPCHK(hipStreamSynchronize(x->stream));
-  // the trials run real solves on the current state: keep a copy of the arena
+  // the trials run real kernels on the current state: keep a copy of the arena
   char *save = nullptr;
   if (hipMalloc(&save, x->arena_bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -838,36 +899,50 @@ int lh_autotune(lh_ctx *x, int reps) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int cand[] = {0, 1, 2, 3, 4, 5, 6, 8, 10};
-  int best = x->shape.solve_cols;
-  float best_ms = 1e30f;
-  const int keep = x->shape.solve_cols;
-  for (int C : cand) {
-    if (!solve_cols_ok(x, C)) continue;
-    x->shape.solve_cols = C;
-    hipError_t e = launch_solve(x->arena, x->d_chans, x->shape, 1, x->stream);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      continue;
+  ShapeMax best = x->shape;
+  // nonlinear stages, in pipeline order (each reads what the previous wrote)
+  const int pts[] = {256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096};
+  const int keys[3] = {LH_TUNE_ZINV_POINTS, LH_TUNE_XPROD_POINTS, LH_TUNE_ZFWD_POINTS};
+  for (int st = 0; st < 3; st++) {
+    float best_ms = 1e30f;
+    for (int E : pts) {
+      ShapeMax m = best;
+      *stage_knob(m, keys[st]) = E;
+      shape_tiles(m);
+      if (stage_lds_bytes(x, m) > 160 * 1024) continue;
+      float ms = time_launches(x, e0, e1, reps,
+                               [&](int) { return launch_nl_stage(x->arena, x->d_chans, m, x->mixed, st, x->stream); });
+      if (ms >= 0.f && ms < best_ms) {
+        best_ms = ms;
+        *stage_knob(best, keys[st]) = E;
+      }
     }
-    HIPCHK(hipEventRecord(e0, x->stream));
-    for (int r = 0; r < reps; r++) HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, 1 + r % 3, x->stream));
-    HIPCHK(hipEventRecord(e1, x->stream));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = C;
+    shape_tiles(best);
+  }
+  // column solve: one lane per column, or C columns per workgroup in LDS
+  {
+    const int cand[] = {0, 1, 2, 3, 4, 5, 6, 8, 10};
+    float best_ms = 1e30f;
+    for (int C : cand) {
+      if (!solve_cols_ok(x, C)) continue;
+      ShapeMax m = best;
+      m.solve_cols = C;
+      float ms = time_launches(x, e0, e1, reps,
+                               [&](int r) { return launch_solve(x->arena, x->d_chans, m, 1 + r % 3, x->stream); });
+      if (ms >= 0.f && ms < best_ms) {
+        best_ms = ms;
+        best.solve_cols = C;
+      }
     }
   }
-  x->shape.solve_cols = keep;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIPCHK(hipMemcpyAsync(x->arena, save, x->arena_bytes, hipMemcpyDeviceToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(hipFree(save));
-  return lh_set_tuning(x, LH_TUNE_SOLVE_COLS, best);
+  x->shape = best;
+  drop_graph(x);
+  return LH_OK;
 }
 
 }  // extern "C"

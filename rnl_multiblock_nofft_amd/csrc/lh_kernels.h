@@ -1,17 +1,26 @@
 // lh_kernels.h -- host-side declarations of the kernel launchers.
 #pragma once
+#include <vector>
+
 #include "lh_common.h"
 
 namespace lh {
 
+struct ChanDims {
+  int N1, N2, Ng1, Ng2, ny;
+};
 // launch geometry covering every channel of a context
 struct ShapeMax {
   int nchan = 0;
   int nslot = 0, ncol = 0, nyu = 0;
+  std::vector<ChanDims> dims;
+  // complex points per field that a workgroup of each nonlinear stage takes
+  int e_zinv = 1024, e_xprod = 512, e_zfwd = 768;
   int zinv_tiles = 0, xprod_tiles = 0, zfwd_tiles = 0;
   size_t lds_zinv = 0, lds_xprod = 0, lds_zfwd = 0;  // in complex elements
   int solve_cols = 0;  // columns per workgroup of the LDS column solve (0: one lane per column)
 };
+void shape_tiles(ShapeMax &m);
 int solve_cols_default(int nyu);
 __host__ __device__ size_t solve_lds_bytes(int nyu, int C);
 

@@ -22,6 +22,8 @@
 //
 // The nonlinear stages run in FP64 (C = double2) or, in mixed mode, FP32
 // (C = float2) with FP64 twiddles; k_rhs and k_solve are always FP64.
+#include <algorithm>
+
 #include "lh_common.h"
 #include "lh_fft.h"
 #include "lh_kernels.h"
@@ -29,21 +31,12 @@
 namespace lh {
 
 // ------------------------------------------------------------------------
-// tile sizes (shared with the host so LDS sizes agree)
-__host__ __device__ int tile_zinv(int Ng2, int nxh) {
-  int t = 2048 / Ng2;
+// tile sizes (shared with the host so LDS sizes agree): a workgroup takes
+// about E complex points per field, whole sequences only
+__host__ __device__ inline int tile_of(int E, int n, int cap) {
+  int t = E / n;
   if (t < 1) t = 1;
-  return t < nxh ? t : nxh;
-}
-__host__ __device__ int tile_xprod(int Ng1, int Ng2) {
-  int t = 512 / Ng1;
-  if (t < 1) t = 1;
-  return t < Ng2 ? t : Ng2;
-}
-__host__ __device__ int tile_zfwd(int Ng2, int nxh) {
-  int t = 512 / Ng2;
-  if (t < 1) t = 1;
-  return t < nxh ? t : nxh;
+  return t < cap ? t : cap;
 }
 
 template <typename C> struct Inter;
@@ -61,25 +54,33 @@ template <typename C> __device__ __forceinline__ double2 to_d(C v) { return make
 // ------------------------------------------------------------------------
 // K1: interpolation + inverse z transform
 template <typename C>
-__global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans) {
+__global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E) {
   const Chan ch = chans[blockIdx.z / 3];
   const View v(arena, ch);
   const int f = blockIdx.z % 3;
   const int slot = blockIdx.y;
   if (slot >= ch.nslot) return;
   const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc;
-  const int TI = tile_zinv(Ng2, nxh);
+  const int TI = tile_of(E, Ng2, nxh);
   const int i0 = blockIdx.x * TI;
   if (i0 >= nxh) return;
   const int ni = min(TI, nxh - i0);
   extern __shared__ double2 smem_d[];
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + TI * Ng2;
+  C *TW = B + TI * Ng2;
   const bool isU = slot < ch.nplu;
   const int j = isU ? slot + 1 : slot - ch.nplu + 1;
 
-  for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) A[t] = CT<C>::mk(0, 0);
-  __syncthreads();
+  stage_twiddles<C>(TW, v.twz, Ng2);
+  // only the zero-padding rows z = N2/2 .. Ngal2-N2/2-1 need clearing
+  {
+    const int half0 = ch.N2 / 2, npad = Ng2 - ch.N2;
+    for (int t = threadIdx.x; t < ni * npad; t += blockDim.x) {
+      int ii = t / npad, z = half0 + (t - ii * npad);
+      A[ii * Ng2 + z] = CT<C>::mk(0, 0);
+    }
+  }
   // interpolation weights of this plane
   double wa = 0, wb = 0, wd = 1;
   const double2 *s0 = nullptr, *s1 = nullptr;
@@ -118,7 +119,7 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans) {
     A[ii * Ng2 + z] = from_d<C>(v);
   }
   __syncthreads();
-  C *R = fft_lds<C>(A, B, Ng2, ni, Ng2, +1, v.twz);
+  C *R = fft_lds<C>(A, B, Ng2, ni, Ng2, +1, TW);
   C *hz = Inter<C>::hz(v) + ((size_t)slot * 3 + f) * Ng2 * nxh;
   for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) {
     int z = t / ni, ii = t - z * ni;
@@ -129,20 +130,22 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans) {
 // ------------------------------------------------------------------------
 // K2: inverse x transform, products, forward x transform
 template <typename C>
-__global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans) {
+__global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, int E) {
   typedef typename CT<C>::R R_t;
   const Chan ch = chans[blockIdx.z];
   const View v(arena, ch);
   const int slot = blockIdx.y;
   if (slot >= ch.nslot) return;
   const int Ng1 = ch.Ng1, Ng2 = ch.Ng2, nxh = ch.nxh, h1 = ch.N1 / 2;
-  const int TZ = tile_xprod(Ng1, Ng2);
+  const int TZ = tile_of(E, Ng1, Ng2);
   const int z0 = blockIdx.x * TZ;
   if (z0 >= Ng2) return;
   const int nz = min(TZ, Ng2 - z0);
   extern __shared__ double2 smem_d[];
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + 2 * TZ * Ng1;
+  C *TW = B + 2 * TZ * Ng1;
+  stage_twiddles<C>(TW, v.twx, Ng1);
   const bool isU = slot < ch.nplu;
   const C *hz = Inter<C>::hz(v) + (size_t)slot * 3 * Ng2 * nxh;
 
@@ -174,7 +177,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans) {
     A[(2 * zz + 1) * Ng1 + m] = F1;
   }
   __syncthreads();
-  C *X = fft_lds<C>(A, B, Ng1, 2 * nz, Ng1, +1, v.twx);
+  C *X = fft_lds<C>(A, B, Ng1, 2 * nz, Ng1, +1, TW);
   for (int t = threadIdx.x; t < nz * Ng1; t += blockDim.x) {
     int zz = t / Ng1, n = t - zz * Ng1;
     C s0 = X[(2 * zz) * Ng1 + n], s1 = X[(2 * zz + 1) * Ng1 + n];
@@ -188,7 +191,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans) {
     }
   }
   __syncthreads();
-  C *Y = fft_lds<C>(X, X == A ? B : A, Ng1, 2 * nz, Ng1, -1, v.twx);
+  C *Y = fft_lds<C>(X, X == A ? B : A, Ng1, 2 * nz, Ng1, -1, TW);
   const int nseq = isU ? 2 : 1;
   const R_t inv = (R_t)(1.0 / Ng1);
   C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxh;
@@ -216,13 +219,13 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans) {
 // ------------------------------------------------------------------------
 // K3: forward z transform, truncation, d/dx + d/dz
 template <typename C>
-__global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans) {
+__global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, int E) {
   const Chan ch = chans[blockIdx.z];
   const View v(arena, ch);
   const int slot = blockIdx.y;
   if (slot >= ch.nslot) return;
   const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc;
-  const int TI = tile_zfwd(Ng2, nxh);
+  const int TI = tile_of(E, Ng2, nxh);
   const int i0 = blockIdx.x * TI;
   if (i0 >= nxh) return;
   const int ni = min(TI, nxh - i0);
@@ -232,6 +235,8 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans) {
   extern __shared__ double2 smem_d[];
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + 4 * TI * Ng2;
+  C *TW = B + 4 * TI * Ng2;
+  stage_twiddles<C>(TW, v.twz, Ng2);
   const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxh;
   for (int t = threadIdx.x; t < np * ni * Ng2; t += blockDim.x) {
     int p = t / (ni * Ng2);
@@ -240,7 +245,7 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans) {
     A[(p * ni + ii) * Ng2 + z] = gx[((size_t)p * Ng2 + z) * nxh + i0 + ii];
   }
   __syncthreads();
-  C *R = fft_lds<C>(A, B, Ng2, np * ni, Ng2, -1, v.twz);
+  C *R = fft_lds<C>(A, B, Ng2, np * ni, Ng2, -1, TW);
   const double inv = 1.0 / Ng2;
   const int half = N2 / 2, dz = Ng2 - N2;
   for (int t = threadIdx.x; t < ni * N2; t += blockDim.x) {
@@ -1141,9 +1146,28 @@ __global__ void k_from_rows(const double2 *__restrict__ src, double2 *__restrict
 
 // ------------------------------------------------------------------------
 // launch wrappers
-static size_t lds_zinv(int Ng2, int nxh, size_t es) { return 2 * (size_t)tile_zinv(Ng2, nxh) * Ng2 * es; }
-static size_t lds_xprod(int Ng1, int Ng2, size_t es) { return 2 * 2 * (size_t)tile_xprod(Ng1, Ng2) * Ng1 * es; }
-static size_t lds_zfwd(int Ng2, int nxh, size_t es) { return 2 * 4 * (size_t)tile_zfwd(Ng2, nxh) * Ng2 * es; }
+// LDS of the three stages, in complex elements: two ping-pong buffers and
+// the twiddle table
+static size_t lds_zinv(int E, int Ng2, int nxh) { return 2 * (size_t)tile_of(E, Ng2, nxh) * Ng2 + Ng2; }
+static size_t lds_xprod(int E, int Ng1, int Ng2) { return 2 * 2 * (size_t)tile_of(E, Ng1, Ng2) * Ng1 + Ng1; }
+static size_t lds_zfwd(int E, int Ng2, int nxh) { return 2 * 4 * (size_t)tile_of(E, Ng2, nxh) * Ng2 + Ng2; }
+
+// tile counts and LDS sizes for the current targets, over every channel
+void shape_tiles(ShapeMax &m) {
+  m.zinv_tiles = m.xprod_tiles = m.zfwd_tiles = 0;
+  m.lds_zinv = m.lds_xprod = m.lds_zfwd = 0;
+  for (const ChanDims &d : m.dims) {
+    const int nxh = d.N1 / 2 + 1;
+    const int ti = tile_of(m.e_zinv, d.Ng2, nxh), tx = tile_of(m.e_xprod, d.Ng1, d.Ng2),
+              tf = tile_of(m.e_zfwd, d.Ng2, nxh);
+    m.zinv_tiles = max(m.zinv_tiles, (nxh + ti - 1) / ti);
+    m.xprod_tiles = max(m.xprod_tiles, (d.Ng2 + tx - 1) / tx);
+    m.zfwd_tiles = max(m.zfwd_tiles, (nxh + tf - 1) / tf);
+    m.lds_zinv = std::max(m.lds_zinv, lds_zinv(m.e_zinv, d.Ng2, nxh));
+    m.lds_xprod = std::max(m.lds_xprod, lds_xprod(m.e_xprod, d.Ng1, d.Ng2));
+    m.lds_zfwd = std::max(m.lds_zfwd, lds_zfwd(m.e_zfwd, d.Ng2, nxh));
+  }
+}
 
 hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s) {
   size_t es = mixed ? sizeof(float2) : sizeof(double2);
@@ -1151,21 +1175,21 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
   if (stage == 0) {
     dim3 g(m.zinv_tiles, m.nslot, m.nchan * 3);
     if (mixed)
-      hipLaunchKernelGGL(k_zinv<float2>, g, b, m.lds_zinv * es, s, arena, d_chans);
+      hipLaunchKernelGGL(k_zinv<float2>, g, b, m.lds_zinv * es, s, arena, d_chans, m.e_zinv);
     else
-      hipLaunchKernelGGL(k_zinv<double2>, g, b, m.lds_zinv * es, s, arena, d_chans);
+      hipLaunchKernelGGL(k_zinv<double2>, g, b, m.lds_zinv * es, s, arena, d_chans, m.e_zinv);
   } else if (stage == 1) {
     dim3 g(m.xprod_tiles, m.nslot, m.nchan);
     if (mixed)
-      hipLaunchKernelGGL(k_xprod<float2>, g, b, m.lds_xprod * es, s, arena, d_chans);
+      hipLaunchKernelGGL(k_xprod<float2>, g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
     else
-      hipLaunchKernelGGL(k_xprod<double2>, g, b, m.lds_xprod * es, s, arena, d_chans);
+      hipLaunchKernelGGL(k_xprod<double2>, g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
   } else {
     dim3 g(m.zfwd_tiles, m.nslot, m.nchan);
     if (mixed)
-      hipLaunchKernelGGL(k_zfwd<float2>, g, b, m.lds_zfwd * es, s, arena, d_chans);
+      hipLaunchKernelGGL(k_zfwd<float2>, g, b, m.lds_zfwd * es, s, arena, d_chans, m.e_zfwd);
     else
-      hipLaunchKernelGGL(k_zfwd<double2>, g, b, m.lds_zfwd * es, s, arena, d_chans);
+      hipLaunchKernelGGL(k_zfwd<double2>, g, b, m.lds_zfwd * es, s, arena, d_chans, m.e_zfwd);
   }
   return hipGetLastError();
 }
@@ -1181,18 +1205,12 @@ hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m,
 void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny) {
   int nxh = N1 / 2 + 1, ncol = nxh * N2;
   int nslot = (ny - 1) + (ny - 2);
-  auto mx = [](int &a, int b) { if (b > a) a = b; };
-  auto mxs = [](size_t &a, size_t b) { if (b > a) a = b; };
-  mx(m.nslot, nslot);
-  mx(m.zinv_tiles, (nxh + tile_zinv(Ng2, nxh) - 1) / tile_zinv(Ng2, nxh));
-  mx(m.xprod_tiles, (Ng2 + tile_xprod(Ng1, Ng2) - 1) / tile_xprod(Ng1, Ng2));
-  mx(m.zfwd_tiles, (nxh + tile_zfwd(Ng2, nxh) - 1) / tile_zfwd(Ng2, nxh));
-  mx(m.ncol, ncol);
-  mx(m.nyu, ny + 1);
+  m.nslot = max(m.nslot, nslot);
+  m.ncol = max(m.ncol, ncol);
+  m.nyu = max(m.nyu, ny + 1);
   m.solve_cols = solve_cols_default(m.nyu);
-  mxs(m.lds_zinv, lds_zinv(Ng2, nxh, 1));
-  mxs(m.lds_xprod, lds_xprod(Ng1, Ng2, 1));
-  mxs(m.lds_zfwd, lds_zfwd(Ng2, nxh, 1));
+  m.dims.push_back(ChanDims{N1, N2, Ng1, Ng2, ny});
+  shape_tiles(m);
   m.nchan++;
 }
 

@@ -27,7 +27,8 @@ __global__ void __launch_bounds__(256) k_plane_z(double *planes, int N1, int Ng1
   extern __shared__ double2 smem_d[];
   if (i0 < nv) {
     const int ni = min(TI, nv - i0);
-    double2 *A = smem_d, *B = smem_d + TI * Ng2;
+    double2 *A = smem_d, *B = smem_d + TI * Ng2, *TW = B + TI * Ng2;
+    stage_twiddles<double2>(TW, twz, Ng2);
     for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) {
       int z = t / ni, ii = t - z * ni;
       double2 v = make_double2(P[(size_t)z * igal + 2 * (i0 + ii)], P[(size_t)z * igal + 2 * (i0 + ii) + 1]);
@@ -35,7 +36,7 @@ __global__ void __launch_bounds__(256) k_plane_z(double *planes, int N1, int Ng1
       A[ii * Ng2 + z] = v;
     }
     __syncthreads();
-    double2 *R = fft_lds<double2>(A, B, Ng2, ni, Ng2, dir, twz);
+    double2 *R = fft_lds<double2>(A, B, Ng2, ni, Ng2, dir, TW);
     const double s = dir < 0 ? 1.0 / Ng2 : 1.0;
     for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) {
       int z = t / ni, ii = t - z * ni;
@@ -64,7 +65,8 @@ __global__ void __launch_bounds__(256) k_plane_x(double *planes, int Ng1, int Ng
   const int nzr = min(2 * TZ, Ng2 - z0);  // rows in this block
   const int npair = (nzr + 1) / 2;
   extern __shared__ double2 smem_d[];
-  double2 *A = smem_d, *B = smem_d + TZ * Ng1;
+  double2 *A = smem_d, *B = smem_d + TZ * Ng1, *TW = B + TZ * Ng1;
+  stage_twiddles<double2>(TW, twx, Ng1);
   if (dir > 0) {
     for (int t = threadIdx.x; t < npair * Ng1; t += blockDim.x) {
       int q = t / Ng1, m = t - q * Ng1;
@@ -83,7 +85,7 @@ __global__ void __launch_bounds__(256) k_plane_x(double *planes, int Ng1, int Ng
       A[q * Ng1 + m] = make_double2(a.x - b.y, a.y + b.x);
     }
     __syncthreads();
-    double2 *R = fft_lds<double2>(A, B, Ng1, npair, Ng1, +1, twx);
+    double2 *R = fft_lds<double2>(A, B, Ng1, npair, Ng1, +1, TW);
     // keep the packed cc(n/2) entries where the reference leaves them
     for (int t = threadIdx.x; t < npair * Ng1; t += blockDim.x) {
       int q = t / Ng1, n = t - q * Ng1;
@@ -101,7 +103,7 @@ __global__ void __launch_bounds__(256) k_plane_x(double *planes, int Ng1, int Ng
       A[q * Ng1 + n] = make_double2(a, b);
     }
     __syncthreads();
-    double2 *R = fft_lds<double2>(A, B, Ng1, npair, Ng1, -1, twx);
+    double2 *R = fft_lds<double2>(A, B, Ng1, npair, Ng1, -1, TW);
     const double inv = 1.0 / Ng1;
     for (int t = threadIdx.x; t < npair * (h + 1); t += blockDim.x) {
       int q = t / (h + 1), m = t - q * (h + 1);
@@ -129,9 +131,9 @@ hipError_t launch_plane_f2p(int N1, int Ng1, int Ng2, double *planes, int nplane
                             const double2 *twx, hipStream_t s) {
   int nv = N1 / 2 + 1, TI = plane_tile(Ng2), TZ = plane_tile(Ng1);
   dim3 gz((nv + TI - 1) / TI, nplanes), gx((Ng2 + 2 * TZ - 1) / (2 * TZ), nplanes);
-  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), 2 * TI * Ng2 * sizeof(double2), s, planes, N1, Ng1, Ng2, +1,
+  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, N1, Ng1, Ng2, +1,
                      twz);
-  hipLaunchKernelGGL(k_plane_x, gx, dim3(256), 2 * TZ * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, +1, twx);
+  hipLaunchKernelGGL(k_plane_x, gx, dim3(256), (2 * TZ + 1) * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, +1, twx);
   return hipGetLastError();
 }
 
@@ -139,8 +141,8 @@ hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplane
                             const double2 *twx, hipStream_t s) {
   int nv = N1 / 2 + 1, TI = plane_tile(Ng2), TZ = plane_tile(Ng1);
   dim3 gz((nv + TI - 1) / TI, nplanes), gx((Ng2 + 2 * TZ - 1) / (2 * TZ), nplanes);
-  hipLaunchKernelGGL(k_plane_x, gx, dim3(256), 2 * TZ * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, -1, twx);
-  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), 2 * TI * Ng2 * sizeof(double2), s, planes, N1, Ng1, Ng2, -1,
+  hipLaunchKernelGGL(k_plane_x, gx, dim3(256), (2 * TZ + 1) * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, -1, twx);
+  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, N1, Ng1, Ng2, -1,
                      twz);
   return hipGetLastError();
 }

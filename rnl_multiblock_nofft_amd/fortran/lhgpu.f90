@@ -18,7 +18,8 @@ module lhgpu
                                LH_R2 = 8, LH_R3 = 9, LH_PSI = 10, LH_DIV = 11
   integer(c_int), parameter :: LH_PHASE_NONLINEAR = 1, LH_PHASE_RHS = 2, &
                                LH_PHASE_SOLVE = 3
-  integer(c_int), parameter :: LH_TUNE_SOLVE_COLS = 1
+  integer(c_int), parameter :: LH_TUNE_SOLVE_COLS = 1, LH_TUNE_ZINV_POINTS = 2, &
+                               LH_TUNE_XPROD_POINTS = 3, LH_TUNE_ZFWD_POINTS = 4
 
   type, bind(C) :: lh_config
     integer(c_int) :: N1, N2, Ngal1, Ngal2, ny

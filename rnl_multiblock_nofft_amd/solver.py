@@ -136,7 +136,13 @@ class Solver:
         """Time every feasible launch shape on this context's channels and
         keep the fastest; the state is left unchanged."""
         check(self.L.lh_autotune(self.h, int(reps)))
-        return {"solve_cols": self.get_tuning(_lib.TUNE_SOLVE_COLS)}
+        return self.tuning()
+
+    def tuning(self):
+        return {"zinv_points": self.get_tuning(_lib.TUNE_ZINV_POINTS),
+                "xprod_points": self.get_tuning(_lib.TUNE_XPROD_POINTS),
+                "zfwd_points": self.get_tuning(_lib.TUNE_ZFWD_POINTS),
+                "solve_cols": self.get_tuning(_lib.TUNE_SOLVE_COLS)}
 
     def sync(self):
         check(self.L.lh_sync(self.h))

@@ -128,6 +128,23 @@ def test_column_solve_layouts(cols):
         assert e <= TOL, (nm, e)
 
 
+@pytest.mark.parametrize("key,points", [(_lib.TUNE_ZINV_POINTS, 256), (_lib.TUNE_ZINV_POINTS, 4096),
+                                        (_lib.TUNE_XPROD_POINTS, 128), (_lib.TUNE_XPROD_POINTS, 2048),
+                                        (_lib.TUNE_ZFWD_POINTS, 100), (_lib.TUNE_ZFWD_POINTS, 3000)])
+def test_stage_tiles(key, points):
+    """Every tile size of the nonlinear stages (ragged last tiles included)
+    gives the oracle's step."""
+    cfg = CASES["N32_ny153"]
+    s, o, f = make_pair(cfg, seed=9)
+    s.set_tuning(key, points)
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("tiles", key, points, nm, e)
+        assert e <= TOL, (nm, e)
+
+
 def test_autotune_keeps_state():
     cfg = CASES["N32_ny153"]
     s, o, f = make_pair(cfg, seed=8)
