@@ -175,6 +175,16 @@ int lh_get_tuning(lh_ctx *ctx, int key, int *value);
  * before and restored after) and keeps the fastest. */
 int lh_autotune(lh_ctx *ctx, int reps);
 
+/* Textured walls (configurations with superhydrophobic stripes or posts).
+ * bottom / top: N1*N2 ints, index z*N1 + x on the N1 x N2 grid of the
+ * channel's modes, holding the reference's planeBC codes (start.f90:
+ * 1776-1829): 1 = no-slip, 2 = free-shear; NULL leaves that wall smooth.
+ * The U/W Helmholtz solves then take the per-point wall rows of the
+ * reference's textured LUsolU / LU_build (tridLU_3D.f90:753-1078,
+ * 1284-1358; inactive there), solved exactly for the unfactored operator,
+ * and the ghost rows of v_corr follow the mask point by point. */
+int lh_set_texture(lh_ctx *ctx, int ch, const int *bottom, const int *top);
+
 #ifdef __cplusplus
 }
 #endif

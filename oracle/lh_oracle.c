@@ -198,6 +198,9 @@ struct lho_state {
   cplx *u1, *u2, *u3, *p, *Nu1, *Nu2, *Nu3, *du1, *du2, *du3, *psi, *div;
   cplx *u1i, *u2i, *u3i, *uvf, *wvf, *vvc;
   double *dg; /* prefactored Poisson LU, dg[(col*nyu + j)*3 + m] */
+  /* wall textures: tex[w][z*N1 + x] = 1 no-slip, 2 free-shear (planeBC
+   * codes, start.f90:1776-1829), w = 0 bottom, 1 top; NULL = smooth wall */
+  int *tex[2];
 };
 
 static double ipow(double x, int n) {
@@ -386,7 +389,7 @@ void lho_destroy(lho_state *s) {
   void *ptrs[] = {s->yu, s->yv, s->dthdyu, s->dthdyv, s->dyu2i, s->dyv2i, s->kx, s->kz,
                   s->k2x, s->k2z, s->u1, s->u2, s->u3, s->p, s->Nu1, s->Nu2, s->Nu3,
                   s->du1, s->du2, s->du3, s->psi, s->div, s->u1i, s->u2i, s->u3i,
-                  s->uvf, s->wvf, s->vvc, s->dg};
+                  s->uvf, s->wvf, s->vvc, s->dg, s->tex[0], s->tex[1]};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
   free(s);
 }
@@ -813,44 +816,247 @@ void lho_lusolp(const lho_state *s, double *reim) {
 
 /* solveU_W + LUsolU_W + LU_buildU, littleharsh.f90:791-828,
  * tridLU_3D.f90:55-132,257-291 */
+static void build_u_matrix(const lho_state *s, double beta, int c, double *a) {
+  int nyu = s->nyu, i = c % s->nxh, k = c / s->nxh;
+  double k2x = s->k2x[i], k2z = s->k2z[k];
+  a[0] = 0.0;
+  a[1] = 1.0;
+  a[2] = s->gw1;
+  for (int j = 1; j <= nyu - 2; j++) {
+    const double *d = s->dyu2i + 3 * j;
+    a[3 * j] = -beta * d[0];
+    a[3 * j + 1] = 1.0 - beta * (d[1] + k2x + k2z);
+    a[3 * j + 2] = -beta * d[2];
+  }
+  a[3 * (nyu - 1)] = s->gw2;
+  a[3 * (nyu - 1) + 1] = 1.0;
+  a[3 * (nyu - 1) + 2] = 0.0;
+}
+
+static void tex_solve(lho_state *s, double beta, cplx *rhs[2], cplx *out[2]);
+
 void lho_solve_uw(lho_state *s, int kRK) {
   int nyu = s->nyu;
   double beta = s->bRK[kRK - 1] * s->dt / s->c.Re;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < s->ncol; c++) {
+    cplx *u = s->u1 + (size_t)c * nyu, *w = s->u3 + (size_t)c * nyu;
+    cplx *du = s->du1 + (size_t)c * nyu, *dw = s->du3 + (size_t)c * nyu;
+    du[0] = 0;
+    dw[0] = 0;
+    for (int j = 1; j <= nyu - 2; j++) {
+      du[j] = u[j] + s->dt * du[j];
+      dw[j] = w[j] + s->dt * dw[j];
+    }
+    du[nyu - 1] = 0;
+    dw[nyu - 1] = 0;
+  }
+  cplx *rhs[2] = {s->du1, s->du3}, *out[2] = {s->u1, s->u3};
+  tex_solve(s, beta, rhs, out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Textured walls.  The reference builds the free-shear / no-slip mask
+ * planeBC (start.f90:1776-1829) and, in its commented-out LUsolU
+ * (tridLU_3D.f90:753-1078) with LU_build (tridLU_3D.f90:1284-1358), replaces
+ * the wall row of the U/W Helmholtz systems point by point in physical space
+ * on the N1 x N2 grid (four_to_phys_N / phys_to_four_N): no-slip keeps
+ * u0 + gw1 u1 = 0, free-shear has the zero-gradient row u0 - u1 = 0 (top
+ * wall: u_ny - u_{ny-1} = 0 against u_ny + gw2 u_{ny-1} = 0).
+ *
+ * This restatement solves that mode-coupled system exactly, for the
+ * unfactored operator: the wall rows differ from the smooth ones by a
+ * boundary source, A(k) u = r + e_0 sb + e_ny st, where
+ *   sb = T[Pb (T^-1 u_1)],   Pb = 1 + gw1 on free-shear points, 0 else,
+ *   st = T[Pt (T^-1 u_{ny-1})], Pt = 1 + gw2 on free-shear points,
+ * T the N-grid transform.  With u = A^-1 r + gb sb + .. the sources solve a
+ * fixed point, iterated here to convergence; then u = A^-1 (r + e_0 sb +
+ * e_ny st).  The ghost rows of v_corr follow the mask point by point:
+ * u0 = gwbc u1 on no-slip points, u0 = u1 on free-shear points (tex_ghosts). */
+
+/* four_to_phys_N / phys_to_four_N (FOU3D.f90:786-804, 2274-2293) on an
+ * (N1+2) x N2 plane: no oddball row zeroing */
+static void four_to_phys_N(const lho_state *s, double *P) {
+  int ig = s->c.N1 + 2;
+  lho_cft(P, s->c.N2, ig, 2, (s->c.N1 + 2) / 2, +1);
+  lho_rft(P, s->c.N1, ig, s->c.N2, +1);
+}
+static void phys_to_four_N(const lho_state *s, double *P) {
+  int ig = s->c.N1 + 2;
+  lho_rft(P, s->c.N1, ig, s->c.N2, -1);
+  lho_cft(P, s->c.N2, ig, 2, (s->c.N1 + 2) / 2, -1);
+}
+/* column (i,k) <-> plane (2i, 2i+1) of row k */
+static void col_to_planeN(const lho_state *s, const cplx *v, double *P) {
+  int ig = s->c.N1 + 2;
+  for (int c = 0; c < s->ncol; c++) {
+    int i = c % s->nxh, k = c / s->nxh;
+    P[(size_t)k * ig + 2 * i] = creal(v[c]);
+    P[(size_t)k * ig + 2 * i + 1] = cimag(v[c]);
+  }
+}
+static void planeN_to_col(const lho_state *s, const double *P, cplx *v) {
+  int ig = s->c.N1 + 2;
+  for (int c = 0; c < s->ncol; c++) {
+    int i = c % s->nxh, k = c / s->nxh;
+    v[c] = P[(size_t)k * ig + 2 * i] + I * P[(size_t)k * ig + 2 * i + 1];
+  }
+}
+/* v <- T[m(x) . T^-1 v] with m = fs on free-shear points and ns elsewhere */
+static void mask_product(const lho_state *s, const int *mask, double fs, double ns, cplx *v, double *P) {
+  int ig = s->c.N1 + 2, N1 = s->c.N1, N2 = s->c.N2;
+  memset(P, 0, sizeof(double) * (size_t)ig * N2);
+  col_to_planeN(s, v, P);
+  four_to_phys_N(s, P);
+  for (int z = 0; z < N2; z++)
+    for (int x = 0; x < N1; x++) P[(size_t)z * ig + x] *= mask[z * N1 + x] == 2 ? fs : ns;
+  phys_to_four_N(s, P);
+  planeN_to_col(s, P, v);
+}
+
+static void tex_solve(lho_state *s, double beta, cplx *rhs[2], cplx *out[2]) {
+  int nyu = s->nyu, ncol = s->ncol;
+  int tex = s->tex[0] || s->tex[1];
+  /* smooth solve of every right-hand side; for textured walls also the
+   * responses g of rows 1 / ny-1 to a unit source in the wall rows */
+  double *gbb = NULL, *gbt = NULL, *gtb = NULL, *gtt = NULL;
+  if (tex) {
+    gbb = calloc(ncol, sizeof(double));
+    gbt = calloc(ncol, sizeof(double));
+    gtb = calloc(ncol, sizeof(double));
+    gtt = calloc(ncol, sizeof(double));
+  }
 #pragma omp parallel
   {
     double *a = malloc(sizeof(double) * 3 * nyu);
+    cplx *e = malloc(sizeof(cplx) * nyu), *x = malloc(sizeof(cplx) * nyu);
 #pragma omp for schedule(static)
-    for (int c = 0; c < s->ncol; c++) {
-      int i = c % s->nxh, k = c / s->nxh;
-      cplx *u = s->u1 + (size_t)c * nyu, *w = s->u3 + (size_t)c * nyu;
-      cplx *du = s->du1 + (size_t)c * nyu, *dw = s->du3 + (size_t)c * nyu;
-      du[0] = 0;
-      dw[0] = 0;
-      for (int j = 1; j <= nyu - 2; j++) {
-        du[j] = u[j] + s->dt * du[j];
-        dw[j] = w[j] + s->dt * dw[j];
-      }
-      du[nyu - 1] = 0;
-      dw[nyu - 1] = 0;
-      double k2x = s->k2x[i], k2z = s->k2z[k];
-      a[0] = 0.0;
-      a[1] = 1.0;
-      a[2] = s->gw1;
-      for (int j = 1; j <= nyu - 2; j++) {
-        const double *d = s->dyu2i + 3 * j;
-        a[3 * j] = -beta * d[0];
-        a[3 * j + 1] = 1.0 - beta * (d[1] + k2x + k2z);
-        a[3 * j + 2] = -beta * d[2];
-      }
-      a[3 * (nyu - 1)] = s->gw2;
-      a[3 * (nyu - 1) + 1] = 1.0;
-      a[3 * (nyu - 1) + 2] = 0.0;
+    for (int c = 0; c < ncol; c++) {
+      build_u_matrix(s, beta, c, a);
       lu_dec(a, nyu);
-      lu_sol_c(a, du, u, nyu);
-      lu_sol_c(a, dw, w, nyu);
+      for (int f = 0; f < 2; f++)
+        lu_sol_c(a, rhs[f] + (size_t)c * nyu, out[f] + (size_t)c * nyu, nyu);
+      if (tex) {
+        memset(e, 0, sizeof(cplx) * nyu);
+        e[0] = 1.0;
+        lu_sol_c(a, e, x, nyu);
+        gbb[c] = creal(x[1]);
+        gtb[c] = creal(x[nyu - 2]);
+        memset(e, 0, sizeof(cplx) * nyu);
+        e[nyu - 1] = 1.0;
+        lu_sol_c(a, e, x, nyu);
+        gbt[c] = creal(x[1]);
+        gtt[c] = creal(x[nyu - 2]);
+      }
     }
     free(a);
+    free(e);
+    free(x);
   }
+  if (!tex) return;
+  int ig = s->c.N1 + 2;
+  double *P = malloc(sizeof(double) * (size_t)ig * s->c.N2);
+  cplx *sb = calloc(ncol, sizeof(cplx)), *st = calloc(ncol, sizeof(cplx));
+  cplx *tb = malloc(sizeof(cplx) * ncol), *tt = malloc(sizeof(cplx) * ncol);
+  const double Pb = 1.0 + s->gw1, Pt = 1.0 + s->gw2;
+  for (int f = 0; f < 2; f++) {
+    memset(sb, 0, sizeof(cplx) * ncol);
+    memset(st, 0, sizeof(cplx) * ncol);
+    for (int it = 0; it < 200; it++) {
+      double change = 0.0, scale = 0.0;
+      for (int c = 0; c < ncol; c++) {
+        tb[c] = out[f][(size_t)c * nyu + 1] + gbb[c] * sb[c] + gbt[c] * st[c];
+        tt[c] = out[f][(size_t)c * nyu + nyu - 2] + gtb[c] * sb[c] + gtt[c] * st[c];
+      }
+      if (s->tex[0]) mask_product(s, s->tex[0], Pb, 0.0, tb, P);
+      else memset(tb, 0, sizeof(cplx) * ncol);
+      if (s->tex[1]) mask_product(s, s->tex[1], Pt, 0.0, tt, P);
+      else memset(tt, 0, sizeof(cplx) * ncol);
+      for (int c = 0; c < ncol; c++) {
+        change = fmax(change, cabs(tb[c] - sb[c]) + cabs(tt[c] - st[c]));
+        scale = fmax(scale, cabs(tb[c]) + cabs(tt[c]));
+        sb[c] = tb[c];
+        st[c] = tt[c];
+      }
+      if (change <= 1e-17 * scale || scale == 0.0) break;
+    }
+    /* final solve with the boundary sources in the wall rows */
+    for (int c = 0; c < ncol; c++) {
+      rhs[f][(size_t)c * nyu] += sb[c];
+      rhs[f][(size_t)c * nyu + nyu - 1] += st[c];
+    }
+#pragma omp parallel
+    {
+      double *a = malloc(sizeof(double) * 3 * nyu);
+#pragma omp for schedule(static)
+      for (int c = 0; c < ncol; c++) {
+        build_u_matrix(s, beta, c, a);
+        lu_dec(a, nyu);
+        lu_sol_c(a, rhs[f] + (size_t)c * nyu, out[f] + (size_t)c * nyu, nyu);
+      }
+      free(a);
+    }
+  }
+  free(P);
+  free(sb);
+  free(st);
+  free(tb);
+  free(tt);
+  free(gbb);
+  free(gbt);
+  free(gtb);
+  free(gtt);
+}
+
+/* ghost rows of v_corr with the wall masks: u0 = gwbc u1 on no-slip points,
+ * u0 = u1 on free-shear points, applied as u0 = gwbc u1 + T[(1 - gwbc)
+ * chi_fs . T^-1 u1] so that the uniform part stays in Fourier space (the
+ * N-grid transform cannot hold the x-Nyquist column i = N1/2, which the
+ * 3/2-padded step keeps) */
+static void tex_ghosts(lho_state *s) {
+  int nyu = s->nyu, ncol = s->ncol, ig = s->c.N1 + 2;
+  double *P = malloc(sizeof(double) * (size_t)ig * s->c.N2);
+  cplx *v = malloc(sizeof(cplx) * ncol);
+  cplx *fld[2] = {s->u1, s->u3};
+  double gwbc[2] = {s->gwbc1, s->gwbc3};
+  for (int f = 0; f < 2; f++)
+    for (int w = 0; w < 2; w++) {
+      if (!s->tex[w]) continue;
+      int src = w == 0 ? 1 : nyu - 2, dst = w == 0 ? 0 : nyu - 1;
+      for (int c = 0; c < ncol; c++) v[c] = fld[f][(size_t)c * nyu + src];
+      mask_product(s, s->tex[w], 1.0 - gwbc[f], 0.0, v, P);
+      for (int c = 0; c < ncol; c++) fld[f][(size_t)c * nyu + dst] += v[c];
+    }
+  free(P);
+  free(v);
+}
+
+int lho_set_texture(lho_state *s, const int *bottom, const int *top) {
+  size_t n = (size_t)s->c.N1 * s->c.N2;
+  const int *m[2] = {bottom, top};
+  for (int w = 0; w < 2; w++) {
+    free(s->tex[w]);
+    s->tex[w] = NULL;
+    if (!m[w]) continue;
+    for (size_t q = 0; q < n; q++)
+      if (m[w][q] != 1 && m[w][q] != 2) return -1;
+    s->tex[w] = malloc(sizeof(int) * n);
+    memcpy(s->tex[w], m[w], sizeof(int) * n);
+  }
+  return 0;
+}
+
+/* LUsolU_W on arbitrary right-hand sides with the channel's wall textures
+ * (rhs rows 0 and ny are the wall-row right-hand sides) */
+void lho_lusol_uw_tex(lho_state *s, int kRK, const double *rhs_u, const double *rhs_w, double *u, double *w) {
+  size_t n = (size_t)s->ncol * s->nyu;
+  cplx *r[2] = {malloc(sizeof(cplx) * n), malloc(sizeof(cplx) * n)};
+  memcpy(r[0], rhs_u, sizeof(cplx) * n);
+  memcpy(r[1], rhs_w, sizeof(cplx) * n);
+  cplx *o[2] = {(cplx *)u, (cplx *)w};
+  tex_solve(s, s->bRK[kRK - 1] * s->dt / s->c.Re, r, o);
+  free(r[0]);
+  free(r[1]);
 }
 
 /* solveV + LUsolV + LU_buildV, littleharsh.f90:877-907, tridLU_3D.f90:135-189,294-329 */
@@ -1022,6 +1228,7 @@ void lho_v_corr(lho_state *s, int kRK) {
     u3[0] = s->gwbc3 * u3[1];
     u3[nyu - 1] = s->gwbc3 * u3[nyu - 2];
   }
+  if (s->tex[0] || s->tex[1]) tex_ghosts(s);
   lho_divergence(s);
 }
 

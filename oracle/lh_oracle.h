@@ -99,6 +99,13 @@ void lho_lusol_helm(const lho_state *s, int kRK, int vgrid, const double *rhs, d
 void lho_lusolp(const lho_state *s, double *reim);
 void lho_lusol0(const lho_state *s, int kRK, double *x); /* x(0:ny), LUsol0 */
 
+/* Wall textures (planeBC codes per point of the N1 x N2 grid, index
+ * z*N1 + x: 1 no-slip, 2 free-shear); NULL leaves that wall smooth.
+ * Returns -1 on a bad code. */
+int lho_set_texture(lho_state *s, const int *bottom, const int *top);
+/* the textured U/W Helmholtz solve on arbitrary right-hand sides */
+void lho_lusol_uw_tex(lho_state *s, int kRK, const double *rhs_u, const double *rhs_w, double *u, double *w);
+
 #ifdef __cplusplus
 }
 #endif

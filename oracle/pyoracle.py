@@ -80,6 +80,10 @@ def lib():
         L.lho_lusol0.argtypes = [vp, ctypes.c_int, dp]
         L.lho_lusol_helm.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.lho_lusolp.argtypes = [vp, dp]
+        ip_ = ctypes.POINTER(ctypes.c_int)
+        L.lho_set_texture.argtypes = [vp, ip_, ip_]
+        L.lho_set_texture.restype = ctypes.c_int
+        L.lho_lusol_uw_tex.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -206,6 +210,31 @@ class Oracle:
         x = np.zeros_like(r)
         lib().lho_lusol_helm(self.h, kRK, int(vgrid), _dp(r.view(np.float64)), _dp(x.view(np.float64)))
         return x
+
+    def set_texture(self, bottom=None, top=None):
+        """Wall masks on the N1 x N2 grid, shape (N2, N1), planeBC codes
+        (1 no-slip, 2 free-shear); None leaves a wall smooth."""
+        keep = []
+
+        def arg(m):
+            if m is None:
+                return None
+            a = np.ascontiguousarray(m, dtype=np.int32)
+            assert a.shape == (self.cfg["N2"], self.cfg["N1"]), a.shape
+            keep.append(a)
+            return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        rc = lib().lho_set_texture(self.h, arg(bottom), arg(top))
+        if rc != 0:
+            raise ValueError("texture masks must hold 1 (no-slip) or 2 (free-shear)")
+
+    def lusol_uw_tex(self, kRK, rhs_u, rhs_w):
+        ru = np.ascontiguousarray(rhs_u, dtype=np.complex128)
+        rw = np.ascontiguousarray(rhs_w, dtype=np.complex128)
+        u = np.zeros_like(ru)
+        w = np.zeros_like(rw)
+        lib().lho_lusol_uw_tex(self.h, kRK, _dp(ru.view(np.float64)), _dp(rw.view(np.float64)),
+                               _dp(u.view(np.float64)), _dp(w.view(np.float64)))
+        return u, w
 
     def lusolp(self, rhs):
         x = np.ascontiguousarray(rhs, dtype=np.complex128).copy()

@@ -26,7 +26,7 @@ EXPORTS = ["lh_last_error", "lh_ctx_create", "lh_ctx_destroy", "lh_set_grid", "l
            "lh_init_flowrate", "lh_substep", "lh_step", "lh_sync", "lh_time_kernels", "lh_run_phase",
            "lh_four_to_phys_planes", "lh_phys_to_four_planes", "lh_lusol_uw", "lh_lusol_v",
            "lh_dev_alloc", "lh_dev_free", "lh_copy_to_device", "lh_copy_to_host",
-           "lh_set_tuning", "lh_get_tuning", "lh_autotune"]
+           "lh_set_tuning", "lh_get_tuning", "lh_autotune", "lh_set_texture"]
 
 
 class LhConfig(ctypes.Structure):
@@ -82,6 +82,7 @@ def load():
     L.lh_set_tuning.argtypes = [vp, ip, ip]
     L.lh_get_tuning.argtypes = [vp, ip, ctypes.POINTER(ip)]
     L.lh_autotune.argtypes = [vp, ip]
+    L.lh_set_texture.argtypes = [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)]
     for name in EXPORTS:
         if name != "lh_last_error":
             getattr(L, name).restype = ctypes.c_int

@@ -217,6 +217,9 @@ struct lh_ctx {
   size_t staging_elems = 0;
   hipGraphExec_t graph = nullptr;
   int device = 0;
+  std::vector<TexGroup> tex_groups;  // N-grid work planes per (N1, N2) shape
+  int ntex = 0;                      // channels with a textured wall
+  int tex_nit = 4;                   // wall-source iterations per substep
 };
 
 namespace {
@@ -313,6 +316,10 @@ size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
   d.o_twx = take(al<double2>(c.Ngal1));
   d.o_scal = take(al<double>(SC_NSCAL));
   d.o_fac = take(al<double>(14 * fld));
+  d.o_mask = take(al<int>(2 * (size_t)c.N1 * c.N2));
+  d.o_iscal = take(al<int>(4));
+  d.o_wtab = take(al<double>(6 * (size_t)h.nyu * h.ldc));
+  d.o_texu = take(al<double2>(4 * (size_t)h.ncol));
   if (x->mixed) {
     d.o_hzf = take(al<float2>(3 * ninter));
     d.o_gxf = take(al<float2>(4 * ninter));
@@ -346,6 +353,8 @@ size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
 
 int init_chan(lh_ctx *x, HostChan &h) {
   View d(x->arena, h.dev);
+  std::vector<int> ns(2 * (size_t)h.cfg.N1 * h.cfg.N2, 1);
+  HIPCHK(hipMemcpy(d.mask, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice));
   double sc[SC_NSCAL] = {0};
   sc[SC_MPGX] = h.cfg.mpgx;
   sc[SC_DT] = h.cfg.dt;
@@ -386,10 +395,28 @@ int check_ctx(lh_ctx *x) {
   return LH_OK;
 }
 
+// column solves of a substep; with textured walls, the wall sources before
+// and the masked ghost rows after
+int solve_phase(lh_ctx *x, int kRK) {
+  if (x->ntex)
+    HIPCHK(launch_tex_sources(x->arena, x->d_chans, x->shape, x->tex_groups, kRK, x->tex_nit, x->stream));
+  HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
+  if (x->ntex) HIPCHK(launch_tex_ghosts(x->arena, x->d_chans, x->shape, x->tex_groups, x->stream));
+  return LH_OK;
+}
+
 int run_substep(lh_ctx *x, int kRK) {
   HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream));
   HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
-  HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
+  return solve_phase(x, kRK);
+}
+
+// Thomas factors and the textured-wall inverse rows (after a grid change)
+int refresh_tables(lh_ctx *x) {
+  for (auto &h : x->ch) HIPCHK(hipMemsetAsync(x->arena + h.dev.o_iscal, 0, 4 * sizeof(int), x->stream));
+  HIPCHK(launch_factors(x->arena, x->d_chans, x->shape, x->stream));
+  HIPCHK(launch_tex_tables(x->arena, x->d_chans, x->shape, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
   return LH_OK;
 }
 
@@ -430,6 +457,30 @@ int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
   if (rc == LH_OK) {
     size_t off = 0;
     for (int c = 0; c < nchan; c++) off = layout_chan(x, x->ch[c], off);
+    // texture work planes, channels of one (N1, N2) shape side by side, and
+    // the N-grid twiddles of each shape
+    for (int c = 0; c < nchan; c++) {
+      HostChan &h = x->ch[c];
+      TexGroup *g = nullptr;
+      for (auto &q : x->tex_groups)
+        if (q.N1 == h.cfg.N1 && q.N2 == h.cfg.N2) g = &q;
+      if (!g) {
+        x->tex_groups.push_back(TexGroup{h.cfg.N1, h.cfg.N2, 0, 0, 0, 0});
+        g = &x->tex_groups.back();
+      }
+      g->nplanes += 4;
+    }
+    for (auto &g : x->tex_groups) {
+      g.o_twz = off;
+      off += al<double2>(g.N2);
+      g.o_twx = off;
+      off += al<double2>(g.N1);
+      g.o_texw = off;
+      off += al<double2>((size_t)g.nplanes * (g.N1 / 2 + 1) * g.N2);
+      int q = 0;
+      for (auto &h : x->ch)
+        if (h.cfg.N1 == g.N1 && h.cfg.N2 == g.N2) h.dev.o_texw = g.o_texw + (size_t)4 * (q++) * h.ncol * sizeof(double2);
+    }
     e = hipMalloc((void **)&x->arena, off);
     if (e != hipSuccess) rc = set_err(LH_ENOMEM, "hipMalloc(%zu): %s", off, hipGetErrorString(e));
     x->arena_bytes = off;
@@ -439,6 +490,15 @@ int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
     if (e != hipSuccess) rc = set_err(LH_EHIP, "hipMemset: %s", hipGetErrorString(e));
   }
   for (int c = 0; c < nchan && rc == LH_OK; c++) rc = init_chan(x, x->ch[c]);
+  for (size_t q = 0; q < x->tex_groups.size() && rc == LH_OK; q++) {
+    const TexGroup &g = x->tex_groups[q];
+    std::vector<double2> tz, tx;
+    twiddles(tz, g.N2);
+    twiddles(tx, g.N1);
+    e = hipMemcpy(x->arena + g.o_twz, tz.data(), tz.size() * sizeof(double2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(x->arena + g.o_twx, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = set_err(LH_EHIP, "texture twiddles: %s", hipGetErrorString(e));
+  }
   if (rc == LH_OK) {
     e = hipMalloc(&x->d_chans, nchan * sizeof(Chan));
     if (e != hipSuccess) rc = set_err(LH_ENOMEM, "hipMalloc(descriptors): %s", hipGetErrorString(e));
@@ -449,11 +509,7 @@ int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
     x->staging_elems = stage;
   }
   if (rc == LH_OK) rc = push_descriptors(x);
-  if (rc == LH_OK) {
-    e = launch_factors(x->arena, x->d_chans, x->shape, x->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
-    if (e != hipSuccess) rc = set_err(LH_EHIP, "factor set-up: %s", hipGetErrorString(e));
-  }
+  if (rc == LH_OK) rc = refresh_tables(x);
   if (rc != LH_OK) {
     std::string keep = g_err;
     lh_ctx_destroy(x);
@@ -494,9 +550,7 @@ int lh_set_grid(lh_ctx *x, int ch, const double *yu, const double *yv, const dou
   if (rc) return rc;
   rc = push_descriptors(x);
   if (rc) return rc;
-  HIPCHK(launch_factors(x->arena, x->d_chans, x->shape, x->stream));
-  HIPCHK(hipStreamSynchronize(x->stream));
-  return LH_OK;
+  return refresh_tables(x);
 }
 
 int lh_get_grid(lh_ctx *x, int ch, double *yu, double *yv, double *dthdyu, double *dthdyv) {
@@ -589,7 +643,11 @@ int lh_run_phase(lh_ctx *x, int phase, int kRK) {
   switch (phase) {
     case LH_PHASE_NONLINEAR: HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream)); break;
     case LH_PHASE_RHS: HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream)); break;
-    case LH_PHASE_SOLVE: HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream)); break;
+    case LH_PHASE_SOLVE: {
+      int r = solve_phase(x, kRK);
+      if (r) return r;
+      break;
+    }
     default: return set_err(LH_EINVAL, "unknown phase %d", phase);
   }
   if (kRK == 1 && phase == LH_PHASE_NONLINEAR)
@@ -656,7 +714,8 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
     }
     HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
     HIPCHK(hipEventRecord(ev[4], x->stream));
-    HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
+    rc = solve_phase(x, kRK);
+    if (rc) break;
     HIPCHK(hipEventRecord(ev[5], x->stream));
     HIPCHK(hipEventSynchronize(ev[5]));
     for (int q = 0; q < 5; q++) {
@@ -941,6 +1000,36 @@ int lh_autotune(lh_ctx *x, int reps) {
   HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(hipFree(save));
   x->shape = best;
+  drop_graph(x);
+  return LH_OK;
+}
+
+int lh_set_texture(lh_ctx *x, int ch, const int *bottom, const int *top) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size()) return set_err(LH_EINVAL, "lh_set_texture: bad channel %d", ch);
+  HostChan &h = x->ch[ch];
+  const size_t n = (size_t)h.cfg.N1 * h.cfg.N2;
+  std::vector<int> m(2 * n, 1);
+  const int *src[2] = {bottom, top};
+  int bits = 0;
+  for (int w = 0; w < 2; w++) {
+    if (!src[w]) continue;
+    for (size_t q = 0; q < n; q++) {
+      if (src[w][q] != 1 && src[w][q] != 2)
+        return set_err(LH_EINVAL, "lh_set_texture: mask codes are 1 (no-slip) or 2 (free-shear), got %d", src[w][q]);
+      m[w * n + q] = src[w][q];
+    }
+    bits |= 1 << w;
+  }
+  HIPCHK(hipStreamSynchronize(x->stream));
+  View d(x->arena, h.dev);
+  HIPCHK(hipMemcpy(d.mask, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice));
+  h.dev.tex = bits;
+  x->ntex = 0;
+  for (auto &q : x->ch) x->ntex += q.dev.tex != 0;
+  rc = push_descriptors(x);
+  if (rc) return rc;
   drop_graph(x);
   return LH_OK;
 }

@@ -54,6 +54,14 @@ struct Chan {
   // [3][nyu]; hz/gx (or hzf/gxf in FP32): nonlinear pipeline intermediates;
   // scal: SC_* scalars
   size_t o_scr, o_u11, o_hz, o_gx, o_hzf, o_gxf, o_scal;
+  // textured walls (tex bit 0 bottom, bit 1 top): mask[w][z*N1 + x] planeBC
+  // codes (1 no-slip, 2 free-shear); wtab[(kRK-1)*2 + w][row][ldc] rows 1
+  // (w = 0) and ny-1 (w = 1) of the inverse U/W Helmholtz operator; texu /
+  // texw [f*2 + w][ncol] adjacent-row values and the wall-source work planes
+  // (N-grid, (N1+2) x N2 doubles each); iscal[0..1] truncation depths of the
+  // two inverse rows
+  int tex;
+  size_t o_mask, o_wtab, o_texu, o_texw, o_iscal;
   // Thomas factors of the fixed-dt operators, [row][ldc] doubles each:
   // o_fac + ((kRK-1)*4 + m) * nyu*ldc*8 for m = 0 dU, 1 cU, 2 dV, 3 cV
   // (U/W and V Helmholtz, LU_dec order), then o_fac + 12*nyu*ldc*8 dP and
@@ -70,6 +78,9 @@ struct View {
   double2 *hz, *gx;
   float2 *hzf, *gxf;
   double *fac;
+  int *mask, *iscal;
+  double *wtab;
+  double2 *texu, *texw;
   __host__ __device__ View(char *A, const Chan &c)
       : yu((const double *)(A + c.o_yu)), yv((const double *)(A + c.o_yv)),
         dthdyu((const double *)(A + c.o_dthdyu)), dthdyv((const double *)(A + c.o_dthdyv)),
@@ -84,7 +95,9 @@ struct View {
         fwv((double2 *)(A + c.o_fwv)), fvv((double2 *)(A + c.o_fvv)), psi((double2 *)(A + c.o_psi)),
         div((double2 *)(A + c.o_div)), scr((double *)(A + c.o_scr)), u11((double *)(A + c.o_u11)),
         scal((double *)(A + c.o_scal)), hz((double2 *)(A + c.o_hz)), gx((double2 *)(A + c.o_gx)),
-        hzf((float2 *)(A + c.o_hzf)), gxf((float2 *)(A + c.o_gxf)), fac((double *)(A + c.o_fac)) {}
+        hzf((float2 *)(A + c.o_hzf)), gxf((float2 *)(A + c.o_gxf)), fac((double *)(A + c.o_fac)),
+        mask((int *)(A + c.o_mask)), iscal((int *)(A + c.o_iscal)), wtab((double *)(A + c.o_wtab)),
+        texu((double2 *)(A + c.o_texu)), texw((double2 *)(A + c.o_texw)) {}
 };
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }

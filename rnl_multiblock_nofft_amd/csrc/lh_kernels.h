@@ -19,6 +19,7 @@ struct ShapeMax {
   int zinv_tiles = 0, xprod_tiles = 0, zfwd_tiles = 0;
   size_t lds_zinv = 0, lds_xprod = 0, lds_zfwd = 0;  // in complex elements
   int solve_cols = 0;  // columns per workgroup of the LDS column solve (0: one lane per column)
+  int nphys = 0;       // N1*N2, largest over channels (texture mask kernels)
 };
 void shape_tiles(ShapeMax &m);
 int solve_cols_default(int nyu);
@@ -35,11 +36,23 @@ hipError_t launch_to_rows(const double2 *src, double2 *dst, int ncol, int nrows,
                           hipStream_t s);
 hipError_t launch_from_rows(const double2 *src, double2 *dst, int ncol, int nrows, int jlo, int ldc,
                             hipStream_t s);
-// stand-alone plane transforms (reference plane layout)
+// stand-alone plane transforms (reference plane layout); zero_nyq zeroes
+// row Ngal2/2 as four_to_phys_du / phys_to_four_du do (the N-grid
+// four_to_phys_N / phys_to_four_N do not)
 hipError_t launch_plane_f2p(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
-                            const double2 *twx, hipStream_t s);
+                            const double2 *twx, hipStream_t s, bool zero_nyq = true);
 hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
-                            const double2 *twx, hipStream_t s);
+                            const double2 *twx, hipStream_t s, bool zero_nyq = true);
+// textured walls: channels of one (N1, N2) shape share a run of work planes
+struct TexGroup {
+  int N1, N2, nplanes;
+  size_t o_texw, o_twz, o_twx;  // arena offsets: planes, N2- and N1-point twiddles
+};
+hipError_t launch_tex_tables(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
+hipError_t launch_tex_sources(char *arena, const Chan *d_chans, const ShapeMax &m,
+                              const std::vector<TexGroup> &groups, int kRK, int nit, hipStream_t s);
+hipError_t launch_tex_ghosts(char *arena, const Chan *d_chans, const ShapeMax &m,
+                             const std::vector<TexGroup> &groups, hipStream_t s);
 // stand-alone Helmholtz solves on reference-layout device data
 hipError_t launch_lusol_ref2(char *arena, const Chan *d_chan, int kRK, int vgrid, const double2 *rhs_a,
                              const double2 *rhs_b, double2 *x_a, double2 *x_b, double *scr, int ncol,

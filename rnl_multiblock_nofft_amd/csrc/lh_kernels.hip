@@ -1209,6 +1209,7 @@ void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny) {
   m.ncol = max(m.ncol, ncol);
   m.nyu = max(m.nyu, ny + 1);
   m.solve_cols = solve_cols_default(m.nyu);
+  m.nphys = max(m.nphys, N1 * N2);
   m.dims.push_back(ChanDims{N1, N2, Ng1, Ng2, ny});
   shape_tiles(m);
   m.nchan++;

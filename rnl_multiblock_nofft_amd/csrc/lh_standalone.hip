@@ -19,7 +19,7 @@ static __host__ __device__ int plane_tile(int n) {
 // dir = +1: inverse (zero row Ngal2/2 first); dir = -1: forward, scaled, then
 // zero row Ngal2/2 of the whole plane.
 __global__ void __launch_bounds__(256) k_plane_z(double *planes, int N1, int Ng1, int Ng2, int dir,
-                                                 const double2 *twz) {
+                                                 const double2 *twz, int zero_nyq) {
   const int igal = Ng1 + 2, nv = N1 / 2 + 1;
   double *P = planes + (size_t)blockIdx.y * igal * Ng2;
   const int TI = plane_tile(Ng2);
@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(256) k_plane_z(double *planes, int N1, int Ng1
     for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) {
       int z = t / ni, ii = t - z * ni;
       double2 v = make_double2(P[(size_t)z * igal + 2 * (i0 + ii)], P[(size_t)z * igal + 2 * (i0 + ii) + 1]);
-      if (dir > 0 && z == Ng2 / 2) v = make_double2(0, 0);
+      if (zero_nyq && dir > 0 && z == Ng2 / 2) v = make_double2(0, 0);
       A[ii * Ng2 + z] = v;
     }
     __syncthreads();
@@ -41,14 +41,14 @@ __global__ void __launch_bounds__(256) k_plane_z(double *planes, int N1, int Ng1
     for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) {
       int z = t / ni, ii = t - z * ni;
       double2 v = R[ii * Ng2 + z];
-      if (dir < 0 && z == Ng2 / 2) v = make_double2(0, 0);
+      if (zero_nyq && dir < 0 && z == Ng2 / 2) v = make_double2(0, 0);
       P[(size_t)z * igal + 2 * (i0 + ii)] = v.x * s;
       P[(size_t)z * igal + 2 * (i0 + ii) + 1] = v.y * s;
     }
   }
   // entries not transformed: row Ngal2/2 is zeroed (four_to_phys zeroes it
   // before, phys_to_four after the transforms)
-  if (blockIdx.x == 0) {
+  if (zero_nyq && blockIdx.x == 0) {
     __syncthreads();
     for (int ii = 2 * nv + threadIdx.x; ii < igal; ii += blockDim.x) P[(size_t)(Ng2 / 2) * igal + ii] = 0.0;
   }
@@ -128,22 +128,22 @@ __global__ void __launch_bounds__(256) k_plane_x(double *planes, int Ng1, int Ng
 }
 
 hipError_t launch_plane_f2p(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
-                            const double2 *twx, hipStream_t s) {
+                            const double2 *twx, hipStream_t s, bool zero_nyq) {
   int nv = N1 / 2 + 1, TI = plane_tile(Ng2), TZ = plane_tile(Ng1);
   dim3 gz((nv + TI - 1) / TI, nplanes), gx((Ng2 + 2 * TZ - 1) / (2 * TZ), nplanes);
   hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, N1, Ng1, Ng2, +1,
-                     twz);
+                     twz, (int)zero_nyq);
   hipLaunchKernelGGL(k_plane_x, gx, dim3(256), (2 * TZ + 1) * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, +1, twx);
   return hipGetLastError();
 }
 
 hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
-                            const double2 *twx, hipStream_t s) {
+                            const double2 *twx, hipStream_t s, bool zero_nyq) {
   int nv = N1 / 2 + 1, TI = plane_tile(Ng2), TZ = plane_tile(Ng1);
   dim3 gz((nv + TI - 1) / TI, nplanes), gx((Ng2 + 2 * TZ - 1) / (2 * TZ), nplanes);
   hipLaunchKernelGGL(k_plane_x, gx, dim3(256), (2 * TZ + 1) * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, -1, twx);
   hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, N1, Ng1, Ng2, -1,
-                     twz);
+                     twz, (int)zero_nyq);
   return hipGetLastError();
 }
 

@@ -1,0 +1,257 @@
+// lh_texture.hip -- textured walls (free-shear / no-slip point masks) for the
+// U/W Helmholtz solves of the RK3 substep.
+//
+// The reference builds the mask planeBC (start.f90:1776-1829) and, in its
+// commented-out LUsolU / LU_build (tridLU_3D.f90:753-1078, 1284-1358),
+// replaces the wall row of the U/W systems point by point on the N1 x N2 grid
+// (four_to_phys_N / phys_to_four_N): no-slip keeps u0 + gw1 u1 = 0,
+// free-shear has u0 - u1 = 0 (top: u_ny - u_{ny-1} = 0 against
+// u_ny + gw2 u_{ny-1} = 0).  Here that mode-coupled system is solved exactly
+// for the unfactored operator A(k) as a boundary source on the wall rows:
+//   A u = r + e_0 sb + e_ny st,  sb = T[Pb . T^-1 u_1],  st = T[Pt . T^-1 u_{ny-1}]
+// with Pb = 1 + gw1, Pt = 1 + gw2 on free-shear points and 0 elsewhere, T the
+// N-grid transform.  Writing u_1 = U_b + gbb sb + gbt st (U the smooth
+// solution, g entries of A^-1), the sources are a fixed point, iterated with
+// the column-independent part gbb(0) moved to the left:
+//   sb <- T[Pb . T^-1(U_b + (gbb - gbb(0)) sb + gbt st) / (1 - gbb(0) Pb)]
+// which contracts by max|gbb - gbb(0)| |Pb| (O(beta k^2) small) per pass.
+// sb, st then become the right-hand sides of the wall rows and the ordinary
+// column solve runs; afterwards the ghost rows follow the mask point by point
+// (u0 = gwbc u1 on no-slip, u0 = u1 on free-shear points), as v_corr's
+// ghost update (littleharsh.f90:992-1044) generalised to a mask, applied as
+// u0 = gwbc u1 + T[(1 - gwbc) chi_fs . T^-1 u1]: the uniform part stays in
+// Fourier space, because the N-grid transform cannot hold the x-Nyquist
+// column i = N1/2 that the 3/2-padded step keeps.
+//
+// U_b and U_t come from rows 1 and ny-1 of A^-1 (tables built once), which
+// decay exponentially away from their wall: only the first rows are summed.
+#include "lh_common.h"
+#include "lh_kernels.h"
+
+namespace lh {
+
+// rows 1 and ny-1 of A^-1 per column and substep, from the LU factors:
+// A^T z = e_m by U^T w = e_m, L^T z = w
+__global__ void __launch_bounds__(64) k_tex_tables(char *arena, const Chan *chans) {
+  const Chan ch = chans[blockIdx.z];
+  const View v(arena, ch);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ch.ncol) return;
+  const int n = ch.nyu, ldc = ch.ldc;
+  const size_t plane = (size_t)(n + PADROWS) * ldc, tab = (size_t)n * ldc;
+  const double dt = v.scal[SC_DT];
+  const double tol = 1e-18;
+  int jb = 1, jt = 1;
+  for (int kRK = 1; kRK <= 3; kRK++) {
+    const double beta = rk_b(kRK) * dt / ch.Re;
+    const double *dU = v.fac + (size_t)((kRK - 1) * 4) * plane, *cU = dU + plane;
+    double *Wb = v.wtab + (size_t)((kRK - 1) * 2) * tab, *Wt = Wb + tab;
+    auto a1 = [&](int j) { return j == 0 ? 0.0 : (j <= n - 2 ? -beta * v.dyu2i[3 * j] : ch.gw2); };
+    // row 1: w_0 = 0, w_1 = 1, w_j = -c_{j-1} w_{j-1}
+    double w = 0.0;
+    Wb[c] = 0.0;
+    for (int j = 1; j < n; j++) {
+      w = j == 1 ? 1.0 : -cU[(size_t)(j - 1) * ldc + c] * w;
+      Wb[(size_t)j * ldc + c] = w;
+    }
+    double z = Wb[(size_t)(n - 1) * ldc + c] * dU[(size_t)(n - 1) * ldc + c];
+    Wb[(size_t)(n - 1) * ldc + c] = z;
+    double zmax = fabs(z);
+    for (int j = n - 2; j >= 0; j--) {
+      z = (Wb[(size_t)j * ldc + c] - a1(j + 1) * z) * dU[(size_t)j * ldc + c];
+      Wb[(size_t)j * ldc + c] = z;
+      zmax = fmax(zmax, fabs(z));
+    }
+    for (int j = n - 1; j >= 0; j--)
+      if (fabs(Wb[(size_t)j * ldc + c]) > tol * zmax) {
+        jb = max(jb, j + 1);
+        break;
+      }
+    // row n-2: w_{n-2} = 1, w_{n-1} = -c_{n-2}, zero below
+    z = -cU[(size_t)(n - 2) * ldc + c] * dU[(size_t)(n - 1) * ldc + c];
+    Wt[(size_t)(n - 1) * ldc + c] = z;
+    zmax = fabs(z);
+    for (int j = n - 2; j >= 0; j--) {
+      z = ((j == n - 2 ? 1.0 : 0.0) - a1(j + 1) * z) * dU[(size_t)j * ldc + c];
+      Wt[(size_t)j * ldc + c] = z;
+      zmax = fmax(zmax, fabs(z));
+    }
+    for (int j = 0; j < n; j++)
+      if (fabs(Wt[(size_t)j * ldc + c]) > tol * zmax) {
+        jt = max(jt, n - j);
+        break;
+      }
+  }
+  atomicMax(&v.iscal[0], jb);
+  atomicMax(&v.iscal[1], jt);
+}
+
+// adjacent-row values of the smooth solution, U = sum_j W[j] r_j over the
+// rows where the inverse row is significant; grid (cols, 2 walls, channels)
+__global__ void __launch_bounds__(256) k_tex_adj(char *arena, const Chan *chans, int kRK) {
+  const Chan ch = chans[blockIdx.z];
+  if (!ch.tex) return;
+  const View v(arena, ch);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ch.ncol) return;
+  const int n = ch.nyu, ldc = ch.ldc, w = blockIdx.y;
+  const size_t tab = (size_t)n * ldc;
+  const double *W = v.wtab + (size_t)((kRK - 1) * 2 + w) * tab;
+  const int J = w == 0 ? v.iscal[0] : v.iscal[1];
+  const int j0 = w == 0 ? 0 : n - J, j1 = w == 0 ? J : n;
+  double2 a = make_double2(0, 0), b = make_double2(0, 0);
+  for (int j = j0; j < j1; j++) {
+    const size_t o = (size_t)j * ldc + c;
+    const double q = W[o];
+    const double2 r1 = v.r1[o], r3 = v.r3[o];
+    a = make_double2(a.x + q * r1.x, a.y + q * r1.y);
+    b = make_double2(b.x + q * r3.x, b.y + q * r3.y);
+  }
+  v.texu[(size_t)(0 * 2 + w) * ch.ncol + c] = a;
+  v.texu[(size_t)(1 * 2 + w) * ch.ncol + c] = b;
+}
+
+// texw <- U + (g - gbar) texw (iteration `it`; texw holds the previous
+// sources, unused when it == 0)
+__global__ void __launch_bounds__(256) k_tex_pre(char *arena, const Chan *chans, int kRK, int it) {
+  const Chan ch = chans[blockIdx.z];
+  if (!ch.tex) return;
+  const View v(arena, ch);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ch.ncol) return;
+  const int n = ch.nyu, ldc = ch.ldc;
+  const size_t tab = (size_t)n * ldc;
+  const double *Wb = v.wtab + (size_t)((kRK - 1) * 2) * tab, *Wt = Wb + tab;
+  const double gbb = Wb[c], gbt = Wb[(size_t)(n - 1) * ldc + c];
+  const double gtb = Wt[c], gtt = Wt[(size_t)(n - 1) * ldc + c];
+  const double gbb0 = Wb[0], gtt0 = Wt[(size_t)(n - 1) * ldc];
+  const int nc = ch.ncol;
+  for (int f = 0; f < 2; f++) {
+    const double2 Ub = v.texu[(size_t)(f * 2) * nc + c], Ut = v.texu[(size_t)(f * 2 + 1) * nc + c];
+    double2 sb = make_double2(0, 0), st = make_double2(0, 0);
+    if (it > 0) {
+      sb = v.texw[(size_t)(f * 2) * nc + c];
+      st = v.texw[(size_t)(f * 2 + 1) * nc + c];
+    }
+    const double db = gbb - gbb0, dt_ = gtt - gtt0;
+    v.texw[(size_t)(f * 2) * nc + c] =
+        make_double2(Ub.x + db * sb.x + gbt * st.x, Ub.y + db * sb.y + gbt * st.y);
+    v.texw[(size_t)(f * 2 + 1) * nc + c] =
+        make_double2(Ut.x + gtb * sb.x + dt_ * st.x, Ut.y + gtb * sb.y + dt_ * st.y);
+  }
+}
+
+// pointwise mask on the physical N-grid planes of texw; grid (points, 4, ch)
+// mode 0: v <- P v / (1 - gbar P)   (wall sources)
+// mode 1: v <- (1 - gwbc) v on free-shear points, 0 elsewhere (ghost rows)
+__global__ void __launch_bounds__(256) k_tex_mask(char *arena, const Chan *chans, int kRK, int mode) {
+  const Chan ch = chans[blockIdx.z];
+  if (!ch.tex) return;
+  const View v(arena, ch);
+  const int N1 = ch.N1, N2 = ch.N2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N1 * N2) return;
+  const int p = blockIdx.y, f = p >> 1, w = p & 1;
+  const int z = t / N1, x = t - z * N1;
+  double *P = reinterpret_cast<double *>(v.texw + (size_t)p * ch.ncol);
+  const bool fs = v.mask[w * N1 * N2 + t] == 2;
+  double &val = P[(size_t)z * (N1 + 2) + x];
+  if (mode == 0) {
+    const int n = ch.nyu, ldc = ch.ldc;
+    const size_t tab = (size_t)n * ldc;
+    const double *Wb = v.wtab + (size_t)((kRK - 1) * 2) * tab, *Wt = Wb + tab;
+    const double g0 = w == 0 ? Wb[0] : Wt[(size_t)(n - 1) * ldc];
+    const double Pw = fs ? 1.0 + (w == 0 ? ch.gw1 : ch.gw2) : 0.0;
+    val = Pw * val / (1.0 - g0 * Pw);
+  } else {
+    val = fs ? (1.0 - (f == 0 ? ch.gwbc1 : ch.gwbc3)) * val : 0.0;
+  }
+}
+
+// mode 0: wall-row right-hand sides r_f(0), r_f(ny) <- sources;
+// mode 1: ghost rows u_f(0), u_f(ny) (gwbc u1 from the column solve) += texw;
+// only textured walls
+__global__ void __launch_bounds__(256) k_tex_put(char *arena, const Chan *chans, int mode) {
+  const Chan ch = chans[blockIdx.z];
+  if (!ch.tex) return;
+  const View v(arena, ch);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ch.ncol) return;
+  const int n = ch.nyu, ldc = ch.ldc, nc = ch.ncol;
+  for (int f = 0; f < 2; f++) {
+    double2 *dst = mode == 0 ? (f == 0 ? v.r1 : v.r3) : (f == 0 ? v.u1 : v.u3);
+    const double2 wb = v.texw[(size_t)(f * 2) * nc + c], wt = v.texw[(size_t)(f * 2 + 1) * nc + c];
+    const size_t ot = (size_t)(n - 1) * ldc + c;
+    if (mode == 0) {
+      if (ch.tex & 1) dst[c] = wb;
+      if (ch.tex & 2) dst[ot] = wt;
+    } else {
+      if (ch.tex & 1) dst[c] = cadd(dst[c], wb);
+      if (ch.tex & 2) dst[ot] = cadd(dst[ot], wt);
+    }
+  }
+}
+
+// texw <- rows 1 and ny-1 of u1, u3 (after the projection)
+__global__ void __launch_bounds__(256) k_tex_get(char *arena, const Chan *chans) {
+  const Chan ch = chans[blockIdx.z];
+  if (!ch.tex) return;
+  const View v(arena, ch);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ch.ncol) return;
+  const int n = ch.nyu, ldc = ch.ldc, nc = ch.ncol;
+  for (int f = 0; f < 2; f++) {
+    const double2 *src = f == 0 ? v.u1 : v.u3;
+    v.texw[(size_t)(f * 2) * nc + c] = src[(size_t)1 * ldc + c];
+    v.texw[(size_t)(f * 2 + 1) * nc + c] = src[(size_t)(n - 2) * ldc + c];
+  }
+}
+
+hipError_t launch_tex_tables(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s) {
+  dim3 g((m.ncol + 63) / 64, 1, m.nchan);
+  hipLaunchKernelGGL(k_tex_tables, g, dim3(64), 0, s, arena, d_chans);
+  return hipGetLastError();
+}
+
+static hipError_t tex_transforms(char *arena, const std::vector<TexGroup> &groups, int dir, hipStream_t s) {
+  for (const TexGroup &g : groups) {
+    double *planes = reinterpret_cast<double *>(arena + g.o_texw);
+    const double2 *tz = reinterpret_cast<const double2 *>(arena + g.o_twz);
+    const double2 *tx = reinterpret_cast<const double2 *>(arena + g.o_twx);
+    hipError_t e = dir > 0 ? launch_plane_f2p(g.N1, g.N1, g.N2, planes, g.nplanes, tz, tx, s, false)
+                           : launch_plane_p2f(g.N1, g.N1, g.N2, planes, g.nplanes, tz, tx, s, false);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_tex_sources(char *arena, const Chan *d_chans, const ShapeMax &m,
+                              const std::vector<TexGroup> &groups, int kRK, int nit, hipStream_t s) {
+  const dim3 b(256), gc((m.ncol + 255) / 256, 1, m.nchan);
+  hipLaunchKernelGGL(k_tex_adj, dim3(gc.x, 2, m.nchan), b, 0, s, arena, d_chans, kRK);
+  for (int it = 0; it < nit; it++) {
+    hipLaunchKernelGGL(k_tex_pre, gc, b, 0, s, arena, d_chans, kRK, it);
+    hipError_t e = tex_transforms(arena, groups, +1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tex_mask, dim3((m.nphys + 255) / 256, 4, m.nchan), b, 0, s, arena, d_chans, kRK, 0);
+    e = tex_transforms(arena, groups, -1, s);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_tex_put, gc, b, 0, s, arena, d_chans, 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_tex_ghosts(char *arena, const Chan *d_chans, const ShapeMax &m,
+                             const std::vector<TexGroup> &groups, hipStream_t s) {
+  const dim3 b(256), gc((m.ncol + 255) / 256, 1, m.nchan);
+  hipLaunchKernelGGL(k_tex_get, gc, b, 0, s, arena, d_chans);
+  hipError_t e = tex_transforms(arena, groups, +1, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_tex_mask, dim3((m.nphys + 255) / 256, 4, m.nchan), b, 0, s, arena, d_chans, 1, 1);
+  e = tex_transforms(arena, groups, -1, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_tex_put, gc, b, 0, s, arena, d_chans, 1);
+  return hipGetLastError();
+}
+
+}  // namespace lh

@@ -195,6 +195,12 @@ module lhgpu
       integer(c_int), value :: reps
       integer(c_int) :: rc
     end function
+    function lh_set_texture(ctx, ch, bottom, top) bind(C, name='lh_set_texture') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, bottom, top
+      integer(c_int), value :: ch
+      integer(c_int) :: rc
+    end function
     function c_strlen(s) bind(C, name='strlen') result(n)
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s

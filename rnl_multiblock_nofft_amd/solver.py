@@ -124,6 +124,22 @@ class Solver:
         check(self.L.lh_time_kernels(self.h, int(nsub), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return ms
 
+    def set_texture(self, ch, bottom=None, top=None):
+        """Wall masks of channel ch, arrays of shape (N2, N1) with the codes
+        1 (no-slip) / 2 (free-shear); None leaves a wall smooth."""
+        keep = []
+
+        def arg(m):
+            if m is None:
+                return None
+            a = np.ascontiguousarray(m, dtype=np.int32)
+            c = self.cfgs[ch]
+            if a.shape != (c["N2"], c["N1"]):
+                raise ValueError("mask shape %s, expected (N2, N1) = %s" % (a.shape, (c["N2"], c["N1"])))
+            keep.append(a)
+            return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        check(self.L.lh_set_texture(self.h, int(ch), arg(bottom), arg(top)))
+
     def set_tuning(self, key, value):
         check(self.L.lh_set_tuning(self.h, int(key), int(value)))
 

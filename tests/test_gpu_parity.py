@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from helpers import init_fields, oracle_for, relerr, small_cfg
-from rnl_multiblock_nofft_amd import Solver, _lib, configs
+from rnl_multiblock_nofft_amd import Solver, _lib, configs, textures
 
 pytestmark = pytest.mark.gpu
 
@@ -159,6 +159,121 @@ def test_autotune_keeps_state():
     o.step(1)
     for nm in ("u1", "u2", "u3", "p"):
         assert relerr(s.download(0, nm), o.get(nm)) <= TOL, nm
+
+
+def profiles(f, nxh):
+    """Mean (mode (0,0), real part) and rms (all other modes; x-modes i > 0
+    counted twice for their conjugates) of a field, per y row.  Column
+    c = i + nxh k with nxh = N1/2+1."""
+    ncol = f.shape[0]
+    w = np.where(np.arange(ncol) % nxh == 0, 1.0, 2.0)
+    w[0] = 0.0
+    return f[0].real.copy(), np.sqrt((w[:, None] * np.abs(f) ** 2).sum(axis=0))
+
+
+def test_100_steps_mean_and_rms_profiles():
+    """The plain FP64 step against the oracle over 100 RK3 steps of the
+    32x32x153 smooth channel: mean-velocity and rms profiles within 1e-6."""
+    cfg = configs.smooth_32()[0]
+    say("100 steps")
+    s, o, f = make_pair(cfg, seed=2024)
+    s.step(100)
+    o.step(100)
+    for nm in ("u1", "u2", "u3"):
+        a, b = s.download(0, nm), o.get(nm)
+        nxh = cfg["N1"] // 2 + 1
+        ma, ra = profiles(a, nxh)
+        mb, rb = profiles(b, nxh)
+        # the mean of v vanishes by continuity: errors are measured against
+        # the field's own size (largest mean or rms value)
+        scale = max(np.abs(mb).max(), rb.max(), 1e-300)
+        em = np.abs(ma - mb).max() / scale
+        er = np.abs(ra - rb).max() / scale
+        say("100 steps", nm, "mean", em, "rms", er, "field", relerr(a, b))
+        assert em <= 1e-6 and er <= 1e-6, (nm, em, er)
+    assert abs(s.scalars(0)["t"] - o.scalars()["t"]) <= 1e-12
+
+
+def tex_pair(cfg, masks, seed=5):
+    s, o, f = make_pair(cfg, seed=seed)
+    s.set_texture(0, *masks)
+    o.set_texture(*masks)
+    return s, o
+
+
+TEX_CASES = {
+    "N16_ny33_stripes_posts": (small_cfg(16, 33), lambda: (textures.stripes(16, 16, 4, 0.5),
+                                                           textures.posts(16, 16, 8, 4))),
+    "N32_ny153_ridges_both": (configs.smooth_32()[0], lambda: (textures.stripes(32, 32, 8, 0.5),
+                                                               textures.stripes(32, 32, 8, 0.5))),
+    "N16_ny33_bottom_only_ctP": (small_cfg(16, 33, grid_kind=0, flag_ctpress=1, mpgx=-2.0 / 4200.0),
+                                 lambda: (textures.posts(16, 16, 4, 2), None)),
+}
+
+
+@pytest.mark.parametrize("name", list(TEX_CASES))
+def test_textured_step(name):
+    cfg, mk = TEX_CASES[name]
+    say("texture", name)
+    s, o = tex_pair(cfg, mk())
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say(name, nm, e)
+        assert e <= TOL, (nm, e)
+    s.step(2)
+    o.step(2)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say(name, "3 steps", nm, e)
+        assert e <= 1e-11, (nm, e)
+
+
+def test_texture_all_no_slip_is_smooth():
+    cfg = CASES["N16_ny33_poly_ctP"]
+    s, o, f = make_pair(cfg, seed=12)
+    ns = np.ones((16, 16), np.int32)
+    s.set_texture(0, ns, ns)
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        assert e <= TOL, (nm, e)
+
+
+def test_ensemble_textured_and_smooth():
+    cfgs = [small_cfg(16, 33), small_cfg(8, 17), small_cfg(16, 33), small_cfg(16, 25)]
+    masks = [(textures.stripes(16, 16, 2, 0.5), None), None,
+             (textures.posts(16, 16, 8, 4), textures.stripes(16, 16, 4, 0.25)), (None, textures.posts(16, 16, 4, 2))]
+    s = Solver(cfgs)
+    oracles = []
+    for q, c in enumerate(cfgs):
+        o = oracle_for(c)
+        f = init_fields(c, s.grid(q), seed=300 + q)
+        for name in ("u1", "u2", "u3", "p"):
+            s.upload(q, name, f[name])
+            o.set(name, f[name])
+        if masks[q] is not None:
+            s.set_texture(q, *masks[q])
+            o.set_texture(*masks[q])
+        o.init_flowrate()
+        oracles.append(o)
+    s.init_flowrate()
+    s.step(2)
+    for q, o in enumerate(oracles):
+        o.step(2)
+        for nm in ("u1", "u2", "u3", "p"):
+            e = relerr(s.download(q, nm), o.get(nm))
+            say("tex chan", q, nm, e)
+            assert e <= 1e-11, (q, nm, e)
+
+
+def test_bad_texture_code_raises():
+    s = Solver([small_cfg(8, 17)])
+    m = np.full((8, 8), 3, np.int32)
+    with pytest.raises(_lib.LhError):
+        s.set_texture(0, m, None)
 
 
 def test_ensemble_uneven():
