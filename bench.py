@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--workload", default="ridges_64x64x153")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--smooth", action="store_true",
+                    help="drop the workload's wall textures (smooth no-slip walls)")
     ap.add_argument("--kernel-substeps", type=int, default=30)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB, 5 times (calibrates "
@@ -121,6 +123,8 @@ def main():
         print(json.dumps(dict(copy_probe_bytes=n * 8, launches=5)), flush=True)
         return
     cfgs = configs.WORKLOADS[args.workload]()
+    if args.smooth:
+        cfgs = configs.smooth(cfgs)
     s = Solver(cfgs, precision=args.precision)
     for q, c in enumerate(cfgs):
         kmax = min(8, c["N1"] // 2 - 1, c["N2"] // 2 - 1)
@@ -167,7 +171,9 @@ def main():
                dtype="f64" if args.precision == "fp64" else "f32-fft/f64-solve", data="synthetic",
                config=dict(workload=args.workload, channels=len(cfgs),
                            grids=sorted({"%dx%dx%d" % (c["N1"], c["N2"], c["ny"]) for c in cfgs}),
-                           precision=args.precision, walls="smooth", dt=cfgs[0]["dt"],
+                           precision=args.precision,
+                           walls="textured" if any(c.get("texture") is not None for c in cfgs) else "smooth",
+                           dt=cfgs[0]["dt"],
                            grid_points=gp, hbm_pct_of_peak_model=round(hbm_pct, 3),
                            tuning=tuning),
                roofline=roofline, cpu_baseline=cpu)

@@ -9,6 +9,10 @@ runs on a 48x48 physical grid.
 """
 import copy
 
+import numpy as np
+
+from . import textures
+
 BASE = dict(alp=1.0, bet=2.0, grid_kind=1, gamma=2.0, dyq=10.0, ppp=5,
             bslpu1=0.0, bslpu3=0.0, flag_ctpress=0, mpgx=0.0, dt=1e-3)
 
@@ -24,26 +28,52 @@ def re_code(re_bulk):
     return 1.5 * re_bulk
 
 
-# The five workloads (smooth walls; see DESIGN.md for textures)
+# The workloads.  Textures follow the configurations they model: free-shear
+# / no-slip masks on both walls (textures.py), kept in cfg["texture"].
 def smooth_32():
+    """Smooth-wall 32x32x153 channel, Re_b = 2800."""
     return [channel(32, 32, 153, re_code(2800.0))]
 
 
 def ridges_64():
-    return [channel(64, 64, 153, re_code(2800.0))]
+    """64x64x153, Re_b = 2800, both walls with streamwise ridges: alternating
+    free-slip / no-slip z-stripes of period Lz/8, gas fraction 0.5."""
+    m = textures.stripes(64, 64, 8, 0.5)
+    return [channel(64, 64, 153, re_code(2800.0), texture=(m, m))]
 
 
 def ensemble_64():
-    return [channel(32, 32, 153, re_code(2800.0)) for _ in range(48)] + \
-           [channel(64, 64, 153, re_code(2800.0)) for _ in range(16)]
+    """48 channels of 32x32x153 and 16 of 64x64x153, each with square posts
+    at a spacing drawn from 2..8 points and a gas fraction from U(0.3, 0.9),
+    seeded per channel (seed 1234 + channel)."""
+    out = []
+    for q in range(64):
+        N = 32 if q < 48 else 64
+        rng = np.random.default_rng(1234 + q)
+        spacing = int(rng.integers(2, 9))
+        phi = float(rng.uniform(0.3, 0.9))
+        m = textures.posts_for_gas_fraction(N, N, spacing, phi)
+        out.append(channel(N, N, 153, re_code(2800.0), texture=(m, m)))
+    return out
 
 
 def large_256():
-    return [channel(256, 256, 257, re_code(10000.0))]
+    """256x256x257, Re_b = 10000, square posts of 8x8 points at spacing 16
+    (gas fraction 0.75) on both walls."""
+    m = textures.posts(256, 256, 16, 8)
+    return [channel(256, 256, 257, re_code(10000.0), texture=(m, m))]
 
 
 def fine_128():
     return [channel(128, 128, 153, re_code(2800.0))]
+
+
+def smooth(cfgs):
+    """The same channels with smooth walls."""
+    out = [dict(c) for c in cfgs]
+    for c in out:
+        c.pop("texture", None)
+    return out
 
 
 WORKLOADS = {"smooth_32x32x153": smooth_32, "ridges_64x64x153": ridges_64,

@@ -219,7 +219,8 @@ struct lh_ctx {
   int device = 0;
   std::vector<TexGroup> tex_groups;  // N-grid work planes per (N1, N2) shape
   int ntex = 0;                      // channels with a textured wall
-  int tex_nit = 4;                   // wall-source iterations per substep
+  int tex_nit = 3;                   // wall-source iterations per substep
+  int tex_fused = 1;                 // LDS-resident wall transforms when they fit
 };
 
 namespace {
@@ -398,10 +399,22 @@ int check_ctx(lh_ctx *x) {
 // column solves of a substep; with textured walls, the wall sources before
 // and the masked ghost rows after
 int solve_phase(lh_ctx *x, int kRK) {
-  if (x->ntex)
-    HIPCHK(launch_tex_sources(x->arena, x->d_chans, x->shape, x->tex_groups, kRK, x->tex_nit, x->stream));
+  // each field's wall planes in one workgroup's LDS when they fit
+  const size_t lds = tex_fused_bytes(x->tex_groups);
+  const bool fused = x->tex_fused && lds <= 160 * 1024;
+  if (x->ntex) {
+    if (fused)
+      HIPCHK(launch_tex_sources_fused(x->arena, x->d_chans, x->shape, lds, kRK, x->tex_nit, x->stream));
+    else
+      HIPCHK(launch_tex_sources(x->arena, x->d_chans, x->shape, x->tex_groups, kRK, x->tex_nit, x->stream));
+  }
   HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
-  if (x->ntex) HIPCHK(launch_tex_ghosts(x->arena, x->d_chans, x->shape, x->tex_groups, x->stream));
+  if (x->ntex) {
+    if (fused)
+      HIPCHK(launch_tex_ghosts_fused(x->arena, x->d_chans, x->shape, lds, x->stream));
+    else
+      HIPCHK(launch_tex_ghosts(x->arena, x->d_chans, x->shape, x->tex_groups, x->stream));
+  }
   return LH_OK;
 }
 
@@ -479,7 +492,11 @@ int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
       off += al<double2>((size_t)g.nplanes * (g.N1 / 2 + 1) * g.N2);
       int q = 0;
       for (auto &h : x->ch)
-        if (h.cfg.N1 == g.N1 && h.cfg.N2 == g.N2) h.dev.o_texw = g.o_texw + (size_t)4 * (q++) * h.ncol * sizeof(double2);
+        if (h.cfg.N1 == g.N1 && h.cfg.N2 == g.N2) {
+          h.dev.o_texw = g.o_texw + (size_t)4 * (q++) * h.ncol * sizeof(double2);
+          h.dev.o_twzN = g.o_twz;
+          h.dev.o_twxN = g.o_twx;
+        }
     }
     e = hipMalloc((void **)&x->arena, off);
     if (e != hipSuccess) rc = set_err(LH_ENOMEM, "hipMalloc(%zu): %s", off, hipGetErrorString(e));

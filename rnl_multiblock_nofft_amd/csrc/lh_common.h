@@ -62,6 +62,7 @@ struct Chan {
   // two inverse rows
   int tex;
   size_t o_mask, o_wtab, o_texu, o_texw, o_iscal;
+  size_t o_twzN, o_twxN;  // N2- and N1-point twiddles of the N-grid
   // Thomas factors of the fixed-dt operators, [row][ldc] doubles each:
   // o_fac + ((kRK-1)*4 + m) * nyu*ldc*8 for m = 0 dU, 1 cU, 2 dV, 3 cV
   // (U/W and V Helmholtz, LU_dec order), then o_fac + 12*nyu*ldc*8 dP and

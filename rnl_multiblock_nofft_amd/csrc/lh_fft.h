@@ -163,6 +163,9 @@ __device__ __forceinline__ C *fft_lds_s(C *x, C *y, int n, int nb, int sstr, con
     case 192: return fft_ct_run<C, 192, 192, 1, SIGN>(x, y, nb, sstr, tw);
     case 384: return fft_ct_run<C, 384, 384, 1, SIGN>(x, y, nb, sstr, tw);
     case 768: return fft_ct_run<C, 768, 768, 1, SIGN>(x, y, nb, sstr, tw);
+    case 8: return fft_ct_run<C, 8, 8, 1, SIGN>(x, y, nb, sstr, tw);
+    case 16: return fft_ct_run<C, 16, 16, 1, SIGN>(x, y, nb, sstr, tw);
+    case 32: return fft_ct_run<C, 32, 32, 1, SIGN>(x, y, nb, sstr, tw);
     case 64: return fft_ct_run<C, 64, 64, 1, SIGN>(x, y, nb, sstr, tw);
     case 128: return fft_ct_run<C, 128, 128, 1, SIGN>(x, y, nb, sstr, tw);
     case 256: return fft_ct_run<C, 256, 256, 1, SIGN>(x, y, nb, sstr, tw);

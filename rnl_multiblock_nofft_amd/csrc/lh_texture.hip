@@ -25,7 +25,10 @@
 //
 // U_b and U_t come from rows 1 and ny-1 of A^-1 (tables built once), which
 // decay exponentially away from their wall: only the first rows are summed.
+#include <algorithm>
+
 #include "lh_common.h"
+#include "lh_fft.h"
 #include "lh_kernels.h"
 
 namespace lh {
@@ -251,6 +254,148 @@ hipError_t launch_tex_ghosts(char *arena, const Chan *d_chans, const ShapeMax &m
   e = tex_transforms(arena, groups, -1, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_tex_put, gc, b, 0, s, arena, d_chans, 1);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// The same iteration with one wall plane of one field resident in LDS:
+// one workgroup per (field, wall, channel) runs every pass -- Fourier
+// update, inverse z and x transforms, mask, forward transforms -- without
+// leaving LDS.  The two walls are solved independently here: their coupling
+// through the interior, gbt and gtb, is the product of ny pivots of order
+// beta/dy^2 and vanishes in FP64 for every grid this solver takes (the
+// oracle keeps it and checks that).  Layouts: Fourier plane [i][z] (z
+// contiguous for the z transforms); physical rows in pairs [q][x] holding
+// rows 2q + i*(2q+1).
+// mode 0: wall source into the wall row of r (nit passes);
+// mode 1: masked ghost row added to u (one pass).
+__host__ __device__ inline size_t tex_fused_lds(int N1, int N2) {
+  return (2 * (size_t)(N1 / 2 + 1) * N2 + N1 + N2) * sizeof(double2) + (size_t)N1 * N2;
+}
+
+__global__ void __launch_bounds__(1024) k_tex_fused(char *arena, const Chan *chans, int kRK, int nit, int mode) {
+  const Chan ch = chans[blockIdx.z];
+  const int f = blockIdx.x >> 1, w = blockIdx.x & 1;  // field (u1, u3), wall (bottom, top)
+  if (!(ch.tex & (1 << w))) return;
+  const View v(arena, ch);
+  const int N1 = ch.N1, N2 = ch.N2, nxh = ch.nxh, nc = ch.ncol, n = ch.nyu, ldc = ch.ldc;
+  const int h = N1 / 2;
+  const int nt = blockDim.x, tid = threadIdx.x;
+  extern __shared__ double2 smem_d[];
+  double2 *A = smem_d, *B = A + (size_t)nxh * N2;
+  double2 *TZ = B + (size_t)nxh * N2, *TX = TZ + N2;
+  unsigned char *FS = reinterpret_cast<unsigned char *>(TX + N1);  // 1 on free-shear points
+  stage_twiddles<double2>(TZ, reinterpret_cast<const double2 *>(arena + ch.o_twzN), N2);
+  stage_twiddles<double2>(TX, reinterpret_cast<const double2 *>(arena + ch.o_twxN), N1);
+  {
+    const int *mk = v.mask + (size_t)w * N1 * N2;
+    for (int t = tid; t < N1 * N2; t += nt) FS[t] = mk[t] == 2;
+  }
+  const size_t tab = (size_t)n * ldc;
+  const double *Wr = v.wtab + (size_t)((kRK - 1) * 2 + w) * tab;
+  const int gj = w == 0 ? 0 : n - 1;  // g = W[gj]: response of this row to this wall's source
+  const double g0 = Wr[(size_t)gj * ldc];
+  const double gw = w == 0 ? ch.gw1 : ch.gw2;
+  const double Pw = 1.0 + gw, cfs = mode == 0 ? Pw / (1.0 - g0 * Pw) : 1.0 - (f == 0 ? ch.gwbc1 : ch.gwbc3);
+  const double2 *src = f == 0 ? v.u1 : v.u3;
+  double2 *S = A;  // Fourier plane [i][z], the current source
+
+  const int passes = mode == 0 ? nit : 1;
+  for (int it = 0; it < passes; it++) {
+    for (int c = tid; c < nc; c += nt) {
+      const int i = c % nxh, k = c / nxh;
+      const size_t o = (size_t)i * N2 + k;
+      double2 val;
+      if (mode == 0) {
+        const double2 U = v.texu[(size_t)(f * 2 + w) * nc + c];
+        const double dg = Wr[(size_t)gj * ldc + c] - g0;
+        const double2 sg = it > 0 ? S[o] : make_double2(0, 0);
+        val = make_double2(U.x + dg * sg.x, U.y + dg * sg.y);
+      } else {
+        val = src[(size_t)(w == 0 ? 1 : n - 2) * ldc + c];
+      }
+      S[o] = val;
+    }
+    __syncthreads();
+    // inverse z transforms of the nxh columns
+    double2 *R = fft_lds<double2>(S, S == A ? B : A, N2, nxh, N2, +1, TZ);
+    double2 *X = R == A ? B : A;
+    // Hermitian extension in x, two rows per complex sequence (rft inverse:
+    // imaginary parts of modes 0 and N1/2 ignored)
+    for (int t = tid; t < (N2 / 2) * N1; t += nt) {
+      const int m = t % N1, q = t / N1;
+      const int mm = m <= h ? m : N1 - m;
+      const double sg = m <= h ? 1.0 : -1.0;
+      double2 a = R[(size_t)mm * N2 + 2 * q], b = R[(size_t)mm * N2 + 2 * q + 1];
+      a.y *= sg;
+      b.y *= sg;
+      if (mm == 0 || mm == h) {
+        a.y = 0.0;
+        b.y = 0.0;
+      }
+      X[t] = make_double2(a.x - b.y, a.y + b.x);
+    }
+    __syncthreads();
+    double2 *Y = fft_lds<double2>(X, R, N1, N2 / 2, N1, +1, TX);
+    // pointwise mask in physical space: re -> row 2q, im -> row 2q+1
+    for (int t = tid; t < (N2 / 2) * N1; t += nt) {
+      const int x = t % N1, q = t / N1;
+      const double2 val = Y[t];
+      const double ca = FS[(2 * q) * N1 + x] ? cfs : 0.0, cb = FS[(2 * q + 1) * N1 + x] ? cfs : 0.0;
+      Y[t] = make_double2(ca * val.x, cb * val.y);
+    }
+    __syncthreads();
+    double2 *Z = fft_lds<double2>(Y, Y == A ? B : A, N1, N2 / 2, N1, -1, TX);
+    double2 *F = Z == A ? B : A;
+    // separate the two rows' half spectra (1/N1; imaginary parts of modes 0
+    // and N1/2 zero) into the [i][z] layout
+    const double inv = 1.0 / N1;
+    for (int t = tid; t < (N2 / 2) * (h + 1); t += nt) {
+      const int m = t % (h + 1), q = t / (h + 1);
+      const double2 Ym = Z[(size_t)q * N1 + m], Yc = Z[(size_t)q * N1 + (m == 0 ? 0 : N1 - m)];
+      double2 Pa, Pb;
+      if (m == 0 || m == h) {
+        Pa = make_double2(0.5 * (Ym.x + Yc.x) * inv, 0.0);
+        Pb = make_double2(0.5 * (Ym.y + Yc.y) * inv, 0.0);
+      } else {
+        Pa = make_double2(0.5 * (Ym.x + Yc.x) * inv, 0.5 * (Ym.y - Yc.y) * inv);
+        Pb = make_double2(0.5 * (Ym.y + Yc.y) * inv, -0.5 * (Ym.x - Yc.x) * inv);
+      }
+      F[(size_t)m * N2 + 2 * q] = Pa;
+      F[(size_t)m * N2 + 2 * q + 1] = Pb;
+    }
+    __syncthreads();
+    double2 *G = fft_lds<double2>(F, Z, N2, nxh, N2, -1, TZ);
+    const double invz = 1.0 / N2;
+    for (int t = tid; t < nxh * N2; t += nt) G[t] = make_double2(G[t].x * invz, G[t].y * invz);
+    __syncthreads();
+    S = G;
+  }
+  // wall row: source into r (mode 0), or ghost added to u (mode 1)
+  double2 *dst = (mode == 0 ? (f == 0 ? v.r1 : v.r3) : (f == 0 ? v.u1 : v.u3)) + (w == 0 ? 0 : (size_t)(n - 1) * ldc);
+  for (int c = tid; c < nc; c += nt) {
+    const int i = c % nxh, k = c / nxh;
+    const double2 val = S[(size_t)i * N2 + k];
+    dst[c] = mode == 0 ? val : cadd(dst[c], val);
+  }
+}
+
+size_t tex_fused_bytes(const std::vector<TexGroup> &groups) {
+  size_t b = 0;
+  for (const TexGroup &g : groups) b = std::max(b, tex_fused_lds(g.N1, g.N2));
+  return b;
+}
+
+hipError_t launch_tex_sources_fused(char *arena, const Chan *d_chans, const ShapeMax &m, size_t lds, int kRK,
+                                    int nit, hipStream_t s) {
+  const dim3 gc((m.ncol + 255) / 256, 2, m.nchan);
+  hipLaunchKernelGGL(k_tex_adj, gc, dim3(256), 0, s, arena, d_chans, kRK);
+  hipLaunchKernelGGL(k_tex_fused, dim3(4, 1, m.nchan), dim3(1024), lds, s, arena, d_chans, kRK, nit, 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_tex_ghosts_fused(char *arena, const Chan *d_chans, const ShapeMax &m, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL(k_tex_fused, dim3(4, 1, m.nchan), dim3(1024), lds, s, arena, d_chans, 1, 1, 1);
   return hipGetLastError();
 }
 

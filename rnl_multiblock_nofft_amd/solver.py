@@ -46,6 +46,11 @@ class Solver:
         check(L.lh_ctx_create(arr, n, ctypes.byref(h)))
         self.h = h
         self.L = L
+        # wall textures carried by the configurations: cfg["texture"] =
+        # (bottom mask, top mask), see textures.py
+        for q, c in enumerate(self.cfgs):
+            if c.get("texture") is not None:
+                self.set_texture(q, *c["texture"])
 
     def close(self):
         if getattr(self, "h", None):

@@ -10,7 +10,10 @@ ORACLE_KEYS = {f for f, _ in po.Cfg._fields_}
 def oracle_for(cfg):
     kw = {k: v for k, v in cfg.items() if k in ORACLE_KEYS}
     kw.setdefault("fixed_dt", 1)
-    return po.Oracle(**kw)
+    o = po.Oracle(**kw)
+    if cfg.get("texture") is not None:
+        o.set_texture(*cfg["texture"])
+    return o
 
 
 def relerr(a, b):
