@@ -24,6 +24,9 @@ sys.path.insert(0, ROOT)
 METRIC = "grid-pt RK3 substeps/s, FP64 plain vs mixed/batched/closure; HBM % of peak"
 HBM_PEAK_GBS = 8000.0
 KERNELS = ["k_zinv", "k_xprod", "k_zfwd", "k_rhs", "k_solve"]
+# the textured-wall kernels (k_tex_adj + k_tex_fused x 2 launches), timed as
+# one bucket; not a single kernel, so not a roofline candidate
+TEX = "k_tex"
 
 
 def algorithmic_bytes(cfg, precision):
@@ -45,7 +48,10 @@ def algorithmic_bytes(cfg, precision):
     zfwd = gx + (3 * nplu + 3 * nplv) * row
     rhs = ((2 * nyu + nyv + (nyu - 2)) + (2 * nplu + nplv) * 2 + (2 * nyv + nplu)) * row \
         + (2 * nyu + nyv + 2 * nplu + nplv) * row
-    solve = ((2 * nyu + nyv) + (nyu - 2)) * row + ((2 * nyu + nyv) + 2 * (nyu - 2)) * row
+    # reads r1, r3, r2, p and the pivots of the U/W, V and Poisson operators
+    # (nyu doubles per column each); writes u1, u3, u2, p, psi, div
+    ne = nyu - 2
+    solve = (2 * nyu + nyv + ne) * row + 3 * nyu * ncol * 8 + (2 * nyu + nyv + 3 * ne) * row
     return dict(k_zinv=zinv, k_xprod=xprod, k_zfwd=zfwd, k_rhs=rhs, k_solve=solve)
 
 
@@ -55,6 +61,20 @@ def baseline_model_bytes(cfg, precision):
     S = (N1 // 2 + 1) * N2 * ny * 16
     F = (cfg["Ngal1"] + 2) * cfg["Ngal2"] * ny * (4 if precision == "mixed" else 8)
     return 78 * S + 57 * F
+
+
+def measured_traffic(workload, precision, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what
+    the counter reads for a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_hbm_traffic.md" % (workload, precision))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == kernel:
+            return int(f[2])
+    return None
 
 
 def cpu_baseline(cfg, seconds=12.0, max_sub=60):
@@ -95,6 +115,9 @@ def main():
     ap.add_argument("--workload", default="ridges_64x64x153")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", default="",
+                    help="fixed launch shapes instead of autotuning, e.g. "
+                         "zinv_points=768,xprod_points=384,zfwd_points=384,solve_cols=3")
     ap.add_argument("--smooth", action="store_true",
                     help="drop the workload's wall textures (smooth no-slip walls)")
     ap.add_argument("--kernel-substeps", type=int, default=30)
@@ -110,7 +133,7 @@ def main():
 
     import torch
 
-    from rnl_multiblock_nofft_amd import Solver, configs, initial
+    from rnl_multiblock_nofft_amd import Solver, _lib, configs, initial
 
     torch.cuda.set_device(0)
     if args.copy_probe_mib:
@@ -132,7 +155,15 @@ def main():
         for name in ("u1", "u2", "u3", "p"):
             s.upload(q, name, f[name])
     s.init_flowrate()
-    tuning = s.autotune(10)
+    if args.tune:
+        keys = {"zinv_points": _lib.TUNE_ZINV_POINTS, "xprod_points": _lib.TUNE_XPROD_POINTS,
+                "zfwd_points": _lib.TUNE_ZFWD_POINTS, "solve_cols": _lib.TUNE_SOLVE_COLS}
+        for kv in args.tune.split(","):
+            k, v = kv.split("=")
+            s.set_tuning(keys[k], int(v))
+        tuning = s.tuning()
+    else:
+        tuning = s.autotune(10)
     s.step(args.warmup)
     s.sync()
     torch.cuda.synchronize()
@@ -152,13 +183,18 @@ def main():
     for c in cfgs:
         for k, v in algorithmic_bytes(c, args.precision).items():
             alg[k] += v
-    dom = KERNELS[int(max(range(5), key=lambda q: ms[q]))]
+    dom = KERNELS[int(max(range(5), key=lambda q: ms[q]))]  # the largest single kernel
     dom_ms = float(ms[KERNELS.index(dom)])
     achieved = alg[dom] / (dom_ms * 1e-3) / 1e9
+    kname = dom
+    if dom == "k_solve" and tuning.get("solve_cols", 0) > 0:
+        kname = "k_solve_lds"  # the LDS-staged form the autotuner chose
+    traffic = measured_traffic(args.workload, args.precision, kname)
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel=dom,
+                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic, kernel=kname,
                     kernel_ms=round(dom_ms, 5), alg_bytes=alg[dom],
-                    kernels_ms={k: round(float(ms[q]), 5) for q, k in enumerate(KERNELS)})
+                    kernels_ms=dict({k: round(float(ms[q]), 5) for q, k in enumerate(KERNELS)},
+                                    **{TEX: round(float(ms[5]), 5)}))
     bmodel = sum(baseline_model_bytes(c, args.precision) for c in cfgs)
     hbm_pct = 100.0 * bmodel * (3 * args.steps / elapsed) / (HBM_PEAK_GBS * 1e9)
 

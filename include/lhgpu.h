@@ -114,11 +114,12 @@ int lh_substep(lh_ctx *ctx, int kRK);
 int lh_step(lh_ctx *ctx, int nsteps);
 int lh_sync(lh_ctx *ctx);
 /* Measurement: run `nsub` substeps (kRK cycling 1,2,3 from kRK = 1) with the
- * five kernels of a substep launched one by one between HIP events on the
- * context's stream; ms_out[0..4] receive the summed milliseconds of the
- * z-inverse, x-product, z-forward, right-hand-side and column-solve kernels.
- * Advances the state like lh_substep.  No counterpart in the reference. */
-int lh_time_kernels(lh_ctx *ctx, int nsub, double *ms_out5);
+ * kernels of a substep launched one by one between HIP events on the
+ * context's stream; ms_out[0..5] receive the summed milliseconds of the
+ * z-inverse, x-product, z-forward, right-hand-side and column-solve kernels
+ * and of the textured-wall kernels (0 without textures).  Advances the state
+ * like lh_substep.  No counterpart in the reference. */
+int lh_time_kernels(lh_ctx *ctx, int nsub, double *ms_out6);
 
 /* Phases of a substep, for diagnostics and per-phase parity checks:
  * LH_PHASE_NONLINEAR: interp_u/v + modes_to_planes + ops_in_planes2 +

@@ -289,7 +289,8 @@ int upload_chan_constants(lh_ctx *x, HostChan &h) {
 size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
   const lh_config &c = h.cfg;
   const size_t fld = (size_t)(h.nyu + PADROWS) * h.ldc;  // complex elements per field (with padding rows)
-  const size_t ninter = (size_t)h.nslot * c.Ngal2 * h.nxh;
+  const int nxp = (h.nxh + 15) / 16 * 16;
+  const size_t ninter = (size_t)h.nslot * c.Ngal2 * nxp;
   Chan &d = h.dev;
   memset(&d, 0, sizeof d);
   size_t off = base;
@@ -340,6 +341,7 @@ size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
   d.nyu = h.nyu;
   d.nyv = h.nyv;
   d.nxh = h.nxh;
+  d.nxp = nxp;
   d.ncol = h.ncol;
   d.ldc = h.ldc;
   d.nplu = h.nyu - 2;
@@ -396,26 +398,33 @@ int check_ctx(lh_ctx *x) {
   return LH_OK;
 }
 
-// column solves of a substep; with textured walls, the wall sources before
-// and the masked ghost rows after
-int solve_phase(lh_ctx *x, int kRK) {
-  // each field's wall planes in one workgroup's LDS when they fit
+// textured walls: the wall sources before the column solves (part 0) and
+// the masked ghost rows after them (part 1); each field's wall planes stay in
+// one workgroup's LDS when they fit
+int tex_part(lh_ctx *x, int kRK, int part) {
+  if (!x->ntex) return LH_OK;
   const size_t lds = tex_fused_bytes(x->tex_groups);
   const bool fused = x->tex_fused && lds <= 160 * 1024;
-  if (x->ntex) {
+  if (part == 0) {
     if (fused)
       HIPCHK(launch_tex_sources_fused(x->arena, x->d_chans, x->shape, lds, kRK, x->tex_nit, x->stream));
     else
       HIPCHK(launch_tex_sources(x->arena, x->d_chans, x->shape, x->tex_groups, kRK, x->tex_nit, x->stream));
-  }
-  HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
-  if (x->ntex) {
+  } else {
     if (fused)
       HIPCHK(launch_tex_ghosts_fused(x->arena, x->d_chans, x->shape, lds, x->stream));
     else
       HIPCHK(launch_tex_ghosts(x->arena, x->d_chans, x->shape, x->tex_groups, x->stream));
   }
   return LH_OK;
+}
+
+// column solves of a substep (with the texture parts around them)
+int solve_phase(lh_ctx *x, int kRK) {
+  int rc = tex_part(x, kRK, 0);
+  if (rc) return rc;
+  HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
+  return tex_part(x, kRK, 1);
 }
 
 int run_substep(lh_ctx *x, int kRK) {
@@ -719,9 +728,9 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
   int rc = check_ctx(x);
   if (rc) return rc;
   if (nsub < 1 || !ms) return set_err(LH_EINVAL, "lh_time_kernels: bad arguments");
-  hipEvent_t ev[6];
-  for (int q = 0; q < 6; q++) HIPCHK(hipEventCreate(&ev[q]));
-  for (int q = 0; q < 5; q++) ms[q] = 0.0;
+  hipEvent_t ev[8];
+  for (int q = 0; q < 8; q++) HIPCHK(hipEventCreate(&ev[q]));
+  for (int q = 0; q < 6; q++) ms[q] = 0.0;
   int kRK = 1;
   for (int n = 0; n < nsub && rc == LH_OK; n++) {
     HIPCHK(hipEventRecord(ev[0], x->stream));
@@ -731,22 +740,28 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
     }
     HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
     HIPCHK(hipEventRecord(ev[4], x->stream));
-    rc = solve_phase(x, kRK);
-    if (rc) break;
+    if ((rc = tex_part(x, kRK, 0))) break;
     HIPCHK(hipEventRecord(ev[5], x->stream));
-    HIPCHK(hipEventSynchronize(ev[5]));
-    for (int q = 0; q < 5; q++) {
-      float t = 0.f;
-      HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
-      ms[q] += t;
-    }
+    HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
+    HIPCHK(hipEventRecord(ev[6], x->stream));
+    if ((rc = tex_part(x, kRK, 1))) break;
+    HIPCHK(hipEventRecord(ev[7], x->stream));
+    HIPCHK(hipEventSynchronize(ev[7]));
+    float t[7];
+    for (int q = 0; q < 7; q++) HIPCHK(hipEventElapsedTime(&t[q], ev[q], ev[q + 1]));
+    ms[0] += t[0];
+    ms[1] += t[1];
+    ms[2] += t[2];
+    ms[3] += t[3];
+    ms[4] += t[5];
+    ms[5] += t[4] + t[6];
     if (kRK == 1)
       for (auto &h : x->ch) h.t += h.cfg.dt;
     if (kRK == 3)
       for (auto &h : x->ch) h.iter += 1;
     kRK = kRK % 3 + 1;
   }
-  for (int q = 0; q < 6; q++) (void)hipEventDestroy(ev[q]);
+  for (int q = 0; q < 8; q++) (void)hipEventDestroy(ev[q]);
   return rc;
 }
 

@@ -8,8 +8,10 @@
 //                    plane's spectral data is one contiguous slice (the
 //                    plane transpose is free) and column-wise solves read
 //                    row j of 64 consecutive columns in one coalesced access.
-//   z-spectra        hz[slot][3][Ngal2][nxh]   (nonlinear pipeline stage 1)
-//   x-half-spectra   gx[slot][4][Ngal2][nxh]   (stage 2)
+//   z-spectra        hz[slot][3][Ngal2][nxp]   (nonlinear pipeline stage 1)
+//   x-half-spectra   gx[slot][4][Ngal2][nxp]   (stage 2)
+//                    nxp = nxh rounded up to 16, so that tiles of 8 or 16
+//                    x-modes are whole 128-byte lines
 //   slot = j-1 for u-plane j = 1..ny-1, ny-1 + j-1 for v-plane j = 1..ny-2.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -38,6 +40,7 @@ enum { SC_MPGX = 0, SC_QXT, SC_QX, SC_DT, SC_T, SC_QX1, SC_NSCAL = 8 };  // SC_Q
 struct Chan {
   int N1, N2, Ng1, Ng2;
   int ny, nyu, nyv, nxh, ncol, ldc;
+  int nxp;  // row length of the hz/gx intermediates: nxh rounded up to 16
   int nplu, nplv, nslot;
   int flag_ctpress;
   double Re, alp, bet;

@@ -38,6 +38,12 @@ __host__ __device__ inline int tile_of(int E, int n, int cap) {
   if (t < 1) t = 1;
   return t < cap ? t : cap;
 }
+// the same for tiles of x-modes in the hz/gx rows: whole groups of 8 modes
+// (128-byte lines of the padded rows) when the tile is at least 8 wide
+__host__ __device__ inline int tile_of8(int E, int n, int cap) {
+  int t = tile_of(E, n, cap);
+  return t >= 8 && t < cap ? t / 8 * 8 : t;
+}
 
 template <typename C> struct Inter;
 template <> struct Inter<double2> {
@@ -60,8 +66,8 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
   const int f = blockIdx.z % 3;
   const int slot = blockIdx.y;
   if (slot >= ch.nslot) return;
-  const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc;
-  const int TI = tile_of(E, Ng2, nxh);
+  const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
+  const int TI = tile_of8(E, Ng2, nxh);
   const int i0 = blockIdx.x * TI;
   if (i0 >= nxh) return;
   const int ni = min(TI, nxh - i0);
@@ -120,10 +126,10 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
   }
   __syncthreads();
   C *R = fft_lds<C>(A, B, Ng2, ni, Ng2, +1, TW);
-  C *hz = Inter<C>::hz(v) + ((size_t)slot * 3 + f) * Ng2 * nxh;
+  C *hz = Inter<C>::hz(v) + ((size_t)slot * 3 + f) * Ng2 * nxp;
   for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) {
     int z = t / ni, ii = t - z * ni;
-    hz[(size_t)z * nxh + i0 + ii] = R[ii * Ng2 + z];
+    hz[(size_t)z * nxp + i0 + ii] = R[ii * Ng2 + z];
   }
 }
 
@@ -136,7 +142,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   const View v(arena, ch);
   const int slot = blockIdx.y;
   if (slot >= ch.nslot) return;
-  const int Ng1 = ch.Ng1, Ng2 = ch.Ng2, nxh = ch.nxh, h1 = ch.N1 / 2;
+  const int Ng1 = ch.Ng1, Ng2 = ch.Ng2, nxp = ch.nxp, h1 = ch.N1 / 2;
   const int TZ = tile_of(E, Ng1, Ng2);
   const int z0 = blockIdx.x * TZ;
   if (z0 >= Ng2) return;
@@ -147,7 +153,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   C *TW = B + 2 * TZ * Ng1;
   stage_twiddles<C>(TW, v.twx, Ng1);
   const bool isU = slot < ch.nplu;
-  const C *hz = Inter<C>::hz(v) + (size_t)slot * 3 * Ng2 * nxh;
+  const C *hz = Inter<C>::hz(v) + (size_t)slot * 3 * Ng2 * nxp;
 
   // Hermitian extension of the three half spectra, packed two per sequence:
   // seq 2zz = f0 + i f2, seq 2zz+1 = f1
@@ -156,9 +162,9 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
     int z = z0 + zz;
     C F0 = CT<C>::mk(0, 0), F1 = F0, F2 = F0;
     if (m <= h1) {
-      F0 = hz[((size_t)0 * Ng2 + z) * nxh + m];
-      F1 = hz[((size_t)1 * Ng2 + z) * nxh + m];
-      F2 = hz[((size_t)2 * Ng2 + z) * nxh + m];
+      F0 = hz[((size_t)0 * Ng2 + z) * nxp + m];
+      F1 = hz[((size_t)1 * Ng2 + z) * nxp + m];
+      F2 = hz[((size_t)2 * Ng2 + z) * nxp + m];
       if (m == 0) {  // rft ignores imag(cc(0)) on the inverse transform
         F0.y = 0;
         F1.y = 0;
@@ -166,9 +172,9 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
       }
     } else if (m >= Ng1 - h1) {
       int mm = Ng1 - m;
-      F0 = hz[((size_t)0 * Ng2 + z) * nxh + mm];
-      F1 = hz[((size_t)1 * Ng2 + z) * nxh + mm];
-      F2 = hz[((size_t)2 * Ng2 + z) * nxh + mm];
+      F0 = hz[((size_t)0 * Ng2 + z) * nxp + mm];
+      F1 = hz[((size_t)1 * Ng2 + z) * nxp + mm];
+      F2 = hz[((size_t)2 * Ng2 + z) * nxp + mm];
       F0.y = -F0.y;
       F1.y = -F1.y;
       F2.y = -F2.y;
@@ -194,7 +200,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   C *Y = fft_lds<C>(X, X == A ? B : A, Ng1, 2 * nz, Ng1, -1, TW);
   const int nseq = isU ? 2 : 1;
   const R_t inv = (R_t)(1.0 / Ng1);
-  C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxh;
+  C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxp;
   for (int t = threadIdx.x; t < nz * nseq * (h1 + 1); t += blockDim.x) {
     int m = t % (h1 + 1);
     int r = t / (h1 + 1);
@@ -211,8 +217,8 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
       Q = CT<C>::mk((R_t)0.5 * (Ym.y + Yc.y) * inv, -(R_t)0.5 * (Ym.x - Yc.x) * inv);
     }
     int z = z0 + zz;
-    gx[((size_t)(2 * q) * Ng2 + z) * nxh + m] = P;
-    gx[((size_t)(2 * q + 1) * Ng2 + z) * nxh + m] = Q;
+    gx[((size_t)(2 * q) * Ng2 + z) * nxp + m] = P;
+    gx[((size_t)(2 * q + 1) * Ng2 + z) * nxp + m] = Q;
   }
 }
 
@@ -224,8 +230,8 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
   const View v(arena, ch);
   const int slot = blockIdx.y;
   if (slot >= ch.nslot) return;
-  const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc;
-  const int TI = tile_of(E, Ng2, nxh);
+  const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
+  const int TI = tile_of8(E, Ng2, nxh);
   const int i0 = blockIdx.x * TI;
   if (i0 >= nxh) return;
   const int ni = min(TI, nxh - i0);
@@ -237,12 +243,12 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
   C *B = A + 4 * TI * Ng2;
   C *TW = B + 4 * TI * Ng2;
   stage_twiddles<C>(TW, v.twz, Ng2);
-  const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxh;
+  const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxp;
   for (int t = threadIdx.x; t < np * ni * Ng2; t += blockDim.x) {
     int p = t / (ni * Ng2);
     int rem = t - p * ni * Ng2;
     int z = rem / ni, ii = rem - z * ni;
-    A[(p * ni + ii) * Ng2 + z] = gx[((size_t)p * Ng2 + z) * nxh + i0 + ii];
+    A[(p * ni + ii) * Ng2 + z] = gx[((size_t)p * Ng2 + z) * nxp + i0 + ii];
   }
   __syncthreads();
   C *R = fft_lds<C>(A, B, Ng2, np * ni, Ng2, -1, TW);
@@ -274,11 +280,17 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
 
 // ------------------------------------------------------------------------
 // K4: right-hand sides
-__global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int kRK) {
+__global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int kRK, int gx, int gy) {
   const Chan ch = chans[blockIdx.z];
   const View v(arena, ch);
-  const int j = blockIdx.y;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-aware order: tiles (column block, row j) go round-robin to the 8
+  // XCDs; hand each XCD a contiguous run so that rows j-1, j, j+1 of a column
+  // block are read through one L2.  gridDim.x is gx*gy rounded up to 8.
+  const int per = gridDim.x / 8;
+  const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (tile >= gx * gy) return;
+  const int j = tile / gx;
+  const int c = (tile - j * gx) * blockDim.x + threadIdx.x;
   if (j >= ch.nyu || c >= ch.ncol) return;
   const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu, nyv = ch.nyv;
   const int i = c % nxh, k = c / nxh;
@@ -1148,9 +1160,9 @@ __global__ void k_from_rows(const double2 *__restrict__ src, double2 *__restrict
 // launch wrappers
 // LDS of the three stages, in complex elements: two ping-pong buffers and
 // the twiddle table
-static size_t lds_zinv(int E, int Ng2, int nxh) { return 2 * (size_t)tile_of(E, Ng2, nxh) * Ng2 + Ng2; }
+static size_t lds_zinv(int E, int Ng2, int nxh) { return 2 * (size_t)tile_of8(E, Ng2, nxh) * Ng2 + Ng2; }
 static size_t lds_xprod(int E, int Ng1, int Ng2) { return 2 * 2 * (size_t)tile_of(E, Ng1, Ng2) * Ng1 + Ng1; }
-static size_t lds_zfwd(int E, int Ng2, int nxh) { return 2 * 4 * (size_t)tile_of(E, Ng2, nxh) * Ng2 + Ng2; }
+static size_t lds_zfwd(int E, int Ng2, int nxh) { return 2 * 4 * (size_t)tile_of8(E, Ng2, nxh) * Ng2 + Ng2; }
 
 // tile counts and LDS sizes for the current targets, over every channel
 void shape_tiles(ShapeMax &m) {
@@ -1158,8 +1170,8 @@ void shape_tiles(ShapeMax &m) {
   m.lds_zinv = m.lds_xprod = m.lds_zfwd = 0;
   for (const ChanDims &d : m.dims) {
     const int nxh = d.N1 / 2 + 1;
-    const int ti = tile_of(m.e_zinv, d.Ng2, nxh), tx = tile_of(m.e_xprod, d.Ng1, d.Ng2),
-              tf = tile_of(m.e_zfwd, d.Ng2, nxh);
+    const int ti = tile_of8(m.e_zinv, d.Ng2, nxh), tx = tile_of(m.e_xprod, d.Ng1, d.Ng2),
+              tf = tile_of8(m.e_zfwd, d.Ng2, nxh);
     m.zinv_tiles = max(m.zinv_tiles, (nxh + ti - 1) / ti);
     m.xprod_tiles = max(m.xprod_tiles, (d.Ng2 + tx - 1) / tx);
     m.zfwd_tiles = max(m.zfwd_tiles, (nxh + tf - 1) / tf);
@@ -1216,8 +1228,9 @@ void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny) {
 }
 
 hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s) {
-  dim3 g((m.ncol + 255) / 256, m.nyu, m.nchan);
-  hipLaunchKernelGGL(k_rhs, g, dim3(256), 0, s, arena, d_chans, kRK);
+  const int gx = (m.ncol + 255) / 256, gy = m.nyu;
+  dim3 g((gx * gy + 7) / 8 * 8, 1, m.nchan);
+  hipLaunchKernelGGL(k_rhs, g, dim3(256), 0, s, arena, d_chans, kRK, gx, gy);
   return hipGetLastError();
 }
 

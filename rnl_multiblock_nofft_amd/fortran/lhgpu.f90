@@ -114,11 +114,11 @@ module lhgpu
       type(c_ptr), value :: ctx
       integer(c_int) :: rc
     end function
-    function lh_time_kernels(ctx, nsub, ms_out5) bind(C, name='lh_time_kernels') result(rc)
+    function lh_time_kernels(ctx, nsub, ms_out6) bind(C, name='lh_time_kernels') result(rc)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
       integer(c_int), value :: nsub
-      real(c_double), intent(out) :: ms_out5(5)
+      real(c_double), intent(out) :: ms_out6(6)
       integer(c_int) :: rc
     end function
     function lh_run_phase(ctx, phase, kRK) bind(C, name='lh_run_phase') result(rc)

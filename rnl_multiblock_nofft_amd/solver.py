@@ -123,9 +123,9 @@ class Solver:
         check(self.L.lh_run_phase(self.h, phase, kRK))
 
     def time_kernels(self, nsub):
-        """HIP-event times (ms, summed over nsub substeps) of the five kernels
-        [zinv, xprod, zfwd, rhs, solve]."""
-        ms = np.zeros(5)
+        """HIP-event times (ms, summed over nsub substeps) of the kernels
+        [zinv, xprod, zfwd, rhs, solve, texture]."""
+        ms = np.zeros(6)
         check(self.L.lh_time_kernels(self.h, int(nsub), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return ms
 
