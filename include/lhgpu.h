@@ -46,8 +46,11 @@ typedef struct lh_config {
   double bslpu1, bslpu3; /* Navier slip lengths of the ghost update */
   int flag_ctpress;    /* 0: constant flow rate; 1: constant pressure gradient */
   double mpgx;         /* mean pressure gradient (initial / prescribed) */
-  double dt;           /* fixed time step */
+  double dt;           /* time step (the initial one when adaptive_dt) */
   int precision;       /* LH_PREC_FP64 or LH_PREC_MIXED */
+  int adaptive_dt;     /* 0: dt fixed; 1: dt = min(2.5 dtv, CFL dtc) at every
+                          first substep (FOU3D.f90:288-297, dtc_calc :699-736) */
+  double CFL;          /* Courant number of the adaptive step */
 } lh_config;
 
 #define LH_PREC_FP64 0   /* everything in FP64 */
@@ -97,7 +100,8 @@ int lh_get_grid(lh_ctx *ctx, int ch, double *yu, double *yv, double *dthdyu,
 int lh_field_upload(lh_ctx *ctx, int ch, int field, const double *reim);
 int lh_field_download(lh_ctx *ctx, int ch, int field, double *reim);
 
-/* scalars: out[0..7] = t, dt, mpgx, QxT, Qx, dtv, 0, iter */
+/* scalars: out[0..7] = t, dt, mpgx, QxT, Qx, dtv, dtc, iter; dtc is the CFL
+ * step of the last first substep (dtc_calc, FOU3D.f90:699-736), 0 before */
 int lh_get_scalars(lh_ctx *ctx, int ch, double *out8);
 int lh_set_scalars(lh_ctx *ctx, int ch, double t, double mpgx, double QxT, int iter);
 /* QxT := flow rate of the current u1 mean mode (getini, start.f90:2166-2171;

@@ -36,7 +36,8 @@ class LhConfig(ctypes.Structure):
                 ("grid_kind", ctypes.c_int), ("dyq", ctypes.c_double), ("ppp", ctypes.c_int),
                 ("gamma", ctypes.c_double), ("bslpu1", ctypes.c_double),
                 ("bslpu3", ctypes.c_double), ("flag_ctpress", ctypes.c_int),
-                ("mpgx", ctypes.c_double), ("dt", ctypes.c_double), ("precision", ctypes.c_int)]
+                ("mpgx", ctypes.c_double), ("dt", ctypes.c_double), ("precision", ctypes.c_int),
+                ("adaptive_dt", ctypes.c_int), ("CFL", ctypes.c_double)]
 
 
 class LhError(RuntimeError):

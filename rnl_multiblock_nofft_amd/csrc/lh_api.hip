@@ -47,7 +47,6 @@ struct HostChan {
   std::vector<double2> twz, twx;
   double gw1, gw2, gwbc1, gwbc3, gwi1, gwi2, dtv;
   Chan dev;  // offsets into the context arena
-  double t = 0.0;
   int iter = 0;
 };
 
@@ -221,6 +220,7 @@ struct lh_ctx {
   int ntex = 0;                      // channels with a textured wall
   int tex_nit = 3;                   // wall-source iterations per substep
   int tex_fused = 1;                 // LDS-resident wall transforms when they fit
+  bool adaptive = false;             // a channel with an adaptive time step
 };
 
 namespace {
@@ -322,6 +322,9 @@ size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
   d.o_iscal = take(al<int>(4));
   d.o_wtab = take(al<double>(6 * (size_t)h.nyu * h.ldc));
   d.o_texu = take(al<double2>(4 * (size_t)h.ncol));
+  int ng2max = 0;
+  for (const ChanDims &q : x->shape.dims) ng2max = std::max(ng2max, q.Ng2);
+  d.o_cfl = take(al<double>(2 * (size_t)h.nslot * ng2max));
   if (x->mixed) {
     d.o_hzf = take(al<float2>(3 * ninter));
     d.o_gxf = take(al<float2>(4 * ninter));
@@ -348,6 +351,8 @@ size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
   d.nplv = h.nyv - 2;
   d.nslot = h.nslot;
   d.flag_ctpress = c.flag_ctpress;
+  d.adaptive = c.adaptive_dt != 0;
+  d.CFL = c.CFL;
   d.Re = c.Re;
   d.alp = c.alp;
   d.bet = c.bet;
@@ -361,6 +366,7 @@ int init_chan(lh_ctx *x, HostChan &h) {
   double sc[SC_NSCAL] = {0};
   sc[SC_MPGX] = h.cfg.mpgx;
   sc[SC_DT] = h.cfg.dt;
+  sc[SC_DTV] = h.dtv;
   HIPCHK(hipMemcpy(d.scal, sc, sizeof sc, hipMemcpyHostToDevice));
   return upload_chan_constants(x, h);
 }
@@ -427,8 +433,24 @@ int solve_phase(lh_ctx *x, int kRK) {
   return tex_part(x, kRK, 1);
 }
 
+// first substep: dtc and (adaptive channels) the new dt, then everything
+// that depends on dt -- pivots, mean-mode unit-forcing solutions, texture
+// inverse rows -- for the whole step
+int time_step(lh_ctx *x, int kRK) {
+  if (kRK != 1) return LH_OK;
+  HIPCHK(launch_dt_update(x->arena, x->d_chans, x->shape, x->stream));
+  if (x->adaptive) {
+    HIPCHK(launch_factors(x->arena, x->d_chans, x->shape, x->stream));
+    HIPCHK(launch_u11(x->arena, x->d_chans, x->shape, x->stream));
+    if (x->ntex) HIPCHK(launch_tex_tables(x->arena, x->d_chans, x->shape, x->stream));
+  }
+  return LH_OK;
+}
+
 int run_substep(lh_ctx *x, int kRK) {
   HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream));
+  int rc = time_step(x, kRK);
+  if (rc) return rc;
   HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
   return solve_phase(x, kRK);
 }
@@ -466,6 +488,10 @@ int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
   size_t stage = 0;
   for (int c = 0; c < nchan && rc == LH_OK; c++) {
     rc = setup_host(x->ch[c], cfgs[c]);
+    if (rc == LH_OK && cfgs[c].adaptive_dt) {
+      if (!(cfgs[c].CFL > 0.0)) rc = set_err(LH_EINVAL, "channel %d: adaptive_dt needs CFL > 0", c);
+      x->adaptive = true;
+    }
     if (rc == LH_OK) {
       shape_add(x->shape, cfgs[c].N1, cfgs[c].N2, cfgs[c].Ngal1, cfgs[c].Ngal2, cfgs[c].ny);
       size_t s = (size_t)x->ch[c].ncol * x->ch[c].nyu;
@@ -627,13 +653,13 @@ int lh_get_scalars(lh_ctx *x, int ch, double *o) {
   double sc[SC_NSCAL];
   HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(hipMemcpy(sc, View(x->arena, h.dev).scal, sizeof sc, hipMemcpyDeviceToHost));
-  o[0] = h.t;
+  o[0] = sc[SC_T];
   o[1] = sc[SC_DT];
   o[2] = sc[SC_MPGX];
   o[3] = sc[SC_QXT];
   o[4] = sc[SC_QX];
   o[5] = h.dtv;
-  o[6] = 0.0;
+  o[6] = sc[SC_DTC];
   o[7] = h.iter;
   return LH_OK;
 }
@@ -648,8 +674,8 @@ int lh_set_scalars(lh_ctx *x, int ch, double t, double mpgx, double QxT, int ite
   HIPCHK(hipMemcpy(sc, View(x->arena, h.dev).scal, sizeof sc, hipMemcpyDeviceToHost));
   sc[SC_MPGX] = mpgx;
   sc[SC_QXT] = QxT;
+  sc[SC_T] = t;
   HIPCHK(hipMemcpy(View(x->arena, h.dev).scal, sc, sizeof sc, hipMemcpyHostToDevice));
-  h.t = t;
   h.iter = iter;
   return LH_OK;
 }
@@ -667,7 +693,12 @@ int lh_run_phase(lh_ctx *x, int phase, int kRK) {
   if (rc) return rc;
   if (kRK < 1 || kRK > 3) return set_err(LH_EINVAL, "kRK must be 1..3");
   switch (phase) {
-    case LH_PHASE_NONLINEAR: HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream)); break;
+    case LH_PHASE_NONLINEAR: {
+      HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream));
+      int r = time_step(x, kRK);
+      if (r) return r;
+      break;
+    }
     case LH_PHASE_RHS: HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream)); break;
     case LH_PHASE_SOLVE: {
       int r = solve_phase(x, kRK);
@@ -676,8 +707,6 @@ int lh_run_phase(lh_ctx *x, int phase, int kRK) {
     }
     default: return set_err(LH_EINVAL, "unknown phase %d", phase);
   }
-  if (kRK == 1 && phase == LH_PHASE_NONLINEAR)
-    for (auto &h : x->ch) h.t += h.cfg.dt;
   HIPCHK(hipStreamSynchronize(x->stream));
   return LH_OK;
 }
@@ -688,8 +717,6 @@ int lh_substep(lh_ctx *x, int kRK) {
   if (kRK < 1 || kRK > 3) return set_err(LH_EINVAL, "kRK must be 1..3");
   rc = run_substep(x, kRK);
   if (rc) return rc;
-  if (kRK == 1)
-    for (auto &h : x->ch) h.t += h.cfg.dt;
   if (kRK == 3)
     for (auto &h : x->ch) h.iter += 1;
   return LH_OK;
@@ -717,10 +744,7 @@ int lh_step(lh_ctx *x, int nsteps) {
   }
   for (int n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(x->graph, x->stream));
   HIPCHK(hipGetLastError());
-  for (auto &h : x->ch) {
-    h.t += nsteps * h.cfg.dt;
-    h.iter += nsteps;
-  }
+  for (auto &h : x->ch) h.iter += nsteps;
   return LH_OK;
 }
 
@@ -738,6 +762,7 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
       HIPCHK(launch_nl_stage(x->arena, x->d_chans, x->shape, x->mixed, st, x->stream));
       HIPCHK(hipEventRecord(ev[st + 1], x->stream));
     }
+    if ((rc = time_step(x, kRK))) break;
     HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
     HIPCHK(hipEventRecord(ev[4], x->stream));
     if ((rc = tex_part(x, kRK, 0))) break;
@@ -755,8 +780,6 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
     ms[3] += t[3];
     ms[4] += t[5];
     ms[5] += t[4] + t[6];
-    if (kRK == 1)
-      for (auto &h : x->ch) h.t += h.cfg.dt;
     if (kRK == 3)
       for (auto &h : x->ch) h.iter += 1;
     kRK = kRK % 3 + 1;

@@ -31,7 +31,11 @@ __host__ __device__ inline double rk_c(int k) { return k == 1 ? 0.0 : (k == 2 ? 
 __host__ __device__ inline double rk_d(int k) { return k == 1 ? 8.0 / 15.0 : (k == 2 ? 2.0 / 15.0 : 1.0 / 3.0); }
 
 // per-channel device scalars (in HBM so that graph replays see updates)
-enum { SC_MPGX = 0, SC_QXT, SC_QX, SC_DT, SC_T, SC_QX1, SC_NSCAL = 8 };  // SC_QX1..+2: per kRK
+// SC_QX1..+2: flow rate of the unit-forcing solution per kRK; SC_DTC the CFL
+// time step of dtc_calc; SC_DTV the viscous one; SC_M1..M3 the bit patterns of
+// the CFL maxima max|u1|, max|v dthdyv|, max|u3| (non-negative doubles
+// compare as integers)
+enum { SC_MPGX = 0, SC_QXT, SC_QX, SC_DT, SC_T, SC_QX1, SC_DTC = 8, SC_DTV, SC_M1, SC_M2, SC_M3, SC_NSCAL = 16 };
 
 // Per-channel descriptor.  Every device array lives in one context-wide
 // arena; the descriptor holds byte offsets, and kernels derive typed pointers
@@ -43,6 +47,8 @@ struct Chan {
   int nxp;  // row length of the hz/gx intermediates: nxh rounded up to 16
   int nplu, nplv, nslot;
   int flag_ctpress;
+  int adaptive;  // dt = min(2.5 dtv, CFL dtc) at every kRK = 1 (else fixed)
+  double CFL;
   double Re, alp, bet;
   double gw1, gw2, gwbc1, gwbc3, gwi1, gwi2;
   // grid / wavenumbers: yu,yv,dthdyu,dthdyv [rows], dyu2i/dyv2i [3*j+m],
@@ -66,6 +72,7 @@ struct Chan {
   int tex;
   size_t o_mask, o_wtab, o_texu, o_texw, o_iscal;
   size_t o_twzN, o_twxN;  // N2- and N1-point twiddles of the N-grid
+  size_t o_cfl;           // [slot][tile][2] CFL maxima of k_xprod's workgroups
   // Thomas factors of the fixed-dt operators, [row][ldc] doubles each:
   // o_fac + ((kRK-1)*4 + m) * nyu*ldc*8 for m = 0 dU, 1 cU, 2 dV, 3 cV
   // (U/W and V Helmholtz, LU_dec order), then o_fac + 12*nyu*ldc*8 dP and
@@ -83,7 +90,7 @@ struct View {
   float2 *hzf, *gxf;
   double *fac;
   int *mask, *iscal;
-  double *wtab;
+  double *wtab, *cfl;
   double2 *texu, *texw;
   __host__ __device__ View(char *A, const Chan &c)
       : yu((const double *)(A + c.o_yu)), yv((const double *)(A + c.o_yv)),
@@ -101,6 +108,7 @@ struct View {
         scal((double *)(A + c.o_scal)), hz((double2 *)(A + c.o_hz)), gx((double2 *)(A + c.o_gx)),
         hzf((float2 *)(A + c.o_hzf)), gxf((float2 *)(A + c.o_gxf)), fac((double *)(A + c.o_fac)),
         mask((int *)(A + c.o_mask)), iscal((int *)(A + c.o_iscal)), wtab((double *)(A + c.o_wtab)),
+        cfl((double *)(A + c.o_cfl)),
         texu((double2 *)(A + c.o_texu)), texw((double2 *)(A + c.o_texw)) {}
 };
 

@@ -184,10 +184,19 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   }
   __syncthreads();
   C *X = fft_lds<C>(A, B, Ng1, 2 * nz, Ng1, +1, TW);
+  // CFL maxima of dtc_calc (FOU3D.f90:699-736): |u1|, |u3| on u-planes,
+  // |u2| on v-planes (times dthdyv(j) below)
+  R_t ma = 0, mb = 0;
   for (int t = threadIdx.x; t < nz * Ng1; t += blockDim.x) {
     int zz = t / Ng1, n = t - zz * Ng1;
     C s0 = X[(2 * zz) * Ng1 + n], s1 = X[(2 * zz + 1) * Ng1 + n];
     R_t a1 = s0.x, a3 = s0.y, a2 = s1.x;
+    if (isU) {
+      ma = fmax(ma, fabs(a1));
+      mb = fmax(mb, fabs(a3));
+    } else {
+      ma = fmax(ma, fabs(a2));
+    }
     if (isU) {
       X[(2 * zz) * Ng1 + n] = CT<C>::mk(a1 * a1, a1 * a3);      // uu + i uw
       X[(2 * zz + 1) * Ng1 + n] = CT<C>::mk(a2 * a2, a3 * a3);  // vv + i ww
@@ -198,6 +207,25 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   }
   __syncthreads();
   C *Y = fft_lds<C>(X, X == A ? B : A, Ng1, 2 * nz, Ng1, -1, TW);
+  {
+    // one (max a, max b) pair per workgroup, reduced by k_dt_update
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = (double)ma;
+    red[1][threadIdx.x] = (double)mb;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + w]);
+        red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + w]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      double *cm = v.cfl + 2 * ((size_t)slot * gridDim.x + blockIdx.x);
+      cm[0] = red[0][0];
+      cm[1] = red[1][0];
+    }
+  }
   const int nseq = isU ? 2 : 1;
   const R_t inv = (R_t)(1.0 / Ng1);
   C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxp;
@@ -1154,6 +1182,101 @@ __global__ void k_from_rows(const double2 *__restrict__ src, double2 *__restrict
     int c = c0 + q, r = r0 + tx;
     if (c < ncol && r < nrows) dst[(size_t)c * nrows + r] = tile[tx][q];
   }
+}
+
+// ------------------------------------------------------------------------
+// Time step at the first substep (FOU3D.f90:288-297): dtc from the CFL
+// maxima of this substep's physical planes (dtc_calc, FOU3D.f90:699-736,
+// with u2mloc starting at 1e-6), then dt = min(2.5 dtv, CFL dtc) when the
+// channel's step is adaptive, and t += dt.  One workgroup per channel.
+__global__ void __launch_bounds__(256) k_dt_update(char *arena, const Chan *chans, int ntiles_max, int e_xprod) {
+  const Chan ch = chans[blockIdx.x];
+  View v(arena, ch);
+  double m1 = 0.0, m2 = 1e-6, m3 = 0.0;
+  // k_xprod's tiles of this channel (stride: the launch's tile count)
+  const int TZ = tile_of(e_xprod, ch.Ng1, ch.Ng2), nt = (ch.Ng2 + TZ - 1) / TZ;
+  for (int q = threadIdx.x; q < ch.nslot * nt; q += blockDim.x) {
+    const int slot = q / nt, tz = q - slot * nt;
+    const double *cm = v.cfl + 2 * ((size_t)slot * ntiles_max + tz);
+    if (slot < ch.nplu) {
+      m1 = fmax(m1, cm[0]);
+      m3 = fmax(m3, cm[1]);
+    } else {
+      const int j = slot - ch.nplu + 1;
+      m2 = fmax(m2, cm[0] * v.dthdyv[j]);
+    }
+  }
+  __shared__ double red[3][256];
+  red[0][threadIdx.x] = m1;
+  red[1][threadIdx.x] = m2;
+  red[2][threadIdx.x] = m3;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int r = 0; r < 3; r++) red[r][threadIdx.x] = fmax(red[r][threadIdx.x], red[r][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    m1 = red[0][0];
+    m2 = red[1][0];
+    m3 = red[2][0];
+    const double dtc = fmin(fmin(1.0 / (ch.alp * (ch.N1 / 2) * m1), 1.0 / (ch.bet * (ch.N2 / 2) * m3)), 1.0 / (m2 * 1.0));
+    v.scal[SC_DTC] = dtc;
+    if (ch.adaptive) v.scal[SC_DT] = fmin(2.5 * v.scal[SC_DTV], ch.CFL * dtc);
+    v.scal[SC_T] = v.scal[SC_T] + v.scal[SC_DT];
+  }
+}
+
+// unit-forcing solutions of the mean mode per substep (flowrate_corr's
+// u11 = LUsol0(-dRK dt), littleharsh.f90:484-494, with LU_build0/LU_dec0,
+// tridLU_3D.f90:409-435,520-537) and their flow rates, for the current dt;
+// one thread per (kRK, channel)
+__global__ void k_u11(char *arena, const Chan *chans) {
+  const Chan ch = chans[blockIdx.x];
+  View v(arena, ch);
+  const int kRK = threadIdx.x + 1;
+  if (kRK > 3) return;
+  const int n = ch.nyu;
+  const double dt = v.scal[SC_DT];
+  const double beta = rk_b(kRK) * dt / ch.Re, bdt = -rk_d(kRK) * dt;
+  double *x = v.u11 + (size_t)(kRK - 1) * n;
+  // forward elimination with the pivots held in x's storage order
+  // (d_0 = 1: row 0 is u0 + gw1 u1 = 0)
+  double c = ch.gw1;  // a(3,0) * d_0
+  x[0] = 0.0;
+  double *cs = v.scr + (size_t)(kRK - 1) * n;  // upper diagonals c_j
+  cs[0] = c;
+  for (int j = 1; j < n; j++) {
+    double a1, a2, a3;
+    if (j <= n - 2) {
+      a1 = -beta * v.dyu2i[3 * j];
+      a2 = 1.0 - beta * v.dyu2i[3 * j + 1];
+      a3 = -beta * v.dyu2i[3 * j + 2];
+    } else {
+      a1 = ch.gw2;
+      a2 = 1.0;
+      a3 = 0.0;
+    }
+    const double d = 1.0 / (a2 - a1 * c);
+    c = a3 * d;
+    cs[j] = c;
+    const double r = j <= n - 2 ? bdt : 0.0;
+    x[j] = (r - a1 * x[j - 1]) * d;
+  }
+  for (int j = n - 2; j >= 0; j--) x[j] = x[j] - cs[j] * x[j + 1];
+  double Q = 0.0;
+  for (int j = 1; j <= n - 2; j++) Q = Q + (v.yv[j] - v.yv[j - 1]) * x[j];
+  v.scal[SC_QX1 + kRK - 1] = Q;
+}
+
+hipError_t launch_dt_update(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s) {
+  hipLaunchKernelGGL(k_dt_update, dim3(m.nchan), dim3(256), 0, s, arena, d_chans, m.xprod_tiles, m.e_xprod);
+  return hipGetLastError();
+}
+
+hipError_t launch_u11(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s) {
+  hipLaunchKernelGGL(k_u11, dim3(m.nchan), dim3(64), 0, s, arena, d_chans);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------

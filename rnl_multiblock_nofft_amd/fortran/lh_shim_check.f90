@@ -31,7 +31,7 @@ program lh_shim_check
   read(u, nml=check)
   close(u)
   cfg(1) = lh_config(N1, N2, 3 * N1 / 2, 3 * N2 / 2, ny, alp, bet, Re, grid_kind, dyq, ppp, &
-                     gamma, 0d0, 0d0, flag_ctpress, 0d0, dt, LH_PREC_FP64)
+                     gamma, 0d0, 0d0, flag_ctpress, 0d0, dt, LH_PREC_FP64, 0, 0.5d0)
   call lh_host_setup(cfg)
   ncol = (N1 / 2 + 1) * N2
 

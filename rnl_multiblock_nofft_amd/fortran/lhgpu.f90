@@ -31,6 +31,8 @@ module lhgpu
     integer(c_int) :: flag_ctpress
     real(c_double) :: mpgx, dt
     integer(c_int) :: precision
+    integer(c_int) :: adaptive_dt
+    real(c_double) :: CFL
   end type lh_config
 
   interface

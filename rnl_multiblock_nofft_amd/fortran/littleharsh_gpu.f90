@@ -37,7 +37,7 @@ program littleharsh_gpu
   close(u)
 
   cfg(1) = lh_config(N1, N2, 3 * N1 / 2, 3 * N2 / 2, ny, alp, bet, Re, grid_kind, dyq, ppp, &
-                     gamma, bslpu1, bslpu3, flag_ctpress, mpgx, dt, precision)
+                     gamma, bslpu1, bslpu3, flag_ctpress, mpgx, dt, precision, 0, 0.5d0)
   call lh_host_setup(cfg)
   ncol = (N1 / 2 + 1) * N2
   elo = [0, 0, 0, 1]

@@ -103,7 +103,7 @@ class Solver:
     def scalars(self, ch=0):
         o = np.zeros(8)
         check(self.L.lh_get_scalars(self.h, ch, o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
-        return dict(t=o[0], dt=o[1], mpgx=o[2], QxT=o[3], Qx=o[4], dtv=o[5], iter=int(o[7]))
+        return dict(t=o[0], dt=o[1], mpgx=o[2], QxT=o[3], Qx=o[4], dtv=o[5], dtc=o[6], iter=int(o[7]))
 
     def set_scalars(self, ch, t=0.0, mpgx=None, QxT=None, iter=0):
         s = self.scalars(ch)

@@ -276,6 +276,47 @@ def test_bad_texture_code_raises():
         s.set_texture(0, m, None)
 
 
+def test_dtc_matches_oracle():
+    """dtc_calc on the device (CFL maxima of the first substep's physical
+    planes) equals the oracle's, with dt fixed."""
+    cfg = CASES["N32_ny153"]
+    s, o, f = make_pair(cfg, seed=31)
+    s.step(1)
+    o.step(1)
+    a, b = s.scalars(0), o.scalars()
+    say("dtc", a["dtc"], b["dtc"], "t", a["t"], b["t"])
+    assert abs(a["dtc"] - b["dtc"]) <= 1e-13 * b["dtc"]
+    assert abs(a["t"] - b["t"]) <= 1e-15
+
+
+@pytest.mark.parametrize("name", ["N16_ny33_poly_ctP", "N32_ny153"])
+def test_adaptive_time_step(name):
+    """Adaptive dt = min(2.5 dtv, CFL dtc) at every first substep, with the
+    pivots and mean-mode solutions rebuilt on the device (FOU3D.f90:288-297)."""
+    cfg = dict(CASES[name], adaptive_dt=1, CFL=0.5)
+    say("adaptive", name)
+    s = Solver([cfg])
+    o = oracle_for(dict(cfg, fixed_dt=0))
+    f = init_fields(cfg, s.grid(0), 41)
+    for nm in ("u1", "u2", "u3", "p"):
+        s.upload(0, nm, f[nm])
+        o.set(nm, f[nm])
+    s.init_flowrate()
+    o.init_flowrate()
+    for n in range(3):
+        s.step(1)
+        o.step(1)
+        a, b = s.scalars(0), o.scalars()
+        say("adaptive step", n, "dt", a["dt"], b["dt"], "t", a["t"], b["t"])
+        assert abs(a["dt"] - b["dt"]) <= 1e-12 * b["dt"]
+        assert abs(a["t"] - b["t"]) <= 1e-12 * b["t"]
+        assert a["dt"] != cfg["dt"]
+        for nm in ("u1", "u2", "u3", "p"):
+            e = relerr(s.download(0, nm), o.get(nm))
+            say(name, n, nm, e)
+            assert e <= 1e-11, (nm, e)
+
+
 def test_ensemble_uneven():
     cfgs = [small_cfg(8, 17), small_cfg(16, 33), small_cfg(8, 17, grid_kind=0), small_cfg(16, 25)]
     say("ensemble", len(cfgs))
