@@ -190,6 +190,50 @@ int lh_autotune(lh_ctx *ctx, int reps);
  * and the ghost rows of v_corr follow the mask point by point. */
 int lh_set_texture(lh_ctx *ctx, int ch, const int *bottom, const int *top);
 
+/* Running statistics and spectra, replacing stats (stats.f90:1-107) with
+ * its gathers recvX (stats.f90:443-470) and spectra / buildsp / buildspUV
+ * (spectra.f90:1-56).  lh_stats_accumulate adds one sample of the current
+ * fields of every channel (the reference samples at the first substep of
+ * every nstat-th step, FOU3D.f90:58-63, i.e. on the state between two
+ * steps) and counts it (istat).  lh_stats_reset zeroes everything, as
+ * write_stats / write_spect do after writing (stats.f90:409-440,
+ * spectra.f90:101-105). */
+int lh_stats_accumulate(lh_ctx *ctx);
+int lh_stats_reset(lh_ctx *ctx);
+/* Row profiles, prof[LH_ST_N][ny+1], profile q at prof[q*(ny+1) + j] for
+ * the reference's row j; the order is write_stats' (stats.f90:356-368):
+ * u-grid profiles (Um U2m Wm W2m UVm UWm) on rows 0..ny, v-grid profiles
+ * (Vm V2m VWm wxm wx2m) on 0..ny-1, p profiles (Pm P2m) on 1..ny-1; other
+ * entries are 0.  Sums over grid points, not yet divided by the sample or
+ * point counts, as the reference writes them.  At the ghost / wall rows and
+ * for p the reference's planes hold spectral coefficients, and the sums are
+ * those of the stored real and imaginary parts, as there. */
+#define LH_ST_UM 0
+#define LH_ST_U2M 1
+#define LH_ST_VM 2
+#define LH_ST_V2M 3
+#define LH_ST_WM 4
+#define LH_ST_W2M 5
+#define LH_ST_PM 6
+#define LH_ST_P2M 7
+#define LH_ST_UVM 8
+#define LH_ST_UWM 9
+#define LH_ST_VWM 10
+#define LH_ST_WXM 11
+#define LH_ST_WX2M 12
+#define LH_ST_N 13
+int lh_stats_get(lh_ctx *ctx, int ch, double *prof, int *istat);
+/* One accumulated spectrum in the reference's column layout
+ * sp(jlo:jhi, ncol) (spU, spV, spW, spUV, spP of spectra.f90): rows 0..ny
+ * for U, W, UV, 0..ny-1 for V, 1..ny-1 for P. */
+#define LH_SP_U 0
+#define LH_SP_V 1
+#define LH_SP_W 2
+#define LH_SP_UV 3
+#define LH_SP_P 4
+#define LH_SP_N 5
+int lh_spectrum_get(lh_ctx *ctx, int ch, int which, double *sp);
+
 #ifdef __cplusplus
 }
 #endif

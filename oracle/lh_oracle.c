@@ -1251,3 +1251,112 @@ void lho_step(lho_state *s, int nsteps) {
     s->iter = s->iter + 1;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* stats (stats.f90:1-107) and spectra / buildsp / buildspUV
+ * (spectra.f90:1-56): one sample of the current fields, summed in physical
+ * space exactly as the reference forms its planes at a statistics step:
+ * u1PL, u3PL, u2PL_itp on the u rows and u2PL, u3PL_itp on the v rows are
+ * filled by modes_to_planes_UVP (all rows, FOU3D.f90:1000-1098) and only
+ * the interior rows are transformed by ops_in_planes2 (FOU3D.f90:2116,
+ * 2151); ppPL is the spectral plane of p (FOU3D.f90:156-157); wx is
+ * -der_z_N(u2) + der_yv_h_wx(u3) on the N grid, transformed by
+ * four_to_phys_N (FOU3D.f90:302-327).  Sums run over i = 1..igal-2,
+ * k = 1..kgal (N1 x N2 for wx) as in stats.f90. */
+void lho_stats(lho_state *s, double *prof, double *sp) {
+  const int nyu = s->nyu, nyv = s->nyv, ny = s->c.ny, igal = s->igal, kgal = s->kgal;
+  const int N1 = s->c.N1, N2 = s->c.N2, ig = N1 + 2;
+  const size_t psz = (size_t)igal * kgal;
+  interp_all(s);
+  double *P1 = malloc(sizeof(double) * psz), *P2 = malloc(sizeof(double) * psz);
+  double *P3 = malloc(sizeof(double) * psz), *W = malloc(sizeof(double) * (size_t)ig * N2);
+  cplx *wc = malloc(sizeof(cplx) * (size_t)s->ncol);
+#define PROF(q, j) prof[(size_t)(q) * nyu + (j)]
+  for (int j = 0; j < nyu; j++) {
+    /* u rows */
+    modes_to_plane(s, s->u1, nyu, j, P1);
+    modes_to_plane(s, s->u2i, nyu, j, P2);
+    modes_to_plane(s, s->u3, nyu, j, P3);
+    if (j >= 1 && j <= ny - 1) {
+      lho_four_to_phys_plane(s, P1);
+      lho_four_to_phys_plane(s, P2);
+      lho_four_to_phys_plane(s, P3);
+    }
+    for (int k = 0; k < kgal; k++)
+      for (int i = 0; i < igal - 2; i++) {
+        size_t o = (size_t)k * igal + i;
+        PROF(0, j) += P1[o];
+        PROF(1, j) += P1[o] * P1[o];
+        PROF(4, j) += P3[o];
+        PROF(5, j) += P3[o] * P3[o];
+        PROF(8, j) += P1[o] * P2[o];
+        PROF(9, j) += P1[o] * P3[o];
+      }
+    /* v rows */
+    if (j <= nyv - 1) {
+      modes_to_plane(s, s->u2, nyv, j, P1);
+      modes_to_plane(s, s->u3i, nyv, j, P3);
+      if (j >= 1 && j <= ny - 2) {
+        lho_four_to_phys_plane(s, P1);
+        lho_four_to_phys_plane(s, P3);
+      }
+      for (int k = 0; k < kgal; k++)
+        for (int i = 0; i < igal - 2; i++) {
+          size_t o = (size_t)k * igal + i;
+          PROF(2, j) += P1[o];
+          PROF(3, j) += P1[o] * P1[o];
+          PROF(10, j) += P1[o] * P3[o];
+        }
+      for (int c = 0; c < s->ncol; c++) {
+        int k = c / s->nxh;
+        cplx u2 = s->u2[(size_t)c * nyv + j];
+        cplx dz = (I * s->kz[k]) * u2;
+        cplx dy = (s->u3[(size_t)c * nyu + j + 1] - s->u3[(size_t)c * nyu + j]) * s->dthdyv[j] * s->ddthetavi;
+        wc[c] = -dz + dy;
+      }
+      memset(W, 0, sizeof(double) * (size_t)ig * N2);
+      col_to_planeN(s, wc, W);
+      four_to_phys_N(s, W);
+      for (int k = 0; k < N2; k++)
+        for (int i = 0; i < N1; i++) {
+          double w = W[(size_t)k * ig + i];
+          PROF(11, j) += w;
+          PROF(12, j) += w * w;
+        }
+    }
+    /* p rows: the spectral plane */
+    if (j >= 1 && j <= ny - 1) {
+      modes_to_plane(s, s->p, nyu, j, P1);
+      for (int k = 0; k < kgal; k++)
+        for (int i = 0; i < igal - 2; i++) {
+          size_t o = (size_t)k * igal + i;
+          PROF(6, j) += P1[o];
+          PROF(7, j) += P1[o] * P1[o];
+        }
+    }
+  }
+#undef PROF
+  /* spectra: sp[w][c*nyu + j], w = U, V, W, UV, P */
+  const size_t fs = (size_t)s->ncol * nyu;
+  for (int c = 0; c < s->ncol; c++)
+    for (int j = 0; j < nyu; j++) {
+      size_t o = (size_t)c * nyu + j;
+      cplx a = s->u1[o], w = s->u3[o], vi = s->u2i[o];
+      sp[0 * fs + o] = sp[0 * fs + o] + creal(a) * creal(a) + cimag(a) * cimag(a);
+      sp[2 * fs + o] = sp[2 * fs + o] + creal(w) * creal(w) + cimag(w) * cimag(w);
+      sp[3 * fs + o] = sp[3 * fs + o] + creal(a) * creal(vi) + cimag(a) * cimag(vi);
+      if (j <= nyv - 1) {
+        cplx v = s->u2[(size_t)c * nyv + j];
+        sp[1 * fs + o] = sp[1 * fs + o] + creal(v) * creal(v) + cimag(v) * cimag(v);
+      }
+      if (j >= 1 && j <= ny - 1) {
+        cplx p = s->p[o];
+        sp[4 * fs + o] = sp[4 * fs + o] + creal(p) * creal(p) + cimag(p) * cimag(p);
+      }
+    }
+  free(P1);
+  free(P2);
+  free(P3);
+  free(W);
+  free(wc);
+}

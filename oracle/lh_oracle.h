@@ -106,6 +106,14 @@ int lho_set_texture(lho_state *s, const int *bottom, const int *top);
 /* the textured U/W Helmholtz solve on arbitrary right-hand sides */
 void lho_lusol_uw_tex(lho_state *s, int kRK, const double *rhs_u, const double *rhs_w, double *u, double *w);
 
+/* One statistics sample of the current fields (stats.f90:1-107 and
+ * spectra.f90:1-56), added to prof[13][ny+1] (profile q of write_stats'
+ * order Um U2m Vm V2m Wm W2m Pm P2m UVm UWm VWm wxm wx2m at q*(ny+1) + j,
+ * j the reference row) and sp[5][ncol][ny+1] (spU spV spW spUV spP at
+ * w*ncol*(ny+1) + c*(ny+1) + j).  Physical planes are formed and summed as
+ * the reference does. */
+void lho_stats(lho_state *s, double *prof, double *sp);
+
 #ifdef __cplusplus
 }
 #endif

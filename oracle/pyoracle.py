@@ -84,6 +84,7 @@ def lib():
         L.lho_set_texture.argtypes = [vp, ip_, ip_]
         L.lho_set_texture.restype = ctypes.c_int
         L.lho_lusol_uw_tex.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
+        L.lho_stats.argtypes = [vp, dp, dp]
         _lib = L
     return _lib
 
@@ -245,3 +246,17 @@ class Oracle:
         x = np.ascontiguousarray(x, dtype=np.float64).copy()
         lib().lho_lusol0(self.h, kRK, _dp(x))
         return x
+
+    def stats(self, prof=None, sp=None):
+        """Add one statistics sample (stats.f90:1-107, spectra.f90:1-56) to
+        prof (13, ny+1) and sp (5, ncol, ny+1); new zero arrays when None.
+        Returns (prof, sp)."""
+        nyu, ncol = self.nyu, self.ncol
+        if prof is None:
+            prof = np.zeros((13, nyu))
+        if sp is None:
+            sp = np.zeros((5, ncol, nyu))
+        assert prof.shape == (13, nyu) and prof.dtype == np.float64 and prof.flags.c_contiguous
+        assert sp.shape == (5, ncol, nyu) and sp.dtype == np.float64 and sp.flags.c_contiguous
+        lib().lho_stats(self.h, _dp(prof), _dp(sp))
+        return prof, sp

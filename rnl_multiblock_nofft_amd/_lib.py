@@ -26,7 +26,10 @@ EXPORTS = ["lh_last_error", "lh_ctx_create", "lh_ctx_destroy", "lh_set_grid", "l
            "lh_init_flowrate", "lh_substep", "lh_step", "lh_sync", "lh_time_kernels", "lh_run_phase",
            "lh_four_to_phys_planes", "lh_phys_to_four_planes", "lh_lusol_uw", "lh_lusol_v",
            "lh_dev_alloc", "lh_dev_free", "lh_copy_to_device", "lh_copy_to_host",
-           "lh_set_tuning", "lh_get_tuning", "lh_autotune", "lh_set_texture"]
+           "lh_set_tuning", "lh_get_tuning", "lh_autotune", "lh_set_texture",
+           "lh_stats_accumulate", "lh_stats_reset", "lh_stats_get", "lh_spectrum_get"]
+ST_N = 13
+SP_IDS = {"U": 0, "V": 1, "W": 2, "UV": 3, "P": 4}
 
 
 class LhConfig(ctypes.Structure):
@@ -84,6 +87,10 @@ def load():
     L.lh_get_tuning.argtypes = [vp, ip, ctypes.POINTER(ip)]
     L.lh_autotune.argtypes = [vp, ip]
     L.lh_set_texture.argtypes = [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)]
+    L.lh_stats_accumulate.argtypes = [vp]
+    L.lh_stats_reset.argtypes = [vp]
+    L.lh_stats_get.argtypes = [vp, ip, dp, ctypes.POINTER(ip)]
+    L.lh_spectrum_get.argtypes = [vp, ip, ip, dp]
     for name in EXPORTS:
         if name != "lh_last_error":
             getattr(L, name).restype = ctypes.c_int

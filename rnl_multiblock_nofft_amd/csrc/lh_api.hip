@@ -221,6 +221,9 @@ struct lh_ctx {
   int tex_nit = 3;                   // wall-source iterations per substep
   int tex_fused = 1;                 // LDS-resident wall transforms when they fit
   bool adaptive = false;             // a channel with an adaptive time step
+  char *stats = nullptr;             // statistics accumulators (allocated on first use)
+  size_t stats_bytes = 0;
+  int istat = 0;                     // samples since the last reset
 };
 
 namespace {
@@ -536,6 +539,15 @@ int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
     e = hipMalloc((void **)&x->arena, off);
     if (e != hipSuccess) rc = set_err(LH_ENOMEM, "hipMalloc(%zu): %s", off, hipGetErrorString(e));
     x->arena_bytes = off;
+    // statistics accumulators: ST_N row profiles and SP_N spectra per channel
+    size_t so = 0;
+    for (auto &h : x->ch) {
+      h.dev.o_stat = so;
+      so += al<double>((size_t)ST_N * h.nyu);
+      h.dev.o_spec = so;
+      so += al<double>((size_t)SP_N * h.nyu * h.ldc);
+    }
+    x->stats_bytes = so;
   }
   if (rc == LH_OK) {
     e = hipMemset(x->arena, 0, x->arena_bytes);
@@ -580,6 +592,7 @@ int lh_ctx_destroy(lh_ctx *x) {
   if (x->arena) (void)hipFree(x->arena);
   if (x->d_chans) (void)hipFree(x->d_chans);
   if (x->staging) (void)hipFree(x->staging);
+  if (x->stats) (void)hipFree(x->stats);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return LH_OK;
@@ -792,6 +805,79 @@ int lh_sync(lh_ctx *x) {
   int rc = check_ctx(x);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// statistics and spectra (stats.f90:1-107, spectra.f90:1-56)
+
+static int stats_alloc(lh_ctx *x) {
+  if (x->stats) return LH_OK;
+  hipError_t e = hipMalloc((void **)&x->stats, x->stats_bytes);
+  if (e != hipSuccess) {
+    x->stats = nullptr;
+    return set_err(LH_ENOMEM, "statistics buffer hipMalloc(%zu): %s", x->stats_bytes, hipGetErrorString(e));
+  }
+  HIPCHK(hipMemsetAsync(x->stats, 0, x->stats_bytes, x->stream));
+  x->istat = 0;
+  return LH_OK;
+}
+
+static_assert(LH_ST_N == ST_N && LH_SP_N == SP_N && LH_ST_WX2M == ST_WX2M && LH_SP_P == SP_P,
+              "statistics enums of the C-ABI and the kernels differ");
+
+int lh_stats_accumulate(lh_ctx *x) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if ((rc = stats_alloc(x))) return rc;
+  HIPCHK(launch_stats(x->arena, x->d_chans, x->shape, x->stats, x->stream));
+  x->istat++;
+  return LH_OK;
+}
+
+int lh_stats_reset(lh_ctx *x) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (x->stats) HIPCHK(hipMemsetAsync(x->stats, 0, x->stats_bytes, x->stream));
+  x->istat = 0;
+  return LH_OK;
+}
+
+int lh_stats_get(lh_ctx *x, int ch, double *prof, int *istat) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || !prof) return set_err(LH_EINVAL, "lh_stats_get: bad arguments");
+  const HostChan &h = x->ch[ch];
+  const size_t n = (size_t)LH_ST_N * h.nyu;
+  if (!x->stats) {
+    std::fill(prof, prof + n, 0.0);
+  } else {
+    HIPCHK(hipMemcpyAsync(prof, x->stats + h.dev.o_stat, n * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+  }
+  if (istat) *istat = x->istat;
+  return LH_OK;
+}
+
+int lh_spectrum_get(lh_ctx *x, int ch, int which, double *sp) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || which < 0 || which >= LH_SP_N || !sp)
+    return set_err(LH_EINVAL, "lh_spectrum_get: bad arguments");
+  const HostChan &h = x->ch[ch];
+  // rows of the spectrum's grid: u (0..ny), v (0..ny-1), p (1..ny-1)
+  const int jlo = which == LH_SP_P ? 1 : 0;
+  const int nrows = which == LH_SP_V ? h.nyv : (which == LH_SP_P ? h.nyu - 2 : h.nyu);
+  std::vector<double> rows((size_t)nrows * h.ldc);
+  if (!x->stats) {
+    std::fill(sp, sp + (size_t)nrows * h.ncol, 0.0);
+    return LH_OK;
+  }
+  const char *src = x->stats + h.dev.o_spec + ((size_t)which * h.nyu + jlo) * h.ldc * sizeof(double);
+  HIPCHK(hipMemcpyAsync(rows.data(), src, rows.size() * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  for (int c = 0; c < h.ncol; c++)
+    for (int r = 0; r < nrows; r++) sp[(size_t)c * nrows + r] = rows[(size_t)r * h.ldc + c];
   return LH_OK;
 }
 

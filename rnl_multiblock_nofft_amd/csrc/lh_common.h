@@ -37,6 +37,11 @@ __host__ __device__ inline double rk_d(int k) { return k == 1 ? 8.0 / 15.0 : (k 
 // compare as integers)
 enum { SC_MPGX = 0, SC_QXT, SC_QX, SC_DT, SC_T, SC_QX1, SC_DTC = 8, SC_DTV, SC_M1, SC_M2, SC_M3, SC_NSCAL = 16 };
 
+// statistics profiles in the order write_stats writes them (stats.f90:356-368)
+enum { ST_UM = 0, ST_U2M, ST_VM, ST_V2M, ST_WM, ST_W2M, ST_PM, ST_P2M, ST_UVM, ST_UWM, ST_VWM, ST_WXM, ST_WX2M, ST_N };
+// spectra (spectra.f90:9-13): u, v, w, u.v_itp, p
+enum { SP_U = 0, SP_V, SP_W, SP_UV, SP_P, SP_N };
+
 // Per-channel descriptor.  Every device array lives in one context-wide
 // arena; the descriptor holds byte offsets, and kernels derive typed pointers
 // from the arena base they receive as an argument (so that the compiler sees
@@ -73,6 +78,10 @@ struct Chan {
   size_t o_mask, o_wtab, o_texu, o_texw, o_iscal;
   size_t o_twzN, o_twxN;  // N2- and N1-point twiddles of the N-grid
   size_t o_cfl;           // [slot][tile][2] CFL maxima of k_xprod's workgroups
+  // statistics accumulators, byte offsets into the context's separate
+  // statistics buffer (allocated on first use): prof[ST_*][nyu] row profiles
+  // and spec[SP_*][nyu][ldc] per-mode spectra (doubles)
+  size_t o_stat, o_spec;
   // Thomas factors of the fixed-dt operators, [row][ldc] doubles each:
   // o_fac + ((kRK-1)*4 + m) * nyu*ldc*8 for m = 0 dU, 1 cU, 2 dV, 3 cV
   // (U/W and V Helmholtz, LU_dec order), then o_fac + 12*nyu*ldc*8 dP and

@@ -60,6 +60,9 @@ size_t tex_fused_bytes(const std::vector<TexGroup> &groups);
 hipError_t launch_tex_sources_fused(char *arena, const Chan *d_chans, const ShapeMax &m, size_t lds, int kRK,
                                     int nit, hipStream_t s);
 hipError_t launch_tex_ghosts_fused(char *arena, const Chan *d_chans, const ShapeMax &m, size_t lds, hipStream_t s);
+// one statistics sample of every channel: row profiles and spectra added to
+// the accumulators at sbase + Chan::o_stat / o_spec
+hipError_t launch_stats(char *arena, const Chan *d_chans, const ShapeMax &m, char *sbase, hipStream_t s);
 // stand-alone Helmholtz solves on reference-layout device data
 hipError_t launch_lusol_ref2(char *arena, const Chan *d_chan, int kRK, int vgrid, const double2 *rhs_a,
                              const double2 *rhs_b, double2 *x_a, double2 *x_b, double *scr, int ncol,

@@ -12,6 +12,7 @@
 !   four_to_phys_u, four_to_phys_du, phys_to_four_du   (FOU3D.f90:738-784,2253)
 !   LUsolU_W, LUsolV                                    (tridLU_3D.f90:55,135)
 !   read_in_field, record_out_field                     (start.f90:2444, FOU3D.f90:2856)
+!   write_stats_file                                    (stats.f90:254-441)
 ! The LU factor argument `a`, `grid` and `myid` of LUsolU_W/LUsolV are
 ! dropped: the factors of the fixed-dt operators live on the device (built
 ! once at setup, as LU_buildP is in the reference), and there is one rank.
@@ -265,5 +266,53 @@ contains
     end do
     close(u)
   end subroutine record_out_field
+
+  ! ---------------------------------------------------------------------
+  ! Statistics file of write_stats (stats.f90:254-441) for one process,
+  ! from the device accumulators of band `iband`: header, Ngal, yu/dthetai/
+  ! dthdyu, yv/dthetai/dthdyv, istat, then Um U2m Vm V2m Wm W2m Pm P2m UVm
+  ! UWm VWm wxm wx2m on their rows (u 0..ny, v 0..ny-1, p 1..ny-1).  The
+  ! tile-conditional records of the reference's wall bands are not written.
+  ! Like write_stats, the accumulators are reset afterwards (all bands).
+  subroutine write_stats_file(fname, hdr, ngal_arr, iband)
+    character(*), intent(in) :: fname
+    type(field_header), intent(in) :: hdr
+    integer, intent(in) :: ngal_arr(:,:), iband
+    real(8), allocatable :: prof(:,:), yu(:), yv(:), du(:), dv(:)
+    integer(c_int) :: istat
+    integer :: u, m, q, ny, j0, nr
+    integer, parameter :: zeros(88) = 0
+    ! grid of each profile: 0 u (rows 0..ny), 1 v (0..ny-1), 2 p (1..ny-1)
+    integer, parameter :: grid_of(0:12) = [0, 0, 1, 1, 0, 0, 2, 2, 0, 0, 1, 1, 1]
+    ny = nyb(iband)
+    allocate(prof(0:ny, 0:LH_ST_N - 1), yu(0:ny), yv(0:ny - 1), du(0:ny), dv(0:ny - 1))
+    call lh_check(lh_stats_get(lhctx, int(iband - 1, c_int), prof, istat), 'lh_stats_get')
+    call lh_check(lh_get_grid(lhctx, int(iband - 1, c_int), yu, yv, du, dv), 'lh_get_grid')
+    open(newunit=u, file=fname, access='stream', form='unformatted', status='replace', action='write')
+    m = 5 * 8 + 2 * 4 + 88 * 4
+    write(u) m, hdr%t, hdr%Re, hdr%alp, hdr%bet, hdr%mpgx, hdr%nband, hdr%iter, zeros, m
+    m = 4 * size(ngal_arr)
+    write(u) m, ngal_arr, m
+    m = 8 * (2 * (ny + 1) + 1)
+    write(u) m, yu, hdr%dthetavi, du, m
+    m = 8 * (2 * ny + 1)
+    write(u) m, yv, hdr%dthetavi, dv, m
+    m = 4
+    write(u) m, int(istat), m
+    do q = 0, LH_ST_N - 1
+      select case (grid_of(q))
+      case (0)
+        j0 = 0; nr = ny + 1
+      case (1)
+        j0 = 0; nr = ny
+      case default
+        j0 = 1; nr = ny - 1
+      end select
+      m = 8 * nr
+      write(u) m, prof(j0:j0 + nr - 1, q), m
+    end do
+    close(u)
+    call lh_check(lh_stats_reset(lhctx), 'lh_stats_reset')
+  end subroutine write_stats_file
 
 end module lh_host

@@ -20,6 +20,13 @@ module lhgpu
                                LH_PHASE_SOLVE = 3
   integer(c_int), parameter :: LH_TUNE_SOLVE_COLS = 1, LH_TUNE_ZINV_POINTS = 2, &
                                LH_TUNE_XPROD_POINTS = 3, LH_TUNE_ZFWD_POINTS = 4
+  ! statistics profiles in write_stats' order (stats.f90:356-368), spectra
+  integer(c_int), parameter :: LH_ST_UM = 0, LH_ST_U2M = 1, LH_ST_VM = 2, LH_ST_V2M = 3, &
+                               LH_ST_WM = 4, LH_ST_W2M = 5, LH_ST_PM = 6, LH_ST_P2M = 7, &
+                               LH_ST_UVM = 8, LH_ST_UWM = 9, LH_ST_VWM = 10, LH_ST_WXM = 11, &
+                               LH_ST_WX2M = 12, LH_ST_N = 13
+  integer(c_int), parameter :: LH_SP_U = 0, LH_SP_V = 1, LH_SP_W = 2, LH_SP_UV = 3, &
+                               LH_SP_P = 4, LH_SP_N = 5
 
   type, bind(C) :: lh_config
     integer(c_int) :: N1, N2, Ngal1, Ngal2, ny
@@ -201,6 +208,34 @@ module lhgpu
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx, bottom, top
       integer(c_int), value :: ch
+      integer(c_int) :: rc
+    end function
+    function lh_stats_accumulate(ctx) bind(C, name='lh_stats_accumulate') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function lh_stats_reset(ctx) bind(C, name='lh_stats_reset') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    ! prof(0:ny, 0:LH_ST_N-1): profile q in prof(:, q), rows by the
+    ! reference's row index
+    function lh_stats_get(ctx, ch, prof, istat) bind(C, name='lh_stats_get') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch
+      real(c_double), intent(out) :: prof(*)
+      integer(c_int), intent(out) :: istat
+      integer(c_int) :: rc
+    end function
+    ! sp(jlo:jhi, ncol), the layout of spU..spP (spectra.f90)
+    function lh_spectrum_get(ctx, ch, which, sp) bind(C, name='lh_spectrum_get') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch, which
+      real(c_double), intent(out) :: sp(*)
       integer(c_int) :: rc
     end function
     function c_strlen(s) bind(C, name='strlen') result(n)

@@ -9,25 +9,28 @@
 ! run.nml holds one namelist &run with the channel (N1, N2, ny, alp, bet,
 ! Re, grid_kind, dyq, ppp, gamma, bslpu1, bslpu3, flag_ctpress, mpgx, dt,
 ! precision), nsteps, and the input/output file names fin(1:4), fout(1:4)
-! of u1, u2, u3, p.
+! of u1, u2, u3, p.  With nstat > 0 a statistics sample is taken after every
+! nstat steps (the reference samples at the first substep of every nstat-th
+! step, FOU3D.f90:58-63) and written to fstats at the end (write_stats,
+! stats.f90:254-441).
 program littleharsh_gpu
   use iso_c_binding
   use lhgpu
   use lh_host
   implicit none
   integer :: N1 = 32, N2 = 32, ny = 153, grid_kind = 1, ppp = 5, flag_ctpress = 0
-  integer :: precision = 0, nsteps = 1
+  integer :: precision = 0, nsteps = 1, nstat = 0
   real(8) :: alp = 1d0, bet = 2d0, Re = 4200d0, dyq = 10d0, gamma = 2d0
   real(8) :: bslpu1 = 0d0, bslpu3 = 0d0, mpgx = 0d0, dt = 1d-3
-  character(len=1024) :: fin(4) = '', fout(4) = '', nmlfile
+  character(len=1024) :: fin(4) = '', fout(4) = '', fstats = '', nmlfile
   namelist /run/ N1, N2, ny, alp, bet, Re, grid_kind, dyq, ppp, gamma, bslpu1, bslpu3, &
-                 flag_ctpress, mpgx, dt, precision, nsteps, fin, fout
+                 flag_ctpress, mpgx, dt, precision, nsteps, fin, fout, nstat, fstats
   type(lh_config) :: cfg(1)
-  type(field_header) :: hdr(4)
+  type(field_header) :: hdr(4), shdr
   complex(8), allocatable :: f(:,:)
   real(8), allocatable :: yu(:), yv(:), du(:), dv(:), yrow(:)
   real(8) :: sc(8)
-  integer :: u, q, jlo, jhi, ncol, grid
+  integer :: u, q, jlo, jhi, ncol, grid, done, ngal_arr(4, 3)
   integer, parameter :: ids(4) = [LH_U1, LH_U2, LH_U3, LH_P]
   integer :: elo(4), ehi(4)
 
@@ -59,9 +62,29 @@ program littleharsh_gpu
   call lh_check(lh_init_flowrate(lhctx), 'lh_init_flowrate')
 
   ! main loop (littleharsh.f90:172-228)
-  call lh_check(lh_step(lhctx, int(nsteps, c_int)), 'lh_step')
+  if (nstat > 0) then
+    call lh_check(lh_stats_reset(lhctx), 'lh_stats_reset')
+    done = 0
+    do while (done + nstat <= nsteps)
+      call lh_check(lh_step(lhctx, int(nstat, c_int)), 'lh_step')
+      call lh_check(lh_stats_accumulate(lhctx), 'lh_stats_accumulate')
+      done = done + nstat
+    end do
+    if (done < nsteps) call lh_check(lh_step(lhctx, int(nsteps - done, c_int)), 'lh_step')
+  else
+    call lh_check(lh_step(lhctx, int(nsteps, c_int)), 'lh_step')
+  end if
   call lh_check(lh_sync(lhctx), 'lh_sync')
   call lh_check(lh_get_scalars(lhctx, 0_c_int, sc), 'lh_get_scalars')
+  if (nstat > 0 .and. len_trim(fstats) > 0) then
+    ! Ngal(4, 0:nband+1) of the one band that holds the whole channel
+    ngal_arr = 0
+    ngal_arr(1:2, 2) = [3 * N1 / 2, 3 * N2 / 2]
+    ngal_arr(3:4, 2) = [ny - 2, ny - 1]
+    shdr%t = sc(1); shdr%iter = int(sc(8)); shdr%mpgx = sc(3)
+    shdr%Re = Re; shdr%alp = alp; shdr%bet = bet; shdr%nband = 1
+    call write_stats_file(trim(fstats), shdr, ngal_arr, 1)
+  end if
   write(*, '(a,i8,a,es14.6,a,es22.14,a,es22.14)') 'iter', int(sc(8)), '  t', sc(1), &
       '  Qx', sc(5), '  mpgx', sc(3)
 

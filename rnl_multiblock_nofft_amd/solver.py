@@ -167,3 +167,31 @@ class Solver:
 
     def sync(self):
         check(self.L.lh_sync(self.h))
+
+    # statistics and spectra (stats.f90:1-107, spectra.f90:1-56)
+    def stats_accumulate(self):
+        """Add one sample of every channel's current fields."""
+        check(self.L.lh_stats_accumulate(self.h))
+
+    def stats_reset(self):
+        check(self.L.lh_stats_reset(self.h))
+
+    def stats(self, ch=0):
+        """(prof, istat): prof (13, ny+1) in the order of statsio.PROFILES,
+        rows by the reference's row index."""
+        c = self.cfgs[ch]
+        prof = np.zeros((_lib.ST_N, c["ny"] + 1))
+        n = ctypes.c_int()
+        check(self.L.lh_stats_get(self.h, ch, prof.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  ctypes.byref(n)))
+        return prof, n.value
+
+    def spectrum(self, ch, which):
+        """Accumulated spectrum `which` in U, V, W, UV, P as (ncol, rows),
+        the reference's sp(jlo:jhi, ncol) layout."""
+        c = self.cfgs[ch]
+        rows = {"U": c["ny"] + 1, "W": c["ny"] + 1, "UV": c["ny"] + 1, "V": c["ny"], "P": c["ny"] - 1}[which]
+        out = np.zeros((_ncol(c), rows))
+        check(self.L.lh_spectrum_get(self.h, ch, _lib.SP_IDS[which],
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out

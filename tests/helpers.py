@@ -32,3 +32,25 @@ def small_cfg(N=8, ny=17, **kw):
 def init_fields(cfg, grid, seed=1234, amp=0.1):
     kmax = min(8, cfg["N1"] // 2 - 1, cfg["N2"] // 2 - 1)
     return initial.perturbed_poiseuille(cfg, grid, seed, amp=amp, kmax=kmax)
+
+
+STAT_NAMES = ("Um", "U2m", "Vm", "V2m", "Wm", "W2m", "Pm", "P2m", "UVm", "UWm", "VWm", "wxm", "wx2m")
+
+
+def profile_errors(prof, ref, cfg, istat=1):
+    """Max error of each statistics profile (write_stats order, rows by
+    reference index) relative to its natural magnitude: a second moment's
+    own maximum; a first moment's sqrt(points * second moment); a cross
+    moment's sqrt of the product of the two second moments' maxima.  (Mean
+    profiles of v are zero up to rounding, so their own maxima are no
+    scale.)"""
+    q = {n: i for i, n in enumerate(STAT_NAMES)}
+    n = cfg["Ngal1"] * cfg["Ngal2"] * istat
+    nN = cfg["N1"] * cfg["N2"] * istat
+    m2 = {k: max(np.max(np.abs(ref[q[k]])), 1e-300) for k in ("U2m", "V2m", "W2m", "P2m", "wx2m")}
+    sc = dict(U2m=m2["U2m"], V2m=m2["V2m"], W2m=m2["W2m"], P2m=m2["P2m"], wx2m=m2["wx2m"],
+              Um=np.sqrt(n * m2["U2m"]), Vm=np.sqrt(n * m2["V2m"]), Wm=np.sqrt(n * m2["W2m"]),
+              Pm=np.sqrt(n * m2["P2m"]), wxm=np.sqrt(nN * m2["wx2m"]),
+              UVm=np.sqrt(m2["U2m"] * m2["V2m"]), UWm=np.sqrt(m2["U2m"] * m2["W2m"]),
+              VWm=np.sqrt(m2["V2m"] * m2["W2m"]))
+    return {k: float(np.max(np.abs(prof[q[k]] - ref[q[k]])) / sc[k]) for k in STAT_NAMES}

@@ -13,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import init_fields, oracle_for, relerr, small_cfg
+from helpers import init_fields, oracle_for, profile_errors, relerr, small_cfg
 from rnl_multiblock_nofft_amd import fieldio
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -140,3 +140,53 @@ def test_fortran_shims_match_oracle(tmp_path):
     ev = relerr(sv, o.lusol_helm(kstage, 1, rv))
     say("lusol", eu, ew, ev)
     assert max(eu, ew, ev) <= 1e-13
+
+
+@pytest.mark.gpu
+def test_fortran_main_program_writes_stats(tmp_path):
+    """nstat > 0: samples after every nstat steps, written in write_stats'
+    layout (stats.f90:254-441), against the oracle's samples."""
+    from rnl_multiblock_nofft_amd import statsio
+    cfg = small_cfg(16, 33)
+    nsteps, nstat = 5, 2
+    o = oracle_for(cfg)
+    g = o.grid()
+    f = init_fields(cfg, g, seed=78)
+    N = np.zeros((4, 3), dtype=np.int32)
+    N[0, :] = cfg["N1"]
+    N[1, :] = cfg["N2"]
+    fin, fout = [], []
+    rows = {"u1": 0, "u2": 0, "u3": 0, "p": 1}
+    for name in ("u1", "u2", "u3", "p"):
+        p = str(tmp_path / ("%s_in.dat" % name))
+        y = g["yv"] if name == "u2" else g["yu"]
+        d = g["dthdyv"] if name == "u2" else g["dthdyu"]
+        fieldio.write_field(p, f[name], rows[name], 2 if name == "u2" else 1, y, d,
+                            Re=cfg["Re"], alp=cfg["alp"], bet=cfg["bet"], N=N)
+        fin.append(p)
+        fout.append(str(tmp_path / ("%s_out.dat" % name)))
+        o.set(name, f[name])
+    o.init_flowrate()
+    prof = None
+    for _ in range(nsteps // nstat):
+        o.step(nstat)
+        prof, sp = o.stats() if prof is None else o.stats(prof, sp)
+    o.step(nsteps % nstat)
+    fst = str(tmp_path / "stats.dat")
+    nml = str(tmp_path / "run.nml")
+    d = {k: cfg[k] for k in CFG_KEYS}
+    d.update(mpgx=0.0, precision=0, nsteps=nsteps, fin=fin, fout=fout, nstat=nstat, fstats=fst)
+    _nml(nml, "run", d)
+    _run(os.path.join(BIN, "littleharsh_gpu"), nml)
+    r = statsio.read_stats(fst)
+    assert r["istat"] == nsteps // nstat and r["iter"] == nsteps and r["nband"] == 1
+    assert np.allclose(r["yu"], g["yu"], rtol=0, atol=1e-15)
+    ny = cfg["ny"]
+    got = np.zeros_like(prof)
+    for q, name in enumerate(statsio.PROFILES):
+        j0, n = {"u": (0, ny + 1), "v": (0, ny), "p": (1, ny - 1)}[statsio.GRID_OF[name]]
+        got[q, j0:j0 + n] = r[name]
+    errs = profile_errors(got, prof, cfg, nsteps // nstat)
+    say(errs)
+    for name, e in errs.items():
+        assert e <= 1e-11, (name, e)
