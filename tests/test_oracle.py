@@ -272,3 +272,37 @@ def test_nonlinear_term_matches_brute_force_convolution():
     assert relerr(o.get("Nu1"), Nu1) <= 1e-12
     assert relerr(o.get("Nu2"), Nu2) <= 1e-12
     assert relerr(o.get("Nu3"), Nu3) <= 1e-12
+
+
+def test_dtc_closed_forms():
+    """dtc_calc (FOU3D.f90:699-736) on uniform fields: a streamwise flow U
+    gives 1/(alp (N1/2) U), a spanwise flow W gives 1/(bet (N2/2) W), a v
+    field gives 1/max_j(|V| dthdyv(j)) over the v-planes; the adaptive dt is
+    min(2.5 dtv, CFL dtc) and t advances by it."""
+    cfg = small_cfg(N=8, ny=13, fixed_dt=0, CFL=0.4)
+    ny = cfg["ny"]
+    for U, W, V in ((1.3, 0.0, 0.0), (0.0, 0.7, 0.0), (0.0, 0.0, 2e-3), (0.9, 0.4, 1e-3)):
+        o = oracle_for(cfg)
+        ncol = o.ncol
+        u1 = np.zeros((ncol, ny + 1), complex)
+        u3 = np.zeros((ncol, ny + 1), complex)
+        u2 = np.zeros((ncol, ny), complex)
+        u1[0] = U
+        u3[0] = W
+        u2[0] = V
+        o.set("u1", u1)
+        o.set("u3", u3)
+        o.set("u2", u2)
+        o.set("p", np.zeros((ncol, ny - 1), complex))
+        dv = o.grid()["dthdyv"]
+        m2 = max(1e-6, abs(V) * dv[1:ny - 1].max())
+        cands = [1.0 / m2]
+        if U:
+            cands.append(1.0 / (cfg["alp"] * (cfg["N1"] // 2) * U))
+        if W:
+            cands.append(1.0 / (cfg["bet"] * (cfg["N2"] // 2) * W))
+        o.phase("nonlinear", 1)
+        sc = o.scalars()
+        assert np.isclose(sc["dtc"], min(cands), rtol=1e-14), (U, W, V, sc["dtc"], cands)
+        dt = min(2.5 * sc["dtv"], cfg["CFL"] * sc["dtc"])
+        assert np.isclose(sc["dt"], dt, rtol=1e-15) and np.isclose(sc["t"], dt, rtol=1e-15)
