@@ -151,6 +151,33 @@ int lh_phys_to_four_planes(int N1, int Ngal1, int Ngal2, double *d_planes,
 int lh_lusol_uw(lh_ctx *ctx, int ch, int kRK, const double *d_rhsu,
                 const double *d_rhsw, double *d_u, double *d_w);
 int lh_lusol_v(lh_ctx *ctx, int ch, int kRK, const double *d_rhsv, double *d_v);
+/* LUsolP (tridLU_3D.f90:192-226): the pressure Poisson system of LU_buildP
+ * (tridLU_3D.f90:333-407, with the mean-mode pin of its first row) solved in
+ * place on device data x(1:ny-1, ncol) of channel `ch`. */
+int lh_lusol_p(lh_ctx *ctx, int ch, double *d_x);
+/* LUsol0 (tridLU_3D.f90:228-254): the real mean-mode U system of LU_build0
+ * (tridLU_3D.f90:409-435) for substep kRK and the current dt, in place on
+ * device data x(0:ny) (real). */
+int lh_lusol0(lh_ctx *ctx, int ch, int kRK, double *d_x);
+
+/* N-grid plane transforms four_to_phys_N / phys_to_four_N (FOU3D.f90:
+ * 786-804, 2274-2293): (N1+2) x N2 planes, no oddball-row zeroing. */
+int lh_four_to_phys_N_planes(int N1, int N2, double *d_planes, int nplanes, void *stream);
+int lh_phys_to_four_N_planes(int N1, int N2, double *d_planes, int nplanes, void *stream);
+
+/* Column <-> plane transposes on device data, replacing
+ * modes_to_planes_UVP (FOU3D.f90:1000-1098) and planes_to_modes_UVP /
+ * planes_to_modes_NUVP (FOU3D.f90:2295-2442, 2772-2854) on one GPU (no MPI).
+ * cols: x(jlo:jhi, ncol) complex (re/im interleaved), nrows = jhi-jlo+1;
+ * planes: nrows planes of (Ngal1+2) x Ngal2 doubles, Fortran order, mode
+ * (i, k) at (2i+1, krow) and (2i+2, krow), krow = k for k <= N2/2 and
+ * k + Ngal2 - N2 above (1-based).  modes_to_planes zero-fills the rest of
+ * each plane (the reference's callers zero the planes first).  Ngal = N
+ * gives the N-grid planes of modes_to_planes_phys. */
+int lh_modes_to_planes(int N1, int N2, int Ngal1, int Ngal2, const double *d_cols, int nrows,
+                       double *d_planes, void *stream);
+int lh_planes_to_modes(int N1, int N2, int Ngal1, int Ngal2, const double *d_planes, int nrows,
+                       double *d_cols, void *stream);
 
 /* Device memory for the device-pointer entry points above, for hosts without
  * HIP bindings of their own (the Fortran shims stage the reference's host

@@ -886,6 +886,8 @@ static void phys_to_four_N(const lho_state *s, double *P) {
   lho_rft(P, s->c.N1, ig, s->c.N2, -1);
   lho_cft(P, s->c.N2, ig, 2, (s->c.N1 + 2) / 2, -1);
 }
+void lho_four_to_phys_N_plane(const lho_state *s, double *P) { four_to_phys_N(s, P); }
+void lho_phys_to_four_N_plane(const lho_state *s, double *P) { phys_to_four_N(s, P); }
 /* column (i,k) <-> plane (2i, 2i+1) of row k */
 static void col_to_planeN(const lho_state *s, const cplx *v, double *P) {
   int ig = s->c.N1 + 2;

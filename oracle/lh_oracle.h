@@ -89,6 +89,10 @@ void lho_divergence(lho_state *s);
  * (four_to_phys_du FOU3D.f90:766, phys_to_four_du FOU3D.f90:2253). */
 void lho_four_to_phys_plane(const lho_state *s, double *plane);
 void lho_phys_to_four_plane(const lho_state *s, double *plane);
+/* four_to_phys_N / phys_to_four_N (FOU3D.f90:786-804, 2274-2293) on one
+ * (N1+2) x N2 plane, in place */
+void lho_four_to_phys_N_plane(const lho_state *s, double *plane);
+void lho_phys_to_four_N_plane(const lho_state *s, double *plane);
 /* FFTPACK-convention wrappers (cft_buff.f:1-65, rft_buff.f:24-90) */
 void lho_cft(double *c, int n, int ise, int isa, int m, int iopt);
 void lho_rft(double *c, int n, int isa, int m, int iopt);

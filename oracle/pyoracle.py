@@ -74,6 +74,8 @@ def lib():
         L.lho_divergence.argtypes = [vp]
         L.lho_four_to_phys_plane.argtypes = [vp, dp]
         L.lho_phys_to_four_plane.argtypes = [vp, dp]
+        L.lho_four_to_phys_N_plane.argtypes = [vp, dp]
+        L.lho_phys_to_four_N_plane.argtypes = [vp, dp]
         L.lho_cft.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_int, ctypes.c_int]
         L.lho_rft.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -204,6 +206,16 @@ class Oracle:
     def phys_to_four(self, plane):
         p = np.ascontiguousarray(plane, dtype=np.float64).copy()
         lib().lho_phys_to_four_plane(self.h, _dp(p))
+        return p
+
+    def four_to_phys_N(self, plane):
+        p = np.ascontiguousarray(plane, dtype=np.float64).copy()
+        lib().lho_four_to_phys_N_plane(self.h, _dp(p))
+        return p
+
+    def phys_to_four_N(self, plane):
+        p = np.ascontiguousarray(plane, dtype=np.float64).copy()
+        lib().lho_phys_to_four_N_plane(self.h, _dp(p))
         return p
 
     def lusol_helm(self, kRK, vgrid, rhs):

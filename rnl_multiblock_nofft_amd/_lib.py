@@ -27,7 +27,9 @@ EXPORTS = ["lh_last_error", "lh_ctx_create", "lh_ctx_destroy", "lh_set_grid", "l
            "lh_four_to_phys_planes", "lh_phys_to_four_planes", "lh_lusol_uw", "lh_lusol_v",
            "lh_dev_alloc", "lh_dev_free", "lh_copy_to_device", "lh_copy_to_host",
            "lh_set_tuning", "lh_get_tuning", "lh_autotune", "lh_set_texture",
-           "lh_stats_accumulate", "lh_stats_reset", "lh_stats_get", "lh_spectrum_get"]
+           "lh_stats_accumulate", "lh_stats_reset", "lh_stats_get", "lh_spectrum_get",
+           "lh_lusol_p", "lh_lusol0", "lh_four_to_phys_N_planes", "lh_phys_to_four_N_planes",
+           "lh_modes_to_planes", "lh_planes_to_modes"]
 ST_N = 13
 SP_IDS = {"U": 0, "V": 1, "W": 2, "UV": 3, "P": 4}
 
@@ -91,6 +93,12 @@ def load():
     L.lh_stats_reset.argtypes = [vp]
     L.lh_stats_get.argtypes = [vp, ip, dp, ctypes.POINTER(ip)]
     L.lh_spectrum_get.argtypes = [vp, ip, ip, dp]
+    L.lh_lusol_p.argtypes = [vp, ip, vp]
+    L.lh_lusol0.argtypes = [vp, ip, ip, vp]
+    L.lh_four_to_phys_N_planes.argtypes = [ip, ip, vp, ip, vp]
+    L.lh_phys_to_four_N_planes.argtypes = [ip, ip, vp, ip, vp]
+    L.lh_modes_to_planes.argtypes = [ip, ip, ip, ip, vp, ip, vp, vp]
+    L.lh_planes_to_modes.argtypes = [ip, ip, ip, ip, vp, ip, vp, vp]
     for name in EXPORTS:
         if name != "lh_last_error":
             getattr(L, name).restype = ctypes.c_int

@@ -963,6 +963,74 @@ int lh_lusol_v(lh_ctx *x, int ch, int kRK, const double *rv, double *v) {
   return LH_OK;
 }
 
+int lh_lusol_p(lh_ctx *x, int ch, double *d) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || !d) return set_err(LH_EINVAL, "lh_lusol_p: bad arguments");
+  HostChan &h = x->ch[ch];
+  HIPCHK(launch_lusol_p_ref(x->arena, x->d_chans + ch, (double2 *)d, h.ncol, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+int lh_lusol0(lh_ctx *x, int ch, int kRK, double *d) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || kRK < 1 || kRK > 3 || !d) return set_err(LH_EINVAL, "lh_lusol0: bad arguments");
+  HostChan &h = x->ch[ch];
+  HIPCHK(launch_lusol0_ref(x->arena, x->d_chans + ch, kRK, d, View(x->arena, h.dev).scr, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+static int planeN_args(int N1, int N2, double *d, int np) {
+  if (!d || np < 1 || N1 < 2 || (N1 & 1) || N2 < 2 || (N2 & 1) || !valid_len(N1) || !valid_len(N2))
+    return set_err(LH_EINVAL, "N-grid plane transform: bad arguments");
+  return LH_OK;
+}
+
+int lh_four_to_phys_N_planes(int N1, int N2, double *d, int np, void *stream) {
+  int rc = planeN_args(N1, N2, d, np);
+  if (rc) return rc;
+  double2 *tz, *tx;
+  rc = plane_tw(N1, N2, &tz, &tx);
+  if (rc) return rc;
+  HIPCHK(launch_plane_f2p(N1, N1, N2, d, np, tz, tx, (hipStream_t)stream, false));
+  return LH_OK;
+}
+
+int lh_phys_to_four_N_planes(int N1, int N2, double *d, int np, void *stream) {
+  int rc = planeN_args(N1, N2, d, np);
+  if (rc) return rc;
+  double2 *tz, *tx;
+  rc = plane_tw(N1, N2, &tz, &tx);
+  if (rc) return rc;
+  HIPCHK(launch_plane_p2f(N1, N1, N2, d, np, tz, tx, (hipStream_t)stream, false));
+  return LH_OK;
+}
+
+static int transpose_args(int N1, int N2, int Ngal1, int Ngal2, const void *a, const void *b, int nrows) {
+  if (!a || !b || nrows < 1 || N1 < 2 || (N1 & 1) || N2 < 2 || (N2 & 1) || Ngal1 < N1 || Ngal2 < N2 || (Ngal1 & 1))
+    return set_err(LH_EINVAL, "transpose: bad arguments");
+  return LH_OK;
+}
+
+int lh_modes_to_planes(int N1, int N2, int Ngal1, int Ngal2, const double *cols, int nrows, double *planes,
+                       void *stream) {
+  int rc = transpose_args(N1, N2, Ngal1, Ngal2, cols, planes, nrows);
+  if (rc) return rc;
+  HIPCHK(launch_modes_to_planes(N1, N2, Ngal1, Ngal2, (const double2 *)cols, nrows, planes, (hipStream_t)stream));
+  return LH_OK;
+}
+
+int lh_planes_to_modes(int N1, int N2, int Ngal1, int Ngal2, const double *planes, int nrows, double *cols,
+                       void *stream) {
+  int rc = transpose_args(N1, N2, Ngal1, Ngal2, planes, cols, nrows);
+  if (rc) return rc;
+  HIPCHK(launch_planes_to_modes(N1, N2, Ngal1, Ngal2, planes, nrows, (double2 *)cols, (hipStream_t)stream));
+  return LH_OK;
+}
+
 int lh_dev_alloc(size_t bytes, void **out) {
   if (!out || bytes == 0) return set_err(LH_EINVAL, "lh_dev_alloc: bad arguments");
   *out = nullptr;

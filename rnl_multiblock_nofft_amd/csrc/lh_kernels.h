@@ -68,4 +68,12 @@ hipError_t launch_lusol_ref2(char *arena, const Chan *d_chan, int kRK, int vgrid
                              const double2 *rhs_b, double2 *x_a, double2 *x_b, double *scr, int ncol,
                              hipStream_t s);
 
+hipError_t launch_lusol_p_ref(char *arena, const Chan *d_chan, double2 *x, int ncol, hipStream_t s);
+hipError_t launch_lusol0_ref(char *arena, const Chan *d_chan, int kRK, double *x, double *scr, hipStream_t s);
+// column <-> plane transposes on reference-layout device data
+hipError_t launch_modes_to_planes(int N1, int N2, int Ng1, int Ng2, const double2 *cols, int nrows, double *planes,
+                                  hipStream_t s);
+hipError_t launch_planes_to_modes(int N1, int N2, int Ng1, int Ng2, const double *planes, int nrows, double2 *cols,
+                                  hipStream_t s);
+
 }  // namespace lh

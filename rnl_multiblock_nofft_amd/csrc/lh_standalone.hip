@@ -2,8 +2,12 @@
 //   * plane transforms four_to_phys_du / phys_to_four_du (FOU3D.f90:766-784,
 //     2253-2272) on (Ngal1+2) x Ngal2 Fortran-order planes, FFTPACK
 //     conventions (cft_buff.f:1-65, rft_buff.f:24-90);
-//   * the Helmholtz solves of LUsolU_W / LUsolV (tridLU_3D.f90:55-189) on
-//     x(jlo:jhi, ncol) arrays.
+//   * the Helmholtz solves of LUsolU_W / LUsolV (tridLU_3D.f90:55-189), the
+//     Poisson solve LUsolP (tridLU_3D.f90:192-226) and the mean-mode solve
+//     LUsol0 (tridLU_3D.f90:228-254) on x(jlo:jhi, ncol) arrays;
+//   * the column <-> plane transposes modes_to_planes_UVP /
+//     planes_to_modes_UVP / planes_to_modes_NUVP (FOU3D.f90:1000-1098,
+//     2295-2442, 2772-2854) as LDS-tiled transposes.
 #include "lh_common.h"
 #include "lh_fft.h"
 #include "lh_kernels.h"
@@ -208,6 +212,143 @@ hipError_t launch_lusol_ref2(char *arena, const Chan *d_chan, int kRK, int vgrid
                              hipStream_t s) {
   hipLaunchKernelGGL(k_lusol_ref, dim3((ncol + 63) / 64), dim3(64), 0, s, arena, d_chan, kRK, vgrid, rhs_a, rhs_b,
                      x_a, x_b, scr, ncol);
+  return hipGetLastError();
+}
+
+// LUsolP (tridLU_3D.f90:192-226) on x(1:ny-1, ncol): the Poisson operator of
+// LU_buildP with the mean-mode pin, from the factors k_factors eliminated
+// from the top row down (dP: inverse pivots, eP: multipliers); the upper
+// off-diagonal is recomputed from the grid.  One lane per column.
+__global__ void __launch_bounds__(64) k_lusol_p_ref(char *arena, const Chan *chp, double2 *x, int ncol) {
+  const Chan ch = *chp;
+  const View v(arena, ch);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int ldc = ch.ldc, ns = 1, ne = ch.nyu - 2, n = ne - ns + 1;
+  const size_t plane = (size_t)(ch.nyu + PADROWS) * ldc;
+  const double *dP = v.fac + (size_t)12 * plane, *eP = dP + plane;
+  double2 *col = x + (size_t)c * n - ns;  // col[j], j = ns..ne
+  double2 y = col[ne];
+  y = make_double2(y.x * dP[(size_t)ne * ldc + c], y.y * dP[(size_t)ne * ldc + c]);
+  col[ne] = y;
+  for (int q = ne - 1; q >= ns; q--) {
+    const double a3 = (c == 0 && q == ns) ? 0.0 : v.dthdyu[q] * v.dthdyv[q];
+    const double d = dP[(size_t)q * ldc + c];
+    const double2 r = col[q];
+    y = make_double2((r.x - a3 * y.x) * d, (r.y - a3 * y.y) * d);
+    col[q] = y;
+  }
+  for (int q = ns + 1; q <= ne; q++) {
+    const double e = eP[(size_t)q * ldc + c];
+    const double2 r = col[q];
+    y = make_double2(r.x - e * y.x, r.y - e * y.y);
+    col[q] = y;
+  }
+}
+
+// LUsol0 (tridLU_3D.f90:228-254) with LU_build0 / LU_dec0 (:409-435,
+// :520-537): the real mean-mode U system of substep kRK on x(0:ny), in place.
+__global__ void k_lusol0_ref(char *arena, const Chan *chp, int kRK, double *x, double *cs) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const Chan ch = *chp;
+  const View v(arena, ch);
+  const int n = ch.nyu;
+  const double beta = rk_b(kRK) * v.scal[SC_DT] / ch.Re;
+  double c = ch.gw1;  // row 0: x0 + gw1 x1, pivot 1
+  cs[0] = c;
+  for (int j = 1; j < n; j++) {
+    double a1, a2, a3;
+    if (j <= n - 2) {
+      a1 = -beta * v.dyu2i[3 * j];
+      a2 = 1.0 - beta * v.dyu2i[3 * j + 1];
+      a3 = -beta * v.dyu2i[3 * j + 2];
+    } else {
+      a1 = ch.gw2;
+      a2 = 1.0;
+      a3 = 0.0;
+    }
+    const double d = 1.0 / (a2 - a1 * c);
+    c = a3 * d;
+    cs[j] = c;
+    x[j] = (x[j] - a1 * x[j - 1]) * d;
+  }
+  for (int j = n - 2; j >= 0; j--) x[j] = x[j] - cs[j] * x[j + 1];
+}
+
+hipError_t launch_lusol_p_ref(char *arena, const Chan *d_chan, double2 *x, int ncol, hipStream_t s) {
+  hipLaunchKernelGGL(k_lusol_p_ref, dim3((ncol + 63) / 64), dim3(64), 0, s, arena, d_chan, x, ncol);
+  return hipGetLastError();
+}
+
+hipError_t launch_lusol0_ref(char *arena, const Chan *d_chan, int kRK, double *x, double *scr, hipStream_t s) {
+  hipLaunchKernelGGL(k_lusol0_ref, dim3(1), dim3(64), 0, s, arena, d_chan, kRK, x, scr);
+  return hipGetLastError();
+}
+
+// Column <-> plane transposes.  Columns: x(jlo:jhi, ncol) complex, rows
+// contiguous (the reference's band arrays); planes: (Ngal1+2) x Ngal2
+// doubles per row, Fortran order, mode (i, k) at xPL(2i+1, krow),
+// xPL(2i+2, krow) with krow = k below N2/2 and k + Ngal2 - N2 above (negative
+// z modes at the top, modes_to_planes_UVP's k - dk).  A workgroup moves a
+// tile of 64 columns x 16 rows through LDS, so both sides are read and
+// written in contiguous runs.
+constexpr int TR_C = 64, TR_J = 16;
+
+__device__ __forceinline__ size_t plane_pos(int c, int nxh, int N2, int Ng2, int igal) {
+  const int i = c % nxh, k = c / nxh;
+  const int krow = k < N2 / 2 ? k : k + Ng2 - N2;
+  return (size_t)krow * igal + 2 * i;
+}
+
+__global__ void __launch_bounds__(256) k_modes_to_planes(int N1, int N2, int Ng1, int Ng2, const double2 *cols,
+                                                         int nrows, double *planes) {
+  __shared__ double2 tile[TR_C][TR_J + 1];
+  const int nxh = N1 / 2 + 1, ncol = nxh * N2, igal = Ng1 + 2;
+  const int c0 = blockIdx.x * TR_C, j0 = blockIdx.y * TR_J;
+  for (int e = threadIdx.x; e < TR_C * TR_J; e += blockDim.x) {
+    const int cl = e / TR_J, jl = e % TR_J, c = c0 + cl, j = j0 + jl;
+    if (c < ncol && j < nrows) tile[cl][jl] = cols[(size_t)c * nrows + j];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < TR_C * TR_J; e += blockDim.x) {
+    const int jl = e / TR_C, cl = e % TR_C, c = c0 + cl, j = j0 + jl;
+    if (c < ncol && j < nrows)
+      *(double2 *)(planes + (size_t)j * igal * Ng2 + plane_pos(c, nxh, N2, Ng2, igal)) = tile[cl][jl];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_planes_to_modes(int N1, int N2, int Ng1, int Ng2, const double *planes,
+                                                         int nrows, double2 *cols) {
+  __shared__ double2 tile[TR_C][TR_J + 1];
+  const int nxh = N1 / 2 + 1, ncol = nxh * N2, igal = Ng1 + 2;
+  const int c0 = blockIdx.x * TR_C, j0 = blockIdx.y * TR_J;
+  for (int e = threadIdx.x; e < TR_C * TR_J; e += blockDim.x) {
+    const int jl = e / TR_C, cl = e % TR_C, c = c0 + cl, j = j0 + jl;
+    if (c < ncol && j < nrows)
+      tile[cl][jl] = *(const double2 *)(planes + (size_t)j * igal * Ng2 + plane_pos(c, nxh, N2, Ng2, igal));
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < TR_C * TR_J; e += blockDim.x) {
+    const int cl = e / TR_J, jl = e % TR_J, c = c0 + cl, j = j0 + jl;
+    if (c < ncol && j < nrows) cols[(size_t)c * nrows + j] = tile[cl][jl];
+  }
+}
+
+hipError_t launch_modes_to_planes(int N1, int N2, int Ng1, int Ng2, const double2 *cols, int nrows, double *planes,
+                                  hipStream_t s) {
+  const int ncol = (N1 / 2 + 1) * N2;
+  hipError_t e = hipMemsetAsync(planes, 0, (size_t)nrows * (Ng1 + 2) * Ng2 * sizeof(double), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_modes_to_planes, dim3((ncol + TR_C - 1) / TR_C, (nrows + TR_J - 1) / TR_J), dim3(256), 0, s,
+                     N1, N2, Ng1, Ng2, cols, nrows, planes);
+  return hipGetLastError();
+}
+
+hipError_t launch_planes_to_modes(int N1, int N2, int Ng1, int Ng2, const double *planes, int nrows, double2 *cols,
+                                  hipStream_t s) {
+  const int ncol = (N1 / 2 + 1) * N2;
+  hipLaunchKernelGGL(k_planes_to_modes, dim3((ncol + TR_C - 1) / TR_C, (nrows + TR_J - 1) / TR_J), dim3(256), 0, s,
+                     N1, N2, Ng1, Ng2, planes, nrows, cols);
   return hipGetLastError();
 }
 

@@ -10,7 +10,9 @@
 !
 ! Routines with the reference's names:
 !   four_to_phys_u, four_to_phys_du, phys_to_four_du   (FOU3D.f90:738-784,2253)
-!   LUsolU_W, LUsolV                                    (tridLU_3D.f90:55,135)
+!   four_to_phys_N, phys_to_four_N                      (FOU3D.f90:786-804,2274-2293)
+!   LUsolU_W, LUsolV, LUsolP, LUsol0                    (tridLU_3D.f90:55,135,192,228)
+!   modes_to_planes_UVP, planes_to_modes_UVP/_NUVP      (FOU3D.f90:1000,2295,2772)
 !   read_in_field, record_out_field                     (start.f90:2444, FOU3D.f90:2856)
 !   write_stats_file                                    (stats.f90:254-441)
 ! The LU factor argument `a`, `grid` and `myid` of LUsolU_W/LUsolV are
@@ -127,6 +129,120 @@ contains
     call lh_check(lh_copy_to_host(c_loc(pl), d, bytes), 'lh_copy_to_host')
   end subroutine plane_transform
 
+  ! FOU3D.f90:786 four_to_phys_N / :2274 phys_to_four_N: the same transforms
+  ! on the N1 x N2 grid, planes (N1+2, N2), no oddball-row zeroing
+  subroutine four_to_phys_N(du, iband)
+    integer, intent(in) :: iband
+    real(8), intent(inout), target :: du(N(1, iband) + 2, N(2, iband))
+    call plane_transform_N(du, iband, 1)
+  end subroutine four_to_phys_N
+
+  subroutine phys_to_four_N(du, iband)
+    integer, intent(in) :: iband
+    real(8), intent(inout), target :: du(N(1, iband) + 2, N(2, iband))
+    call plane_transform_N(du, iband, -1)
+  end subroutine phys_to_four_N
+
+  subroutine plane_transform_N(pl, iband, dir)
+    integer, intent(in) :: iband, dir
+    real(8), intent(inout), target :: pl(N(1, iband) + 2, N(2, iband))
+    integer(c_size_t) :: bytes
+    type(c_ptr) :: d
+    bytes = int(size(pl), c_size_t) * 8_c_size_t
+    d = staging(bytes, 0_c_size_t)
+    call lh_check(lh_copy_to_device(d, c_loc(pl), bytes), 'lh_copy_to_device')
+    if (dir > 0) then
+      call lh_check(lh_four_to_phys_N_planes(int(N(1, iband), c_int), int(N(2, iband), c_int), d, 1_c_int, &
+                    c_null_ptr), 'lh_four_to_phys_N_planes')
+    else
+      call lh_check(lh_phys_to_four_N_planes(int(N(1, iband), c_int), int(N(2, iband), c_int), d, 1_c_int, &
+                    c_null_ptr), 'lh_phys_to_four_N_planes')
+    end if
+    call lh_check(lh_copy_to_host(c_loc(pl), d, bytes), 'lh_copy_to_host')
+  end subroutine plane_transform_N
+
+  ! ---------------------------------------------------------------------
+  ! Column <-> plane transposes (FOU3D.f90:1000 modes_to_planes_UVP, :2295
+  ! planes_to_modes_UVP, :2772 planes_to_modes_NUVP) for one band: x holds
+  ! rows jlo:jhi of the grid (1 = v: 0..ny-1, 2 = u: 0..ny, 3 = p: 1..ny-1),
+  ! xPL the same rows as (Ngal1+2, Ngal2) planes.  modes_to_planes_UVP zeroes
+  ! the planes outside the stored modes; planes_to_modes_NUVP fills only the
+  ! rows inside the walls (jlo+1..jhi-1), as the reference's does.
+  subroutine grid_rows(grid, iband, jlo, jhi)
+    integer, intent(in) :: grid, iband
+    integer, intent(out) :: jlo, jhi
+    select case (grid)
+    case (1)
+      jlo = 0; jhi = nyb(iband) - 1
+    case (2)
+      jlo = 0; jhi = nyb(iband)
+    case default
+      jlo = 1; jhi = nyb(iband) - 1
+    end select
+  end subroutine grid_rows
+
+  subroutine modes_to_planes_UVP(xPL, x, grid, iband)
+    integer, intent(in) :: grid, iband
+    real(8), intent(out), target :: xPL(Ngal(1, iband) + 2, Ngal(2, iband), *)
+    complex(8), intent(in), target :: x(*)
+    call transpose_band(xPL, x, grid, iband, 1, 0)
+  end subroutine modes_to_planes_UVP
+
+  subroutine planes_to_modes_UVP(x, xPL, grid, iband)
+    integer, intent(in) :: grid, iband
+    complex(8), intent(inout), target :: x(*)
+    real(8), intent(in), target :: xPL(Ngal(1, iband) + 2, Ngal(2, iband), *)
+    call transpose_band(xPL, x, grid, iband, -1, 0)
+  end subroutine planes_to_modes_UVP
+
+  subroutine planes_to_modes_NUVP(x, xPL, grid, iband)
+    integer, intent(in) :: grid, iband
+    complex(8), intent(inout), target :: x(*)
+    real(8), intent(in), target :: xPL(Ngal(1, iband) + 2, Ngal(2, iband), *)
+    call transpose_band(xPL, x, grid, iband, -1, 1)
+  end subroutine planes_to_modes_NUVP
+
+  subroutine transpose_band(xPL, x, grid, iband, dir, inner)
+    integer, intent(in) :: grid, iband, dir, inner
+    ! written or read according to dir
+    real(8), target :: xPL(Ngal(1, iband) + 2, Ngal(2, iband), *)
+    complex(8), target :: x(*)
+    integer :: jlo, jhi, nrows, ncol, q
+    integer(c_size_t) :: bc, bp
+    type(c_ptr) :: dc, dp
+    complex(8), allocatable, target :: tmp(:,:)
+    call grid_rows(grid, iband, jlo, jhi)
+    nrows = jhi - jlo + 1
+    ncol = (N(1, iband) / 2 + 1) * N(2, iband)
+    bc = int(nrows, c_size_t) * int(ncol, c_size_t) * 16_c_size_t
+    bp = int(nrows, c_size_t) * int(Ngal(1, iband) + 2, c_size_t) * int(Ngal(2, iband), c_size_t) * 8_c_size_t
+    dc = staging(bc + bp, 0_c_size_t)
+    dp = staging(bc + bp, bc)
+    if (dir > 0) then
+      call lh_check(lh_copy_to_device(dc, c_loc(x), bc), 'lh_copy_to_device')
+      call lh_check(lh_modes_to_planes(int(N(1, iband), c_int), int(N(2, iband), c_int), &
+                    int(Ngal(1, iband), c_int), int(Ngal(2, iband), c_int), dc, int(nrows, c_int), dp, &
+                    c_null_ptr), 'lh_modes_to_planes')
+      call lh_check(lh_copy_to_host(c_loc(xPL), dp, bp), 'lh_copy_to_host')
+    else
+      call lh_check(lh_copy_to_device(dp, c_loc(xPL), bp), 'lh_copy_to_device')
+      call lh_check(lh_planes_to_modes(int(N(1, iband), c_int), int(N(2, iband), c_int), &
+                    int(Ngal(1, iband), c_int), int(Ngal(2, iband), c_int), dp, int(nrows, c_int), dc, &
+                    c_null_ptr), 'lh_planes_to_modes')
+      if (inner == 0) then
+        call lh_check(lh_copy_to_host(c_loc(x), dc, bc), 'lh_copy_to_host')
+      else
+        ! only the rows inside the walls change
+        allocate(tmp(jlo:jhi, ncol))
+        call lh_check(lh_copy_to_host(c_loc(tmp), dc, bc), 'lh_copy_to_host')
+        do q = 1, ncol
+          x((q - 1) * nrows + 2:(q - 1) * nrows + nrows - 1) = tmp(jlo + 1:jhi - 1, q)
+        end do
+        deallocate(tmp)
+      end if
+    end if
+  end subroutine transpose_band
+
   ! ---------------------------------------------------------------------
   ! tridLU_3D.f90:55 LUsolU_W: (I - beta (D2 + k^2)) u = rhsu and the same
   ! operator for w, for substep kRK, all (kx,kz) columns of band iband.
@@ -165,6 +281,34 @@ contains
     call lh_check(lh_lusol_v(lhctx, int(iband - 1, c_int), int(kRK, c_int), d1, d2), 'lh_lusol_v')
     call lh_check(lh_copy_to_host(c_loc(u), d2, b), 'lh_copy_to_host')
   end subroutine LUsolV
+
+  ! tridLU_3D.f90:192 LUsolP: the Poisson operator of LU_buildP (with the
+  ! mean-mode pin) solved in place on x(1:ny-1, ncol)
+  subroutine LUsolP(x, iband)
+    integer, intent(in) :: iband
+    complex(8), intent(inout), target :: x(1:nyb(iband) - 1, (N(1, iband) / 2 + 1) * N(2, iband))
+    integer(c_size_t) :: b
+    type(c_ptr) :: d
+    b = int(size(x), c_size_t) * 16_c_size_t
+    d = staging(b, 0_c_size_t)
+    call lh_check(lh_copy_to_device(d, c_loc(x), b), 'lh_copy_to_device')
+    call lh_check(lh_lusol_p(lhctx, int(iband - 1, c_int), d), 'lh_lusol_p')
+    call lh_check(lh_copy_to_host(c_loc(x), d, b), 'lh_copy_to_host')
+  end subroutine LUsolP
+
+  ! tridLU_3D.f90:228 LUsol0: the real (0,0) U system of substep kRK, in
+  ! place on x(0:ny)
+  subroutine LUsol0(x, iband)
+    integer, intent(in) :: iband
+    real(8), intent(inout), target :: x(0:nyb(iband))
+    integer(c_size_t) :: b
+    type(c_ptr) :: d
+    b = int(size(x), c_size_t) * 8_c_size_t
+    d = staging(b, 0_c_size_t)
+    call lh_check(lh_copy_to_device(d, c_loc(x), b), 'lh_copy_to_device')
+    call lh_check(lh_lusol0(lhctx, int(iband - 1, c_int), int(kRK, c_int), d), 'lh_lusol0')
+    call lh_check(lh_copy_to_host(c_loc(x), d, b), 'lh_copy_to_host')
+  end subroutine LUsol0
 
   ! ---------------------------------------------------------------------
   ! Field files in the format record_out writes and read_in reads

@@ -1,15 +1,24 @@
 ! lh_shim_check -- drives the reference-named shims of lh_host on raw data so
 ! that a test can compare them with an independent implementation:
 !   four_to_phys_du / phys_to_four_du on `nplanes` planes (Ngal1+2, Ngal2),
-!   LUsolU_W and LUsolV on right-hand sides in the reference layout.
+!   four_to_phys_N / phys_to_four_N on planes (N1+2, N2),
+!   LUsolU_W, LUsolV, LUsolP and LUsol0 on right-hand sides in the reference
+!   layout, and the transposes modes_to_planes_UVP, planes_to_modes_UVP and
+!   planes_to_modes_NUVP on u-grid data.
 !
 !   lh_shim_check check.nml
 !
 ! Files are raw little-endian FP64 streams (no record markers):
 !   planes_in   nplanes * (Ngal1+2) * Ngal2
 !   rhsu, rhsw  (0:ny, ncol) complex;  rhsv (0:ny-1, ncol) complex
+!   planesN_in  nplanes * (N1+2) * N2
+!   rhsp (1:ny-1, ncol) complex; rhs0 (0:ny) real
+!   colsu (0:ny, ncol) complex; plu (Ngal1+2, Ngal2, 0:ny)
 ! Outputs: planes_phys (after four_to_phys_du), planes_four (after
-! phys_to_four_du of planes_phys), sol_u, sol_w, sol_v.
+! phys_to_four_du of planes_phys), planesN_phys, planesN_four, sol_u, sol_w,
+! sol_v, sol_p, sol_0, plu_out (modes_to_planes_UVP of colsu), colsu_out
+! (planes_to_modes_UVP of plu), colsn_out (colsu after planes_to_modes_NUVP
+! of plu).
 program lh_shim_check
   use iso_c_binding
   use lhgpu
@@ -22,8 +31,9 @@ program lh_shim_check
   namelist /check/ N1, N2, ny, alp, bet, Re, grid_kind, dyq, ppp, gamma, flag_ctpress, dt, &
                    nplanes, kstage, dir
   type(lh_config) :: cfg(1)
-  real(8), allocatable :: pl(:,:,:)
-  complex(8), allocatable :: ru(:,:), rw(:,:), su(:,:), sw(:,:), rv(:,:), sv(:,:)
+  real(8), allocatable :: pl(:,:,:), plN(:,:,:), r0(:), plu(:,:,:), plo(:,:,:)
+  complex(8), allocatable :: ru(:,:), rw(:,:), su(:,:), sw(:,:), rv(:,:), sv(:,:), rp(:,:)
+  complex(8), allocatable :: cu(:,:), co(:,:)
   integer :: u, q, ncol
 
   call get_command_argument(1, nmlfile)
@@ -57,6 +67,36 @@ program lh_shim_check
   call wrc(trim(dir) // '/sol_u', su, size(su))
   call wrc(trim(dir) // '/sol_w', sw, size(sw))
   call wrc(trim(dir) // '/sol_v', sv, size(sv))
+
+  allocate(plN(N1 + 2, N2, nplanes))
+  call rd(trim(dir) // '/planesN_in', plN, size(plN))
+  do q = 1, nplanes
+    call four_to_phys_N(plN(:, :, q), 1)
+  end do
+  call wr(trim(dir) // '/planesN_phys', plN, size(plN))
+  do q = 1, nplanes
+    call phys_to_four_N(plN(:, :, q), 1)
+  end do
+  call wr(trim(dir) // '/planesN_four', plN, size(plN))
+
+  allocate(rp(1:ny - 1, ncol), r0(0:ny))
+  call rdc(trim(dir) // '/rhsp', rp, size(rp))
+  call LUsolP(rp, 1)
+  call wrc(trim(dir) // '/sol_p', rp, size(rp))
+  call rd(trim(dir) // '/rhs0', r0, size(r0))
+  call LUsol0(r0, 1)
+  call wr(trim(dir) // '/sol_0', r0, size(r0))
+
+  allocate(cu(0:ny, ncol), co(0:ny, ncol))
+  allocate(plu(Ngal(1, 1) + 2, Ngal(2, 1), 0:ny), plo(Ngal(1, 1) + 2, Ngal(2, 1), 0:ny))
+  call rdc(trim(dir) // '/colsu', cu, size(cu))
+  call rd(trim(dir) // '/plu', plu, size(plu))
+  call modes_to_planes_UVP(plo, cu, 2, 1)
+  call wr(trim(dir) // '/plu_out', plo, size(plo))
+  call planes_to_modes_UVP(co, plu, 2, 1)
+  call wrc(trim(dir) // '/colsu_out', co, size(co))
+  call planes_to_modes_NUVP(cu, plu, 2, 1)
+  call wrc(trim(dir) // '/colsn_out', cu, size(cu))
   call lh_host_finish()
   write(*, '(a)') 'lh_shim_check: done'
 

@@ -210,6 +210,46 @@ module lhgpu
       integer(c_int), value :: ch
       integer(c_int) :: rc
     end function
+    function lh_lusol_p(ctx, ch, x) bind(C, name='lh_lusol_p') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, x
+      integer(c_int), value :: ch
+      integer(c_int) :: rc
+    end function
+    function lh_lusol0(ctx, ch, kRK, x) bind(C, name='lh_lusol0') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, x
+      integer(c_int), value :: ch, kRK
+      integer(c_int) :: rc
+    end function
+    function lh_four_to_phys_N_planes(N1, N2, planes, nplanes, stream) &
+        bind(C, name='lh_four_to_phys_N_planes') result(rc)
+      import :: c_ptr, c_int
+      integer(c_int), value :: N1, N2, nplanes
+      type(c_ptr), value :: planes, stream
+      integer(c_int) :: rc
+    end function
+    function lh_phys_to_four_N_planes(N1, N2, planes, nplanes, stream) &
+        bind(C, name='lh_phys_to_four_N_planes') result(rc)
+      import :: c_ptr, c_int
+      integer(c_int), value :: N1, N2, nplanes
+      type(c_ptr), value :: planes, stream
+      integer(c_int) :: rc
+    end function
+    function lh_modes_to_planes(N1, N2, Ngal1, Ngal2, cols, nrows, planes, stream) &
+        bind(C, name='lh_modes_to_planes') result(rc)
+      import :: c_ptr, c_int
+      integer(c_int), value :: N1, N2, Ngal1, Ngal2, nrows
+      type(c_ptr), value :: cols, planes, stream
+      integer(c_int) :: rc
+    end function
+    function lh_planes_to_modes(N1, N2, Ngal1, Ngal2, planes, nrows, cols, stream) &
+        bind(C, name='lh_planes_to_modes') result(rc)
+      import :: c_ptr, c_int
+      integer(c_int), value :: N1, N2, Ngal1, Ngal2, nrows
+      type(c_ptr), value :: planes, cols, stream
+      integer(c_int) :: rc
+    end function
     function lh_stats_accumulate(ctx) bind(C, name='lh_stats_accumulate') result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx

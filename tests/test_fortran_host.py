@@ -118,6 +118,16 @@ def test_fortran_shims_match_oracle(tmp_path):
     rv = rng.standard_normal((ncol, ny)) + 1j * rng.standard_normal((ncol, ny))
     for nm, a in (("rhsu", ru), ("rhsw", rw), ("rhsv", rv)):
         a.astype(np.complex128).tofile(str(tmp_path / nm))
+    planesN = rng.standard_normal((nplanes, cfg["N2"], cfg["N1"] + 2))
+    planesN.tofile(str(tmp_path / "planesN_in"))
+    rp = rng.standard_normal((ncol, ny - 1)) + 1j * rng.standard_normal((ncol, ny - 1))
+    rp.tofile(str(tmp_path / "rhsp"))
+    r0 = rng.standard_normal(ny + 1)
+    r0.tofile(str(tmp_path / "rhs0"))
+    cu = rng.standard_normal((ncol, ny + 1)) + 1j * rng.standard_normal((ncol, ny + 1))
+    cu.tofile(str(tmp_path / "colsu"))
+    plu = rng.standard_normal((ny + 1, Ng2, Ng1 + 2))
+    plu.tofile(str(tmp_path / "plu"))
     nml = str(tmp_path / "check.nml")
     d = {k: cfg[k] for k in CFG_KEYS}
     d.update(nplanes=nplanes, kstage=kstage, dir=str(tmp_path))
@@ -140,6 +150,42 @@ def test_fortran_shims_match_oracle(tmp_path):
     ev = relerr(sv, o.lusol_helm(kstage, 1, rv))
     say("lusol", eu, ew, ev)
     assert max(eu, ew, ev) <= 1e-13
+
+    # N-grid transforms (four_to_phys_N / phys_to_four_N)
+    N1, N2 = cfg["N1"], cfg["N2"]
+    pN = np.fromfile(str(tmp_path / "planesN_phys")).reshape(planesN.shape)
+    fN = np.fromfile(str(tmp_path / "planesN_four")).reshape(planesN.shape)
+    for q in range(nplanes):
+        ref_p = o.four_to_phys_N(planesN[q])
+        e1 = relerr(pN[q][:, :N1], ref_p[:, :N1])
+        e2 = relerr(fN[q], o.phys_to_four_N(ref_p))
+        say("planeN", q, e1, e2)
+        assert e1 <= 1e-13 and e2 <= 1e-13
+    # LUsolP, LUsol0
+    sp = np.fromfile(str(tmp_path / "sol_p"), dtype=np.complex128).reshape(ncol, ny - 1)
+    ep = relerr(sp, o.lusolp(rp))
+    s0 = np.fromfile(str(tmp_path / "sol_0"))
+    e0 = relerr(s0, o.lusol0(kstage, r0.copy()))
+    say("lusolp", ep, "lusol0", e0)
+    assert ep <= 1e-13 and e0 <= 1e-14
+    # transposes (modes_to_planes_UVP FOU3D.f90:1000-1098: mode (i, k) at
+    # (2i+1, krow), krow = k or k + Ngal2 - N2 for the upper half)
+    nxh = N1 // 2 + 1
+    kr = np.array([k if k < N2 // 2 else k + Ng2 - N2 for k in range(N2)])
+    exp_pl = np.zeros((ny + 1, Ng2, Ng1 + 2))
+    cc = cu.T.reshape(ny + 1, N2, nxh)
+    exp_pl[:, kr, 0:2 * nxh:2] = cc.real
+    exp_pl[:, kr, 1:2 * nxh:2] = cc.imag
+    plo = np.fromfile(str(tmp_path / "plu_out")).reshape(ny + 1, Ng2, Ng1 + 2)
+    assert np.array_equal(plo, exp_pl)
+    back = plu[:, kr, 0:2 * nxh:2] + 1j * plu[:, kr, 1:2 * nxh:2]     # (rows, N2, nxh)
+    exp_c = back.reshape(ny + 1, ncol).T
+    co = np.fromfile(str(tmp_path / "colsu_out"), dtype=np.complex128).reshape(ncol, ny + 1)
+    assert np.array_equal(co, exp_c)
+    cn = np.fromfile(str(tmp_path / "colsn_out"), dtype=np.complex128).reshape(ncol, ny + 1)
+    exp_n = cu.copy()
+    exp_n[:, 1:ny] = exp_c[:, 1:ny]
+    assert np.array_equal(cn, exp_n)
 
 
 @pytest.mark.gpu

@@ -190,3 +190,20 @@ def test_mask_generators():
     assert (bc == 1).all()                             # posts fill the tiles
     bc = textures.planebc(16, 16, 2, 2, Lfracx=2.0, Lfracz=8.0 / 3.0)
     assert bc[0, 0] == 1 and bc.dtype == np.int32
+
+
+def test_oracle_N_grid_transforms_match_numpy():
+    """four_to_phys_N / phys_to_four_N of the oracle (FOU3D.f90:786-804,
+    2274-2293) against numpy's transforms with the FFTPACK conventions."""
+    cfg = small_cfg(N=8, ny=9)
+    o = oracle_for(cfg)
+    N1, N2 = cfg["N1"], cfg["N2"]
+    rng = np.random.default_rng(12)
+    nxh = N1 // 2 + 1
+    P = rng.standard_normal((N2, N1 + 2))
+    cols = (P[:, 0::2] + 1j * P[:, 1::2]).reshape(-1)
+    phys = o.four_to_phys_N(P)
+    assert np.allclose(phys[:, :N1], T_inv(cols, N1, N2), rtol=0, atol=1e-13)
+    back = o.phys_to_four_N(phys)
+    fwd = T_fwd(phys[:, :N1], N1, N2).reshape(N2, nxh)
+    assert np.allclose(back[:, 0::2], fwd.real, atol=1e-14) and np.allclose(back[:, 1::2], fwd.imag, atol=1e-14)
