@@ -200,6 +200,11 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
 #define LH_TUNE_ZINV_POINTS 2
 #define LH_TUNE_XPROD_POINTS 3
 #define LH_TUNE_ZFWD_POINTS 4
+/* LH_TUNE_TEX_SEPARABLE (default 1): channels whose textured walls have
+ * x-independent masks (streamwise ridges) iterate the wall sources per
+ * x-mode with z transforms only; 0 forces the two-dimensional wall
+ * transforms for every textured channel.  Results agree to rounding. */
+#define LH_TUNE_TEX_SEPARABLE 5
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

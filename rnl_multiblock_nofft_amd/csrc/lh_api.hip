@@ -48,6 +48,7 @@ struct HostChan {
   double gw1, gw2, gwbc1, gwbc3, gwi1, gwi2, dtv;
   Chan dev;  // offsets into the context arena
   int iter = 0;
+  int xinv = 0;  // every textured wall's mask is independent of x
 };
 
 double ipow(double x, int n) {
@@ -220,6 +221,8 @@ struct lh_ctx {
   int ntex = 0;                      // channels with a textured wall
   int tex_nit = 3;                   // wall-source iterations per substep
   int tex_fused = 1;                 // LDS-resident wall transforms when they fit
+  int tex_sep = 1;                   // per-x-mode wall iteration for x-independent masks
+  int ntex1d = 0;                    // channels on that path (the rest: ntex - ntex1d)
   bool adaptive = false;             // a channel with an adaptive time step
   char *stats = nullptr;             // statistics accumulators (allocated on first use)
   size_t stats_bytes = 0;
@@ -414,18 +417,37 @@ int tex_part(lh_ctx *x, int kRK, int part) {
   if (!x->ntex) return LH_OK;
   const size_t lds = tex_fused_bytes(x->tex_groups);
   const bool fused = x->tex_fused && lds <= 160 * 1024;
+  const bool any2d = x->ntex > x->ntex1d;
   if (part == 0) {
-    if (fused)
-      HIPCHK(launch_tex_sources_fused(x->arena, x->d_chans, x->shape, lds, kRK, x->tex_nit, x->stream));
-    else
-      HIPCHK(launch_tex_sources(x->arena, x->d_chans, x->shape, x->tex_groups, kRK, x->tex_nit, x->stream));
+    HIPCHK(launch_tex_adj(x->arena, x->d_chans, x->shape, kRK, x->stream));
+    if (any2d) {
+      if (fused)
+        HIPCHK(launch_tex_sources_fused(x->arena, x->d_chans, x->shape, lds, kRK, x->tex_nit, x->stream));
+      else
+        HIPCHK(launch_tex_sources(x->arena, x->d_chans, x->shape, x->tex_groups, kRK, x->tex_nit, x->stream));
+    }
+    if (x->ntex1d) HIPCHK(launch_tex_1d(x->arena, x->d_chans, x->shape, kRK, x->tex_nit, 0, x->stream));
   } else {
-    if (fused)
-      HIPCHK(launch_tex_ghosts_fused(x->arena, x->d_chans, x->shape, lds, x->stream));
-    else
-      HIPCHK(launch_tex_ghosts(x->arena, x->d_chans, x->shape, x->tex_groups, x->stream));
+    if (any2d) {
+      if (fused)
+        HIPCHK(launch_tex_ghosts_fused(x->arena, x->d_chans, x->shape, lds, x->stream));
+      else
+        HIPCHK(launch_tex_ghosts(x->arena, x->d_chans, x->shape, x->tex_groups, x->stream));
+    }
+    if (x->ntex1d) HIPCHK(launch_tex_1d(x->arena, x->d_chans, x->shape, 1, 1, 1, x->stream));
   }
   return LH_OK;
+}
+
+// which textured channels take the per-x-mode path (all their textured
+// walls x-independent, and the path enabled)
+void tex_classify(lh_ctx *x) {
+  x->ntex = x->ntex1d = 0;
+  for (auto &h : x->ch) {
+    h.dev.tex1d = (h.dev.tex && x->tex_sep && h.xinv) ? 1 : 0;
+    x->ntex += h.dev.tex != 0;
+    x->ntex1d += h.dev.tex1d;
+  }
 }
 
 // column solves of a substep (with the texture parts around them)
@@ -1092,6 +1114,15 @@ static void drop_graph(lh_ctx *x) {
 int lh_set_tuning(lh_ctx *x, int key, int value) {
   int rc = check_ctx(x);
   if (rc) return rc;
+  if (key == LH_TUNE_TEX_SEPARABLE) {
+    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_TEX_SEPARABLE is 0 or 1");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->tex_sep = value;
+    tex_classify(x);
+    if ((rc = push_descriptors(x))) return rc;
+    drop_graph(x);
+    return LH_OK;
+  }
   if (key == LH_TUNE_SOLVE_COLS) {
     if (!solve_cols_ok(x, value))
       return set_err(LH_EINVAL,
@@ -1119,6 +1150,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
   int rc = check_ctx(x);
   if (rc) return rc;
   if (!value) return set_err(LH_EINVAL, "lh_get_tuning: null value");
+  if (key == LH_TUNE_TEX_SEPARABLE) {
+    *value = x->tex_sep;
+    return LH_OK;
+  }
   if (key == LH_TUNE_SOLVE_COLS) {
     *value = x->shape.solve_cols;
     return LH_OK;
@@ -1221,13 +1256,15 @@ int lh_set_texture(lh_ctx *x, int ch, const int *bottom, const int *top) {
   const size_t n = (size_t)h.cfg.N1 * h.cfg.N2;
   std::vector<int> m(2 * n, 1);
   const int *src[2] = {bottom, top};
-  int bits = 0;
+  int bits = 0, xinv = 1;
+  const int N1 = h.cfg.N1;
   for (int w = 0; w < 2; w++) {
     if (!src[w]) continue;
     for (size_t q = 0; q < n; q++) {
       if (src[w][q] != 1 && src[w][q] != 2)
         return set_err(LH_EINVAL, "lh_set_texture: mask codes are 1 (no-slip) or 2 (free-shear), got %d", src[w][q]);
       m[w * n + q] = src[w][q];
+      if (src[w][q] != src[w][q - q % N1]) xinv = 0;
     }
     bits |= 1 << w;
   }
@@ -1235,8 +1272,8 @@ int lh_set_texture(lh_ctx *x, int ch, const int *bottom, const int *top) {
   View d(x->arena, h.dev);
   HIPCHK(hipMemcpy(d.mask, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice));
   h.dev.tex = bits;
-  x->ntex = 0;
-  for (auto &q : x->ch) x->ntex += q.dev.tex != 0;
+  h.xinv = xinv;
+  tex_classify(x);
   rc = push_descriptors(x);
   if (rc) return rc;
   drop_graph(x);

@@ -75,6 +75,9 @@ struct Chan {
   // (N-grid, (N1+2) x N2 doubles each); iscal[0..1] truncation depths of the
   // two inverse rows
   int tex;
+  // 1 when every textured wall's mask is independent of x (streamwise
+  // ridges): the wall iteration then decouples per x-mode (k_tex_1d)
+  int tex1d;
   size_t o_mask, o_wtab, o_texu, o_texw, o_iscal;
   size_t o_twzN, o_twxN;  // N2- and N1-point twiddles of the N-grid
   size_t o_cfl;           // [slot][tile][2] CFL maxima of k_xprod's workgroups

@@ -51,6 +51,8 @@ struct TexGroup {
   size_t o_texw, o_twz, o_twx;  // arena offsets: planes, N2- and N1-point twiddles
 };
 hipError_t launch_tex_tables(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
+// adjacent-row values U of the smooth solution (before every source pass)
+hipError_t launch_tex_adj(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
 hipError_t launch_tex_sources(char *arena, const Chan *d_chans, const ShapeMax &m,
                               const std::vector<TexGroup> &groups, int kRK, int nit, hipStream_t s);
 hipError_t launch_tex_ghosts(char *arena, const Chan *d_chans, const ShapeMax &m,
@@ -60,6 +62,10 @@ size_t tex_fused_bytes(const std::vector<TexGroup> &groups);
 hipError_t launch_tex_sources_fused(char *arena, const Chan *d_chans, const ShapeMax &m, size_t lds, int kRK,
                                     int nit, hipStream_t s);
 hipError_t launch_tex_ghosts_fused(char *arena, const Chan *d_chans, const ShapeMax &m, size_t lds, hipStream_t s);
+// walls with x-independent masks (Chan::tex1d): the wall iteration per
+// x-mode; mode 0 sources (nit passes, after k_tex_adj), mode 1 ghost rows
+hipError_t launch_tex_1d(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, int nit, int mode,
+                         hipStream_t s);
 // one statistics sample of every channel: row profiles and spectra added to
 // the accumulators at sbase + Chan::o_stat / o_spec
 hipError_t launch_stats(char *arena, const Chan *d_chans, const ShapeMax &m, char *sbase, hipStream_t s);

@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) k_tex_adj(char *arena, const Chan *chans,
 // sources, unused when it == 0)
 __global__ void __launch_bounds__(256) k_tex_pre(char *arena, const Chan *chans, int kRK, int it) {
   const Chan ch = chans[blockIdx.z];
-  if (!ch.tex) return;
+  if (!ch.tex || ch.tex1d) return;
   const View v(arena, ch);
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ch.ncol) return;
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_tex_pre(char *arena, const Chan *chans,
 // mode 1: v <- (1 - gwbc) v on free-shear points, 0 elsewhere (ghost rows)
 __global__ void __launch_bounds__(256) k_tex_mask(char *arena, const Chan *chans, int kRK, int mode) {
   const Chan ch = chans[blockIdx.z];
-  if (!ch.tex) return;
+  if (!ch.tex || ch.tex1d) return;
   const View v(arena, ch);
   const int N1 = ch.N1, N2 = ch.N2;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(256) k_tex_mask(char *arena, const Chan *chans
 // only textured walls
 __global__ void __launch_bounds__(256) k_tex_put(char *arena, const Chan *chans, int mode) {
   const Chan ch = chans[blockIdx.z];
-  if (!ch.tex) return;
+  if (!ch.tex || ch.tex1d) return;
   const View v(arena, ch);
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ch.ncol) return;
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256) k_tex_put(char *arena, const Chan *chans,
 // texw <- rows 1 and ny-1 of u1, u3 (after the projection)
 __global__ void __launch_bounds__(256) k_tex_get(char *arena, const Chan *chans) {
   const Chan ch = chans[blockIdx.z];
-  if (!ch.tex) return;
+  if (!ch.tex || ch.tex1d) return;
   const View v(arena, ch);
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ch.ncol) return;
@@ -208,6 +208,11 @@ __global__ void __launch_bounds__(256) k_tex_get(char *arena, const Chan *chans)
     v.texw[(size_t)(f * 2) * nc + c] = src[(size_t)1 * ldc + c];
     v.texw[(size_t)(f * 2 + 1) * nc + c] = src[(size_t)(n - 2) * ldc + c];
   }
+}
+
+hipError_t launch_tex_adj(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s) {
+  hipLaunchKernelGGL(k_tex_adj, dim3((m.ncol + 255) / 256, 2, m.nchan), dim3(256), 0, s, arena, d_chans, kRK);
+  return hipGetLastError();
 }
 
 hipError_t launch_tex_tables(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s) {
@@ -231,7 +236,6 @@ static hipError_t tex_transforms(char *arena, const std::vector<TexGroup> &group
 hipError_t launch_tex_sources(char *arena, const Chan *d_chans, const ShapeMax &m,
                               const std::vector<TexGroup> &groups, int kRK, int nit, hipStream_t s) {
   const dim3 b(256), gc((m.ncol + 255) / 256, 1, m.nchan);
-  hipLaunchKernelGGL(k_tex_adj, dim3(gc.x, 2, m.nchan), b, 0, s, arena, d_chans, kRK);
   for (int it = 0; it < nit; it++) {
     hipLaunchKernelGGL(k_tex_pre, gc, b, 0, s, arena, d_chans, kRK, it);
     hipError_t e = tex_transforms(arena, groups, +1, s);
@@ -276,7 +280,7 @@ __host__ __device__ inline size_t tex_fused_lds(int N1, int N2) {
 __global__ void __launch_bounds__(1024) k_tex_fused(char *arena, const Chan *chans, int kRK, int nit, int mode) {
   const Chan ch = chans[blockIdx.z];
   const int f = blockIdx.x >> 1, w = blockIdx.x & 1;  // field (u1, u3), wall (bottom, top)
-  if (!(ch.tex & (1 << w))) return;
+  if (!(ch.tex & (1 << w)) || ch.tex1d) return;
   const View v(arena, ch);
   const int N1 = ch.N1, N2 = ch.N2, nxh = ch.nxh, nc = ch.ncol, n = ch.nyu, ldc = ch.ldc;
   const int h = N1 / 2;
@@ -380,6 +384,94 @@ __global__ void __launch_bounds__(1024) k_tex_fused(char *arena, const Chan *cha
   }
 }
 
+// ------------------------------------------------------------------------
+// Walls whose mask does not depend on x (streamwise ridges).  The mask
+// product then commutes with the x transforms: for x-mode i the N-grid pair
+// T[m . T^-1 v] reduces to Tz[m(z) . Tz^-1 v_i], except that rft keeps only
+// the real part of the z-transformed lines of i = 0 and i = N1/2, so those
+// two take the real part before the product.  The wall iteration (the same
+// passes as k_tex_fused) therefore decouples per x-mode: one wave per
+// (x-mode, field, wall, channel), an N2-point transform pair per pass.
+__global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, int kRK, int nit, int mode) {
+  const Chan ch = chans[blockIdx.z];
+  const int i = blockIdx.x, f = blockIdx.y >> 1, w = blockIdx.y & 1;
+  if (!ch.tex1d || !(ch.tex & (1 << w)) || i >= ch.nxh) return;
+  const View v(arena, ch);
+  const int N1 = ch.N1, N2 = ch.N2, nxh = ch.nxh, nc = ch.ncol, n = ch.nyu, ldc = ch.ldc;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  extern __shared__ double2 smem_d[];
+  double2 *A = smem_d, *B = A + N2, *TZ = B + N2;
+  double *M = reinterpret_cast<double *>(TZ + N2);
+  stage_twiddles<double2>(TZ, reinterpret_cast<const double2 *>(arena + ch.o_twzN), N2);
+  const size_t tab = (size_t)n * ldc;
+  const double *Wr = v.wtab + (size_t)((kRK - 1) * 2 + w) * tab;
+  const int gj = w == 0 ? 0 : n - 1;
+  const double g0 = Wr[(size_t)gj * ldc];
+  const double gw = w == 0 ? ch.gw1 : ch.gw2;
+  const double Pw = 1.0 + gw, cfs = mode == 0 ? Pw / (1.0 - g0 * Pw) : 1.0 - (f == 0 ? ch.gwbc1 : ch.gwbc3);
+  {
+    const int *mk = v.mask + (size_t)w * N1 * N2;
+    for (int z = tid; z < N2; z += nt) M[z] = mk[(size_t)z * N1] == 2 ? cfs : 0.0;
+  }
+  const bool realpart = i == 0 || i == N1 / 2;
+  const double2 *src = f == 0 ? v.u1 : v.u3;
+  const double invz = 1.0 / N2;
+  double2 *S = A;
+  const int passes = mode == 0 ? nit : 1;
+  for (int it = 0; it < passes; it++) {
+    for (int k = tid; k < N2; k += nt) {
+      const int c = i + nxh * k;
+      double2 val;
+      if (mode == 0) {
+        const double2 U = v.texu[(size_t)(f * 2 + w) * nc + c];
+        const double dg = Wr[(size_t)gj * ldc + c] - g0;
+        const double2 sg = it > 0 ? S[k] : make_double2(0, 0);
+        val = make_double2(U.x + dg * sg.x, U.y + dg * sg.y);
+      } else {
+        val = src[(size_t)(w == 0 ? 1 : n - 2) * ldc + c];
+      }
+      S[k] = val;
+    }
+    __syncthreads();
+    double2 *R = fft_lds<double2>(S, S == A ? B : A, N2, 1, N2, +1, TZ);
+    for (int z = tid; z < N2; z += nt) {
+      const double2 h = R[z];
+      R[z] = make_double2(M[z] * h.x, realpart ? 0.0 : M[z] * h.y);
+    }
+    __syncthreads();
+    double2 *G = fft_lds<double2>(R, R == A ? B : A, N2, 1, N2, -1, TZ);
+    for (int k = tid; k < N2; k += nt) G[k] = make_double2(G[k].x * invz, G[k].y * invz);
+    __syncthreads();
+    S = G;
+  }
+  double2 *dst = (mode == 0 ? (f == 0 ? v.r1 : v.r3) : (f == 0 ? v.u1 : v.u3)) + (w == 0 ? 0 : (size_t)(n - 1) * ldc);
+  for (int k = tid; k < N2; k += nt) {
+    const int c = i + nxh * k;
+    dst[c] = mode == 0 ? S[k] : cadd(dst[c], S[k]);
+  }
+}
+
+static size_t tex_1d_lds(int n2max) { return (size_t)3 * n2max * sizeof(double2) + (size_t)n2max * sizeof(double); }
+
+static int shape_n2max(const ShapeMax &m) {
+  int n2 = 0;
+  for (const ChanDims &d : m.dims) n2 = std::max(n2, d.N2);
+  return n2;
+}
+static int shape_nxhmax(const ShapeMax &m) {
+  int h = 0;
+  for (const ChanDims &d : m.dims) h = std::max(h, d.N1 / 2 + 1);
+  return h;
+}
+
+hipError_t launch_tex_1d(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, int nit, int mode,
+                         hipStream_t s) {
+  const int n2 = shape_n2max(m);
+  hipLaunchKernelGGL(k_tex_1d, dim3(shape_nxhmax(m), 4, m.nchan), dim3(64), tex_1d_lds(n2), s, arena, d_chans, kRK,
+                     nit, mode);
+  return hipGetLastError();
+}
+
 size_t tex_fused_bytes(const std::vector<TexGroup> &groups) {
   size_t b = 0;
   for (const TexGroup &g : groups) b = std::max(b, tex_fused_lds(g.N1, g.N2));
@@ -388,8 +480,6 @@ size_t tex_fused_bytes(const std::vector<TexGroup> &groups) {
 
 hipError_t launch_tex_sources_fused(char *arena, const Chan *d_chans, const ShapeMax &m, size_t lds, int kRK,
                                     int nit, hipStream_t s) {
-  const dim3 gc((m.ncol + 255) / 256, 2, m.nchan);
-  hipLaunchKernelGGL(k_tex_adj, gc, dim3(256), 0, s, arena, d_chans, kRK);
   hipLaunchKernelGGL(k_tex_fused, dim3(4, 1, m.nchan), dim3(1024), lds, s, arena, d_chans, kRK, nit, 0);
   return hipGetLastError();
 }

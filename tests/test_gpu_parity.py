@@ -230,6 +230,23 @@ def test_textured_step(name):
         assert e <= 1e-11, (nm, e)
 
 
+@pytest.mark.parametrize("sep", [1, 0])
+def test_ridges_wall_paths(sep):
+    """x-independent masks: the per-x-mode wall iteration (sep = 1) and the
+    two-dimensional wall transforms (sep = 0) both match the oracle."""
+    cfg = small_cfg(32, 33)
+    m = textures.stripes(32, 32, 4, 0.5)
+    s, o = tex_pair(cfg, (m, m), seed=17)
+    s.set_tuning(_lib.TUNE_TEX_SEPARABLE, sep)
+    assert s.get_tuning(_lib.TUNE_TEX_SEPARABLE) == sep
+    s.step(2)
+    o.step(2)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("ridges sep", sep, nm, e)
+        assert e <= 1e-11, (nm, e)
+
+
 def test_texture_all_no_slip_is_smooth():
     cfg = CASES["N16_ny33_poly_ctP"]
     s, o, f = make_pair(cfg, seed=12)
