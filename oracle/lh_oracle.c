@@ -201,6 +201,14 @@ struct lho_state {
   /* wall textures: tex[w][z*N1 + x] = 1 no-slip, 2 free-shear (planeBC
    * codes, start.f90:1776-1829), w = 0 bottom, 1 top; NULL = smooth wall */
   int *tex[2];
+  /* triad lists of the restricted nonlinear path (nonlinRead,
+   * start.f90:2983-3069): tri_len[q*9 + t-1] rows of type t in file q,
+   * tri_rows[q*9 + t-1] their int8 (len, 4) Fortran-order data; tri_u/tri_v
+   * the file of each u plane j = 1..ny-1 / v plane j = 1..ny-2 (-1: none) */
+  int ntri;
+  int *tri_len;
+  signed char **tri_rows;
+  int *tri_u, *tri_v;
 };
 
 static double ipow(double x, int n) {
@@ -391,6 +399,7 @@ void lho_destroy(lho_state *s) {
                   s->du1, s->du2, s->du3, s->psi, s->div, s->u1i, s->u2i, s->u3i,
                   s->uvf, s->wvf, s->vvc, s->dg, s->tex[0], s->tex[1]};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+  lho_set_triads(s, 0, NULL, NULL, NULL, NULL);
   free(s);
 }
 
@@ -634,6 +643,162 @@ static void interp_all(lho_state *s) {
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Restricted nonlinear path: ops_in_planes (FOU3D.f90:1724-1987) with
+ * nonlinInter (FOU3D.f90:1989-2030) and the index maps iLkup/kLkup/iNeg
+ * (start.f90:1139-1163), on (Ngal1+2) x Ngal2 spectral planes. */
+int lho_set_triads(lho_state *s, int nfile, const int *lens, const signed char *rows, const int *map_u,
+                   const int *map_v) {
+  if (s->tri_rows) {
+    for (int q = 0; q < 9 * s->ntri; q++) free(s->tri_rows[q]);
+  }
+  free(s->tri_rows);
+  free(s->tri_len);
+  free(s->tri_u);
+  free(s->tri_v);
+  s->tri_rows = NULL;
+  s->tri_len = s->tri_u = s->tri_v = NULL;
+  s->ntri = 0;
+  if (nfile <= 0) return 0;
+  int nx = s->c.N1 / 2 - 1, nz = s->c.N2 / 2 - 1;
+  s->ntri = nfile;
+  s->tri_len = malloc(sizeof(int) * 9 * nfile);
+  s->tri_rows = calloc((size_t)9 * nfile, sizeof(signed char *));
+  size_t off = 0;
+  for (int q = 0; q < 9 * nfile; q++) {
+    int n = lens[q];
+    s->tri_len[q] = n;
+    s->tri_rows[q] = malloc((size_t)4 * (n > 0 ? n : 1));
+    memcpy(s->tri_rows[q], rows + off, (size_t)4 * n);
+    off += (size_t)4 * n;
+    const signed char *d = s->tri_rows[q];
+    for (int l = 0; l < n; l++) {
+      int la = d[l], ka = d[n + l], lp = d[2 * n + l], kp = d[3 * n + l];
+      if (abs(la) > nx || abs(lp) > nx || abs(ka) > nz || abs(kp) > nz || la + lp < 0 || la + lp > nx ||
+          abs(ka + kp) > nz) {
+        lho_set_triads(s, 0, NULL, NULL, NULL, NULL);
+        return -1;
+      }
+    }
+  }
+  s->tri_u = malloc(sizeof(int) * (s->nyu - 2));
+  s->tri_v = malloc(sizeof(int) * (s->nyv - 2));
+  memcpy(s->tri_u, map_u, sizeof(int) * (s->nyu - 2));
+  memcpy(s->tri_v, map_v, sizeof(int) * (s->nyv - 2));
+  return 0;
+}
+
+static int ilk(int l) { return 2 * abs(l); } /* iLkup - 1 */
+static int ineg(int l) { return l < 0 ? -1 : 1; }
+/* kLkup - 1: k in 0..N2/2-1 -> k, -N2/2..-1 -> k + Ngal2, N2/2 -> Ngal2 - N2/2 */
+static int klk(const lho_state *s, int k) {
+  int h = s->c.N2 / 2;
+  if (k >= 0 && k < h) return k;
+  if (k == h) return s->c.Ngal2 - h;
+  return k + s->c.Ngal2;
+}
+
+/* nonlinInter(jlist, x_cPL, uaPL, upPL), FOU3D.f90:1989-2030 */
+static void nonlin_inter(const lho_state *s, int q, int type, double *X, const double *A, const double *P) {
+  if (q < 0) return;
+  int n = s->tri_len[q * 9 + type - 1], ig = s->igal;
+  const signed char *d = s->tri_rows[q * 9 + type - 1];
+  for (int l = 0; l < n; l++) {
+    int la = d[l], ka0 = d[n + l], lp = d[2 * n + l], kp0 = d[3 * n + l];
+    int ia = ilk(la), ip = ilk(lp), it = ilk(la + lp);
+    for (int p = 0; p <= 1; p++) {
+      if (ka0 + kp0 == 0 && p == 1) continue;
+      int pn = p * 2 - 1;
+      int ka = klk(s, pn * ineg(la) * ka0), kp = klk(s, pn * ineg(lp) * kp0), kt = klk(s, pn * (ka0 + kp0));
+      const double *a = A + (size_t)ka * ig, *b = P + (size_t)kp * ig;
+      double *x = X + (size_t)kt * ig;
+      x[it] = x[it] + a[ia] * b[ip] - ineg(la) * ineg(lp) * a[ia + 1] * b[ip + 1];
+      x[it + 1] = x[it + 1] + ineg(lp) * a[ia] * b[ip + 1] + ineg(la) * a[ia + 1] * b[ip];
+    }
+  }
+}
+
+/* the "0th mode" sums of ops_in_planes (FOU3D.f90:1763-1778, 1892-1908):
+ * x(0,0) += sum over i, k of a(i,k) b(-i,-k), k from -kneg to N2/2-1 */
+static void zeroth_mode(const lho_state *s, double *X, const double *A, const double *B, int kneg) {
+  int h1 = s->c.N1 / 2, h2 = s->c.N2 / 2, ig = s->igal;
+  for (int i = -h1; i <= h1; i++)
+    for (int k = -kneg; k <= h2 - 1; k++) {
+      int la = i, lp = -i;
+      int ia = ilk(la), ip = ilk(lp);
+      int ka = klk(s, ineg(la) * k), kp = klk(s, ineg(lp) * (-k));
+      const double *a = A + (size_t)ka * ig, *b = B + (size_t)kp * ig;
+      X[0] = X[0] + a[ia] * b[ip] - ineg(i) * ineg(-i) * a[ia + 1] * b[ip + 1];
+      X[1] = X[1] + a[ia] * b[ip + 1] * ineg(-i) + ineg(i) * a[ia + 1] * b[ip];
+    }
+}
+
+/* u plane j of ops_in_planes (FOU3D.f90:1751-1881): Nu1, Nu3 rows and vv */
+static void ops_list_u(lho_state *s, int j, double *P1, double *P2, double *P3, double *W[6]) {
+  size_t psz = (size_t)s->igal * s->kgal;
+  int nyu = s->nyu;
+  double *uu = W[0], *uw = W[1], *vv = W[2], *wu = W[3], *ww = W[4], *R = W[5];
+  memset(uu, 0, psz * 8); memset(uw, 0, psz * 8); memset(vv, 0, psz * 8);
+  memset(wu, 0, psz * 8); memset(ww, 0, psz * 8);
+  modes_to_plane(s, s->u1, nyu, j, P1);
+  modes_to_plane(s, s->u2i, nyu, j, P2);
+  modes_to_plane(s, s->u3, nyu, j, P3);
+  zeroth_mode(s, vv, P2, P2, s->c.N2 / 2);
+  /* linear advection by the mean mode (FOU3D.f90:1784-1798) */
+  double a1 = P1[0], a2 = P2[0], a3 = P3[0];
+  for (size_t o = 0; o < psz; o++) {
+    uu[o] = uu[o] + 2 * (a1 * P1[o]);
+    ww[o] = ww[o] + 2 * (a3 * P3[o]);
+    double b = o < 2 ? 0.0 : 2 * (a2 * P2[o]);
+    vv[o] = vv[o] + b;
+    double c = a1 * P3[o] + a3 * P1[o];
+    wu[o] = wu[o] + c;
+    uw[o] = uw[o] + c;
+  }
+  int q = s->tri_u[j - 1];
+  nonlin_inter(s, q, 1, uu, P1, P1);
+  nonlin_inter(s, q, 3, uw, P3, P1);
+  nonlin_inter(s, q, 5, vv, P2, P2);
+  nonlin_inter(s, q, 7, wu, P1, P3);
+  nonlin_inter(s, q, 9, ww, P3, P3);
+  derxz_sum(s, uu, uw, R);
+  plane_to_modes(s, R, s->Nu1, nyu, j);
+  derxz_sum(s, wu, ww, R);
+  plane_to_modes(s, R, s->Nu3, nyu, j);
+  plane_to_modes(s, vv, s->vvc, nyu, j);
+}
+
+/* v plane j of ops_in_planes (FOU3D.f90:1889-1984): Nu2 rows, uv and wv */
+static void ops_list_v(lho_state *s, int j, double *P1, double *P2, double *P3, double *W[6]) {
+  size_t psz = (size_t)s->igal * s->kgal;
+  int nyv = s->nyv;
+  double *uv = W[0], *vu = W[1], *vw = W[2], *wv = W[3], *R = W[5];
+  memset(uv, 0, psz * 8); memset(vu, 0, psz * 8); memset(vw, 0, psz * 8); memset(wv, 0, psz * 8);
+  modes_to_plane(s, s->u1i, nyv, j, P1);
+  modes_to_plane(s, s->u2, nyv, j, P2);
+  modes_to_plane(s, s->u3i, nyv, j, P3);
+  zeroth_mode(s, uv, P1, P2, s->c.N2 / 2 - 1);
+  zeroth_mode(s, wv, P3, P2, s->c.N2 / 2 - 1);
+  double a1 = P1[0], a2 = P2[0], a3 = P3[0];
+  for (size_t o = 0; o < psz; o++) {
+    double b = a1 * P2[o] + a2 * P1[o];
+    vu[o] = vu[o] + b;
+    uv[o] = uv[o] + (o < 2 ? 0.0 : b);
+    double c = a2 * P3[o] + a3 * P2[o];
+    vw[o] = vw[o] + c;
+    wv[o] = wv[o] + (o < 2 ? 0.0 : c);
+  }
+  int q = s->tri_v[j - 1];
+  nonlin_inter(s, q, 2, uv, P2, P1);
+  nonlin_inter(s, q, 4, vu, P1, P2);
+  nonlin_inter(s, q, 6, vw, P3, P2);
+  nonlin_inter(s, q, 8, wv, P2, P3);
+  derxz_sum(s, vu, vw, R);
+  plane_to_modes(s, R, s->Nu2, nyv, j);
+  plane_to_modes(s, uv, s->uvf, nyv, j);
+  plane_to_modes(s, wv, s->wvf, nyv, j);
+}
+
 /* nonlinear (FOU3D.f90:41-410) with ops_in_planes2 (FOU3D.f90:2090-2251) */
 void lho_nonlinear(lho_state *s, int kRK) {
   int nyu = s->nyu, nyv = s->nyv, igal = s->igal, kgal = s->kgal;
@@ -647,8 +812,23 @@ void lho_nonlinear(lho_state *s, int kRK) {
     double *P1 = malloc(sizeof(double) * psz), *P2 = malloc(sizeof(double) * psz);
     double *P3 = malloc(sizeof(double) * psz), *Q = malloc(sizeof(double) * psz);
     double *R = malloc(sizeof(double) * psz);
+    double *W[6] = {NULL};
+    if (s->ntri)
+      for (int q = 0; q < 6; q++) W[q] = malloc(sizeof(double) * psz);
 #pragma omp for schedule(dynamic)
     for (int j = 1; j <= nyu - 2; j++) {
+      if (s->ntri) { /* restricted path; u1PL, u3PL go to physical space after */
+        ops_list_u(s, j, P1, P2, P3, W);
+        lho_four_to_phys_plane(s, P1);
+        lho_four_to_phys_plane(s, P3);
+        for (int k = 0; k < Ngal2; k++)
+          for (int i = 0; i < Ngal1; i++) {
+            size_t o = (size_t)k * igal + i;
+            if (fabs(P1[o]) > m1) m1 = fabs(P1[o]);
+            if (fabs(P3[o]) > m3) m3 = fabs(P3[o]);
+          }
+        continue;
+      }
       modes_to_plane(s, s->u1, nyu, j, P1);
       modes_to_plane(s, s->u2i, nyu, j, P2);
       modes_to_plane(s, s->u3, nyu, j, P3);
@@ -677,14 +857,28 @@ void lho_nonlinear(lho_state *s, int kRK) {
       plane_to_modes(s, P2, s->vvc, nyu, j);
     }
     free(P1); free(P2); free(P3); free(Q); free(R);
+    for (int q = 0; q < 6; q++) free(W[q]);
   }
   /* v-planes: uv, wv -> Nu2 (PL part), uv, wv */
 #pragma omp parallel reduction(max : m2)
   {
     double *P1 = malloc(sizeof(double) * psz), *P2 = malloc(sizeof(double) * psz);
     double *P3 = malloc(sizeof(double) * psz), *R = malloc(sizeof(double) * psz);
+    double *W[6] = {NULL};
+    if (s->ntri)
+      for (int q = 0; q < 6; q++) W[q] = malloc(sizeof(double) * psz);
 #pragma omp for schedule(dynamic)
     for (int j = 1; j <= nyv - 2; j++) {
+      if (s->ntri) {
+        ops_list_v(s, j, P1, P2, P3, W);
+        lho_four_to_phys_plane(s, P2);
+        for (int k = 0; k < Ngal2; k++)
+          for (int i = 0; i < Ngal1; i++) {
+            double av = fabs(P2[(size_t)k * igal + i]) * s->dthdyv[j];
+            if (av > m2) m2 = av;
+          }
+        continue;
+      }
       modes_to_plane(s, s->u1i, nyv, j, P1);
       modes_to_plane(s, s->u2, nyv, j, P2);
       modes_to_plane(s, s->u3i, nyv, j, P3);
@@ -708,6 +902,7 @@ void lho_nonlinear(lho_state *s, int kRK) {
       plane_to_modes(s, P3, s->wvf, nyv, j);
     }
     free(P1); free(P2); free(P3); free(R);
+    for (int q = 0; q < 6; q++) free(W[q]);
   }
   /* wall faces / ghost centres of the flux planes are zero */
   for (int c = 0; c < s->ncol; c++) {

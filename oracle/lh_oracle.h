@@ -110,6 +110,17 @@ int lho_set_texture(lho_state *s, const int *bottom, const int *top);
 /* the textured U/W Helmholtz solve on arbitrary right-hand sides */
 void lho_lusol_uw_tex(lho_state *s, int kRK, const double *rhs_u, const double *rhs_w, double *u, double *w);
 
+/* Triad lists of the restricted nonlinear path (ops_in_planes,
+ * FOU3D.f90:1724-1987; nonlinInter :1989-2030; nonlinRead, start.f90:
+ * 2983-3069).  nfile list files; lens[q*9 + t-1] the rows of type t in file
+ * q; rows the int8 lists concatenated (file, then type order), each in the
+ * reference's (len, 4) Fortran layout; map_u[j-1], j = 1..ny-1, and
+ * map_v[j-1], j = 1..ny-2, the file of each plane (-1: no triads).  Once
+ * set, lho_nonlinear takes the list path.  nfile = 0 returns to the plain
+ * path.  Returns -1 (and clears) on an entry outside the band. */
+int lho_set_triads(lho_state *s, int nfile, const int *lens, const signed char *rows, const int *map_u,
+                   const int *map_v);
+
 /* One statistics sample of the current fields (stats.f90:1-107 and
  * spectra.f90:1-56), added to prof[13][ny+1] (profile q of write_stats'
  * order Um U2m Vm V2m Wm W2m Pm P2m UVm UWm VWm wxm wx2m at q*(ny+1) + j,

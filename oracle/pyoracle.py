@@ -87,6 +87,9 @@ def lib():
         L.lho_set_texture.restype = ctypes.c_int
         L.lho_lusol_uw_tex.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
         L.lho_stats.argtypes = [vp, dp, dp]
+        L.lho_set_triads.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
+                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.lho_set_triads.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -272,3 +275,23 @@ class Oracle:
         assert sp.shape == (5, ncol, nyu) and sp.dtype == np.float64 and sp.flags.c_contiguous
         lib().lho_stats(self.h, _dp(prof), _dp(sp))
         return prof, sp
+
+    def set_triads(self, files, map_u, map_v):
+        """files: list of {type: int8 (len, 4)} (None: no triads); map_u /
+        map_v: file index of u planes 1..ny-1 and v planes 1..ny-2 (-1 or a
+        None file: no triads).  Switches lho_nonlinear to the list path."""
+        lens, chunks = [], []
+        for f in files:
+            for t in range(1, 10):
+                a = np.zeros((0, 4), np.int8) if f is None else np.asarray(f[t], np.int8).reshape(-1, 4)
+                lens.append(a.shape[0])
+                chunks.append(np.ascontiguousarray(a.T).reshape(-1))
+        rows = np.concatenate(chunks) if chunks else np.zeros(0, np.int8)
+        mu = np.array([m if m is not None and m >= 0 and files[m] is not None else -1 for m in map_u], np.int32)
+        mv = np.array([m if m is not None and m >= 0 and files[m] is not None else -1 for m in map_v], np.int32)
+        ln = np.array(lens, np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        rc = lib().lho_set_triads(self.h, len(files), ln.ctypes.data_as(ip), rows.ctypes.data,
+                                  mu.ctypes.data_as(ip), mv.ctypes.data_as(ip))
+        if rc != 0:
+            raise ValueError("triad list entry outside the band")

@@ -206,25 +206,31 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
     }
   }
   __syncthreads();
-  C *Y = fft_lds<C>(X, X == A ? B : A, Ng1, 2 * nz, Ng1, -1, TW);
+  // one (max a, max b) pair per workgroup, reduced by k_dt_update: wave
+  // maxima here, combined after the forward transform's barriers
+  __shared__ double red[2][4];
   {
-    // one (max a, max b) pair per workgroup, reduced by k_dt_update
-    __shared__ double red[2][256];
-    red[0][threadIdx.x] = (double)ma;
-    red[1][threadIdx.x] = (double)mb;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) {
-        red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + w]);
-        red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + w]);
-      }
-      __syncthreads();
+    double a = (double)ma, b = (double)mb;
+    for (int off = 32; off >= 1; off >>= 1) {
+      a = fmax(a, __shfl_xor(a, off));
+      b = fmax(b, __shfl_xor(b, off));
     }
-    if (threadIdx.x == 0) {
-      double *cm = v.cfl + 2 * ((size_t)slot * gridDim.x + blockIdx.x);
-      cm[0] = red[0][0];
-      cm[1] = red[1][0];
+    if ((threadIdx.x & 63) == 0) {
+      red[0][threadIdx.x >> 6] = a;
+      red[1][threadIdx.x >> 6] = b;
     }
+  }
+  C *Y = fft_lds<C>(X, X == A ? B : A, Ng1, 2 * nz, Ng1, -1, TW);
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    double a = red[0][0], b = red[1][0];
+    for (int q = 1; q < nw; q++) {
+      a = fmax(a, red[0][q]);
+      b = fmax(b, red[1][q]);
+    }
+    double *cm = v.cfl + 2 * ((size_t)slot * gridDim.x + blockIdx.x);
+    cm[0] = a;
+    cm[1] = b;
   }
   const int nseq = isU ? 2 : 1;
   const R_t inv = (R_t)(1.0 / Ng1);
@@ -1189,37 +1195,50 @@ __global__ void k_from_rows(const double2 *__restrict__ src, double2 *__restrict
 // maxima of this substep's physical planes (dtc_calc, FOU3D.f90:699-736,
 // with u2mloc starting at 1e-6), then dt = min(2.5 dtv, CFL dtc) when the
 // channel's step is adaptive, and t += dt.  One workgroup per channel.
-__global__ void __launch_bounds__(256) k_dt_update(char *arena, const Chan *chans, int ntiles_max, int e_xprod) {
+__global__ void __launch_bounds__(1024) k_dt_update(char *arena, const Chan *chans, int ntiles_max, int e_xprod) {
   const Chan ch = chans[blockIdx.x];
   View v(arena, ch);
   double m1 = 0.0, m2 = 1e-6, m3 = 0.0;
   // k_xprod's tiles of this channel (stride: the launch's tile count)
   const int TZ = tile_of(e_xprod, ch.Ng1, ch.Ng2), nt = (ch.Ng2 + TZ - 1) / TZ;
-  for (int q = threadIdx.x; q < ch.nslot * nt; q += blockDim.x) {
+  const double2 *cm = reinterpret_cast<const double2 *>(v.cfl);
+  const int nu = ch.nplu * nt, nall = ch.nslot * nt;
+#pragma unroll 4
+  for (int q = threadIdx.x; q < nu; q += blockDim.x) {
     const int slot = q / nt, tz = q - slot * nt;
-    const double *cm = v.cfl + 2 * ((size_t)slot * ntiles_max + tz);
-    if (slot < ch.nplu) {
-      m1 = fmax(m1, cm[0]);
-      m3 = fmax(m3, cm[1]);
-    } else {
-      const int j = slot - ch.nplu + 1;
-      m2 = fmax(m2, cm[0] * v.dthdyv[j]);
-    }
+    const double2 c = cm[(size_t)slot * ntiles_max + tz];
+    m1 = fmax(m1, c.x);
+    m3 = fmax(m3, c.y);
   }
-  __shared__ double red[3][256];
-  red[0][threadIdx.x] = m1;
-  red[1][threadIdx.x] = m2;
-  red[2][threadIdx.x] = m3;
+#pragma unroll 4
+  for (int q = nu + threadIdx.x; q < nall; q += blockDim.x) {
+    const int slot = q / nt, tz = q - slot * nt;
+    const int j = slot - ch.nplu + 1;
+    m2 = fmax(m2, cm[(size_t)slot * ntiles_max + tz].x * v.dthdyv[j]);
+  }
+  __shared__ double red[3][16];
+  for (int off = 32; off >= 1; off >>= 1) {
+    m1 = fmax(m1, __shfl_xor(m1, off));
+    m2 = fmax(m2, __shfl_xor(m2, off));
+    m3 = fmax(m3, __shfl_xor(m3, off));
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][wv] = m1;
+    red[1][wv] = m2;
+    red[2][wv] = m3;
+  }
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int r = 0; r < 3; r++) red[r][threadIdx.x] = fmax(red[r][threadIdx.x], red[r][threadIdx.x + w]);
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    m1 = red[0][0];
-    m2 = red[1][0];
-    m3 = red[2][0];
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int q = 1; q < nw; q++) {
+      m1 = fmax(m1, red[0][q]);
+      m2 = fmax(m2, red[1][q]);
+      m3 = fmax(m3, red[2][q]);
+    }
+    m1 = fmax(m1, red[0][0]);
+    m2 = fmax(m2, red[1][0]);
+    m3 = fmax(m3, red[2][0]);
     const double dtc = fmin(fmin(1.0 / (ch.alp * (ch.N1 / 2) * m1), 1.0 / (ch.bet * (ch.N2 / 2) * m3)), 1.0 / (m2 * 1.0));
     v.scal[SC_DTC] = dtc;
     if (ch.adaptive) v.scal[SC_DT] = fmin(2.5 * v.scal[SC_DTV], ch.CFL * dtc);
@@ -1270,7 +1289,7 @@ __global__ void k_u11(char *arena, const Chan *chans) {
 }
 
 hipError_t launch_dt_update(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s) {
-  hipLaunchKernelGGL(k_dt_update, dim3(m.nchan), dim3(256), 0, s, arena, d_chans, m.xprod_tiles, m.e_xprod);
+  hipLaunchKernelGGL(k_dt_update, dim3(m.nchan), dim3(1024), 0, s, arena, d_chans, m.xprod_tiles, m.e_xprod);
   return hipGetLastError();
 }
 
