@@ -222,6 +222,22 @@ int lh_autotune(lh_ctx *ctx, int reps);
  * and the ghost rows of v_corr follow the mask point by point. */
 int lh_set_texture(lh_ctx *ctx, int ch, const int *bottom, const int *top);
 
+/* The restricted nonlinear path of the reference: products summed over
+ * triad interaction lists in Fourier space (ops_in_planes, FOU3D.f90:
+ * 1724-1987; nonlinInter :1989-2030) instead of the FFT products, for channel
+ * `ch`.  Arguments as nonlinRead (start.f90:2983-3069) loads them: nfile
+ * list files; lens[q*9 + t-1] the rows of list type t (1..9: uu uv uw vu vv
+ * vw wu wv ww) in file q; rows the int8 lists concatenated in file, then
+ * type order, each in the reference's list(len, 4) Fortran layout (all la,
+ * ka, lp, kp); map_u[j-1] (u planes j = 1..ny-1) and map_v[j-1] (v planes
+ * j = 1..ny-2) the file of each plane, -1 for none (the mean-mode terms
+ * only).  Rows must satisfy |la|, |lp| <= N1/2-1, |ka|, |kp| <= N2/2-1,
+ * 0 <= la+lp <= N1/2-1, |ka+kp| <= N2/2-1.  The channel must have a fixed
+ * time step (the list path forms no physical planes for dtc_calc).
+ * nfile = 0 returns the channel to the FFT path. */
+int lh_set_triads(lh_ctx *ctx, int ch, int nfile, const int *lens, const signed char *rows,
+                  const int *map_u, const int *map_v);
+
 /* Running statistics and spectra, replacing stats (stats.f90:1-107) with
  * its gathers recvX (stats.f90:443-470) and spectra / buildsp / buildspUV
  * (spectra.f90:1-56).  lh_stats_accumulate adds one sample of the current

@@ -225,6 +225,8 @@ struct lh_ctx {
   int ntex1d = 0;                    // channels on that path (the rest: ntex - ntex1d)
   bool adaptive = false;             // a channel with an adaptive time step
   char *stats = nullptr;             // statistics accumulators (allocated on first use)
+  std::vector<char *> tri_mem;       // per channel: triad-list tables and products (or null)
+  int ntri = 0;                      // channels on the triad-list path
   size_t stats_bytes = 0;
   int istat = 0;                     // samples since the last reset
 };
@@ -461,6 +463,14 @@ int solve_phase(lh_ctx *x, int kRK) {
 // first substep: dtc and (adaptive channels) the new dt, then everything
 // that depends on dt -- pivots, mean-mode unit-forcing solutions, texture
 // inverse rows -- for the whole step
+// the nonlinear products of every channel: the FFT stages for channels
+// without triad lists, the list kernels for those with
+int nonlinear_stage(lh_ctx *x) {
+  if (x->ntri < (int)x->ch.size()) HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream));
+  if (x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
+  return LH_OK;
+}
+
 int time_step(lh_ctx *x, int kRK) {
   if (kRK != 1) return LH_OK;
   HIPCHK(launch_dt_update(x->arena, x->d_chans, x->shape, x->stream));
@@ -473,8 +483,9 @@ int time_step(lh_ctx *x, int kRK) {
 }
 
 int run_substep(lh_ctx *x, int kRK) {
-  HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream));
-  int rc = time_step(x, kRK);
+  int rc = nonlinear_stage(x);
+  if (rc) return rc;
+  rc = time_step(x, kRK);
   if (rc) return rc;
   HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
   return solve_phase(x, kRK);
@@ -615,6 +626,8 @@ int lh_ctx_destroy(lh_ctx *x) {
   if (x->d_chans) (void)hipFree(x->d_chans);
   if (x->staging) (void)hipFree(x->staging);
   if (x->stats) (void)hipFree(x->stats);
+  for (char *p : x->tri_mem)
+    if (p) (void)hipFree(p);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return LH_OK;
@@ -729,8 +742,9 @@ int lh_run_phase(lh_ctx *x, int phase, int kRK) {
   if (kRK < 1 || kRK > 3) return set_err(LH_EINVAL, "kRK must be 1..3");
   switch (phase) {
     case LH_PHASE_NONLINEAR: {
-      HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream));
-      int r = time_step(x, kRK);
+      int r = nonlinear_stage(x);
+      if (r) return r;
+      r = time_step(x, kRK);
       if (r) return r;
       break;
     }
@@ -794,7 +808,10 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
   for (int n = 0; n < nsub && rc == LH_OK; n++) {
     HIPCHK(hipEventRecord(ev[0], x->stream));
     for (int st = 0; st < 3; st++) {
-      HIPCHK(launch_nl_stage(x->arena, x->d_chans, x->shape, x->mixed, st, x->stream));
+      if (x->ntri < (int)x->ch.size())
+        HIPCHK(launch_nl_stage(x->arena, x->d_chans, x->shape, x->mixed, st, x->stream));
+      // the triad-list kernels are counted with the x-product stage
+      if (st == 1 && x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
       HIPCHK(hipEventRecord(ev[st + 1], x->stream));
     }
     if ((rc = time_step(x, kRK))) break;
@@ -1244,6 +1261,111 @@ int lh_autotune(lh_ctx *x, int reps) {
   HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(hipFree(save));
   x->shape = best;
+  drop_graph(x);
+  return LH_OK;
+}
+
+int lh_set_triads(lh_ctx *x, int ch, int nfile, const int *lens, const signed char *rows, const int *map_u,
+                  const int *map_v) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size()) return set_err(LH_EINVAL, "lh_set_triads: bad channel %d", ch);
+  if (nfile < 0 || (nfile > 0 && (!lens || !rows || !map_u || !map_v)))
+    return set_err(LH_EINVAL, "lh_set_triads: bad arguments");
+  HostChan &h = x->ch[ch];
+  if (x->tri_mem.size() != x->ch.size()) x->tri_mem.assign(x->ch.size(), nullptr);
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (nfile == 0) {
+    if (x->tri_mem[ch]) HIPCHK(hipFree(x->tri_mem[ch]));
+    x->tri_mem[ch] = nullptr;
+    h.dev.tri = 0;
+    h.dev.tri_map = nullptr;
+    h.dev.tri_base = nullptr;
+    h.dev.tri_off = h.dev.tri_ent = nullptr;
+    h.dev.tri_prod = nullptr;
+  } else {
+    const lh_config &c = h.cfg;
+    if (c.adaptive_dt) return set_err(LH_EINVAL, "lh_set_triads: the list path runs with a fixed time step");
+    const int ncol = h.ncol, nxh = h.nxh, N2 = c.N2, nx = c.N1 / 2 - 1, nz = c.N2 / 2 - 1;
+    if (ncol > 32767) return set_err(LH_EINVAL, "lh_set_triads: %d spectral columns exceed the 15-bit index", ncol);
+    const int nslot = h.nslot, nplu = h.nyu - 2;
+    std::vector<int> map(nslot);
+    for (int s = 0; s < nslot; s++) {
+      const int q = s < nplu ? map_u[s] : map_v[s - nplu];
+      if (q >= nfile) return set_err(LH_EINVAL, "lh_set_triads: plane %d names file %d of %d", s, q, nfile);
+      map[s] = q < 0 ? -1 : q;
+    }
+    auto kk = [&](int k) { return k >= 0 ? k : k + N2; };
+    // (file, type) target-sorted entries: counting sort by target column
+    const size_t nlist = (size_t)nfile * 9;
+    std::vector<size_t> base(nlist);
+    std::vector<unsigned> off(nlist * (ncol + 1));
+    std::vector<unsigned> ent;
+    size_t roff = 0;
+    std::vector<unsigned> tgt, val;
+    for (size_t lt = 0; lt < nlist; lt++) {
+      const int n = lens[lt];
+      if (n < 0) return set_err(LH_EINVAL, "lh_set_triads: negative list length");
+      const signed char *d = rows + roff;
+      roff += (size_t)4 * n;
+      tgt.clear();
+      val.clear();
+      for (int l = 0; l < n; l++) {
+        const int la = d[l], ka = d[n + l], lp = d[2 * n + l], kp = d[3 * n + l];
+        if (abs(la) > nx || abs(lp) > nx || abs(ka) > nz || abs(kp) > nz || la + lp < 0 || la + lp > nx ||
+            abs(ka + kp) > nz)
+          return set_err(LH_EINVAL, "lh_set_triads: row (%d,%d,%d,%d) of list %zu outside the band", la, ka, lp, kp,
+                         lt);
+        // p = 0: the z mirror into -(ka+kp); p = 1 unless ka + kp = 0
+        for (int p = 0; p <= 1; p++) {
+          if (p == 1 && ka + kp == 0) continue;
+          const int sg = p == 0 ? -1 : 1;
+          const unsigned ca = la >= 0 ? (unsigned)(la + nxh * kk(sg * ka)) : (unsigned)(-la + nxh * kk(-sg * ka));
+          const unsigned cp = lp >= 0 ? (unsigned)(lp + nxh * kk(sg * kp)) : (unsigned)(-lp + nxh * kk(-sg * kp));
+          const unsigned w = ca | (la < 0 ? 0x8000u : 0u) | (cp << 16) | (lp < 0 ? 0x80000000u : 0u);
+          tgt.push_back((unsigned)((la + lp) + nxh * kk(sg * (ka + kp))));
+          val.push_back(w);
+        }
+      }
+      unsigned *o = off.data() + lt * (ncol + 1);
+      for (unsigned t : tgt) o[t + 1]++;
+      for (int c2 = 0; c2 < ncol; c2++) o[c2 + 1] += o[c2];
+      base[lt] = ent.size();
+      ent.resize(ent.size() + val.size());
+      std::vector<unsigned> fill(o, o + ncol);
+      for (size_t e = 0; e < val.size(); e++) ent[base[lt] + fill[tgt[e]]++] = val[e];
+    }
+    // one device buffer: map, base, off, entries, products
+    auto a8 = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t b_map = a8(nslot * sizeof(int)), b_base = a8(nlist * sizeof(size_t));
+    const size_t b_off = a8(off.size() * sizeof(unsigned)), b_ent = a8(std::max<size_t>(ent.size(), 1) * sizeof(unsigned));
+    const size_t b_prod = a8((size_t)nslot * 5 * ncol * sizeof(double2));
+    char *mem = nullptr;
+    hipError_t e = hipMalloc((void **)&mem, b_map + b_base + b_off + b_ent + b_prod);
+    if (e != hipSuccess) return set_err(LH_ENOMEM, "lh_set_triads: hipMalloc: %s", hipGetErrorString(e));
+    char *p = mem;
+    e = hipMemcpy(p, map.data(), nslot * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p + b_map, base.data(), nlist * sizeof(size_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p + b_map + b_base, off.data(), off.size() * sizeof(unsigned), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !ent.empty())
+      e = hipMemcpy(p + b_map + b_base + b_off, ent.data(), ent.size() * sizeof(unsigned), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(mem);
+      return set_err(LH_EHIP, "lh_set_triads: copy: %s", hipGetErrorString(e));
+    }
+    if (x->tri_mem[ch]) HIPCHK(hipFree(x->tri_mem[ch]));
+    x->tri_mem[ch] = mem;
+    h.dev.tri = 1;
+    h.dev.tri_map = (const int *)p;
+    h.dev.tri_base = (const size_t *)(p + b_map);
+    h.dev.tri_off = (const unsigned *)(p + b_map + b_base);
+    h.dev.tri_ent = (const unsigned *)(p + b_map + b_base + b_off);
+    h.dev.tri_prod = (double2 *)(p + b_map + b_base + b_off + b_ent);
+  }
+  x->ntri = 0;
+  for (auto &q : x->ch) x->ntri += q.dev.tri;
+  rc = push_descriptors(x);
+  if (rc) return rc;
   drop_graph(x);
   return LH_OK;
 }

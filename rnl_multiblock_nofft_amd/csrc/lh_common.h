@@ -85,6 +85,17 @@ struct Chan {
   // statistics buffer (allocated on first use): prof[ST_*][nyu] row profiles
   // and spec[SP_*][nyu][ldc] per-mode spectra (doubles)
   size_t o_stat, o_spec;
+  // triad-list nonlinear path (tri = 1): file of each plane slot (-1: none),
+  // per (file, type) the target-sorted rows (tri_off[(file*9 + type-1) *
+  // (ncol+1) + c] .. +1 into tri_ent + tri_base[file*9 + type-1]; an entry
+  // packs the passive column (bits 0-14), its conjugate flag (15), the active
+  // column (16-30) and its flag (31)), and the product fields
+  // tri_prod[slot][5][ncol]; separate device allocation
+  int tri;
+  const int *tri_map;
+  const size_t *tri_base;
+  const unsigned *tri_off, *tri_ent;
+  double2 *tri_prod;
   // Thomas factors of the fixed-dt operators, [row][ldc] doubles each:
   // o_fac + ((kRK-1)*4 + m) * nyu*ldc*8 for m = 0 dU, 1 cU, 2 dV, 3 cV
   // (U/W and V Helmholtz, LU_dec order), then o_fac + 12*nyu*ldc*8 dP and

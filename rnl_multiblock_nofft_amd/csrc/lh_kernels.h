@@ -66,6 +66,8 @@ hipError_t launch_tex_ghosts_fused(char *arena, const Chan *d_chans, const Shape
 // x-mode; mode 0 sources (nit passes, after k_tex_adj), mode 1 ghost rows
 hipError_t launch_tex_1d(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, int nit, int mode,
                          hipStream_t s);
+// triad-list products and their derivatives for channels with Chan::tri
+hipError_t launch_triads(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 // one statistics sample of every channel: row profiles and spectra added to
 // the accumulators at sbase + Chan::o_stat / o_spec
 hipError_t launch_stats(char *arena, const Chan *d_chans, const ShapeMax &m, char *sbase, hipStream_t s);

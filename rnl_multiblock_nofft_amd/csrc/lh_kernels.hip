@@ -62,6 +62,7 @@ template <typename C> __device__ __forceinline__ double2 to_d(C v) { return make
 template <typename C>
 __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E) {
   const Chan ch = chans[blockIdx.z / 3];
+  if (ch.tri) return;  // triad-list channel: k_triad computes its products
   const View v(arena, ch);
   const int f = blockIdx.z % 3;
   const int slot = blockIdx.y;
@@ -139,6 +140,7 @@ template <typename C>
 __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, int E) {
   typedef typename CT<C>::R R_t;
   const Chan ch = chans[blockIdx.z];
+  if (ch.tri) return;
   const View v(arena, ch);
   const int slot = blockIdx.y;
   if (slot >= ch.nslot) return;
@@ -261,6 +263,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
 template <typename C>
 __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, int E) {
   const Chan ch = chans[blockIdx.z];
+  if (ch.tri) return;
   const View v(arena, ch);
   const int slot = blockIdx.y;
   if (slot >= ch.nslot) return;
@@ -1198,6 +1201,10 @@ __global__ void k_from_rows(const double2 *__restrict__ src, double2 *__restrict
 __global__ void __launch_bounds__(1024) k_dt_update(char *arena, const Chan *chans, int ntiles_max, int e_xprod) {
   const Chan ch = chans[blockIdx.x];
   View v(arena, ch);
+  if (ch.tri) {  // no physical planes on the list path: dt stays fixed
+    if (threadIdx.x == 0) v.scal[SC_T] = v.scal[SC_T] + v.scal[SC_DT];
+    return;
+  }
   double m1 = 0.0, m2 = 1e-6, m3 = 0.0;
   // k_xprod's tiles of this channel (stride: the launch's tile count)
   const int TZ = tile_of(e_xprod, ch.Ng1, ch.Ng2), nt = (ch.Ng2 + TZ - 1) / TZ;

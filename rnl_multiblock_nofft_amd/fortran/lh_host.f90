@@ -15,6 +15,7 @@
 !   modes_to_planes_UVP, planes_to_modes_UVP/_NUVP      (FOU3D.f90:1000,2295,2772)
 !   read_in_field, record_out_field                     (start.f90:2444, FOU3D.f90:2856)
 !   write_stats_file                                    (stats.f90:254-441)
+!   nonlinRead                                          (start.f90:2983-3069)
 ! The LU factor argument `a`, `grid` and `myid` of LUsolU_W/LUsolV are
 ! dropped: the factors of the fixed-dt operators live on the device (built
 ! once at setup, as LU_buildP is in the reference), and there is one rank.
@@ -458,5 +459,93 @@ contains
     close(u)
     call lh_check(lh_stats_reset(lhctx), 'lh_stats_reset')
   end subroutine write_stats_file
+
+  ! ---------------------------------------------------------------------
+  ! start.f90:2983 nonlinRead: the triad lists of band iband from the files
+  ! trim(dirlist)//trim(heading)//'NN.dat' (NN = 01.., stream access: nx, nz,
+  ! j, then 9 x (type, length, int8 list(length,4))), with the reference's
+  ! plane mapping (u planes above the centre use index j-1, FOU3D.f90:
+  ! 1811-1815; planes above the centre read file Ngal3+1-j, start.f90:
+  ! 3025-3030, Ngal3 = ny-2), handed to the device (lh_set_triads).  A plane
+  ! whose file is absent gets no triads, and so does every plane when
+  ! `missing_ok` is false and a file is absent: then the routine stops.
+  subroutine nonlinRead(dirlist, heading, iband, missing_ok)
+    character(*), intent(in) :: dirlist, heading
+    integer, intent(in) :: iband
+    logical, intent(in) :: missing_ok
+    type :: rowbuf
+      integer(c_signed_char), allocatable :: b(:)
+    end type rowbuf
+    type(rowbuf) :: per(9)
+    integer :: ny, g3, nfile, q, j, jj, u, ios, t, nt, hdr(3), th(2), nrow, mid
+    integer(c_int), allocatable :: lens(:), mu(:), mv(:), have(:)
+    integer(c_signed_char), allocatable :: rows(:), tmp(:)
+    character(len=2) :: ext
+    character(len=1024) :: fname
+    logical :: ex
+    ny = nyb(iband)
+    g3 = ny - 2
+    mid = (g3 + 1) / 2
+    nfile = mid
+    allocate(lens(9 * nfile), mu(ny - 1), mv(ny - 2), have(nfile))
+    lens = 0
+    have = 0
+    allocate(rows(0))
+    nrow = 0
+    do q = 1, nfile
+      write(ext, '(i2.2)') q
+      fname = trim(dirlist) // trim(heading) // ext // '.dat'
+      inquire(file=trim(fname), exist=ex)
+      if (.not. ex) then
+        if (.not. missing_ok) then
+          write(*, '(a,a)') 'nonlinRead: missing ', trim(fname)
+          error stop 1
+        end if
+        cycle
+      end if
+      have(q) = 1
+      open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='old', action='read', &
+           iostat=ios)
+      read(u) hdr
+      if (hdr(1) /= N(1, iband) / 2 - 1 .or. hdr(2) /= N(2, iband) / 2 - 1 .or. hdr(3) /= q) then
+        write(*, '(a,a,3i6)') 'nonlinRead: header does not match the run: ', trim(fname), hdr
+        error stop 1
+      end if
+      do t = 1, 9
+        read(u) th
+        nt = th(2)
+        lens(9 * (q - 1) + th(1)) = nt
+        if (allocated(per(th(1))%b)) deallocate(per(th(1))%b)
+        allocate(per(th(1))%b(4 * nt))
+        read(u) per(th(1))%b
+      end do
+      close(u)
+      ! rows of a file in type order 1..9
+      do t = 1, 9
+        allocate(tmp(nrow + size(per(t)%b)))
+        tmp(1:nrow) = rows(1:nrow)
+        tmp(nrow + 1:) = per(t)%b
+        nrow = size(tmp)
+        call move_alloc(tmp, rows)
+      end do
+    end do
+    do j = 1, ny - 1
+      jj = j
+      if (j > mid) jj = j - 1
+      if (jj > mid) jj = g3 + 1 - jj
+      mu(j) = merge(jj - 1, -1, have(jj) == 1)
+    end do
+    do j = 1, ny - 2
+      jj = j
+      if (jj > mid) jj = g3 + 1 - jj
+      mv(j) = merge(jj - 1, -1, have(jj) == 1)
+    end do
+    if (nrow == 0) then
+      deallocate(rows)
+      allocate(rows(1))
+    end if
+    call lh_check(lh_set_triads(lhctx, int(iband - 1, c_int), int(nfile, c_int), lens, rows, mu, mv), &
+                  'lh_set_triads')
+  end subroutine nonlinRead
 
 end module lh_host

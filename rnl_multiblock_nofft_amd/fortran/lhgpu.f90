@@ -211,6 +211,14 @@ module lhgpu
       integer(c_int), value :: ch
       integer(c_int) :: rc
     end function
+    function lh_set_triads(ctx, ch, nfile, lens, rows, map_u, map_v) bind(C, name='lh_set_triads') result(rc)
+      import :: c_ptr, c_int, c_signed_char
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch, nfile
+      integer(c_int), intent(in) :: lens(*), map_u(*), map_v(*)
+      integer(c_signed_char), intent(in) :: rows(*)
+      integer(c_int) :: rc
+    end function
     function lh_lusol_p(ctx, ch, x) bind(C, name='lh_lusol_p') result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx, x

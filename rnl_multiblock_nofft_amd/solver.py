@@ -145,6 +145,25 @@ class Solver:
             return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
         check(self.L.lh_set_texture(self.h, int(ch), arg(bottom), arg(top)))
 
+    def set_triads(self, ch, files, map_u, map_v):
+        """Triad lists for channel ch (the reference's restricted nonlinear
+        path): files[q] a {type: int8 array (len, 4)} dict or None, map_u /
+        map_v the file index of u planes 1..ny-1 and v planes 1..ny-2 (-1:
+        none).  files = [] returns the channel to the FFT path."""
+        lens, chunks = [], []
+        for f in files:
+            for t in range(1, 10):
+                a = np.zeros((0, 4), np.int8) if f is None else np.asarray(f[t], np.int8).reshape(-1, 4)
+                lens.append(a.shape[0])
+                chunks.append(np.ascontiguousarray(a.T).reshape(-1))
+        rows = np.concatenate(chunks) if chunks else np.zeros(1, np.int8)
+        mu = np.array([m if m is not None and m >= 0 and files[m] is not None else -1 for m in map_u], np.int32)
+        mv = np.array([m if m is not None and m >= 0 and files[m] is not None else -1 for m in map_v], np.int32)
+        ln = np.array(lens if lens else [0], np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        check(self.L.lh_set_triads(self.h, int(ch), len(files), ln.ctypes.data_as(ip), rows.ctypes.data,
+                                   mu.ctypes.data_as(ip), mv.ctypes.data_as(ip)))
+
     def set_tuning(self, key, value):
         check(self.L.lh_set_tuning(self.h, int(key), int(value)))
 

@@ -251,3 +251,115 @@ def test_list_outside_band_rejected():
     bad = {t: np.array([[3, 0, 1, 0]], np.int8) for t in range(1, 10)}   # la + lp = 4 > nx = 3
     with pytest.raises(ValueError):
         o.set_triads([bad], [0] * 8, [0] * 7)
+
+
+def _golden_lists():
+    import importlib.util
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "make_triad_golden.py")
+    spec = importlib.util.spec_from_file_location("make_triad_golden", p)
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m.load()
+
+
+@pytest.mark.skipif(not os.path.isdir(REFDIR), reason="reference list files not present")
+def test_golden_lists_rebuild_reference_files():
+    g = _golden_lists()
+    for q, lists in g.items():
+        ref = triads.read_list(os.path.join(REFDIR, "full_y%02d.dat" % q))[3]
+        for t in range(1, 10):
+            assert len(lists[t]) == len(ref[t])
+            assert set(map(tuple, lists[t].tolist())) == set(map(tuple, ref[t].tolist()))
+
+
+# ---------------------------------------------------------------- GPU
+
+
+def _gpu_list_pair(cfg, files, mu, mv, seed, fields=None):
+    from rnl_multiblock_nofft_amd import Solver
+    s = Solver([cfg])
+    o = oracle_for(cfg)
+    f = fields if fields is not None else band_limited(cfg, s.grid(0), seed)
+    for k, v in f.items():
+        s.upload(0, k, v)
+        o.set(k, v)
+    s.init_flowrate()
+    o.init_flowrate()
+    s.set_triads(0, files, mu, mv)
+    o.set_triads(files, mu, mv)
+    return s, o
+
+
+def _compare(s, o, tol, what):
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        print(what, nm, "%.2e" % e, flush=True)
+        assert e <= tol, (what, nm, e)
+
+
+@pytest.mark.gpu
+def test_gpu_full_lists_match_oracle():
+    """Threshold-0 lists on every plane: one substep's nonlinear terms, one
+    step and three steps against the oracle's list path."""
+    cfg = small_cfg(N=16, ny=17)
+    full = triads.full_lists(16, 16)
+    from rnl_multiblock_nofft_amd import _lib
+    s, o = _gpu_list_pair(cfg, [full], [0] * 16, [0] * 15, seed=8)
+    s.phase(_lib.PHASE_NONLINEAR, 1)
+    s.phase(_lib.PHASE_RHS, 1)
+    o.phase("rhs0", 1)
+    o.phase("nonlinear", 1)
+    for nm in ("Nu1", "Nu2", "Nu3"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        print("nonlinear", nm, "%.2e" % e, flush=True)
+        assert e <= 1e-13, (nm, e)
+    s2, o2 = _gpu_list_pair(cfg, [full], [0] * 16, [0] * 15, seed=8)
+    s2.step(1)
+    o2.step(1)
+    _compare(s2, o2, 1e-12, "1 step")
+    s2.step(2)
+    o2.step(2)
+    _compare(s2, o2, 1e-11, "3 steps")
+
+
+@pytest.mark.gpu
+def test_gpu_thinned_lists_match_oracle():
+    """Asymmetric (thinned) lists, two files and planes without lists."""
+    cfg = small_cfg(N=8, ny=13)
+    full = triads.full_lists(8, 8)
+    files = [triads.thinned(full, 0.6, seed=1), triads.thinned(full, 0.3, seed=2), None]
+    mu = [0, 1, -1, 0, 1, 2, 1, 0, 0, 1, 0, 1]
+    mv = [1, 0, 0, -1, 1, 0, 1, 2, 0, 1, 1]
+    s, o = _gpu_list_pair(cfg, files, mu, mv, seed=3)
+    s.step(2)
+    o.step(2)
+    _compare(s, o, 1e-11, "thinned")
+
+
+@pytest.mark.gpu
+def test_gpu_reference_lists_32():
+    """The reference's own lists (files 1, 6, 9 of its 32x32x153 case, from
+    the committed fixture) on the planes that read them; the other planes
+    keep the mean-mode terms only.  One step against the oracle."""
+    from rnl_multiblock_nofft_amd import configs
+    cfg = dict(configs.smooth_32()[0])
+    g = _golden_lists()
+    ny = cfg["ny"]
+    u, v = triads.plane_files(ny)
+    files = [g.get(q) for q in range(1, max(u + v) + 1)]
+    mu = [q - 1 if g.get(q) is not None else -1 for q in u]
+    mv = [q - 1 if g.get(q) is not None else -1 for q in v]
+    assert sum(m >= 0 for m in mu) >= 6 and sum(m >= 0 for m in mv) >= 6
+    s, o = _gpu_list_pair(cfg, files, mu, mv, seed=21)
+    s.step(1)
+    o.step(1)
+    _compare(s, o, 1e-12, "reference lists")
+
+
+@pytest.mark.gpu
+def test_gpu_lists_need_fixed_dt():
+    from rnl_multiblock_nofft_amd import Solver, _lib
+    cfg = small_cfg(N=8, ny=9, adaptive_dt=1, CFL=0.5)
+    s = Solver([cfg])
+    with pytest.raises(_lib.LhError):
+        s.set_triads(0, [triads.full_lists(8, 8)], [0] * 8, [0] * 7)
