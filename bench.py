@@ -77,7 +77,41 @@ def measured_traffic(workload, precision, kernel):
     return None
 
 
-def cpu_baseline(cfg, seconds=12.0, max_sub=60):
+def triad_setup(cfg, mode):
+    """Lists of the triad-list path for one channel: 'full' = the threshold-0
+    lists on every plane, 'thinned' = 65 % of their rows (the density of the
+    reference's own 32x32x153 files, 791 627 of 1 216 945 rows for plane 1).
+    Returns (files, map_u, map_v)."""
+    from rnl_multiblock_nofft_amd import triads
+    full = triads.full_lists(cfg["N1"], cfg["N2"])
+    lists = full if mode == "full" else triads.thinned(full, 0.65, seed=1)
+    ny = cfg["ny"]
+    return [lists], [0] * (ny - 1), [0] * (ny - 2)
+
+
+def triad_bytes(cfg, files):
+    """Algorithmic bytes of the list stage per substep (k_triad + k_triad_fin):
+    every table entry (4 B per row and z mirror) and target offset read once,
+    the operand rows of each plane read once, the product rows written and
+    read back once, the derivative and flux rows written."""
+    N1, N2, ny = cfg["N1"], cfg["N2"], cfg["ny"]
+    ncol = (N1 // 2 + 1) * N2
+    row = ncol * 16
+    nplu, nplv = ny - 1, ny - 2
+    lists = files[0]
+
+    def ent(types):
+        n = 0
+        for t in types:
+            a = lists[t].astype(int)
+            n += 2 * len(a) - int(((a[:, 1] + a[:, 3]) == 0).sum())
+        return n * 4 + len(types) * (ncol + 1) * 4
+
+    return (nplu * ent((1, 3, 5, 7, 9)) + nplv * ent((2, 4, 6, 8)) + (4 * nplu + 5 * nplv) * row
+            + 2 * (5 * nplu + 4 * nplv) * row + 3 * (nplu + nplv) * row)
+
+
+def cpu_baseline(cfg, seconds=12.0, max_sub=60, triads_mode=None):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import init_fields, oracle_for
     from oracle import pyoracle as po
@@ -92,6 +126,8 @@ def cpu_baseline(cfg, seconds=12.0, max_sub=60):
     for name in ("u1", "u2", "u3", "p"):
         o.set(name, f[name])
     o.init_flowrate()
+    if triads_mode:
+        o.set_triads(*triad_setup(cfg, triads_mode))
     o.substep(1)  # warm plans / threads
     n = 0
     t0 = time.perf_counter()
@@ -103,8 +139,9 @@ def cpu_baseline(cfg, seconds=12.0, max_sub=60):
     dt = time.perf_counter() - t0
     gp = cfg["N1"] * cfg["N2"] * cfg["ny"]
     return dict(value=gp * n / dt, unit="grid-pt RK3 substeps/s", cores=cores, kind="port",
-                sample="%d RK3 substeps of one %dx%dx%d channel (FP64 CPU oracle, OpenMP, %d threads), %.1f s"
-                       % (n, cfg["N1"], cfg["N2"], cfg["ny"], cores, dt))
+                sample="%d RK3 substeps of one %dx%dx%d channel (FP64 CPU oracle%s, OpenMP, %d threads), %.1f s"
+                       % (n, cfg["N1"], cfg["N2"], cfg["ny"],
+                          ", triad-list path" if triads_mode else "", cores, dt))
 
 
 def main():
@@ -121,6 +158,9 @@ def main():
     ap.add_argument("--smooth", action="store_true",
                     help="drop the workload's wall textures (smooth no-slip walls)")
     ap.add_argument("--kernel-substeps", type=int, default=30)
+    ap.add_argument("--triads", default="", choices=["", "full", "thinned"],
+                    help="run the triad-list nonlinear path (the reference's restricted "
+                         "nonlinear term) with generated lists on every plane")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB, 5 times (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -155,6 +195,11 @@ def main():
         for name in ("u1", "u2", "u3", "p"):
             s.upload(q, name, f[name])
     s.init_flowrate()
+    tri_files = {}
+    if args.triads:
+        for q, c in enumerate(cfgs):
+            tri_files[q] = triad_setup(c, args.triads)
+            s.set_triads(q, *tri_files[q])
     if args.tune:
         keys = {"zinv_points": _lib.TUNE_ZINV_POINTS, "xprod_points": _lib.TUNE_XPROD_POINTS,
                 "zfwd_points": _lib.TUNE_ZFWD_POINTS, "solve_cols": _lib.TUNE_SOLVE_COLS}
@@ -179,12 +224,19 @@ def main():
     # live per-kernel durations (HIP events on the library's stream)
     nsub = max(3, args.kernel_substeps)
     ms = s.time_kernels(nsub) / nsub
+    kernels = list(KERNELS)
     alg = {k: 0 for k in KERNELS}
     for c in cfgs:
         for k, v in algorithmic_bytes(c, args.precision).items():
             alg[k] += v
-    dom = KERNELS[int(max(range(5), key=lambda q: ms[q]))]  # the largest single kernel
-    dom_ms = float(ms[KERNELS.index(dom)])
+    if args.triads:
+        # the list kernels are timed in the x-product slot; the FFT stages
+        # do not run
+        kernels[1] = "k_triad"
+        alg["k_triad"] = sum(triad_bytes(c, tri_files[q][0]) for q, c in enumerate(cfgs))
+        ms[0] = ms[2] = 0.0
+    dom = kernels[int(max(range(5), key=lambda q: ms[q]))]  # the largest single kernel
+    dom_ms = float(ms[kernels.index(dom)])
     achieved = alg[dom] / (dom_ms * 1e-3) / 1e9
     kname = dom
     if dom == "k_solve" and tuning.get("solve_cols", 0) > 0:
@@ -193,14 +245,14 @@ def main():
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic, kernel=kname,
                     kernel_ms=round(dom_ms, 5), alg_bytes=alg[dom],
-                    kernels_ms=dict({k: round(float(ms[q]), 5) for q, k in enumerate(KERNELS)},
+                    kernels_ms=dict({k: round(float(ms[q]), 5) for q, k in enumerate(kernels)},
                                     **{TEX: round(float(ms[5]), 5)}))
     bmodel = sum(baseline_model_bytes(c, args.precision) for c in cfgs)
     hbm_pct = 100.0 * bmodel * (3 * args.steps / elapsed) / (HBM_PEAK_GBS * 1e9)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(max(cfgs, key=lambda c: c["N1"] * c["N2"] * c["ny"]))
+        cpu = cpu_baseline(max(cfgs, key=lambda c: c["N1"] * c["N2"] * c["ny"]), triads_mode=args.triads or None)
     rec = dict(metric=METRIC, value=round(value, 1), unit="grid-pt RK3 substeps/s", n_gpus=1,
                steps=args.steps, warmup=args.warmup, ms_per_step=round(ms_per_step, 5),
                higher_is_better=True, scaling="weak", vs_baseline=None,
@@ -210,6 +262,7 @@ def main():
                            precision=args.precision,
                            walls="textured" if any(c.get("texture") is not None for c in cfgs) else "smooth",
                            dt=cfgs[0]["dt"],
+                           nonlinear=("triad lists (%s)" % args.triads) if args.triads else "fft",
                            grid_points=gp, hbm_pct_of_peak_model=round(hbm_pct, 3),
                            tuning=tuning),
                roofline=roofline, cpu_baseline=cpu)

@@ -6,8 +6,9 @@
 // The host turns every list row (la, ka, lp, kp) and its z mirror into
 // (target column, passive column + conjugate flag, active column +
 // conjugate flag) and sorts them by target (compressed rows per target
-// column), so that one thread owns a target and sums its products without
-// atomics, in a fixed order.  k_triad stages the plane's two operand rows in
+// column), so that one wave owns a target and sums its products without
+// atomics, in a fixed order (a wave per target, lanes striding its
+// entries, a butterfly sum).  k_triad stages the plane's two operand rows in
 // LDS and writes one product field per (plane, list type); k_triad_fin forms
 // the x/z derivatives and the y-flux rows the right-hand-side kernel reads,
 // exactly where k_zfwd writes them on the FFT path.
@@ -116,20 +117,14 @@ __global__ void __launch_bounds__(256) k_triad(char *arena, const Chan *chans) {
   }
   double2 *out = ch.tri_prod + ((size_t)slot * 5 + ti) * ncol;
   const bool square = type == 1 || type == 5 || type == 9;
-  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
-    double2 x;
-    if (square) {
-      x = make_double2(2.0 * (a00 * A[c].x), 2.0 * (a00 * A[c].y));
-    } else if (type == 3 || type == 2 || type == 6) {  // buff = p00 a + a00 p
-      x = make_double2(p00 * A[c].x + a00 * P[c].x, p00 * A[c].y + a00 * P[c].y);
-    } else {  // 7, 4, 8: buff = a00 p + p00 a
-      x = make_double2(a00 * P[c].x + p00 * A[c].x, a00 * P[c].y + p00 * A[c].y);
-    }
-    if (c == 0 && zeroth) x = zsum;
+  // one wave per target column: the lanes read consecutive table entries
+  // (one 256-byte run per step) and the partial sums meet in a butterfly
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  for (int c = wave; c < ncol; c += nwave) {
+    double2 acc = make_double2(0, 0);
     if (off) {
-      double2 acc = make_double2(0, 0);
       const unsigned e1 = off[c + 1];
-      for (unsigned e = off[c]; e < e1; e++) {
+      for (unsigned e = off[c] + lane; e < e1; e += 64) {
         const unsigned w = ent[e];
         double2 a = A[w & 0x7fffu], b = P[(w >> 16) & 0x7fffu];
         if (w & 0x8000u) a.y = -a.y;
@@ -137,9 +132,23 @@ __global__ void __launch_bounds__(256) k_triad(char *arena, const Chan *chans) {
         const double2 pr = cm(a, b);
         acc = make_double2(acc.x + pr.x, acc.y + pr.y);
       }
-      x = make_double2(x.x + acc.x, x.y + acc.y);
+      for (int o = 32; o >= 1; o >>= 1) {
+        acc.x += __shfl_xor(acc.x, o);
+        acc.y += __shfl_xor(acc.y, o);
+      }
     }
-    out[c] = x;
+    if (lane == 0) {
+      double2 x;
+      if (square) {
+        x = make_double2(2.0 * (a00 * A[c].x), 2.0 * (a00 * A[c].y));
+      } else if (type == 3 || type == 2 || type == 6) {  // buff = p00 a + a00 p
+        x = make_double2(p00 * A[c].x + a00 * P[c].x, p00 * A[c].y + a00 * P[c].y);
+      } else {  // 7, 4, 8: buff = a00 p + p00 a
+        x = make_double2(a00 * P[c].x + p00 * A[c].x, a00 * P[c].y + p00 * A[c].y);
+      }
+      if (c == 0 && zeroth) x = zsum;
+      out[c] = make_double2(x.x + acc.x, x.y + acc.y);
+    }
   }
 }
 
