@@ -6,8 +6,8 @@
 // The host turns every list row (la, ka, lp, kp) and its z mirror into
 // (target column, passive column + conjugate flag, active column +
 // conjugate flag) and sorts them by target (compressed rows per target
-// column), so that one wave owns a target and sums its products without
-// atomics, in a fixed order (a wave per target, lanes striding its
+// column), so that a lane group owns a target and sums its products without
+// atomics, in a fixed order (eight lanes per target striding its
 // entries, a butterfly sum).  k_triad stages the plane's two operand rows in
 // LDS and writes one product field per (plane, list type); k_triad_fin forms
 // the x/z derivatives and the y-flux rows the right-hand-side kernel reads,
@@ -117,27 +117,42 @@ __global__ void __launch_bounds__(256) k_triad(char *arena, const Chan *chans) {
   }
   double2 *out = ch.tri_prod + ((size_t)slot * 5 + ti) * ncol;
   const bool square = type == 1 || type == 5 || type == 9;
-  // one wave per target column: the lanes read consecutive table entries
-  // (one 256-byte run per step) and the partial sums meet in a butterfly
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  for (int c = wave; c < ncol; c += nwave) {
+  // eight lanes per target column, eight targets per wave: each load
+  // instruction fetches eight contiguous 32-byte runs of the table, and the
+  // partial sums meet in a three-level butterfly
+  const int lane = threadIdx.x & 63, grp = lane >> 3, sub = lane & 7;
+  const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  unsigned *OFF = reinterpret_cast<unsigned *>(P + ncol);
+  if (off)
+    for (int c = threadIdx.x; c <= ncol; c += blockDim.x) OFF[c] = off[c];
+  __syncthreads();
+  auto term = [&](unsigned w) -> double2 {
+    double2 a = A[w & 0x7fffu], b = P[(w >> 16) & 0x7fffu];
+    if (w & 0x8000u) a.y = -a.y;
+    if (w & 0x80000000u) b.y = -b.y;
+    return cm(a, b);
+  };
+  for (int c0 = wave * 8; c0 < ncol; c0 += nwave * 8) {
+    const int c = c0 + grp;
     double2 acc = make_double2(0, 0);
-    if (off) {
-      const unsigned e1 = off[c + 1];
-      for (unsigned e = off[c] + lane; e < e1; e += 64) {
-        const unsigned w = ent[e];
-        double2 a = A[w & 0x7fffu], b = P[(w >> 16) & 0x7fffu];
-        if (w & 0x8000u) a.y = -a.y;
-        if (w & 0x80000000u) b.y = -b.y;
-        const double2 pr = cm(a, b);
-        acc = make_double2(acc.x + pr.x, acc.y + pr.y);
+    if (off && c < ncol) {
+      const unsigned e1 = OFF[c + 1];
+      unsigned e = OFF[c] + sub;
+      for (; e + 24 < e1; e += 32) {
+        const unsigned w0 = ent[e], w1 = ent[e + 8], w2 = ent[e + 16], w3 = ent[e + 24];
+        const double2 t0 = term(w0), t1 = term(w1), t2 = term(w2), t3 = term(w3);
+        acc = make_double2(acc.x + ((t0.x + t1.x) + (t2.x + t3.x)), acc.y + ((t0.y + t1.y) + (t2.y + t3.y)));
       }
-      for (int o = 32; o >= 1; o >>= 1) {
-        acc.x += __shfl_xor(acc.x, o);
-        acc.y += __shfl_xor(acc.y, o);
+      for (; e < e1; e += 8) {
+        const double2 t0 = term(ent[e]);
+        acc = make_double2(acc.x + t0.x, acc.y + t0.y);
       }
     }
-    if (lane == 0) {
+    for (int o = 4; o >= 1; o >>= 1) {
+      acc.x += __shfl_xor(acc.x, o);
+      acc.y += __shfl_xor(acc.y, o);
+    }
+    if (sub == 0 && c < ncol) {
       double2 x;
       if (square) {
         x = make_double2(2.0 * (a00 * A[c].x), 2.0 * (a00 * A[c].y));
@@ -184,8 +199,8 @@ __global__ void __launch_bounds__(256) k_triad_fin(char *arena, const Chan *chan
 }
 
 hipError_t launch_triads(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s) {
-  hipLaunchKernelGGL(k_triad, dim3(m.nslot, 5, m.nchan), dim3(256), 2 * (size_t)m.ncol * sizeof(double2), s, arena,
-                     d_chans);
+  const size_t lds = 2 * (size_t)m.ncol * sizeof(double2) + ((size_t)m.ncol + 1) * sizeof(unsigned);
+  hipLaunchKernelGGL(k_triad, dim3(m.nslot, 5, m.nchan), dim3(256), lds, s, arena, d_chans);
   hipLaunchKernelGGL(k_triad_fin, dim3((m.ncol + 255) / 256, m.nslot, m.nchan), dim3(256), 0, s, arena, d_chans);
   return hipGetLastError();
 }
