@@ -104,6 +104,13 @@ int lh_field_download(lh_ctx *ctx, int ch, int field, double *reim);
  * step of the last first substep (dtc_calc, FOU3D.f90:699-736), 0 before */
 int lh_get_scalars(lh_ctx *ctx, int ch, double *out8);
 int lh_set_scalars(lh_ctx *ctx, int ch, double t, double mpgx, double QxT, int iter);
+/* The history record record_out writes (FOU3D.f90:3040-3070), reduced on the
+ * device: out[0..11] = t, dtv, dtc, dt, err (max |div| over the pressure
+ * rows, `error` littleharsh.f90:342-375), Qx (flowrateIm of the mean mode),
+ * QxT, mpgx, dgx (the last flow-rate correction g, flowrate_corr
+ * littleharsh.f90:476-509), Umax (maxvel, littleharsh.f90:511-527), Uslip
+ * (the mean u1 interpolated to y = -1), iter. */
+int lh_history(lh_ctx *ctx, int ch, double *out12);
 /* QxT := flow rate of the current u1 mean mode (getini, start.f90:2166-2171;
  * flowrateIm littleharsh.f90:435) */
 int lh_init_flowrate(lh_ctx *ctx);

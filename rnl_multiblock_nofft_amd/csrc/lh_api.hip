@@ -225,6 +225,7 @@ struct lh_ctx {
   int ntex1d = 0;                    // channels on that path (the rest: ntex - ntex1d)
   bool adaptive = false;             // a channel with an adaptive time step
   char *stats = nullptr;             // statistics accumulators (allocated on first use)
+  double *hist = nullptr;            // history-record reductions, 8 doubles per channel
   std::vector<char *> tri_mem;       // per channel: triad-list tables and products (or null)
   int ntri = 0;                      // channels on the triad-list path
   size_t stats_bytes = 0;
@@ -626,6 +627,7 @@ int lh_ctx_destroy(lh_ctx *x) {
   if (x->d_chans) (void)hipFree(x->d_chans);
   if (x->staging) (void)hipFree(x->staging);
   if (x->stats) (void)hipFree(x->stats);
+  if (x->hist) (void)hipFree(x->hist);
   for (char *p : x->tri_mem)
     if (p) (void)hipFree(p);
   if (x->stream) (void)hipStreamDestroy(x->stream);
@@ -709,6 +711,38 @@ int lh_get_scalars(lh_ctx *x, int ch, double *o) {
   o[5] = h.dtv;
   o[6] = sc[SC_DTC];
   o[7] = h.iter;
+  return LH_OK;
+}
+
+int lh_history(lh_ctx *x, int ch, double *o) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || !o) return set_err(LH_EINVAL, "lh_history: bad arguments");
+  if (!x->hist) {
+    hipError_t e = hipMalloc((void **)&x->hist, 8 * x->ch.size() * sizeof(double));
+    if (e != hipSuccess) {
+      x->hist = nullptr;
+      return set_err(LH_ENOMEM, "lh_history: hipMalloc: %s", hipGetErrorString(e));
+    }
+  }
+  HostChan &h = x->ch[ch];
+  HIPCHK(launch_history(x->arena, x->d_chans, x->shape, x->hist, x->stream));
+  double r[8], sc[SC_NSCAL];
+  HIPCHK(hipMemcpyAsync(r, x->hist + 8 * ch, sizeof r, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(sc, View(x->arena, h.dev).scal, sizeof sc, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  o[0] = sc[SC_T];
+  o[1] = h.dtv;
+  o[2] = sc[SC_DTC];
+  o[3] = sc[SC_DT];
+  o[4] = r[0];
+  o[5] = r[1];
+  o[6] = sc[SC_QXT];
+  o[7] = sc[SC_MPGX];
+  o[8] = sc[SC_DGX];
+  o[9] = r[2];
+  o[10] = r[3];
+  o[11] = h.iter;
   return LH_OK;
 }
 

@@ -35,7 +35,8 @@ __host__ __device__ inline double rk_d(int k) { return k == 1 ? 8.0 / 15.0 : (k 
 // time step of dtc_calc; SC_DTV the viscous one; SC_M1..M3 the bit patterns of
 // the CFL maxima max|u1|, max|v dthdyv|, max|u3| (non-negative doubles
 // compare as integers)
-enum { SC_MPGX = 0, SC_QXT, SC_QX, SC_DT, SC_T, SC_QX1, SC_DTC = 8, SC_DTV, SC_M1, SC_M2, SC_M3, SC_NSCAL = 16 };
+// SC_DGX: the last flow-rate correction g (dgx of flowrate_corr)
+enum { SC_MPGX = 0, SC_QXT, SC_QX, SC_DT, SC_T, SC_QX1, SC_DTC = 8, SC_DTV, SC_M1, SC_M2, SC_M3, SC_DGX, SC_NSCAL = 16 };
 
 // statistics profiles in the order write_stats writes them (stats.f90:356-368)
 enum { ST_UM = 0, ST_U2M, ST_VM, ST_V2M, ST_WM, ST_W2M, ST_PM, ST_P2M, ST_UVM, ST_UWM, ST_VWM, ST_WXM, ST_WX2M, ST_N };

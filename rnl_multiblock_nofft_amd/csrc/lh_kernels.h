@@ -32,6 +32,8 @@ hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int k
 hipError_t launch_factors(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 hipError_t launch_dt_update(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 hipError_t launch_u11(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
+// err, Qx, Umax, Uslip of every channel into out[ch*8 + 0..3] (device)
+hipError_t launch_history(char *arena, const Chan *d_chans, const ShapeMax &m, double *out, hipStream_t s);
 hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
 hipError_t launch_flowrate(char *arena, const Chan *d_chans, int nchan, int set_target, hipStream_t s);
 hipError_t launch_to_rows(const double2 *src, double2 *dst, int ncol, int nrows, int jlo, int ldc,

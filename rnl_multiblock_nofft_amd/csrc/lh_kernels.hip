@@ -691,6 +691,7 @@ __global__ void __launch_bounds__(64) k_solve(char *arena, const Chan *chans, in
   if (col0 && ch.flag_ctpress == 0) {
     g = (v.scal[SC_QXT] - Qx0) / v.scal[SC_QX1 + kRK - 1];
     v.scal[SC_MPGX] = v.scal[SC_MPGX] + g;
+    v.scal[SC_DGX] = g;
   }
   const double *__restrict__ u11 = v.u11 + (size_t)(kRK - 1) * nyu;
 
@@ -1010,6 +1011,7 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
       if (ch.flag_ctpress == 0) {
         g = (v.scal[SC_QXT] - part) / v.scal[SC_QX1 + kRK - 1];
         v.scal[SC_MPGX] = v.scal[SC_MPGX] + g;
+        v.scal[SC_DGX] = g;
       }
       g_sh = g;
     }
@@ -1293,6 +1295,46 @@ __global__ void k_u11(char *arena, const Chan *chans) {
   double Q = 0.0;
   for (int j = 1; j <= n - 2; j++) Q = Q + (v.yv[j] - v.yv[j - 1]) * x[j];
   v.scal[SC_QX1 + kRK - 1] = Q;
+}
+
+// History record of record_out (FOU3D.f90:3040-3070): err = max |div| over
+// the pressure rows (error, littleharsh.f90:342-375), the flow rate of the
+// mean mode (flowrateIm), Umax = max Re u1(0,0) over rows 0..ny (maxvel,
+// littleharsh.f90:511-527) and the mean slip velocity interpolated to
+// y = -1.  One workgroup per channel; out[ch*8 + 0..3] = err, Qx, Umax, Uslip.
+__global__ void __launch_bounds__(1024) k_history(char *arena, const Chan *chans, double *out) {
+  const Chan ch = chans[blockIdx.x];
+  const View v(arena, ch);
+  const int ncol = ch.ncol, L = ch.ldc, nyu = ch.nyu;
+  double e = 0.0;
+  const int n = (nyu - 2) * ncol;  // p rows 1..ny-1
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const int r = q / ncol, c = q - r * ncol;
+    const double2 d = v.div[(size_t)(r + 1) * L + c];
+    e = fmax(e, sqrt(d.x * d.x + d.y * d.y));
+  }
+  for (int off = 32; off >= 1; off >>= 1) e = fmax(e, __shfl_xor(e, off));
+  __shared__ double red[16];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int q = 1; q < (int)(blockDim.x >> 6); q++) e = fmax(e, red[q]);
+  e = fmax(e, red[0]);
+  double Q = 0.0, um = 0.0;
+  for (int j = 1; j <= nyu - 2; j++) Q = Q + (v.yv[j] - v.yv[j - 1]) * v.u1[(size_t)j * L].x;
+  for (int j = 0; j < nyu; j++) um = fmax(um, v.u1[(size_t)j * L].x);
+  const double2 a = v.u1[L], b = v.u1[0];
+  const double us = (a.x * (-1.0 - v.yu[0]) + b.x * (v.yu[1] + 1.0)) / (v.yu[1] - v.yu[0]);
+  double *o = out + 8 * blockIdx.x;
+  o[0] = e;
+  o[1] = Q;
+  o[2] = um;
+  o[3] = us;
+}
+
+hipError_t launch_history(char *arena, const Chan *d_chans, const ShapeMax &m, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_history, dim3(m.nchan), dim3(1024), 0, s, arena, d_chans, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_dt_update(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s) {

@@ -103,3 +103,53 @@ def write_field(path, field, jlo, grid, y, dthdy, t=0.0, Re=0.0, alp=1.0, bet=2.
                    + pl.T.astype("<f8").tobytes())
     with open(path, "wb") as f:
         f.write(bytes(out))
+
+
+# ---------------------------------------------------------------------------
+# History file (hist_NNNNxNNNNxNNNN.dat): start.f90:238-253 opens it and
+# writes two header records, (t, Re, alp, bet, mpgx, nband) and N(4,
+# 0:nband+1); record_out appends one record per output (FOU3D.f90:3063-3068):
+#   flag_ctpress = 0: flag, iter, t, dtv, dtc, dt, err, Qx, QxT, mpgx, dgx, Umax, Uslip
+#   flag_ctpress = 1: flag, iter, t, dtv, dtc, dt, err, Qx, mpgx, Umax, Uslip
+HIST_KEYS = {0: ("t", "dtv", "dtc", "dt", "err", "Qx", "QxT", "mpgx", "dgx", "Umax", "Uslip"),
+             1: ("t", "dtv", "dtc", "dt", "err", "Qx", "mpgx", "Umax", "Uslip")}
+
+
+def read_hist(path):
+    """dict(t, Re, alp, bet, mpgx, nband, N, records=[dict(flag, iter, ...)])."""
+    with open(path, "rb") as f:
+        recs = list(_records(f.read()))
+    t, Re, alp, bet, mpgx = struct.unpack_from("<5d", recs[0], 0)
+    (nband,) = struct.unpack_from("<i", recs[0], 40)
+    N = np.frombuffer(recs[1], dtype="<i4").reshape(nband + 2, 4).T.copy()
+    out = []
+    for r in recs[2:]:
+        flag, it = struct.unpack_from("<2i", r, 0)
+        vals = struct.unpack_from("<%dd" % ((len(r) - 8) // 8), r, 8)
+        d = dict(zip(HIST_KEYS[flag], vals))
+        d.update(flag=flag, iter=it)
+        out.append(d)
+    return dict(t=t, Re=Re, alp=alp, bet=bet, mpgx=mpgx, nband=nband, N=N, records=out)
+
+
+def hist_record(flag_ctpress, h):
+    """One history record's bytes from a dict with the HIST_KEYS values and
+    iter (Solver.history() gives them)."""
+    payload = struct.pack("<2i", int(flag_ctpress), int(h["iter"])) + \
+        struct.pack("<%dd" % len(HIST_KEYS[flag_ctpress]), *[float(h[k]) for k in HIST_KEYS[flag_ctpress]])
+    return struct.pack("<i", len(payload)) + payload + struct.pack("<i", len(payload))
+
+
+def write_hist(path, t, Re, alp, bet, mpgx, N, records, flag_ctpress):
+    """A history file with its two header records and `records` (dicts)."""
+    N = np.asarray(N, dtype="<i4")
+    nband = N.shape[1] - 2
+
+    def rec(p):
+        return struct.pack("<i", len(p)) + p + struct.pack("<i", len(p))
+
+    with open(path, "wb") as f:
+        f.write(rec(struct.pack("<5d", t, Re, alp, bet, mpgx) + struct.pack("<i", nband)))
+        f.write(rec(N.T.astype("<i4").tobytes()))
+        for h in records:
+            f.write(hist_record(flag_ctpress, h))

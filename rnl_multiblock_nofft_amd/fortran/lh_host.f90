@@ -15,6 +15,7 @@
 !   modes_to_planes_UVP, planes_to_modes_UVP/_NUVP      (FOU3D.f90:1000,2295,2772)
 !   read_in_field, record_out_field                     (start.f90:2444, FOU3D.f90:2856)
 !   write_stats_file                                    (stats.f90:254-441)
+!   hist_open, hist_write                               (start.f90:238-253, FOU3D.f90:3040-3070)
 !   nonlinRead                                          (start.f90:2983-3069)
 ! The LU factor argument `a`, `grid` and `myid` of LUsolU_W/LUsolV are
 ! dropped: the factors of the fixed-dt operators live on the device (built
@@ -40,6 +41,9 @@ module lh_host
     integer, allocatable :: Nb(:,:)
     real(8), allocatable :: y(:), dthdy(:)
     real(8) :: dthetavi = 1d0
+    ! as read: the grid code and y of each plane record (yrow(jlo:jhi))
+    integer :: grid = 0
+    real(8), allocatable :: yrow(:)
   end type field_header
 
 contains
@@ -365,7 +369,7 @@ contains
       write(*, '(a)') 'read_in_field: rows are not a contiguous range'
       error stop 1
     end if
-    allocate(f(jlo:jhi, (nx / 2) * nz), pl(nx, nz), js(nrec))
+    allocate(f(jlo:jhi, (nx / 2) * nz), pl(nx, nz), js(nrec), hdr%yrow(jlo:jhi))
     pos = 1
     do q = 1, 3
       read(u, pos=pos) m
@@ -374,6 +378,8 @@ contains
     do q = 1, nrec
       read(u, pos=pos) m
       read(u) j, grid, nx, nz, yj, pl
+      hdr%grid = grid
+      hdr%yrow(j) = yj
       read(u) m2
       if (m2 /= m) then
         write(*, '(a)') 'read_in_field: corrupt record marker'
@@ -459,6 +465,47 @@ contains
     close(u)
     call lh_check(lh_stats_reset(lhctx), 'lh_stats_reset')
   end subroutine write_stats_file
+
+  ! ---------------------------------------------------------------------
+  ! History file (start.f90:238-253): opened for appending when it exists,
+  ! else created with the records (t, Re, alp, bet, mpgx, nband) and
+  ! N(4, 0:nband+1); hist_write appends record_out's line (FOU3D.f90:
+  ! 3063-3068) from the device's history reduction (lh_history).
+  subroutine hist_open(fname, t, Re, alp, bet, mpgx, Nb, u)
+    character(*), intent(in) :: fname
+    real(8), intent(in) :: t, Re, alp, bet, mpgx
+    integer, intent(in) :: Nb(:,:)
+    integer, intent(out) :: u
+    logical :: ex
+    integer :: m
+    inquire(file=fname, exist=ex)
+    if (ex) then
+      open(newunit=u, file=fname, access='stream', form='unformatted', status='old', position='append', &
+           action='write')
+    else
+      open(newunit=u, file=fname, access='stream', form='unformatted', status='new', action='write')
+      m = 5 * 8 + 4
+      write(u) m, t, Re, alp, bet, mpgx, size(Nb, 2) - 2, m
+      m = 4 * size(Nb)
+      write(u) m, Nb, m
+    end if
+  end subroutine hist_open
+
+  subroutine hist_write(u, flag_ctpress, iband)
+    integer, intent(in) :: u, flag_ctpress, iband
+    real(c_double) :: h(12)
+    integer :: m
+    call lh_check(lh_history(lhctx, int(iband - 1, c_int), h), 'lh_history')
+    ! h = t, dtv, dtc, dt, err, Qx, QxT, mpgx, dgx, Umax, Uslip, iter
+    if (flag_ctpress == 0) then
+      m = 2 * 4 + 11 * 8
+      write(u) m, flag_ctpress, nint(h(12)), h(1:11), m
+    else
+      m = 2 * 4 + 9 * 8
+      write(u) m, flag_ctpress, nint(h(12)), h(1:6), h(8), h(10:11), m
+    end if
+    flush(u)
+  end subroutine hist_write
 
   ! ---------------------------------------------------------------------
   ! start.f90:2983 nonlinRead: the triad lists of band iband from the files

@@ -211,6 +211,13 @@ module lhgpu
       integer(c_int), value :: ch
       integer(c_int) :: rc
     end function
+    function lh_history(ctx, ch, out12) bind(C, name='lh_history') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch
+      real(c_double), intent(out) :: out12(12)
+      integer(c_int) :: rc
+    end function
     function lh_set_triads(ctx, ch, nfile, lens, rows, map_u, map_v) bind(C, name='lh_set_triads') result(rc)
       import :: c_ptr, c_int, c_signed_char
       type(c_ptr), value :: ctx

@@ -12,7 +12,9 @@
 ! of u1, u2, u3, p.  With nstat > 0 a statistics sample is taken after every
 ! nstat steps (the reference samples at the first substep of every nstat-th
 ! step, FOU3D.f90:58-63) and written to fstats at the end (write_stats,
-! stats.f90:254-441).
+! stats.f90:254-441).  With fhist set, the history file gets a record
+! (record_out's, FOU3D.f90:3040-3070) at the start and after every nstat
+! steps (or at the end).
 program littleharsh_gpu
   use iso_c_binding
   use lhgpu
@@ -22,15 +24,15 @@ program littleharsh_gpu
   integer :: precision = 0, nsteps = 1, nstat = 0
   real(8) :: alp = 1d0, bet = 2d0, Re = 4200d0, dyq = 10d0, gamma = 2d0
   real(8) :: bslpu1 = 0d0, bslpu3 = 0d0, mpgx = 0d0, dt = 1d-3
-  character(len=1024) :: fin(4) = '', fout(4) = '', fstats = '', nmlfile
+  character(len=1024) :: fin(4) = '', fout(4) = '', fstats = '', fhist = '', nmlfile
   namelist /run/ N1, N2, ny, alp, bet, Re, grid_kind, dyq, ppp, gamma, bslpu1, bslpu3, &
-                 flag_ctpress, mpgx, dt, precision, nsteps, fin, fout, nstat, fstats
+                 flag_ctpress, mpgx, dt, precision, nsteps, fin, fout, nstat, fstats, fhist
   type(lh_config) :: cfg(1)
   type(field_header) :: hdr(4), shdr
   complex(8), allocatable :: f(:,:)
   real(8), allocatable :: yu(:), yv(:), du(:), dv(:), yrow(:)
   real(8) :: sc(8)
-  integer :: u, q, jlo, jhi, ncol, grid, done, ngal_arr(4, 3)
+  integer :: u, q, jlo, jhi, ncol, grid, done, ngal_arr(4, 3), uh
   integer, parameter :: ids(4) = [LH_U1, LH_U2, LH_U3, LH_P]
   integer :: elo(4), ehi(4)
 
@@ -62,18 +64,25 @@ program littleharsh_gpu
   call lh_check(lh_init_flowrate(lhctx), 'lh_init_flowrate')
 
   ! main loop (littleharsh.f90:172-228)
+  if (len_trim(fhist) > 0) then
+    call hist_open(trim(fhist), hdr(1)%t, Re, alp, bet, mpgx, hdr(1)%Nb, uh)
+    call hist_write(uh, flag_ctpress, 1)
+  end if
   if (nstat > 0) then
     call lh_check(lh_stats_reset(lhctx), 'lh_stats_reset')
     done = 0
     do while (done + nstat <= nsteps)
       call lh_check(lh_step(lhctx, int(nstat, c_int)), 'lh_step')
       call lh_check(lh_stats_accumulate(lhctx), 'lh_stats_accumulate')
+      if (len_trim(fhist) > 0) call hist_write(uh, flag_ctpress, 1)
       done = done + nstat
     end do
     if (done < nsteps) call lh_check(lh_step(lhctx, int(nsteps - done, c_int)), 'lh_step')
   else
     call lh_check(lh_step(lhctx, int(nsteps, c_int)), 'lh_step')
+    if (len_trim(fhist) > 0) call hist_write(uh, flag_ctpress, 1)
   end if
+  if (len_trim(fhist) > 0) close(uh)
   call lh_check(lh_sync(lhctx), 'lh_sync')
   call lh_check(lh_get_scalars(lhctx, 0_c_int, sc), 'lh_get_scalars')
   if (nstat > 0 .and. len_trim(fstats) > 0) then

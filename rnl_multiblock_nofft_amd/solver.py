@@ -105,6 +105,16 @@ class Solver:
         check(self.L.lh_get_scalars(self.h, ch, o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return dict(t=o[0], dt=o[1], mpgx=o[2], QxT=o[3], Qx=o[4], dtv=o[5], dtc=o[6], iter=int(o[7]))
 
+    def history(self, ch=0):
+        """The record_out history values (FOU3D.f90:3040-3070), reduced on
+        the device."""
+        o = np.zeros(12)
+        check(self.L.lh_history(self.h, ch, o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        keys = ("t", "dtv", "dtc", "dt", "err", "Qx", "QxT", "mpgx", "dgx", "Umax", "Uslip", "iter")
+        d = dict(zip(keys, o))
+        d["iter"] = int(d["iter"])
+        return d
+
     def set_scalars(self, ch, t=0.0, mpgx=None, QxT=None, iter=0):
         s = self.scalars(ch)
         check(self.L.lh_set_scalars(self.h, ch, float(t), float(s["mpgx"] if mpgx is None else mpgx),

@@ -221,7 +221,8 @@ def test_fortran_main_program_writes_stats(tmp_path):
     fst = str(tmp_path / "stats.dat")
     nml = str(tmp_path / "run.nml")
     d = {k: cfg[k] for k in CFG_KEYS}
-    d.update(mpgx=0.0, precision=0, nsteps=nsteps, fin=fin, fout=fout, nstat=nstat, fstats=fst)
+    fh = str(tmp_path / "hist.dat")
+    d.update(mpgx=0.0, precision=0, nsteps=nsteps, fin=fin, fout=fout, nstat=nstat, fstats=fst, fhist=fh)
     _nml(nml, "run", d)
     _run(os.path.join(BIN, "littleharsh_gpu"), nml)
     r = statsio.read_stats(fst)
@@ -236,3 +237,9 @@ def test_fortran_main_program_writes_stats(tmp_path):
     say(errs)
     for name, e in errs.items():
         assert e <= 1e-11, (name, e)
+    # history file: a record at the start and after every nstat steps
+    h = fieldio.read_hist(fh)
+    assert [r["iter"] for r in h["records"]] == [0, 2, 4]
+    assert all(r["flag"] == 0 for r in h["records"])
+    last = h["records"][-1]
+    assert last["Umax"] > 0.5 and last["err"] < 1e-12 and last["dt"] == cfg["dt"]
