@@ -288,6 +288,15 @@ int lh_stats_get(lh_ctx *ctx, int ch, double *prof, int *istat);
 #define LH_SP_P 4
 #define LH_SP_N 5
 int lh_spectrum_get(lh_ctx *ctx, int ch, int which, double *sp);
+/* The wall maps of sl_stats (sl_stats2.f90:33-161), accumulated by
+ * lh_stats_accumulate with the other statistics: for wall 0 (bottom, v row
+ * 0) or 1 (top, v row ny-1), maps[4][(N1+2)*N2] = u1 and u3 interpolated to
+ * the wall face and their wall-normal derivatives (the top one with the sign
+ * flipped), as sums of spectral coefficients in the (N1+2) x N2 plane layout
+ * of modes_to_planes_phys_lims_2 (bslip_u1_M, bslip_du1dy_M, bslip_u3_M,
+ * bslip_du3dy_M).  write_sl_stats adds the two walls (stats.f90 of several
+ * ranks: rank 0 the bottom, the last rank the top). */
+int lh_slip_stats_get(lh_ctx *ctx, int ch, int wall, double *maps);
 
 #ifdef __cplusplus
 }

@@ -1557,3 +1557,33 @@ void lho_stats(lho_state *s, double *prof, double *sp) {
   free(W);
   free(wc);
 }
+
+/* sl_stats (sl_stats2.f90:33-161): one sample of the wall maps, in the
+ * (N1+2) x N2 plane layout of modes_to_planes_phys_lims_2 (spectral
+ * coefficients, re/im interleaved): u1 and u3 interpolated to the wall
+ * faces (interp_u) and their wall-normal derivatives, the top wall's with
+ * the sign flipped.  maps[w][q][(N1+2)*N2], w = 0 bottom (v row 0), 1 top
+ * (v row ny-1), q = u1, du1dy, u3, du3dy; added to what maps holds. */
+void lho_slstats(lho_state *s, double *maps) {
+  const int nyu = s->nyu, ig = s->c.N1 + 2;
+  const size_t msz = (size_t)ig * s->c.N2;
+  interp_all(s);
+  for (int w = 0; w < 2; w++) {
+    const int j = w == 0 ? 0 : s->nyv - 1;
+    const double sg = w == 0 ? 1.0 : -1.0;
+    for (int c = 0; c < s->ncol; c++) {
+      int i = c % s->nxh, k = c / s->nxh;
+      const cplx *u1 = s->u1 + (size_t)c * nyu, *u3 = s->u3 + (size_t)c * nyu;
+      cplx v[4];
+      v[0] = s->u1i[(size_t)c * s->nyv + j];
+      v[1] = sg * ((u1[j + 1] - u1[j]) * s->dthdyv[j] * s->ddthetavi);
+      v[2] = s->u3i[(size_t)c * s->nyv + j];
+      v[3] = sg * ((u3[j + 1] - u3[j]) * s->dthdyv[j] * s->ddthetavi);
+      for (int q = 0; q < 4; q++) {
+        double *m = maps + (size_t)(w * 4 + q) * msz + (size_t)k * ig + 2 * i;
+        m[0] = m[0] + creal(v[q]);
+        m[1] = m[1] + cimag(v[q]);
+      }
+    }
+  }
+}

@@ -129,6 +129,12 @@ int lho_set_triads(lho_state *s, int nfile, const int *lens, const signed char *
  * the reference does. */
 void lho_stats(lho_state *s, double *prof, double *sp);
 
+/* sl_stats (sl_stats2.f90:33-161): adds one sample of the wall maps to
+ * maps[2][4][(N1+2)*N2] (bottom / top wall; u1, du1dy, u3, du3dy at the wall
+ * face, spectral coefficients in the (N1+2) x N2 plane layout, the top
+ * gradient with the sign flipped as there). */
+void lho_slstats(lho_state *s, double *maps);
+
 #ifdef __cplusplus
 }
 #endif

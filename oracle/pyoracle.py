@@ -87,6 +87,7 @@ def lib():
         L.lho_set_texture.restype = ctypes.c_int
         L.lho_lusol_uw_tex.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
         L.lho_stats.argtypes = [vp, dp, dp]
+        L.lho_slstats.argtypes = [vp, dp]
         L.lho_set_triads.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.lho_set_triads.restype = ctypes.c_int
@@ -295,3 +296,13 @@ class Oracle:
                                   mu.ctypes.data_as(ip), mv.ctypes.data_as(ip))
         if rc != 0:
             raise ValueError("triad list entry outside the band")
+
+    def slstats(self, maps=None):
+        """Add one sl_stats sample (sl_stats2.f90:33-161) to maps
+        (2, 4, N2, N1+2); a new zero array when None."""
+        N1, N2 = self.cfg["N1"], self.cfg["N2"]
+        if maps is None:
+            maps = np.zeros((2, 4, N2, N1 + 2))
+        assert maps.shape == (2, 4, N2, N1 + 2) and maps.flags.c_contiguous
+        lib().lho_slstats(self.h, _dp(maps))
+        return maps

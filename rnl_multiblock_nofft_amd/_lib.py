@@ -30,7 +30,7 @@ EXPORTS = ["lh_last_error", "lh_ctx_create", "lh_ctx_destroy", "lh_set_grid", "l
            "lh_set_tuning", "lh_get_tuning", "lh_autotune", "lh_set_texture",
            "lh_stats_accumulate", "lh_stats_reset", "lh_stats_get", "lh_spectrum_get",
            "lh_lusol_p", "lh_lusol0", "lh_four_to_phys_N_planes", "lh_phys_to_four_N_planes",
-           "lh_modes_to_planes", "lh_planes_to_modes", "lh_set_triads", "lh_history"]
+           "lh_modes_to_planes", "lh_planes_to_modes", "lh_set_triads", "lh_history", "lh_slip_stats_get"]
 ST_N = 13
 SP_IDS = {"U": 0, "V": 1, "W": 2, "UV": 3, "P": 4}
 
@@ -95,6 +95,7 @@ def load():
     L.lh_stats_get.argtypes = [vp, ip, dp, ctypes.POINTER(ip)]
     L.lh_spectrum_get.argtypes = [vp, ip, ip, dp]
     L.lh_history.argtypes = [vp, ip, dp]
+    L.lh_slip_stats_get.argtypes = [vp, ip, ip, dp]
     L.lh_set_triads.argtypes = [vp, ip, ip, ctypes.POINTER(ip), vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
     L.lh_lusol_p.argtypes = [vp, ip, vp]
     L.lh_lusol0.argtypes = [vp, ip, ip, vp]

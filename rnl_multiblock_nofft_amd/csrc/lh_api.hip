@@ -466,8 +466,9 @@ int solve_phase(lh_ctx *x, int kRK) {
 // inverse rows -- for the whole step
 // the nonlinear products of every channel: the FFT stages for channels
 // without triad lists, the list kernels for those with
-int nonlinear_stage(lh_ctx *x) {
-  if (x->ntri < (int)x->ch.size()) HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream));
+int nonlinear_stage(lh_ctx *x, int kRK) {
+  if (x->ntri < (int)x->ch.size())
+    HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream, kRK));
   if (x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
   return LH_OK;
 }
@@ -484,7 +485,7 @@ int time_step(lh_ctx *x, int kRK) {
 }
 
 int run_substep(lh_ctx *x, int kRK) {
-  int rc = nonlinear_stage(x);
+  int rc = nonlinear_stage(x, kRK);
   if (rc) return rc;
   rc = time_step(x, kRK);
   if (rc) return rc;
@@ -580,6 +581,8 @@ int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
       so += al<double>((size_t)ST_N * h.nyu);
       h.dev.o_spec = so;
       so += al<double>((size_t)SP_N * h.nyu * h.ldc);
+      h.dev.o_slip = so;
+      so += al<double2>((size_t)8 * h.ncol);
     }
     x->stats_bytes = so;
   }
@@ -776,7 +779,7 @@ int lh_run_phase(lh_ctx *x, int phase, int kRK) {
   if (kRK < 1 || kRK > 3) return set_err(LH_EINVAL, "kRK must be 1..3");
   switch (phase) {
     case LH_PHASE_NONLINEAR: {
-      int r = nonlinear_stage(x);
+      int r = nonlinear_stage(x, kRK);
       if (r) return r;
       r = time_step(x, kRK);
       if (r) return r;
@@ -843,7 +846,7 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
     HIPCHK(hipEventRecord(ev[0], x->stream));
     for (int st = 0; st < 3; st++) {
       if (x->ntri < (int)x->ch.size())
-        HIPCHK(launch_nl_stage(x->arena, x->d_chans, x->shape, x->mixed, st, x->stream));
+        HIPCHK(launch_nl_stage(x->arena, x->d_chans, x->shape, x->mixed, st, x->stream, kRK));
       // the triad-list kernels are counted with the x-product stage
       if (st == 1 && x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
       HIPCHK(hipEventRecord(ev[st + 1], x->stream));
@@ -929,6 +932,31 @@ int lh_stats_get(lh_ctx *x, int ch, double *prof, int *istat) {
     HIPCHK(hipStreamSynchronize(x->stream));
   }
   if (istat) *istat = x->istat;
+  return LH_OK;
+}
+
+int lh_slip_stats_get(lh_ctx *x, int ch, int wall, double *maps) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (ch < 0 || ch >= (int)x->ch.size() || wall < 0 || wall > 1 || !maps)
+    return set_err(LH_EINVAL, "lh_slip_stats_get: bad arguments");
+  const HostChan &h = x->ch[ch];
+  const int N1 = h.cfg.N1, N2 = h.cfg.N2, ig = N1 + 2;
+  const size_t msz = (size_t)ig * N2;
+  std::fill(maps, maps + 4 * msz, 0.0);
+  if (!x->stats) return LH_OK;
+  std::vector<double2> acc((size_t)4 * h.ncol);
+  HIPCHK(hipMemcpyAsync(acc.data(), x->stats + h.dev.o_slip + (size_t)wall * 4 * h.ncol * sizeof(double2),
+                        acc.size() * sizeof(double2), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  // plane layout of modes_to_planes_phys_lims_2: mode (i, k) at (2i+1, k+1)
+  for (int q = 0; q < 4; q++)
+    for (int c = 0; c < h.ncol; c++) {
+      const int i = c % h.nxh, k = c / h.nxh;
+      double *m = maps + q * msz + (size_t)k * ig + 2 * i;
+      m[0] = acc[(size_t)q * h.ncol + c].x;
+      m[1] = acc[(size_t)q * h.ncol + c].y;
+    }
   return LH_OK;
 }
 

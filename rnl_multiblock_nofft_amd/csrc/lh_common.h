@@ -83,9 +83,10 @@ struct Chan {
   size_t o_twzN, o_twxN;  // N2- and N1-point twiddles of the N-grid
   size_t o_cfl;           // [slot][tile][2] CFL maxima of k_xprod's workgroups
   // statistics accumulators, byte offsets into the context's separate
-  // statistics buffer (allocated on first use): prof[ST_*][nyu] row profiles
-  // and spec[SP_*][nyu][ldc] per-mode spectra (doubles)
-  size_t o_stat, o_spec;
+  // statistics buffer (allocated on first use): prof[ST_*][nyu] row profiles,
+  // spec[SP_*][nyu][ldc] per-mode spectra (doubles) and the wall maps of
+  // sl_stats, slip[wall][4][ncol] (double2: u1, du1dy, u3, du3dy)
+  size_t o_stat, o_spec, o_slip;
   // triad-list nonlinear path (tri = 1): file of each plane slot (-1: none),
   // per (file, type) the target-sorted rows (tri_off[(file*9 + type-1) *
   // (ncol+1) + c] .. +1 into tri_ent + tri_base[file*9 + type-1]; an entry

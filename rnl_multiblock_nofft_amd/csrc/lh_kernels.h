@@ -26,8 +26,11 @@ int solve_cols_default(int nyu);
 __host__ __device__ size_t solve_lds_bytes(int nyu, int C);
 
 void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny);
-hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s);
-hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s);
+// kRK = 1 also records the CFL maxima of dtc_calc in the x-product stage
+hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s,
+                           int kRK = 1);
+hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s,
+                            int kRK = 1);
 hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
 hipError_t launch_factors(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 hipError_t launch_dt_update(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);

@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
 
 // ------------------------------------------------------------------------
 // K2: inverse x transform, products, forward x transform
-template <typename C>
+template <typename C, bool CFL>
 __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, int E) {
   typedef typename CT<C>::R R_t;
   const Chan ch = chans[blockIdx.z];
@@ -186,18 +186,20 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   }
   __syncthreads();
   C *X = fft_lds<C>(A, B, Ng1, 2 * nz, Ng1, +1, TW);
-  // CFL maxima of dtc_calc (FOU3D.f90:699-736): |u1|, |u3| on u-planes,
-  // |u2| on v-planes (times dthdyv(j) below)
+  // CFL maxima of dtc_calc (FOU3D.f90:699-736), at the first substep only:
+  // |u1|, |u3| on u-planes, |u2| on v-planes (times dthdyv(j) in k_dt_update)
   R_t ma = 0, mb = 0;
   for (int t = threadIdx.x; t < nz * Ng1; t += blockDim.x) {
     int zz = t / Ng1, n = t - zz * Ng1;
     C s0 = X[(2 * zz) * Ng1 + n], s1 = X[(2 * zz + 1) * Ng1 + n];
     R_t a1 = s0.x, a3 = s0.y, a2 = s1.x;
-    if (isU) {
-      ma = fmax(ma, fabs(a1));
-      mb = fmax(mb, fabs(a3));
-    } else {
-      ma = fmax(ma, fabs(a2));
+    if (CFL) {
+      if (isU) {
+        ma = fmax(ma, fabs(a1));
+        mb = fmax(mb, fabs(a3));
+      } else {
+        ma = fmax(ma, fabs(a2));
+      }
     }
     if (isU) {
       X[(2 * zz) * Ng1 + n] = CT<C>::mk(a1 * a1, a1 * a3);      // uu + i uw
@@ -211,7 +213,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   // one (max a, max b) pair per workgroup, reduced by k_dt_update: wave
   // maxima here, combined after the forward transform's barriers
   __shared__ double red[2][4];
-  {
+  if (CFL) {
     double a = (double)ma, b = (double)mb;
     for (int off = 32; off >= 1; off >>= 1) {
       a = fmax(a, __shfl_xor(a, off));
@@ -223,7 +225,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
     }
   }
   C *Y = fft_lds<C>(X, X == A ? B : A, Ng1, 2 * nz, Ng1, -1, TW);
-  if (threadIdx.x == 0) {
+  if (CFL && threadIdx.x == 0) {
     const int nw = (blockDim.x + 63) >> 6;
     double a = red[0][0], b = red[1][0];
     for (int q = 1; q < nw; q++) {
@@ -1372,7 +1374,8 @@ void shape_tiles(ShapeMax &m) {
   }
 }
 
-hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s) {
+hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s,
+                           int kRK) {
   size_t es = mixed ? sizeof(float2) : sizeof(double2);
   dim3 b(256);
   if (stage == 0) {
@@ -1383,10 +1386,15 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
       hipLaunchKernelGGL(k_zinv<double2>, g, b, m.lds_zinv * es, s, arena, d_chans, m.e_zinv);
   } else if (stage == 1) {
     dim3 g(m.xprod_tiles, m.nslot, m.nchan);
-    if (mixed)
-      hipLaunchKernelGGL(k_xprod<float2>, g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
+    const bool cfl = kRK == 1;
+    if (mixed && cfl)
+      hipLaunchKernelGGL((k_xprod<float2, true>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
+    else if (mixed)
+      hipLaunchKernelGGL((k_xprod<float2, false>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
+    else if (cfl)
+      hipLaunchKernelGGL((k_xprod<double2, true>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
     else
-      hipLaunchKernelGGL(k_xprod<double2>, g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
+      hipLaunchKernelGGL((k_xprod<double2, false>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
   } else {
     dim3 g(m.zfwd_tiles, m.nslot, m.nchan);
     if (mixed)
@@ -1397,9 +1405,9 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
   return hipGetLastError();
 }
 
-hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s) {
+hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s, int kRK) {
   for (int st = 0; st < 3; st++) {
-    hipError_t e = launch_nl_stage(arena, d_chans, m, mixed, st, s);
+    hipError_t e = launch_nl_stage(arena, d_chans, m, mixed, st, s, kRK);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -233,8 +233,36 @@ __global__ void __launch_bounds__(256) k_stats(char *arena, const Chan *chans, c
 
 }  // namespace
 
+// sl_stats (sl_stats2.f90:33-161): u1, u3 interpolated to the wall faces
+// (v rows 0 and ny-1, interp_u) and their wall-normal derivatives, the top
+// one with the sign flipped, summed per mode; one thread per column
+__global__ void __launch_bounds__(256) k_slstats(char *arena, const Chan *chans, char *sbase) {
+  const Chan ch = chans[blockIdx.y];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ch.ncol) return;
+  const View v(arena, ch);
+  const size_t L = ch.ldc;
+  double2 *sl = (double2 *)(sbase + ch.o_slip);
+  for (int w = 0; w < 2; w++) {
+    const int j = w == 0 ? 0 : ch.nyv - 1;
+    const double sg = w == 0 ? 1.0 : -1.0;
+    const double w1 = v.yv[j] - v.yu[j], w2 = v.yu[j + 1] - v.yv[j], d = v.yu[j + 1] - v.yu[j];
+    const double dy = v.dthdyv[j];
+    for (int f = 0; f < 2; f++) {
+      const double2 *u = f == 0 ? v.u1 : v.u3;
+      const double2 a = u[(size_t)(j + 1) * L + c], b = u[(size_t)j * L + c];
+      const double2 ui = make_double2((w1 * a.x + w2 * b.x) / d, (w1 * a.y + w2 * b.y) / d);
+      const double2 du = make_double2(sg * ((a.x - b.x) * dy), sg * ((a.y - b.y) * dy));
+      double2 *o = sl + ((size_t)(w * 4 + 2 * f) * ch.ncol + c);
+      o[0] = cadd(o[0], ui);
+      o[ch.ncol] = cadd(o[ch.ncol], du);
+    }
+  }
+}
+
 hipError_t launch_stats(char *arena, const Chan *d_chans, const ShapeMax &m, char *sbase, hipStream_t s) {
   hipLaunchKernelGGL(k_stats, dim3(m.nyu, m.nchan), dim3(256), 0, s, arena, d_chans, sbase);
+  hipLaunchKernelGGL(k_slstats, dim3((m.ncol + 255) / 256, m.nchan), dim3(256), 0, s, arena, d_chans, sbase);
   return hipGetLastError();
 }
 

@@ -211,6 +211,13 @@ module lhgpu
       integer(c_int), value :: ch
       integer(c_int) :: rc
     end function
+    function lh_slip_stats_get(ctx, ch, wall, maps) bind(C, name='lh_slip_stats_get') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch, wall
+      real(c_double), intent(out) :: maps(*)
+      integer(c_int) :: rc
+    end function
     function lh_history(ctx, ch, out12) bind(C, name='lh_history') result(rc)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

@@ -215,6 +215,16 @@ class Solver:
                                   ctypes.byref(n)))
         return prof, n.value
 
+    def slip_stats(self, ch=0):
+        """sl_stats wall maps, (2 walls, 4 quantities, N2, N1+2): u1, du1dy,
+        u3, du3dy at the bottom and top wall faces (plane layout of spectral
+        coefficients)."""
+        c = self.cfgs[ch]
+        out = np.zeros((2, 4, c["N2"], c["N1"] + 2))
+        for w in range(2):
+            check(self.L.lh_slip_stats_get(self.h, ch, w, out[w].ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
     def spectrum(self, ch, which):
         """Accumulated spectrum `which` in U, V, W, UV, P as (ncol, rows),
         the reference's sp(jlo:jhi, ncol) layout."""

@@ -137,3 +137,31 @@ def read_spect(path, N1, N2):
     a = np.frombuffer(recs[5], dtype="<f8")
     out["sp"] = a.reshape(-1, N2 // 2 + 1, N1 // 2 + 1).copy()
     return out
+
+
+def write_sl_stats(path, maps, istat, hdr):
+    """write_sl_stats (sl_stats2.f90:163-217): header (t, Re, alp, bet, mpgx,
+    nband, iter, 88 zeros), N(4, 0:nband+1), istat_sl, then the four maps
+    bslip_u1_M, bslip_du1dy_M, bslip_u3_M, bslip_du3dy_M, each (N1+2) x N2 in
+    Fortran order, the two walls added as the reference's writer adds its
+    first and last ranks.  maps: (2, 4, N2, N1+2) (Solver.slip_stats)."""
+    m = np.asarray(maps, dtype=np.float64)
+    with open(path, "wb") as f:
+        _rec(f, struct.pack("<5d", hdr["t"], hdr["Re"], hdr["alp"], hdr["bet"], hdr["mpgx"])
+             + struct.pack("<2i", int(hdr["nband"]), int(hdr["iter"])) + bytes(4 * 88))
+        _rec(f, np.asarray(hdr["N"], dtype="<i4").tobytes(order="F"))
+        _rec(f, struct.pack("<i", int(istat)))
+        for q in range(4):
+            _rec(f, (m[0, q] + m[1, q]).astype("<f8").tobytes())
+
+
+def read_sl_stats(path):
+    with open(path, "rb") as f:
+        recs = list(_records(f.read()))
+    t, Re, alp, bet, mpgx = struct.unpack_from("<5d", recs[0], 0)
+    nband, it = struct.unpack_from("<2i", recs[0], 40)
+    N = np.frombuffer(recs[1], dtype="<i4").reshape(nband + 2, 4).T.copy()
+    (istat,) = struct.unpack_from("<i", recs[2], 0)
+    nz, ig = int(N[1, 1]), int(N[0, 1]) + 2
+    maps = [np.frombuffer(r, dtype="<f8").reshape(nz, ig).copy() for r in recs[3:7]]
+    return dict(t=t, Re=Re, alp=alp, bet=bet, mpgx=mpgx, nband=nband, iter=it, N=N, istat=istat, maps=maps)

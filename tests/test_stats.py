@@ -346,3 +346,96 @@ def test_gpu_stats_reset():
     s.stats_accumulate()
     prof4, n4 = s.stats(0)
     assert n4 == 1 and np.allclose(prof4, prof2 / 2, rtol=1e-15, atol=0)
+
+
+# ---------------------------------------------------------------- sl_stats
+
+
+def test_oracle_slstats_closed_forms():
+    """Wall maps of sl_stats (sl_stats2.f90:33-161) for a profile linear
+    near the walls: the interpolated wall velocity is the linear
+    extrapolation, du/dy the slope; a profile symmetric about the centre
+    gives equal bottom and (sign-flipped) top gradients."""
+    cfg = small_cfg(N=8, ny=13)
+    o = oracle_for(cfg)
+    g = o.grid()
+    yu = g["yu"]
+    ny, ncol = cfg["ny"], o.ncol
+    u1 = np.zeros((ncol, ny + 1), complex)
+    u1[0] = 1.0 - yu ** 2                      # symmetric about y = 0
+    u3 = np.zeros((ncol, ny + 1), complex)
+    u3[0] = 0.5 + 2.0 * yu                     # linear
+    o.set("u1", u1)
+    o.set("u3", u3)
+    o.set("u2", np.zeros((ncol, ny), complex))
+    o.set("p", np.zeros((ncol, ny - 1), complex))
+    m = o.slstats()
+    yv = g["yv"]
+    # linear u3: value at the wall faces and slope (bottom +2, top -2 after the flip)
+    assert m[0, 2, 0, 0] == pytest.approx(0.5 + 2.0 * yv[0], rel=1e-13)
+    assert m[1, 2, 0, 0] == pytest.approx(0.5 + 2.0 * yv[ny - 1], rel=1e-13)
+    # the staggered difference across the wall face, d/dtheta times dtheta/dy
+    dv = g["dthdyv"]
+    assert m[0, 3, 0, 0] == pytest.approx(2.0 * (yu[1] - yu[0]) * dv[0], rel=1e-13)
+    assert m[1, 3, 0, 0] == pytest.approx(-2.0 * (yu[ny] - yu[ny - 1]) * dv[ny - 1], rel=1e-13)
+    assert m[0, 3, 0, 0] == pytest.approx(2.0, rel=0.05)        # the slope, to O(dy^2)
+    # symmetric u1: equal gradients at both walls after the flip, equal values
+    assert m[0, 1, 0, 0] == pytest.approx(m[1, 1, 0, 0], rel=1e-12)
+    assert m[0, 0, 0, 0] == pytest.approx(m[1, 0, 0, 0], rel=1e-12)
+    # only the mean mode is set; the imaginary slot and other modes stay 0
+    assert np.count_nonzero(m[:, :, :, 1:]) == 0
+
+
+def test_sl_stats_file_layout(tmp_path):
+    import os
+    cfg = small_cfg(N=8, ny=13)
+    o = oracle_for(cfg)
+    rand_fields(o, np.random.default_rng(2))
+    m = o.slstats()
+    N = np.array([[0, 8, 8, 8, -2], [0, 8, 8, 8, 0], [0, 10, 10, 10, 0], [0, 11, 11, 11, 0]], np.int32)
+    hdr = dict(t=0.5, Re=cfg["Re"], alp=1.0, bet=2.0, mpgx=0.0, nband=3, iter=7, N=N)
+    p = str(tmp_path / "sl.dat")
+    statsio.write_sl_stats(p, m, 1, hdr)
+    r = statsio.read_sl_stats(p)
+    assert r["istat"] == 1 and r["iter"] == 7 and np.array_equal(r["N"], N)
+    for q in range(4):
+        assert np.array_equal(r["maps"][q], m[0, q] + m[1, q])
+    ref = "/root/reference/output/sl_stats_0032x0032x0153_00052.dat"
+    if os.path.exists(ref):
+        rr = statsio.read_sl_stats(ref)
+        assert [mm.shape for mm in rr["maps"]] == [(32, 34)] * 4 and rr["istat"] == 1
+        assert os.path.getsize(ref) == 8 + 400 + 8 + 80 + 8 + 4 + 4 * (8 + 8 * 34 * 32)
+
+
+@pytest.mark.gpu
+def test_gpu_slip_stats_match_oracle():
+    from rnl_multiblock_nofft_amd import Solver
+    cfg = small_cfg(N=16, ny=21)
+    s = Solver([cfg])
+    o, m = oracle_for(cfg), oracle_for(cfg)
+    f = init_fields(cfg, s.grid(0), seed=13)
+    for k, v in f.items():
+        s.upload(0, k, v)
+        o.set(k, v)
+    s.init_flowrate()
+    o.init_flowrate()
+    s.stats_reset()
+    ref = mir = None
+    for _ in range(2):
+        s.step(1)
+        o.step(1)
+        ref = o.slstats(ref)
+        for name in ("u1", "u2", "u3", "p"):
+            m.set(name, s.download(0, name))
+        mir = m.slstats(mir)
+        s.stats_accumulate()
+    got = s.slip_stats(0)
+    for what, r, tol in (("same fields", mir, 1e-14), ("independent run", ref, 1e-11)):
+        for w in range(2):
+            # the wall velocity of a no-slip wall is round-off: every map is
+            # measured against the largest map of its wall (the gradients)
+            sc = max(np.abs(r[w]).max(), 1e-300)
+            for q in range(4):
+                e = np.abs(got[w, q] - r[w, q]).max() / sc
+                print(what, w, q, "%.1e" % e, flush=True)
+                assert e < tol, (what, w, q, e)
