@@ -298,6 +298,94 @@ int lh_spectrum_get(lh_ctx *ctx, int ch, int which, double *sp);
  * ranks: rank 0 the bottom, the last rank the top). */
 int lh_slip_stats_get(lh_ctx *ctx, int ch, int wall, double *maps);
 
+/* Learned closure of the nonlinear term.  No counterpart in the reference:
+ * a coarse channel adds a network's estimate of what its grid misses.
+ *
+ * The network maps the 3x3x5 stencil of u1, u2, u3 around a point of the
+ * channel's N1 x N2 physical grid (four_to_phys_N conventions) to a
+ * correction of Nu1, Nu2, Nu3 at that point:
+ *   135 inputs -> 128 -> 128 -> 128 -> 3, GELU (tanh form) after the hidden
+ *   layers, bf16 weights and activations, fp32 accumulation.
+ * Input f = field*45 + (dy+2)*9 + (dz+1)*3 + (dx+1): field 0..2 = u1, u2, u3
+ * (row j of each field's own grid), dy = -2..2 rows (clamped to the field's
+ * rows), dz, dx = -1..1 points (periodic), normalised as
+ * (u - mean[field]) * inv_scale[field].  Output o times out_scale[o] is the
+ * correction of Nu(o+1) at the point, applied on u rows 1..ny-1 (Nu1, Nu3)
+ * and v rows 1..ny-2 (Nu2): the correction planes are transformed to modes
+ * (phys_to_four_N conventions) and added to the nonlinear term of every mode
+ * except the x-Nyquist (i = N1/2) and z-Nyquist (k = N2/2) ones.  Points are
+ * the N1*N2*(ny-1) nodes of u rows 1..ny-1, index (j-1)*N1*N2 + z*N1 + x.
+ *
+ * params: LH_CL_NPARAM floats, row-major y = W x + b:
+ *   W1[128][135] b1[128] W2[128][128] b2[128] W3[128][128] b3[128]
+ *   W4[3][128] b4[3]
+ * norm9: mean[3], inv_scale[3], out_scale[3].
+ * The closure runs on channels of at most 64 x 64 modes. */
+#define LH_CL_NIN 135
+#define LH_CL_NHID 128
+#define LH_CL_NOUT 3
+#define LH_CL_NPARAM 50819
+int lh_closure_set(lh_ctx *ctx, const float *params, const float *norm9);
+/* on = 1: every substep of channel ch adds the correction (after the
+ * products, before the right-hand side).  Drops the captured step graph. */
+int lh_closure_enable(lh_ctx *ctx, int ch, int on);
+/* On the current fields of channel ch: d_phys (device, floats
+ * [3][ny+1][N2][N1]) gets u1, u2, u3 on the N1 x N2 grid (u2 rows 0..ny-1,
+ * row ny zero); d_corr (device, same shape, or NULL) the network's
+ * correction (rows outside the applied ranges zero). */
+int lh_closure_eval(lh_ctx *ctx, int ch, float *d_phys, float *d_corr);
+/* A training sample from channel chf of a fine context for channel chc of a
+ * coarse one (same box, same y grid, at least the coarse modes): the fine
+ * fields, box-filtered over a coarse cell (transfer function
+ * sinc(pi i/N1c) sinc(pi m/N2c) for x-mode i, z wavenumber m; the coarse
+ * z-Nyquist mode cleared) and restricted to the coarse modes, REPLACE the
+ * coarse channel's u1, u2, u3, p.  Then, in the coarse layout,
+ *   tau = filtered full nonlinear term of the fine channel
+ *       - full nonlinear term of the coarse channel,
+ * with Nyquist modes cleared, as the network's target.  d_phys receives the
+ * coarse fields and d_tau the target, both as lh_closure_eval's d_phys
+ * (device floats [3][ny+1][N2c][N1c]). */
+int lh_closure_sample(lh_ctx *fine, int ch_fine, lh_ctx *coarse, int ch_coarse, float *d_phys, float *d_tau);
+/* Average time (ms) of the closure's three parts over nrep launches on the
+ * current fields: physical planes, network, correction to modes. */
+int lh_closure_time(lh_ctx *ctx, int nrep, double *ms3);
+
+/* Training of the closure network on one device (no counterpart in the
+ * reference).  A trainer holds fp32 master parameters, AdamW moments and the
+ * activations of a batch of `batch` points (a multiple of 64).  Forward and
+ * backward passes are bf16 matrix-core GEMMs with fp32 accumulation; the
+ * loss is the mean over points and outputs of (y - t)^2, t the normalised
+ * target (tau / out_scale). */
+typedef struct lh_mlp lh_mlp;
+int lh_mlp_create(int batch, lh_mlp **out);
+int lh_mlp_destroy(lh_mlp *m);
+/* parameters as lh_closure_set; resets the AdamW state */
+int lh_mlp_set_params(lh_mlp *m, const float *params, const float *norm9);
+int lh_mlp_get_params(lh_mlp *m, float *params);
+/* the batch from host arrays: x[batch][135] normalised inputs,
+ * t[batch][3] normalised targets */
+int lh_mlp_set_batch(lh_mlp *m, const float *x, const float *t);
+/* a random batch from nsnap snapshots of lh_closure_sample (d_phys, d_tau:
+ * nsnap consecutive [3][ny+1][N2][N1] blocks): sample n is point
+ *   h mod (nsnap * (ny-1) * N1 * N2),
+ *   h = splitmix64(seed ^ (n * 0xD1B54A32D192ED03)),
+ * counted snapshot-major, then as lh_closure_eval's point index */
+int lh_mlp_sample(lh_mlp *m, int N1, int N2, int ny, const float *d_phys, const float *d_tau, int nsnap,
+                  uint64_t seed);
+/* inputs at given points (host pts[batch]) of one physical-plane set, with
+ * zero targets */
+int lh_mlp_set_points(lh_mlp *m, int N1, int N2, int ny, const float *d_phys, const int *pts);
+/* outputs of the current batch, y[batch][3] (normalised) */
+int lh_mlp_forward(lh_mlp *m, float *y);
+/* forward, loss and backward on the current batch; the gradient stays on
+ * the device (lh_mlp_get_grad copies it out, LH_CL_NPARAM floats) */
+int lh_mlp_grad(lh_mlp *m, double *loss);
+int lh_mlp_get_grad(lh_mlp *m, float *grad);
+/* one AdamW step with the last gradient (bias-corrected moments, decoupled
+ * weight decay wd) */
+int lh_mlp_adam(lh_mlp *m, float lr, float beta1, float beta2, float eps, float wd);
+int lh_mlp_sync(lh_mlp *m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,7 +30,12 @@ EXPORTS = ["lh_last_error", "lh_ctx_create", "lh_ctx_destroy", "lh_set_grid", "l
            "lh_set_tuning", "lh_get_tuning", "lh_autotune", "lh_set_texture",
            "lh_stats_accumulate", "lh_stats_reset", "lh_stats_get", "lh_spectrum_get",
            "lh_lusol_p", "lh_lusol0", "lh_four_to_phys_N_planes", "lh_phys_to_four_N_planes",
-           "lh_modes_to_planes", "lh_planes_to_modes", "lh_set_triads", "lh_history", "lh_slip_stats_get"]
+           "lh_modes_to_planes", "lh_planes_to_modes", "lh_set_triads", "lh_history", "lh_slip_stats_get",
+           "lh_closure_set", "lh_closure_enable", "lh_closure_eval", "lh_closure_sample", "lh_closure_time",
+           "lh_mlp_create", "lh_mlp_destroy", "lh_mlp_set_params", "lh_mlp_get_params", "lh_mlp_set_batch",
+           "lh_mlp_sample", "lh_mlp_set_points", "lh_mlp_forward", "lh_mlp_grad", "lh_mlp_get_grad",
+           "lh_mlp_adam", "lh_mlp_sync"]
+CL_NIN, CL_NHID, CL_NOUT, CL_NPARAM = 135, 128, 3, 50819
 ST_N = 13
 SP_IDS = {"U": 0, "V": 1, "W": 2, "UV": 3, "P": 4}
 
@@ -103,6 +108,25 @@ def load():
     L.lh_phys_to_four_N_planes.argtypes = [ip, ip, vp, ip, vp]
     L.lh_modes_to_planes.argtypes = [ip, ip, ip, ip, vp, ip, vp, vp]
     L.lh_planes_to_modes.argtypes = [ip, ip, ip, ip, vp, ip, vp, vp]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.lh_closure_set.argtypes = [vp, fp, fp]
+    L.lh_closure_enable.argtypes = [vp, ip, ip]
+    L.lh_closure_eval.argtypes = [vp, ip, vp, vp]
+    L.lh_closure_sample.argtypes = [vp, ip, vp, ip, vp, vp]
+    L.lh_closure_time.argtypes = [vp, ip, dp]
+    L.lh_mlp_create.argtypes = [ip, ctypes.POINTER(vp)]
+    L.lh_mlp_destroy.argtypes = [vp]
+    L.lh_mlp_set_params.argtypes = [vp, fp, fp]
+    L.lh_mlp_get_params.argtypes = [vp, fp]
+    L.lh_mlp_set_batch.argtypes = [vp, fp, fp]
+    L.lh_mlp_sample.argtypes = [vp, ip, ip, ip, vp, vp, ip, ctypes.c_uint64]
+    L.lh_mlp_set_points.argtypes = [vp, ip, ip, ip, vp, ctypes.POINTER(ip)]
+    L.lh_mlp_forward.argtypes = [vp, fp]
+    L.lh_mlp_grad.argtypes = [vp, dp]
+    L.lh_mlp_get_grad.argtypes = [vp, fp]
+    L.lh_mlp_adam.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float]
+    L.lh_mlp_sync.argtypes = [vp]
     for name in EXPORTS:
         if name != "lh_last_error":
             getattr(L, name).restype = ctypes.c_int

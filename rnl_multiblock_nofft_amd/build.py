@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblhgpu.so")
-SOURCES = ["lh_kernels.hip", "lh_standalone.hip", "lh_texture.hip", "lh_stats.hip", "lh_triad.hip", "lh_api.hip"]
+SOURCES = ["lh_kernels.hip", "lh_standalone.hip", "lh_texture.hip", "lh_stats.hip", "lh_triad.hip", "lh_closure.hip", "lh_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/lhgpu.h"
+#include "lh_closure.h"
 #include "lh_common.h"
 #include "lh_kernels.h"
 
@@ -230,6 +231,14 @@ struct lh_ctx {
   int ntri = 0;                      // channels on the triad-list path
   size_t stats_bytes = 0;
   int istat = 0;                     // samples since the last reset
+  // learned closure: the packed network (CL_FRAG_BYTES of weight fragments,
+  // then CL_NBIAS biases), its normalisation, and per channel the physical
+  // planes, correction planes and training-target modes (allocated on use)
+  char *cl_net = nullptr;
+  ClNorm cl_norm{};
+  std::vector<char *> cl_mem;
+  std::vector<int> cl_on;
+  int ncl = 0;  // channels whose step adds the correction
 };
 
 namespace {
@@ -466,11 +475,58 @@ int solve_phase(lh_ctx *x, int kRK) {
 // inverse rows -- for the whole step
 // the nonlinear products of every channel: the FFT stages for channels
 // without triad lists, the list kernels for those with
-int nonlinear_stage(lh_ctx *x, int kRK) {
+int raw_nonlinear(lh_ctx *x, int kRK) {
   if (x->ntri < (int)x->ch.size())
     HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream, kRK));
   if (x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
   return LH_OK;
+}
+
+// per-channel closure buffers: phys and corr planes (float [3][nyu][N2][N1])
+// and training-target modes (double2 [3][nyu][ldc])
+size_t cl_plane_floats(const HostChan &h) { return (size_t)3 * h.nyu * h.cfg.N1 * h.cfg.N2; }
+int cl_buffers(lh_ctx *x, int c, float **phys, float **corr, double2 **tau) {
+  HostChan &h = x->ch[c];
+  const size_t pf = al<float>(cl_plane_floats(h));
+  if (!x->cl_mem[c]) {
+    const size_t bytes = 2 * pf + al<double2>((size_t)3 * h.nyu * h.ldc);
+    hipError_t e = hipMalloc((void **)&x->cl_mem[c], bytes);
+    if (e != hipSuccess) {
+      x->cl_mem[c] = nullptr;
+      return set_err(LH_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    HIPCHK(hipMemset(x->cl_mem[c], 0, bytes));
+  }
+  *phys = (float *)x->cl_mem[c];
+  *corr = (float *)(x->cl_mem[c] + pf);
+  *tau = (double2 *)(x->cl_mem[c] + 2 * pf);
+  return LH_OK;
+}
+
+// the network's correction of the nonlinear term of every channel that has
+// it switched on (after the products, before the right-hand side)
+int closure_stage(lh_ctx *x) {
+  for (size_t c = 0; c < x->ch.size(); c++) {
+    if (!x->cl_on[c]) continue;
+    HostChan &h = x->ch[c];
+    float *phys, *corr;
+    double2 *tau;
+    int rc = cl_buffers(x, (int)c, &phys, &corr, &tau);
+    if (rc) return rc;
+    View d(x->arena, h.dev);
+    ClSrc src{{d.u1, d.u2, d.u3}, {h.nyu, h.nyv, h.nyu}};
+    HIPCHK(launch_cl_phys(x->arena, x->d_chans + c, h.dev, src, phys, x->stream));
+    HIPCHK(launch_cl_infer(x->cl_net, (const float *)(x->cl_net + CL_FRAG_BYTES), x->cl_norm, phys, corr, h.cfg.N1,
+                           h.cfg.N2, h.nyu, h.nyv, x->stream));
+    HIPCHK(launch_cl_to_modes(x->arena, x->d_chans + c, h.dev, corr, x->stream));
+  }
+  return LH_OK;
+}
+
+int nonlinear_stage(lh_ctx *x, int kRK) {
+  int rc = raw_nonlinear(x, kRK);
+  if (rc || !x->ncl) return rc;
+  return closure_stage(x);
 }
 
 int time_step(lh_ctx *x, int kRK) {
@@ -633,6 +689,9 @@ int lh_ctx_destroy(lh_ctx *x) {
   if (x->hist) (void)hipFree(x->hist);
   for (char *p : x->tri_mem)
     if (p) (void)hipFree(p);
+  for (char *p : x->cl_mem)
+    if (p) (void)hipFree(p);
+  if (x->cl_net) (void)hipFree(x->cl_net);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return LH_OK;
@@ -847,10 +906,13 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
     for (int st = 0; st < 3; st++) {
       if (x->ntri < (int)x->ch.size())
         HIPCHK(launch_nl_stage(x->arena, x->d_chans, x->shape, x->mixed, st, x->stream, kRK));
-      // the triad-list kernels are counted with the x-product stage
+      // the triad-list kernels are counted with the x-product stage, the
+      // closure with the z-forward one
       if (st == 1 && x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
+      if (st == 2 && x->ncl && (rc = closure_stage(x))) break;
       HIPCHK(hipEventRecord(ev[st + 1], x->stream));
     }
+    if (rc) break;
     if ((rc = time_step(x, kRK))) break;
     HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
     HIPCHK(hipEventRecord(ev[4], x->stream));
@@ -1461,6 +1523,499 @@ int lh_set_texture(lh_ctx *x, int ch, const int *bottom, const int *top) {
   rc = push_descriptors(x);
   if (rc) return rc;
   drop_graph(x);
+  return LH_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// learned closure (no counterpart in the reference)
+
+static unsigned short host_bf16(float f) {
+  unsigned u;
+  memcpy(&u, &f, 4);
+  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+// compact parameter offsets: W1[128][135] b1 W2[128][128] b2 W3 b3 W4[3][128] b4[3]
+static const int CC_W1 = 0, CC_B1 = CC_W1 + CL_NH * CL_NIN, CC_W2 = CC_B1 + CL_NH, CC_B2 = CC_W2 + CL_NH * CL_NH,
+                 CC_W3 = CC_B2 + CL_NH, CC_B3 = CC_W3 + CL_NH * CL_NH, CC_W4 = CC_B3 + CL_NH,
+                 CC_B4 = CC_W4 + CL_NOUT * CL_NH, CC_N = CC_B4 + CL_NOUT;
+static_assert(CC_N == LH_CL_NPARAM, "parameter count");
+
+static ClNorm make_norm(const float *n9) {
+  ClNorm nm;
+  for (int f = 0; f < 3; f++) {
+    nm.mean[f] = n9[f];
+    nm.inv_scale[f] = n9[3 + f];
+    nm.out_scale[f] = n9[6 + f];
+  }
+  return nm;
+}
+static void cl_ensure(lh_ctx *x) {
+  if (x->cl_mem.size() != x->ch.size()) {
+    x->cl_mem.assign(x->ch.size(), nullptr);
+    x->cl_on.assign(x->ch.size(), 0);
+  }
+}
+static int cl_chan_ok(lh_ctx *x, int ch, const char *who) {
+  if (ch < 0 || ch >= (int)x->ch.size()) return set_err(LH_EINVAL, "%s: bad channel %d", who, ch);
+  const HostChan &h = x->ch[ch];
+  if (h.cfg.N1 * h.cfg.N2 > CL_MAXPLANE)
+    return set_err(LH_EINVAL, "%s: the closure runs on grids of at most %d modes (N1*N2 = %d)", who, CL_MAXPLANE,
+                   h.cfg.N1 * h.cfg.N2);
+  cl_ensure(x);
+  return LH_OK;
+}
+
+int lh_closure_set(lh_ctx *x, const float *params, const float *norm9) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (!params || !norm9) return set_err(LH_EINVAL, "lh_closure_set: null argument");
+  std::vector<unsigned short> fr((size_t)CLF_N * 64 * 8, 0);
+  std::vector<float> bias(CL_NBIAS, 0.f);
+  for (int lane = 0; lane < 64; lane++) {
+    const int q = lane & 15, g = lane >> 4;
+    for (int v = 0; v < 8; v++) {
+      const int cs = v < 4 ? 4 * g + v : 16 + 4 * g + (v - 4);
+      for (int b = 0; b < 8; b++)
+        for (int st = 0; st < 5; st++) {
+          const int row = 16 * b + q, col = 32 * st + 8 * g + v;
+          const float w = col < CL_NIN ? params[CC_W1 + row * CL_NIN + col] : 0.f;
+          fr[((size_t)(CLF_L1 + b * 5 + st) * 64 + lane) * 8 + v] = host_bf16(w);
+        }
+      for (int L = 0; L < 2; L++)
+        for (int b = 0; b < 8; b++)
+          for (int st = 0; st < 4; st++) {
+            const int row = 16 * b + q, col = 32 * st + cs;
+            const float w = params[(L == 0 ? CC_W2 : CC_W3) + row * CL_NH + col];
+            fr[((size_t)((L == 0 ? CLF_L2 : CLF_L3) + b * 4 + st) * 64 + lane) * 8 + v] = host_bf16(w);
+          }
+      for (int st = 0; st < 4; st++) {
+        const int col = 32 * st + cs;
+        const float w = q < CL_NOUT ? params[CC_W4 + q * CL_NH + col] : 0.f;
+        fr[((size_t)(CLF_L4 + st) * 64 + lane) * 8 + v] = host_bf16(w);
+      }
+    }
+  }
+  for (int r = 0; r < CL_NH; r++) {
+    bias[r] = params[CC_B1 + r];
+    bias[CL_NH + r] = params[CC_B2 + r];
+    bias[2 * CL_NH + r] = params[CC_B3 + r];
+  }
+  for (int o = 0; o < CL_NOUT; o++) bias[3 * CL_NH + o] = params[CC_B4 + o];
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (!x->cl_net) {
+    hipError_t e = hipMalloc((void **)&x->cl_net, CL_FRAG_BYTES + CL_NBIAS * sizeof(float));
+    if (e != hipSuccess) {
+      x->cl_net = nullptr;
+      return set_err(LH_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
+    }
+  }
+  HIPCHK(hipMemcpy(x->cl_net, fr.data(), CL_FRAG_BYTES, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(x->cl_net + CL_FRAG_BYTES, bias.data(), CL_NBIAS * sizeof(float), hipMemcpyHostToDevice));
+  x->cl_norm = make_norm(norm9);
+  drop_graph(x);  // the normalisation is a kernel argument
+  return LH_OK;
+}
+
+int lh_closure_enable(lh_ctx *x, int ch, int on) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if ((rc = cl_chan_ok(x, ch, "lh_closure_enable"))) return rc;
+  if (on && !x->cl_net) return set_err(LH_ESTATE, "lh_closure_enable: no network set (lh_closure_set)");
+  if (on) {
+    float *ph, *co;
+    double2 *ta;
+    if ((rc = cl_buffers(x, ch, &ph, &co, &ta))) return rc;
+  }
+  x->cl_on[ch] = on ? 1 : 0;
+  x->ncl = 0;
+  for (int c : x->cl_on) x->ncl += c;
+  drop_graph(x);
+  return LH_OK;
+}
+
+int lh_closure_eval(lh_ctx *x, int ch, float *d_phys, float *d_corr) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if ((rc = cl_chan_ok(x, ch, "lh_closure_eval"))) return rc;
+  if (!d_phys) return set_err(LH_EINVAL, "lh_closure_eval: null d_phys");
+  if (d_corr && !x->cl_net) return set_err(LH_ESTATE, "lh_closure_eval: no network set (lh_closure_set)");
+  HostChan &h = x->ch[ch];
+  View d(x->arena, h.dev);
+  ClSrc src{{d.u1, d.u2, d.u3}, {h.nyu, h.nyv, h.nyu}};
+  HIPCHK(launch_cl_phys(x->arena, x->d_chans + ch, h.dev, src, d_phys, x->stream));
+  if (d_corr) {
+    HIPCHK(hipMemsetAsync(d_corr, 0, cl_plane_floats(h) * sizeof(float), x->stream));
+    HIPCHK(launch_cl_infer(x->cl_net, (const float *)(x->cl_net + CL_FRAG_BYTES), x->cl_norm, d_phys, d_corr,
+                           h.cfg.N1, h.cfg.N2, h.nyu, h.nyv, x->stream));
+  }
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return LH_OK;
+}
+
+int lh_closure_sample(lh_ctx *f, int chf, lh_ctx *c, int chc, float *d_phys, float *d_tau) {
+  int rc = check_ctx(f);
+  if (rc) return rc;
+  if (!c || !d_phys || !d_tau) return set_err(LH_EINVAL, "lh_closure_sample: null argument");
+  if (c->device != f->device) return set_err(LH_EINVAL, "lh_closure_sample: contexts on different devices");
+  if (chf < 0 || chf >= (int)f->ch.size()) return set_err(LH_EINVAL, "lh_closure_sample: bad fine channel %d", chf);
+  if ((rc = cl_chan_ok(c, chc, "lh_closure_sample"))) return rc;
+  if (f->ntri || c->ntri) return set_err(LH_EINVAL, "lh_closure_sample: FFT-path channels only");
+  const HostChan &F = f->ch[chf];
+  HostChan &C = c->ch[chc];
+  if (F.cfg.ny != C.cfg.ny || F.cfg.N1 < C.cfg.N1 || F.cfg.N2 < C.cfg.N2 || F.cfg.alp != C.cfg.alp ||
+      F.cfg.bet != C.cfg.bet)
+    return set_err(LH_EINVAL, "lh_closure_sample: the fine channel must have the coarse one's box and ny and at least "
+                              "its modes");
+  for (int j = 0; j < F.nyu; j++)
+    if (F.yu[j] != C.yu[j]) return set_err(LH_EINVAL, "lh_closure_sample: y grids differ (row %d)", j);
+  float *ph, *co;
+  double2 *tau;
+  if ((rc = cl_buffers(c, chc, &ph, &co, &tau))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // fine: products, filtered full nonlinear term into tau, filtered fields
+  // into the coarse state
+  if ((rc = raw_nonlinear(f, 2))) return rc;
+  HIPCHK(launch_cl_tau(f->arena, f->d_chans + chf, c->arena, c->d_chans + chc, C.dev, tau, 0, f->stream));
+  HIPCHK(launch_cl_restrict_fields(f->arena, f->d_chans + chf, c->arena, c->d_chans + chc, C.dev, f->stream));
+  HIPCHK(hipStreamSynchronize(f->stream));
+  // coarse: its own nonlinear term of the filtered fields, subtracted
+  if ((rc = raw_nonlinear(c, 2))) return rc;
+  HIPCHK(launch_cl_tau(c->arena, c->d_chans + chc, c->arena, c->d_chans + chc, C.dev, tau, 1, c->stream));
+  View d(c->arena, C.dev);
+  const size_t pl = (size_t)C.ldc * C.nyu;
+  ClSrc su{{d.u1, d.u2, d.u3}, {C.nyu, C.nyv, C.nyu}};
+  ClSrc st{{tau, tau + pl, tau + 2 * pl}, {C.nyu, C.nyu, C.nyu}};
+  HIPCHK(launch_cl_phys(c->arena, c->d_chans + chc, C.dev, su, d_phys, c->stream));
+  HIPCHK(launch_cl_phys(c->arena, c->d_chans + chc, C.dev, st, d_tau, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return LH_OK;
+}
+
+int lh_closure_time(lh_ctx *x, int nrep, double *ms3) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (nrep < 1 || !ms3) return set_err(LH_EINVAL, "lh_closure_time: bad arguments");
+  if (!x->ncl) return set_err(LH_ESTATE, "lh_closure_time: no channel has the closure on");
+  hipEvent_t ev[4];
+  for (int q = 0; q < 4; q++) HIPCHK(hipEventCreate(&ev[q]));
+  for (int q = 0; q < 3; q++) ms3[q] = 0.0;
+  for (int n = 0; n < nrep && rc == LH_OK; n++) {
+    HIPCHK(hipEventRecord(ev[0], x->stream));
+    for (int part = 0; part < 3 && rc == LH_OK; part++) {
+      for (size_t c = 0; c < x->ch.size(); c++) {
+        if (!x->cl_on[c]) continue;
+        HostChan &h = x->ch[c];
+        float *phys, *corr;
+        double2 *tau;
+        if ((rc = cl_buffers(x, (int)c, &phys, &corr, &tau))) break;
+        View d(x->arena, h.dev);
+        ClSrc src{{d.u1, d.u2, d.u3}, {h.nyu, h.nyv, h.nyu}};
+        if (part == 0)
+          HIPCHK(launch_cl_phys(x->arena, x->d_chans + c, h.dev, src, phys, x->stream));
+        else if (part == 1)
+          HIPCHK(launch_cl_infer(x->cl_net, (const float *)(x->cl_net + CL_FRAG_BYTES), x->cl_norm, phys, corr,
+                                 h.cfg.N1, h.cfg.N2, h.nyu, h.nyv, x->stream));
+        else
+          HIPCHK(launch_cl_to_modes(x->arena, x->d_chans + c, h.dev, corr, x->stream));
+      }
+      HIPCHK(hipEventRecord(ev[part + 1], x->stream));
+    }
+    HIPCHK(hipEventSynchronize(ev[3]));
+    for (int q = 0; q < 3; q++) {
+      float t;
+      HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+      ms3[q] += t;
+    }
+  }
+  for (int q = 0; q < 4; q++) (void)hipEventDestroy(ev[q]);
+  for (int q = 0; q < 3; q++) ms3[q] /= nrep;
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// closure training: the network's forward and backward passes as bf16
+// matrix-core GEMMs (fp32 accumulation) over a batch of B points
+
+struct lh_mlp {
+  int B = 0, device = 0, step = 0;
+  hipStream_t stream = nullptr;
+  char *mem = nullptr;
+  float *P, *G, *Mo, *Vo;  // padded master parameters, gradient, AdamW moments
+  uint16_t *Wb;            // bf16 copies of the matrices
+  uint16_t *X, *H[3], *dY, *dZ;
+  float *T, *Z[3], *Y, *dH;
+  double *loss;
+  int *pts;
+  ClNorm norm{};
+};
+
+static int mlp_ok(lh_mlp *m) {
+  if (!m) return set_err(LH_EINVAL, "null trainer");
+  HIPCHK(hipSetDevice(m->device));
+  return LH_OK;
+}
+
+static int gemm(lh_mlp *m, const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk, long sbn, float *C,
+                long scm, int M, int N, int K, int kchunk, int accum) {
+  GemmP p{A, B, C, M, N, K, sam, sak, sbk, sbn, scm, 1, kchunk, accum};
+  HIPCHK(launch_gemm(p, m->stream));
+  return LH_OK;
+}
+
+static int mlp_forward(lh_mlp *m) {
+  const int B = m->B;
+  int rc;
+  const uint16_t *in = m->X;
+  int K = CL_KIN;
+  const int wo[3] = {CLB_W1, CLB_W2, CLB_W3}, bo[3] = {CLP_B1, CLP_B2, CLP_B3};
+  for (int L = 0; L < 3; L++) {
+    if ((rc = gemm(m, m->Wb + wo[L], K, 1, in, B, 1, m->Z[L], B, CL_NH, B, K, K, 0))) return rc;
+    HIPCHK(launch_bias_gelu(m->Z[L], m->P + bo[L], m->H[L], CL_NH, B, m->stream));
+    in = m->H[L];
+    K = CL_NH;
+  }
+  return gemm(m, m->Wb + CLB_W4, CL_NH, 1, in, B, 1, m->Y, B, CL_MOUT, B, CL_NH, CL_NH, 0);
+}
+
+int lh_mlp_create(int batch, lh_mlp **out) {
+  if (!out || batch < 64 || batch % 64 || batch > (1 << 22))
+    return set_err(LH_EINVAL, "lh_mlp_create: batch must be a multiple of 64 in [64, 2^22]");
+  *out = nullptr;
+  lh_mlp *m = new lh_mlp();
+  m->B = batch;
+  hipError_t e = hipGetDevice(&m->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete m;
+    return set_err(LH_EHIP, "lh_mlp_create: %s", hipGetErrorString(e));
+  }
+  const size_t B = batch;
+  size_t off = 0;
+  size_t o_P = off; off += al<float>(CLP_N);
+  size_t o_G = off; off += al<float>(CLP_N);
+  size_t o_M = off; off += al<float>(CLP_N);
+  size_t o_V = off; off += al<float>(CLP_N);
+  size_t o_Wb = off; off += al<uint16_t>(CLB_N);
+  size_t o_X = off; off += al<uint16_t>(CL_KIN * B);
+  size_t o_T = off; off += al<float>(CL_NOUT * B);
+  size_t o_Z = off; off += 3 * al<float>(CL_NH * B);
+  size_t o_H = off; off += 3 * al<uint16_t>(CL_NH * B);
+  size_t o_Y = off; off += al<float>(CL_MOUT * B);
+  size_t o_dY = off; off += al<uint16_t>(CL_MOUT * B);
+  size_t o_dH = off; off += al<float>(CL_NH * B);
+  size_t o_dZ = off; off += al<uint16_t>(CL_NH * B);
+  size_t o_loss = off; off += al<double>(1);
+  size_t o_pts = off; off += al<int>(B);
+  e = hipMalloc((void **)&m->mem, off);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(m->stream);
+    delete m;
+    return set_err(LH_ENOMEM, "lh_mlp_create: hipMalloc(%zu): %s", off, hipGetErrorString(e));
+  }
+  (void)hipMemset(m->mem, 0, off);
+  m->P = (float *)(m->mem + o_P);
+  m->G = (float *)(m->mem + o_G);
+  m->Mo = (float *)(m->mem + o_M);
+  m->Vo = (float *)(m->mem + o_V);
+  m->Wb = (uint16_t *)(m->mem + o_Wb);
+  m->X = (uint16_t *)(m->mem + o_X);
+  m->T = (float *)(m->mem + o_T);
+  for (int L = 0; L < 3; L++) {
+    m->Z[L] = (float *)(m->mem + o_Z + L * al<float>(CL_NH * B));
+    m->H[L] = (uint16_t *)(m->mem + o_H + L * al<uint16_t>(CL_NH * B));
+  }
+  m->Y = (float *)(m->mem + o_Y);
+  m->dY = (uint16_t *)(m->mem + o_dY);
+  m->dH = (float *)(m->mem + o_dH);
+  m->dZ = (uint16_t *)(m->mem + o_dZ);
+  m->loss = (double *)(m->mem + o_loss);
+  m->pts = (int *)(m->mem + o_pts);
+  *out = m;
+  return LH_OK;
+}
+
+int lh_mlp_destroy(lh_mlp *m) {
+  if (!m) return LH_OK;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  if (m->mem) (void)hipFree(m->mem);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return LH_OK;
+}
+
+int lh_mlp_set_params(lh_mlp *m, const float *params, const float *norm9) {
+  int rc = mlp_ok(m);
+  if (rc) return rc;
+  if (!params || !norm9) return set_err(LH_EINVAL, "lh_mlp_set_params: null argument");
+  std::vector<float> P(CLP_N, 0.f);
+  for (int r = 0; r < CL_NH; r++) {
+    for (int i = 0; i < CL_NIN; i++) P[CLP_W1 + r * CL_KIN + i] = params[CC_W1 + r * CL_NIN + i];
+    for (int i = 0; i < CL_NH; i++) {
+      P[CLP_W2 + r * CL_NH + i] = params[CC_W2 + r * CL_NH + i];
+      P[CLP_W3 + r * CL_NH + i] = params[CC_W3 + r * CL_NH + i];
+    }
+    P[CLP_B1 + r] = params[CC_B1 + r];
+    P[CLP_B2 + r] = params[CC_B2 + r];
+    P[CLP_B3 + r] = params[CC_B3 + r];
+  }
+  for (int o = 0; o < CL_NOUT; o++) {
+    for (int i = 0; i < CL_NH; i++) P[CLP_W4 + o * CL_NH + i] = params[CC_W4 + o * CL_NH + i];
+    P[CLP_B4 + o] = params[CC_B4 + o];
+  }
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(m->P, P.data(), CLP_N * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(m->Mo, 0, CLP_N * sizeof(float)));
+  HIPCHK(hipMemset(m->Vo, 0, CLP_N * sizeof(float)));
+  m->step = 0;
+  m->norm = make_norm(norm9);
+  HIPCHK(launch_pack_bf16(m->P, m->Wb, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return LH_OK;
+}
+
+static void padded_to_compact(const std::vector<float> &P, float *out) {
+  for (int r = 0; r < CL_NH; r++) {
+    for (int i = 0; i < CL_NIN; i++) out[CC_W1 + r * CL_NIN + i] = P[CLP_W1 + r * CL_KIN + i];
+    for (int i = 0; i < CL_NH; i++) {
+      out[CC_W2 + r * CL_NH + i] = P[CLP_W2 + r * CL_NH + i];
+      out[CC_W3 + r * CL_NH + i] = P[CLP_W3 + r * CL_NH + i];
+    }
+    out[CC_B1 + r] = P[CLP_B1 + r];
+    out[CC_B2 + r] = P[CLP_B2 + r];
+    out[CC_B3 + r] = P[CLP_B3 + r];
+  }
+  for (int o = 0; o < CL_NOUT; o++) {
+    for (int i = 0; i < CL_NH; i++) out[CC_W4 + o * CL_NH + i] = P[CLP_W4 + o * CL_NH + i];
+    out[CC_B4 + o] = P[CLP_B4 + o];
+  }
+}
+
+int lh_mlp_get_params(lh_mlp *m, float *params) {
+  int rc = mlp_ok(m);
+  if (rc) return rc;
+  if (!params) return set_err(LH_EINVAL, "lh_mlp_get_params: null argument");
+  std::vector<float> P(CLP_N);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(P.data(), m->P, CLP_N * sizeof(float), hipMemcpyDeviceToHost));
+  padded_to_compact(P, params);
+  return LH_OK;
+}
+
+int lh_mlp_get_grad(lh_mlp *m, float *grad) {
+  int rc = mlp_ok(m);
+  if (rc) return rc;
+  if (!grad) return set_err(LH_EINVAL, "lh_mlp_get_grad: null argument");
+  std::vector<float> P(CLP_N);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(P.data(), m->G, CLP_N * sizeof(float), hipMemcpyDeviceToHost));
+  padded_to_compact(P, grad);
+  return LH_OK;
+}
+
+int lh_mlp_set_batch(lh_mlp *m, const float *x, const float *t) {
+  int rc = mlp_ok(m);
+  if (rc) return rc;
+  if (!x || !t) return set_err(LH_EINVAL, "lh_mlp_set_batch: null argument");
+  const size_t B = m->B;
+  std::vector<unsigned short> X(CL_KIN * B, 0);
+  std::vector<float> T(CL_NOUT * B);
+  for (size_t n = 0; n < B; n++) {
+    for (int f = 0; f < CL_NIN; f++) X[f * B + n] = host_bf16(x[n * CL_NIN + f]);
+    for (int o = 0; o < CL_NOUT; o++) T[o * B + n] = t[n * CL_NOUT + o];
+  }
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(m->X, X.data(), X.size() * 2, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m->T, T.data(), T.size() * 4, hipMemcpyHostToDevice));
+  return LH_OK;
+}
+
+int lh_mlp_sample(lh_mlp *m, int N1, int N2, int ny, const float *d_phys, const float *d_tau, int nsnap,
+                  uint64_t seed) {
+  int rc = mlp_ok(m);
+  if (rc) return rc;
+  if (!d_phys || !d_tau || nsnap < 1 || N1 < 4 || N2 < 4 || ny < 4)
+    return set_err(LH_EINVAL, "lh_mlp_sample: bad arguments");
+  HIPCHK(launch_cl_batch(d_phys, d_tau, nsnap, N1, N2, ny + 1, ny, m->norm, seed, m->X, m->T, m->B, m->stream));
+  return LH_OK;
+}
+
+int lh_mlp_set_points(lh_mlp *m, int N1, int N2, int ny, const float *d_phys, const int *pts) {
+  int rc = mlp_ok(m);
+  if (rc) return rc;
+  if (!d_phys || !pts || N1 < 4 || N2 < 4 || ny < 4) return set_err(LH_EINVAL, "lh_mlp_set_points: bad arguments");
+  const int npts = (ny - 1) * N1 * N2;
+  for (int n = 0; n < m->B; n++)
+    if (pts[n] < 0 || pts[n] >= npts) return set_err(LH_EINVAL, "lh_mlp_set_points: point %d out of range", pts[n]);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(m->pts, pts, m->B * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(launch_cl_features(d_phys, N1, N2, ny + 1, ny, m->norm, m->pts, m->X, m->B, m->stream));
+  HIPCHK(hipMemsetAsync(m->T, 0, (size_t)CL_NOUT * m->B * sizeof(float), m->stream));
+  return LH_OK;
+}
+
+int lh_mlp_forward(lh_mlp *m, float *y) {
+  int rc = mlp_ok(m);
+  if (rc) return rc;
+  if (!y) return set_err(LH_EINVAL, "lh_mlp_forward: null argument");
+  if ((rc = mlp_forward(m))) return rc;
+  const size_t B = m->B;
+  std::vector<float> Y(CL_NOUT * B), b4(CL_NOUT);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(Y.data(), m->Y, Y.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(b4.data(), m->P + CLP_B4, CL_NOUT * 4, hipMemcpyDeviceToHost));
+  for (size_t n = 0; n < B; n++)
+    for (int o = 0; o < CL_NOUT; o++) y[n * CL_NOUT + o] = Y[o * B + n] + b4[o];
+  return LH_OK;
+}
+
+int lh_mlp_grad(lh_mlp *m, double *loss) {
+  int rc = mlp_ok(m);
+  if (rc) return rc;
+  const int B = m->B;
+  HIPCHK(hipMemsetAsync(m->G, 0, CLP_N * sizeof(float), m->stream));
+  HIPCHK(hipMemsetAsync(m->loss, 0, sizeof(double), m->stream));
+  if ((rc = mlp_forward(m))) return rc;
+  const float scale = 1.f / (3.f * (float)B);
+  HIPCHK(launch_out_loss(m->Y, m->P + CLP_B4, m->T, m->dY, m->loss, B, scale, m->stream));
+  const int KC = 256;  // points per block of the weight-gradient GEMMs
+  // output layer
+  if ((rc = gemm(m, m->dY, B, 1, m->H[2], 1, B, m->G + CLP_W4, CL_NH, CL_MOUT, CL_NH, B, KC, 1))) return rc;
+  HIPCHK(launch_rowsum(m->dY, m->G + CLP_B4, CL_MOUT, B, m->stream));
+  if ((rc = gemm(m, m->Wb + CLB_W4, 1, CL_NH, m->dY, B, 1, m->dH, B, CL_NH, B, CL_MOUT, CL_MOUT, 0))) return rc;
+  // hidden layers 3, 2, 1
+  const int wo[3] = {CLB_W1, CLB_W2, CLB_W3}, gw[3] = {CLP_W1, CLP_W2, CLP_W3}, gb[3] = {CLP_B1, CLP_B2, CLP_B3};
+  for (int L = 2; L >= 0; L--) {
+    HIPCHK(launch_dgelu(m->dH, m->Z[L], m->dZ, CL_NH, B, m->stream));
+    const uint16_t *in = L ? m->H[L - 1] : m->X;
+    const int K = L ? CL_NH : CL_KIN;
+    if ((rc = gemm(m, m->dZ, B, 1, in, 1, B, m->G + gw[L], K, CL_NH, K, B, KC, 1))) return rc;
+    HIPCHK(launch_rowsum(m->dZ, m->G + gb[L], CL_NH, B, m->stream));
+    if (L)
+      if ((rc = gemm(m, m->Wb + wo[L], 1, CL_NH, m->dZ, B, 1, m->dH, B, CL_NH, B, CL_NH, CL_NH, 0))) return rc;
+  }
+  double l = 0.0;
+  HIPCHK(hipMemcpyAsync(&l, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  if (loss) *loss = l / (3.0 * B);
+  return LH_OK;
+}
+
+int lh_mlp_adam(lh_mlp *m, float lr, float beta1, float beta2, float eps, float wd) {
+  int rc = mlp_ok(m);
+  if (rc) return rc;
+  m->step += 1;
+  const float bc1 = 1.f - powf(beta1, (float)m->step), bc2 = 1.f - powf(beta2, (float)m->step);
+  HIPCHK(launch_adam(m->P, m->G, m->Mo, m->Vo, CLP_N, lr, beta1, beta2, eps, wd, bc1, bc2, m->stream));
+  HIPCHK(launch_pack_bf16(m->P, m->Wb, m->stream));
+  return LH_OK;
+}
+
+int lh_mlp_sync(lh_mlp *m) {
+  int rc = mlp_ok(m);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(m->stream));
   return LH_OK;
 }
 

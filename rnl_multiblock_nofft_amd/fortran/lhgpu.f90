@@ -28,6 +28,8 @@ module lhgpu
                                LH_ST_WX2M = 12, LH_ST_N = 13
   integer(c_int), parameter :: LH_SP_U = 0, LH_SP_V = 1, LH_SP_W = 2, LH_SP_UV = 3, &
                                LH_SP_P = 4, LH_SP_N = 5
+  integer(c_int), parameter :: LH_CL_NIN = 135, LH_CL_NHID = 128, LH_CL_NOUT = 3, &
+                               LH_CL_NPARAM = 50819
 
   type, bind(C) :: lh_config
     integer(c_int) :: N1, N2, Ngal1, Ngal2, ny
@@ -299,6 +301,112 @@ module lhgpu
       type(c_ptr), value :: ctx
       integer(c_int), value :: ch, which
       real(c_double), intent(out) :: sp(*)
+      integer(c_int) :: rc
+    end function
+    ! learned closure of the nonlinear term (no counterpart in the reference):
+    ! params LH_CL_NPARAM reals W1 b1 W2 b2 W3 b3 W4 b4, norm9 = mean(3),
+    ! inv_scale(3), out_scale(3); device buffers as type(c_ptr)
+    function lh_closure_set(ctx, params, norm9) bind(C, name='lh_closure_set') result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(in) :: params(*), norm9(9)
+      integer(c_int) :: rc
+    end function
+    function lh_closure_enable(ctx, ch, on) bind(C, name='lh_closure_enable') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch, on
+      integer(c_int) :: rc
+    end function
+    function lh_closure_eval(ctx, ch, d_phys, d_corr) bind(C, name='lh_closure_eval') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, d_phys, d_corr
+      integer(c_int), value :: ch
+      integer(c_int) :: rc
+    end function
+    function lh_closure_sample(fine, chf, coarse, chc, d_phys, d_tau) bind(C, name='lh_closure_sample') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: fine, coarse, d_phys, d_tau
+      integer(c_int), value :: chf, chc
+      integer(c_int) :: rc
+    end function
+    function lh_closure_time(ctx, nrep, ms3) bind(C, name='lh_closure_time') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nrep
+      real(c_double), intent(out) :: ms3(3)
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_create(batch, m) bind(C, name='lh_mlp_create') result(rc)
+      import :: c_ptr, c_int
+      integer(c_int), value :: batch
+      type(c_ptr), intent(out) :: m
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_destroy(m) bind(C, name='lh_mlp_destroy') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: m
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_set_params(m, params, norm9) bind(C, name='lh_mlp_set_params') result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: m
+      real(c_float), intent(in) :: params(*), norm9(9)
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_get_params(m, params) bind(C, name='lh_mlp_get_params') result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: m
+      real(c_float), intent(out) :: params(*)
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_set_batch(m, x, t) bind(C, name='lh_mlp_set_batch') result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: m
+      real(c_float), intent(in) :: x(*), t(*)
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_sample(m, N1, N2, ny, d_phys, d_tau, nsnap, seed) bind(C, name='lh_mlp_sample') result(rc)
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: m, d_phys, d_tau
+      integer(c_int), value :: N1, N2, ny, nsnap
+      integer(c_int64_t), value :: seed
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_set_points(m, N1, N2, ny, d_phys, pts) bind(C, name='lh_mlp_set_points') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: m, d_phys
+      integer(c_int), value :: N1, N2, ny
+      integer(c_int), intent(in) :: pts(*)
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_forward(m, y) bind(C, name='lh_mlp_forward') result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: m
+      real(c_float), intent(out) :: y(*)
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_grad(m, loss) bind(C, name='lh_mlp_grad') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: m
+      real(c_double), intent(out) :: loss
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_get_grad(m, grad) bind(C, name='lh_mlp_get_grad') result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: m
+      real(c_float), intent(out) :: grad(*)
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_adam(m, lr, beta1, beta2, eps, wd) bind(C, name='lh_mlp_adam') result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: m
+      real(c_float), value :: lr, beta1, beta2, eps, wd
+      integer(c_int) :: rc
+    end function
+    function lh_mlp_sync(m) bind(C, name='lh_mlp_sync') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: m
       integer(c_int) :: rc
     end function
     function c_strlen(s) bind(C, name='strlen') result(n)
