@@ -84,6 +84,12 @@ class DeviceArray:
         check(self.L.lh_copy_to_host(out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes))
         return out
 
+    def get_rows(self, first, count):
+        """Elements first .. first+count-1 of the first axis."""
+        out = np.empty((count,) + self.shape[1:], np.float32)
+        check(self.L.lh_copy_to_host(out.ctypes.data_as(ctypes.c_void_p), self.at(first), out.nbytes))
+        return out
+
     def set(self, arr):
         a = np.ascontiguousarray(arr, np.float32)
         assert a.shape == self.shape
@@ -156,8 +162,9 @@ class Snapshots:
         """Normalisation from the stored samples: per field mean and
         1/std of the inputs (interior rows), rms of each target."""
         n = self.count if max_snap is None else min(self.count, max_snap)
-        ph = self.phys.get()[:n, :, 1:-1]
-        ta = self.tau.get()[:n, :, 1:-1]
+        pick = np.unique(np.linspace(0, n - 1, min(n, 64)).astype(int))
+        ph = np.stack([self.phys.get_rows(int(q), 1)[0] for q in pick])[:, :, 1:-1]
+        ta = np.stack([self.tau.get_rows(int(q), 1)[0] for q in pick])[:, :, 1:-1]
         mean = ph.mean(axis=(0, 2, 3, 4))
         std = ph.std(axis=(0, 2, 3, 4))
         rms = np.sqrt((ta.astype(np.float64) ** 2).mean(axis=(0, 2, 3, 4)))
@@ -271,22 +278,24 @@ def train(trainer, snaps, steps, lr=1e-3, first=0, count=None, seed=1, log_every
 def apriori(trainer, snaps, first, count, max_points=None):
     """A-priori error of the network on snapshots first .. first+count-1:
     per output, sum (y - t)^2 / sum t^2 over every point of u rows
-    1..ny-1 (t = tau / out_scale; the zero correction scores 1)."""
+    1..ny-1 (t = tau / out_scale; the zero correction scores 1), or over
+    max_points points spread evenly through each snapshot."""
     c = snaps.cfg
     plane = c["N1"] * c["N2"]
     npts = (c["ny"] - 1) * plane
     B = trainer.batch
-    tau = snaps.tau.get()
     scale = trainer.norm9[6:9].astype(np.float64)
     num, den = np.zeros(NOUT), np.zeros(NOUT)
+    use = np.arange(npts) if max_points is None or max_points >= npts else \
+        np.linspace(0, npts - 1, max_points).astype(np.int64)
     for s in range(first, first + count):
-        t_all = tau[s][:, 1:c["ny"]].reshape(NOUT, -1).T / scale
-        n_use = npts if max_points is None else min(npts, max_points)
-        for p0 in range(0, n_use, B):
-            pts = np.arange(p0, p0 + B) % npts
+        tau = snaps.tau.get_rows(s, 1)[0]
+        t_all = tau[:, 1:c["ny"]].reshape(NOUT, -1).T / scale
+        for p0 in range(0, len(use), B):
+            k = min(B, len(use) - p0)
+            pts = use[np.arange(p0, p0 + B) % len(use)]
             trainer.set_points(c, snaps.phys.at(s), pts)
             y = trainer.forward().astype(np.float64)
-            k = min(B, n_use - p0)
             t = t_all[pts[:k]]
             num += ((y[:k] - t) ** 2).sum(axis=0)
             den += (t ** 2).sum(axis=0)

@@ -42,15 +42,25 @@ __device__ __forceinline__ f32x4 mfma(const u32x4 &a, const u32x4 &b, const f32x
                                                  0, 0);
 }
 
-// GELU, tanh form: x * sigmoid(2u), u = sqrt(2/pi) (x + 0.044715 x^3)
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+// two floats to packed bf16 (v_cvt_pk_bf16_f32, round to nearest even)
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+}
+
+// GELU, tanh form: x * sigmoid(2u), u = sqrt(2/pi) (x + 0.044715 x^3), with
+// the hardware exp2 and reciprocal (the result is rounded to bf16)
+// exp(-2u) = exp2(x (C1 + C2 x^2)), C1 = -2 sqrt(2/pi) log2(e), C2 = 0.044715 C1
 __device__ __forceinline__ float gelu_f(float x) {
-  const float u = 0.7978845608028654f * x * (1.f + 0.044715f * x * x);
-  return x / (1.f + __expf(-2.f * u));
+  const float C1 = -2.3022082f, C2 = -0.10294324f;
+  return x * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(x * __builtin_fmaf(C2, x * x, C1)));
 }
 __device__ __forceinline__ float dgelu_f(float x) {
   const float x2 = x * x;
   const float u = 0.7978845608028654f * x * (1.f + 0.044715f * x2);
-  const float s = 1.f / (1.f + __expf(-2.f * u));
+  const float s = __builtin_amdgcn_rcpf(1.f + __expf(-2.f * u));
   const float du = 0.7978845608028654f * (1.f + 3.f * 0.044715f * x2);
   return s + 2.f * x * s * (1.f - s) * du;
 }
@@ -128,27 +138,29 @@ hipError_t launch_cl_phys(char *arena, const Chan *d_chan, const Chan &h, ClSrc 
 }
 
 // ------------------------------------------------------------------------
-// the network on every point (j, z, x), j = 1..ny-1.  A 256-thread
-// workgroup holds the weights in LDS; each of its four waves takes 32 points
-// (two column blocks of 16) per pass: the wave gathers their inputs into LDS
-// (lanes over consecutive points, so each input is one coalesced load), then
-// runs the four layers with the hidden activations in registers.
-constexpr int CL_NB = 2;
+// the network on every point (j, z, x), j = 1..ny-1.  A workgroup of NW
+// waves holds the weights in LDS; each wave takes 16 CL_NB points per pass,
+// gathers their inputs into its own LDS rows (lanes over consecutive points,
+// so each input is one coalesced load), then runs the four layers with the
+// hidden activations in registers.  Eight waves of 16 points (two per SIMD)
+// measured fastest: the network is bound by the VALU work of the GELUs.
 constexpr int CL_XP = 168;  // LDS pitch of one point's inputs (bf16; 336 bytes)
-constexpr int CL_WPTS = 16 * CL_NB;
 
-template <int NOB>
-__device__ __forceinline__ void cl_zero(f32x4 (&acc)[NOB][CL_NB]) {
+// accumulators start from the bias of their rows (16b + 4g + r)
+template <int NOB, int CL_NB>
+__device__ __forceinline__ void cl_bias(f32x4 (&acc)[NOB][CL_NB], const float *bias, int g) {
 #pragma unroll
-  for (int b = 0; b < NOB; b++)
+  for (int b = 0; b < NOB; b++) {
+    const f32x4 v = *(const f32x4 *)(bias + 16 * b + 4 * g);
 #pragma unroll
-    for (int n = 0; n < CL_NB; n++) acc[b][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int n = 0; n < CL_NB; n++) acc[b][n] = v;
+  }
 }
 
-// bias + GELU of a hidden layer's accumulators, repacked as the next
-// layer's B operand (k slots 0-3 from block 2s, 4-7 from block 2s+1)
-__device__ __forceinline__ void cl_act(const f32x4 (&acc)[8][CL_NB], const float *bias, int g,
-                                       u32x4 (&hb)[CL_NB][4]) {
+// GELU of a hidden layer's accumulators (bias included), repacked as the
+// next layer's B operand (k slots 0-3 from block 2s, 4-7 from block 2s+1)
+template <int CL_NB>
+__device__ __forceinline__ void cl_act(const f32x4 (&acc)[8][CL_NB], u32x4 (&hb)[CL_NB][4]) {
 #pragma unroll
   for (int n = 0; n < CL_NB; n++)
 #pragma unroll
@@ -157,60 +169,112 @@ __device__ __forceinline__ void cl_act(const f32x4 (&acc)[8][CL_NB], const float
 #pragma unroll
       for (int half = 0; half < 2; half++) {
         const f32x4 a = acc[2 * s + half][n];
-        const float *bb = bias + 32 * s + 16 * half + 4 * g;
-        const unsigned e0 = f2bf(gelu_f(a[0] + bb[0])), e1 = f2bf(gelu_f(a[1] + bb[1]));
-        const unsigned e2 = f2bf(gelu_f(a[2] + bb[2])), e3 = f2bf(gelu_f(a[3] + bb[3]));
-        w[2 * half] = e0 | (e1 << 16);
-        w[2 * half + 1] = e2 | (e3 << 16);
+        w[2 * half] = pack_bf16(gelu_f(a[0]), gelu_f(a[1]));
+        w[2 * half + 1] = pack_bf16(gelu_f(a[2]), gelu_f(a[3]));
       }
       hb[n][s] = u32x4{w[0], w[1], w[2], w[3]};
     }
 }
 
-__device__ __forceinline__ void cl_hidden(const u32x4 *W, int f0, const u32x4 (&hin)[CL_NB][4],
-                                          f32x4 (&acc)[8][CL_NB], int lane) {
-  cl_zero<8>(acc);
+template <int CL_NB>
+__device__ __forceinline__ void cl_hidden(const u32x4 *W, int f0, const float *bias, const u32x4 (&hin)[CL_NB][4],
+                                          f32x4 (&acc)[8][CL_NB], int lane, int g) {
+  cl_bias<8, CL_NB>(acc, bias, g);
 #pragma unroll
-  for (int s = 0; s < 4; s++)
+  for (int s = 0; s < 4; s++) {
 #pragma unroll
     for (int b = 0; b < 8; b++) {
       const u32x4 a = W[(f0 + b * 4 + s) * 64 + lane];
 #pragma unroll
       for (int n = 0; n < CL_NB; n++) acc[b][n] = mfma(a, hin[n][s], acc[b][n]);
     }
+    // at most one k-step of weight fragments in flight (register budget)
+    __builtin_amdgcn_sched_barrier(0);
+  }
 }
 
-__global__ void __launch_bounds__(256) k_cl_infer(const u32x4 *frags, const float *bias, ClNorm nm,
+template <int CL_NB, int NW>
+__global__ void __launch_bounds__(64 * NW) k_cl_infer(const u32x4 *frags, const float *bias, ClNorm nm,
                                                   const float *phys, float *corr, int N1, int N2, int nyu, int nyv) {
   extern __shared__ u32x4 smem_w[];
   u32x4 *W = smem_w;
   float *Bs = (float *)(smem_w + CLF_N * 64);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, q = lane & 15;
-  uint16_t *Xs = (uint16_t *)(Bs + CL_NBIAS) + wv * CL_WPTS * CL_XP;
+  uint16_t *Xs = (uint16_t *)(Bs + CL_NBIAS) + wv * 16 * CL_NB * CL_XP;
   for (int t = threadIdx.x; t < CLF_N * 64; t += blockDim.x) W[t] = frags[t];
   for (int t = threadIdx.x; t < CL_NBIAS; t += blockDim.x) Bs[t] = bias[t];
   const int plane = N1 * N2;
   const int npts = (nyu - 2) * plane;  // u rows 1..ny-1
-  const int per_pass = 4 * CL_WPTS;
-  // every wave runs the same number of passes (the barriers below)
-  for (int wgbase = blockIdx.x * per_pass; wgbase < npts; wgbase += gridDim.x * per_pass) {
-    const int base = wgbase + wv * CL_WPTS;
+  constexpr int CL_WPTS = 16 * CL_NB;
+  __syncthreads();  // weights and biases staged
+  // the waves run independently (no workgroup barrier below), so one wave's
+  // activations overlap another's matrix work
+  for (int base = (blockIdx.x * NW + wv) * CL_WPTS; base < npts; base += gridDim.x * NW * CL_WPTS) {
     {
-      // inputs: lanes 0-31 take input f of points 0-31, lanes 32-63 input f+1
-      const int pt = lane & 31, hf = lane >> 5;
+      // inputs: lane l takes point l % PTS and every PARTS-th input from
+      // l / PTS on; every input's decode is a compile-time constant of the
+      // unrolled loop (selected by the part), all loads issue before the
+      // stores
+      constexpr int PTS = 16 * CL_NB, PARTS = 64 / PTS, NIT = (CL_NIN + PARTS - 1) / PARTS;
+      const int pt = lane % PTS, part = lane / PTS;
       const int p = base + pt;
       const bool ok = p < npts;
       const int pp = ok ? p : npts - 1;
       const int j = 1 + pp / plane, r = pp - (j - 1) * plane, z = r / N1, x = r - z * N1;
-      uint16_t *xr = Xs + pt * CL_XP;
-      for (int f = hf; f < CL_KIN; f += 2) {
-        const float val = (f < CL_NIN && ok) ? cl_feature(phys, f, j, z, x, N1, N2, nyu, nyv, nm) : 0.f;
-        xr[f] = (uint16_t)f2bf(val);
+      int ro[3][5], zo[3], xo[3];
+#pragma unroll
+      for (int fl = 0; fl < 3; fl++)
+#pragma unroll
+        for (int dy = 0; dy < 5; dy++) {
+          const int jmax = (fl == 1 ? nyv : nyu) - 1;
+          int jj = j + dy - 2;
+          jj = jj < 0 ? 0 : (jj > jmax ? jmax : jj);
+          ro[fl][dy] = (fl * nyu + jj) * plane;
+        }
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        int zz = z + d - 1, xx = x + d - 1;
+        zz += zz < 0 ? N2 : 0;
+        zz -= zz >= N2 ? N2 : 0;
+        xx += xx < 0 ? N1 : 0;
+        xx -= xx >= N1 ? N1 : 0;
+        zo[d] = zz * N1;
+        xo[d] = xx;
       }
+      float v[NIT];
+#pragma unroll
+      for (int it = 0; it < NIT; it++) {
+        int a = ro[0][0] + zo[0] + xo[0];
+#pragma unroll
+        for (int q = 0; q < PARTS; q++) {
+          const int f = PARTS * it + q;
+          if (f < CL_NIN && part == q) a = ro[f / 45][(f % 45) / 9] + zo[(f % 9) / 3] + xo[f % 3];
+        }
+        v[it] = phys[a];
+      }
+      uint16_t *xr = Xs + pt * CL_XP;
+#pragma unroll
+      for (int it = 0; it < NIT; it++) {
+        float m = 0.f, sc = 0.f;
+#pragma unroll
+        for (int q = 0; q < PARTS; q++) {
+          const int f = PARTS * it + q;
+          if (f < CL_NIN && part == q) {
+            const int fl = f / 45;
+            m = fl == 0 ? nm.mean[0] : (fl == 1 ? nm.mean[1] : nm.mean[2]);
+            sc = fl == 0 ? nm.inv_scale[0] : (fl == 1 ? nm.inv_scale[1] : nm.inv_scale[2]);
+          }
+        }
+        xr[PARTS * it + part] = (uint16_t)f2bf(ok ? (v[it] - m) * sc : 0.f);
+      }
+      for (int f = PARTS * NIT + part; f < CL_KIN; f += PARTS) xr[f] = 0;
     }
-    __syncthreads();
+    // the wave's own LDS rows: its stores are ordered before its loads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     f32x4 acc[8][CL_NB];
-    cl_zero<8>(acc);
+    cl_bias<8, CL_NB>(acc, Bs, g);
 #pragma unroll
     for (int s = 0; s < 5; s++) {
       u32x4 xb[CL_NB];
@@ -222,15 +286,16 @@ __global__ void __launch_bounds__(256) k_cl_infer(const u32x4 *frags, const floa
 #pragma unroll
         for (int n = 0; n < CL_NB; n++) acc[b][n] = mfma(a, xb[n], acc[b][n]);
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
     u32x4 h[CL_NB][4];
-    cl_act(acc, Bs, g, h);
-    cl_hidden(W, CLF_L2, h, acc, lane);
-    cl_act(acc, Bs + CL_NH, g, h);
-    cl_hidden(W, CLF_L3, h, acc, lane);
-    cl_act(acc, Bs + 2 * CL_NH, g, h);
+    cl_act(acc, h);
+    cl_hidden(W, CLF_L2, Bs + CL_NH, h, acc, lane, g);
+    cl_act(acc, h);
+    cl_hidden(W, CLF_L3, Bs + 2 * CL_NH, h, acc, lane, g);
+    cl_act(acc, h);
     f32x4 o[1][CL_NB];
-    cl_zero<1>(o);
+    cl_bias<1, CL_NB>(o, Bs + 3 * CL_NH, g);
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const u32x4 a = W[(CLF_L4 + s) * 64 + lane];
@@ -246,24 +311,28 @@ __global__ void __launch_bounds__(256) k_cl_infer(const u32x4 *frags, const floa
         const int j = 1 + p / plane, r = p - (j - 1) * plane;
         const size_t at = (size_t)j * plane + r;
         const size_t fs = (size_t)nyu * plane;
-        corr[at] = (o[0][n][0] + Bs[3 * CL_NH + 0]) * nm.out_scale[0];
-        corr[fs + at] = (o[0][n][1] + Bs[3 * CL_NH + 1]) * nm.out_scale[1];
-        corr[2 * fs + at] = (o[0][n][2] + Bs[3 * CL_NH + 2]) * nm.out_scale[2];
+        corr[at] = o[0][n][0] * nm.out_scale[0];
+        corr[fs + at] = o[0][n][1] * nm.out_scale[1];
+        corr[2 * fs + at] = o[0][n][2] * nm.out_scale[2];
       }
     }
-    __syncthreads();  // this pass's inputs are read before the next gather
+    // this pass's inputs are read before the next pass stores over them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
 hipError_t launch_cl_infer(const void *frags, const float *bias, ClNorm nm, const float *phys, float *corr, int N1,
                            int N2, int nyu, int nyv, hipStream_t s) {
+  constexpr int NB = 1, NW = 8;
   const int npts = (nyu - 2) * N1 * N2;
-  const int per_pass = 4 * CL_WPTS;
+  const int per_pass = NW * 16 * NB;
   int nb = (npts + per_pass - 1) / per_pass;
   if (nb > 256) nb = 256;
-  const size_t lds = (size_t)CL_FRAG_BYTES + CL_NBIAS * sizeof(float) + (size_t)4 * CL_WPTS * CL_XP * 2;
-  hipLaunchKernelGGL(k_cl_infer, dim3(nb), dim3(256), lds, s, (const u32x4 *)frags, bias, nm, phys, corr, N1, N2,
-                     nyu, nyv);
+  const size_t lds = (size_t)CL_FRAG_BYTES + CL_NBIAS * sizeof(float) + (size_t)NW * 16 * NB * CL_XP * 2;
+  hipLaunchKernelGGL((k_cl_infer<NB, NW>), dim3(nb), dim3(64 * NW), lds, s, (const u32x4 *)frags, bias, nm, phys, corr,
+                     N1, N2, nyu, nyv);
   return hipGetLastError();
 }
 
