@@ -1,5 +1,9 @@
-// lh_fft.h -- batched complex FFTs in LDS for gfx950 (Stockham autosort,
-// mixed radix 3/4/2), executed cooperatively by all threads of a workgroup.
+// lh_fft.h -- batched complex FFTs in LDS for gfx950 (Stockham autosort),
+// executed cooperatively by all threads of a workgroup.  Each stage is one
+// LDS round trip: a thread reads R points, twiddles them, runs an R-point
+// DFT in registers (R = 6, 8, 12, 16 built from 2, 3, 4 with compile-time
+// twiddles) and writes them back, so a 96-point transform takes two stages
+// (12 x 8) instead of four radix-3/4/2 passes.
 //
 // These replace FFTPACK's cfftf/cfftb and rfftf/rfftb (cft_buff.f,
 // rft_buff.f) as used by four_to_phys_* / phys_to_four_* (FOU3D.f90:738-804,
@@ -11,6 +15,8 @@
 // workgroup stages in LDS once (stage_twiddles) from the FP64 table in HBM;
 // the FP32 instantiation rounds each entry once to FP32.
 #pragma once
+#include <type_traits>
+
 #include "lh_common.h"
 
 namespace lh {
@@ -18,16 +24,16 @@ namespace lh {
 template <typename C> struct CT;
 template <> struct CT<double2> {
   typedef double R;
-  static __device__ __forceinline__ double2 mk(double a, double b) { return make_double2(a, b); }
-  static __device__ __forceinline__ double2 from(double2 w) { return w; }
+  static __host__ __device__ __forceinline__ double2 mk(double a, double b) { return make_double2(a, b); }
+  static __host__ __device__ __forceinline__ double2 from(double2 w) { return w; }
 };
 template <> struct CT<float2> {
   typedef float R;
-  static __device__ __forceinline__ float2 mk(float a, float b) { return make_float2(a, b); }
-  static __device__ __forceinline__ float2 from(double2 w) { return make_float2((float)w.x, (float)w.y); }
+  static __host__ __device__ __forceinline__ float2 mk(float a, float b) { return make_float2(a, b); }
+  static __host__ __device__ __forceinline__ float2 from(double2 w) { return make_float2((float)w.x, (float)w.y); }
 };
 // twiddle m of the table, conjugated for the backward transform
-template <typename C> __device__ __forceinline__ C tw_at(const C *t, int m, int sign) {
+template <typename C> __host__ __device__ __forceinline__ C tw_at(const C *t, int m, int sign) {
   C w = t[m];
   return CT<C>::mk(w.x, sign > 0 ? -w.y : w.y);
 }
@@ -37,118 +43,368 @@ template <typename C> __device__ __forceinline__ void stage_twiddles(C *dst, con
   for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = CT<C>::from(src[m]);
 }
 
-template <typename C> __device__ __forceinline__ C c_add(C a, C b) { return CT<C>::mk(a.x + b.x, a.y + b.y); }
-template <typename C> __device__ __forceinline__ C c_sub(C a, C b) { return CT<C>::mk(a.x - b.x, a.y - b.y); }
-template <typename C> __device__ __forceinline__ C c_mul(C a, C b) {
+template <typename C> __host__ __device__ __forceinline__ C c_add(C a, C b) { return CT<C>::mk(a.x + b.x, a.y + b.y); }
+template <typename C> __host__ __device__ __forceinline__ C c_sub(C a, C b) { return CT<C>::mk(a.x - b.x, a.y - b.y); }
+template <typename C> __host__ __device__ __forceinline__ C c_mul(C a, C b) {
   return CT<C>::mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// number of Stockham stages and the radix of stage s for length n
-__host__ __device__ inline int fft_radix(int n, int s) {
-  int m = n, st = 0;
-  while (m % 3 == 0) { if (st == s) return 3; m /= 3; st++; }
-  while (m % 4 == 0) { if (st == s) return 4; m /= 4; st++; }
-  while (m % 2 == 0) { if (st == s) return 2; m /= 2; st++; }
-  return 0;
+// cos(2 pi q / 48) for any integer q.  Every radix below divides 48, so
+// these values cover all in-register twiddles; they are the exact values
+// cos(7.5 q degrees) rounded to double.
+__host__ __device__ constexpr double cos48(int q) {
+  q %= 48;
+  if (q < 0) q += 48;
+  const bool neg = q > 12 && q < 36;
+  const int r = q <= 12 ? q : (q < 24 ? 24 - q : (q <= 36 ? q - 24 : 48 - q));
+  double c = 0.0;
+  switch (r) {
+    case 0: c = 1.0; break;
+    case 1: c = 0.9914448613738104111446; break;
+    case 2: c = 0.9659258262890682867497; break;
+    case 3: c = 0.9238795325112867561282; break;
+    case 4: c = 0.8660254037844386467637; break;
+    case 5: c = 0.7933533402912351645798; break;
+    case 6: c = 0.7071067811865475244008; break;
+    case 7: c = 0.6087614290087206394161; break;
+    case 8: c = 0.5; break;
+    case 9: c = 0.3826834323650897717285; break;
+    case 10: c = 0.2588190451025207623489; break;
+    case 11: c = 0.1305261922200515915484; break;
+    default: c = 0.0; break;
+  }
+  return neg ? -c : c;
 }
+// sin(2 pi q / 48) = cos(2 pi (q - 12) / 48)
+__host__ __device__ constexpr double sin48(int q) { return cos48(q - 12); }
 
-// One radix-R butterfly of a Stockham stage: reads x[jj + r*nbf] of one
-// sequence, applies the twiddles of output group k = jj % Ns and writes
-// y[idxD + r*Ns].
-template <typename C, int R, int SIGN>
-__device__ __forceinline__ void butterfly(const C *xs, C *ys, int jj, int nbf, int Ns, int k, int idxD, int span,
-                                          const C *tw) {
+// compile-time loop: f(integral_constant<int, i>) for i = 0 .. N-1
+template <int I, int N, typename F> __host__ __device__ __forceinline__ void sfor_(F &f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>());
+    sfor_<I + 1, N>(f);
+  }
+}
+template <int N, typename F> __host__ __device__ __forceinline__ void sfor(F f) { sfor_<0, N>(f); }
+
+// t = hi * n + lo for t = t0, t0 + step, t0 + 2 step, ...: two divisions
+// when the walk starts, none per step (a run-time division costs ~30 VALU
+// instructions on gfx950)
+struct Walk2 {
+  int hi, lo, dhi, dlo, n;
+  __device__ __forceinline__ Walk2(int t0, int step, int n_) : n(n_) {
+    hi = t0 / n;
+    lo = t0 - hi * n;
+    dhi = step / n;
+    dlo = step - dhi * n;
+  }
+  __device__ __forceinline__ void next() {
+    hi += dhi;
+    lo += dlo;
+    if (lo >= n) {
+      lo -= n;
+      hi++;
+    }
+  }
+};
+
+// t = (hi * nm + mid) * nl + lo for t = t0, t0 + step, ...
+struct Walk3 {
+  int hi, mid, lo, dhi, dmid, dlo, nm, nl;
+  __device__ __forceinline__ Walk3(int t0, int step, int nl_, int nm_) : nm(nm_), nl(nl_) {
+    int q = t0 / nl;
+    lo = t0 - q * nl;
+    hi = q / nm;
+    mid = q - hi * nm;
+    q = step / nl;
+    dlo = step - q * nl;
+    dhi = q / nm;
+    dmid = q - dhi * nm;
+  }
+  __device__ __forceinline__ void next() {
+    lo += dlo;
+    mid += dmid;
+    hi += dhi;
+    if (lo >= nl) {
+      lo -= nl;
+      mid++;
+    }
+    if (mid >= nm) {
+      mid -= nm;
+      hi++;
+    }
+  }
+};
+
+// a * exp(SIGN 2 pi i Q / 48), Q a compile-time constant; quarter turns are
+// exact swaps
+template <typename C, int SIGN, int Q> __host__ __device__ __forceinline__ C rot48(C a) {
   typedef typename CT<C>::R Rt;
-  const Rt s3 = (Rt)0.86602540378443864676;  // sqrt(3)/2
-  if (R == 4) {
-    C v0 = xs[jj], v1 = xs[jj + nbf], v2 = xs[jj + 2 * nbf], v3 = xs[jj + 3 * nbf];
-    if (Ns > 1) {
-      int m = k * span;
-      v1 = c_mul(v1, tw_at<C>(tw, m, SIGN));
-      v2 = c_mul(v2, tw_at<C>(tw, 2 * m, SIGN));
-      v3 = c_mul(v3, tw_at<C>(tw, 3 * m, SIGN));
-    }
-    C t0 = c_add(v0, v2), t1 = c_sub(v0, v2), t2 = c_add(v1, v3), d = c_sub(v1, v3);
-    C t3 = SIGN < 0 ? CT<C>::mk(d.y, -d.x) : CT<C>::mk(-d.y, d.x);  // d * (sign*i)
-    ys[idxD] = c_add(t0, t2);
-    ys[idxD + Ns] = c_add(t1, t3);
-    ys[idxD + 2 * Ns] = c_sub(t0, t2);
-    ys[idxD + 3 * Ns] = c_sub(t1, t3);
-  } else if (R == 3) {
-    C v0 = xs[jj], v1 = xs[jj + nbf], v2 = xs[jj + 2 * nbf];
-    if (Ns > 1) {
-      int m = k * span;
-      v1 = c_mul(v1, tw_at<C>(tw, m, SIGN));
-      v2 = c_mul(v2, tw_at<C>(tw, 2 * m, SIGN));
-    }
-    C tt = c_add(v1, v2), d = c_sub(v1, v2);
-    C mm = CT<C>::mk(v0.x - (Rt)0.5 * tt.x, v0.y - (Rt)0.5 * tt.y);
-    // e = sign*i*s3*d
-    C e = SIGN < 0 ? CT<C>::mk(s3 * d.y, -s3 * d.x) : CT<C>::mk(-s3 * d.y, s3 * d.x);
-    ys[idxD] = c_add(v0, tt);
-    ys[idxD + Ns] = c_add(mm, e);
-    ys[idxD + 2 * Ns] = c_sub(mm, e);
+  constexpr int q = ((Q % 48) + 48) % 48;
+  if constexpr (q == 0) {
+    return a;
+  } else if constexpr (q == 24) {
+    return CT<C>::mk(-a.x, -a.y);
+  } else if constexpr (q == 12) {  // times SIGN i
+    return SIGN > 0 ? CT<C>::mk(-a.y, a.x) : CT<C>::mk(a.y, -a.x);
+  } else if constexpr (q == 36) {  // times -SIGN i
+    return SIGN > 0 ? CT<C>::mk(a.y, -a.x) : CT<C>::mk(-a.y, a.x);
   } else {
-    C v0 = xs[jj], v1 = xs[jj + nbf];
-    if (Ns > 1) v1 = c_mul(v1, tw_at<C>(tw, k * span, SIGN));
-    ys[idxD] = c_add(v0, v1);
-    ys[idxD + Ns] = c_sub(v0, v1);
+    constexpr Rt c = (Rt)cos48(q), s = (Rt)(SIGN * sin48(q));
+    return CT<C>::mk(a.x * c - a.y * s, a.x * s + a.y * c);
   }
 }
 
-// In-LDS batched FFT of any length 3^a 4^b 2^c.  nb sequences of length n;
+// In-register DFT of length R (a divisor of 48), natural order in and out:
+// v[k] <- sum_n v[n] exp(SIGN 2 pi i n k / R).  Lengths 2, 3, 4 directly;
+// R = A B by the four-step split n = B n1 + n2, k = k1 + A k2:
+//   A-point DFTs over n1, twiddle exp(SIGN 2 pi i n2 k1 / R), B-point DFTs
+//   over n2.
+template <typename C, int R, int SIGN> __host__ __device__ __forceinline__ void dft(C (&v)[R]) {
+  typedef typename CT<C>::R Rt;
+  static_assert(48 % R == 0, "in-register DFT lengths divide 48");
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
+    const C a = v[0];
+    v[0] = c_add(a, v[1]);
+    v[1] = c_sub(a, v[1]);
+  } else if constexpr (R == 3) {
+    const Rt s3 = (Rt)0.86602540378443864676;  // sqrt(3)/2
+    const C tt = c_add(v[1], v[2]), d = c_sub(v[1], v[2]);
+    const C mm = CT<C>::mk(v[0].x - (Rt)0.5 * tt.x, v[0].y - (Rt)0.5 * tt.y);
+    // e = SIGN i s3 d
+    const C e = SIGN < 0 ? CT<C>::mk(s3 * d.y, -s3 * d.x) : CT<C>::mk(-s3 * d.y, s3 * d.x);
+    v[0] = c_add(v[0], tt);
+    v[1] = c_add(mm, e);
+    v[2] = c_sub(mm, e);
+  } else if constexpr (R == 4) {
+    const C t0 = c_add(v[0], v[2]), t1 = c_sub(v[0], v[2]), t2 = c_add(v[1], v[3]), d = c_sub(v[1], v[3]);
+    const C t3 = SIGN < 0 ? CT<C>::mk(d.y, -d.x) : CT<C>::mk(-d.y, d.x);  // d * (SIGN i)
+    v[0] = c_add(t0, t2);
+    v[1] = c_add(t1, t3);
+    v[2] = c_sub(t0, t2);
+    v[3] = c_sub(t1, t3);
+  } else {
+    constexpr int A = (R % 4 == 0) ? 4 : ((R % 3 == 0) ? 3 : 2);
+    constexpr int B = R / A;
+    C y[R];  // y[n2 * A + k1]
+    sfor<B>([&](auto n2c) __attribute__((always_inline)) {
+      constexpr int n2 = decltype(n2c)::value;
+      C t[A];
+      sfor<A>([&](auto n1c) __attribute__((always_inline)) { t[n1c] = v[B * n1c + n2]; });
+      dft<C, A, SIGN>(t);
+      sfor<A>([&](auto k1c) __attribute__((always_inline)) {
+        constexpr int k1 = decltype(k1c)::value;
+        y[n2 * A + k1] = rot48<C, SIGN, (48 / R) * n2 * k1>(t[k1]);
+      });
+    });
+    sfor<A>([&](auto k1c) __attribute__((always_inline)) {
+      constexpr int k1 = decltype(k1c)::value;
+      C t[B];
+      sfor<B>([&](auto n2c) __attribute__((always_inline)) { t[n2c] = y[n2c * A + k1]; });
+      dft<C, B, SIGN>(t);
+      sfor<B>([&](auto k2c) __attribute__((always_inline)) { v[k1 + A * k2c] = t[k2c]; });
+    });
+  }
+}
+
+// One radix-R item of a Stockham stage (Ns = product of the earlier
+// radices): reads x[jj + r*nbf] of one sequence, applies the twiddles
+// exp(SIGN 2 pi i r k / (Ns R)) of output group k = jj % Ns, transforms in
+// registers and writes y[(jj / Ns) Ns R + k + r*Ns].
+template <typename C, int R, int SIGN>
+__host__ __device__ __forceinline__ void stage_item(const C *xs, C *ys, int jj, int nbf, int Ns, int span,
+                                                    const C *tw) {
+  C v[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) v[r] = xs[jj + r * nbf];
+  const int k = jj % Ns;
+  if (Ns > 1) {
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = c_mul(v[r], tw_at<C>(tw, r * k * span, SIGN));
+  }
+  dft<C, R, SIGN>(v);
+  const int idxD = (jj / Ns) * Ns * R + k;
+#pragma unroll
+  for (int r = 0; r < R; r++) ys[idxD + r * Ns] = v[r];
+}
+
+// The radix of the first stage for a sequence of length n; the later stages
+// follow the same rule on what is left (n / radix).  Two stages up to 256
+// points (96 = 12 x 8, 192 = 12 x 16), three at 384 = 8 x 6 x 8 and 768;
+// lengths outside the table take radices 3, then 4, then 2.
+__host__ __device__ constexpr int plan_radix(int n) {
+  switch (n) {
+    case 2: return 2;
+    case 3: return 3;
+    case 4: return 4;
+    case 6: return 6;
+    case 8: return 8;
+    case 12: return 12;
+    case 16: return 16;
+    case 24: return 6;
+    case 32: return 8;
+    case 48: return 6;
+    case 64: return 8;
+    case 96: return 12;
+    case 128: return 16;
+    case 192: return 12;
+    case 256: return 16;
+    case 384: return 8;
+    case 768: return 12;
+    default: return n % 3 == 0 ? 3 : (n % 4 == 0 ? 4 : (n % 2 == 0 ? 2 : 0));
+  }
+}
+
+// In-LDS batched FFT of any length 3^a 2^b.  nb sequences of length n;
 // element e of sequence b is at x[b*sstr + e].  y is scratch of the same
 // extent, tw the staged twiddle table.  Returns the buffer holding the
 // result (x or y).  Must be called by all threads of the block.
 template <typename C, int SIGN>
 __device__ __forceinline__ C *fft_lds_rt(C *x, C *y, int n, int nb, int sstr, const C *tw) {
   int Ns = 1;
-  for (int st = 0;; st++) {
-    int Rr = fft_radix(n, st);
+  for (int m = n; m > 1;) {
+    const int Rr = plan_radix(m);
     if (Rr == 0) break;
-    int nbf = n / Rr;
-    int span = n / (Ns * Rr);  // twiddle exponent unit
-    int total = nb * nbf;
+    const int nbf = n / Rr;
+    const int span = n / (Ns * Rr);  // twiddle exponent unit
+    const int total = nb * nbf;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
-      int b = t / nbf, jj = t - b * nbf;
-      int k = jj % Ns;
-      int idxD = (jj / Ns) * Ns * Rr + k;
-      if (Rr == 4)
-        butterfly<C, 4, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, k, idxD, span, tw);
-      else if (Rr == 3)
-        butterfly<C, 3, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, k, idxD, span, tw);
-      else
-        butterfly<C, 2, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, k, idxD, span, tw);
+      const int b = t / nbf, jj = t - b * nbf;
+      switch (Rr) {
+        case 16: stage_item<C, 16, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        case 12: stage_item<C, 12, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        case 8: stage_item<C, 8, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        case 6: stage_item<C, 6, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        case 4: stage_item<C, 4, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        case 3: stage_item<C, 3, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        default: stage_item<C, 2, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+      }
     }
     __syncthreads();
     C *tmp = x;
     x = y;
     y = tmp;
     Ns *= Rr;
+    m /= Rr;
   }
   return x;
 }
 
-// The same with the length a compile-time constant N: the radix sequence
-// (3s, then 4s, then a 2) unrolls and every index division is by a constant.
+// The same with the length a compile-time constant N (M points of it left
+// to split): the radix plan unrolls and every index division is by a
+// constant.
 template <typename C, int N, int M, int Ns, int SIGN>
 __device__ __forceinline__ C *fft_ct_run(C *x, C *y, int nb, int sstr, const C *tw) {
   if constexpr (M == 1) {
     return x;
   } else {
-    constexpr int R = (M % 3 == 0) ? 3 : ((M % 4 == 0) ? 4 : 2);
+    constexpr int R = plan_radix(M);
+    static_assert(R > 1 && M % R == 0, "no radix plan");
     constexpr int nbf = N / R;
     constexpr int span = N / (Ns * R);
     const int total = nb * nbf;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
       const int b = t / nbf, jj = t - b * nbf;
-      const int k = jj % Ns;
-      const int idxD = (jj / Ns) * (Ns * R) + k;
-      butterfly<C, R, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, k, idxD, span, tw);
+      stage_item<C, R, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw);
     }
     __syncthreads();
     return fft_ct_run<C, N, M / R, Ns * R, SIGN>(y, x, nb, sstr, tw);
   }
+}
+
+// ------------------------------------------------------------------------
+// Batch-fastest variant for padded layouts.  Consecutive threads take the
+// same item of consecutive sequences, and sequences lie seq_stride() apart,
+// a stride whose residue modulo G = 256 / sizeof(C) (the lanes whose
+// accesses one LDS cycle serves) spreads those lanes over distinct banks.
+// Every read and write of a stage is then conflict-free once nb >= G; the
+// twiddle reads are broadcasts (one address per lane group).
+
+// LDS stride (elements) of nb sequences of length n with elements of es
+// bytes: the smallest S >= n that is an odd multiple of rho = G / p, p the
+// largest power of two <= min(nb, G); then b S mod G, b = 0 .. p-1, are
+// distinct multiples of rho and p consecutive sequences fill the G lanes'
+// banks without overlap
+__host__ __device__ inline int seq_stride(int n, int nb, int es) {
+  const int G = 256 / es;
+  int p = 1;
+  while (2 * p <= nb && 2 * p <= G) p *= 2;
+  if (p == 1) return n;
+  const int rho = G / p;
+  int S = (n + rho - 1) / rho * rho;
+  if ((S / rho) % 2 == 0) S += rho;
+  return S;
+}
+
+template <typename C, int N, int M, int Ns, int SIGN>
+__device__ __forceinline__ C *fft_bf_run(C *x, C *y, int nb, int sstr, const C *tw) {
+  if constexpr (M == 1) {
+    return x;
+  } else {
+    constexpr int R = plan_radix(M);
+    static_assert(R > 1 && M % R == 0, "no radix plan");
+    constexpr int nbf = N / R;
+    constexpr int span = N / (Ns * R);
+    const int total = nb * nbf;
+    Walk2 w(threadIdx.x, blockDim.x, nb);  // t = jj nb + b
+    for (int t = threadIdx.x; t < total; t += blockDim.x, w.next())
+      stage_item<C, R, SIGN>(x + w.lo * sstr, y + w.lo * sstr, w.hi, nbf, Ns, span, tw);
+    __syncthreads();
+    return fft_bf_run<C, N, M / R, Ns * R, SIGN>(y, x, nb, sstr, tw);
+  }
+}
+
+template <typename C, int SIGN>
+__device__ __forceinline__ C *fft_bf_rt(C *x, C *y, int n, int nb, int sstr, const C *tw) {
+  int Ns = 1;
+  for (int m = n; m > 1;) {
+    const int Rr = plan_radix(m);
+    if (Rr == 0) break;
+    const int nbf = n / Rr;
+    const int span = n / (Ns * Rr);
+    const int total = nb * nbf;
+    Walk2 w(threadIdx.x, blockDim.x, nb);  // t = jj nb + b
+    for (int t = threadIdx.x; t < total; t += blockDim.x, w.next()) {
+      const int jj = w.hi, b = w.lo;
+      switch (Rr) {
+        case 16: stage_item<C, 16, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        case 12: stage_item<C, 12, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        case 8: stage_item<C, 8, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        case 6: stage_item<C, 6, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        case 4: stage_item<C, 4, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        case 3: stage_item<C, 3, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+        default: stage_item<C, 2, SIGN>(x + b * sstr, y + b * sstr, jj, nbf, Ns, span, tw); break;
+      }
+    }
+    __syncthreads();
+    C *tmp = x;
+    x = y;
+    y = tmp;
+    Ns *= Rr;
+    m /= Rr;
+  }
+  return x;
+}
+
+template <typename C, int SIGN>
+__device__ __forceinline__ C *fft_bf_s(C *x, C *y, int n, int nb, int sstr, const C *tw) {
+  switch (n) {
+    case 12: return fft_bf_run<C, 12, 12, 1, SIGN>(x, y, nb, sstr, tw);
+    case 24: return fft_bf_run<C, 24, 24, 1, SIGN>(x, y, nb, sstr, tw);
+    case 48: return fft_bf_run<C, 48, 48, 1, SIGN>(x, y, nb, sstr, tw);
+    case 96: return fft_bf_run<C, 96, 96, 1, SIGN>(x, y, nb, sstr, tw);
+    case 192: return fft_bf_run<C, 192, 192, 1, SIGN>(x, y, nb, sstr, tw);
+    case 384: return fft_bf_run<C, 384, 384, 1, SIGN>(x, y, nb, sstr, tw);
+    default: return fft_bf_rt<C, SIGN>(x, y, n, nb, sstr, tw);
+  }
+}
+
+// sign -1 forward, +1 backward (unscaled); sequences sstr = seq_stride(n,
+// nb, sizeof(C)) apart
+template <typename C>
+__device__ __forceinline__ C *fft_lds_bf(C *x, C *y, int n, int nb, int sstr, int sign, const C *tw) {
+  return sign < 0 ? fft_bf_s<C, -1>(x, y, n, nb, sstr, tw) : fft_bf_s<C, +1>(x, y, n, nb, sstr, tw);
 }
 
 // Dispatch on the lengths the solver meets (Ngal = 3*2^k or 2^k); other

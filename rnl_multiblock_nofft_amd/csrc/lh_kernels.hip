@@ -73,9 +73,11 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
   if (i0 >= nxh) return;
   const int ni = min(TI, nxh - i0);
   extern __shared__ double2 smem_d[];
+  // ni sequences of Ng2 points, S apart (padded: conflict-free stages)
+  const int S = seq_stride(Ng2, TI, sizeof(C));
   C *A = reinterpret_cast<C *>(smem_d);
-  C *B = A + TI * Ng2;
-  C *TW = B + TI * Ng2;
+  C *B = A + TI * S;
+  C *TW = B + TI * S;
   const bool isU = slot < ch.nplu;
   const int j = isU ? slot + 1 : slot - ch.nplu + 1;
 
@@ -83,10 +85,8 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
   // only the zero-padding rows z = N2/2 .. Ngal2-N2/2-1 need clearing
   {
     const int half0 = ch.N2 / 2, npad = Ng2 - ch.N2;
-    for (int t = threadIdx.x; t < ni * npad; t += blockDim.x) {
-      int ii = t / npad, z = half0 + (t - ii * npad);
-      A[ii * Ng2 + z] = CT<C>::mk(0, 0);
-    }
+    Walk2 w(threadIdx.x, blockDim.x, npad);  // t = ii npad + (z - half0)
+    for (int t = threadIdx.x; t < ni * npad; t += blockDim.x, w.next()) A[w.hi * S + half0 + w.lo] = CT<C>::mk(0, 0);
   }
   // interpolation weights of this plane
   double wa = 0, wb = 0, wd = 1;
@@ -114,8 +114,9 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
     }
   }
   const int half = N2 / 2, dz = Ng2 - N2;
-  for (int t = threadIdx.x; t < N2 * ni; t += blockDim.x) {
-    int k = t / ni, ii = t - k * ni;
+  Walk2 wl(threadIdx.x, blockDim.x, ni);  // t = k ni + ii
+  for (int t = threadIdx.x; t < N2 * ni; t += blockDim.x, wl.next()) {
+    const int k = wl.hi, ii = wl.lo;
     int col = k * nxh + i0 + ii;
     double2 v = s0[col];
     if (s1) {
@@ -123,15 +124,14 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
       v = make_double2((wa * v.x + wb * w.x) / wd, (wa * v.y + wb * w.y) / wd);
     }
     int z = k < half ? k : k + dz;
-    A[ii * Ng2 + z] = from_d<C>(v);
+    A[ii * S + z] = from_d<C>(v);
   }
   __syncthreads();
-  C *R = fft_lds<C>(A, B, Ng2, ni, Ng2, +1, TW);
+  C *R = fft_lds_bf<C>(A, B, Ng2, ni, S, +1, TW);
   C *hz = Inter<C>::hz(v) + ((size_t)slot * 3 + f) * Ng2 * nxp;
-  for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) {
-    int z = t / ni, ii = t - z * ni;
-    hz[(size_t)z * nxp + i0 + ii] = R[ii * Ng2 + z];
-  }
+  Walk2 ws(threadIdx.x, blockDim.x, ni);  // t = z ni + ii
+  for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x, ws.next())
+    hz[(size_t)ws.hi * nxp + i0 + ws.lo] = R[ws.lo * S + ws.hi];
 }
 
 // ------------------------------------------------------------------------
@@ -150,17 +150,20 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   if (z0 >= Ng2) return;
   const int nz = min(TZ, Ng2 - z0);
   extern __shared__ double2 smem_d[];
+  // 2 nz sequences of Ng1 points, S apart (padded: conflict-free stages)
+  const int S = seq_stride(Ng1, 2 * TZ, sizeof(C));
   C *A = reinterpret_cast<C *>(smem_d);
-  C *B = A + 2 * TZ * Ng1;
-  C *TW = B + 2 * TZ * Ng1;
+  C *B = A + 2 * TZ * S;
+  C *TW = B + 2 * TZ * S;
   stage_twiddles<C>(TW, v.twx, Ng1);
   const bool isU = slot < ch.nplu;
   const C *hz = Inter<C>::hz(v) + (size_t)slot * 3 * Ng2 * nxp;
 
   // Hermitian extension of the three half spectra, packed two per sequence:
   // seq 2zz = f0 + i f2, seq 2zz+1 = f1
-  for (int t = threadIdx.x; t < nz * Ng1; t += blockDim.x) {
-    int zz = t / Ng1, m = t - zz * Ng1;
+  Walk2 wh(threadIdx.x, blockDim.x, Ng1);  // t = zz Ng1 + m
+  for (int t = threadIdx.x; t < nz * Ng1; t += blockDim.x, wh.next()) {
+    const int zz = wh.hi, m = wh.lo;
     int z = z0 + zz;
     C F0 = CT<C>::mk(0, 0), F1 = F0, F2 = F0;
     if (m <= h1) {
@@ -181,17 +184,18 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
       F1.y = -F1.y;
       F2.y = -F2.y;
     }
-    A[(2 * zz) * Ng1 + m] = CT<C>::mk(F0.x - F2.y, F0.y + F2.x);
-    A[(2 * zz + 1) * Ng1 + m] = F1;
+    A[(2 * zz) * S + m] = CT<C>::mk(F0.x - F2.y, F0.y + F2.x);
+    A[(2 * zz + 1) * S + m] = F1;
   }
   __syncthreads();
-  C *X = fft_lds<C>(A, B, Ng1, 2 * nz, Ng1, +1, TW);
+  C *X = fft_lds_bf<C>(A, B, Ng1, 2 * nz, S, +1, TW);
   // CFL maxima of dtc_calc (FOU3D.f90:699-736), at the first substep only:
   // |u1|, |u3| on u-planes, |u2| on v-planes (times dthdyv(j) in k_dt_update)
   R_t ma = 0, mb = 0;
-  for (int t = threadIdx.x; t < nz * Ng1; t += blockDim.x) {
-    int zz = t / Ng1, n = t - zz * Ng1;
-    C s0 = X[(2 * zz) * Ng1 + n], s1 = X[(2 * zz + 1) * Ng1 + n];
+  Walk2 wp(threadIdx.x, blockDim.x, Ng1);  // t = zz Ng1 + n
+  for (int t = threadIdx.x; t < nz * Ng1; t += blockDim.x, wp.next()) {
+    const int zz = wp.hi, n = wp.lo;
+    C s0 = X[(2 * zz) * S + n], s1 = X[(2 * zz + 1) * S + n];
     R_t a1 = s0.x, a3 = s0.y, a2 = s1.x;
     if (CFL) {
       if (isU) {
@@ -202,11 +206,11 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
       }
     }
     if (isU) {
-      X[(2 * zz) * Ng1 + n] = CT<C>::mk(a1 * a1, a1 * a3);      // uu + i uw
-      X[(2 * zz + 1) * Ng1 + n] = CT<C>::mk(a2 * a2, a3 * a3);  // vv + i ww
+      X[(2 * zz) * S + n] = CT<C>::mk(a1 * a1, a1 * a3);      // uu + i uw
+      X[(2 * zz + 1) * S + n] = CT<C>::mk(a2 * a2, a3 * a3);  // vv + i ww
     } else {
-      X[(2 * zz) * Ng1 + n] = CT<C>::mk(a1 * a2, a3 * a2);      // uv + i wv
-      X[(2 * zz + 1) * Ng1 + n] = CT<C>::mk(0, 0);
+      X[(2 * zz) * S + n] = CT<C>::mk(a1 * a2, a3 * a2);      // uv + i wv
+      X[(2 * zz + 1) * S + n] = CT<C>::mk(0, 0);
     }
   }
   __syncthreads();
@@ -224,7 +228,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
       red[1][threadIdx.x >> 6] = b;
     }
   }
-  C *Y = fft_lds<C>(X, X == A ? B : A, Ng1, 2 * nz, Ng1, -1, TW);
+  C *Y = fft_lds_bf<C>(X, X == A ? B : A, Ng1, 2 * nz, S, -1, TW);
   if (CFL && threadIdx.x == 0) {
     const int nw = (blockDim.x + 63) >> 6;
     double a = red[0][0], b = red[1][0];
@@ -239,11 +243,11 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   const int nseq = isU ? 2 : 1;
   const R_t inv = (R_t)(1.0 / Ng1);
   C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxp;
-  for (int t = threadIdx.x; t < nz * nseq * (h1 + 1); t += blockDim.x) {
-    int m = t % (h1 + 1);
-    int r = t / (h1 + 1);
-    int q = r % nseq, zz = r / nseq;
-    const C *ys = Y + (2 * zz + q) * Ng1;
+  Walk2 wo(threadIdx.x, blockDim.x, h1 + 1);  // t = (zz nseq + q) (h1 + 1) + m
+  for (int t = threadIdx.x; t < nz * nseq * (h1 + 1); t += blockDim.x, wo.next()) {
+    const int m = wo.lo, r = wo.hi;
+    const int q = nseq == 2 ? (r & 1) : 0, zz = nseq == 2 ? (r >> 1) : r;
+    const C *ys = Y + (2 * zz + q) * S;
     C Ym = ys[m], Yc = ys[m == 0 ? 0 : Ng1 - m];
     C P, Q;
     if (m == 0) {
@@ -278,34 +282,36 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
   const int np = isU ? 4 : 2;
   const int j = isU ? slot + 1 : slot - ch.nplu + 1;
   extern __shared__ double2 smem_d[];
+  // np ni sequences of Ng2 points, S apart (padded: conflict-free stages)
+  const int S = seq_stride(Ng2, 4 * TI, sizeof(C));
   C *A = reinterpret_cast<C *>(smem_d);
-  C *B = A + 4 * TI * Ng2;
-  C *TW = B + 4 * TI * Ng2;
+  C *B = A + 4 * TI * S;
+  C *TW = B + 4 * TI * S;
   stage_twiddles<C>(TW, v.twz, Ng2);
   const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxp;
-  for (int t = threadIdx.x; t < np * ni * Ng2; t += blockDim.x) {
-    int p = t / (ni * Ng2);
-    int rem = t - p * ni * Ng2;
-    int z = rem / ni, ii = rem - z * ni;
-    A[(p * ni + ii) * Ng2 + z] = gx[((size_t)p * Ng2 + z) * nxp + i0 + ii];
+  Walk3 wl(threadIdx.x, blockDim.x, ni, Ng2);  // t = (p Ng2 + z) ni + ii
+  for (int t = threadIdx.x; t < np * ni * Ng2; t += blockDim.x, wl.next()) {
+    const int p = wl.hi, z = wl.mid, ii = wl.lo;
+    A[(p * ni + ii) * S + z] = gx[((size_t)p * Ng2 + z) * nxp + i0 + ii];
   }
   __syncthreads();
-  C *R = fft_lds<C>(A, B, Ng2, np * ni, Ng2, -1, TW);
+  C *R = fft_lds_bf<C>(A, B, Ng2, np * ni, S, -1, TW);
   const double inv = 1.0 / Ng2;
   const int half = N2 / 2, dz = Ng2 - N2;
-  for (int t = threadIdx.x; t < ni * N2; t += blockDim.x) {
-    int k = t / ni, ii = t - k * ni;
+  Walk2 wo(threadIdx.x, blockDim.x, ni);  // t = k ni + ii
+  for (int t = threadIdx.x; t < ni * N2; t += blockDim.x, wo.next()) {
+    const int k = wo.hi, ii = wo.lo;
     int i = i0 + ii;
     int col = k * nxh + i;
     int z = k < half ? k : k + dz;
     bool live = k != half;  // kz = -N2/2 is zeroed by der_x/der_z
     double kxi = v.kx[i], kzk = v.kz[k];
     size_t o = (size_t)j * ldc + col;
-    double2 X0 = cscale(inv, to_d(R[(0 * ni + ii) * Ng2 + z]));
-    double2 X1 = cscale(inv, to_d(R[(1 * ni + ii) * Ng2 + z]));
+    double2 X0 = cscale(inv, to_d(R[(0 * ni + ii) * S + z]));
+    double2 X1 = cscale(inv, to_d(R[(1 * ni + ii) * S + z]));
     if (isU) {
-      double2 X2 = cscale(inv, to_d(R[(2 * ni + ii) * Ng2 + z]));
-      double2 X3 = cscale(inv, to_d(R[(3 * ni + ii) * Ng2 + z]));
+      double2 X2 = cscale(inv, to_d(R[(2 * ni + ii) * S + z]));
+      double2 X3 = cscale(inv, to_d(R[(3 * ni + ii) * S + z]));
       v.nl1[o] = live ? cadd(cimul(kxi, X0), cimul(kzk, X1)) : make_double2(0, 0);
       v.nl3[o] = live ? cadd(cimul(kxi, X1), cimul(kzk, X3)) : make_double2(0, 0);
       v.fvv[o] = X2;
@@ -1351,11 +1357,15 @@ hipError_t launch_u11(char *arena, const Chan *d_chans, const ShapeMax &m, hipSt
 
 // ------------------------------------------------------------------------
 // launch wrappers
-// LDS of the three stages, in complex elements: two ping-pong buffers and
-// the twiddle table
-static size_t lds_zinv(int E, int Ng2, int nxh) { return 2 * (size_t)tile_of8(E, Ng2, nxh) * Ng2 + Ng2; }
-static size_t lds_xprod(int E, int Ng1, int Ng2) { return 2 * 2 * (size_t)tile_of(E, Ng1, Ng2) * Ng1 + Ng1; }
-static size_t lds_zfwd(int E, int Ng2, int nxh) { return 2 * 4 * (size_t)tile_of8(E, Ng2, nxh) * Ng2 + Ng2; }
+// LDS of the three stages, in complex elements: two ping-pong buffers of
+// nb padded sequences (the padded stride of either element size) and the
+// twiddle table
+static size_t seq_buf(int n, int nb) {
+  return (size_t)nb * std::max(seq_stride(n, nb, sizeof(double2)), seq_stride(n, nb, sizeof(float2)));
+}
+static size_t lds_zinv(int E, int Ng2, int nxh) { return 2 * seq_buf(Ng2, tile_of8(E, Ng2, nxh)) + Ng2; }
+static size_t lds_xprod(int E, int Ng1, int Ng2) { return 2 * seq_buf(Ng1, 2 * tile_of(E, Ng1, Ng2)) + Ng1; }
+static size_t lds_zfwd(int E, int Ng2, int nxh) { return 2 * seq_buf(Ng2, 4 * tile_of8(E, Ng2, nxh)) + Ng2; }
 
 // tile counts and LDS sizes for the current targets, over every channel
 void shape_tiles(ShapeMax &m) {
