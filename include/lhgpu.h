@@ -212,6 +212,12 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
  * x-mode with z transforms only; 0 forces the two-dimensional wall
  * transforms for every textured channel.  Results agree to rounding. */
 #define LH_TUNE_TEX_SEPARABLE 5
+/* LH_TUNE_SOLVE_SCAN (default 1): in the LDS form of the column solve, each
+ * Thomas recurrence (forward elimination, back substitution) is split over
+ * a group of lanes -- per-lane chunks composed as affine maps and scanned
+ * across the group, then each chunk rerun from its true start value; 0 runs
+ * every recurrence on one lane.  Results agree to rounding. */
+#define LH_TUNE_SOLVE_SCAN 6
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

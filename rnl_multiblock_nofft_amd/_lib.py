@@ -19,6 +19,7 @@ FIELDS = {"u1": LH_U1, "u2": LH_U2, "u3": LH_U3, "p": LH_P, "Nu1": LH_NU1,
 PHASE_NONLINEAR, PHASE_RHS, PHASE_SOLVE = 1, 2, 3
 TUNE_SOLVE_COLS, TUNE_ZINV_POINTS, TUNE_XPROD_POINTS, TUNE_ZFWD_POINTS = 1, 2, 3, 4
 TUNE_TEX_SEPARABLE = 5
+TUNE_SOLVE_SCAN = 6
 PREC_FP64, PREC_MIXED = 0, 1
 
 # every symbol include/lhgpu.h declares

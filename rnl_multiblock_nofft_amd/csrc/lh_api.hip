@@ -1264,6 +1264,13 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     drop_graph(x);
     return LH_OK;
   }
+  if (key == LH_TUNE_SOLVE_SCAN) {
+    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_SOLVE_SCAN is 0 or 1");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->shape.solve_scan = value;
+    drop_graph(x);
+    return LH_OK;
+  }
   if (key == LH_TUNE_SOLVE_COLS) {
     if (!solve_cols_ok(x, value))
       return set_err(LH_EINVAL,
@@ -1297,6 +1304,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
   }
   if (key == LH_TUNE_SOLVE_COLS) {
     *value = x->shape.solve_cols;
+    return LH_OK;
+  }
+  if (key == LH_TUNE_SOLVE_SCAN) {
+    *value = x->shape.solve_scan;
     return LH_OK;
   }
   int *k = stage_knob(x->shape, key);
@@ -1364,18 +1375,23 @@ int lh_autotune(lh_ctx *x, int reps) {
     shape_tiles(best);
   }
   // column solve: one lane per column, or C columns per workgroup in LDS
+  // with serial or lane-split recurrences
   {
     const int cand[] = {0, 1, 2, 3, 4, 5, 6, 8, 10};
     float best_ms = 1e30f;
     for (int C : cand) {
       if (!solve_cols_ok(x, C)) continue;
-      ShapeMax m = best;
-      m.solve_cols = C;
-      float ms = time_launches(x, e0, e1, reps,
-                               [&](int r) { return launch_solve(x->arena, x->d_chans, m, 1 + r % 3, x->stream); });
-      if (ms >= 0.f && ms < best_ms) {
-        best_ms = ms;
-        best.solve_cols = C;
+      for (int sc = 0; sc <= (C > 0 ? 1 : 0); sc++) {
+        ShapeMax m = best;
+        m.solve_cols = C;
+        m.solve_scan = sc;
+        float ms = time_launches(x, e0, e1, reps,
+                                 [&](int r) { return launch_solve(x->arena, x->d_chans, m, 1 + r % 3, x->stream); });
+        if (ms >= 0.f && ms < best_ms) {
+          best_ms = ms;
+          best.solve_cols = C;
+          best.solve_scan = sc;
+        }
       }
     }
   }

@@ -19,6 +19,7 @@ struct ShapeMax {
   int zinv_tiles = 0, xprod_tiles = 0, zfwd_tiles = 0;
   size_t lds_zinv = 0, lds_xprod = 0, lds_zfwd = 0;  // in complex elements
   int solve_cols = 0;  // columns per workgroup of the LDS column solve (0: one lane per column)
+  int solve_scan = 1;  // LDS column solve: recurrences split over lane groups (0: one lane per sequence)
   int nphys = 0;       // N1*N2, largest over channels (texture mask kernels)
 };
 void shape_tiles(ShapeMax &m);

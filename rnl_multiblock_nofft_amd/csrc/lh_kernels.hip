@@ -816,7 +816,63 @@ __host__ __device__ size_t solve_lds_bytes(int nyu, int C) {
 #define SB2 8
 #define LB 8
 
-__global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans, int kRK, int C) {
+// One real first-order recurrence of a Thomas sweep, split over the L lanes
+// k = 0 .. L-1 of an aligned lane group (L a power of two):
+//   FORM 0: z_j = (r_j - a_j z_prev) d_j      FORM 1: z_j = r_j - (a_j d_j) z_prev
+// over rows j = first + dir i, i = 0 .. count-1, in place in X (row j at
+// X[j xs]; a_j at A[j], d_j at D[j ds]); zin is the value before the first
+// row.  Each lane owns a contiguous chunk of rows: it composes the chunk's
+// affine map z -> alpha z + beta (alpha = -a d), the group scans the maps
+// (log2 L shuffle steps), and each lane reruns its chunk from its true start
+// value, so every stored row comes from the same operations as the serial
+// sweep and only the chunk start values carry the scan's rounding.  With
+// serial set (or L = 1), lane 0 runs the whole sweep alone.  Returns the
+// last row's value on every lane of the group.
+template <int FORM>
+__device__ __forceinline__ double rec_step(double z, double r, double a, double d) {
+  return FORM == 0 ? (r - a * z) * d : r - (a * d) * z;
+}
+template <int FORM>
+__device__ double rec_sweep(double *X, int xs, const double *A, const double *D, int ds, int first, int dir,
+                            int count, double zin, int k, int L, bool serial) {
+  serial = serial || L == 1;
+  const int cs = serial ? count : (count + L - 1) / L;
+  const int b = min(count, k * cs), e = min(count, b + cs);
+  double z = zin;
+  if (!serial) {
+    double Am = 1.0, Bm = 0.0;
+    for (int i = b; i < e; i++) {
+      const int j = first + dir * i;
+      const double a = A[j], d = D[j * ds];
+      Bm = rec_step<FORM>(Bm, X[j * xs], a, d);
+      Am = FORM == 0 ? (0.0 - a * Am) * d : 0.0 - (a * d) * Am;
+    }
+    for (int o = 1; o < L; o <<= 1) {
+      const double Ap = __shfl_up(Am, o, L), Bp = __shfl_up(Bm, o, L);
+      if (k >= o) {
+        Bm = Am * Bp + Bm;
+        Am = Am * Ap;
+      }
+    }
+    const double Ai = __shfl_up(Am, 1, L), Bi = __shfl_up(Bm, 1, L);
+    z = k == 0 ? zin : Ai * zin + Bi;
+  }
+  for (int i = b; i < e; i++) {
+    const int j = first + dir * i;
+    z = rec_step<FORM>(z, X[j * xs], A[j], D[j * ds]);
+    X[j * xs] = z;
+  }
+  return __shfl(z, serial ? 0 : L - 1, L);
+}
+
+// lanes per sequence: the largest power of two with nseq * L <= 64
+__host__ __device__ inline int lanes_per_seq(int nseq) {
+  int L = 1;
+  while (nseq * 2 * L <= 64) L *= 2;
+  return L;
+}
+
+__global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans, int kRK, int C, int scan) {
   const Chan ch = chans[blockIdx.z];
   const View v(arena, ch);
   const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu, nyv = ch.nyv, ncol = ch.ncol;
@@ -937,8 +993,26 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
   __syncthreads();
 
   // ---- H: Helmholtz forward and back substitution, one real sequence per
-  //      lane, rows read SB2 ahead into registers
-  {
+  //      lane (rows read SB2 ahead into registers), or, with scan, split over
+  //      the lanes of a group (rec_sweep)
+  if (scan) {
+    const int L = lanes_per_seq(6 * C), sq = lane / L, k = lane % L;
+    const int role = sq / C, cl = sq % C;
+    if (role < 6 && cl < ncl) {
+      const int sys = role >> 1, part = role & 1;
+      double *X = reinterpret_cast<double *>(sys == 0 ? X1 : (sys == 1 ? X3 : X2)) + 2 * cl + part;
+      const double *D = (sys == 2 ? DV : DU) + cl;
+      const double *A1 = sys == 2 ? A1V : A1U, *A3 = sys == 2 ? A3V : A3U;
+      const int n = sys == 2 ? nyv : nyu;
+      const int C2 = 2 * C;
+      // forward: rows 1 .. n-1 from y0 = r0 (row 0: a(2,0) = 1)
+      const double y = rec_sweep<0>(X, C2, A1, D, C, 1, 1, n - 1, X[0], k, L, false);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      // back substitution: rows n-2 .. 1 from x(n-1) = y(n-1); c_j = a3_j d_j
+      const double x = rec_sweep<1>(X, C2, A3, D, C, n - 2, -1, n - 2, y, k, L, false);
+      if (k == 0) X[0] = X[0] - (sys == 2 ? 0.0 : ch.gw1) * x;
+    }
+  } else {
     const int cl = lane % C, role = lane / C;
     if (role < 6 && cl < ncl) {
       const int sys = role >> 1, part = role & 1;
@@ -1024,7 +1098,26 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
       g_sh = g;
     }
   }
-  {
+  if (scan) {
+    const int L = lanes_per_seq(2 * C), sq = lane / L, k = lane % L;
+    const int role = sq / C, cl = sq % C;
+    const bool col0 = c0 + cl == 0;
+    if (role < 2 && cl < ncl) {
+      double *Y = reinterpret_cast<double *>(Ps) + 2 * cl + role;
+      const double *D = DP + cl;
+      const int C2 = 2 * C;
+      // elimination from the top: rows ne .. 2 from 0 (the mean mode too: its
+      // Neumann multipliers are 1, a prefix sum that the scan reassociates)
+      const double yp = rec_sweep<0>(Y, C2, A3P, D, C, ne, -1, ne - 1, 0.0, k, L, false);
+      // row ns = 1; the mean mode's row is pinned (a3 = 0, rhs = 0)
+      const double d1 = D[C];
+      const double y1 = col0 ? (0.0 - 0.0 * yp) * d1 : (Y[C2] - A3P[1] * yp) * d1;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (k == 0) Y[C2] = y1;
+      // back substitution: rows 2 .. ne from psi(1)
+      rec_sweep<1>(Y, C2, A1P, D, C, 2, 1, ne - 1, y1, k, L, false);
+    }
+  } else {
     const int cl = lane % C, role = lane / C;
     const bool col0 = c0 + cl == 0;
     if (role < 2 && cl < ncl) {
@@ -1451,7 +1544,8 @@ hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int
   } else {
     const int ngrp = (m.ncol + C - 1) / C;
     dim3 g((ngrp + 7) / 8 * 8, 1, m.nchan);
-    hipLaunchKernelGGL(k_solve_lds, g, dim3(64), solve_lds_bytes(m.nyu, C), s, arena, d_chans, kRK, C);
+    hipLaunchKernelGGL(k_solve_lds, g, dim3(64), solve_lds_bytes(m.nyu, C), s, arena, d_chans, kRK, C,
+                       m.solve_scan);
   }
   return hipGetLastError();
 }

@@ -192,7 +192,8 @@ class Solver:
         return {"zinv_points": self.get_tuning(_lib.TUNE_ZINV_POINTS),
                 "xprod_points": self.get_tuning(_lib.TUNE_XPROD_POINTS),
                 "zfwd_points": self.get_tuning(_lib.TUNE_ZFWD_POINTS),
-                "solve_cols": self.get_tuning(_lib.TUNE_SOLVE_COLS)}
+                "solve_cols": self.get_tuning(_lib.TUNE_SOLVE_COLS),
+                "solve_scan": self.get_tuning(_lib.TUNE_SOLVE_SCAN)}
 
     def sync(self):
         check(self.L.lh_sync(self.h))
