@@ -218,6 +218,10 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
  * across the group, then each chunk rerun from its true start value; 0 runs
  * every recurrence on one lane.  Results agree to rounding. */
 #define LH_TUNE_SOLVE_SCAN 6
+/* LH_TUNE_SOLVE_WAVES (default 1): waves per workgroup of the LDS column
+ * solve (1, 2 or 4); more waves give each staged column more lanes for its
+ * recurrences and more loads in flight. */
+#define LH_TUNE_SOLVE_WAVES 7
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

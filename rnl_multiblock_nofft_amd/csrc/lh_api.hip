@@ -1264,6 +1264,14 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     drop_graph(x);
     return LH_OK;
   }
+  if (key == LH_TUNE_SOLVE_WAVES) {
+    if (value != 1 && value != 2 && value != 4)
+      return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_SOLVE_WAVES is 1, 2 or 4");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->shape.solve_waves = value;
+    drop_graph(x);
+    return LH_OK;
+  }
   if (key == LH_TUNE_SOLVE_SCAN) {
     if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_SOLVE_SCAN is 0 or 1");
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1308,6 +1316,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
   }
   if (key == LH_TUNE_SOLVE_SCAN) {
     *value = x->shape.solve_scan;
+    return LH_OK;
+  }
+  if (key == LH_TUNE_SOLVE_WAVES) {
+    *value = x->shape.solve_waves;
     return LH_OK;
   }
   int *k = stage_knob(x->shape, key);
@@ -1381,18 +1393,21 @@ int lh_autotune(lh_ctx *x, int reps) {
     float best_ms = 1e30f;
     for (int C : cand) {
       if (!solve_cols_ok(x, C)) continue;
-      for (int sc = 0; sc <= (C > 0 ? 1 : 0); sc++) {
-        ShapeMax m = best;
-        m.solve_cols = C;
-        m.solve_scan = sc;
-        float ms = time_launches(x, e0, e1, reps,
-                                 [&](int r) { return launch_solve(x->arena, x->d_chans, m, 1 + r % 3, x->stream); });
-        if (ms >= 0.f && ms < best_ms) {
-          best_ms = ms;
-          best.solve_cols = C;
-          best.solve_scan = sc;
+      for (int sc = 0; sc <= (C > 0 ? 1 : 0); sc++)
+        for (int nw = 1; nw <= (C > 0 ? 4 : 1); nw *= 2) {
+          ShapeMax m = best;
+          m.solve_cols = C;
+          m.solve_scan = sc;
+          m.solve_waves = nw;
+          float ms = time_launches(x, e0, e1, reps,
+                                   [&](int r) { return launch_solve(x->arena, x->d_chans, m, 1 + r % 3, x->stream); });
+          if (ms >= 0.f && ms < best_ms) {
+            best_ms = ms;
+            best.solve_cols = C;
+            best.solve_scan = sc;
+            best.solve_waves = nw;
+          }
         }
-      }
     }
   }
   (void)hipEventDestroy(e0);

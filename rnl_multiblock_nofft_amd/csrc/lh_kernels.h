@@ -20,6 +20,7 @@ struct ShapeMax {
   size_t lds_zinv = 0, lds_xprod = 0, lds_zfwd = 0;  // in complex elements
   int solve_cols = 0;  // columns per workgroup of the LDS column solve (0: one lane per column)
   int solve_scan = 1;  // LDS column solve: recurrences split over lane groups (0: one lane per sequence)
+  int solve_waves = 1;  // LDS column solve: waves per workgroup (1, 2 or 4)
   int nphys = 0;       // N1*N2, largest over channels (texture mask kernels)
 };
 void shape_tiles(ShapeMax &m);

@@ -865,14 +865,15 @@ __device__ double rec_sweep(double *X, int xs, const double *A, const double *D,
   return __shfl(z, serial ? 0 : L - 1, L);
 }
 
-// lanes per sequence: the largest power of two with nseq * L <= 64
-__host__ __device__ inline int lanes_per_seq(int nseq) {
+// lanes per sequence: the largest power of two L <= 64 with nseq * L <= nt
+// (nt threads in the workgroup; a group never straddles two waves)
+__host__ __device__ inline int lanes_per_seq(int nseq, int nt) {
   int L = 1;
-  while (nseq * 2 * L <= 64) L *= 2;
+  while (nseq * 2 * L <= nt && 2 * L <= 64) L *= 2;
   return L;
 }
 
-__global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans, int kRK, int C, int scan) {
+__global__ void __launch_bounds__(256) k_solve_lds(char *arena, const Chan *chans, int kRK, int C, int scan) {
   const Chan ch = chans[blockIdx.z];
   const View v(arena, ch);
   const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu, nyv = ch.nyv, ncol = ch.ncol;
@@ -884,7 +885,7 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
   const int grp = (blockIdx.x % 8) * per + blockIdx.x / 8;
   const int c0 = grp * C;
   if (c0 >= ncol) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x, NT = blockDim.x;  // 1, 2 or 4 waves
   const size_t plane = (size_t)(nyu + PADROWS) * ldc;
 
   extern __shared__ double2 smem_d[];
@@ -909,12 +910,12 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
   //      issued LB iterations at a time so that they are all in flight
   {
     const int nU = nyu * C, nV = nyv * C;
-    for (int t0 = 0; t0 < nU; t0 += 64 * LB) {
+    for (int t0 = 0; t0 < nU; t0 += NT * LB) {
       double2 a[LB], b[LB];
       double d[LB], e[LB];
 #pragma unroll
       for (int u = 0; u < LB; u++) {
-        const int t = min(t0 + u * 64 + lane, nU - 1);
+        const int t = min(t0 + u * NT + lane, nU - 1);
         const int j = t / C, c = min(c0 + (t - j * C), ncol - 1);
         const size_t o = (size_t)j * ldc + c;
         a[u] = v.r1[o];
@@ -924,7 +925,7 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
       }
 #pragma unroll
       for (int u = 0; u < LB; u++) {
-        const int t = t0 + u * 64 + lane;
+        const int t = t0 + u * NT + lane;
         if (t < nU) {
           X1[t] = a[u];
           X3[t] = b[u];
@@ -933,12 +934,12 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
         }
       }
     }
-    for (int t0 = 0; t0 < nV; t0 += 64 * LB) {
+    for (int t0 = 0; t0 < nV; t0 += NT * LB) {
       double2 a[LB];
       double d[LB];
 #pragma unroll
       for (int u = 0; u < LB; u++) {
-        const int t = min(t0 + u * 64 + lane, nV - 1);
+        const int t = min(t0 + u * NT + lane, nV - 1);
         const int j = t / C, c = min(c0 + (t - j * C), ncol - 1);
         const size_t o = (size_t)j * ldc + c;
         a[u] = v.r2[o];
@@ -946,7 +947,7 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
       }
 #pragma unroll
       for (int u = 0; u < LB; u++) {
-        const int t = t0 + u * 64 + lane;
+        const int t = t0 + u * NT + lane;
         if (t < nV) {
           X2[t] = a[u];
           DV[t] = d[u];
@@ -954,11 +955,11 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
       }
     }
   }
-  for (int j0 = 0; j0 < nyu; j0 += 256) {
+  for (int j0 = 0; j0 < nyu; j0 += 4 * NT) {
     double c0u[4], c2u[4], c0v[4], c2v[4], hu[4], hv[4], hvm[4], yvj[4], yvm[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int j = min(j0 + u * 64 + lane, nyu - 1), jv = min(j, nyv - 1), jm = max(min(j - 1, nyv - 1), 0);
+      const int j = min(j0 + u * NT + lane, nyu - 1), jv = min(j, nyv - 1), jm = max(min(j - 1, nyv - 1), 0);
       c0u[u] = v.dyu2i[3 * j];
       c2u[u] = v.dyu2i[3 * j + 2];
       c0v[u] = v.dyv2i[3 * jv];
@@ -971,7 +972,7 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int j = j0 + u * 64 + lane;
+      const int j = j0 + u * NT + lane;
       if (j >= nyu) break;
       const bool iu = j >= 1 && j <= nyu - 2, iv = j >= 1 && j <= nyv - 2;
       A1U[j] = iu ? -beta * c0u[u] : ch.gw2;
@@ -996,7 +997,7 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
   //      lane (rows read SB2 ahead into registers), or, with scan, split over
   //      the lanes of a group (rec_sweep)
   if (scan) {
-    const int L = lanes_per_seq(6 * C), sq = lane / L, k = lane % L;
+    const int L = lanes_per_seq(6 * C, NT), sq = lane / L, k = lane % L;
     const int role = sq / C, cl = sq % C;
     if (role < 6 && cl < ncl) {
       const int sys = role >> 1, part = role & 1;
@@ -1068,7 +1069,7 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
   __syncthreads();
 
   // ---- R: Poisson right-hand side on pressure rows 1..ne
-  for (int t = lane; t < ne * C; t += 64) {
+  for (int t = lane; t < ne * C; t += NT) {
     const int q = 1 + t / C, cl = t % C;
     const double2 x1 = X1[q * C + cl], x3 = X3[q * C + cl], x2 = X2[q * C + cl], n2 = X2[(q - 1) * C + cl];
     const double2 a = cimul(KX[cl], x1), b = cimul(KZ[cl], x3);
@@ -1082,7 +1083,7 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
   //      and flowrate_corr's g; Poisson elimination (top down) and back
   //      substitution
   __shared__ double g_sh;
-  if (c0 == 0) {
+  if (c0 == 0 && lane < 64) {  // one wave
     const double *x1r = reinterpret_cast<const double *>(X1);  // column 0, real part
     double part = 0.0;
     for (int r = 1 + lane; r <= ne; r += 64) part = part + DYV[r] * x1r[r * 2 * C];
@@ -1099,7 +1100,7 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
     }
   }
   if (scan) {
-    const int L = lanes_per_seq(2 * C), sq = lane / L, k = lane % L;
+    const int L = lanes_per_seq(2 * C, NT), sq = lane / L, k = lane % L;
     const int role = sq / C, cl = sq % C;
     const bool col0 = c0 + cl == 0;
     if (role < 2 && cl < ncl) {
@@ -1179,17 +1180,17 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
   const double *__restrict__ u11 = v.u11 + (size_t)(kRK - 1) * nyu;
   const bool has0 = c0 == 0;
   const double g = has0 ? g_sh : 0.0;
-  for (int t0 = 0; t0 < ne * C; t0 += 64 * LB) {
+  for (int t0 = 0; t0 < ne * C; t0 += NT * LB) {
     double2 P[LB];
 #pragma unroll
     for (int u = 0; u < LB; u++) {
-      const int t = min(t0 + u * 64 + lane, ne * C - 1);
+      const int t = min(t0 + u * NT + lane, ne * C - 1);
       const int q = 1 + t / C, c = min(c0 + t % C, ncol - 1);
       P[u] = v.p[(size_t)q * ldc + c];
     }
 #pragma unroll
     for (int u = 0; u < LB; u++) {
-      const int t = t0 + u * 64 + lane;
+      const int t = t0 + u * NT + lane;
       if (t >= ne * C) break;
       const int q = 1 + t / C, cl = t % C, c = c0 + cl;
       const double2 ps = Ps[q * C + cl];
@@ -1216,7 +1217,7 @@ __global__ void __launch_bounds__(64) k_solve_lds(char *arena, const Chan *chans
   __syncthreads();
 
   // ---- W: divergence, ghost and wall rows, stores
-  for (int t = lane; t < nyu * C; t += 64) {
+  for (int t = lane; t < nyu * C; t += NT) {
     const int j = t / C, cl = t % C, c = c0 + cl;
     if (c >= ncol) continue;
     const size_t o = (size_t)j * ldc + c;
@@ -1544,8 +1545,8 @@ hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int
   } else {
     const int ngrp = (m.ncol + C - 1) / C;
     dim3 g((ngrp + 7) / 8 * 8, 1, m.nchan);
-    hipLaunchKernelGGL(k_solve_lds, g, dim3(64), solve_lds_bytes(m.nyu, C), s, arena, d_chans, kRK, C,
-                       m.solve_scan);
+    hipLaunchKernelGGL(k_solve_lds, g, dim3(64 * m.solve_waves), solve_lds_bytes(m.nyu, C), s, arena, d_chans, kRK,
+                       C, m.solve_scan);
   }
   return hipGetLastError();
 }

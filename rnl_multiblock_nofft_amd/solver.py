@@ -193,7 +193,8 @@ class Solver:
                 "xprod_points": self.get_tuning(_lib.TUNE_XPROD_POINTS),
                 "zfwd_points": self.get_tuning(_lib.TUNE_ZFWD_POINTS),
                 "solve_cols": self.get_tuning(_lib.TUNE_SOLVE_COLS),
-                "solve_scan": self.get_tuning(_lib.TUNE_SOLVE_SCAN)}
+                "solve_scan": self.get_tuning(_lib.TUNE_SOLVE_SCAN),
+                "solve_waves": self.get_tuning(_lib.TUNE_SOLVE_WAVES)}
 
     def sync(self):
         check(self.L.lh_sync(self.h))

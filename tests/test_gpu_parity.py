@@ -111,27 +111,31 @@ def test_one_step_graph(name):
         assert e <= 1e-11, (nm, e)
 
 
-@pytest.mark.parametrize("cols,scan,case", [(0, 0, "N32_ny153"), (1, 0, "N32_ny153"), (1, 1, "N32_ny153"),
-                                            (2, 1, "N32_ny153"), (3, 0, "N32_ny153"), (3, 1, "N32_ny153"),
-                                            (1, 1, "N8_ny17"), (10, 0, "N16_ny33_poly_ctP"),
-                                            (10, 1, "N16_ny33_poly_ctP")])
-def test_column_solve_layouts(cols, scan, case):
+@pytest.mark.parametrize("cols,scan,waves,case", [(0, 0, 1, "N32_ny153"), (1, 0, 1, "N32_ny153"),
+                                                  (1, 1, 1, "N32_ny153"), (2, 1, 1, "N32_ny153"),
+                                                  (3, 0, 1, "N32_ny153"), (3, 1, 1, "N32_ny153"),
+                                                  (3, 1, 4, "N32_ny153"), (4, 0, 2, "N32_ny153"),
+                                                  (1, 1, 1, "N8_ny17"), (2, 1, 4, "N8_ny17"),
+                                                  (10, 0, 1, "N16_ny33_poly_ctP"), (10, 1, 4, "N16_ny33_poly_ctP")])
+def test_column_solve_layouts(cols, scan, waves, case):
     """Both column-solve forms (one lane per column from HBM, or C columns
     per workgroup staged in LDS, with each recurrence on one lane or split
     over a lane group) give the oracle's step.  At 8 x 8 x 17 the Poisson
     sweeps have fewer rows than lanes (empty chunks)."""
     cfg = CASES[case]
-    say("solve layout", cols, scan, case)
+    say("solve layout", cols, scan, waves, case)
     s, o, f = make_pair(cfg, seed=7)
     s.set_tuning(_lib.TUNE_SOLVE_COLS, cols)
     s.set_tuning(_lib.TUNE_SOLVE_SCAN, scan)
+    s.set_tuning(_lib.TUNE_SOLVE_WAVES, waves)
     assert s.get_tuning(_lib.TUNE_SOLVE_COLS) == cols
     assert s.get_tuning(_lib.TUNE_SOLVE_SCAN) == scan
+    assert s.get_tuning(_lib.TUNE_SOLVE_WAVES) == waves
     s.step(1)
     o.step(1)
     for nm in ("u1", "u2", "u3", "p"):
         e = relerr(s.download(0, nm), o.get(nm))
-        say("cols", cols, "scan", scan, nm, e)
+        say("cols", cols, "scan", scan, "waves", waves, nm, e)
         assert e <= TOL, (nm, e)
 
 
