@@ -114,17 +114,31 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
     }
   }
   const int half = N2 / 2, dz = Ng2 - N2;
-  Walk2 wl(threadIdx.x, blockDim.x, ni);  // t = k ni + ii
-  for (int t = threadIdx.x; t < N2 * ni; t += blockDim.x, wl.next()) {
-    const int k = wl.hi, ii = wl.lo;
-    int col = k * nxh + i0 + ii;
-    double2 v = s0[col];
-    if (s1) {
-      double2 w = s1[col];
-      v = make_double2((wa * v.x + wb * w.x) / wd, (wa * v.y + wb * w.y) / wd);
+  {
+    constexpr int LU = 4;  // loads in flight per thread before the LDS stores
+    const int total = N2 * ni;
+    Walk2 wl(threadIdx.x, blockDim.x, ni);  // t = k ni + ii
+    for (int t0 = threadIdx.x; t0 < total; t0 += LU * blockDim.x) {
+      double2 va[LU], vb[LU];
+      int at[LU];
+#pragma unroll
+      for (int u = 0; u < LU; u++, wl.next()) {
+        const int k = wl.hi, ii = wl.lo;
+        at[u] = -1;
+        if (t0 + u * (int)blockDim.x >= total) continue;
+        const int col = k * nxh + i0 + ii;
+        at[u] = ii * S + (k < half ? k : k + dz);
+        va[u] = s0[col];
+        if (s1) vb[u] = s1[col];
+      }
+#pragma unroll
+      for (int u = 0; u < LU; u++) {
+        if (at[u] < 0) continue;
+        double2 v = va[u];
+        if (s1) v = make_double2((wa * v.x + wb * vb[u].x) / wd, (wa * v.y + wb * vb[u].y) / wd);
+        A[at[u]] = from_d<C>(v);
+      }
     }
-    int z = k < half ? k : k + dz;
-    A[ii * S + z] = from_d<C>(v);
   }
   __syncthreads();
   C *R = fft_lds_bf<C>(A, B, Ng2, ni, S, +1, TW);
@@ -161,31 +175,49 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
 
   // Hermitian extension of the three half spectra, packed two per sequence:
   // seq 2zz = f0 + i f2, seq 2zz+1 = f1
-  Walk2 wh(threadIdx.x, blockDim.x, Ng1);  // t = zz Ng1 + m
-  for (int t = threadIdx.x; t < nz * Ng1; t += blockDim.x, wh.next()) {
-    const int zz = wh.hi, m = wh.lo;
-    int z = z0 + zz;
-    C F0 = CT<C>::mk(0, 0), F1 = F0, F2 = F0;
-    if (m <= h1) {
-      F0 = hz[((size_t)0 * Ng2 + z) * nxp + m];
-      F1 = hz[((size_t)1 * Ng2 + z) * nxp + m];
-      F2 = hz[((size_t)2 * Ng2 + z) * nxp + m];
-      if (m == 0) {  // rft ignores imag(cc(0)) on the inverse transform
-        F0.y = 0;
-        F1.y = 0;
-        F2.y = 0;
+  {
+    // LU items per thread per batch: every load of a batch is in flight
+    // before the first LDS store (one item measured fastest here)
+    constexpr int LU = 1;
+    const int total = nz * Ng1;
+    Walk2 wh(threadIdx.x, blockDim.x, Ng1);  // t = zz Ng1 + m
+    for (int t0 = threadIdx.x; t0 < total; t0 += LU * blockDim.x) {
+      C F0[LU], F1[LU], F2[LU];
+      int at[LU];
+#pragma unroll
+      for (int u = 0; u < LU; u++, wh.next()) {
+        const int zz = wh.hi, m = wh.lo;
+        const int z = z0 + zz;
+        at[u] = -1;
+        F0[u] = F1[u] = F2[u] = CT<C>::mk(0, 0);
+        if (t0 + u * (int)blockDim.x >= total) continue;
+        at[u] = (2 * zz) * S + m;
+        if (m <= h1) {
+          F0[u] = hz[((size_t)0 * Ng2 + z) * nxp + m];
+          F1[u] = hz[((size_t)1 * Ng2 + z) * nxp + m];
+          F2[u] = hz[((size_t)2 * Ng2 + z) * nxp + m];
+          if (m == 0) {  // rft ignores imag(cc(0)) on the inverse transform
+            F0[u].y = 0;
+            F1[u].y = 0;
+            F2[u].y = 0;
+          }
+        } else if (m >= Ng1 - h1) {
+          const int mm = Ng1 - m;
+          F0[u] = hz[((size_t)0 * Ng2 + z) * nxp + mm];
+          F1[u] = hz[((size_t)1 * Ng2 + z) * nxp + mm];
+          F2[u] = hz[((size_t)2 * Ng2 + z) * nxp + mm];
+          F0[u].y = -F0[u].y;
+          F1[u].y = -F1[u].y;
+          F2[u].y = -F2[u].y;
+        }
       }
-    } else if (m >= Ng1 - h1) {
-      int mm = Ng1 - m;
-      F0 = hz[((size_t)0 * Ng2 + z) * nxp + mm];
-      F1 = hz[((size_t)1 * Ng2 + z) * nxp + mm];
-      F2 = hz[((size_t)2 * Ng2 + z) * nxp + mm];
-      F0.y = -F0.y;
-      F1.y = -F1.y;
-      F2.y = -F2.y;
+#pragma unroll
+      for (int u = 0; u < LU; u++) {
+        if (at[u] < 0) continue;
+        A[at[u]] = CT<C>::mk(F0[u].x - F2[u].y, F0[u].y + F2[u].x);
+        A[at[u] + S] = F1[u];
+      }
     }
-    A[(2 * zz) * S + m] = CT<C>::mk(F0.x - F2.y, F0.y + F2.x);
-    A[(2 * zz + 1) * S + m] = F1;
   }
   __syncthreads();
   C *X = fft_lds_bf<C>(A, B, Ng1, 2 * nz, S, +1, TW);
@@ -289,10 +321,27 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
   C *TW = B + 4 * TI * S;
   stage_twiddles<C>(TW, v.twz, Ng2);
   const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxp;
-  Walk3 wl(threadIdx.x, blockDim.x, ni, Ng2);  // t = (p Ng2 + z) ni + ii
-  for (int t = threadIdx.x; t < np * ni * Ng2; t += blockDim.x, wl.next()) {
-    const int p = wl.hi, z = wl.mid, ii = wl.lo;
-    A[(p * ni + ii) * S + z] = gx[((size_t)p * Ng2 + z) * nxp + i0 + ii];
+  {
+    // loads in flight per thread before the LDS stores: batches pay in FP32;
+    // in FP64 their registers cost more occupancy than they gain
+    constexpr int LU = sizeof(C) == 16 ? 1 : 8;
+    const int total = np * ni * Ng2;
+    Walk3 wl(threadIdx.x, blockDim.x, ni, Ng2);  // t = (p Ng2 + z) ni + ii
+    for (int t0 = threadIdx.x; t0 < total; t0 += LU * blockDim.x) {
+      C g[LU];
+      int at[LU];
+#pragma unroll
+      for (int u = 0; u < LU; u++, wl.next()) {
+        const int p = wl.hi, z = wl.mid, ii = wl.lo;
+        at[u] = -1;
+        if (t0 + u * (int)blockDim.x >= total) continue;
+        at[u] = (p * ni + ii) * S + z;
+        g[u] = gx[((size_t)p * Ng2 + z) * nxp + i0 + ii];
+      }
+#pragma unroll
+      for (int u = 0; u < LU; u++)
+        if (at[u] >= 0) A[at[u]] = g[u];
+    }
   }
   __syncthreads();
   C *R = fft_lds_bf<C>(A, B, Ng2, np * ni, S, -1, TW);
