@@ -58,18 +58,35 @@ template <typename C> __device__ __forceinline__ C from_d(double2 v) { return CT
 template <typename C> __device__ __forceinline__ double2 to_d(C v) { return make_double2(v.x, v.y); }
 
 // ------------------------------------------------------------------------
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs, each with its own L2.  The launch pads ntile * nslot workgroups to a
+// multiple of 8 and gives each XCD a contiguous run of (tile, slot) pairs,
+// so neighbouring x-mode tiles of a plane, which share 128-byte lines of the
+// spectral rows, meet in one L2.  Returns false for the padding.
+__device__ __forceinline__ bool xcd_tile(int ntile, int nslot, int &tile, int &slot) {
+  const int per = gridDim.x / 8;
+  const int L = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (L >= ntile * nslot) return false;
+  tile = L % ntile;
+  slot = L / ntile;
+  return true;
+}
+static inline unsigned xcd_grid(int ntile, int nslot) { return (unsigned)((ntile * nslot + 7) / 8 * 8); }
+
+// ------------------------------------------------------------------------
 // K1: interpolation + inverse z transform
 template <typename C>
-__global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E) {
+__global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E, int ntile, int nslot) {
   const Chan ch = chans[blockIdx.z / 3];
   if (ch.tri) return;  // triad-list channel: k_triad computes its products
   const View v(arena, ch);
   const int f = blockIdx.z % 3;
-  const int slot = blockIdx.y;
+  int tile, slot;
+  if (!xcd_tile(ntile, nslot, tile, slot)) return;
   if (slot >= ch.nslot) return;
   const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
   const int TI = tile_of8(E, Ng2, nxh);
-  const int i0 = blockIdx.x * TI;
+  const int i0 = tile * TI;
   if (i0 >= nxh) return;
   const int ni = min(TI, nxh - i0);
   extern __shared__ double2 smem_d[];
@@ -299,15 +316,16 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
 // ------------------------------------------------------------------------
 // K3: forward z transform, truncation, d/dx + d/dz
 template <typename C>
-__global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, int E) {
+__global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, int E, int ntile, int nslot) {
   const Chan ch = chans[blockIdx.z];
   if (ch.tri) return;
   const View v(arena, ch);
-  const int slot = blockIdx.y;
+  int tile, slot;
+  if (!xcd_tile(ntile, nslot, tile, slot)) return;
   if (slot >= ch.nslot) return;
   const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
   const int TI = tile_of8(E, Ng2, nxh);
-  const int i0 = blockIdx.x * TI;
+  const int i0 = tile * TI;
   if (i0 >= nxh) return;
   const int ni = min(TI, nxh - i0);
   const bool isU = slot < ch.nplu;
@@ -1532,11 +1550,11 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
   size_t es = mixed ? sizeof(float2) : sizeof(double2);
   dim3 b(256);
   if (stage == 0) {
-    dim3 g(m.zinv_tiles, m.nslot, m.nchan * 3);
+    dim3 g(xcd_grid(m.zinv_tiles, m.nslot), 1, m.nchan * 3);
     if (mixed)
-      hipLaunchKernelGGL(k_zinv<float2>, g, b, m.lds_zinv * es, s, arena, d_chans, m.e_zinv);
+      hipLaunchKernelGGL(k_zinv<float2>, g, b, m.lds_zinv * es, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot);
     else
-      hipLaunchKernelGGL(k_zinv<double2>, g, b, m.lds_zinv * es, s, arena, d_chans, m.e_zinv);
+      hipLaunchKernelGGL(k_zinv<double2>, g, b, m.lds_zinv * es, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot);
   } else if (stage == 1) {
     dim3 g(m.xprod_tiles, m.nslot, m.nchan);
     const bool cfl = kRK == 1;
@@ -1549,11 +1567,11 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
     else
       hipLaunchKernelGGL((k_xprod<double2, false>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
   } else {
-    dim3 g(m.zfwd_tiles, m.nslot, m.nchan);
+    dim3 g(xcd_grid(m.zfwd_tiles, m.nslot), 1, m.nchan);
     if (mixed)
-      hipLaunchKernelGGL(k_zfwd<float2>, g, b, m.lds_zfwd * es, s, arena, d_chans, m.e_zfwd);
+      hipLaunchKernelGGL(k_zfwd<float2>, g, b, m.lds_zfwd * es, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot);
     else
-      hipLaunchKernelGGL(k_zfwd<double2>, g, b, m.lds_zfwd * es, s, arena, d_chans, m.e_zfwd);
+      hipLaunchKernelGGL(k_zfwd<double2>, g, b, m.lds_zfwd * es, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot);
   }
   return hipGetLastError();
 }
