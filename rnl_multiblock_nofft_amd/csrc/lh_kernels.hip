@@ -84,6 +84,12 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
   int tile, slot;
   if (!xcd_tile(ntile, nslot, tile, slot)) return;
   if (slot >= ch.nslot) return;
+  // logical order u0 v0 u1 v1 ... (then the last u-plane): u-plane j and
+  // v-planes j-1, j interpolate from the same spectral rows
+  {
+    const int nplv = ch.nslot - ch.nplu;
+    slot = slot < 2 * nplv ? ((slot & 1) ? ch.nplu + (slot >> 1) : (slot >> 1)) : slot - nplv;
+  }
   const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
   const int TI = tile_of8(E, Ng2, nxh);
   const int i0 = tile * TI;
