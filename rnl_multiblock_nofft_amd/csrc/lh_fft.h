@@ -400,6 +400,82 @@ __device__ __forceinline__ C *fft_bf_s(C *x, C *y, int n, int nb, int sstr, cons
   }
 }
 
+// ------------------------------------------------------------------------
+// Stages of a plan one at a time, for callers that fuse work into the last
+// backward stage and the first forward stage.  plan_len(n) stages; stage s
+// has radix plan_at(n, s) and follows stages with product plan_ns(n, s).
+// The reversed plan (plan_at(n, L-1-s)) transforms the same lengths.
+__host__ __device__ constexpr int plan_len(int n) {
+  int k = 0;
+  while (n > 1) {
+    n /= plan_radix(n);
+    k++;
+  }
+  return k;
+}
+__host__ __device__ constexpr int plan_at(int n, int s) {
+  while (s-- > 0) n /= plan_radix(n);
+  return plan_radix(n);
+}
+__host__ __device__ constexpr int plan_ns(int n, int s) {
+  int p = 1;
+  for (int i = 0; i < s; i++) p *= plan_at(n, i);
+  return p;
+}
+// the same for the reversed plan
+__host__ __device__ constexpr int rplan_at(int n, int s) { return plan_at(n, plan_len(n) - 1 - s); }
+__host__ __device__ constexpr int rplan_ns(int n, int s) {
+  int p = 1;
+  for (int i = 0; i < s; i++) p *= rplan_at(n, i);
+  return p;
+}
+
+// one batch-fastest stage (radix R after stages of product Ns) from x to y
+template <typename C, int N, int R, int Ns, int SIGN>
+__device__ __forceinline__ void bf_stage(const C *x, C *y, int nb, int sstr, const C *tw) {
+  constexpr int nbf = N / R, span = N / (Ns * R);
+  const int total = nb * nbf;
+  Walk2 w(threadIdx.x, blockDim.x, nb);  // t = jj nb + b
+  for (int t = threadIdx.x; t < total; t += blockDim.x, w.next())
+    stage_item<C, R, SIGN>(x + w.lo * sstr, y + w.lo * sstr, w.hi, nbf, Ns, span, tw);
+  __syncthreads();
+}
+// stages S0 .. S1-1 of the plan (REV: the reversed plan), ping-ponging
+// between x and y; returns the buffer with the result
+template <typename C, int N, int SIGN, bool REV, int S0, int S1>
+__device__ __forceinline__ C *bf_stages(C *x, C *y, int nb, int sstr, const C *tw) {
+  if constexpr (S0 >= S1) {
+    return x;
+  } else {
+    constexpr int R = REV ? rplan_at(N, S0) : plan_at(N, S0);
+    constexpr int Ns = REV ? rplan_ns(N, S0) : plan_ns(N, S0);
+    bf_stage<C, N, R, Ns, SIGN>(x, y, nb, sstr, tw);
+    return bf_stages<C, N, SIGN, REV, S0 + 1, S1>(y, x, nb, sstr, tw);
+  }
+}
+// The last stage of the plan for item jj (0 <= jj < N / R) of one sequence,
+// kept in registers: v[r] = the transform at point jj + r N / R.  The first
+// stage of the reversed plan reads exactly these points for its item jj:
+// first_item_rev writes its result to y (ys[jj R + r]).
+template <typename C, int N, int SIGN>
+__device__ __forceinline__ void last_item(const C *xs, int jj, const C *tw, C (&v)[plan_at(N, plan_len(N) - 1)]) {
+  constexpr int R = plan_at(N, plan_len(N) - 1), Ns = N / R;
+#pragma unroll
+  for (int r = 0; r < R; r++) v[r] = xs[jj + r * Ns];
+  if constexpr (Ns > 1) {
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = c_mul(v[r], tw_at<C>(tw, r * jj, SIGN));  // span = 1
+  }
+  dft<C, R, SIGN>(v);
+}
+template <typename C, int N, int SIGN>
+__device__ __forceinline__ void first_item_rev(C (&v)[plan_at(N, plan_len(N) - 1)], int jj, C *ys) {
+  constexpr int R = plan_at(N, plan_len(N) - 1);
+  dft<C, R, SIGN>(v);
+#pragma unroll
+  for (int r = 0; r < R; r++) ys[jj * R + r] = v[r];
+}
+
 // sign -1 forward, +1 backward (unscaled); sequences sstr = seq_stride(n,
 // nb, sizeof(C)) apart
 template <typename C>

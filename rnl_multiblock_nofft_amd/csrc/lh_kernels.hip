@@ -172,6 +172,58 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
 }
 
 // ------------------------------------------------------------------------
+// K2 core for lengths whose plan ends in a radix of at most 8: the inverse
+// transform's stages but the last run through LDS; then, per (row zz, item
+// jj), the last inverse stage of both sequences (u + i w, v), the products
+// and the first stage of the forward transform's reversed plan run in
+// registers (last_item / first_item_rev: the points one item produces are
+// the points the reversed plan's first item reads); the remaining forward
+// stages run through LDS.  Output sequence zz nseq + q.  Returns the buffer
+// with the forward result.
+template <typename C, bool CFL, int N>
+__device__ __forceinline__ C *xprod_fused(C *A, C *B, const C *TW, int S, int nz, bool isU, typename CT<C>::R &ma,
+                                          typename CT<C>::R &mb) {
+  typedef typename CT<C>::R R_t;
+  constexpr int L = plan_len(N), R = plan_at(N, L - 1), NB = N / R;
+  C *X = bf_stages<C, N, +1, false, 0, L - 1>(A, B, 2 * nz, S, TW);
+  C *Yp = X == A ? B : A;
+  const int total = nz * NB;
+  Walk2 w(threadIdx.x, blockDim.x, nz);  // t = jj nz + zz
+  for (int t = threadIdx.x; t < total; t += blockDim.x, w.next()) {
+    const int jj = w.hi, zz = w.lo;
+    C a[R], b[R];
+    last_item<C, N, +1>(X + (2 * zz) * S, jj, TW, a);
+    last_item<C, N, +1>(X + (2 * zz + 1) * S, jj, TW, b);
+    if (isU) {
+      C pp[R], qq[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const R_t a1 = a[r].x, a3 = a[r].y, a2 = b[r].x;
+        if (CFL) {
+          ma = fmax(ma, fabs(a1));
+          mb = fmax(mb, fabs(a3));
+        }
+        pp[r] = CT<C>::mk(a1 * a1, a1 * a3);  // uu + i uw
+        qq[r] = CT<C>::mk(a2 * a2, a3 * a3);  // vv + i ww
+      }
+      first_item_rev<C, N, -1>(pp, jj, Yp + (2 * zz) * S);
+      first_item_rev<C, N, -1>(qq, jj, Yp + (2 * zz + 1) * S);
+    } else {
+      C pp[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const R_t a1 = a[r].x, a3 = a[r].y, a2 = b[r].x;
+        if (CFL) ma = fmax(ma, fabs(a2));
+        pp[r] = CT<C>::mk(a1 * a2, a3 * a2);  // uv + i wv
+      }
+      first_item_rev<C, N, -1>(pp, jj, Yp + zz * S);
+    }
+  }
+  __syncthreads();
+  return bf_stages<C, N, -1, true, 1, L>(Yp, X, (isU ? 2 : 1) * nz, S, TW);
+}
+
+// ------------------------------------------------------------------------
 // K2: inverse x transform, products, forward x transform
 template <typename C, bool CFL>
 __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, int E) {
@@ -243,34 +295,44 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
     }
   }
   __syncthreads();
-  C *X = fft_lds_bf<C>(A, B, Ng1, 2 * nz, S, +1, TW);
+  const int nseq = isU ? 2 : 1;
   // CFL maxima of dtc_calc (FOU3D.f90:699-736), at the first substep only:
   // |u1|, |u3| on u-planes, |u2| on v-planes (times dthdyv(j) in k_dt_update)
   R_t ma = 0, mb = 0;
-  Walk2 wp(threadIdx.x, blockDim.x, Ng1);  // t = zz Ng1 + n
-  for (int t = threadIdx.x; t < nz * Ng1; t += blockDim.x, wp.next()) {
-    const int zz = wp.hi, n = wp.lo;
-    C s0 = X[(2 * zz) * S + n], s1 = X[(2 * zz + 1) * S + n];
-    R_t a1 = s0.x, a3 = s0.y, a2 = s1.x;
-    if (CFL) {
-      if (isU) {
-        ma = fmax(ma, fabs(a1));
-        mb = fmax(mb, fabs(a3));
-      } else {
-        ma = fmax(ma, fabs(a2));
+  C *Y;
+  switch (Ng1) {
+    case 24: Y = xprod_fused<C, CFL, 24>(A, B, TW, S, nz, isU, ma, mb); break;
+    case 48: Y = xprod_fused<C, CFL, 48>(A, B, TW, S, nz, isU, ma, mb); break;
+    case 96: Y = xprod_fused<C, CFL, 96>(A, B, TW, S, nz, isU, ma, mb); break;
+    case 384: Y = xprod_fused<C, CFL, 384>(A, B, TW, S, nz, isU, ma, mb); break;
+    default: {
+      C *X = fft_lds_bf<C>(A, B, Ng1, 2 * nz, S, +1, TW);
+      C *Yp = X == A ? B : A;
+      Walk2 wp(threadIdx.x, blockDim.x, Ng1);  // t = zz Ng1 + n
+      for (int t = threadIdx.x; t < nz * Ng1; t += blockDim.x, wp.next()) {
+        const int zz = wp.hi, n = wp.lo;
+        const C s0 = X[(2 * zz) * S + n], s1 = X[(2 * zz + 1) * S + n];
+        const R_t a1 = s0.x, a3 = s0.y, a2 = s1.x;
+        if (CFL) {
+          if (isU) {
+            ma = fmax(ma, fabs(a1));
+            mb = fmax(mb, fabs(a3));
+          } else {
+            ma = fmax(ma, fabs(a2));
+          }
+        }
+        if (isU) {
+          Yp[(2 * zz) * S + n] = CT<C>::mk(a1 * a1, a1 * a3);      // uu + i uw
+          Yp[(2 * zz + 1) * S + n] = CT<C>::mk(a2 * a2, a3 * a3);  // vv + i ww
+        } else {
+          Yp[zz * S + n] = CT<C>::mk(a1 * a2, a3 * a2);  // uv + i wv
+        }
       }
-    }
-    if (isU) {
-      X[(2 * zz) * S + n] = CT<C>::mk(a1 * a1, a1 * a3);      // uu + i uw
-      X[(2 * zz + 1) * S + n] = CT<C>::mk(a2 * a2, a3 * a3);  // vv + i ww
-    } else {
-      X[(2 * zz) * S + n] = CT<C>::mk(a1 * a2, a3 * a2);      // uv + i wv
-      X[(2 * zz + 1) * S + n] = CT<C>::mk(0, 0);
+      __syncthreads();
+      Y = fft_lds_bf<C>(Yp, X, Ng1, nseq * nz, S, -1, TW);
     }
   }
-  __syncthreads();
-  // one (max a, max b) pair per workgroup, reduced by k_dt_update: wave
-  // maxima here, combined after the forward transform's barriers
+  // one (max a, max b) pair per workgroup, reduced by k_dt_update
   __shared__ double red[2][4];
   if (CFL) {
     double a = (double)ma, b = (double)mb;
@@ -282,27 +344,25 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
       red[0][threadIdx.x >> 6] = a;
       red[1][threadIdx.x >> 6] = b;
     }
-  }
-  C *Y = fft_lds_bf<C>(X, X == A ? B : A, Ng1, 2 * nz, S, -1, TW);
-  if (CFL && threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    double a = red[0][0], b = red[1][0];
-    for (int q = 1; q < nw; q++) {
-      a = fmax(a, red[0][q]);
-      b = fmax(b, red[1][q]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int nw = (blockDim.x + 63) >> 6;
+      for (int q = 1; q < nw; q++) {
+        a = fmax(a, red[0][q]);
+        b = fmax(b, red[1][q]);
+      }
+      double *cm = v.cfl + 2 * ((size_t)slot * gridDim.x + blockIdx.x);
+      cm[0] = a;
+      cm[1] = b;
     }
-    double *cm = v.cfl + 2 * ((size_t)slot * gridDim.x + blockIdx.x);
-    cm[0] = a;
-    cm[1] = b;
   }
-  const int nseq = isU ? 2 : 1;
   const R_t inv = (R_t)(1.0 / Ng1);
   C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxp;
   Walk2 wo(threadIdx.x, blockDim.x, h1 + 1);  // t = (zz nseq + q) (h1 + 1) + m
   for (int t = threadIdx.x; t < nz * nseq * (h1 + 1); t += blockDim.x, wo.next()) {
     const int m = wo.lo, r = wo.hi;
     const int q = nseq == 2 ? (r & 1) : 0, zz = nseq == 2 ? (r >> 1) : r;
-    const C *ys = Y + (2 * zz + q) * S;
+    const C *ys = Y + r * S;
     C Ym = ys[m], Yc = ys[m == 0 ? 0 : Ng1 - m];
     C P, Q;
     if (m == 0) {
