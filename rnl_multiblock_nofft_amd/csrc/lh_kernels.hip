@@ -74,6 +74,25 @@ __device__ __forceinline__ bool xcd_tile(int ntile, int nslot, int &tile, int &s
 static inline unsigned xcd_grid(int ntile, int nslot) { return (unsigned)((ntile * nslot + 7) / 8 * 8); }
 
 // ------------------------------------------------------------------------
+// K1 tail for the compiled lengths: every stage of the inverse z transform
+// but the last through LDS, the last in registers per (item jj, x-mode ii),
+// its points z = jj + r N / R stored straight to the hz rows (consecutive
+// lanes take consecutive x-modes: coalesced)
+template <typename C, int N>
+__device__ __forceinline__ void zinv_tail(C *A, C *B, const C *TW, int S, int ni, int nxp, C *hz) {
+  constexpr int L = plan_len(N), R = plan_at(N, L - 1), NB = N / R;
+  const C *X = bf_stages<C, N, +1, false, 0, L - 1>(A, B, ni, S, TW);
+  Walk2 w(threadIdx.x, blockDim.x, ni);  // t = jj ni + ii
+  for (int t = threadIdx.x; t < ni * NB; t += blockDim.x, w.next()) {
+    const int jj = w.hi, ii = w.lo;
+    C o[R];
+    last_item<C, N, +1>(X + ii * S, jj, TW, o);
+#pragma unroll
+    for (int r = 0; r < R; r++) hz[(size_t)(jj + r * NB) * nxp + ii] = o[r];
+  }
+}
+
+// ------------------------------------------------------------------------
 // K1: interpolation + inverse z transform
 template <typename C>
 __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E, int ntile, int nslot) {
@@ -164,11 +183,19 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
     }
   }
   __syncthreads();
-  C *R = fft_lds_bf<C>(A, B, Ng2, ni, S, +1, TW);
-  C *hz = Inter<C>::hz(v) + ((size_t)slot * 3 + f) * Ng2 * nxp;
-  Walk2 ws(threadIdx.x, blockDim.x, ni);  // t = z ni + ii
-  for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x, ws.next())
-    hz[(size_t)ws.hi * nxp + i0 + ws.lo] = R[ws.lo * S + ws.hi];
+  C *hz = Inter<C>::hz(v) + ((size_t)slot * 3 + f) * Ng2 * nxp + i0;
+  switch (Ng2) {
+    case 24: zinv_tail<C, 24>(A, B, TW, S, ni, nxp, hz); break;
+    case 48: zinv_tail<C, 48>(A, B, TW, S, ni, nxp, hz); break;
+    case 96: zinv_tail<C, 96>(A, B, TW, S, ni, nxp, hz); break;
+    case 384: zinv_tail<C, 384>(A, B, TW, S, ni, nxp, hz); break;
+    default: {
+      C *R = fft_lds_bf<C>(A, B, Ng2, ni, S, +1, TW);
+      Walk2 ws(threadIdx.x, blockDim.x, ni);  // t = z ni + ii
+      for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x, ws.next())
+        hz[(size_t)ws.hi * nxp + ws.lo] = R[ws.lo * S + ws.hi];
+    }
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -380,6 +407,33 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
 }
 
 // ------------------------------------------------------------------------
+// K3 outputs of one (kz index k, x-mode i0 + ii): u-planes Nu1 = ikx X0 +
+// ikz X1, Nu3 = ikx X1 + ikz X3, fvv = X2; v-planes Nu2 = ikx X0 + ikz X1,
+// fuv = X0, fwv = X1 (der_x/der_z FOU3D.f90:476-593; kz = -N2/2 zeroed)
+struct ZOut {
+  const double *kx, *kz;
+  double2 *o0, *o1, *o2;  // u: nl1, nl3, fvv; v: nl2, fuv, fwv
+  size_t row;             // j * ldc
+  int nxh, i0, N2, Ng2;
+  bool isU;
+  __device__ __forceinline__ void put(int k, int ii, double2 X0, double2 X1, double2 X2, double2 X3) const {
+    const int i = i0 + ii;
+    const size_t o = row + (size_t)k * nxh + i;
+    const bool live = k != N2 / 2;
+    const double kxi = kx[i], kzk = kz[k];
+    if (isU) {
+      o0[o] = live ? cadd(cimul(kxi, X0), cimul(kzk, X1)) : make_double2(0, 0);
+      o1[o] = live ? cadd(cimul(kxi, X1), cimul(kzk, X3)) : make_double2(0, 0);
+      o2[o] = X2;
+    } else {
+      o0[o] = live ? cadd(cimul(kxi, X0), cimul(kzk, X1)) : make_double2(0, 0);
+      o1[o] = X0;
+      o2[o] = X1;
+    }
+  }
+};
+
+// ------------------------------------------------------------------------
 // K3: forward z transform, truncation, d/dx + d/dz
 template <typename C>
 __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, int E, int ntile, int nslot) {
@@ -428,31 +482,22 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
     }
   }
   __syncthreads();
+  ZOut zo{v.kx, v.kz, isU ? v.nl1 : v.nl2, isU ? v.nl3 : v.fuv, isU ? v.fvv : v.fwv, (size_t)j * ldc, nxh, i0,
+          N2, Ng2, isU};
+  // (a register-fused last stage, as in k_zinv, needs two sequences' points
+  // live per item; at FP64 that cost occupancy more than it saved)
   C *R = fft_lds_bf<C>(A, B, Ng2, np * ni, S, -1, TW);
   const double inv = 1.0 / Ng2;
   const int half = N2 / 2, dz = Ng2 - N2;
   Walk2 wo(threadIdx.x, blockDim.x, ni);  // t = k ni + ii
   for (int t = threadIdx.x; t < ni * N2; t += blockDim.x, wo.next()) {
     const int k = wo.hi, ii = wo.lo;
-    int i = i0 + ii;
-    int col = k * nxh + i;
-    int z = k < half ? k : k + dz;
-    bool live = k != half;  // kz = -N2/2 is zeroed by der_x/der_z
-    double kxi = v.kx[i], kzk = v.kz[k];
-    size_t o = (size_t)j * ldc + col;
-    double2 X0 = cscale(inv, to_d(R[(0 * ni + ii) * S + z]));
-    double2 X1 = cscale(inv, to_d(R[(1 * ni + ii) * S + z]));
-    if (isU) {
-      double2 X2 = cscale(inv, to_d(R[(2 * ni + ii) * S + z]));
-      double2 X3 = cscale(inv, to_d(R[(3 * ni + ii) * S + z]));
-      v.nl1[o] = live ? cadd(cimul(kxi, X0), cimul(kzk, X1)) : make_double2(0, 0);
-      v.nl3[o] = live ? cadd(cimul(kxi, X1), cimul(kzk, X3)) : make_double2(0, 0);
-      v.fvv[o] = X2;
-    } else {
-      v.nl2[o] = live ? cadd(cimul(kxi, X0), cimul(kzk, X1)) : make_double2(0, 0);
-      v.fuv[o] = X0;
-      v.fwv[o] = X1;
-    }
+    const int z = k < half ? k : k + dz;
+    double2 X[4];
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+      if (p < np) X[p] = cscale(inv, to_d(R[(p * ni + ii) * S + z]));
+    zo.put(k, ii, X[0], X[1], np == 4 ? X[2] : X[0], np == 4 ? X[3] : X[1]);
   }
 }
 
