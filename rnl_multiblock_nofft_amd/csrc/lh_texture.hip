@@ -90,28 +90,45 @@ __global__ void __launch_bounds__(64) k_tex_tables(char *arena, const Chan *chan
 }
 
 // adjacent-row values of the smooth solution, U = sum_j W[j] r_j over the
-// rows where the inverse row is significant; grid (cols, 2 walls, channels)
+// rows where the inverse row is significant; grid (cols / 64, 2 walls,
+// channels), 256 threads: 64 columns x 4 interleaved row subsets, each
+// thread's loads unrolled so they are in flight together, partial sums
+// added in a fixed order through LDS
 __global__ void __launch_bounds__(256) k_tex_adj(char *arena, const Chan *chans, int kRK) {
   const Chan ch = chans[blockIdx.z];
   if (!ch.tex) return;
   const View v(arena, ch);
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ch.ncol) return;
+  const int cl = threadIdx.x & 63, part = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
+  const bool live = c < ch.ncol;
   const int n = ch.nyu, ldc = ch.ldc, w = blockIdx.y;
   const size_t tab = (size_t)n * ldc;
   const double *W = v.wtab + (size_t)((kRK - 1) * 2 + w) * tab;
   const int J = w == 0 ? v.iscal[0] : v.iscal[1];
   const int j0 = w == 0 ? 0 : n - J, j1 = w == 0 ? J : n;
   double2 a = make_double2(0, 0), b = make_double2(0, 0);
-  for (int j = j0; j < j1; j++) {
-    const size_t o = (size_t)j * ldc + c;
-    const double q = W[o];
-    const double2 r1 = v.r1[o], r3 = v.r3[o];
-    a = make_double2(a.x + q * r1.x, a.y + q * r1.y);
-    b = make_double2(b.x + q * r3.x, b.y + q * r3.y);
+  if (live) {
+#pragma unroll 4
+    for (int j = j0 + part; j < j1; j += 4) {
+      const size_t o = (size_t)j * ldc + c;
+      const double q = W[o];
+      const double2 r1 = v.r1[o], r3 = v.r3[o];
+      a = make_double2(a.x + q * r1.x, a.y + q * r1.y);
+      b = make_double2(b.x + q * r3.x, b.y + q * r3.y);
+    }
   }
-  v.texu[(size_t)(0 * 2 + w) * ch.ncol + c] = a;
-  v.texu[(size_t)(1 * 2 + w) * ch.ncol + c] = b;
+  __shared__ double2 red[2][4][64];
+  red[0][part][cl] = a;
+  red[1][part][cl] = b;
+  __syncthreads();
+  if (part == 0 && live) {
+    for (int q = 1; q < 4; q++) {
+      a = cadd(a, red[0][q][cl]);
+      b = cadd(b, red[1][q][cl]);
+    }
+    v.texu[(size_t)(0 * 2 + w) * ch.ncol + c] = a;
+    v.texu[(size_t)(1 * 2 + w) * ch.ncol + c] = b;
+  }
 }
 
 // texw <- U + (g - gbar) texw (iteration `it`; texw holds the previous
@@ -211,7 +228,7 @@ __global__ void __launch_bounds__(256) k_tex_get(char *arena, const Chan *chans)
 }
 
 hipError_t launch_tex_adj(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s) {
-  hipLaunchKernelGGL(k_tex_adj, dim3((m.ncol + 255) / 256, 2, m.nchan), dim3(256), 0, s, arena, d_chans, kRK);
+  hipLaunchKernelGGL(k_tex_adj, dim3((m.ncol + 63) / 64, 2, m.nchan), dim3(256), 0, s, arena, d_chans, kRK);
   return hipGetLastError();
 }
 
