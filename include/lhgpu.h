@@ -222,6 +222,11 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
  * solve (1, 2 or 4); more waves give each staged column more lanes for its
  * recurrences and more loads in flight. */
 #define LH_TUNE_SOLVE_WAVES 7
+/* LH_TUNE_XPROD_INPLACE (default 0): 1 runs the x-product stage's transforms
+ * in place in one LDS buffer where every stage has at most one item per
+ * thread (more workgroups per CU, more registers per thread).  Results are
+ * identical. */
+#define LH_TUNE_XPROD_INPLACE 8
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

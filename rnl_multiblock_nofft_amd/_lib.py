@@ -21,6 +21,7 @@ TUNE_SOLVE_COLS, TUNE_ZINV_POINTS, TUNE_XPROD_POINTS, TUNE_ZFWD_POINTS = 1, 2, 3
 TUNE_TEX_SEPARABLE = 5
 TUNE_SOLVE_SCAN = 6
 TUNE_SOLVE_WAVES = 7
+TUNE_XPROD_INPLACE = 8
 PREC_FP64, PREC_MIXED = 0, 1
 
 # every symbol include/lhgpu.h declares

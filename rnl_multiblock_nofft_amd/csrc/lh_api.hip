@@ -1264,6 +1264,17 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     drop_graph(x);
     return LH_OK;
   }
+  if (key == LH_TUNE_XPROD_INPLACE) {
+    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_XPROD_INPLACE is 0 or 1");
+    ShapeMax m = x->shape;
+    m.xprod_ip = value;
+    shape_tiles(m);
+    if (stage_lds_bytes(x, m) > 160 * 1024) return set_err(LH_EINVAL, "lh_set_tuning: LDS exceeds 160 KiB");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->shape = m;
+    drop_graph(x);
+    return LH_OK;
+  }
   if (key == LH_TUNE_SOLVE_WAVES) {
     if (value != 1 && value != 2 && value != 4)
       return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_SOLVE_WAVES is 1, 2 or 4");
@@ -1322,6 +1333,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
     *value = x->shape.solve_waves;
     return LH_OK;
   }
+  if (key == LH_TUNE_XPROD_INPLACE) {
+    *value = x->shape.xprod_ip;
+    return LH_OK;
+  }
   int *k = stage_knob(x->shape, key);
   if (!k) return set_err(LH_EINVAL, "lh_get_tuning: unknown key %d", key);
   *value = *k;
@@ -1372,18 +1387,21 @@ int lh_autotune(lh_ctx *x, int reps) {
   const int keys[3] = {LH_TUNE_ZINV_POINTS, LH_TUNE_XPROD_POINTS, LH_TUNE_ZFWD_POINTS};
   for (int st = 0; st < 3; st++) {
     float best_ms = 1e30f;
-    for (int E : pts) {
-      ShapeMax m = best;
-      *stage_knob(m, keys[st]) = E;
-      shape_tiles(m);
-      if (stage_lds_bytes(x, m) > 160 * 1024) continue;
-      float ms = time_launches(x, e0, e1, reps,
-                               [&](int) { return launch_nl_stage(x->arena, x->d_chans, m, x->mixed, st, x->stream); });
-      if (ms >= 0.f && ms < best_ms) {
-        best_ms = ms;
-        *stage_knob(best, keys[st]) = E;
+    for (int E : pts)
+      for (int ip = 0; ip <= (st == 1 ? 1 : 0); ip++) {
+        ShapeMax m = best;
+        *stage_knob(m, keys[st]) = E;
+        if (st == 1) m.xprod_ip = ip;
+        shape_tiles(m);
+        if (stage_lds_bytes(x, m) > 160 * 1024) continue;
+        float ms = time_launches(
+            x, e0, e1, reps, [&](int) { return launch_nl_stage(x->arena, x->d_chans, m, x->mixed, st, x->stream); });
+        if (ms >= 0.f && ms < best_ms) {
+          best_ms = ms;
+          *stage_knob(best, keys[st]) = E;
+          if (st == 1) best.xprod_ip = ip;
+        }
       }
-    }
     shape_tiles(best);
   }
   // column solve: one lane per column, or C columns per workgroup in LDS

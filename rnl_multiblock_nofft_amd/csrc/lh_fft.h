@@ -453,6 +453,56 @@ __device__ __forceinline__ C *bf_stages(C *x, C *y, int nb, int sstr, const C *t
     return bf_stages<C, N, SIGN, REV, S0 + 1, S1>(y, x, nb, sstr, tw);
   }
 }
+// In-place variants for stages with at most one item per thread: each
+// thread reads and transforms its item in registers, the workgroup
+// synchronises, then every item is written back into the same buffer.
+// One LDS buffer instead of two doubles the workgroups that fit a CU.
+template <typename C, int R, int SIGN>
+__device__ __forceinline__ void item_regs(const C *xs, int jj, int nbf, int Ns, int span, const C *tw, C (&v)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; r++) v[r] = xs[jj + r * nbf];
+  const int k = jj % Ns;
+  if (Ns > 1) {
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = c_mul(v[r], tw_at<C>(tw, r * k * span, SIGN));
+  }
+  dft<C, R, SIGN>(v);
+}
+template <typename C, int N, int R, int Ns, int SIGN>
+__device__ __forceinline__ void bf_stage_ip(C *x, int nb, int sstr, const C *tw) {
+  constexpr int nbf = N / R, span = N / (Ns * R);
+  const int t = threadIdx.x, jj = t / nb, b = t - jj * nb;
+  const bool has = t < nb * nbf;
+  C v[R];
+  if (has) item_regs<C, R, SIGN>(x + b * sstr, jj, nbf, Ns, span, tw, v);
+  __syncthreads();
+  if (has) {
+    const int idxD = (jj / Ns) * Ns * R + jj % Ns;
+#pragma unroll
+    for (int r = 0; r < R; r++) x[b * sstr + idxD + r * Ns] = v[r];
+  }
+  __syncthreads();
+}
+template <typename C, int N, int SIGN, bool REV, int S0, int S1>
+__device__ __forceinline__ void bf_stages_ip(C *x, int nb, int sstr, const C *tw) {
+  if constexpr (S0 < S1) {
+    constexpr int R = REV ? rplan_at(N, S0) : plan_at(N, S0);
+    constexpr int Ns = REV ? rplan_ns(N, S0) : plan_ns(N, S0);
+    bf_stage_ip<C, N, R, Ns, SIGN>(x, nb, sstr, tw);
+    bf_stages_ip<C, N, SIGN, REV, S0 + 1, S1>(x, nb, sstr, tw);
+  }
+}
+// items of the widest stage of N's plan per sequence (N / smallest radix)
+__host__ __device__ constexpr int plan_max_items(int n) {
+  int m = n, it = 0;
+  while (m > 1) {
+    const int r = plan_radix(m);
+    it = n / r > it ? n / r : it;
+    m /= r;
+  }
+  return it;
+}
+
 // The last stage of the plan for item jj (0 <= jj < N / R) of one sequence,
 // kept in registers: v[r] = the transform at point jj + r N / R.  The first
 // stage of the reversed plan reads exactly these points for its item jj:

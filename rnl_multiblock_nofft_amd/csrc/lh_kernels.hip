@@ -250,9 +250,57 @@ __device__ __forceinline__ C *xprod_fused(C *A, C *B, const C *TW, int S, int nz
   return bf_stages<C, N, -1, true, 1, L>(Yp, X, (isU ? 2 : 1) * nz, S, TW);
 }
 
+// The same in one LDS buffer (bf_stage_ip): possible when every stage has at
+// most one item per thread, i.e. 2 tz plan_max_items(N) <= 256 (tz rows per
+// workgroup); the host then allocates one buffer (xprod_inplace).
+__host__ __device__ inline bool xprod_inplace(int N, int tz) {
+  switch (N) {
+    case 24: case 48: case 96: case 384: return 2 * tz * plan_max_items(N) <= 256;
+    default: return false;
+  }
+}
+template <typename C, bool CFL, int N>
+__device__ __forceinline__ C *xprod_fused_ip(C *A, const C *TW, int S, int nz, bool isU, typename CT<C>::R &ma,
+                                             typename CT<C>::R &mb) {
+  typedef typename CT<C>::R R_t;
+  constexpr int L = plan_len(N), R = plan_at(N, L - 1), NB = N / R;
+  bf_stages_ip<C, N, +1, false, 0, L - 1>(A, 2 * nz, S, TW);
+  const int t = threadIdx.x, jj = t / nz, zz = t - jj * nz;
+  const bool has = t < nz * NB;
+  C pp[R], qq[R];
+  if (has) {
+    C a[R], b[R];
+    last_item<C, N, +1>(A + (2 * zz) * S, jj, TW, a);
+    last_item<C, N, +1>(A + (2 * zz + 1) * S, jj, TW, b);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const R_t a1 = a[r].x, a3 = a[r].y, a2 = b[r].x;
+      if (isU) {
+        if (CFL) {
+          ma = fmax(ma, fabs(a1));
+          mb = fmax(mb, fabs(a3));
+        }
+        pp[r] = CT<C>::mk(a1 * a1, a1 * a3);  // uu + i uw
+        qq[r] = CT<C>::mk(a2 * a2, a3 * a3);  // vv + i ww
+      } else {
+        if (CFL) ma = fmax(ma, fabs(a2));
+        pp[r] = CT<C>::mk(a1 * a2, a3 * a2);  // uv + i wv
+      }
+    }
+  }
+  __syncthreads();
+  if (has) {
+    first_item_rev<C, N, -1>(pp, jj, A + (isU ? 2 * zz : zz) * S);
+    if (isU) first_item_rev<C, N, -1>(qq, jj, A + (2 * zz + 1) * S);
+  }
+  __syncthreads();
+  bf_stages_ip<C, N, -1, true, 1, L>(A, (isU ? 2 : 1) * nz, S, TW);
+  return A;
+}
+
 // ------------------------------------------------------------------------
 // K2: inverse x transform, products, forward x transform
-template <typename C, bool CFL>
+template <typename C, bool CFL, bool IP>
 __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, int E) {
   typedef typename CT<C>::R R_t;
   const Chan ch = chans[blockIdx.z];
@@ -266,11 +314,13 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   if (z0 >= Ng2) return;
   const int nz = min(TZ, Ng2 - z0);
   extern __shared__ double2 smem_d[];
-  // 2 nz sequences of Ng1 points, S apart (padded: conflict-free stages)
+  // 2 nz sequences of Ng1 points, S apart (padded: conflict-free stages);
+  // one buffer when the stages can run in place
   const int S = seq_stride(Ng1, 2 * TZ, sizeof(C));
+  const bool inplace = IP && xprod_inplace(Ng1, TZ);
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + 2 * TZ * S;
-  C *TW = B + 2 * TZ * S;
+  C *TW = inplace ? B : B + 2 * TZ * S;
   stage_twiddles<C>(TW, v.twx, Ng1);
   const bool isU = slot < ch.nplu;
   const C *hz = Inter<C>::hz(v) + (size_t)slot * 3 * Ng2 * nxp;
@@ -327,11 +377,19 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   // |u1|, |u3| on u-planes, |u2| on v-planes (times dthdyv(j) in k_dt_update)
   R_t ma = 0, mb = 0;
   C *Y;
-  switch (Ng1) {
-    case 24: Y = xprod_fused<C, CFL, 24>(A, B, TW, S, nz, isU, ma, mb); break;
-    case 48: Y = xprod_fused<C, CFL, 48>(A, B, TW, S, nz, isU, ma, mb); break;
-    case 96: Y = xprod_fused<C, CFL, 96>(A, B, TW, S, nz, isU, ma, mb); break;
-    case 384: Y = xprod_fused<C, CFL, 384>(A, B, TW, S, nz, isU, ma, mb); break;
+  // (the in-place path is its own instantiation: its registers would lower
+  // the occupancy of the two-buffer path)
+  switch (inplace ? Ng1 : -Ng1) {
+#define LH_IP(n) Y = IP ? xprod_fused_ip<C, CFL, n>(A, TW, S, nz, isU, ma, mb) : nullptr
+    case 24: LH_IP(24); break;
+    case 48: LH_IP(48); break;
+    case 96: LH_IP(96); break;
+    case 384: LH_IP(384); break;
+#undef LH_IP
+    case -24: Y = xprod_fused<C, CFL, 24>(A, B, TW, S, nz, isU, ma, mb); break;
+    case -48: Y = xprod_fused<C, CFL, 48>(A, B, TW, S, nz, isU, ma, mb); break;
+    case -96: Y = xprod_fused<C, CFL, 96>(A, B, TW, S, nz, isU, ma, mb); break;
+    case -384: Y = xprod_fused<C, CFL, 384>(A, B, TW, S, nz, isU, ma, mb); break;
     default: {
       C *X = fft_lds_bf<C>(A, B, Ng1, 2 * nz, S, +1, TW);
       C *Yp = X == A ? B : A;
@@ -1636,7 +1694,10 @@ static size_t seq_buf(int n, int nb) {
   return (size_t)nb * std::max(seq_stride(n, nb, sizeof(double2)), seq_stride(n, nb, sizeof(float2)));
 }
 static size_t lds_zinv(int E, int Ng2, int nxh) { return 2 * seq_buf(Ng2, tile_of8(E, Ng2, nxh)) + Ng2; }
-static size_t lds_xprod(int E, int Ng1, int Ng2) { return 2 * seq_buf(Ng1, 2 * tile_of(E, Ng1, Ng2)) + Ng1; }
+static size_t lds_xprod(int E, int Ng1, int Ng2, bool ip) {
+  const int tz = tile_of(E, Ng1, Ng2);
+  return (ip && xprod_inplace(Ng1, tz) ? 1 : 2) * seq_buf(Ng1, 2 * tz) + Ng1;
+}
 static size_t lds_zfwd(int E, int Ng2, int nxh) { return 2 * seq_buf(Ng2, 4 * tile_of8(E, Ng2, nxh)) + Ng2; }
 
 // tile counts and LDS sizes for the current targets, over every channel
@@ -1651,7 +1712,7 @@ void shape_tiles(ShapeMax &m) {
     m.xprod_tiles = max(m.xprod_tiles, (d.Ng2 + tx - 1) / tx);
     m.zfwd_tiles = max(m.zfwd_tiles, (nxh + tf - 1) / tf);
     m.lds_zinv = std::max(m.lds_zinv, lds_zinv(m.e_zinv, d.Ng2, nxh));
-    m.lds_xprod = std::max(m.lds_xprod, lds_xprod(m.e_xprod, d.Ng1, d.Ng2));
+    m.lds_xprod = std::max(m.lds_xprod, lds_xprod(m.e_xprod, d.Ng1, d.Ng2, m.xprod_ip != 0));
     m.lds_zfwd = std::max(m.lds_zfwd, lds_zfwd(m.e_zfwd, d.Ng2, nxh));
   }
 }
@@ -1669,14 +1730,22 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
   } else if (stage == 1) {
     dim3 g(m.xprod_tiles, m.nslot, m.nchan);
     const bool cfl = kRK == 1;
-    if (mixed && cfl)
-      hipLaunchKernelGGL((k_xprod<float2, true>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
-    else if (mixed)
-      hipLaunchKernelGGL((k_xprod<float2, false>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
-    else if (cfl)
-      hipLaunchKernelGGL((k_xprod<double2, true>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
-    else
-      hipLaunchKernelGGL((k_xprod<double2, false>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod);
+    const bool ip = m.xprod_ip != 0;
+#define LH_XP(T, F, I) hipLaunchKernelGGL((k_xprod<T, F, I>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod)
+    if (mixed) {
+      if (cfl) {
+        if (ip) LH_XP(float2, true, true); else LH_XP(float2, true, false);
+      } else {
+        if (ip) LH_XP(float2, false, true); else LH_XP(float2, false, false);
+      }
+    } else {
+      if (cfl) {
+        if (ip) LH_XP(double2, true, true); else LH_XP(double2, true, false);
+      } else {
+        if (ip) LH_XP(double2, false, true); else LH_XP(double2, false, false);
+      }
+    }
+#undef LH_XP
   } else {
     dim3 g(xcd_grid(m.zfwd_tiles, m.nslot), 1, m.nchan);
     if (mixed)

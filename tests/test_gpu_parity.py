@@ -156,6 +156,25 @@ def test_stage_tiles(key, points):
         assert e <= TOL, (nm, e)
 
 
+@pytest.mark.parametrize("points,case", [(384, "N32_ny153"), (768, "N32_ny153"), (2048, "N32_ny153"),
+                                         (512, "N16_ny33_poly_ctP"), (256, "N8_ny17")])
+def test_xprod_inplace(points, case):
+    """The x-product stage with its transforms in place in one LDS buffer
+    (where every stage has at most one item per thread; otherwise the
+    two-buffer path) gives the oracle's step."""
+    cfg = CASES[case]
+    s, o, f = make_pair(cfg, seed=11)
+    s.set_tuning(_lib.TUNE_XPROD_POINTS, points)
+    s.set_tuning(_lib.TUNE_XPROD_INPLACE, 1)
+    assert s.get_tuning(_lib.TUNE_XPROD_INPLACE) == 1
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("xprod in place", points, case, nm, e)
+        assert e <= TOL, (nm, e)
+
+
 def test_autotune_keeps_state():
     cfg = CASES["N32_ny153"]
     s, o, f = make_pair(cfg, seed=8)
