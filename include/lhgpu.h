@@ -227,6 +227,8 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
  * thread (more workgroups per CU, more registers per thread).  Results are
  * identical. */
 #define LH_TUNE_XPROD_INPLACE 8
+/* LH_TUNE_ZFWD_INPLACE (default 0): the same for the z-forward stage. */
+#define LH_TUNE_ZFWD_INPLACE 9
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

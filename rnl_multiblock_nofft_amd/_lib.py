@@ -22,6 +22,7 @@ TUNE_TEX_SEPARABLE = 5
 TUNE_SOLVE_SCAN = 6
 TUNE_SOLVE_WAVES = 7
 TUNE_XPROD_INPLACE = 8
+TUNE_ZFWD_INPLACE = 9
 PREC_FP64, PREC_MIXED = 0, 1
 
 # every symbol include/lhgpu.h declares

@@ -1264,10 +1264,10 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     drop_graph(x);
     return LH_OK;
   }
-  if (key == LH_TUNE_XPROD_INPLACE) {
-    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_XPROD_INPLACE is 0 or 1");
+  if (key == LH_TUNE_XPROD_INPLACE || key == LH_TUNE_ZFWD_INPLACE) {
+    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: in-place switches are 0 or 1");
     ShapeMax m = x->shape;
-    m.xprod_ip = value;
+    (key == LH_TUNE_XPROD_INPLACE ? m.xprod_ip : m.zfwd_ip) = value;
     shape_tiles(m);
     if (stage_lds_bytes(x, m) > 160 * 1024) return set_err(LH_EINVAL, "lh_set_tuning: LDS exceeds 160 KiB");
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1337,6 +1337,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
     *value = x->shape.xprod_ip;
     return LH_OK;
   }
+  if (key == LH_TUNE_ZFWD_INPLACE) {
+    *value = x->shape.zfwd_ip;
+    return LH_OK;
+  }
   int *k = stage_knob(x->shape, key);
   if (!k) return set_err(LH_EINVAL, "lh_get_tuning: unknown key %d", key);
   *value = *k;
@@ -1388,10 +1392,11 @@ int lh_autotune(lh_ctx *x, int reps) {
   for (int st = 0; st < 3; st++) {
     float best_ms = 1e30f;
     for (int E : pts)
-      for (int ip = 0; ip <= (st == 1 ? 1 : 0); ip++) {
+      for (int ip = 0; ip <= (st >= 1 ? 1 : 0); ip++) {
         ShapeMax m = best;
         *stage_knob(m, keys[st]) = E;
         if (st == 1) m.xprod_ip = ip;
+        if (st == 2) m.zfwd_ip = ip;
         shape_tiles(m);
         if (stage_lds_bytes(x, m) > 160 * 1024) continue;
         float ms = time_launches(
@@ -1400,6 +1405,7 @@ int lh_autotune(lh_ctx *x, int reps) {
           best_ms = ms;
           *stage_knob(best, keys[st]) = E;
           if (st == 1) best.xprod_ip = ip;
+          if (st == 2) best.zfwd_ip = ip;
         }
       }
     shape_tiles(best);

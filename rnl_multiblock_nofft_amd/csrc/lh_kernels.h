@@ -22,6 +22,7 @@ struct ShapeMax {
   int solve_scan = 1;  // LDS column solve: recurrences split over lane groups (0: one lane per sequence)
   int solve_waves = 1;  // LDS column solve: waves per workgroup (1, 2 or 4)
   int xprod_ip = 0;     // x-product stage: in-place single-buffer stages where they fit
+  int zfwd_ip = 0;      // z-forward stage: the same
   int nphys = 0;       // N1*N2, largest over channels (texture mask kernels)
 };
 void shape_tiles(ShapeMax &m);

@@ -491,9 +491,18 @@ struct ZOut {
   }
 };
 
+// K3 in place (one LDS buffer) for the compiled lengths when every stage has
+// at most one item per thread: 4 ti plan_max_items(N) <= 256
+__host__ __device__ inline bool zfwd_inplace(int N, int ti) {
+  switch (N) {
+    case 24: case 48: case 96: case 192: case 384: return 4 * ti * plan_max_items(N) <= 256;
+    default: return false;
+  }
+}
+
 // ------------------------------------------------------------------------
 // K3: forward z transform, truncation, d/dx + d/dz
-template <typename C>
+template <typename C, bool IP>
 __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, int E, int ntile, int nslot) {
   const Chan ch = chans[blockIdx.z];
   if (ch.tri) return;
@@ -510,11 +519,13 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
   const int np = isU ? 4 : 2;
   const int j = isU ? slot + 1 : slot - ch.nplu + 1;
   extern __shared__ double2 smem_d[];
-  // np ni sequences of Ng2 points, S apart (padded: conflict-free stages)
+  // np ni sequences of Ng2 points, S apart (padded: conflict-free stages);
+  // one buffer when the stages run in place
   const int S = seq_stride(Ng2, 4 * TI, sizeof(C));
+  const bool inplace = IP && zfwd_inplace(Ng2, TI);
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + 4 * TI * S;
-  C *TW = B + 4 * TI * S;
+  C *TW = inplace ? B : B + 4 * TI * S;
   stage_twiddles<C>(TW, v.twz, Ng2);
   const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * Ng2 * nxp;
   {
@@ -544,7 +555,21 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
           N2, Ng2, isU};
   // (a register-fused last stage, as in k_zinv, needs two sequences' points
   // live per item; at FP64 that cost occupancy more than it saved)
-  C *R = fft_lds_bf<C>(A, B, Ng2, np * ni, S, -1, TW);
+  C *R = A;
+  bool done = false;
+  if constexpr (IP) {
+    if (inplace) {
+      done = true;
+      switch (Ng2) {
+      case 24: bf_stages_ip<C, 24, -1, false, 0, plan_len(24)>(A, np * ni, S, TW); break;
+      case 48: bf_stages_ip<C, 48, -1, false, 0, plan_len(48)>(A, np * ni, S, TW); break;
+      case 96: bf_stages_ip<C, 96, -1, false, 0, plan_len(96)>(A, np * ni, S, TW); break;
+      case 192: bf_stages_ip<C, 192, -1, false, 0, plan_len(192)>(A, np * ni, S, TW); break;
+        default: bf_stages_ip<C, 384, -1, false, 0, plan_len(384)>(A, np * ni, S, TW); break;
+      }
+    }
+  }
+  if (!done) R = fft_lds_bf<C>(A, B, Ng2, np * ni, S, -1, TW);
   const double inv = 1.0 / Ng2;
   const int half = N2 / 2, dz = Ng2 - N2;
   Walk2 wo(threadIdx.x, blockDim.x, ni);  // t = k ni + ii
@@ -1698,7 +1723,10 @@ static size_t lds_xprod(int E, int Ng1, int Ng2, bool ip) {
   const int tz = tile_of(E, Ng1, Ng2);
   return (ip && xprod_inplace(Ng1, tz) ? 1 : 2) * seq_buf(Ng1, 2 * tz) + Ng1;
 }
-static size_t lds_zfwd(int E, int Ng2, int nxh) { return 2 * seq_buf(Ng2, 4 * tile_of8(E, Ng2, nxh)) + Ng2; }
+static size_t lds_zfwd(int E, int Ng2, int nxh, bool ip) {
+  const int ti = tile_of8(E, Ng2, nxh);
+  return (ip && zfwd_inplace(Ng2, ti) ? 1 : 2) * seq_buf(Ng2, 4 * ti) + Ng2;
+}
 
 // tile counts and LDS sizes for the current targets, over every channel
 void shape_tiles(ShapeMax &m) {
@@ -1713,7 +1741,7 @@ void shape_tiles(ShapeMax &m) {
     m.zfwd_tiles = max(m.zfwd_tiles, (nxh + tf - 1) / tf);
     m.lds_zinv = std::max(m.lds_zinv, lds_zinv(m.e_zinv, d.Ng2, nxh));
     m.lds_xprod = std::max(m.lds_xprod, lds_xprod(m.e_xprod, d.Ng1, d.Ng2, m.xprod_ip != 0));
-    m.lds_zfwd = std::max(m.lds_zfwd, lds_zfwd(m.e_zfwd, d.Ng2, nxh));
+    m.lds_zfwd = std::max(m.lds_zfwd, lds_zfwd(m.e_zfwd, d.Ng2, nxh, m.zfwd_ip != 0));
   }
 }
 
@@ -1748,10 +1776,14 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
 #undef LH_XP
   } else {
     dim3 g(xcd_grid(m.zfwd_tiles, m.nslot), 1, m.nchan);
-    if (mixed)
-      hipLaunchKernelGGL(k_zfwd<float2>, g, b, m.lds_zfwd * es, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot);
-    else
-      hipLaunchKernelGGL(k_zfwd<double2>, g, b, m.lds_zfwd * es, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot);
+#define LH_ZF(T, I) \
+  hipLaunchKernelGGL((k_zfwd<T, I>), g, b, m.lds_zfwd * es, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot)
+    if (mixed) {
+      if (m.zfwd_ip) LH_ZF(float2, true); else LH_ZF(float2, false);
+    } else {
+      if (m.zfwd_ip) LH_ZF(double2, true); else LH_ZF(double2, false);
+    }
+#undef LH_ZF
   }
   return hipGetLastError();
 }

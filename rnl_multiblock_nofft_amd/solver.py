@@ -195,7 +195,8 @@ class Solver:
                 "solve_cols": self.get_tuning(_lib.TUNE_SOLVE_COLS),
                 "solve_scan": self.get_tuning(_lib.TUNE_SOLVE_SCAN),
                 "solve_waves": self.get_tuning(_lib.TUNE_SOLVE_WAVES),
-                "xprod_inplace": self.get_tuning(_lib.TUNE_XPROD_INPLACE)}
+                "xprod_inplace": self.get_tuning(_lib.TUNE_XPROD_INPLACE),
+                "zfwd_inplace": self.get_tuning(_lib.TUNE_ZFWD_INPLACE)}
 
     def sync(self):
         check(self.L.lh_sync(self.h))

@@ -156,22 +156,27 @@ def test_stage_tiles(key, points):
         assert e <= TOL, (nm, e)
 
 
-@pytest.mark.parametrize("points,case", [(384, "N32_ny153"), (768, "N32_ny153"), (2048, "N32_ny153"),
-                                         (512, "N16_ny33_poly_ctP"), (256, "N8_ny17")])
-def test_xprod_inplace(points, case):
-    """The x-product stage with its transforms in place in one LDS buffer
-    (where every stage has at most one item per thread; otherwise the
-    two-buffer path) gives the oracle's step."""
+@pytest.mark.parametrize("stage,points,case", [("xprod", 384, "N32_ny153"), ("xprod", 768, "N32_ny153"),
+                                               ("xprod", 2048, "N32_ny153"), ("xprod", 512, "N16_ny33_poly_ctP"),
+                                               ("xprod", 256, "N8_ny17"), ("zfwd", 384, "N32_ny153"),
+                                               ("zfwd", 100, "N32_ny153"), ("zfwd", 3000, "N32_ny153"),
+                                               ("zfwd", 200, "N16_ny33_poly_ctP")])
+def test_stage_inplace(stage, points, case):
+    """The x-product and z-forward stages with their transforms in place in
+    one LDS buffer (where every stage has at most one item per thread;
+    otherwise the two-buffer path) give the oracle's step."""
     cfg = CASES[case]
     s, o, f = make_pair(cfg, seed=11)
-    s.set_tuning(_lib.TUNE_XPROD_POINTS, points)
-    s.set_tuning(_lib.TUNE_XPROD_INPLACE, 1)
-    assert s.get_tuning(_lib.TUNE_XPROD_INPLACE) == 1
+    pk, ik = ((_lib.TUNE_XPROD_POINTS, _lib.TUNE_XPROD_INPLACE) if stage == "xprod"
+              else (_lib.TUNE_ZFWD_POINTS, _lib.TUNE_ZFWD_INPLACE))
+    s.set_tuning(pk, points)
+    s.set_tuning(ik, 1)
+    assert s.get_tuning(ik) == 1
     s.step(1)
     o.step(1)
     for nm in ("u1", "u2", "u3", "p"):
         e = relerr(s.download(0, nm), o.get(nm))
-        say("xprod in place", points, case, nm, e)
+        say(stage, "in place", points, case, nm, e)
         assert e <= TOL, (nm, e)
 
 
