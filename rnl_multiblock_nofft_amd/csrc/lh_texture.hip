@@ -433,6 +433,16 @@ __global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, i
   const bool realpart = i == 0 || i == N1 / 2;
   const double2 *src = f == 0 ? v.u1 : v.u3;
   const double invz = 1.0 / N2;
+  // the no-slip values U and table offsets g - gbar of this x-mode's columns,
+  // read once for every pass (each thread keeps its own k)
+  double2 *Us = reinterpret_cast<double2 *>(M + N2);
+  double *Ds = reinterpret_cast<double *>(Us + N2);
+  if (mode == 0)
+    for (int k = tid; k < N2; k += nt) {
+      const int c = i + nxh * k;
+      Us[k] = v.texu[(size_t)(f * 2 + w) * nc + c];
+      Ds[k] = Wr[(size_t)gj * ldc + c] - g0;
+    }
   double2 *S = A;
   const int passes = mode == 0 ? nit : 1;
   for (int it = 0; it < passes; it++) {
@@ -440,8 +450,8 @@ __global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, i
       const int c = i + nxh * k;
       double2 val;
       if (mode == 0) {
-        const double2 U = v.texu[(size_t)(f * 2 + w) * nc + c];
-        const double dg = Wr[(size_t)gj * ldc + c] - g0;
+        const double2 U = Us[k];
+        const double dg = Ds[k];
         const double2 sg = it > 0 ? S[k] : make_double2(0, 0);
         val = make_double2(U.x + dg * sg.x, U.y + dg * sg.y);
       } else {
@@ -468,7 +478,7 @@ __global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, i
   }
 }
 
-static size_t tex_1d_lds(int n2max) { return (size_t)3 * n2max * sizeof(double2) + (size_t)n2max * sizeof(double); }
+static size_t tex_1d_lds(int n2max) { return (size_t)4 * n2max * sizeof(double2) + (size_t)2 * n2max * sizeof(double); }
 
 static int shape_n2max(const ShapeMax &m) {
   int n2 = 0;
