@@ -204,13 +204,14 @@ def main():
         keys = {"zinv_points": _lib.TUNE_ZINV_POINTS, "xprod_points": _lib.TUNE_XPROD_POINTS,
                 "zfwd_points": _lib.TUNE_ZFWD_POINTS, "solve_cols": _lib.TUNE_SOLVE_COLS,
                 "solve_scan": _lib.TUNE_SOLVE_SCAN, "solve_waves": _lib.TUNE_SOLVE_WAVES,
-                "xprod_inplace": _lib.TUNE_XPROD_INPLACE, "zfwd_inplace": _lib.TUNE_ZFWD_INPLACE}
+                "xprod_inplace": _lib.TUNE_XPROD_INPLACE, "zfwd_inplace": _lib.TUNE_ZFWD_INPLACE,
+                "xprod_threads": _lib.TUNE_XPROD_THREADS}
         for kv in args.tune.split(","):
             k, v = kv.split("=")
             s.set_tuning(keys[k], int(v))
         tuning = s.tuning()
     else:
-        tuning = s.autotune(10)
+        tuning = s.autotune(30)
     s.step(args.warmup)
     s.sync()
     torch.cuda.synchronize()

@@ -229,6 +229,9 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
 #define LH_TUNE_XPROD_INPLACE 8
 /* LH_TUNE_ZFWD_INPLACE (default 0): the same for the z-forward stage. */
 #define LH_TUNE_ZFWD_INPLACE 9
+/* LH_TUNE_XPROD_THREADS (default 256): threads per workgroup of the
+ * x-product stage, 128 or 256. */
+#define LH_TUNE_XPROD_THREADS 10
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

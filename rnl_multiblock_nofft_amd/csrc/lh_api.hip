@@ -1264,6 +1264,17 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     drop_graph(x);
     return LH_OK;
   }
+  if (key == LH_TUNE_XPROD_THREADS) {
+    if (value != 128 && value != 256) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_XPROD_THREADS is 128 or 256");
+    ShapeMax m = x->shape;
+    m.xprod_threads = value;
+    shape_tiles(m);
+    if (stage_lds_bytes(x, m) > 160 * 1024) return set_err(LH_EINVAL, "lh_set_tuning: LDS exceeds 160 KiB");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->shape = m;
+    drop_graph(x);
+    return LH_OK;
+  }
   if (key == LH_TUNE_XPROD_INPLACE || key == LH_TUNE_ZFWD_INPLACE) {
     if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: in-place switches are 0 or 1");
     ShapeMax m = x->shape;
@@ -1341,6 +1352,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
     *value = x->shape.zfwd_ip;
     return LH_OK;
   }
+  if (key == LH_TUNE_XPROD_THREADS) {
+    *value = x->shape.xprod_threads;
+    return LH_OK;
+  }
   int *k = stage_knob(x->shape, key);
   if (!k) return set_err(LH_EINVAL, "lh_get_tuning: unknown key %d", key);
   *value = *k;
@@ -1392,10 +1407,14 @@ int lh_autotune(lh_ctx *x, int reps) {
   for (int st = 0; st < 3; st++) {
     float best_ms = 1e30f;
     for (int E : pts)
-      for (int ip = 0; ip <= (st >= 1 ? 1 : 0); ip++) {
+      for (int v2 = 0; v2 < (st >= 1 ? 2 : 1) * (st == 1 ? 2 : 1); v2++) {
+        const int ip = v2 & 1, nt = (v2 >> 1) ? 128 : 256;
         ShapeMax m = best;
         *stage_knob(m, keys[st]) = E;
-        if (st == 1) m.xprod_ip = ip;
+        if (st == 1) {
+          m.xprod_ip = ip;
+          m.xprod_threads = nt;
+        }
         if (st == 2) m.zfwd_ip = ip;
         shape_tiles(m);
         if (stage_lds_bytes(x, m) > 160 * 1024) continue;
@@ -1404,7 +1423,10 @@ int lh_autotune(lh_ctx *x, int reps) {
         if (ms >= 0.f && ms < best_ms) {
           best_ms = ms;
           *stage_knob(best, keys[st]) = E;
-          if (st == 1) best.xprod_ip = ip;
+          if (st == 1) {
+            best.xprod_ip = ip;
+            best.xprod_threads = nt;
+          }
           if (st == 2) best.zfwd_ip = ip;
         }
       }

@@ -23,6 +23,7 @@ struct ShapeMax {
   int solve_waves = 1;  // LDS column solve: waves per workgroup (1, 2 or 4)
   int xprod_ip = 0;     // x-product stage: in-place single-buffer stages where they fit
   int zfwd_ip = 0;      // z-forward stage: the same
+  int xprod_threads = 256;  // x-product stage: threads per workgroup (128 or 256)
   int nphys = 0;       // N1*N2, largest over channels (texture mask kernels)
 };
 void shape_tiles(ShapeMax &m);

@@ -251,11 +251,11 @@ __device__ __forceinline__ C *xprod_fused(C *A, C *B, const C *TW, int S, int nz
 }
 
 // The same in one LDS buffer (bf_stage_ip): possible when every stage has at
-// most one item per thread, i.e. 2 tz plan_max_items(N) <= 256 (tz rows per
-// workgroup); the host then allocates one buffer (xprod_inplace).
-__host__ __device__ inline bool xprod_inplace(int N, int tz) {
+// most one item per thread, i.e. 2 tz plan_max_items(N) <= nt (tz rows per
+// workgroup of nt threads); the host then allocates one buffer.
+__host__ __device__ inline bool xprod_inplace(int N, int tz, int nt) {
   switch (N) {
-    case 24: case 48: case 96: case 384: return 2 * tz * plan_max_items(N) <= 256;
+    case 24: case 48: case 96: case 384: return 2 * tz * plan_max_items(N) <= nt;
     default: return false;
   }
 }
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   // 2 nz sequences of Ng1 points, S apart (padded: conflict-free stages);
   // one buffer when the stages can run in place
   const int S = seq_stride(Ng1, 2 * TZ, sizeof(C));
-  const bool inplace = IP && xprod_inplace(Ng1, TZ);
+  const bool inplace = IP && xprod_inplace(Ng1, TZ, blockDim.x);
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + 2 * TZ * S;
   C *TW = inplace ? B : B + 2 * TZ * S;
@@ -1719,9 +1719,9 @@ static size_t seq_buf(int n, int nb) {
   return (size_t)nb * std::max(seq_stride(n, nb, sizeof(double2)), seq_stride(n, nb, sizeof(float2)));
 }
 static size_t lds_zinv(int E, int Ng2, int nxh) { return 2 * seq_buf(Ng2, tile_of8(E, Ng2, nxh)) + Ng2; }
-static size_t lds_xprod(int E, int Ng1, int Ng2, bool ip) {
+static size_t lds_xprod(int E, int Ng1, int Ng2, bool ip, int nt) {
   const int tz = tile_of(E, Ng1, Ng2);
-  return (ip && xprod_inplace(Ng1, tz) ? 1 : 2) * seq_buf(Ng1, 2 * tz) + Ng1;
+  return (ip && xprod_inplace(Ng1, tz, nt) ? 1 : 2) * seq_buf(Ng1, 2 * tz) + Ng1;
 }
 static size_t lds_zfwd(int E, int Ng2, int nxh, bool ip) {
   const int ti = tile_of8(E, Ng2, nxh);
@@ -1740,7 +1740,7 @@ void shape_tiles(ShapeMax &m) {
     m.xprod_tiles = max(m.xprod_tiles, (d.Ng2 + tx - 1) / tx);
     m.zfwd_tiles = max(m.zfwd_tiles, (nxh + tf - 1) / tf);
     m.lds_zinv = std::max(m.lds_zinv, lds_zinv(m.e_zinv, d.Ng2, nxh));
-    m.lds_xprod = std::max(m.lds_xprod, lds_xprod(m.e_xprod, d.Ng1, d.Ng2, m.xprod_ip != 0));
+    m.lds_xprod = std::max(m.lds_xprod, lds_xprod(m.e_xprod, d.Ng1, d.Ng2, m.xprod_ip != 0, m.xprod_threads));
     m.lds_zfwd = std::max(m.lds_zfwd, lds_zfwd(m.e_zfwd, d.Ng2, nxh, m.zfwd_ip != 0));
   }
 }
@@ -1759,7 +1759,8 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
     dim3 g(m.xprod_tiles, m.nslot, m.nchan);
     const bool cfl = kRK == 1;
     const bool ip = m.xprod_ip != 0;
-#define LH_XP(T, F, I) hipLaunchKernelGGL((k_xprod<T, F, I>), g, b, m.lds_xprod * es, s, arena, d_chans, m.e_xprod)
+    const dim3 bx(m.xprod_threads);
+#define LH_XP(T, F, I) hipLaunchKernelGGL((k_xprod<T, F, I>), g, bx, m.lds_xprod * es, s, arena, d_chans, m.e_xprod)
     if (mixed) {
       if (cfl) {
         if (ip) LH_XP(float2, true, true); else LH_XP(float2, true, false);

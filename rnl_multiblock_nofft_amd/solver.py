@@ -196,7 +196,8 @@ class Solver:
                 "solve_scan": self.get_tuning(_lib.TUNE_SOLVE_SCAN),
                 "solve_waves": self.get_tuning(_lib.TUNE_SOLVE_WAVES),
                 "xprod_inplace": self.get_tuning(_lib.TUNE_XPROD_INPLACE),
-                "zfwd_inplace": self.get_tuning(_lib.TUNE_ZFWD_INPLACE)}
+                "zfwd_inplace": self.get_tuning(_lib.TUNE_ZFWD_INPLACE),
+                "xprod_threads": self.get_tuning(_lib.TUNE_XPROD_THREADS)}
 
     def sync(self):
         check(self.L.lh_sync(self.h))

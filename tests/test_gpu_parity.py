@@ -180,6 +180,24 @@ def test_stage_inplace(stage, points, case):
         assert e <= TOL, (nm, e)
 
 
+@pytest.mark.parametrize("threads,ip,points", [(128, 0, 768), (128, 1, 384), (128, 1, 768), (128, 0, 2048)])
+def test_xprod_threads(threads, ip, points):
+    """The x-product stage in workgroups of 128 threads (two-buffer and
+    in-place forms) gives the oracle's step."""
+    cfg = CASES["N32_ny153"]
+    s, o, f = make_pair(cfg, seed=12)
+    s.set_tuning(_lib.TUNE_XPROD_POINTS, points)
+    s.set_tuning(_lib.TUNE_XPROD_THREADS, threads)
+    s.set_tuning(_lib.TUNE_XPROD_INPLACE, ip)
+    assert s.get_tuning(_lib.TUNE_XPROD_THREADS) == threads
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("xprod threads", threads, ip, points, nm, e)
+        assert e <= TOL, (nm, e)
+
+
 def test_autotune_keeps_state():
     cfg = CASES["N32_ny153"]
     s, o, f = make_pair(cfg, seed=8)
