@@ -1166,6 +1166,16 @@ __global__ void __launch_bounds__(256) k_solve_lds(char *arena, const Chan *chan
   const int ne = nyu - 2;
   const int ncl = min(C, ncol - c0);  // live columns of this group
 
+  // p of the projection phase's first PB items per thread, loaded now so
+  // that the load is not a round trip after the recurrences
+  constexpr int PB = 4;
+  double2 Ppre[PB];
+#pragma unroll
+  for (int u = 0; u < PB; u++) {
+    const int t = min(u * NT + lane, ne * C - 1);
+    const int q = 1 + t / C, c = min(c0 + t % C, ncol - 1);
+    Ppre[u] = v.p[(size_t)q * ldc + c];
+  }
   // ---- L: stage right-hand sides, pivots, row coefficients and the
   //      columns' wavenumbers (dead columns read column ncol-1); loads are
   //      issued LB iterations at a time so that they are all in flight
@@ -1445,6 +1455,10 @@ __global__ void __launch_bounds__(256) k_solve_lds(char *arena, const Chan *chan
     double2 P[LB];
 #pragma unroll
     for (int u = 0; u < LB; u++) {
+      if (t0 == 0 && u < PB) {
+        P[u] = Ppre[u];
+        continue;
+      }
       const int t = min(t0 + u * NT + lane, ne * C - 1);
       const int q = 1 + t / C, c = min(c0 + t % C, ncol - 1);
       P[u] = v.p[(size_t)q * ldc + c];
