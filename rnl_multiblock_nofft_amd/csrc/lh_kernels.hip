@@ -1094,7 +1094,7 @@ __device__ __forceinline__ double rec_step(double z, double r, double a, double 
   return FORM == 0 ? (r - a * z) * d : r - (a * d) * z;
 }
 template <int FORM>
-__device__ double rec_sweep(double *X, int xs, const double *A, const double *D, int ds, int first, int dir,
+__device__ __forceinline__ double rec_sweep(double *X, int xs, const double *A, const double *D, int ds, int first, int dir,
                             int count, double zin, int k, int L, bool serial) {
   serial = serial || L == 1;
   const int cs = serial ? count : (count + L - 1) / L;
