@@ -111,7 +111,9 @@ def triad_bytes(cfg, files):
             + 2 * (5 * nplu + 4 * nplv) * row + 3 * (nplu + nplv) * row)
 
 
-def cpu_baseline(cfg, seconds=12.0, max_sub=60, triads_mode=None):
+def cpu_baseline(cfg, seconds=12.0, max_sub=100000, triads_mode=None):
+    """The FP64 CPU oracle (OpenMP, up to 16 threads) on one channel of the
+    workload for about `seconds` of CPU work: a reported baseline."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import init_fields, oracle_for
     from oracle import pyoracle as po
