@@ -422,6 +422,37 @@ def test_mixed_precision_error():
         assert e <= 1e-5, (nm, e)
 
 
+@pytest.mark.parametrize("knobs", [dict(xprod_ip=1, xprod_threads=256, zfwd_ip=0, xpts=768, zpts=384),
+                                   dict(xprod_ip=1, xprod_threads=128, zfwd_ip=1, xpts=384, zpts=256),
+                                   dict(xprod_ip=0, xprod_threads=128, zfwd_ip=1, xpts=1024, zpts=384)])
+def test_mixed_precision_launch_forms(knobs):
+    """Mixed precision with the in-place and 128-thread stage forms the
+    autotuner can pick: same bound as the default forms against the FP64
+    oracle, and the default forms' result to FP32 rounding (velocities 1e-7;
+    p, whose mixed-precision error is itself ~6e-6, 1e-5)."""
+    cfg = configs.smooth_32()[0]
+    s, o, f = make_pair(cfg, precision="mixed")
+    ref = Solver([cfg], precision="mixed")
+    for name in ("u1", "u2", "u3", "p"):
+        ref.upload(0, name, f[name])
+    ref.init_flowrate()
+    s.set_tuning(_lib.TUNE_XPROD_POINTS, knobs["xpts"])
+    s.set_tuning(_lib.TUNE_ZFWD_POINTS, knobs["zpts"])
+    s.set_tuning(_lib.TUNE_XPROD_THREADS, knobs["xprod_threads"])
+    s.set_tuning(_lib.TUNE_XPROD_INPLACE, knobs["xprod_ip"])
+    s.set_tuning(_lib.TUNE_ZFWD_INPLACE, knobs["zfwd_ip"])
+    s.step(1)
+    o.step(1)
+    ref.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        d = relerr(s.download(0, nm), ref.download(0, nm))
+        say("mixed forms", knobs, nm, e, d)
+        assert e <= 1e-5, (nm, e)
+        assert d <= (1e-5 if nm == "p" else 1e-7), (nm, d)
+    ref.close()
+
+
 def test_bad_config_raises():
     cfg = small_cfg(8, 17)
     cfg["Ngal1"] = 10  # below the 3/2 rule
