@@ -346,8 +346,12 @@ int lh_slip_stats_get(lh_ctx *ctx, int ch, int wall, double *maps);
 #define LH_CL_NOUT 3
 #define LH_CL_NPARAM 50819
 int lh_closure_set(lh_ctx *ctx, const float *params, const float *norm9);
-/* on = 1: every substep of channel ch adds the correction (after the
- * products, before the right-hand side).  Drops the captured step graph. */
+/* on = 1: every substep of channel ch evaluates the network and adds its
+ * correction (after the products, before the right-hand side).  on = 2: the
+ * network runs at the first substep of each step only, on the state at the
+ * start of the step; substeps 2 and 3 add the same correction (a third of
+ * the network's cost; the correction is held over one step).  on = 0: off.
+ * Drops the captured step graph. */
 int lh_closure_enable(lh_ctx *ctx, int ch, int on);
 /* On the current fields of channel ch: d_phys (device, floats
  * [3][ny+1][N2][N1]) gets u1, u2, u3 on the N1 x N2 grid (u2 rows 0..ny-1,

@@ -242,16 +242,23 @@ def sample(fine, cfg_f, coarse, cfg_c):
     return phys, tau
 
 
-def substep_with_closure(o, kRK, params, norm9, emulate=True):
+def substep_with_closure(o, kRK, params, norm9, emulate=True, held=None):
     """One oracle substep (lho_substep order) with the network's correction
     added to the full nonlinear term (which the next substep also sees as
-    its old N) and to du."""
+    its old N) and to du.  With `held` (a dict), the correction is computed
+    at kRK = 1 only, kept in held["add"] and reused at kRK = 2, 3 (the
+    once-per-step mode)."""
     cfg = o.cfg
     N1, N2, ny = cfg["N1"], cfg["N2"], cfg["ny"]
     o.phase("rhs0", kRK)
     phys = phys_planes(o.get("u1"), o.get("u2"), o.get("u3"), N1, N2)
     o.phase("nonlinear", kRK)
-    add = correction_modes(network_correction(params, norm9, phys, emulate), N1, N2, ny)
+    if held is not None and kRK != 1:
+        add = held["add"]
+    else:
+        add = correction_modes(network_correction(params, norm9, phys, emulate), N1, N2, ny)
+        if held is not None:
+            held["add"] = add
     g = (8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0)[kRK - 1]
     for f, a in enumerate(add):
         nu = "Nu%d" % (f + 1)

@@ -117,8 +117,11 @@ def set_network(solver, params, norm9):
                                   _fp(np.ascontiguousarray(norm9, np.float32))))
 
 
-def enable(solver, ch, on=True):
-    check(solver.L.lh_closure_enable(solver.h, ch, 1 if on else 0))
+def enable(solver, ch, on=True, once_per_step=False):
+    """Switch the correction on (every substep evaluates the network) or, with
+    once_per_step, evaluate it at the first substep of each step and reuse
+    it at the other two."""
+    check(solver.L.lh_closure_enable(solver.h, ch, (2 if once_per_step else 1) if on else 0))
 
 
 def evaluate(solver, ch, with_correction=True):

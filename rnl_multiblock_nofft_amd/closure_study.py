@@ -14,11 +14,12 @@
 2. The network (135 -> 128 -> 128 -> 128 -> 3) trains on the first 90 % of
    the samples with AdamW; its a-priori error is measured on the last 10 %.
 3. From the filtered fine state at the end of sampling, a coarse run with
-   the closure and one without advance `post_steps` steps beside the fine
-   run, all accumulating mean and rms profiles every 10 steps (the fine run's
+   the closure (network at every substep), one with the network at the
+   first substep of each step only (correction held for the step) and one
+   without advance `post_steps` steps beside the fine run, all accumulating mean and rms profiles every 10 steps (the fine run's
    filtered fields are accumulated too); the coarse profiles are compared
    with the fine run's.
-4. GPU time per step of the fine run and of the coarse run with the closure.
+4. GPU time per step of the fine run and of the coarse runs.
 
 Prints progress lines and one JSON line with the results.
 """
@@ -110,7 +111,7 @@ def run(args):
     runs = {}
     tmp = closure.Snapshots(cc, 1)
     mpgx = fine.scalars(0)["mpgx"]
-    for name, on in (("closure", True), ("plain", False)):
+    for name, on in (("closure", 1), ("closure_held", 2), ("plain", 0)):
         s = Solver([cc])
         tmp.count = 0
         tmp.add(fine, 0, s, 0)
@@ -118,7 +119,7 @@ def run(args):
         s.set_scalars(0, t=fine.scalars(0)["t"], mpgx=mpgx)
         if on:
             closure.set_network(s, params, norm)
-            closure.enable(s, 0)
+            closure.enable(s, 0, once_per_step=on == 2)
         s.autotune(5)
         runs[name] = s
     fine.stats_reset()
@@ -151,6 +152,7 @@ def run(args):
 
     ms_fine = 1e3 * time_steps(fine, 50)
     ms_cl = 1e3 * time_steps(runs["closure"], 200)
+    ms_held = 1e3 * time_steps(runs["closure_held"], 200)
     ms_plain = 1e3 * time_steps(runs["plain"], 200)
     ms_parts = closure.closure_time(runs["closure"], 50)
     out = dict(fine="%dx%dx%d" % (args.fine, args.fine, args.ny), coarse="%dx%dx%d" % (args.coarse, args.coarse, args.ny),
@@ -160,9 +162,11 @@ def run(args):
                apriori_heldout=None if held is None else [float(v) for v in held],
                apriori_train=[float(v) for v in train_err],
                post_steps=args.post_steps, profile_errors=res,
-               ms_per_step=dict(fine=ms_fine, coarse_closure=ms_cl, coarse_plain=ms_plain),
+               ms_per_step=dict(fine=ms_fine, coarse_closure=ms_cl, coarse_closure_held=ms_held,
+                                coarse_plain=ms_plain),
                closure_ms_per_substep=dict(planes=ms_parts[0], network=ms_parts[1], to_modes=ms_parts[2]),
                gpu_time_ratio_fine_over_closure=ms_fine / ms_cl,
+               gpu_time_ratio_fine_over_closure_held=ms_fine / ms_held,
                seconds=dict(data=t_data, train=t_train))
     print(json.dumps(out), flush=True)
     return out

@@ -504,8 +504,10 @@ int cl_buffers(lh_ctx *x, int c, float **phys, float **corr, double2 **tau) {
 }
 
 // the network's correction of the nonlinear term of every channel that has
-// it switched on (after the products, before the right-hand side)
-int closure_stage(lh_ctx *x) {
+// it switched on (after the products, before the right-hand side).  Mode 2
+// evaluates the network at the first substep of a step only and adds the
+// same correction planes at the other two.
+int closure_stage(lh_ctx *x, int kRK) {
   for (size_t c = 0; c < x->ch.size(); c++) {
     if (!x->cl_on[c]) continue;
     HostChan &h = x->ch[c];
@@ -515,9 +517,11 @@ int closure_stage(lh_ctx *x) {
     if (rc) return rc;
     View d(x->arena, h.dev);
     ClSrc src{{d.u1, d.u2, d.u3}, {h.nyu, h.nyv, h.nyu}};
-    HIPCHK(launch_cl_phys(x->arena, x->d_chans + c, h.dev, src, phys, x->stream));
-    HIPCHK(launch_cl_infer(x->cl_net, (const float *)(x->cl_net + CL_FRAG_BYTES), x->cl_norm, phys, corr, h.cfg.N1,
-                           h.cfg.N2, h.nyu, h.nyv, x->stream));
+    if (x->cl_on[c] == 1 || kRK == 1) {
+      HIPCHK(launch_cl_phys(x->arena, x->d_chans + c, h.dev, src, phys, x->stream));
+      HIPCHK(launch_cl_infer(x->cl_net, (const float *)(x->cl_net + CL_FRAG_BYTES), x->cl_norm, phys, corr,
+                             h.cfg.N1, h.cfg.N2, h.nyu, h.nyv, x->stream));
+    }
     HIPCHK(launch_cl_to_modes(x->arena, x->d_chans + c, h.dev, corr, x->stream));
   }
   return LH_OK;
@@ -526,7 +530,7 @@ int closure_stage(lh_ctx *x) {
 int nonlinear_stage(lh_ctx *x, int kRK) {
   int rc = raw_nonlinear(x, kRK);
   if (rc || !x->ncl) return rc;
-  return closure_stage(x);
+  return closure_stage(x, kRK);
 }
 
 int time_step(lh_ctx *x, int kRK) {
@@ -909,7 +913,7 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
       // the triad-list kernels are counted with the x-product stage, the
       // closure with the z-forward one
       if (st == 1 && x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
-      if (st == 2 && x->ncl && (rc = closure_stage(x))) break;
+      if (st == 2 && x->ncl && (rc = closure_stage(x, kRK))) break;
       HIPCHK(hipEventRecord(ev[st + 1], x->stream));
     }
     if (rc) break;
@@ -1698,15 +1702,16 @@ int lh_closure_enable(lh_ctx *x, int ch, int on) {
   int rc = check_ctx(x);
   if (rc) return rc;
   if ((rc = cl_chan_ok(x, ch, "lh_closure_enable"))) return rc;
+  if (on < 0 || on > 2) return set_err(LH_EINVAL, "lh_closure_enable: on is 0, 1 or 2");
   if (on && !x->cl_net) return set_err(LH_ESTATE, "lh_closure_enable: no network set (lh_closure_set)");
   if (on) {
     float *ph, *co;
     double2 *ta;
     if ((rc = cl_buffers(x, ch, &ph, &co, &ta))) return rc;
   }
-  x->cl_on[ch] = on ? 1 : 0;
+  x->cl_on[ch] = on;
   x->ncl = 0;
-  for (int c : x->cl_on) x->ncl += c;
+  for (int c : x->cl_on) x->ncl += c != 0;
   drop_graph(x);
   return LH_OK;
 }

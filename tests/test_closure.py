@@ -369,6 +369,36 @@ def test_gpu_closure_step_matches_oracle():
 
 
 @gpu
+def test_gpu_closure_once_per_step_matches_oracle():
+    """Mode 2 (network at the first substep, correction held for the step)
+    equals the oracle's held-correction substeps to 1e-2 of the closure's
+    effect, over two steps (a graph replay)."""
+    from rnl_multiblock_nofft_amd import closure
+    cfg, s, o = _coarse_pair(seed=22)
+    o_plain = oracle_for(cfg)
+    for name in ("u1", "u2", "u3", "p"):
+        o_plain.set(name, o.get(name))
+    o_plain.init_flowrate()
+    P = rand_params(14).astype(np.float32)
+    norm = np.array([0.3, 0.0, 0.0, 2.0, 20.0, 20.0, 0.5, 0.2, 0.3], np.float32)
+    closure.set_network(s, P, norm)
+    closure.enable(s, 0, once_per_step=True)
+    s.step(2)
+    for step in range(2):
+        held = {}
+        for kRK in (1, 2, 3):
+            co.substep_with_closure(o, kRK, P, norm, emulate=True, held=held)
+    o_plain.step(2)
+    for name in ("u1", "u2", "u3", "p"):
+        g, r, r0 = s.download(0, name), o.get(name), o_plain.get(name)
+        effect = np.linalg.norm(r - r0)
+        e = np.linalg.norm(g - r) / effect
+        say(name, "held closure effect", effect / np.linalg.norm(r0), "gpu - oracle / effect", e)
+        assert effect > 1e-8 * np.linalg.norm(r0)
+        assert e < 1e-2, name
+
+
+@gpu
 def test_gpu_sample_matches_oracle():
     from rnl_multiblock_nofft_amd import Solver, closure
     cf, cc = small_cfg(16, 17), small_cfg(8, 17)
