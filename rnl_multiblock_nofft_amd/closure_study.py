@@ -15,7 +15,8 @@
    the samples with AdamW; its a-priori error is measured on the last 10 %.
 3. From the filtered fine state at the end of sampling, a coarse run with
    the closure (network at every substep), one with the network at the
-   first substep of each step only (correction held for the step) and one
+   first substep of each step only (correction held for the step), the
+   held form in mixed precision (FP32 transforms, FP64 solves) and one
    without advance `post_steps` steps beside the fine run, all accumulating mean and rms profiles every 10 steps (the fine run's
    filtered fields are accumulated too); the coarse profiles are compared
    with the fine run's.
@@ -111,8 +112,9 @@ def run(args):
     runs = {}
     tmp = closure.Snapshots(cc, 1)
     mpgx = fine.scalars(0)["mpgx"]
-    for name, on in (("closure", 1), ("closure_held", 2), ("plain", 0)):
-        s = Solver([cc])
+    for name, on, prec in (("closure", 1, "fp64"), ("closure_held", 2, "fp64"), ("closure_held_mixed", 2, "mixed"),
+                           ("plain", 0, "fp64")):
+        s = Solver([cc], precision=prec)
         tmp.count = 0
         tmp.add(fine, 0, s, 0)
         s.init_flowrate()
@@ -153,6 +155,7 @@ def run(args):
     ms_fine = 1e3 * time_steps(fine, 50)
     ms_cl = 1e3 * time_steps(runs["closure"], 200)
     ms_held = 1e3 * time_steps(runs["closure_held"], 200)
+    ms_held_mixed = 1e3 * time_steps(runs["closure_held_mixed"], 200)
     ms_plain = 1e3 * time_steps(runs["plain"], 200)
     ms_parts = closure.closure_time(runs["closure"], 50)
     out = dict(fine="%dx%dx%d" % (args.fine, args.fine, args.ny), coarse="%dx%dx%d" % (args.coarse, args.coarse, args.ny),
@@ -163,10 +166,12 @@ def run(args):
                apriori_train=[float(v) for v in train_err],
                post_steps=args.post_steps, profile_errors=res,
                ms_per_step=dict(fine=ms_fine, coarse_closure=ms_cl, coarse_closure_held=ms_held,
+                                coarse_closure_held_mixed=ms_held_mixed,
                                 coarse_plain=ms_plain),
                closure_ms_per_substep=dict(planes=ms_parts[0], network=ms_parts[1], to_modes=ms_parts[2]),
                gpu_time_ratio_fine_over_closure=ms_fine / ms_cl,
                gpu_time_ratio_fine_over_closure_held=ms_fine / ms_held,
+               gpu_time_ratio_fine_over_closure_held_mixed=ms_fine / ms_held_mixed,
                seconds=dict(data=t_data, train=t_train))
     print(json.dumps(out), flush=True)
     return out
