@@ -482,14 +482,16 @@ int raw_nonlinear(lh_ctx *x, int kRK) {
   return LH_OK;
 }
 
-// per-channel closure buffers: phys and corr planes (float [3][nyu][N2][N1])
-// and training-target modes (double2 [3][nyu][ldc])
+// per-channel closure buffers: phys and corr planes (float [3][nyu][N2][N1]),
+// training-target modes and the held closure's correction modes (each
+// double2 [3][nyu][ldc])
 size_t cl_plane_floats(const HostChan &h) { return (size_t)3 * h.nyu * h.cfg.N1 * h.cfg.N2; }
-int cl_buffers(lh_ctx *x, int c, float **phys, float **corr, double2 **tau) {
+int cl_buffers(lh_ctx *x, int c, float **phys, float **corr, double2 **tau, double2 **held = nullptr) {
   HostChan &h = x->ch[c];
   const size_t pf = al<float>(cl_plane_floats(h));
+  const size_t mf = al<double2>((size_t)3 * h.nyu * h.ldc);
   if (!x->cl_mem[c]) {
-    const size_t bytes = 2 * pf + al<double2>((size_t)3 * h.nyu * h.ldc);
+    const size_t bytes = 2 * pf + 2 * mf;
     hipError_t e = hipMalloc((void **)&x->cl_mem[c], bytes);
     if (e != hipSuccess) {
       x->cl_mem[c] = nullptr;
@@ -500,6 +502,7 @@ int cl_buffers(lh_ctx *x, int c, float **phys, float **corr, double2 **tau) {
   *phys = (float *)x->cl_mem[c];
   *corr = (float *)(x->cl_mem[c] + pf);
   *tau = (double2 *)(x->cl_mem[c] + 2 * pf);
+  if (held) *held = (double2 *)(x->cl_mem[c] + 2 * pf + mf);
   return LH_OK;
 }
 
@@ -512,17 +515,19 @@ int closure_stage(lh_ctx *x, int kRK) {
     if (!x->cl_on[c]) continue;
     HostChan &h = x->ch[c];
     float *phys, *corr;
-    double2 *tau;
-    int rc = cl_buffers(x, (int)c, &phys, &corr, &tau);
+    double2 *tau, *held;
+    int rc = cl_buffers(x, (int)c, &phys, &corr, &tau, &held);
     if (rc) return rc;
     View d(x->arena, h.dev);
     ClSrc src{{d.u1, d.u2, d.u3}, {h.nyu, h.nyv, h.nyu}};
-    if (x->cl_on[c] == 1 || kRK == 1) {
-      HIPCHK(launch_cl_phys(x->arena, x->d_chans + c, h.dev, src, phys, x->stream));
-      HIPCHK(launch_cl_infer(x->cl_net, (const float *)(x->cl_net + CL_FRAG_BYTES), x->cl_norm, phys, corr,
-                             h.cfg.N1, h.cfg.N2, h.nyu, h.nyv, x->stream));
+    if (x->cl_on[c] == 2 && kRK != 1) {  // held: the modes stored at kRK = 1
+      HIPCHK(launch_cl_add_modes(x->arena, x->d_chans + c, h.dev, held, x->stream));
+      continue;
     }
-    HIPCHK(launch_cl_to_modes(x->arena, x->d_chans + c, h.dev, corr, x->stream));
+    HIPCHK(launch_cl_phys(x->arena, x->d_chans + c, h.dev, src, phys, x->stream));
+    HIPCHK(launch_cl_infer(x->cl_net, (const float *)(x->cl_net + CL_FRAG_BYTES), x->cl_norm, phys, corr, h.cfg.N1,
+                           h.cfg.N2, h.nyu, h.nyv, x->stream));
+    HIPCHK(launch_cl_to_modes(x->arena, x->d_chans + c, h.dev, corr, x->stream, x->cl_on[c] == 2 ? held : nullptr));
   }
   return LH_OK;
 }

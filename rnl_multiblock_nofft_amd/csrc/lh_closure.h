@@ -58,7 +58,10 @@ hipError_t launch_cl_phys(char *arena, const Chan *d_chan, const Chan &h, ClSrc 
 hipError_t launch_cl_infer(const void *frags, const float *bias, ClNorm nm, const float *phys, float *corr, int N1,
                            int N2, int nyu, int nyv, hipStream_t s);
 // forward transform of the correction planes, added to nl1, nl2, nl3
-hipError_t launch_cl_to_modes(char *arena, const Chan *d_chan, const Chan &h, const float *corr, hipStream_t s);
+hipError_t launch_cl_to_modes(char *arena, const Chan *d_chan, const Chan &h, const float *corr, hipStream_t s,
+                              double2 *store = nullptr);
+// add the correction modes a previous k_cl_to_modes stored to nl1, nl2, nl3
+hipError_t launch_cl_add_modes(char *arena, const Chan *d_chan, const Chan &h, const double2 *store, hipStream_t s);
 // fine -> coarse box-filtered restriction of u1, u2, u3, p (coarse state)
 hipError_t launch_cl_restrict_fields(char *fa, const Chan *d_fine, char *ca, const Chan *d_coarse, const Chan &hc,
                                      hipStream_t s);

@@ -340,7 +340,7 @@ hipError_t launch_cl_infer(const void *frags, const float *bias, ClNorm nm, cons
 // correction planes -> modes, added to the nonlinear term: one workgroup
 // per (row, field); Nu1, Nu3 on u rows 1..ny-1, Nu2 on v rows 1..ny-2; the
 // x-Nyquist (i = N1/2) and z-Nyquist (k = N2/2) modes are left alone
-__global__ void __launch_bounds__(256) k_cl_to_modes(char *arena, const Chan *chp, const float *corr) {
+__global__ void __launch_bounds__(256) k_cl_to_modes(char *arena, const Chan *chp, const float *corr, double2 *store) {
   const Chan ch = *chp;
   const int j = blockIdx.x, f = blockIdx.y;
   if (j < 1 || j > (f == 1 ? ch.nyv : ch.nyu) - 2) return;
@@ -366,18 +366,46 @@ __global__ void __launch_bounds__(256) k_cl_to_modes(char *arena, const Chan *ch
   double2 *nl = f == 0 ? v.nl1 : (f == 1 ? v.nl2 : v.nl3);
   double2 *dst = nl + (size_t)j * ch.ldc;
   const double s2 = 1.0 / (double)N2;
+  double2 *keep = store ? store + ((size_t)f * ch.nyu + j) * ch.ldc : nullptr;
   for (int c = threadIdx.x; c < ch.ncol; c += blockDim.x) {
     const int i = c % nxh, k = c / nxh;
     if (i == N1 / 2 || k == N2 / 2) continue;
     const float2 t = X[i * N2 + k];
+    const double2 a = make_double2((double)t.x * s2, (double)t.y * s2);
     const double2 o = dst[c];
-    dst[c] = make_double2(o.x + (double)t.x * s2, o.y + (double)t.y * s2);
+    dst[c] = make_double2(o.x + a.x, o.y + a.y);
+    if (keep) keep[c] = a;
   }
 }
 
-hipError_t launch_cl_to_modes(char *arena, const Chan *d_chan, const Chan &h, const float *corr, hipStream_t s) {
+// the stored correction modes added again (held closure, substeps 2 and 3):
+// the same values and the same additions as k_cl_to_modes made
+__global__ void __launch_bounds__(256) k_cl_add_modes(char *arena, const Chan *chp, const double2 *store) {
+  const Chan ch = *chp;
+  const int j = blockIdx.x, f = blockIdx.y;
+  if (j < 1 || j > (f == 1 ? ch.nyv : ch.nyu) - 2) return;
+  const View v(arena, ch);
+  const int N1 = ch.N1, N2 = ch.N2, nxh = ch.nxh;
+  double2 *nl = f == 0 ? v.nl1 : (f == 1 ? v.nl2 : v.nl3);
+  double2 *dst = nl + (size_t)j * ch.ldc;
+  const double2 *keep = store + ((size_t)f * ch.nyu + j) * ch.ldc;
+  for (int c = threadIdx.x; c < ch.ncol; c += blockDim.x) {
+    const int i = c % nxh, k = c / nxh;
+    if (i == N1 / 2 || k == N2 / 2) continue;
+    const double2 a = keep[c], o = dst[c];
+    dst[c] = make_double2(o.x + a.x, o.y + a.y);
+  }
+}
+
+hipError_t launch_cl_to_modes(char *arena, const Chan *d_chan, const Chan &h, const float *corr, hipStream_t s,
+                              double2 *store) {
   const size_t lds = (size_t)(2 * h.N1 * h.N2 + h.N1 + h.N2) * sizeof(float2);
-  hipLaunchKernelGGL(k_cl_to_modes, dim3(h.nyu, 3), dim3(256), lds, s, arena, d_chan, corr);
+  hipLaunchKernelGGL(k_cl_to_modes, dim3(h.nyu, 3), dim3(256), lds, s, arena, d_chan, corr, store);
+  return hipGetLastError();
+}
+
+hipError_t launch_cl_add_modes(char *arena, const Chan *d_chan, const Chan &h, const double2 *store, hipStream_t s) {
+  hipLaunchKernelGGL(k_cl_add_modes, dim3(h.nyu, 3), dim3(256), 0, s, arena, d_chan, store);
   return hipGetLastError();
 }
 
