@@ -232,6 +232,13 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
 /* LH_TUNE_XPROD_THREADS (default 256): threads per workgroup of the
  * x-product stage, 128 or 256. */
 #define LH_TUNE_XPROD_THREADS 10
+/* LH_TUNE_RHS_FUSED (default 1): for contexts without textured walls, the
+ * column solve of lh_step/lh_substep computes the right-hand sides itself
+ * and r1, r2, r3 are not stored (the LH_PHASE_RHS / LH_PHASE_SOLVE phases
+ * and lh_time_kernels keep the separate right-hand-side kernel); 0 always
+ * runs that kernel.  lh_autotune sets it with the solve's shape, from the
+ * timings of both forms. */
+#define LH_TUNE_RHS_FUSED 11
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

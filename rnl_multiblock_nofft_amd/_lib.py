@@ -24,6 +24,7 @@ TUNE_SOLVE_WAVES = 7
 TUNE_XPROD_INPLACE = 8
 TUNE_ZFWD_INPLACE = 9
 TUNE_XPROD_THREADS = 10
+TUNE_RHS_FUSED = 11
 PREC_FP64, PREC_MIXED = 0, 1
 
 # every symbol include/lhgpu.h declares

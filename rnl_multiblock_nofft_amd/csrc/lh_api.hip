@@ -463,10 +463,10 @@ void tex_classify(lh_ctx *x) {
 }
 
 // column solves of a substep (with the texture parts around them)
-int solve_phase(lh_ctx *x, int kRK) {
+int solve_phase(lh_ctx *x, int kRK, int fuse = 0) {
   int rc = tex_part(x, kRK, 0);
   if (rc) return rc;
-  HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
+  HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream, fuse));
   return tex_part(x, kRK, 1);
 }
 
@@ -554,8 +554,11 @@ int run_substep(lh_ctx *x, int kRK) {
   if (rc) return rc;
   rc = time_step(x, kRK);
   if (rc) return rc;
-  HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
-  return solve_phase(x, kRK);
+  // the textured-wall sources read r1 and r3 before the solve; without
+  // textured walls the solve computes the right-hand sides itself
+  const int fuse = x->shape.rhs_fused && !x->ntex;
+  if (!fuse) HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
+  return solve_phase(x, kRK, fuse);
 }
 
 // Thomas factors and the textured-wall inverse rows (after a grid change)
@@ -1303,6 +1306,13 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     drop_graph(x);
     return LH_OK;
   }
+  if (key == LH_TUNE_RHS_FUSED) {
+    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_RHS_FUSED is 0 or 1");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->shape.rhs_fused = value;
+    drop_graph(x);
+    return LH_OK;
+  }
   if (key == LH_TUNE_SOLVE_SCAN) {
     if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_SOLVE_SCAN is 0 or 1");
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1363,6 +1373,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
   }
   if (key == LH_TUNE_XPROD_THREADS) {
     *value = x->shape.xprod_threads;
+    return LH_OK;
+  }
+  if (key == LH_TUNE_RHS_FUSED) {
+    *value = x->shape.rhs_fused;
     return LH_OK;
   }
   int *k = stage_knob(x->shape, key);
@@ -1442,27 +1456,38 @@ int lh_autotune(lh_ctx *x, int reps) {
     shape_tiles(best);
   }
   // column solve: one lane per column, or C columns per workgroup in LDS
-  // with serial or lane-split recurrences
+  // with serial or lane-split recurrences; timed together with the
+  // right-hand side, which smooth-walled contexts may compute inside the LDS
+  // solve (a win only where the columns are many: at a few hundred columns
+  // the separate kernel hides the loads' latency better)
   {
     const int cand[] = {0, 1, 2, 3, 4, 5, 6, 8, 10};
     float best_ms = 1e30f;
     for (int C : cand) {
       if (!solve_cols_ok(x, C)) continue;
       for (int sc = 0; sc <= (C > 0 ? 1 : 0); sc++)
-        for (int nw = 1; nw <= (C > 0 ? 4 : 1); nw *= 2) {
-          ShapeMax m = best;
-          m.solve_cols = C;
-          m.solve_scan = sc;
-          m.solve_waves = nw;
-          float ms = time_launches(x, e0, e1, reps,
-                                   [&](int r) { return launch_solve(x->arena, x->d_chans, m, 1 + r % 3, x->stream); });
-          if (ms >= 0.f && ms < best_ms) {
-            best_ms = ms;
-            best.solve_cols = C;
-            best.solve_scan = sc;
-            best.solve_waves = nw;
+        for (int nw = 1; nw <= (C > 0 ? 4 : 1); nw *= 2)
+          for (int fu = 0; fu <= (C > 0 && !x->ntex ? 1 : 0); fu++) {
+            ShapeMax m = best;
+            m.solve_cols = C;
+            m.solve_scan = sc;
+            m.solve_waves = nw;
+            float ms = time_launches(x, e0, e1, reps, [&](int r) {
+              const int kRK = 1 + r % 3;
+              if (!fu) {
+                hipError_t e = launch_rhs(x->arena, x->d_chans, m, kRK, x->stream);
+                if (e != hipSuccess) return e;
+              }
+              return launch_solve(x->arena, x->d_chans, m, kRK, x->stream, fu);
+            });
+            if (ms >= 0.f && ms < best_ms) {
+              best_ms = ms;
+              best.solve_cols = C;
+              best.solve_scan = sc;
+              best.solve_waves = nw;
+              if (!x->ntex) best.rhs_fused = fu;
+            }
           }
-        }
     }
   }
   (void)hipEventDestroy(e0);

@@ -24,6 +24,7 @@ struct ShapeMax {
   int xprod_ip = 0;     // x-product stage: in-place single-buffer stages where they fit
   int zfwd_ip = 0;      // z-forward stage: the same
   int xprod_threads = 256;  // x-product stage: threads per workgroup (128 or 256)
+  int rhs_fused = 1;   // smooth-walled ensembles: right-hand sides computed inside the column solve
   int nphys = 0;       // N1*N2, largest over channels (texture mask kernels)
 };
 void shape_tiles(ShapeMax &m);
@@ -42,7 +43,9 @@ hipError_t launch_dt_update(char *arena, const Chan *d_chans, const ShapeMax &m,
 hipError_t launch_u11(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 // err, Qx, Umax, Uslip of every channel into out[ch*8 + 0..3] (device)
 hipError_t launch_history(char *arena, const Chan *d_chans, const ShapeMax &m, double *out, hipStream_t s);
-hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
+// fuse: compute the right-hand sides inside the solve instead of reading the
+// r1, r2, r3 that launch_rhs stores (smooth-walled channels only)
+hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse = 0);
 hipError_t launch_flowrate(char *arena, const Chan *d_chans, int nchan, int set_target, hipStream_t s);
 hipError_t launch_to_rows(const double2 *src, double2 *dst, int ncol, int nrows, int jlo, int ldc,
                           hipStream_t s);

@@ -585,29 +585,25 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
 }
 
 // ------------------------------------------------------------------------
-// K4: right-hand sides
-__global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int kRK, int gx, int gy) {
-  const Chan ch = chans[blockIdx.z];
-  const View v(arena, ch);
-  // XCD-aware order: tiles (column block, row j) go round-robin to the 8
-  // XCDs; hand each XCD a contiguous run so that rows j-1, j, j+1 of a column
-  // block are read through one L2.  gridDim.x is gx*gy rounded up to 8.
-  const int per = gridDim.x / 8;
-  const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (tile >= gx * gy) return;
-  const int j = tile / gx;
-  const int c = (tile - j * gx) * blockDim.x + threadIdx.x;
-  if (j >= ch.nyu || c >= ch.ncol) return;
+// K4: right-hand sides.  rhs_point is the right-hand side of one point (row j,
+// column c): r1, r3 on rows 0..nyu-1 and r2 on rows 0..nyv-1 (zero on the
+// wall rows), and, when `put`, the new explicit terms N1o, N3o, N2o of the
+// interior rows.  k_rhs stores r; the column solve of smooth-walled channels
+// calls it in its staging phase instead (the values go straight to LDS).
+__device__ __forceinline__ void rhs_point(const View &v, const Chan &ch, int kRK, int j, int c, bool put, double2 &r1,
+                                          double2 &r3, double2 &r2) {
   const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu, nyv = ch.nyv;
   const int i = c % nxh, k = c / nxh;
   const double dt = v.scal[SC_DT];
   const double C1 = rk_a(kRK) / ch.Re, C2 = -rk_c(kRK), Cg = -rk_g(kRK), dR = rk_d(kRK);
   const size_t o = (size_t)j * ldc + c;
   const double k2x = v.k2x[i], k2z = v.k2z[k];
+  r1 = r3 = r2 = make_double2(0, 0);
   if (j >= 1 && j <= nyu - 2) {
     const double a0 = v.dyu2i[3 * j], a1 = v.dyu2i[3 * j + 1], a2 = v.dyu2i[3 * j + 2];
     const double dd = (k2x + k2z) + a1;
     const double dyu = v.dthdyu[j];
+    const double2 p0 = v.p[o];
     {
       double2 um = v.u1[o - ldc], u0 = v.u1[o], up = v.u1[o + ldc];
       double2 L = make_double2(a0 * um.x + dd * u0.x + a2 * up.x, a0 * um.y + dd * u0.y + a2 * up.y);
@@ -615,12 +611,12 @@ __global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int
       double2 nl = v.nl1[o];
       double2 Nn = make_double2(nl.x + (f0.x - fm.x) * dyu, nl.y + (f0.y - fm.y) * dyu);
       double2 No = v.N1o[o];
-      double2 pc = cimul(-dR * v.kx[i], v.p[o]);
+      double2 pc = cimul(-dR * v.kx[i], p0);
       double2 du = make_double2(C1 * L.x + C2 * No.x + pc.x, C1 * L.y + C2 * No.y + pc.y);
       if (c == 0) du.x = du.x + (-dR * v.scal[SC_MPGX]);
       du = make_double2(du.x + Cg * Nn.x, du.y + Cg * Nn.y);
-      v.r1[o] = make_double2(u0.x + dt * du.x, u0.y + dt * du.y);
-      v.N1o[o] = Nn;
+      r1 = make_double2(u0.x + dt * du.x, u0.y + dt * du.y);
+      if (put) v.N1o[o] = Nn;
     }
     {
       double2 um = v.u3[o - ldc], u0 = v.u3[o], up = v.u3[o + ldc];
@@ -629,15 +625,12 @@ __global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int
       double2 nl = v.nl3[o];
       double2 Nn = make_double2(nl.x + (f0.x - fm.x) * dyu, nl.y + (f0.y - fm.y) * dyu);
       double2 No = v.N3o[o];
-      double2 pc = cimul(-dR * v.kz[k], v.p[o]);
+      double2 pc = cimul(-dR * v.kz[k], p0);
       double2 du = make_double2(C1 * L.x + C2 * No.x + pc.x, C1 * L.y + C2 * No.y + pc.y);
       du = make_double2(du.x + Cg * Nn.x, du.y + Cg * Nn.y);
-      v.r3[o] = make_double2(u0.x + dt * du.x, u0.y + dt * du.y);
-      v.N3o[o] = Nn;
+      r3 = make_double2(u0.x + dt * du.x, u0.y + dt * du.y);
+      if (put) v.N3o[o] = Nn;
     }
-  } else {
-    v.r1[o] = make_double2(0, 0);
-    v.r3[o] = make_double2(0, 0);
   }
   if (j >= 1 && j <= nyv - 2) {
     const double a0 = v.dyv2i[3 * j], a1 = v.dyv2i[3 * j + 1], a2 = v.dyv2i[3 * j + 2];
@@ -654,11 +647,29 @@ __global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int
     double2 du = make_double2(C1 * L.x + C2 * No.x + C3 * (pp.x - p0.x),
                               C1 * L.y + C2 * No.y + C3 * (pp.y - p0.y));
     du = make_double2(du.x + Cg * Nn.x, du.y + Cg * Nn.y);
-    v.r2[o] = make_double2(u0.x + dt * du.x, u0.y + dt * du.y);
-    v.N2o[o] = Nn;
-  } else if (j == 0 || j == nyv - 1) {
-    v.r2[o] = make_double2(0, 0);
+    r2 = make_double2(u0.x + dt * du.x, u0.y + dt * du.y);
+    if (put) v.N2o[o] = Nn;
   }
+}
+
+__global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int kRK, int gx, int gy) {
+  const Chan ch = chans[blockIdx.z];
+  const View v(arena, ch);
+  // XCD-aware order: tiles (column block, row j) go round-robin to the 8
+  // XCDs; hand each XCD a contiguous run so that rows j-1, j, j+1 of a column
+  // block are read through one L2.  gridDim.x is gx*gy rounded up to 8.
+  const int per = gridDim.x / 8;
+  const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (tile >= gx * gy) return;
+  const int j = tile / gx;
+  const int c = (tile - j * gx) * blockDim.x + threadIdx.x;
+  if (j >= ch.nyu || c >= ch.ncol) return;
+  double2 r1, r3, r2;
+  rhs_point(v, ch, kRK, j, c, true, r1, r3, r2);
+  const size_t o = (size_t)j * ch.ldc + c;
+  v.r1[o] = r1;
+  v.r3[o] = r3;
+  if (j < ch.nyv) v.r2[o] = r2;
 }
 
 // ------------------------------------------------------------------------
@@ -1134,7 +1145,7 @@ __host__ __device__ inline int lanes_per_seq(int nseq, int nt) {
   return L;
 }
 
-__global__ void __launch_bounds__(256) k_solve_lds(char *arena, const Chan *chans, int kRK, int C, int scan) {
+__global__ void __launch_bounds__(256) k_solve_lds(char *arena, const Chan *chans, int kRK, int C, int scan, int fuse) {
   const Chan ch = chans[blockIdx.z];
   const View v(arena, ch);
   const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu, nyv = ch.nyv, ncol = ch.ncol;
@@ -1178,8 +1189,28 @@ __global__ void __launch_bounds__(256) k_solve_lds(char *arena, const Chan *chan
   }
   // ---- L: stage right-hand sides, pivots, row coefficients and the
   //      columns' wavenumbers (dead columns read column ncol-1); loads are
-  //      issued LB iterations at a time so that they are all in flight
-  {
+  //      issued LB iterations at a time so that they are all in flight.
+  //      With `fuse` the right-hand sides are computed here (rhs_point)
+  //      rather than read from r1, r2, r3.
+  if (fuse) {
+    const int nU = nyu * C;
+#pragma unroll 2
+    for (int t = lane; t < nU; t += NT) {
+      const int j = t / C, cl = t - j * C, c = min(c0 + cl, ncol - 1);
+      const size_t o = (size_t)j * ldc + c;
+      const double du = fU[o], dp = fP[o], dv = j < nyv ? fV[o] : 0.0;
+      double2 a, b, e;
+      rhs_point(v, ch, kRK, j, c, c0 + cl < ncol, a, b, e);
+      X1[t] = a;
+      X3[t] = b;
+      DU[t] = du;
+      DP[t] = dp;
+      if (j < nyv) {
+        X2[t] = e;
+        DV[t] = dv;
+      }
+    }
+  } else {
     const int nU = nyu * C, nV = nyv * C;
     for (int t0 = 0; t0 < nU; t0 += NT * LB) {
       double2 a[LB], b[LB];
@@ -1831,16 +1862,20 @@ hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int k
   return hipGetLastError();
 }
 
-hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s) {
+hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse) {
   const int C = m.solve_cols;
   if (C <= 0) {
+    if (fuse) {  // the one-column-per-lane solve reads r: compute it first
+      hipError_t e = launch_rhs(arena, d_chans, m, kRK, s);
+      if (e != hipSuccess) return e;
+    }
     dim3 g((m.ncol + 63) / 64, 1, m.nchan);
     hipLaunchKernelGGL(k_solve, g, dim3(64), 0, s, arena, d_chans, kRK);
   } else {
     const int ngrp = (m.ncol + C - 1) / C;
     dim3 g((ngrp + 7) / 8 * 8, 1, m.nchan);
     hipLaunchKernelGGL(k_solve_lds, g, dim3(64 * m.solve_waves), solve_lds_bytes(m.nyu, C), s, arena, d_chans, kRK,
-                       C, m.solve_scan);
+                       C, m.solve_scan, fuse);
   }
   return hipGetLastError();
 }

@@ -197,7 +197,8 @@ class Solver:
                 "solve_waves": self.get_tuning(_lib.TUNE_SOLVE_WAVES),
                 "xprod_inplace": self.get_tuning(_lib.TUNE_XPROD_INPLACE),
                 "zfwd_inplace": self.get_tuning(_lib.TUNE_ZFWD_INPLACE),
-                "xprod_threads": self.get_tuning(_lib.TUNE_XPROD_THREADS)}
+                "xprod_threads": self.get_tuning(_lib.TUNE_XPROD_THREADS),
+                "rhs_fused": self.get_tuning(_lib.TUNE_RHS_FUSED)}
 
     def sync(self):
         check(self.L.lh_sync(self.h))

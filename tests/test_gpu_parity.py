@@ -139,6 +139,35 @@ def test_column_solve_layouts(cols, scan, waves, case):
         assert e <= TOL, (nm, e)
 
 
+@pytest.mark.parametrize("cols,scan,waves,case", [(0, 0, 1, "N32_ny153"), (3, 1, 1, "N32_ny153"),
+                                                  (2, 1, 4, "N8_ny17"), (10, 1, 4, "N16_ny33_poly_ctP")])
+def test_rhs_fused_into_solve(cols, scan, waves, case):
+    """Smooth-walled steps with the right-hand sides computed inside the
+    column solve give the oracle's steps and, to round-off, the steps of the
+    separate right-hand-side kernel (the same arithmetic per point)."""
+    cfg = CASES[case]
+    say("rhs fused", cols, scan, waves, case)
+    out = {}
+    for fused in (1, 0):
+        s, o, f = make_pair(cfg, seed=11)
+        for key, val in ((_lib.TUNE_SOLVE_COLS, cols), (_lib.TUNE_SOLVE_SCAN, scan),
+                         (_lib.TUNE_SOLVE_WAVES, waves), (_lib.TUNE_RHS_FUSED, fused)):
+            s.set_tuning(key, val)
+        assert s.get_tuning(_lib.TUNE_RHS_FUSED) == fused
+        s.step(2)
+        out[fused] = {nm: s.download(0, nm) for nm in ("u1", "u2", "u3", "p", "Nu1", "Nu2", "Nu3")}
+        if fused:
+            o.step(2)
+            for nm in ("u1", "u2", "u3", "p"):
+                e = relerr(out[1][nm], o.get(nm))
+                say("fused", nm, e)
+                assert e <= 1e-11, (nm, e)
+    for nm in out[1]:
+        e = relerr(out[1][nm], out[0][nm])
+        say("fused vs separate", nm, e)
+        assert e <= 1e-13, (nm, e)
+
+
 @pytest.mark.parametrize("key,points", [(_lib.TUNE_ZINV_POINTS, 256), (_lib.TUNE_ZINV_POINTS, 4096),
                                         (_lib.TUNE_XPROD_POINTS, 128), (_lib.TUNE_XPROD_POINTS, 2048),
                                         (_lib.TUNE_ZFWD_POINTS, 100), (_lib.TUNE_ZFWD_POINTS, 3000)])
