@@ -416,6 +416,20 @@ __global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, i
   const View v(arena, ch);
   const int N1 = ch.N1, N2 = ch.N2, nxh = ch.nxh, nc = ch.ncol, n = ch.nyu, ldc = ch.ldc;
   const int tid = threadIdx.x, nt = blockDim.x;
+  // the ghost pass's input row and the rows it adds to, read first so that
+  // these loads overlap the set-up below and are not a round trip after the
+  // transforms (this workgroup alone writes the rows it adds to)
+  const double2 *src = f == 0 ? v.u1 : v.u3;
+  double2 *dst = (mode == 0 ? (f == 0 ? v.r1 : v.r3) : (f == 0 ? v.u1 : v.u3)) + (w == 0 ? 0 : (size_t)(n - 1) * ldc);
+  constexpr int PQ = 8;
+  double2 gin[PQ], pre[PQ];
+#pragma unroll
+  for (int q = 0; q < PQ; q++) {
+    const int k = tid + q * nt;
+    const bool ld = mode == 1 && k < N2;
+    gin[q] = ld ? src[(size_t)(w == 0 ? 1 : n - 2) * ldc + i + nxh * k] : make_double2(0, 0);
+    pre[q] = ld ? dst[i + nxh * k] : make_double2(0, 0);
+  }
   extern __shared__ double2 smem_d[];
   double2 *A = smem_d, *B = A + N2, *TZ = B + N2;
   double *M = reinterpret_cast<double *>(TZ + N2);
@@ -431,7 +445,6 @@ __global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, i
     for (int z = tid; z < N2; z += nt) M[z] = mk[(size_t)z * N1] == 2 ? cfs : 0.0;
   }
   const bool realpart = i == 0 || i == N1 / 2;
-  const double2 *src = f == 0 ? v.u1 : v.u3;
   const double invz = 1.0 / N2;
   // the no-slip values U and table offsets g - gbar of this x-mode's columns,
   // read once for every pass (each thread keeps its own k)
@@ -445,20 +458,20 @@ __global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, i
     }
   double2 *S = A;
   const int passes = mode == 0 ? nit : 1;
+  if (mode == 1) {  // the ghost pass transforms row 1 (or n-2) once
+#pragma unroll
+    for (int q = 0; q < PQ; q++)
+      if (tid + q * nt < N2) A[tid + q * nt] = gin[q];
+    for (int k = tid + PQ * nt; k < N2; k += nt) A[k] = src[(size_t)(w == 0 ? 1 : n - 2) * ldc + i + nxh * k];
+  }
   for (int it = 0; it < passes; it++) {
-    for (int k = tid; k < N2; k += nt) {
-      const int c = i + nxh * k;
-      double2 val;
-      if (mode == 0) {
+    if (mode == 0)
+      for (int k = tid; k < N2; k += nt) {
         const double2 U = Us[k];
         const double dg = Ds[k];
         const double2 sg = it > 0 ? S[k] : make_double2(0, 0);
-        val = make_double2(U.x + dg * sg.x, U.y + dg * sg.y);
-      } else {
-        val = src[(size_t)(w == 0 ? 1 : n - 2) * ldc + c];
+        S[k] = make_double2(U.x + dg * sg.x, U.y + dg * sg.y);
       }
-      S[k] = val;
-    }
     __syncthreads();
     double2 *R = fft_lds<double2>(S, S == A ? B : A, N2, 1, N2, +1, TZ);
     for (int z = tid; z < N2; z += nt) {
@@ -471,8 +484,12 @@ __global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, i
     __syncthreads();
     S = G;
   }
-  double2 *dst = (mode == 0 ? (f == 0 ? v.r1 : v.r3) : (f == 0 ? v.u1 : v.u3)) + (w == 0 ? 0 : (size_t)(n - 1) * ldc);
-  for (int k = tid; k < N2; k += nt) {
+#pragma unroll
+  for (int q = 0; q < PQ; q++) {
+    const int k = tid + q * nt;
+    if (k < N2) dst[i + nxh * k] = mode == 0 ? S[k] : cadd(pre[q], S[k]);
+  }
+  for (int k = tid + PQ * nt; k < N2; k += nt) {
     const int c = i + nxh * k;
     dst[c] = mode == 0 ? S[k] : cadd(dst[c], S[k]);
   }
