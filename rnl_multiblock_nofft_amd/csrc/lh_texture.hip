@@ -442,10 +442,11 @@ __global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, i
   const double Pw = 1.0 + gw, cfs = mode == 0 ? Pw / (1.0 - g0 * Pw) : 1.0 - (f == 0 ? ch.gwbc1 : ch.gwbc3);
   {
     const int *mk = v.mask + (size_t)w * N1 * N2;
-    for (int z = tid; z < N2; z += nt) M[z] = mk[(size_t)z * N1] == 2 ? cfs : 0.0;
+    // the mask carries the forward transform's 1/N2
+    const double cfz = cfs * (1.0 / N2);
+    for (int z = tid; z < N2; z += nt) M[z] = mk[(size_t)z * N1] == 2 ? cfz : 0.0;
   }
   const bool realpart = i == 0 || i == N1 / 2;
-  const double invz = 1.0 / N2;
   // the no-slip values U and table offsets g - gbar of this x-mode's columns,
   // read once for every pass (each thread keeps its own k)
   double2 *Us = reinterpret_cast<double2 *>(M + N2);
@@ -479,10 +480,7 @@ __global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, i
       R[z] = make_double2(M[z] * h.x, realpart ? 0.0 : M[z] * h.y);
     }
     __syncthreads();
-    double2 *G = fft_lds<double2>(R, R == A ? B : A, N2, 1, N2, -1, TZ);
-    for (int k = tid; k < N2; k += nt) G[k] = make_double2(G[k].x * invz, G[k].y * invz);
-    __syncthreads();
-    S = G;
+    S = fft_lds<double2>(R, R == A ? B : A, N2, 1, N2, -1, TZ);  // ends with a barrier
   }
 #pragma unroll
   for (int q = 0; q < PQ; q++) {
