@@ -239,6 +239,12 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
  * runs that kernel.  lh_autotune sets it with the solve's shape, from the
  * timings of both forms. */
 #define LH_TUNE_RHS_FUSED 11
+/* LH_TUNE_GHOST_IN_ZINV (default 1): in lh_step's captured step, the
+ * ghost-row pass of x-independent textured walls after substeps 1 and 2
+ * runs as extra workgroups of the next substep's z-inverse transform (one
+ * launch less per substep; the step's results are the same).  Channels with
+ * N2 of 32 or 64 only; 0 keeps the separate launch. */
+#define LH_TUNE_GHOST_IN_ZINV 12
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

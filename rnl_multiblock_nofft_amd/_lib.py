@@ -25,6 +25,7 @@ TUNE_XPROD_INPLACE = 8
 TUNE_ZFWD_INPLACE = 9
 TUNE_XPROD_THREADS = 10
 TUNE_RHS_FUSED = 11
+TUNE_GHOST_IN_ZINV = 12
 PREC_FP64, PREC_MIXED = 0, 1
 
 # every symbol include/lhgpu.h declares

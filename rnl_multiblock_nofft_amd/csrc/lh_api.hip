@@ -16,6 +16,7 @@
 #include "lh_closure.h"
 #include "lh_common.h"
 #include "lh_kernels.h"
+#include "lh_tex1d.h"
 
 using namespace lh;
 
@@ -425,7 +426,7 @@ int check_ctx(lh_ctx *x) {
 // textured walls: the wall sources before the column solves (part 0) and
 // the masked ghost rows after them (part 1); each field's wall planes stay in
 // one workgroup's LDS when they fit
-int tex_part(lh_ctx *x, int kRK, int part) {
+int tex_part(lh_ctx *x, int kRK, int part, bool skip1d = false) {
   if (!x->ntex) return LH_OK;
   const size_t lds = tex_fused_bytes(x->tex_groups);
   const bool fused = x->tex_fused && lds <= 160 * 1024;
@@ -446,7 +447,7 @@ int tex_part(lh_ctx *x, int kRK, int part) {
       else
         HIPCHK(launch_tex_ghosts(x->arena, x->d_chans, x->shape, x->tex_groups, x->stream));
     }
-    if (x->ntex1d) HIPCHK(launch_tex_1d(x->arena, x->d_chans, x->shape, 1, 1, 1, x->stream));
+    if (x->ntex1d && !skip1d) HIPCHK(launch_tex_1d(x->arena, x->d_chans, x->shape, 1, 1, 1, x->stream));
   }
   return LH_OK;
 }
@@ -463,11 +464,21 @@ void tex_classify(lh_ctx *x) {
 }
 
 // column solves of a substep (with the texture parts around them)
-int solve_phase(lh_ctx *x, int kRK, int fuse = 0) {
+int solve_phase(lh_ctx *x, int kRK, int fuse = 0, bool defer_ghost = false) {
   int rc = tex_part(x, kRK, 0);
   if (rc) return rc;
   HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream, fuse));
-  return tex_part(x, kRK, 1);
+  return tex_part(x, kRK, 1, defer_ghost);
+}
+
+// whether a captured step may leave the per-x-mode ghost pass of substeps 1
+// and 2 to extra workgroups of the next substep's z-inverse launch (the
+// nonlinear stage reads no ghost rows; the right-hand side, after it, does)
+bool ghost_deferrable(const lh_ctx *x) {
+  if (!x->shape.ghost_zinv || !x->ntex1d || x->ntri >= (int)x->ch.size()) return false;
+  for (const auto &h : x->ch)
+    if (h.dev.tex1d && !tex_ghost_small(h.cfg.N2)) return false;
+  return true;
 }
 
 // first substep: dtc and (adaptive channels) the new dt, then everything
@@ -475,9 +486,9 @@ int solve_phase(lh_ctx *x, int kRK, int fuse = 0) {
 // inverse rows -- for the whole step
 // the nonlinear products of every channel: the FFT stages for channels
 // without triad lists, the list kernels for those with
-int raw_nonlinear(lh_ctx *x, int kRK) {
+int raw_nonlinear(lh_ctx *x, int kRK, int ghost = 0) {
   if (x->ntri < (int)x->ch.size())
-    HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream, kRK));
+    HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream, kRK, ghost));
   if (x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
   return LH_OK;
 }
@@ -532,8 +543,8 @@ int closure_stage(lh_ctx *x, int kRK) {
   return LH_OK;
 }
 
-int nonlinear_stage(lh_ctx *x, int kRK) {
-  int rc = raw_nonlinear(x, kRK);
+int nonlinear_stage(lh_ctx *x, int kRK, int ghost = 0) {
+  int rc = raw_nonlinear(x, kRK, ghost);
   if (rc || !x->ncl) return rc;
   return closure_stage(x, kRK);
 }
@@ -549,8 +560,11 @@ int time_step(lh_ctx *x, int kRK) {
   return LH_OK;
 }
 
-int run_substep(lh_ctx *x, int kRK) {
-  int rc = nonlinear_stage(x, kRK);
+// in_step: one of the three substeps of a captured step, which may hand
+// substeps 1 and 2's ghost pass to the next substep (ghost_deferrable)
+int run_substep(lh_ctx *x, int kRK, bool in_step = false) {
+  const bool defer = in_step && ghost_deferrable(x);
+  int rc = nonlinear_stage(x, kRK, defer && kRK > 1);
   if (rc) return rc;
   rc = time_step(x, kRK);
   if (rc) return rc;
@@ -558,7 +572,7 @@ int run_substep(lh_ctx *x, int kRK) {
   // textured walls the solve computes the right-hand sides itself
   const int fuse = x->shape.rhs_fused && !x->ntex;
   if (!fuse) HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
-  return solve_phase(x, kRK, fuse);
+  return solve_phase(x, kRK, fuse, defer && kRK < 3);
 }
 
 // Thomas factors and the textured-wall inverse rows (after a grid change)
@@ -888,7 +902,7 @@ int lh_step(lh_ctx *x, int nsteps) {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
     int r = LH_OK;
-    for (int k = 1; k <= 3 && r == LH_OK; k++) r = run_substep(x, k);
+    for (int k = 1; k <= 3 && r == LH_OK; k++) r = run_substep(x, k, true);
     hipError_t e = hipStreamEndCapture(x->stream, &g);
     if (r) return r;
     if (e != hipSuccess) return set_err(LH_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -1313,6 +1327,13 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     drop_graph(x);
     return LH_OK;
   }
+  if (key == LH_TUNE_GHOST_IN_ZINV) {
+    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_GHOST_IN_ZINV is 0 or 1");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->shape.ghost_zinv = value;
+    drop_graph(x);
+    return LH_OK;
+  }
   if (key == LH_TUNE_SOLVE_SCAN) {
     if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_SOLVE_SCAN is 0 or 1");
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1377,6 +1398,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
   }
   if (key == LH_TUNE_RHS_FUSED) {
     *value = x->shape.rhs_fused;
+    return LH_OK;
+  }
+  if (key == LH_TUNE_GHOST_IN_ZINV) {
+    *value = x->shape.ghost_zinv;
     return LH_OK;
   }
   int *k = stage_knob(x->shape, key);

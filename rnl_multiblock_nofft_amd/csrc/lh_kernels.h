@@ -25,6 +25,7 @@ struct ShapeMax {
   int zfwd_ip = 0;      // z-forward stage: the same
   int xprod_threads = 256;  // x-product stage: threads per workgroup (128 or 256)
   int rhs_fused = 1;   // smooth-walled ensembles: right-hand sides computed inside the column solve
+  int ghost_zinv = 1;  // in a captured step, ghost rows of x-independent textures done by k_zinv's extra workgroups
   int nphys = 0;       // N1*N2, largest over channels (texture mask kernels)
 };
 void shape_tiles(ShapeMax &m);
@@ -32,11 +33,14 @@ int solve_cols_default(int nyu);
 __host__ __device__ size_t solve_lds_bytes(int nyu, int C);
 
 void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny);
-// kRK = 1 also records the CFL maxima of dtc_calc in the x-product stage
+// kRK = 1 also records the CFL maxima of dtc_calc in the x-product stage;
+// ghost: the z-inverse stage also runs the ghost-row pass of channels with
+// x-independent textured walls (the one launch_tex_1d(..., 1, 1, 1, ...)
+// makes; z lengths tex_ghost_small only)
 hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s,
-                           int kRK = 1);
+                           int kRK = 1, int ghost = 0);
 hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s,
-                            int kRK = 1);
+                            int kRK = 1, int ghost = 0);
 hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
 hipError_t launch_factors(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 hipError_t launch_dt_update(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);

@@ -27,6 +27,7 @@
 #include "lh_common.h"
 #include "lh_fft.h"
 #include "lh_kernels.h"
+#include "lh_tex1d.h"
 
 namespace lh {
 
@@ -63,8 +64,8 @@ template <typename C> __device__ __forceinline__ double2 to_d(C v) { return make
 // multiple of 8 and gives each XCD a contiguous run of (tile, slot) pairs,
 // so neighbouring x-mode tiles of a plane, which share 128-byte lines of the
 // spectral rows, meet in one L2.  Returns false for the padding.
-__device__ __forceinline__ bool xcd_tile(int ntile, int nslot, int &tile, int &slot) {
-  const int per = gridDim.x / 8;
+__device__ __forceinline__ bool xcd_tile(int ntile, int nslot, int &tile, int &slot, int nblk = -1) {
+  const int per = (nblk < 0 ? (int)gridDim.x : nblk) / 8;
   const int L = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (L >= ntile * nslot) return false;
   tile = L % ntile;
@@ -95,13 +96,24 @@ __device__ __forceinline__ void zinv_tail(C *A, C *B, const C *TW, int S, int ni
 // ------------------------------------------------------------------------
 // K1: interpolation + inverse z transform
 template <typename C>
-__global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E, int ntile, int nslot) {
+__global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E, int ntile, int nslot, int nzinv) {
+  if ((int)blockIdx.x >= nzinv) {
+    // workgroups past the transform grid: the previous substep's ghost rows
+    // of textured walls with x-independent masks (nothing here reads rows 0
+    // and nyu-1, which they write), one per (x-mode, wall) of u1 and u3
+    const int f = blockIdx.z % 3, bx = blockIdx.x - nzinv, i = bx >> 1, w = bx & 1;
+    if (f == 2) return;
+    const Chan ch = chans[blockIdx.z / 3];
+    if (!ch.tex1d || !(ch.tex & (1 << w)) || i >= ch.nxh) return;
+    tex_1d_block<2, true>(arena, ch, i, f, w, 1, 1, 1);
+    return;
+  }
   const Chan ch = chans[blockIdx.z / 3];
   if (ch.tri) return;  // triad-list channel: k_triad computes its products
   const View v(arena, ch);
   const int f = blockIdx.z % 3;
   int tile, slot;
-  if (!xcd_tile(ntile, nslot, tile, slot)) return;
+  if (!xcd_tile(ntile, nslot, tile, slot, nzinv)) return;
   if (slot >= ch.nslot) return;
   // logical order u0 v0 u1 v1 ... (then the last u-plane): u-plane j and
   // v-planes j-1, j interpolate from the same spectral rows
@@ -1791,15 +1803,23 @@ void shape_tiles(ShapeMax &m) {
 }
 
 hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s,
-                           int kRK) {
+                           int kRK, int ghost) {
   size_t es = mixed ? sizeof(float2) : sizeof(double2);
   dim3 b(256);
   if (stage == 0) {
-    dim3 g(xcd_grid(m.zinv_tiles, m.nslot), 1, m.nchan * 3);
+    const int nz = (int)xcd_grid(m.zinv_tiles, m.nslot);
+    int extra = 0, n2 = 0;
+    if (ghost)
+      for (const ChanDims &d : m.dims) {
+        extra = std::max(extra, 2 * (d.N1 / 2 + 1));
+        n2 = std::max(n2, d.N2);
+      }
+    dim3 g(nz + extra, 1, m.nchan * 3);
+    const size_t lds = std::max(m.lds_zinv * es, ghost ? tex_1d_lds(n2) : (size_t)0);
     if (mixed)
-      hipLaunchKernelGGL(k_zinv<float2>, g, b, m.lds_zinv * es, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot);
+      hipLaunchKernelGGL(k_zinv<float2>, g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz);
     else
-      hipLaunchKernelGGL(k_zinv<double2>, g, b, m.lds_zinv * es, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot);
+      hipLaunchKernelGGL(k_zinv<double2>, g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz);
   } else if (stage == 1) {
     dim3 g(m.xprod_tiles, m.nslot, m.nchan);
     const bool cfl = kRK == 1;
@@ -1834,9 +1854,10 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
   return hipGetLastError();
 }
 
-hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s, int kRK) {
+hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s, int kRK,
+                            int ghost) {
   for (int st = 0; st < 3; st++) {
-    hipError_t e = launch_nl_stage(arena, d_chans, m, mixed, st, s, kRK);
+    hipError_t e = launch_nl_stage(arena, d_chans, m, mixed, st, s, kRK, st == 0 ? ghost : 0);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -30,6 +30,7 @@
 #include "lh_common.h"
 #include "lh_fft.h"
 #include "lh_kernels.h"
+#include "lh_tex1d.h"
 
 namespace lh {
 
@@ -413,87 +414,9 @@ __global__ void __launch_bounds__(64) k_tex_1d(char *arena, const Chan *chans, i
   const Chan ch = chans[blockIdx.z];
   const int i = blockIdx.x, f = blockIdx.y >> 1, w = blockIdx.y & 1;
   if (!ch.tex1d || !(ch.tex & (1 << w)) || i >= ch.nxh) return;
-  const View v(arena, ch);
-  const int N1 = ch.N1, N2 = ch.N2, nxh = ch.nxh, nc = ch.ncol, n = ch.nyu, ldc = ch.ldc;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  // the ghost pass's input row and the rows it adds to, read first so that
-  // these loads overlap the set-up below and are not a round trip after the
-  // transforms (this workgroup alone writes the rows it adds to)
-  const double2 *src = f == 0 ? v.u1 : v.u3;
-  double2 *dst = (mode == 0 ? (f == 0 ? v.r1 : v.r3) : (f == 0 ? v.u1 : v.u3)) + (w == 0 ? 0 : (size_t)(n - 1) * ldc);
-  constexpr int PQ = 8;
-  double2 gin[PQ], pre[PQ];
-#pragma unroll
-  for (int q = 0; q < PQ; q++) {
-    const int k = tid + q * nt;
-    const bool ld = mode == 1 && k < N2;
-    gin[q] = ld ? src[(size_t)(w == 0 ? 1 : n - 2) * ldc + i + nxh * k] : make_double2(0, 0);
-    pre[q] = ld ? dst[i + nxh * k] : make_double2(0, 0);
-  }
-  extern __shared__ double2 smem_d[];
-  double2 *A = smem_d, *B = A + N2, *TZ = B + N2;
-  double *M = reinterpret_cast<double *>(TZ + N2);
-  stage_twiddles<double2>(TZ, reinterpret_cast<const double2 *>(arena + ch.o_twzN), N2);
-  const size_t tab = (size_t)n * ldc;
-  const double *Wr = v.wtab + (size_t)((kRK - 1) * 2 + w) * tab;
-  const int gj = w == 0 ? 0 : n - 1;
-  const double g0 = Wr[(size_t)gj * ldc];
-  const double gw = w == 0 ? ch.gw1 : ch.gw2;
-  const double Pw = 1.0 + gw, cfs = mode == 0 ? Pw / (1.0 - g0 * Pw) : 1.0 - (f == 0 ? ch.gwbc1 : ch.gwbc3);
-  {
-    const int *mk = v.mask + (size_t)w * N1 * N2;
-    // the mask carries the forward transform's 1/N2
-    const double cfz = cfs * (1.0 / N2);
-    for (int z = tid; z < N2; z += nt) M[z] = mk[(size_t)z * N1] == 2 ? cfz : 0.0;
-  }
-  const bool realpart = i == 0 || i == N1 / 2;
-  // the no-slip values U and table offsets g - gbar of this x-mode's columns,
-  // read once for every pass (each thread keeps its own k)
-  double2 *Us = reinterpret_cast<double2 *>(M + N2);
-  double *Ds = reinterpret_cast<double *>(Us + N2);
-  if (mode == 0)
-    for (int k = tid; k < N2; k += nt) {
-      const int c = i + nxh * k;
-      Us[k] = v.texu[(size_t)(f * 2 + w) * nc + c];
-      Ds[k] = Wr[(size_t)gj * ldc + c] - g0;
-    }
-  double2 *S = A;
-  const int passes = mode == 0 ? nit : 1;
-  if (mode == 1) {  // the ghost pass transforms row 1 (or n-2) once
-#pragma unroll
-    for (int q = 0; q < PQ; q++)
-      if (tid + q * nt < N2) A[tid + q * nt] = gin[q];
-    for (int k = tid + PQ * nt; k < N2; k += nt) A[k] = src[(size_t)(w == 0 ? 1 : n - 2) * ldc + i + nxh * k];
-  }
-  for (int it = 0; it < passes; it++) {
-    if (mode == 0)
-      for (int k = tid; k < N2; k += nt) {
-        const double2 U = Us[k];
-        const double dg = Ds[k];
-        const double2 sg = it > 0 ? S[k] : make_double2(0, 0);
-        S[k] = make_double2(U.x + dg * sg.x, U.y + dg * sg.y);
-      }
-    __syncthreads();
-    double2 *R = fft_lds<double2>(S, S == A ? B : A, N2, 1, N2, +1, TZ);
-    for (int z = tid; z < N2; z += nt) {
-      const double2 h = R[z];
-      R[z] = make_double2(M[z] * h.x, realpart ? 0.0 : M[z] * h.y);
-    }
-    __syncthreads();
-    S = fft_lds<double2>(R, R == A ? B : A, N2, 1, N2, -1, TZ);  // ends with a barrier
-  }
-#pragma unroll
-  for (int q = 0; q < PQ; q++) {
-    const int k = tid + q * nt;
-    if (k < N2) dst[i + nxh * k] = mode == 0 ? S[k] : cadd(pre[q], S[k]);
-  }
-  for (int k = tid + PQ * nt; k < N2; k += nt) {
-    const int c = i + nxh * k;
-    dst[c] = mode == 0 ? S[k] : cadd(dst[c], S[k]);
-  }
+  tex_1d_block<8, false>(arena, ch, i, f, w, kRK, nit, mode);
 }
 
-static size_t tex_1d_lds(int n2max) { return (size_t)4 * n2max * sizeof(double2) + (size_t)2 * n2max * sizeof(double); }
 
 static int shape_n2max(const ShapeMax &m) {
   int n2 = 0;

@@ -198,7 +198,8 @@ class Solver:
                 "xprod_inplace": self.get_tuning(_lib.TUNE_XPROD_INPLACE),
                 "zfwd_inplace": self.get_tuning(_lib.TUNE_ZFWD_INPLACE),
                 "xprod_threads": self.get_tuning(_lib.TUNE_XPROD_THREADS),
-                "rhs_fused": self.get_tuning(_lib.TUNE_RHS_FUSED)}
+                "rhs_fused": self.get_tuning(_lib.TUNE_RHS_FUSED),
+                "ghost_in_zinv": self.get_tuning(_lib.TUNE_GHOST_IN_ZINV)}
 
     def sync(self):
         check(self.L.lh_sync(self.h))

@@ -329,6 +329,32 @@ def test_ridges_wall_paths(sep):
         assert e <= 1e-11, (nm, e)
 
 
+@pytest.mark.parametrize("n", [32, 64])
+def test_ghost_pass_in_zinv(n):
+    """Captured steps that leave the ridges' ghost-row pass of substeps 1
+    and 2 to extra workgroups of the next z-inverse launch give the oracle's
+    steps and the separate launch's steps (the same arithmetic)."""
+    cfg = small_cfg(n, 33)
+    m = textures.stripes(n, n, 4, 0.5)
+    out = {}
+    for g in (1, 0):
+        s, o = tex_pair(cfg, (m, m), seed=19)
+        s.set_tuning(_lib.TUNE_GHOST_IN_ZINV, g)
+        assert s.get_tuning(_lib.TUNE_GHOST_IN_ZINV) == g
+        s.step(3)
+        out[g] = {nm: s.download(0, nm) for nm in ("u1", "u2", "u3", "p")}
+        if g:
+            o.step(3)
+            for nm in out[g]:
+                e = relerr(out[g][nm], o.get(nm))
+                say("ghost in zinv", n, nm, e)
+                assert e <= 1e-11, (nm, e)
+    for nm in out[1]:
+        e = relerr(out[1][nm], out[0][nm])
+        say("ghost in zinv vs separate", n, nm, e)
+        assert e <= 1e-14, (nm, e)
+
+
 def test_texture_all_no_slip_is_smooth():
     cfg = CASES["N16_ny33_poly_ctP"]
     s, o, f = make_pair(cfg, seed=12)
