@@ -486,9 +486,9 @@ bool ghost_deferrable(const lh_ctx *x) {
 // inverse rows -- for the whole step
 // the nonlinear products of every channel: the FFT stages for channels
 // without triad lists, the list kernels for those with
-int raw_nonlinear(lh_ctx *x, int kRK, int ghost = 0) {
+int raw_nonlinear(lh_ctx *x, int kRK, int ghost = 0, int dtblk = 0) {
   if (x->ntri < (int)x->ch.size())
-    HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream, kRK, ghost));
+    HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream, kRK, ghost, dtblk));
   if (x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
   return LH_OK;
 }
@@ -543,15 +543,16 @@ int closure_stage(lh_ctx *x, int kRK) {
   return LH_OK;
 }
 
-int nonlinear_stage(lh_ctx *x, int kRK, int ghost = 0) {
-  int rc = raw_nonlinear(x, kRK, ghost);
+int nonlinear_stage(lh_ctx *x, int kRK, int ghost = 0, int dtblk = 0) {
+  int rc = raw_nonlinear(x, kRK, ghost, dtblk);
   if (rc || !x->ncl) return rc;
   return closure_stage(x, kRK);
 }
 
-int time_step(lh_ctx *x, int kRK) {
+// dt_done: the z-forward launch has made the time-step update already
+int time_step(lh_ctx *x, int kRK, bool dt_done = false) {
   if (kRK != 1) return LH_OK;
-  HIPCHK(launch_dt_update(x->arena, x->d_chans, x->shape, x->stream));
+  if (!dt_done) HIPCHK(launch_dt_update(x->arena, x->d_chans, x->shape, x->stream));
   if (x->adaptive) {
     HIPCHK(launch_factors(x->arena, x->d_chans, x->shape, x->stream));
     HIPCHK(launch_u11(x->arena, x->d_chans, x->shape, x->stream));
@@ -564,9 +565,13 @@ int time_step(lh_ctx *x, int kRK) {
 // substeps 1 and 2's ghost pass to the next substep (ghost_deferrable)
 int run_substep(lh_ctx *x, int kRK, bool in_step = false) {
   const bool defer = in_step && ghost_deferrable(x);
-  int rc = nonlinear_stage(x, kRK, defer && kRK > 1);
+  // the time-step update rides on the z-forward launch when there is one
+  // (it reads the x-product stage's CFL maxima; the closure and triad-list
+  // kernels between the two read neither dt nor t)
+  const bool dt_in_zfwd = kRK == 1 && x->ntri < (int)x->ch.size();
+  int rc = nonlinear_stage(x, kRK, defer && kRK > 1, dt_in_zfwd);
   if (rc) return rc;
-  rc = time_step(x, kRK);
+  rc = time_step(x, kRK, dt_in_zfwd);
   if (rc) return rc;
   // the textured-wall sources read r1 and r3 before the solve; without
   // textured walls the solve computes the right-hand sides itself

@@ -36,11 +36,12 @@ void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny);
 // kRK = 1 also records the CFL maxima of dtc_calc in the x-product stage;
 // ghost: the z-inverse stage also runs the ghost-row pass of channels with
 // x-independent textured walls (the one launch_tex_1d(..., 1, 1, 1, ...)
-// makes; z lengths tex_ghost_small only)
+// makes; z lengths tex_ghost_small only); dtblk: the z-forward stage also
+// runs the time-step update that launch_dt_update makes
 hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s,
-                           int kRK = 1, int ghost = 0);
+                           int kRK = 1, int ghost = 0, int dtblk = 0);
 hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s,
-                            int kRK = 1, int ghost = 0);
+                            int kRK = 1, int ghost = 0, int dtblk = 0);
 hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
 hipError_t launch_factors(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 hipError_t launch_dt_update(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);

@@ -64,9 +64,10 @@ template <typename C> __device__ __forceinline__ double2 to_d(C v) { return make
 // multiple of 8 and gives each XCD a contiguous run of (tile, slot) pairs,
 // so neighbouring x-mode tiles of a plane, which share 128-byte lines of the
 // spectral rows, meet in one L2.  Returns false for the padding.
-__device__ __forceinline__ bool xcd_tile(int ntile, int nslot, int &tile, int &slot, int nblk = -1) {
+__device__ __forceinline__ bool xcd_tile(int ntile, int nslot, int &tile, int &slot, int nblk = -1, int off = 0) {
   const int per = (nblk < 0 ? (int)gridDim.x : nblk) / 8;
-  const int L = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int b = (int)blockIdx.x - off;  // blocks b with one residue mod 8 share an XCD
+  const int L = (b % 8) * per + b / 8;
   if (L >= ntile * nslot) return false;
   tile = L % ntile;
   slot = L / ntile;
@@ -96,12 +97,14 @@ __device__ __forceinline__ void zinv_tail(C *A, C *B, const C *TW, int S, int ni
 // ------------------------------------------------------------------------
 // K1: interpolation + inverse z transform
 template <typename C>
-__global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E, int ntile, int nslot, int nzinv) {
-  if ((int)blockIdx.x >= nzinv) {
-    // workgroups past the transform grid: the previous substep's ghost rows
-    // of textured walls with x-independent masks (nothing here reads rows 0
-    // and nyu-1, which they write), one per (x-mode, wall) of u1 and u3
-    const int f = blockIdx.z % 3, bx = blockIdx.x - nzinv, i = bx >> 1, w = bx & 1;
+__global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E, int ntile, int nslot, int nzinv,
+                                              int nghost) {
+  if ((int)blockIdx.x < nghost) {
+    // workgroups before the transform grid (dispatched first, so that they
+    // overlap it): the previous substep's ghost rows of textured walls with
+    // x-independent masks (nothing here reads rows 0 and nyu-1, which they
+    // write), one per (x-mode, wall) of u1 and u3
+    const int f = blockIdx.z % 3, bx = blockIdx.x, i = bx >> 1, w = bx & 1;
     if (f == 2) return;
     const Chan ch = chans[blockIdx.z / 3];
     if (!ch.tex1d || !(ch.tex & (1 << w)) || i >= ch.nxh) return;
@@ -113,7 +116,7 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
   const View v(arena, ch);
   const int f = blockIdx.z % 3;
   int tile, slot;
-  if (!xcd_tile(ntile, nslot, tile, slot, nzinv)) return;
+  if (!xcd_tile(ntile, nslot, tile, slot, nzinv, nghost)) return;
   if (slot >= ch.nslot) return;
   // logical order u0 v0 u1 v1 ... (then the last u-plane): u-plane j and
   // v-planes j-1, j interpolate from the same spectral rows
@@ -514,13 +517,78 @@ __host__ __device__ inline bool zfwd_inplace(int N, int ti) {
 
 // ------------------------------------------------------------------------
 // K3: forward z transform, truncation, d/dx + d/dz
+// the step's time-step update of one channel (every thread of the
+// workgroup calls it; red: 48 doubles of LDS)
+__device__ __forceinline__ void dt_update_block(char *arena, const Chan &ch, int ntiles_max, int e_xprod, double *red) {
+  View v(arena, ch);
+  if (ch.tri) {  // no physical planes on the list path: dt stays fixed
+    if (threadIdx.x == 0) v.scal[SC_T] = v.scal[SC_T] + v.scal[SC_DT];
+    return;
+  }
+  double m1 = 0.0, m2 = 1e-6, m3 = 0.0;
+  // k_xprod's tiles of this channel (stride: the launch's tile count)
+  const int TZ = tile_of(e_xprod, ch.Ng1, ch.Ng2), nt = (ch.Ng2 + TZ - 1) / TZ;
+  const double2 *cm = reinterpret_cast<const double2 *>(v.cfl);
+  const int nu = ch.nplu * nt, nall = ch.nslot * nt;
+#pragma unroll 4
+  for (int q = threadIdx.x; q < nu; q += blockDim.x) {
+    const int slot = q / nt, tz = q - slot * nt;
+    const double2 c = cm[(size_t)slot * ntiles_max + tz];
+    m1 = fmax(m1, c.x);
+    m3 = fmax(m3, c.y);
+  }
+#pragma unroll 4
+  for (int q = nu + threadIdx.x; q < nall; q += blockDim.x) {
+    const int slot = q / nt, tz = q - slot * nt;
+    const int j = slot - ch.nplu + 1;
+    m2 = fmax(m2, cm[(size_t)slot * ntiles_max + tz].x * v.dthdyv[j]);
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    m1 = fmax(m1, __shfl_xor(m1, off));
+    m2 = fmax(m2, __shfl_xor(m2, off));
+    m3 = fmax(m3, __shfl_xor(m3, off));
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wv] = m1;
+    red[16 + wv] = m2;
+    red[32 + wv] = m3;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int q = 1; q < nw; q++) {
+      m1 = fmax(m1, red[q]);
+      m2 = fmax(m2, red[16 + q]);
+      m3 = fmax(m3, red[32 + q]);
+    }
+    m1 = fmax(m1, red[0]);
+    m2 = fmax(m2, red[16]);
+    m3 = fmax(m3, red[32]);
+    const double dtc = fmin(fmin(1.0 / (ch.alp * (ch.N1 / 2) * m1), 1.0 / (ch.bet * (ch.N2 / 2) * m3)), 1.0 / (m2 * 1.0));
+    v.scal[SC_DTC] = dtc;
+    if (ch.adaptive) v.scal[SC_DT] = fmin(2.5 * v.scal[SC_DTV], ch.CFL * dtc);
+    v.scal[SC_T] = v.scal[SC_T] + v.scal[SC_DT];
+  }
+}
+
 template <typename C, bool IP>
-__global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, int E, int ntile, int nslot) {
+__global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, int E, int ntile, int nslot, int nzfwd,
+                                              int ndt, int ntiles_xprod, int e_xprod) {
+  extern __shared__ double2 smem_d[];
+  if ((int)blockIdx.x < ndt) {
+    // with ndt = 1 the first workgroup per channel (dispatched first, so that
+    // it overlaps the transforms) makes the step's time-step update (kRK = 1)
+    // from the CFL maxima of the x-product stage; nothing in this stage reads
+    // dt or t
+    dt_update_block(arena, chans[blockIdx.z], ntiles_xprod, e_xprod, reinterpret_cast<double *>(smem_d));
+    return;
+  }
   const Chan ch = chans[blockIdx.z];
   if (ch.tri) return;
   const View v(arena, ch);
   int tile, slot;
-  if (!xcd_tile(ntile, nslot, tile, slot)) return;
+  if (!xcd_tile(ntile, nslot, tile, slot, nzfwd, ndt)) return;
   if (slot >= ch.nslot) return;
   const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
   const int TI = tile_of8(E, Ng2, nxh);
@@ -530,7 +598,6 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
   const bool isU = slot < ch.nplu;
   const int np = isU ? 4 : 2;
   const int j = isU ? slot + 1 : slot - ch.nplu + 1;
-  extern __shared__ double2 smem_d[];
   // np ni sequences of Ng2 points, S apart (padded: conflict-free stages);
   // one buffer when the stages run in place
   const int S = seq_stride(Ng2, 4 * TI, sizeof(C));
@@ -1621,58 +1688,8 @@ __global__ void k_from_rows(const double2 *__restrict__ src, double2 *__restrict
 // with u2mloc starting at 1e-6), then dt = min(2.5 dtv, CFL dtc) when the
 // channel's step is adaptive, and t += dt.  One workgroup per channel.
 __global__ void __launch_bounds__(1024) k_dt_update(char *arena, const Chan *chans, int ntiles_max, int e_xprod) {
-  const Chan ch = chans[blockIdx.x];
-  View v(arena, ch);
-  if (ch.tri) {  // no physical planes on the list path: dt stays fixed
-    if (threadIdx.x == 0) v.scal[SC_T] = v.scal[SC_T] + v.scal[SC_DT];
-    return;
-  }
-  double m1 = 0.0, m2 = 1e-6, m3 = 0.0;
-  // k_xprod's tiles of this channel (stride: the launch's tile count)
-  const int TZ = tile_of(e_xprod, ch.Ng1, ch.Ng2), nt = (ch.Ng2 + TZ - 1) / TZ;
-  const double2 *cm = reinterpret_cast<const double2 *>(v.cfl);
-  const int nu = ch.nplu * nt, nall = ch.nslot * nt;
-#pragma unroll 4
-  for (int q = threadIdx.x; q < nu; q += blockDim.x) {
-    const int slot = q / nt, tz = q - slot * nt;
-    const double2 c = cm[(size_t)slot * ntiles_max + tz];
-    m1 = fmax(m1, c.x);
-    m3 = fmax(m3, c.y);
-  }
-#pragma unroll 4
-  for (int q = nu + threadIdx.x; q < nall; q += blockDim.x) {
-    const int slot = q / nt, tz = q - slot * nt;
-    const int j = slot - ch.nplu + 1;
-    m2 = fmax(m2, cm[(size_t)slot * ntiles_max + tz].x * v.dthdyv[j]);
-  }
-  __shared__ double red[3][16];
-  for (int off = 32; off >= 1; off >>= 1) {
-    m1 = fmax(m1, __shfl_xor(m1, off));
-    m2 = fmax(m2, __shfl_xor(m2, off));
-    m3 = fmax(m3, __shfl_xor(m3, off));
-  }
-  const int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red[0][wv] = m1;
-    red[1][wv] = m2;
-    red[2][wv] = m3;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int q = 1; q < nw; q++) {
-      m1 = fmax(m1, red[0][q]);
-      m2 = fmax(m2, red[1][q]);
-      m3 = fmax(m3, red[2][q]);
-    }
-    m1 = fmax(m1, red[0][0]);
-    m2 = fmax(m2, red[1][0]);
-    m3 = fmax(m3, red[2][0]);
-    const double dtc = fmin(fmin(1.0 / (ch.alp * (ch.N1 / 2) * m1), 1.0 / (ch.bet * (ch.N2 / 2) * m3)), 1.0 / (m2 * 1.0));
-    v.scal[SC_DTC] = dtc;
-    if (ch.adaptive) v.scal[SC_DT] = fmin(2.5 * v.scal[SC_DTV], ch.CFL * dtc);
-    v.scal[SC_T] = v.scal[SC_T] + v.scal[SC_DT];
-  }
+  __shared__ double red[48];
+  dt_update_block(arena, chans[blockIdx.x], ntiles_max, e_xprod, red);
 }
 
 // unit-forcing solutions of the mean mode per substep (flowrate_corr's
@@ -1803,7 +1820,7 @@ void shape_tiles(ShapeMax &m) {
 }
 
 hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s,
-                           int kRK, int ghost) {
+                           int kRK, int ghost, int dtblk) {
   size_t es = mixed ? sizeof(float2) : sizeof(double2);
   dim3 b(256);
   if (stage == 0) {
@@ -1817,9 +1834,9 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
     dim3 g(nz + extra, 1, m.nchan * 3);
     const size_t lds = std::max(m.lds_zinv * es, ghost ? tex_1d_lds(n2) : (size_t)0);
     if (mixed)
-      hipLaunchKernelGGL(k_zinv<float2>, g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz);
+      hipLaunchKernelGGL(k_zinv<float2>, g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz, extra);
     else
-      hipLaunchKernelGGL(k_zinv<double2>, g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz);
+      hipLaunchKernelGGL(k_zinv<double2>, g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz, extra);
   } else if (stage == 1) {
     dim3 g(m.xprod_tiles, m.nslot, m.nchan);
     const bool cfl = kRK == 1;
@@ -1841,9 +1858,12 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
     }
 #undef LH_XP
   } else {
-    dim3 g(xcd_grid(m.zfwd_tiles, m.nslot), 1, m.nchan);
-#define LH_ZF(T, I) \
-  hipLaunchKernelGGL((k_zfwd<T, I>), g, b, m.lds_zfwd * es, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot)
+    const int nz = (int)xcd_grid(m.zfwd_tiles, m.nslot);
+    dim3 g(nz + (dtblk ? 1 : 0), 1, m.nchan);
+    const size_t lds = std::max(m.lds_zfwd * es, (size_t)48 * sizeof(double));
+#define LH_ZF(T, I)                                                                                              \
+  hipLaunchKernelGGL((k_zfwd<T, I>), g, b, lds, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot, nz, dtblk ? 1 : 0, \
+                     m.xprod_tiles, m.e_xprod)
     if (mixed) {
       if (m.zfwd_ip) LH_ZF(float2, true); else LH_ZF(float2, false);
     } else {
@@ -1855,9 +1875,9 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
 }
 
 hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s, int kRK,
-                            int ghost) {
+                            int ghost, int dtblk) {
   for (int st = 0; st < 3; st++) {
-    hipError_t e = launch_nl_stage(arena, d_chans, m, mixed, st, s, kRK, st == 0 ? ghost : 0);
+    hipError_t e = launch_nl_stage(arena, d_chans, m, mixed, st, s, kRK, st == 0 ? ghost : 0, st == 2 ? dtblk : 0);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
