@@ -217,7 +217,8 @@ struct lh_ctx {
   ShapeMax shape;
   double2 *staging = nullptr;
   size_t staging_elems = 0;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph = nullptr;   // one captured step (the first of an lh_step call)
+  hipGraphExec_t graph2 = nullptr;  // the steps after it, when they take over the ghost pass (ghost_deferrable)
   int device = 0;
   std::vector<TexGroup> tex_groups;  // N-grid work planes per (N1, N2) shape
   int ntex = 0;                      // channels with a textured wall
@@ -562,14 +563,17 @@ int time_step(lh_ctx *x, int kRK, bool dt_done = false) {
 }
 
 // in_step: one of the three substeps of a captured step, which may hand
-// substeps 1 and 2's ghost pass to the next substep (ghost_deferrable)
-int run_substep(lh_ctx *x, int kRK, bool in_step = false) {
+// substeps 1 and 2's ghost pass to the next substep (ghost_deferrable);
+// with chained steps, the third substep's pass goes to the next step's first
+// substep as well (prev_ghost: this step's first substep takes it; keep_last:
+// leave this step's third pass to the next step)
+int run_substep(lh_ctx *x, int kRK, bool in_step = false, bool prev_ghost = false, bool keep_last = false) {
   const bool defer = in_step && ghost_deferrable(x);
   // the time-step update rides on the z-forward launch when there is one
   // (it reads the x-product stage's CFL maxima; the closure and triad-list
   // kernels between the two read neither dt nor t)
   const bool dt_in_zfwd = kRK == 1 && x->ntri < (int)x->ch.size();
-  int rc = nonlinear_stage(x, kRK, defer && kRK > 1, dt_in_zfwd);
+  int rc = nonlinear_stage(x, kRK, defer && (kRK > 1 || prev_ghost), dt_in_zfwd);
   if (rc) return rc;
   rc = time_step(x, kRK, dt_in_zfwd);
   if (rc) return rc;
@@ -577,7 +581,25 @@ int run_substep(lh_ctx *x, int kRK, bool in_step = false) {
   // textured walls the solve computes the right-hand sides itself
   const int fuse = x->shape.rhs_fused && !x->ntex;
   if (!fuse) HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
-  return solve_phase(x, kRK, fuse, defer && kRK < 3);
+  return solve_phase(x, kRK, fuse, defer && (kRK < 3 || keep_last));
+}
+
+// capture one step into *out
+int capture_step(lh_ctx *x, bool prev_ghost, bool keep_last, hipGraphExec_t *out) {
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+  int r = LH_OK;
+  for (int k = 1; k <= 3 && r == LH_OK; k++) r = run_substep(x, k, true, prev_ghost && k == 1, keep_last && k == 3);
+  hipError_t e = hipStreamEndCapture(x->stream, &g);
+  if (r) return r;
+  if (e != hipSuccess) return set_err(LH_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return set_err(LH_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  return LH_OK;
 }
 
 // Thomas factors and the textured-wall inverse rows (after a grid change)
@@ -713,6 +735,7 @@ int lh_ctx_destroy(lh_ctx *x) {
   (void)hipSetDevice(x->device);
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   if (x->graph) (void)hipGraphExecDestroy(x->graph);
+  if (x->graph2) (void)hipGraphExecDestroy(x->graph2);
   if (x->arena) (void)hipFree(x->arena);
   if (x->d_chans) (void)hipFree(x->d_chans);
   if (x->staging) (void)hipFree(x->staging);
@@ -903,22 +926,14 @@ int lh_step(lh_ctx *x, int nsteps) {
   if (rc) return rc;
   if (nsteps < 0) return set_err(LH_EINVAL, "nsteps must be >= 0");
   if (nsteps == 0) return LH_OK;
-  if (!x->graph) {
-    hipGraph_t g;
-    HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
-    int r = LH_OK;
-    for (int k = 1; k <= 3 && r == LH_OK; k++) r = run_substep(x, k, true);
-    hipError_t e = hipStreamEndCapture(x->stream, &g);
-    if (r) return r;
-    if (e != hipSuccess) return set_err(LH_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&x->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-      x->graph = nullptr;
-      return set_err(LH_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-    }
-  }
-  for (int n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(x->graph, x->stream));
+  // chained steps: each step after the first takes over the previous
+  // step's last ghost pass, and one launch after the last step makes it
+  const bool chain = ghost_deferrable(x);
+  if (!x->graph && (rc = capture_step(x, false, chain, &x->graph))) return rc;
+  if (chain && nsteps > 1 && !x->graph2 && (rc = capture_step(x, true, true, &x->graph2))) return rc;
+  HIPCHK(hipGraphLaunch(x->graph, x->stream));
+  for (int n = 1; n < nsteps; n++) HIPCHK(hipGraphLaunch(chain ? x->graph2 : x->graph, x->stream));
+  if (chain) HIPCHK(launch_tex_1d(x->arena, x->d_chans, x->shape, 1, 1, 1, x->stream));
   HIPCHK(hipGetLastError());
   for (auto &h : x->ch) h.iter += nsteps;
   return LH_OK;
@@ -1280,6 +1295,10 @@ static void drop_graph(lh_ctx *x) {
   if (x->graph) {
     (void)hipGraphExecDestroy(x->graph);
     x->graph = nullptr;
+  }
+  if (x->graph2) {
+    (void)hipGraphExecDestroy(x->graph2);
+    x->graph2 = nullptr;
   }
 }
 
