@@ -276,8 +276,8 @@ def test_100_steps_mean_and_rms_profiles():
     assert abs(s.scalars(0)["t"] - o.scalars()["t"]) <= 1e-12
 
 
-def tex_pair(cfg, masks, seed=5):
-    s, o, f = make_pair(cfg, seed=seed)
+def tex_pair(cfg, masks, seed=5, precision="fp64"):
+    s, o, f = make_pair(cfg, seed=seed, precision=precision)
     s.set_texture(0, *masks)
     o.set_texture(*masks)
     return s, o
@@ -329,16 +329,20 @@ def test_ridges_wall_paths(sep):
         assert e <= 1e-11, (nm, e)
 
 
-@pytest.mark.parametrize("n", [32, 64])
-def test_ghost_pass_in_zinv(n):
-    """Captured steps that leave the ridges' ghost-row pass of substeps 1
-    and 2 to extra workgroups of the next z-inverse launch give the oracle's
-    steps and the separate launch's steps (the same arithmetic)."""
+@pytest.mark.parametrize("n,precision", [(32, "fp64"), (64, "fp64"), (32, "mixed")])
+def test_ghost_pass_in_zinv(n, precision):
+    """Captured steps that leave the ridges' ghost-row pass to extra
+    workgroups of the next z-inverse launch (within a step and across the
+    steps of one call) give the oracle's steps and the separate launch's
+    steps (the same arithmetic).  In mixed precision the z-inverse stage is
+    FP32 and the ghost pass FP64; the oracle bound there is the mixed mode's
+    1e-5 (1e-4 for p, which carries ~10x the velocity round-off)."""
     cfg = small_cfg(n, 33)
     m = textures.stripes(n, n, 4, 0.5)
+    tol = 1e-11 if precision == "fp64" else 1e-5
     out = {}
     for g in (1, 0):
-        s, o = tex_pair(cfg, (m, m), seed=19)
+        s, o = tex_pair(cfg, (m, m), seed=19, precision=precision)
         s.set_tuning(_lib.TUNE_GHOST_IN_ZINV, g)
         assert s.get_tuning(_lib.TUNE_GHOST_IN_ZINV) == g
         s.step(3)
@@ -347,8 +351,8 @@ def test_ghost_pass_in_zinv(n):
             o.step(3)
             for nm in out[g]:
                 e = relerr(out[g][nm], o.get(nm))
-                say("ghost in zinv", n, nm, e)
-                assert e <= 1e-11, (nm, e)
+                say("ghost in zinv", n, precision, nm, e)
+                assert e <= (10 * tol if nm == "p" and precision == "mixed" else tol), (nm, e)
     for nm in out[1]:
         e = relerr(out[1][nm], out[0][nm])
         say("ghost in zinv vs separate", n, nm, e)
