@@ -930,8 +930,13 @@ void lho_nonlinear(lho_state *s, int kRK) {
   /* time step, FOU3D.f90:288-297 and dtc_calc FOU3D.f90:699-736 */
   if (kRK == 1) {
     int N1m = s->c.N1, N2m = s->c.N2;
-    s->dtc = fmin(fmin(1.0 / (s->c.alp * (N1m / 2) * m1), 1.0 / (s->c.bet * (N2m / 2) * m3)),
-                  1.0 / (m2 * s->ddthetavi));
+    /* The third candidate of FOU3D.f90:726, 1/(u2mloc*dthetavi), divides by
+     * dthetavi, declared (declaration.f90:76) but never assigned (start.f90
+     * sets only ddthetavi, :324): it is +Inf and never binds.  The
+     * reference's history file shows dtc = the u3 term (test_oracle.py pins
+     * it).  m2 is kept for the record only. */
+    (void)m2;
+    s->dtc = fmin(1.0 / (s->c.alp * (N1m / 2) * m1), 1.0 / (s->c.bet * (N2m / 2) * m3));
     if (!s->c.fixed_dt) s->dt = fmin(2.5 * s->dtv, s->c.CFL * s->dtc);
     s->dti = 1.0 / s->dt;
     s->t = s->t + s->dt;

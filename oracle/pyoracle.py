@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liblhoracle.so")
+# $LHO_LIB selects another build of the same source (the pin tests load
+# copies with planted mistakes this way, in child processes)
+_LIB = os.environ.get("LHO_LIB") or os.path.join(_HERE, "liblhoracle.so")
 
 U1, U2, U3, P, NU1, NU2, NU3, DU1, DU2, DU3, PSI, DIV = range(12)
 FIELD_IDS = {"u1": U1, "u2": U2, "u3": U3, "p": P, "Nu1": NU1, "Nu2": NU2,

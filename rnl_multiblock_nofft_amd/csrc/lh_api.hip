@@ -1659,6 +1659,11 @@ int lh_set_texture(lh_ctx *x, int ch, const int *bottom, const int *top) {
   if (rc) return rc;
   if (ch < 0 || ch >= (int)x->ch.size()) return set_err(LH_EINVAL, "lh_set_texture: bad channel %d", ch);
   HostChan &h = x->ch[ch];
+  // the wall passes transform at lengths N1 and N2 (the N grid of
+  // four_to_phys_N), which the LDS transforms support for 2^a 3^b only
+  if ((bottom || top) && (!valid_len(h.cfg.N1) || !valid_len(h.cfg.N2)))
+    return set_err(LH_EINVAL, "lh_set_texture: textured walls need N1, N2 of the form 2^a 3^b (got %d, %d)",
+                   h.cfg.N1, h.cfg.N2);
   const size_t n = (size_t)h.cfg.N1 * h.cfg.N2;
   std::vector<int> m(2 * n, 1);
   const int *src[2] = {bottom, top};

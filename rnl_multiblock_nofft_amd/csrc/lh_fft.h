@@ -265,7 +265,7 @@ __device__ __forceinline__ C *fft_lds_rt(C *x, C *y, int n, int nb, int sstr, co
   int Ns = 1;
   for (int m = n; m > 1;) {
     const int Rr = plan_radix(m);
-    if (Rr == 0) break;
+    if (Rr == 0) break;  // lengths other than 2^a 3^b are rejected on the host
     const int nbf = n / Rr;
     const int span = n / (Ns * Rr);  // twiddle exponent unit
     const int total = nb * nbf;
@@ -360,7 +360,7 @@ __device__ __forceinline__ C *fft_bf_rt(C *x, C *y, int n, int nb, int sstr, con
   int Ns = 1;
   for (int m = n; m > 1;) {
     const int Rr = plan_radix(m);
-    if (Rr == 0) break;
+    if (Rr == 0) break;  // lengths other than 2^a 3^b are rejected on the host
     const int nbf = n / Rr;
     const int span = n / (Ns * Rr);
     const int total = nb * nbf;

@@ -525,11 +525,11 @@ __device__ __forceinline__ void dt_update_block(char *arena, const Chan &ch, int
     if (threadIdx.x == 0) v.scal[SC_T] = v.scal[SC_T] + v.scal[SC_DT];
     return;
   }
-  double m1 = 0.0, m2 = 1e-6, m3 = 0.0;
+  double m1 = 0.0, m3 = 0.0;
   // k_xprod's tiles of this channel (stride: the launch's tile count)
   const int TZ = tile_of(e_xprod, ch.Ng1, ch.Ng2), nt = (ch.Ng2 + TZ - 1) / TZ;
   const double2 *cm = reinterpret_cast<const double2 *>(v.cfl);
-  const int nu = ch.nplu * nt, nall = ch.nslot * nt;
+  const int nu = ch.nplu * nt;
 #pragma unroll 4
   for (int q = threadIdx.x; q < nu; q += blockDim.x) {
     const int slot = q / nt, tz = q - slot * nt;
@@ -537,21 +537,13 @@ __device__ __forceinline__ void dt_update_block(char *arena, const Chan &ch, int
     m1 = fmax(m1, c.x);
     m3 = fmax(m3, c.y);
   }
-#pragma unroll 4
-  for (int q = nu + threadIdx.x; q < nall; q += blockDim.x) {
-    const int slot = q / nt, tz = q - slot * nt;
-    const int j = slot - ch.nplu + 1;
-    m2 = fmax(m2, cm[(size_t)slot * ntiles_max + tz].x * v.dthdyv[j]);
-  }
   for (int off = 32; off >= 1; off >>= 1) {
     m1 = fmax(m1, __shfl_xor(m1, off));
-    m2 = fmax(m2, __shfl_xor(m2, off));
     m3 = fmax(m3, __shfl_xor(m3, off));
   }
   const int wv = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     red[wv] = m1;
-    red[16 + wv] = m2;
     red[32 + wv] = m3;
   }
   __syncthreads();
@@ -559,13 +551,15 @@ __device__ __forceinline__ void dt_update_block(char *arena, const Chan &ch, int
     const int nw = (blockDim.x + 63) >> 6;
     for (int q = 1; q < nw; q++) {
       m1 = fmax(m1, red[q]);
-      m2 = fmax(m2, red[16 + q]);
       m3 = fmax(m3, red[32 + q]);
     }
     m1 = fmax(m1, red[0]);
-    m2 = fmax(m2, red[16]);
     m3 = fmax(m3, red[32]);
-    const double dtc = fmin(fmin(1.0 / (ch.alp * (ch.N1 / 2) * m1), 1.0 / (ch.bet * (ch.N2 / 2) * m3)), 1.0 / (m2 * 1.0));
+    // The reference's third candidate 1/(u2mloc*dthetavi) (FOU3D.f90:726)
+    // divides by dthetavi, which is declared (declaration.f90:76) but never
+    // assigned (only ddthetavi is, start.f90:324): it is +Inf and never
+    // binds.  Its history file confirms dtc = the u3 term.  Same here.
+    const double dtc = fmin(1.0 / (ch.alp * (ch.N1 / 2) * m1), 1.0 / (ch.bet * (ch.N2 / 2) * m3));
     v.scal[SC_DTC] = dtc;
     if (ch.adaptive) v.scal[SC_DT] = fmin(2.5 * v.scal[SC_DTV], ch.CFL * dtc);
     v.scal[SC_T] = v.scal[SC_T] + v.scal[SC_DT];

@@ -22,6 +22,11 @@ def perturbed_poiseuille(cfg, grid, seed, amp=0.1, kmax=8):
     u, w = even and odd wall-normal shapes (1-y^2)^2, y(1-y^2)^2, the even
     part projected so that kx a + kz b = 0, and v from discrete continuity
     (divergence, littleharsh.f90:237-272) integrated from the lower wall.
+    Closing v at the top wall leaves the last row divergent, so the
+    perturbation is then projected once, as solveP + v_corr do
+    (littleharsh.f90:581-633, 956-1052): (D_y G_y + k^2) phi = div with the
+    Neumann rows of LU_buildP (tridLU_3D.f90:333-407), u -= grad phi; the
+    discrete divergence is then zero to rounding on every pressure row.
     Ghost rows follow the no-slip ghost rule u(0) = gw u(1)
     (v_corr, littleharsh.f90:992-1044)."""
     N1, N2, ny = cfg["N1"], cfg["N2"], cfg["ny"]
@@ -75,6 +80,7 @@ def perturbed_poiseuille(cfg, grid, seed, amp=0.1, kmax=8):
         u2[:, j] = u2[:, j - 1] - D / dyu[j]
     u2[:, ny - 1] = 0.0
     u2[:, 0] = 0.0
+    _project(u1, u2, u3, kxs, kzs, grid)
     # mean flow
     u1[0, 1:nyu - 1] += 1.0 - yu[1:nyu - 1] ** 2
     gw = (yu[0] - yv[0]) / (yu[1] - yv[0])
@@ -84,6 +90,53 @@ def perturbed_poiseuille(cfg, grid, seed, amp=0.1, kmax=8):
     u3[:, nyu - 1] = gw * u3[:, nyu - 2]
     p = np.zeros((ncol, ny - 1), np.complex128)
     return dict(u1=u1, u2=u2, u3=u3, p=p)
+
+
+def divergence(u1, u2, u3, kxs, kzs, dthdyu):
+    """divergence on the pressure rows j = 1..ny-1 (littleharsh.f90:237-272)"""
+    ny = u2.shape[1]
+    j = np.arange(1, ny)
+    return (1j * kxs[:, None] * u1[:, j] + 1j * kzs[:, None] * u3[:, j]
+            + dthdyu[j][None, :] * (u2[:, j] - u2[:, j - 1]))
+
+
+def _project(u1, u2, u3, kxs, kzs, grid):
+    """u -= grad phi with (D_y G_y + k^2) phi = div, v fixed at both walls
+    (Neumann rows); every column but the mean mode (whose divergence is
+    zero: its v vanishes).  Batched Thomas sweep over the columns."""
+    du, dv = grid["dthdyu"], grid["dthdyv"]
+    ny = u2.shape[1]
+    ne = ny - 1                                   # pressure rows 1..ne
+    rhs = divergence(u1, u2, u3, kxs, kzs, du)    # (ncol, ne)
+    k2 = -(kxs * kxs + kzs * kzs)
+    j = np.arange(1, ne + 1)
+    lo = du[j] * dv[j - 1]                        # phi(j-1)
+    up = du[j] * dv[np.minimum(j, ny - 1)]        # phi(j+1)
+    lo[0] = 0.0                                   # v(0) fixed
+    up[-1] = 0.0                                  # v(ny-1) fixed
+    sel = np.arange(1, len(kxs))                  # skip the mean mode
+    diag = (-lo - up)[None, :] + k2[sel, None]
+    r = rhs[sel]
+    n = ne
+    cp = np.zeros((len(sel), n))
+    dp = np.zeros((len(sel), n), np.complex128)
+    cp[:, 0] = up[0] / diag[:, 0]
+    dp[:, 0] = r[:, 0] / diag[:, 0]
+    for q in range(1, n):
+        m = diag[:, q] - lo[q] * cp[:, q - 1]
+        cp[:, q] = up[q] / m
+        dp[:, q] = (r[:, q] - lo[q] * dp[:, q - 1]) / m
+    phi = np.zeros((len(kxs), ne + 2), np.complex128)  # rows 0..ny (ends unused)
+    x = dp[:, n - 1]
+    phi[sel, n] = x
+    for q in range(n - 2, -1, -1):
+        x = dp[:, q] - cp[:, q] * x
+        phi[sel, q + 1] = x
+    jj = np.arange(1, ne + 1)
+    u1[:, jj] -= 1j * kxs[:, None] * phi[:, jj]
+    u3[:, jj] -= 1j * kzs[:, None] * phi[:, jj]
+    jv = np.arange(1, ny - 1)
+    u2[:, jv] -= dv[jv][None, :] * (phi[:, jv + 1] - phi[:, jv])
 
 
 def _phys_max(f, N1, N2, Ng1, Ng2):

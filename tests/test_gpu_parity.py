@@ -398,6 +398,18 @@ def test_ensemble_textured_and_smooth():
             assert e <= 1e-11, (q, nm, e)
 
 
+def test_texture_rejects_unsupported_wall_length():
+    """Textured walls transform at N1, N2 (the N grid); a length the LDS
+    transforms cannot do (20 = 2^2 5) is an error, not a partial transform."""
+    cfg = dict(small_cfg(8, 17), N1=20, N2=20, Ngal1=32, Ngal2=32)
+    s = Solver([cfg])
+    m = np.ones((20, 20), np.int32)
+    m[:, ::2] = 2
+    with pytest.raises(_lib.LhError):
+        s.set_texture(0, m, None)
+    s.close()
+
+
 def test_bad_texture_code_raises():
     s = Solver([small_cfg(8, 17)])
     m = np.full((8, 8), 3, np.int32)

@@ -123,6 +123,10 @@ def test_one_step_close_to_reference_snapshot():
     o.step(1)
     s = o.scalars()
     assert s["dt"] == out["u1"]["t"]                     # 2.5 dtv, exactly
+    # dtc of the first substep against the reference's history record of
+    # iteration 1: the u3 candidate; a v candidate would give 1.13e-2
+    h1 = np.load(os.path.join(GOLD, "reference_case.npz"))["hist_iter1"]
+    assert s["dtc"] == pytest.approx(h1[2], rel=1e-7)
     bounds = {"u1": 2e-3, "u2": 1e-2, "u3": 2e-2, "p": 0.5}
     for f in inp:
         assert relerr(o.get(f), out[f]["field"]) < bounds[f], f
@@ -276,9 +280,10 @@ def test_nonlinear_term_matches_brute_force_convolution():
 
 def test_dtc_closed_forms():
     """dtc_calc (FOU3D.f90:699-736) on uniform fields: a streamwise flow U
-    gives 1/(alp (N1/2) U), a spanwise flow W gives 1/(bet (N2/2) W), a v
-    field gives 1/max_j(|V| dthdyv(j)) over the v-planes; the adaptive dt is
-    min(2.5 dtv, CFL dtc) and t advances by it."""
+    gives 1/(alp (N1/2) U), a spanwise flow W gives 1/(bet (N2/2) W); the v
+    candidate divides by the never-assigned dthetavi (= 0, FOU3D.f90:726), so
+    a v-only field gives +Inf; the adaptive dt is min(2.5 dtv, CFL dtc) and t
+    advances by it."""
     cfg = small_cfg(N=8, ny=13, fixed_dt=0, CFL=0.4)
     ny = cfg["ny"]
     for U, W, V in ((1.3, 0.0, 0.0), (0.0, 0.7, 0.0), (0.0, 0.0, 2e-3), (0.9, 0.4, 1e-3)):
@@ -294,15 +299,34 @@ def test_dtc_closed_forms():
         o.set("u3", u3)
         o.set("u2", u2)
         o.set("p", np.zeros((ncol, ny - 1), complex))
-        dv = o.grid()["dthdyv"]
-        m2 = max(1e-6, abs(V) * dv[1:ny - 1].max())
-        cands = [1.0 / m2]
+        cands = [np.inf]
         if U:
             cands.append(1.0 / (cfg["alp"] * (cfg["N1"] // 2) * U))
         if W:
             cands.append(1.0 / (cfg["bet"] * (cfg["N2"] // 2) * W))
         o.phase("nonlinear", 1)
         sc = o.scalars()
-        assert np.isclose(sc["dtc"], min(cands), rtol=1e-14), (U, W, V, sc["dtc"], cands)
+        assert sc["dtc"] == min(cands) or np.isclose(sc["dtc"], min(cands), rtol=1e-14), \
+            (U, W, V, sc["dtc"], cands)
         dt = min(2.5 * sc["dtv"], cfg["CFL"] * sc["dtc"])
         assert np.isclose(sc["dt"], dt, rtol=1e-15) and np.isclose(sc["t"], dt, rtol=1e-15)
+
+
+@pytest.mark.parametrize("N,ny", [(8, 17), (16, 33), (32, 153)])
+def test_initial_field_is_divergence_free(N, ny):
+    """The synthetic start (Poiseuille + perturbation) is projected once, so
+    the discrete divergence (littleharsh.f90:237-272) vanishes on every
+    pressure row, the top one included, and v is zero at both walls."""
+    from helpers import init_fields
+
+    cfg = small_cfg(N, ny)
+    o = oracle_for(cfg)
+    f = init_fields(cfg, o.grid(), seed=1234)
+    for n in f:
+        o.set(n, f[n])
+    d = o.divergence()
+    nxh = N // 2 + 1
+    kx = np.arange(o.ncol) % nxh
+    scale = np.abs(kx[:, None] * f["u1"]).max()
+    assert np.abs(d).max() <= 1e-14 * scale
+    assert np.abs(f["u2"][:, 0]).max() == 0 and np.abs(f["u2"][:, -1]).max() == 0
