@@ -1,5 +1,7 @@
 """Benchmark: grid-point RK3 substeps per second of the plain-FFT RK3 step
 on one MI355X (FP64 plain by default; --precision mixed for FP32 FFTs).
+The default workload is the largest single-GPU channel: 256x256x257,
+Re_b = 10000, 8x8 posts at spacing 16 on both walls, dt = 1e-3.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
                   [--precision fp64|mixed] [--no-cpu]
@@ -111,17 +113,32 @@ def triad_bytes(cfg, files):
             + 2 * (5 * nplu + 4 * nplv) * row + 3 * (nplu + nplv) * row)
 
 
-def cpu_baseline(cfg, seconds=12.0, max_sub=100000, triads_mode=None):
-    """The FP64 CPU oracle (OpenMP, up to 16 threads) on one channel of the
-    workload for about `seconds` of CPU work: a reported baseline."""
+def host_cpu():
+    """(nproc, CPU model) of this host"""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def cpu_baseline(cfg, seconds=20.0, max_sub=100000, triads_mode=None):
+    """The FP64 CPU oracle (OpenMP) on one channel of the workload for about
+    `seconds` of CPU work: a reported baseline.  Threads: this process's CPU
+    share (OMP_NUM_THREADS, which the GPU hosts set to their per-GPU share),
+    else every core the OS reports."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import init_fields, oracle_for
     from oracle import pyoracle as po
     from rnl_multiblock_nofft_amd import Solver  # noqa: F401
 
-    cores = os.cpu_count() or 1
-    cores = min(cores, 16)
-    po.set_threads(cores)
+    nproc, model = host_cpu()
+    cores = int(os.environ.get("OMP_NUM_THREADS") or nproc)
+    cores = po.set_threads(cores)
     o = oracle_for(cfg)
     g = o.grid()
     f = init_fields(cfg, g, seed=1234)
@@ -130,7 +147,9 @@ def cpu_baseline(cfg, seconds=12.0, max_sub=100000, triads_mode=None):
     o.init_flowrate()
     if triads_mode:
         o.set_triads(*triad_setup(cfg, triads_mode))
-    o.substep(1)  # warm plans / threads
+    gp = cfg["N1"] * cfg["N2"] * cfg["ny"]
+    if gp < 2000000:
+        o.substep(1)  # warm plans / threads (a large channel's first substep is its sample)
     n = 0
     t0 = time.perf_counter()
     while n < max_sub:
@@ -139,19 +158,20 @@ def cpu_baseline(cfg, seconds=12.0, max_sub=100000, triads_mode=None):
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
-    gp = cfg["N1"] * cfg["N2"] * cfg["ny"]
     return dict(value=gp * n / dt, unit="grid-pt RK3 substeps/s", cores=cores, kind="port",
-                sample="%d RK3 substeps of one %dx%dx%d channel (FP64 CPU oracle%s, OpenMP, %d threads), %.1f s"
+                nproc=nproc, cpu_model=model,
+                sample="%d RK3 substeps of one %dx%dx%d channel (FP64 CPU oracle%s, OpenMP, %d threads "
+                       "of %d cores, %s), %.1f s"
                        % (n, cfg["N1"], cfg["N2"], cfg["ny"],
-                          ", triad-list path" if triads_mode else "", cores, dt))
+                          ", triad-list path" if triads_mode else "", cores, nproc, model, dt))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="ridges_64x64x153")
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="large_256x256x257")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", default="",
@@ -254,7 +274,12 @@ def main():
                     kernels_ms=dict({k: round(float(ms[q]), 5) for q, k in enumerate(kernels)},
                                     **{TEX: round(float(ms[5]), 5)}))
     bmodel = sum(baseline_model_bytes(c, args.precision) for c in cfgs)
-    hbm_pct = 100.0 * bmodel * (3 * args.steps / elapsed) / (HBM_PEAK_GBS * 1e9)
+    sub_per_s = 3 * args.steps / elapsed
+    hbm_pct = 100.0 * bmodel * sub_per_s / (HBM_PEAK_GBS * 1e9)
+    # what the fused kernels must move per substep (their algorithmic bytes,
+    # texture passes left out): the bandwidth the step actually uses
+    step_alg = sum(alg[k] for k in kernels)
+    hbm_pct_alg = 100.0 * step_alg * sub_per_s / (HBM_PEAK_GBS * 1e9)
 
     cpu = None
     if not args.no_cpu:
@@ -270,6 +295,8 @@ def main():
                            dt=cfgs[0]["dt"],
                            nonlinear=("triad lists (%s)" % args.triads) if args.triads else "fft",
                            grid_points=gp, hbm_pct_of_peak_model=round(hbm_pct, 3),
+                           alg_bytes_per_substep=step_alg,
+                           hbm_pct_of_peak_alg_bytes=round(hbm_pct_alg, 3),
                            tuning=tuning),
                roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(rec), flush=True)

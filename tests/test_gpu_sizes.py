@@ -1,0 +1,177 @@
+"""GPU parity at the workload sizes the benchmark and the studies run, through
+the C-ABI, against the FP64 CPU oracle on the same seeded fields:
+
+  * 64x64x153 channel with streamwise ridges on both walls (the textured
+    step the mixed-precision comparison is quoted on): FP64 and mixed;
+  * the 64-channel ensemble (48 of 32x32x153, 16 of 64x64x153, square posts
+    of per-channel spacing and gas fraction) in one launch, autotuned and
+    graph-replayed: channels of both sizes;
+  * the 256x256x257 channel with 8x8 posts at spacing 16: one step, plus the
+    divergence and flow-rate invariants;
+  * the learned closure on the 32x32x153 coarse channel;
+  * determinism: two FP64 runs of the same input are bitwise equal.
+
+Bars: FP64 relative L2 <= 1e-12 per field after one step; mixed precision
+<= 1e-5 per field (1e-4 on p, whose error carries ~10x the velocities'
+round-off through the projection's near-cancellation); the closure step
+within 1e-2 of the closure's own effect (bf16 network)."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from helpers import init_fields, oracle_for, relerr
+from rnl_multiblock_nofft_amd import Solver, configs
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u1", "u2", "u3", "p")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def pair(cfg, seed, precision="fp64"):
+    s = Solver([cfg], precision=precision)
+    o = oracle_for(cfg)
+    f = init_fields(cfg, s.grid(0), seed)
+    for nm in FIELDS:
+        s.upload(0, nm, f[nm])
+        o.set(nm, f[nm])
+    s.init_flowrate()
+    o.init_flowrate()
+    return s, o, f
+
+
+def test_ridges_64_fp64_one_step():
+    cfg = configs.ridges_64()[0]
+    say("ridges 64x64x153 fp64")
+    s, o, _ = pair(cfg, seed=1234)
+    s.step(1)
+    o.step(1)
+    for nm in FIELDS:
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("ridges fp64", nm, e)
+        assert e <= 1e-12, (nm, e)
+    s.close()
+
+
+def test_ridges_64_mixed_per_field_error():
+    cfg = configs.ridges_64()[0]
+    say("ridges 64x64x153 mixed")
+    s, o, _ = pair(cfg, seed=1234, precision="mixed")
+    s.step(1)
+    o.step(1)
+    for nm in FIELDS:
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("ridges mixed vs fp64 oracle", nm, e)
+        assert e <= (1e-4 if nm == "p" else 1e-5), (nm, e)
+    s.close()
+
+
+def test_ensemble_64_channels_both_sizes():
+    cfgs = configs.ensemble_64()
+    say("ensemble", len(cfgs), "channels")
+    s = Solver(cfgs)
+    check = [0, 7, 23, 47, 48, 55, 60, 63]          # 4 of 32x32, 4 of 64x64
+    oracles = {}
+    for q, c in enumerate(cfgs):
+        f = init_fields(c, s.grid(q), seed=1234 + q)
+        for nm in FIELDS:
+            s.upload(q, nm, f[nm])
+        if q in check:
+            o = oracle_for(c)
+            for nm in FIELDS:
+                o.set(nm, f[nm])
+            o.init_flowrate()
+            oracles[q] = o
+    s.init_flowrate()
+    t = time.time()
+    s.autotune(reps=2)
+    say("autotuned in", time.time() - t, "s:", s.tuning())
+    s.step(1)
+    for q in check:
+        oracles[q].step(1)
+        for nm in FIELDS:
+            e = relerr(s.download(q, nm), oracles[q].get(nm))
+            say("ensemble chan", q, cfgs[q]["N1"], nm, e)
+            assert e <= 1e-12, (q, nm, e)
+    s.close()
+
+
+@pytest.mark.timeout(900)
+def test_large_256_posts_one_step():
+    cfg = configs.large_256()[0]
+    say("large 256x256x257 posts")
+    s, o, f = pair(cfg, seed=1234)
+    q0 = s.scalars(0)["QxT"]
+    s.step(1)
+    say("gpu step done; oracle step (about 30 s)")
+    t = time.time()
+    o.step(1)
+    say("oracle step", time.time() - t, "s")
+    for nm in FIELDS:
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("large", nm, e)
+        assert e <= 1e-12, (nm, e)
+    h = s.history(0)
+    say("large history", h)
+    # the constant flow rate holds and the projected field is solenoidal
+    assert abs(h["Qx"] - q0) <= 1e-13 * abs(q0)
+    umax = np.abs(s.download(0, "u1")).max()
+    assert h["err"] <= 1e-12 * umax
+    s.close()
+
+
+def test_closure_coarse_32_step_matches_oracle():
+    from oracle import closure_oracle as co
+    from rnl_multiblock_nofft_amd import closure
+    from test_closure import rand_params
+
+    cfg = configs.smooth_32()[0]
+    say("closure coarse 32x32x153")
+    s, o, _ = pair(cfg, seed=77)
+    o_plain = oracle_for(cfg)
+    for nm in FIELDS:
+        o_plain.set(nm, o.get(nm))
+    o_plain.init_flowrate()
+    P = rand_params(13).astype(np.float32)
+    norm = np.array([0.3, 0.0, 0.0, 2.0, 20.0, 20.0, 0.5, 0.2, 0.3], np.float32)
+    closure.set_network(s, P, norm)
+    closure.enable(s, 0)
+    s.step(1)
+    for kRK in (1, 2, 3):
+        co.substep_with_closure(o, kRK, P, norm, emulate=True)
+    o_plain.step(1)
+    for nm in FIELDS:
+        g, r, r0 = s.download(0, nm), o.get(nm), o_plain.get(nm)
+        effect = np.linalg.norm(r - r0)
+        e = np.linalg.norm(g - r) / effect
+        say("closure 32", nm, "effect", effect / np.linalg.norm(r0), "gpu - oracle / effect", e)
+        assert effect > 1e-8 * np.linalg.norm(r0)
+        assert e < 1e-2, nm
+    s.close()
+
+
+@pytest.mark.parametrize("workload", ["ridges_64x64x153", "smooth_32x32x153"])
+def test_fp64_runs_are_bitwise_deterministic(workload):
+    """The FP64 path has no atomics and a fixed reduction order: two runs of
+    the same input give the same bits (three graph-replayed steps)."""
+    cfg = configs.WORKLOADS[workload]()[0]
+    out = []
+    for rep in range(2):
+        s = Solver([cfg])
+        f = init_fields(cfg, s.grid(0), 4321)
+        for nm in FIELDS:
+            s.upload(0, nm, f[nm])
+        s.init_flowrate()
+        s.step(3)
+        out.append({nm: s.download(0, nm) for nm in FIELDS})
+        out[-1]["scal"] = np.array(list(s.history(0).values()), float)
+        s.close()
+    for k in out[0]:
+        say("determinism", workload, k, np.array_equal(out[0][k], out[1][k]))
+        assert np.array_equal(out[0][k], out[1][k]), k
