@@ -151,6 +151,17 @@ int lh_four_to_phys_planes(int N1, int Ngal1, int Ngal2, double *d_planes,
 int lh_phys_to_four_planes(int N1, int Ngal1, int Ngal2, double *d_planes,
                            int nplanes, void *stream);
 
+/* The two plane transforms above without any device allocation: dir = +1
+ * is lh_four_to_phys_planes, dir = -1 lh_phys_to_four_planes, same
+ * arguments and results.  No counterpart in the reference; it is for callers
+ * that cannot allocate (code under HIP graph capture, tight loops, hosts that
+ * forbid hipMalloc).  The twiddle tables live in a fixed 48 KiB device array
+ * of the library, written on `stream` at every call: Ngal1, Ngal2 <= 1536
+ * (LH_EINVAL beyond).  Calls that overlap on one GPU share that array and
+ * race. */
+int lh_plane_transform_noalloc(int dir, int N1, int Ngal1, int Ngal2, double *d_planes, int nplanes,
+                               void *stream);
+
 /* Stand-alone batched tridiagonal solves on device data in the reference
  * layout x(jlo:jhi, ncol) (re/im interleaved), replacing LUsolU_W / LUsolV
  * (tridLU_3D.f90:55,135) for one RK3 substep of channel `ch`: solves

@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1143,6 +1144,71 @@ int lh_phys_to_four_planes(int N1, int Ngal1, int Ngal2, double *d, int np, void
   rc = plane_tw(Ngal1, Ngal2, &tz, &tx);
   if (rc) return rc;
   HIPCHK(launch_plane_p2f(N1, Ngal1, Ngal2, d, np, tz, tx, (hipStream_t)stream));
+  return LH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The plane transforms without allocation (no counterpart in the reference;
+// for callers that cannot allocate device memory: code under HIP graph
+// capture, tight host loops).  lh_four_to_phys_planes / lh_phys_to_four_planes
+// allocate their twiddle tables on the first use of a length; here the two
+// tables live in `scratch`, a file-scope device array (48 KiB, held by every
+// GPU the library runs on until the process ends), and a small kernel writes
+// them on the caller's stream at every call, before the transforms that read
+// them.  Calls that overlap on one GPU share the array and race.
+#define LH_NOALLOC_MAXN 1536
+#define LH_NOALLOC_MAXDEV 64
+static __device__ __align__(256) double2 scratch[2 * LH_NOALLOC_MAXN];
+
+// tw[m] = exp(-2 pi i m / n), table t of the launch (z: n2 at 0, x: n1 at
+// LH_NOALLOC_MAXN); sincospi keeps the quarter turns exact
+__global__ void __launch_bounds__(256) k_noalloc_twiddles(double2 *tw, int n2, int n1) {
+  const int n = blockIdx.y == 0 ? n2 : n1;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double sn, cs;
+  sincospi(2.0 * (double)m / (double)n, &sn, &cs);
+  tw[blockIdx.y * LH_NOALLOC_MAXN + m] = make_double2(cs, -sn);
+}
+
+// the device address of `scratch` on the current device, looked up once per
+// device (host code must never pass the array's name to a kernel)
+static int noalloc_scratch(double2 **out) {
+  static double2 *ptr[LH_NOALLOC_MAXDEV];
+  static std::mutex mu;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= LH_NOALLOC_MAXDEV) return set_err(LH_EINVAL, "lh_plane_transform_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> g(mu);
+  if (!ptr[dev]) {
+    double2 *p = nullptr;
+    hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(LH_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    ptr[dev] = p;
+  }
+  *out = ptr[dev];
+  return LH_OK;
+}
+
+int lh_plane_transform_noalloc(int dir, int N1, int Ngal1, int Ngal2, double *d, int np, void *stream) {
+  if (dir != 1 && dir != -1) return set_err(LH_EINVAL, "lh_plane_transform_noalloc: dir must be +1 or -1");
+  int rc = plane_args(N1, Ngal1, Ngal2, d, np);
+  if (rc) return rc;
+  if (Ngal1 > LH_NOALLOC_MAXN || Ngal2 > LH_NOALLOC_MAXN)
+    return set_err(LH_EINVAL, "lh_plane_transform_noalloc: Ngal1, Ngal2 at most %d (got %d, %d)", LH_NOALLOC_MAXN,
+                   Ngal1, Ngal2);
+  double2 *tw = nullptr;
+  rc = noalloc_scratch(&tw);
+  if (rc) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  const int nmax = Ngal1 > Ngal2 ? Ngal1 : Ngal2;
+  hipLaunchKernelGGL(k_noalloc_twiddles, dim3((nmax + 255) / 256, 2), dim3(256), 0, s, tw, Ngal2, Ngal1);
+  HIPCHK(hipGetLastError());
+  if (dir > 0)
+    HIPCHK(launch_plane_f2p(N1, Ngal1, Ngal2, d, np, tw, tw + LH_NOALLOC_MAXN, s));
+  else
+    HIPCHK(launch_plane_p2f(N1, Ngal1, Ngal2, d, np, tw, tw + LH_NOALLOC_MAXN, s));
   return LH_OK;
 }
 
