@@ -151,6 +151,20 @@ int lh_four_to_phys_planes(int N1, int Ngal1, int Ngal2, double *d_planes,
 int lh_phys_to_four_planes(int N1, int Ngal1, int Ngal2, double *d_planes,
                            int nplanes, void *stream);
 
+/* FFTPACK's vector transforms on device data, with the reference's
+ * argument meaning: cft (cft_buff.f:1-65) transforms m complex vectors of
+ * length n whose element j of vector k sits at d[k isa + j ise] (real) and
+ * the next double (imaginary); rft (rft_buff.f:24-90) m real vectors of
+ * length n at d + k isa (isa >= n + 2), packed on the Fourier side as
+ * cc(k) = d[2k] + i d[2k+1], k = 0..n/2.  iopt < 0: forward, scaled by 1/n
+ * (rft zeroes the imaginary parts of cc(0) and cc(n/2)); iopt >= 0:
+ * inverse, unscaled (rft ignores those two imaginary parts and leaves
+ * d[n], d[n+1] as they were).  n even, 2^a 3^b, at most 2048; the vectors
+ * must not share elements.  n is what the reference keeps in aux(1) (cfti,
+ * rfti). */
+int lh_cft(double *d_c, int ise, int isa, int m, int iopt, int n, void *stream);
+int lh_rft(double *d_c, int isa, int m, int iopt, int n, void *stream);
+
 /* The two plane transforms above without any device allocation: dir = +1
  * is lh_four_to_phys_planes, dir = -1 lh_phys_to_four_planes, same
  * arguments and results.  No counterpart in the reference; it is for callers

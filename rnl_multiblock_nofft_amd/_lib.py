@@ -33,6 +33,7 @@ EXPORTS = ["lh_last_error", "lh_ctx_create", "lh_ctx_destroy", "lh_set_grid", "l
            "lh_field_upload", "lh_field_download", "lh_get_scalars", "lh_set_scalars",
            "lh_init_flowrate", "lh_substep", "lh_step", "lh_sync", "lh_time_kernels", "lh_run_phase",
            "lh_four_to_phys_planes", "lh_phys_to_four_planes", "lh_plane_transform_noalloc",
+           "lh_cft", "lh_rft",
            "lh_lusol_uw", "lh_lusol_v",
            "lh_dev_alloc", "lh_dev_free", "lh_copy_to_device", "lh_copy_to_host",
            "lh_set_tuning", "lh_get_tuning", "lh_autotune", "lh_set_texture",
@@ -94,6 +95,8 @@ def load():
     L.lh_four_to_phys_planes.argtypes = [ip, ip, ip, vp, ip, vp]
     L.lh_phys_to_four_planes.argtypes = [ip, ip, ip, vp, ip, vp]
     L.lh_plane_transform_noalloc.argtypes = [ip, ip, ip, ip, vp, ip, vp]
+    L.lh_cft.argtypes = [vp, ip, ip, ip, ip, ip, vp]
+    L.lh_rft.argtypes = [vp, ip, ip, ip, ip, vp]
     L.lh_lusol_uw.argtypes = [vp, ip, ip, vp, vp, vp, vp]
     L.lh_lusol_v.argtypes = [vp, ip, ip, vp, vp]
     L.lh_dev_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]

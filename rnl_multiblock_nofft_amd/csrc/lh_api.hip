@@ -1147,6 +1147,46 @@ int lh_phys_to_four_planes(int N1, int Ngal1, int Ngal2, double *d, int np, void
   return LH_OK;
 }
 
+// FFTPACK's cft / rft (cft_buff.f:1-65, rft_buff.f:24-90) on device data.
+// The vectors are transformed concurrently, so they must not share
+// elements (the reference runs them one after another).
+static int fft_vec_args(const char *who, int n, int ise, int isa, int m, const double *d, int elem) {
+  if (!d || m < 1 || n < 2 || (n & 1) || !valid_len(n) || n > 2048 || ise < elem || isa < 1)
+    return set_err(LH_EINVAL, "%s: bad arguments (n %d: even, 2^a 3^b, <= 2048)", who, n);
+  const size_t span = (size_t)(m - 1) * isa + (size_t)(n - 1) * ise + elem;
+  if (span > ((size_t)1 << 32)) return set_err(LH_EINVAL, "%s: %zu doubles is too large", who, span);
+  std::vector<unsigned char> used(span, 0);
+  for (int k = 0; k < m; k++)
+    for (int j = 0; j < n; j++)
+      for (int e = 0; e < elem; e++) {
+        unsigned char &u = used[(size_t)k * isa + (size_t)j * ise + e];
+        if (u) return set_err(LH_EINVAL, "%s: vectors overlap (ise %d, isa %d, m %d)", who, ise, isa, m);
+        u = 1;
+      }
+  return LH_OK;
+}
+
+int lh_cft(double *d, int ise, int isa, int m, int iopt, int n, void *stream) {
+  int rc = fft_vec_args("lh_cft", n, ise, isa, m, d, 2);
+  if (rc) return rc;
+  double2 *tw, *tw2;
+  rc = plane_tw(n, n, &tw, &tw2);
+  if (rc) return rc;
+  HIPCHK(launch_cft(n, ise, isa, m, iopt, d, tw, (hipStream_t)stream));
+  return LH_OK;
+}
+
+int lh_rft(double *d, int isa, int m, int iopt, int n, void *stream) {
+  if (isa < n + 2) return set_err(LH_EINVAL, "lh_rft: isa %d below n + 2 = %d", isa, n + 2);
+  int rc = fft_vec_args("lh_rft", n, 1, isa, m, d, 1);
+  if (rc) return rc;
+  double2 *tw, *tw2;
+  rc = plane_tw(n, n, &tw, &tw2);
+  if (rc) return rc;
+  HIPCHK(launch_rft(n, isa, m, iopt, d, tw, (hipStream_t)stream));
+  return LH_OK;
+}
+
 // ---------------------------------------------------------------------------
 // The plane transforms without allocation (no counterpart in the reference;
 // for callers that cannot allocate device memory: code under HIP graph

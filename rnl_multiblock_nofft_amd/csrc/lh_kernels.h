@@ -63,6 +63,8 @@ hipError_t launch_plane_f2p(int N1, int Ng1, int Ng2, double *planes, int nplane
                             const double2 *twx, hipStream_t s, bool zero_nyq = true);
 hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
                             const double2 *twx, hipStream_t s, bool zero_nyq = true);
+hipError_t launch_cft(int n, int ise, int isa, int m, int iopt, double *c, const double2 *tw, hipStream_t s);
+hipError_t launch_rft(int n, int isa, int m, int iopt, double *c, const double2 *tw, hipStream_t s);
 // textured walls: channels of one (N1, N2) shape share a run of work planes
 struct TexGroup {
   int N1, N2, nplanes;

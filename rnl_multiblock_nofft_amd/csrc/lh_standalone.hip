@@ -19,13 +19,16 @@ static __host__ __device__ int plane_tile(int n) {
   return t < 1 ? 1 : t;
 }
 
-// z transform of the retained x-modes i = 0..N1/2 of every plane.
-// dir = +1: inverse (zero row Ngal2/2 first); dir = -1: forward, scaled, then
-// zero row Ngal2/2 of the whole plane.
-__global__ void __launch_bounds__(256) k_plane_z(double *planes, int N1, int Ng1, int Ng2, int dir,
-                                                 const double2 *twz, int zero_nyq) {
-  const int igal = Ng1 + 2, nv = N1 / 2 + 1;
-  double *P = planes + (size_t)blockIdx.y * igal * Ng2;
+// Complex transforms of length Ng2 of nv vectors (cft, cft_buff.f:1-65):
+// element z of vector i at P[z ise + i isa] (re) and the next double (im),
+// P the plane blockIdx.y, pstride doubles apart.  The plane transforms call
+// it for the retained x-modes i = 0..N1/2 (ise = Ngal1+2, isa = 2).
+// dir = +1: inverse (with zero_nyq: zero row Ngal2/2 first); dir = -1:
+// forward, scaled, then (zero_nyq) zero row Ngal2/2 of the whole plane.
+__global__ void __launch_bounds__(256) k_plane_z(double *planes, int nv, int Ng1, int Ng2, int ise, int isa,
+                                                 size_t pstride, int dir, const double2 *twz, int zero_nyq) {
+  const int igal = Ng1 + 2;
+  double *P = planes + (size_t)blockIdx.y * pstride;
   const int TI = plane_tile(Ng2);
   const int i0 = blockIdx.x * TI;
   extern __shared__ double2 smem_d[];
@@ -35,7 +38,8 @@ __global__ void __launch_bounds__(256) k_plane_z(double *planes, int N1, int Ng1
     stage_twiddles<double2>(TW, twz, Ng2);
     for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x) {
       int z = t / ni, ii = t - z * ni;
-      double2 v = make_double2(P[(size_t)z * igal + 2 * (i0 + ii)], P[(size_t)z * igal + 2 * (i0 + ii) + 1]);
+      const size_t at = (size_t)z * ise + (size_t)(i0 + ii) * isa;
+      double2 v = make_double2(P[at], P[at + 1]);
       if (zero_nyq && dir > 0 && z == Ng2 / 2) v = make_double2(0, 0);
       A[ii * Ng2 + z] = v;
     }
@@ -46,8 +50,9 @@ __global__ void __launch_bounds__(256) k_plane_z(double *planes, int N1, int Ng1
       int z = t / ni, ii = t - z * ni;
       double2 v = R[ii * Ng2 + z];
       if (zero_nyq && dir < 0 && z == Ng2 / 2) v = make_double2(0, 0);
-      P[(size_t)z * igal + 2 * (i0 + ii)] = v.x * s;
-      P[(size_t)z * igal + 2 * (i0 + ii) + 1] = v.y * s;
+      const size_t at = (size_t)z * ise + (size_t)(i0 + ii) * isa;
+      P[at] = v.x * s;
+      P[at + 1] = v.y * s;
     }
   }
   // entries not transformed: row Ngal2/2 is zeroed (four_to_phys zeroes it
@@ -58,11 +63,13 @@ __global__ void __launch_bounds__(256) k_plane_z(double *planes, int N1, int Ng1
   }
 }
 
-// x transform (real) of every z-row; two rows share one complex FFT.
-__global__ void __launch_bounds__(256) k_plane_x(double *planes, int Ng1, int Ng2, int dir,
+// Real transforms of length Ng1 of Ng2 vectors (rft, rft_buff.f:24-90):
+// vector z at P + z igal (igal >= Ng1 + 2 doubles apart), P the plane
+// blockIdx.y, pstride doubles apart; two vectors share one complex FFT.
+__global__ void __launch_bounds__(256) k_plane_x(double *planes, int Ng1, int Ng2, int igal, size_t pstride, int dir,
                                                  const double2 *twx) {
-  const int igal = Ng1 + 2, h = Ng1 / 2;
-  double *P = planes + (size_t)blockIdx.y * igal * Ng2;
+  const int h = Ng1 / 2;
+  double *P = planes + (size_t)blockIdx.y * pstride;
   const int TZ = plane_tile(Ng1);  // pairs of rows per block
   const int z0 = blockIdx.x * 2 * TZ;
   if (z0 >= Ng2) return;
@@ -133,21 +140,48 @@ __global__ void __launch_bounds__(256) k_plane_x(double *planes, int Ng1, int Ng
 
 hipError_t launch_plane_f2p(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
                             const double2 *twx, hipStream_t s, bool zero_nyq) {
+  const int igal = Ng1 + 2;
+  const size_t ps = (size_t)igal * Ng2;
   int nv = N1 / 2 + 1, TI = plane_tile(Ng2), TZ = plane_tile(Ng1);
   dim3 gz((nv + TI - 1) / TI, nplanes), gx((Ng2 + 2 * TZ - 1) / (2 * TZ), nplanes);
-  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, N1, Ng1, Ng2, +1,
-                     twz, (int)zero_nyq);
-  hipLaunchKernelGGL(k_plane_x, gx, dim3(256), (2 * TZ + 1) * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, +1, twx);
+  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, nv, Ng1, Ng2, igal,
+                     2, ps, +1, twz, (int)zero_nyq);
+  hipLaunchKernelGGL(k_plane_x, gx, dim3(256), (2 * TZ + 1) * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, igal, ps,
+                     +1, twx);
   return hipGetLastError();
 }
 
 hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
                             const double2 *twx, hipStream_t s, bool zero_nyq) {
+  const int igal = Ng1 + 2;
+  const size_t ps = (size_t)igal * Ng2;
   int nv = N1 / 2 + 1, TI = plane_tile(Ng2), TZ = plane_tile(Ng1);
   dim3 gz((nv + TI - 1) / TI, nplanes), gx((Ng2 + 2 * TZ - 1) / (2 * TZ), nplanes);
-  hipLaunchKernelGGL(k_plane_x, gx, dim3(256), (2 * TZ + 1) * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, -1, twx);
-  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, N1, Ng1, Ng2, -1,
-                     twz, (int)zero_nyq);
+  hipLaunchKernelGGL(k_plane_x, gx, dim3(256), (2 * TZ + 1) * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, igal, ps,
+                     -1, twx);
+  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, nv, Ng1, Ng2, igal,
+                     2, ps, -1, twz, (int)zero_nyq);
+  return hipGetLastError();
+}
+
+// cft(c, ise, isa, m, iopt, aux) (cft_buff.f:1-65) on device data: m complex
+// vectors of length n, strides in doubles; iopt < 0 forward (scaled 1/n).
+hipError_t launch_cft(int n, int ise, int isa, int m, int iopt, double *c, const double2 *tw, hipStream_t s) {
+  const int TI = plane_tile(n);
+  hipLaunchKernelGGL(k_plane_z, dim3((m + TI - 1) / TI, 1), dim3(256), (2 * TI + 1) * n * sizeof(double2), s, c, m,
+                     0, n, ise, isa, (size_t)0, iopt < 0 ? -1 : +1, tw, 0);
+  return hipGetLastError();
+}
+
+// rft(c, isa, m, iopt, aux) (rft_buff.f:24-90) on device data: m real
+// vectors of length n, isa >= n + 2 doubles apart, packed as cc(k) =
+// c(2k+1) + i c(2k+2), k = 0..n/2; iopt < 0 forward (scaled 1/n, imaginary
+// parts of cc(0) and cc(n/2) zeroed), else inverse (they are ignored and
+// c(n+1), c(n+2) keep their values).
+hipError_t launch_rft(int n, int isa, int m, int iopt, double *c, const double2 *tw, hipStream_t s) {
+  const int TZ = plane_tile(n);
+  hipLaunchKernelGGL(k_plane_x, dim3((m + 2 * TZ - 1) / (2 * TZ), 1), dim3(256), (2 * TZ + 1) * n * sizeof(double2), s,
+                     c, n, m, isa, (size_t)0, iopt < 0 ? -1 : +1, tw);
   return hipGetLastError();
 }
 

@@ -29,6 +29,10 @@ module lh_host
   integer, save :: nband = 0
   integer, allocatable, save :: N(:,:), Ngal(:,:), nyb(:)
   integer, save :: kRK = 1
+  ! the device channel (0-based) that the reference-signature routines of
+  ! lh_dropin.f90 act on: the reference's bands of one uniform-resolution
+  ! channel all map to it
+  integer, save :: dropin_chan = 0
 
   ! device staging for the plane and solve shims
   type(c_ptr), save :: dstage = c_null_ptr

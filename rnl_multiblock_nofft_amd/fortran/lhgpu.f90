@@ -153,6 +153,25 @@ module lhgpu
       type(c_ptr), value :: d_planes, stream
       integer(c_int) :: rc
     end function
+    function lh_plane_transform_noalloc(dir, N1, Ngal1, Ngal2, d_planes, nplanes, stream) &
+        bind(C, name='lh_plane_transform_noalloc') result(rc)
+      import :: c_ptr, c_int
+      integer(c_int), value :: dir, N1, Ngal1, Ngal2, nplanes
+      type(c_ptr), value :: d_planes, stream
+      integer(c_int) :: rc
+    end function
+    function lh_cft(d_c, ise, isa, m, iopt, n, stream) bind(C, name='lh_cft') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: d_c, stream
+      integer(c_int), value :: ise, isa, m, iopt, n
+      integer(c_int) :: rc
+    end function
+    function lh_rft(d_c, isa, m, iopt, n, stream) bind(C, name='lh_rft') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: d_c, stream
+      integer(c_int), value :: isa, m, iopt, n
+      integer(c_int) :: rc
+    end function
     function lh_lusol_uw(ctx, ch, kRK, d_rhsu, d_rhsw, d_u, d_w) bind(C, name='lh_lusol_uw') result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx, d_rhsu, d_rhsw, d_u, d_w
