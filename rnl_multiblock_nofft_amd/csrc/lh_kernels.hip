@@ -341,48 +341,51 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   const C *hz = Inter<C>::hz(v) + (size_t)slot * 3 * Ng2 * nxp;
 
   // Hermitian extension of the three half spectra, packed two per sequence:
-  // seq 2zz = f0 + i f2, seq 2zz+1 = f1
+  // seq 2zz = f0 + i f2, seq 2zz+1 = f1.  Each retained mode m <= N1/2 is
+  // read once and written at m and, conjugated, at its image Ng1 - m; the
+  // modes between are zero.
   {
-    // LU items per thread per batch: every load of a batch is in flight
-    // before the first LDS store (one item measured fastest here)
-    constexpr int LU = 1;
-    const int total = nz * Ng1;
-    Walk2 wh(threadIdx.x, blockDim.x, Ng1);  // t = zz Ng1 + m
+    const int nzero = Ng1 - 2 * h1 - 1;         // m = h1+1 .. Ng1-h1-1
+    Walk2 wz(threadIdx.x, blockDim.x, nzero);   // t = seq nzero + q
+    for (int t = threadIdx.x; t < 2 * nz * nzero; t += blockDim.x, wz.next())
+      A[wz.hi * S + h1 + 1 + wz.lo] = CT<C>::mk(0, 0);
+    // LU items per thread per batch, every load of a batch in flight before
+    // the first LDS store
+    constexpr int LU = 2;
+    const int nm = h1 + 1, total = nz * nm;
+    const size_t fs = (size_t)Ng2 * nxp;        // one field of the plane
+    Walk2 wh(threadIdx.x, blockDim.x, nm);      // t = zz nm + m
     for (int t0 = threadIdx.x; t0 < total; t0 += LU * blockDim.x) {
       C F0[LU], F1[LU], F2[LU];
-      int at[LU];
+      int zl[LU], ml[LU];
 #pragma unroll
       for (int u = 0; u < LU; u++, wh.next()) {
-        const int zz = wh.hi, m = wh.lo;
-        const int z = z0 + zz;
-        at[u] = -1;
-        F0[u] = F1[u] = F2[u] = CT<C>::mk(0, 0);
+        zl[u] = -1;
         if (t0 + u * (int)blockDim.x >= total) continue;
-        at[u] = (2 * zz) * S + m;
-        if (m <= h1) {
-          F0[u] = hz[((size_t)0 * Ng2 + z) * nxp + m];
-          F1[u] = hz[((size_t)1 * Ng2 + z) * nxp + m];
-          F2[u] = hz[((size_t)2 * Ng2 + z) * nxp + m];
-          if (m == 0) {  // rft ignores imag(cc(0)) on the inverse transform
-            F0[u].y = 0;
-            F1[u].y = 0;
-            F2[u].y = 0;
-          }
-        } else if (m >= Ng1 - h1) {
-          const int mm = Ng1 - m;
-          F0[u] = hz[((size_t)0 * Ng2 + z) * nxp + mm];
-          F1[u] = hz[((size_t)1 * Ng2 + z) * nxp + mm];
-          F2[u] = hz[((size_t)2 * Ng2 + z) * nxp + mm];
-          F0[u].y = -F0[u].y;
-          F1[u].y = -F1[u].y;
-          F2[u].y = -F2[u].y;
-        }
+        zl[u] = wh.hi;
+        ml[u] = wh.lo;
+        const C *r = hz + (size_t)(z0 + wh.hi) * nxp + wh.lo;
+        F0[u] = r[0];
+        F1[u] = r[fs];
+        F2[u] = r[2 * fs];
       }
 #pragma unroll
       for (int u = 0; u < LU; u++) {
-        if (at[u] < 0) continue;
-        A[at[u]] = CT<C>::mk(F0[u].x - F2[u].y, F0[u].y + F2[u].x);
-        A[at[u] + S] = F1[u];
+        if (zl[u] < 0) continue;
+        const int m = ml[u];
+        C a = F0[u], b = F1[u], c = F2[u];
+        if (m == 0) {  // rft ignores imag(cc(0)) on the inverse transform
+          a.y = 0;
+          b.y = 0;
+          c.y = 0;
+        }
+        C *s0 = A + (2 * zl[u]) * S, *s1 = s0 + S;
+        s0[m] = CT<C>::mk(a.x - c.y, a.y + c.x);
+        s1[m] = b;
+        if (m > 0) {
+          s0[Ng1 - m] = CT<C>::mk(a.x + c.y, c.x - a.y);
+          s1[Ng1 - m] = CT<C>::mk(b.x, -b.y);
+        }
       }
     }
   }
