@@ -230,9 +230,10 @@ __host__ __device__ __forceinline__ void stage_item(const C *xs, C *ys, int jj, 
 }
 
 // The radix of the first stage for a sequence of length n; the later stages
-// follow the same rule on what is left (n / radix).  Two stages up to 256
-// points (96 = 12 x 8, 192 = 12 x 16), three at 384 = 8 x 6 x 8 and 768;
-// lengths outside the table take radices 3, then 4, then 2.
+// follow the same rule on what is left (n / radix), except where
+// stage_radix says otherwise.  Two stages up to 256 points (96 = 12 x 8,
+// 192 = 12 x 16), three at 384 = 8 x 8 x 6 and 768; lengths outside the
+// table take radices 3, then 4, then 2.
 __host__ __device__ constexpr int plan_radix(int n) {
   switch (n) {
     case 2: return 2;
@@ -256,6 +257,13 @@ __host__ __device__ constexpr int plan_radix(int n) {
   }
 }
 
+// The radix of the next stage of an n-point transform with m points left.
+// 384 ends in radix 6 (8 x 8 x 6, not 8 x 6 x 8): the last stage is the
+// one the x-product stage holds in registers for two sequences across the
+// products, and six points per sequence instead of eight keep that kernel
+// at four waves per SIMD.
+__host__ __device__ constexpr int stage_radix(int n, int m) { return (n == 384 && m == 48) ? 8 : plan_radix(m); }
+
 // In-LDS batched FFT of any length 3^a 2^b.  nb sequences of length n;
 // element e of sequence b is at x[b*sstr + e].  y is scratch of the same
 // extent, tw the staged twiddle table.  Returns the buffer holding the
@@ -264,7 +272,7 @@ template <typename C, int SIGN>
 __device__ __forceinline__ C *fft_lds_rt(C *x, C *y, int n, int nb, int sstr, const C *tw) {
   int Ns = 1;
   for (int m = n; m > 1;) {
-    const int Rr = plan_radix(m);
+    const int Rr = stage_radix(n, m);
     if (Rr == 0) break;  // lengths other than 2^a 3^b are rejected on the host
     const int nbf = n / Rr;
     const int span = n / (Ns * Rr);  // twiddle exponent unit
@@ -299,7 +307,7 @@ __device__ __forceinline__ C *fft_ct_run(C *x, C *y, int nb, int sstr, const C *
   if constexpr (M == 1) {
     return x;
   } else {
-    constexpr int R = plan_radix(M);
+    constexpr int R = stage_radix(N, M);
     static_assert(R > 1 && M % R == 0, "no radix plan");
     constexpr int nbf = N / R;
     constexpr int span = N / (Ns * R);
@@ -342,7 +350,7 @@ __device__ __forceinline__ C *fft_bf_run(C *x, C *y, int nb, int sstr, const C *
   if constexpr (M == 1) {
     return x;
   } else {
-    constexpr int R = plan_radix(M);
+    constexpr int R = stage_radix(N, M);
     static_assert(R > 1 && M % R == 0, "no radix plan");
     constexpr int nbf = N / R;
     constexpr int span = N / (Ns * R);
@@ -359,7 +367,7 @@ template <typename C, int SIGN>
 __device__ __forceinline__ C *fft_bf_rt(C *x, C *y, int n, int nb, int sstr, const C *tw) {
   int Ns = 1;
   for (int m = n; m > 1;) {
-    const int Rr = plan_radix(m);
+    const int Rr = stage_radix(n, m);
     if (Rr == 0) break;  // lengths other than 2^a 3^b are rejected on the host
     const int nbf = n / Rr;
     const int span = n / (Ns * Rr);
@@ -407,15 +415,13 @@ __device__ __forceinline__ C *fft_bf_s(C *x, C *y, int n, int nb, int sstr, cons
 // The reversed plan (plan_at(n, L-1-s)) transforms the same lengths.
 __host__ __device__ constexpr int plan_len(int n) {
   int k = 0;
-  while (n > 1) {
-    n /= plan_radix(n);
-    k++;
-  }
+  for (int m = n; m > 1; k++) m /= stage_radix(n, m);
   return k;
 }
 __host__ __device__ constexpr int plan_at(int n, int s) {
-  while (s-- > 0) n /= plan_radix(n);
-  return plan_radix(n);
+  int m = n;
+  while (s-- > 0) m /= stage_radix(n, m);
+  return stage_radix(n, m);
 }
 __host__ __device__ constexpr int plan_ns(int n, int s) {
   int p = 1;
@@ -496,7 +502,7 @@ __device__ __forceinline__ void bf_stages_ip(C *x, int nb, int sstr, const C *tw
 __host__ __device__ constexpr int plan_max_items(int n) {
   int m = n, it = 0;
   while (m > 1) {
-    const int r = plan_radix(m);
+    const int r = stage_radix(n, m);
     it = n / r > it ? n / r : it;
     m /= r;
   }

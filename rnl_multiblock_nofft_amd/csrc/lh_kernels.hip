@@ -96,7 +96,10 @@ __device__ __forceinline__ void zinv_tail(C *A, C *B, const C *TW, int S, int ni
 
 // ------------------------------------------------------------------------
 // K1: interpolation + inverse z transform
-template <typename C>
+// NL: the z length Ngal2 of every channel of the launch when the host
+// specialises it (the stage then holds only that length's path, fewer
+// registers), 0 for a launch whose channels differ
+template <typename C, int NL>
 __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E, int ntile, int nslot, int nzinv,
                                               int nghost) {
   if ((int)blockIdx.x < nghost) {
@@ -124,7 +127,7 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
     const int nplv = ch.nslot - ch.nplu;
     slot = slot < 2 * nplv ? ((slot & 1) ? ch.nplu + (slot >> 1) : (slot >> 1)) : slot - nplv;
   }
-  const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
+  const int Ng2 = NL ? NL : ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
   const int TI = tile_of8(E, Ng2, nxh);
   const int i0 = tile * TI;
   if (i0 >= nxh) return;
@@ -315,7 +318,10 @@ __device__ __forceinline__ C *xprod_fused_ip(C *A, const C *TW, int S, int nz, b
 
 // ------------------------------------------------------------------------
 // K2: inverse x transform, products, forward x transform
-template <typename C, bool CFL, bool IP>
+// NL: the x length Ngal1 of every channel of the launch when the host
+// specialises it (with IP the host also checks that the stages run in
+// place), 0 for a launch whose channels differ
+template <typename C, bool CFL, bool IP, int NL>
 __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, int E) {
   typedef typename CT<C>::R R_t;
   const Chan ch = chans[blockIdx.z];
@@ -323,7 +329,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   const View v(arena, ch);
   const int slot = blockIdx.y;
   if (slot >= ch.nslot) return;
-  const int Ng1 = ch.Ng1, Ng2 = ch.Ng2, nxp = ch.nxp, h1 = ch.N1 / 2;
+  const int Ng1 = NL ? NL : ch.Ng1, Ng2 = ch.Ng2, nxp = ch.nxp, h1 = ch.N1 / 2;
   const int TZ = tile_of(E, Ng1, Ng2);
   const int z0 = blockIdx.x * TZ;
   if (z0 >= Ng2) return;
@@ -332,7 +338,7 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   // 2 nz sequences of Ng1 points, S apart (padded: conflict-free stages);
   // one buffer when the stages can run in place
   const int S = seq_stride(Ng1, 2 * TZ, sizeof(C));
-  const bool inplace = IP && xprod_inplace(Ng1, TZ, blockDim.x);
+  const bool inplace = IP && (NL ? true : xprod_inplace(Ng1, TZ, blockDim.x));
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + 2 * TZ * S;
   C *TW = inplace ? B : B + 2 * TZ * S;
@@ -569,7 +575,8 @@ __device__ __forceinline__ void dt_update_block(char *arena, const Chan &ch, int
   }
 }
 
-template <typename C, bool IP>
+// NL as in k_zinv (with IP the host also checks that the stages run in place)
+template <typename C, bool IP, int NL>
 __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, int E, int ntile, int nslot, int nzfwd,
                                               int ndt, int ntiles_xprod, int e_xprod) {
   extern __shared__ double2 smem_d[];
@@ -587,7 +594,7 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
   int tile, slot;
   if (!xcd_tile(ntile, nslot, tile, slot, nzfwd, ndt)) return;
   if (slot >= ch.nslot) return;
-  const int Ng2 = ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
+  const int Ng2 = NL ? NL : ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
   const int TI = tile_of8(E, Ng2, nxh);
   const int i0 = tile * TI;
   if (i0 >= nxh) return;
@@ -598,7 +605,7 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
   // np ni sequences of Ng2 points, S apart (padded: conflict-free stages);
   // one buffer when the stages run in place
   const int S = seq_stride(Ng2, 4 * TI, sizeof(C));
-  const bool inplace = IP && zfwd_inplace(Ng2, TI);
+  const bool inplace = IP && (NL ? true : zfwd_inplace(Ng2, TI));
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + 4 * TI * S;
   C *TW = inplace ? B : B + 4 * TI * S;
@@ -1816,10 +1823,51 @@ void shape_tiles(ShapeMax &m) {
   }
 }
 
+// The length the host specialises a stage for: the length every channel of
+// the launch shares, if it is one of the specialised lengths and (in-place
+// forms) every channel's tile runs in place; else 0 (the multi-length form).
+static int nl_length(const ShapeMax &m, int stage) {
+  int n = -1;
+  for (const ChanDims &d : m.dims) {
+    const int nxh = d.N1 / 2 + 1;
+    int len = stage == 1 ? d.Ng1 : d.Ng2;
+    if (stage == 1 && m.xprod_ip && !xprod_inplace(len, tile_of(m.e_xprod, d.Ng1, d.Ng2), m.xprod_threads)) len = 0;
+    if (stage == 2 && m.zfwd_ip && !zfwd_inplace(len, tile_of8(m.e_zfwd, d.Ng2, nxh))) len = 0;
+    if (n >= 0 && len != n) return 0;
+    n = len;
+  }
+  return (n == 48 || n == 96 || n == 384) ? n : 0;
+}
+
+template <typename T, int NL>
+static void zinv_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans, const ShapeMax &m,
+                        int nz, int extra) {
+  hipLaunchKernelGGL((k_zinv<T, NL>), g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz, extra);
+}
+template <typename T, bool F, bool I, int NL>
+static void xprod_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans,
+                         const ShapeMax &m) {
+  hipLaunchKernelGGL((k_xprod<T, F, I, NL>), g, b, lds, s, arena, d_chans, m.e_xprod);
+}
+template <typename T, bool I, int NL>
+static void zfwd_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans, const ShapeMax &m,
+                        int nz, int dtblk) {
+  hipLaunchKernelGGL((k_zfwd<T, I, NL>), g, b, lds, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot, nz,
+                     dtblk ? 1 : 0, m.xprod_tiles, m.e_xprod);
+}
+#define LH_BY_LEN(nl, call) \
+  switch (nl) {             \
+    case 48: call(48); break;   \
+    case 96: call(96); break;   \
+    case 384: call(384); break; \
+    default: call(0); break;    \
+  }
+
 hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, int stage, hipStream_t s,
                            int kRK, int ghost, int dtblk) {
   size_t es = mixed ? sizeof(float2) : sizeof(double2);
   dim3 b(256);
+  const int nl = nl_length(m, stage);
   if (stage == 0) {
     const int nz = (int)xcd_grid(m.zinv_tiles, m.nslot);
     int extra = 0, n2 = 0;
@@ -1830,46 +1878,55 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
       }
     dim3 g(nz + extra, 1, m.nchan * 3);
     const size_t lds = std::max(m.lds_zinv * es, ghost ? tex_1d_lds(n2) : (size_t)0);
-    if (mixed)
-      hipLaunchKernelGGL(k_zinv<float2>, g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz, extra);
-    else
-      hipLaunchKernelGGL(k_zinv<double2>, g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz, extra);
+#define LH_ZI(n)                                                                      \
+  if (mixed) zinv_launch<float2, n>(g, b, lds, s, arena, d_chans, m, nz, extra);     \
+  else zinv_launch<double2, n>(g, b, lds, s, arena, d_chans, m, nz, extra)
+    LH_BY_LEN(nl, LH_ZI)
+#undef LH_ZI
   } else if (stage == 1) {
     dim3 g(m.xprod_tiles, m.nslot, m.nchan);
     const bool cfl = kRK == 1;
     const bool ip = m.xprod_ip != 0;
     const dim3 bx(m.xprod_threads);
-#define LH_XP(T, F, I) hipLaunchKernelGGL((k_xprod<T, F, I>), g, bx, m.lds_xprod * es, s, arena, d_chans, m.e_xprod)
-    if (mixed) {
-      if (cfl) {
-        if (ip) LH_XP(float2, true, true); else LH_XP(float2, true, false);
-      } else {
-        if (ip) LH_XP(float2, false, true); else LH_XP(float2, false, false);
-      }
-    } else {
-      if (cfl) {
-        if (ip) LH_XP(double2, true, true); else LH_XP(double2, true, false);
-      } else {
-        if (ip) LH_XP(double2, false, true); else LH_XP(double2, false, false);
-      }
-    }
+    const size_t lds = m.lds_xprod * es;
+#define LH_XP(n)                                                                                       \
+  if (mixed) {                                                                                         \
+    if (cfl) {                                                                                         \
+      if (ip) xprod_launch<float2, true, true, n>(g, bx, lds, s, arena, d_chans, m);                   \
+      else xprod_launch<float2, true, false, n>(g, bx, lds, s, arena, d_chans, m);                     \
+    } else {                                                                                           \
+      if (ip) xprod_launch<float2, false, true, n>(g, bx, lds, s, arena, d_chans, m);                  \
+      else xprod_launch<float2, false, false, n>(g, bx, lds, s, arena, d_chans, m);                    \
+    }                                                                                                  \
+  } else {                                                                                             \
+    if (cfl) {                                                                                         \
+      if (ip) xprod_launch<double2, true, true, n>(g, bx, lds, s, arena, d_chans, m);                  \
+      else xprod_launch<double2, true, false, n>(g, bx, lds, s, arena, d_chans, m);                    \
+    } else {                                                                                           \
+      if (ip) xprod_launch<double2, false, true, n>(g, bx, lds, s, arena, d_chans, m);                 \
+      else xprod_launch<double2, false, false, n>(g, bx, lds, s, arena, d_chans, m);                   \
+    }                                                                                                  \
+  }
+    LH_BY_LEN(nl, LH_XP)
 #undef LH_XP
   } else {
     const int nz = (int)xcd_grid(m.zfwd_tiles, m.nslot);
     dim3 g(nz + (dtblk ? 1 : 0), 1, m.nchan);
     const size_t lds = std::max(m.lds_zfwd * es, (size_t)48 * sizeof(double));
-#define LH_ZF(T, I)                                                                                              \
-  hipLaunchKernelGGL((k_zfwd<T, I>), g, b, lds, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot, nz, dtblk ? 1 : 0, \
-                     m.xprod_tiles, m.e_xprod)
-    if (mixed) {
-      if (m.zfwd_ip) LH_ZF(float2, true); else LH_ZF(float2, false);
-    } else {
-      if (m.zfwd_ip) LH_ZF(double2, true); else LH_ZF(double2, false);
-    }
+#define LH_ZF(n)                                                                                  \
+  if (mixed) {                                                                                    \
+    if (m.zfwd_ip) zfwd_launch<float2, true, n>(g, b, lds, s, arena, d_chans, m, nz, dtblk);       \
+    else zfwd_launch<float2, false, n>(g, b, lds, s, arena, d_chans, m, nz, dtblk);               \
+  } else {                                                                                        \
+    if (m.zfwd_ip) zfwd_launch<double2, true, n>(g, b, lds, s, arena, d_chans, m, nz, dtblk);      \
+    else zfwd_launch<double2, false, n>(g, b, lds, s, arena, d_chans, m, nz, dtblk);              \
+  }
+    LH_BY_LEN(nl, LH_ZF)
 #undef LH_ZF
   }
   return hipGetLastError();
 }
+#undef LH_BY_LEN
 
 hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s, int kRK,
                             int ghost, int dtblk) {
