@@ -1836,7 +1836,7 @@ static int nl_length(const ShapeMax &m, int stage) {
     if (n >= 0 && len != n) return 0;
     n = len;
   }
-  return (n == 48 || n == 96 || n == 384) ? n : 0;
+  return (n == 24 || n == 48 || n == 96 || n == 192 || n == 384) ? n : 0;
 }
 
 template <typename T, int NL>
@@ -1857,7 +1857,9 @@ static void zfwd_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, 
 }
 #define LH_BY_LEN(nl, call) \
   switch (nl) {             \
+    case 24: call(24); break;   \
     case 48: call(48); break;   \
+    case 192: call(192); break; \
     case 96: call(96); break;   \
     case 384: call(384); break; \
     default: call(0); break;    \
