@@ -270,6 +270,8 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
  * launch less per substep; the step's results are the same).  Channels with
  * N2 of 32 or 64 only; 0 keeps the separate launch. */
 #define LH_TUNE_GHOST_IN_ZINV 12
+/* LH_TUNE_ZINV_INPLACE (default 0): the same for the z-inverse stage. */
+#define LH_TUNE_ZINV_INPLACE 13
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

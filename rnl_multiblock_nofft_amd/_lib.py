@@ -26,6 +26,7 @@ TUNE_ZFWD_INPLACE = 9
 TUNE_XPROD_THREADS = 10
 TUNE_RHS_FUSED = 11
 TUNE_GHOST_IN_ZINV = 12
+TUNE_ZINV_INPLACE = 13
 PREC_FP64, PREC_MIXED = 0, 1
 
 # every symbol include/lhgpu.h declares

@@ -1431,10 +1431,10 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     drop_graph(x);
     return LH_OK;
   }
-  if (key == LH_TUNE_XPROD_INPLACE || key == LH_TUNE_ZFWD_INPLACE) {
+  if (key == LH_TUNE_XPROD_INPLACE || key == LH_TUNE_ZFWD_INPLACE || key == LH_TUNE_ZINV_INPLACE) {
     if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: in-place switches are 0 or 1");
     ShapeMax m = x->shape;
-    (key == LH_TUNE_XPROD_INPLACE ? m.xprod_ip : m.zfwd_ip) = value;
+    (key == LH_TUNE_XPROD_INPLACE ? m.xprod_ip : key == LH_TUNE_ZFWD_INPLACE ? m.zfwd_ip : m.zinv_ip) = value;
     shape_tiles(m);
     if (stage_lds_bytes(x, m) > 160 * 1024) return set_err(LH_EINVAL, "lh_set_tuning: LDS exceeds 160 KiB");
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1522,6 +1522,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
     *value = x->shape.zfwd_ip;
     return LH_OK;
   }
+  if (key == LH_TUNE_ZINV_INPLACE) {
+    *value = x->shape.zinv_ip;
+    return LH_OK;
+  }
   if (key == LH_TUNE_XPROD_THREADS) {
     *value = x->shape.xprod_threads;
     return LH_OK;
@@ -1585,7 +1589,7 @@ int lh_autotune(lh_ctx *x, int reps) {
   for (int st = 0; st < 3; st++) {
     float best_ms = 1e30f;
     for (int E : pts)
-      for (int v2 = 0; v2 < (st >= 1 ? 2 : 1) * (st == 1 ? 2 : 1); v2++) {
+      for (int v2 = 0; v2 < 2 * (st == 1 ? 2 : 1); v2++) {
         const int ip = v2 & 1, nt = (v2 >> 1) ? 128 : 256;
         ShapeMax m = best;
         *stage_knob(m, keys[st]) = E;
@@ -1594,6 +1598,7 @@ int lh_autotune(lh_ctx *x, int reps) {
           m.xprod_threads = nt;
         }
         if (st == 2) m.zfwd_ip = ip;
+        if (st == 0) m.zinv_ip = ip;
         shape_tiles(m);
         if (stage_lds_bytes(x, m) > 160 * 1024) continue;
         float ms = time_launches(
@@ -1606,6 +1611,7 @@ int lh_autotune(lh_ctx *x, int reps) {
             best.xprod_threads = nt;
           }
           if (st == 2) best.zfwd_ip = ip;
+          if (st == 0) best.zinv_ip = ip;
         }
       }
     shape_tiles(best);

@@ -23,6 +23,7 @@ struct ShapeMax {
   int solve_waves = 1;  // LDS column solve: waves per workgroup (1, 2 or 4)
   int xprod_ip = 0;     // x-product stage: in-place single-buffer stages where they fit
   int zfwd_ip = 0;      // z-forward stage: the same
+  int zinv_ip = 0;      // z-inverse stage: the same
   int xprod_threads = 256;  // x-product stage: threads per workgroup (128 or 256)
   int rhs_fused = 1;   // smooth-walled ensembles: right-hand sides computed inside the column solve
   int ghost_zinv = 1;  // in a captured step, ghost rows of x-independent textures done by k_zinv's extra workgroups

@@ -94,12 +94,37 @@ __device__ __forceinline__ void zinv_tail(C *A, C *B, const C *TW, int S, int ni
   }
 }
 
+// The same with every stage but the last in place in one LDS buffer
+// (bf_stage_ip: at most one item per thread); the last stage only reads it.
+template <typename C, int N>
+__device__ __forceinline__ void zinv_tail_ip(C *A, const C *TW, int S, int ni, int nxp, C *hz) {
+  constexpr int L = plan_len(N), R = plan_at(N, L - 1), NB = N / R;
+  bf_stages_ip<C, N, +1, false, 0, L - 1>(A, ni, S, TW);
+  Walk2 w(threadIdx.x, blockDim.x, ni);  // t = jj ni + ii
+  for (int t = threadIdx.x; t < ni * NB; t += blockDim.x, w.next()) {
+    const int jj = w.hi, ii = w.lo;
+    C o[R];
+    last_item<C, N, +1>(A + ii * S, jj, TW, o);
+#pragma unroll
+    for (int r = 0; r < R; r++) hz[(size_t)(jj + r * NB) * nxp + ii] = o[r];
+  }
+}
+// K1 in place (one LDS buffer) for the compiled lengths when every stage has
+// at most one item per thread: ti plan_max_items(N) <= 256
+__host__ __device__ inline bool zinv_inplace(int N, int ti) {
+  switch (N) {
+    case 24: case 48: case 96: case 384: return ti * plan_max_items(N) <= 256;
+    default: return false;
+  }
+}
+
 // ------------------------------------------------------------------------
 // K1: interpolation + inverse z transform
 // NL: the z length Ngal2 of every channel of the launch when the host
 // specialises it (the stage then holds only that length's path, fewer
-// registers), 0 for a launch whose channels differ
-template <typename C, int NL>
+// registers), 0 for a launch whose channels differ; IP: the stages run in
+// place where zinv_inplace allows (with NL the host has checked that they do)
+template <typename C, bool IP, int NL>
 __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E, int ntile, int nslot, int nzinv,
                                               int nghost) {
   if ((int)blockIdx.x < nghost) {
@@ -135,9 +160,10 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
   extern __shared__ double2 smem_d[];
   // ni sequences of Ng2 points, S apart (padded: conflict-free stages)
   const int S = seq_stride(Ng2, TI, sizeof(C));
+  const bool inplace = IP && (NL ? true : zinv_inplace(Ng2, TI));
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + TI * S;
-  C *TW = B + TI * S;
+  C *TW = inplace ? B : B + TI * S;
   const bool isU = slot < ch.nplu;
   const int j = isU ? slot + 1 : slot - ch.nplu + 1;
 
@@ -202,11 +228,18 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
   }
   __syncthreads();
   C *hz = Inter<C>::hz(v) + ((size_t)slot * 3 + f) * Ng2 * nxp + i0;
-  switch (Ng2) {
-    case 24: zinv_tail<C, 24>(A, B, TW, S, ni, nxp, hz); break;
-    case 48: zinv_tail<C, 48>(A, B, TW, S, ni, nxp, hz); break;
-    case 96: zinv_tail<C, 96>(A, B, TW, S, ni, nxp, hz); break;
-    case 384: zinv_tail<C, 384>(A, B, TW, S, ni, nxp, hz); break;
+  switch (inplace ? Ng2 : -Ng2) {
+#define LH_IP(n) \
+  if constexpr (IP) zinv_tail_ip<C, n>(A, TW, S, ni, nxp, hz)
+    case 24: LH_IP(24); break;
+    case 48: LH_IP(48); break;
+    case 96: LH_IP(96); break;
+    case 384: LH_IP(384); break;
+#undef LH_IP
+    case -24: zinv_tail<C, 24>(A, B, TW, S, ni, nxp, hz); break;
+    case -48: zinv_tail<C, 48>(A, B, TW, S, ni, nxp, hz); break;
+    case -96: zinv_tail<C, 96>(A, B, TW, S, ni, nxp, hz); break;
+    case -384: zinv_tail<C, 384>(A, B, TW, S, ni, nxp, hz); break;
     default: {
       C *R = fft_lds_bf<C>(A, B, Ng2, ni, S, +1, TW);
       Walk2 ws(threadIdx.x, blockDim.x, ni);  // t = z ni + ii
@@ -1796,7 +1829,10 @@ hipError_t launch_u11(char *arena, const Chan *d_chans, const ShapeMax &m, hipSt
 static size_t seq_buf(int n, int nb) {
   return (size_t)nb * std::max(seq_stride(n, nb, sizeof(double2)), seq_stride(n, nb, sizeof(float2)));
 }
-static size_t lds_zinv(int E, int Ng2, int nxh) { return 2 * seq_buf(Ng2, tile_of8(E, Ng2, nxh)) + Ng2; }
+static size_t lds_zinv(int E, int Ng2, int nxh, bool ip) {
+  const int ti = tile_of8(E, Ng2, nxh);
+  return (ip && zinv_inplace(Ng2, ti) ? 1 : 2) * seq_buf(Ng2, ti) + Ng2;
+}
 static size_t lds_xprod(int E, int Ng1, int Ng2, bool ip, int nt) {
   const int tz = tile_of(E, Ng1, Ng2);
   return (ip && xprod_inplace(Ng1, tz, nt) ? 1 : 2) * seq_buf(Ng1, 2 * tz) + Ng1;
@@ -1817,7 +1853,7 @@ void shape_tiles(ShapeMax &m) {
     m.zinv_tiles = max(m.zinv_tiles, (nxh + ti - 1) / ti);
     m.xprod_tiles = max(m.xprod_tiles, (d.Ng2 + tx - 1) / tx);
     m.zfwd_tiles = max(m.zfwd_tiles, (nxh + tf - 1) / tf);
-    m.lds_zinv = std::max(m.lds_zinv, lds_zinv(m.e_zinv, d.Ng2, nxh));
+    m.lds_zinv = std::max(m.lds_zinv, lds_zinv(m.e_zinv, d.Ng2, nxh, m.zinv_ip != 0));
     m.lds_xprod = std::max(m.lds_xprod, lds_xprod(m.e_xprod, d.Ng1, d.Ng2, m.xprod_ip != 0, m.xprod_threads));
     m.lds_zfwd = std::max(m.lds_zfwd, lds_zfwd(m.e_zfwd, d.Ng2, nxh, m.zfwd_ip != 0));
   }
@@ -1832,6 +1868,7 @@ static int nl_length(const ShapeMax &m, int stage) {
     const int nxh = d.N1 / 2 + 1;
     int len = stage == 1 ? d.Ng1 : d.Ng2;
     if (stage == 1 && m.xprod_ip && !xprod_inplace(len, tile_of(m.e_xprod, d.Ng1, d.Ng2), m.xprod_threads)) len = 0;
+    if (stage == 0 && m.zinv_ip && !zinv_inplace(len, tile_of8(m.e_zinv, d.Ng2, nxh))) len = 0;
     if (stage == 2 && m.zfwd_ip && !zfwd_inplace(len, tile_of8(m.e_zfwd, d.Ng2, nxh))) len = 0;
     if (n >= 0 && len != n) return 0;
     n = len;
@@ -1842,7 +1879,11 @@ static int nl_length(const ShapeMax &m, int stage) {
 template <typename T, int NL>
 static void zinv_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans, const ShapeMax &m,
                         int nz, int extra) {
-  hipLaunchKernelGGL((k_zinv<T, NL>), g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz, extra);
+  if (m.zinv_ip)
+    hipLaunchKernelGGL((k_zinv<T, true, NL>), g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz, extra);
+  else
+    hipLaunchKernelGGL((k_zinv<T, false, NL>), g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz,
+                       extra);
 }
 template <typename T, bool F, bool I, int NL>
 static void xprod_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans,

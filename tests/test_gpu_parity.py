@@ -189,15 +189,18 @@ def test_stage_tiles(key, points):
                                                ("xprod", 2048, "N32_ny153"), ("xprod", 512, "N16_ny33_poly_ctP"),
                                                ("xprod", 256, "N8_ny17"), ("zfwd", 384, "N32_ny153"),
                                                ("zfwd", 100, "N32_ny153"), ("zfwd", 3000, "N32_ny153"),
-                                               ("zfwd", 200, "N16_ny33_poly_ctP")])
+                                               ("zfwd", 200, "N16_ny33_poly_ctP"), ("zinv", 384, "N32_ny153"),
+                                               ("zinv", 4096, "N32_ny153"), ("zinv", 768, "N16_ny33_poly_ctP"),
+                                               ("zinv", 256, "N8_ny17")])
 def test_stage_inplace(stage, points, case):
-    """The x-product and z-forward stages with their transforms in place in
-    one LDS buffer (where every stage has at most one item per thread;
-    otherwise the two-buffer path) give the oracle's step."""
+    """The z-inverse, x-product and z-forward stages with their transforms
+    in place in one LDS buffer (where every stage has at most one item per
+    thread; otherwise the two-buffer path) give the oracle's step."""
     cfg = CASES[case]
     s, o, f = make_pair(cfg, seed=11)
-    pk, ik = ((_lib.TUNE_XPROD_POINTS, _lib.TUNE_XPROD_INPLACE) if stage == "xprod"
-              else (_lib.TUNE_ZFWD_POINTS, _lib.TUNE_ZFWD_INPLACE))
+    pk, ik = {"xprod": (_lib.TUNE_XPROD_POINTS, _lib.TUNE_XPROD_INPLACE),
+              "zfwd": (_lib.TUNE_ZFWD_POINTS, _lib.TUNE_ZFWD_INPLACE),
+              "zinv": (_lib.TUNE_ZINV_POINTS, _lib.TUNE_ZINV_INPLACE)}[stage]
     s.set_tuning(pk, points)
     s.set_tuning(ik, 1)
     assert s.get_tuning(ik) == 1
