@@ -228,6 +228,7 @@ def main():
                 "solve_scan": _lib.TUNE_SOLVE_SCAN, "solve_waves": _lib.TUNE_SOLVE_WAVES,
                 "xprod_inplace": _lib.TUNE_XPROD_INPLACE, "zfwd_inplace": _lib.TUNE_ZFWD_INPLACE,
                 "zinv_inplace": _lib.TUNE_ZINV_INPLACE,
+                "zfwd_fieldwise": _lib.TUNE_ZFWD_FIELDWISE,
                 "xprod_threads": _lib.TUNE_XPROD_THREADS, "rhs_fused": _lib.TUNE_RHS_FUSED,
                 "ghost_in_zinv": _lib.TUNE_GHOST_IN_ZINV}
         for kv in args.tune.split(","):

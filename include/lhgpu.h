@@ -272,6 +272,12 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
 #define LH_TUNE_GHOST_IN_ZINV 12
 /* LH_TUNE_ZINV_INPLACE (default 0): the same for the z-inverse stage. */
 #define LH_TUNE_ZINV_INPLACE 13
+/* LH_TUNE_ZFWD_FIELDWISE (default 0): the z-forward stage transforms a
+ * plane's fields one after another in one LDS buffer, which leaves room for
+ * tiles several x-modes wide (contiguous loads and stores); used when every
+ * channel shares one of the lengths 24, 48, 96, 192, 384 with N2 = 2 Ngal2 / 3
+ * and the LH_TUNE_ZFWD_POINTS tile fits, else the stage runs as before. */
+#define LH_TUNE_ZFWD_FIELDWISE 14
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

@@ -27,6 +27,7 @@ TUNE_XPROD_THREADS = 10
 TUNE_RHS_FUSED = 11
 TUNE_GHOST_IN_ZINV = 12
 TUNE_ZINV_INPLACE = 13
+TUNE_ZFWD_FIELDWISE = 14
 PREC_FP64, PREC_MIXED = 0, 1
 
 # every symbol include/lhgpu.h declares

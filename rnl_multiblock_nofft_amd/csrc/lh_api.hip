@@ -1431,10 +1431,14 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     drop_graph(x);
     return LH_OK;
   }
-  if (key == LH_TUNE_XPROD_INPLACE || key == LH_TUNE_ZFWD_INPLACE || key == LH_TUNE_ZINV_INPLACE) {
-    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: in-place switches are 0 or 1");
+  if (key == LH_TUNE_XPROD_INPLACE || key == LH_TUNE_ZFWD_INPLACE || key == LH_TUNE_ZINV_INPLACE ||
+      key == LH_TUNE_ZFWD_FIELDWISE) {
+    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: in-place and field-wise switches are 0 or 1");
     ShapeMax m = x->shape;
-    (key == LH_TUNE_XPROD_INPLACE ? m.xprod_ip : key == LH_TUNE_ZFWD_INPLACE ? m.zfwd_ip : m.zinv_ip) = value;
+    (key == LH_TUNE_XPROD_INPLACE  ? m.xprod_ip
+     : key == LH_TUNE_ZFWD_INPLACE ? m.zfwd_ip
+     : key == LH_TUNE_ZINV_INPLACE ? m.zinv_ip
+                                   : m.zfwd_fs) = value;
     shape_tiles(m);
     if (stage_lds_bytes(x, m) > 160 * 1024) return set_err(LH_EINVAL, "lh_set_tuning: LDS exceeds 160 KiB");
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1526,6 +1530,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
     *value = x->shape.zinv_ip;
     return LH_OK;
   }
+  if (key == LH_TUNE_ZFWD_FIELDWISE) {
+    *value = x->shape.zfwd_fs;
+    return LH_OK;
+  }
   if (key == LH_TUNE_XPROD_THREADS) {
     *value = x->shape.xprod_threads;
     return LH_OK;
@@ -1589,17 +1597,23 @@ int lh_autotune(lh_ctx *x, int reps) {
   for (int st = 0; st < 3; st++) {
     float best_ms = 1e30f;
     for (int E : pts)
-      for (int v2 = 0; v2 < 2 * (st == 1 ? 2 : 1); v2++) {
-        const int ip = v2 & 1, nt = (v2 >> 1) ? 128 : 256;
+      for (int v2 = 0; v2 < (st == 1 ? 4 : st == 2 ? 3 : 2); v2++) {
+        // x-product: in place x 256/128 threads; z-forward: in place, field-wise
+        const int ip = (v2 & 1) || (st == 2 && v2 == 2), nt = (st == 1 && (v2 >> 1)) ? 128 : 256;
+        const int fs = st == 2 && v2 == 2;
         ShapeMax m = best;
         *stage_knob(m, keys[st]) = E;
         if (st == 1) {
           m.xprod_ip = ip;
           m.xprod_threads = nt;
         }
-        if (st == 2) m.zfwd_ip = ip;
+        if (st == 2) {
+          m.zfwd_ip = ip;
+          m.zfwd_fs = fs;
+        }
         if (st == 0) m.zinv_ip = ip;
         shape_tiles(m);
+        if (fs && !zfwd_fs_on(m)) continue;
         if (stage_lds_bytes(x, m) > 160 * 1024) continue;
         float ms = time_launches(
             x, e0, e1, reps, [&](int) { return launch_nl_stage(x->arena, x->d_chans, m, x->mixed, st, x->stream); });
@@ -1610,7 +1624,10 @@ int lh_autotune(lh_ctx *x, int reps) {
             best.xprod_ip = ip;
             best.xprod_threads = nt;
           }
-          if (st == 2) best.zfwd_ip = ip;
+          if (st == 2) {
+            best.zfwd_ip = ip;
+            best.zfwd_fs = fs;
+          }
           if (st == 0) best.zinv_ip = ip;
         }
       }

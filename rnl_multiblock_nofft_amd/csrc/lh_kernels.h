@@ -24,12 +24,14 @@ struct ShapeMax {
   int xprod_ip = 0;     // x-product stage: in-place single-buffer stages where they fit
   int zfwd_ip = 0;      // z-forward stage: the same
   int zinv_ip = 0;      // z-inverse stage: the same
+  int zfwd_fs = 0;      // z-forward stage: fields one after another in one LDS buffer, wider tiles
   int xprod_threads = 256;  // x-product stage: threads per workgroup (128 or 256)
   int rhs_fused = 1;   // smooth-walled ensembles: right-hand sides computed inside the column solve
   int ghost_zinv = 1;  // in a captured step, ghost rows of x-independent textures done by k_zinv's extra workgroups
   int nphys = 0;       // N1*N2, largest over channels (texture mask kernels)
 };
 void shape_tiles(ShapeMax &m);
+bool zfwd_fs_on(const ShapeMax &m);  // the field-by-field z-forward stage is in use
 int solve_cols_default(int nyu);
 __host__ __device__ size_t solve_lds_bytes(int nyu, int C);
 

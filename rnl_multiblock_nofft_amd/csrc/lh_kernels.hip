@@ -701,6 +701,122 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
 }
 
 // ------------------------------------------------------------------------
+// K3, field by field.  With one x-mode per tile (the tile k_zfwd can afford
+// when it holds all four fields in LDS) every lane of a load reads 16 bytes
+// of its own gx row (rows nxp points apart) and every output store is as
+// scattered; those accesses, not the transforms, set k_zfwd's time.  Here a
+// workgroup takes up to fs_ti_max(N) x-modes, so a z row is read, and an
+// output row k written, as a run of TI consecutive points, and transforms
+// the two (v-planes) or four (u-planes) fields one after another in a single
+// in-place LDS buffer of TI sequences.  The outputs are sums over fields
+// (Nu1 = ikx X0 + ikz X1, ...): each thread keeps the partial sums of the
+// output points it owns in registers and stores an output once its last
+// field is transformed.  Same arithmetic per output as k_zfwd (the sums are
+// formed in the same order).  Lengths with a compiled plan only; the host
+// checks zfwd_fs_fits for every channel (it includes N2 = 2 Ng2 / 3).
+// (Issuing the next field's loads before the current transform was tried:
+// the compiler keeps that register buffer in scratch memory, 0.62 -> 0.86 ms
+// at 256 x 256 x 257.)
+__host__ __device__ constexpr int fs_ti_max(int n) {
+  return 256 / plan_max_items(n) < 6 * 256 / n ? 256 / plan_max_items(n) : 6 * 256 / n;
+}
+__host__ __device__ inline bool zfwd_fs_fits(int N, int N2, int ti) {
+  switch (N) {
+    case 24: case 48: case 96: case 192: case 384: return ti <= fs_ti_max(N) && 3 * N2 == 2 * N;
+    default: return false;
+  }
+}
+// NP fields (4: u-plane X0..X3, 2: v-plane X0, X1) of one tile; o0..o2 as in
+// ZOut (u: nl1, nl3, fvv; v: nl2, fuv, fwv), orow = j ldc + i0
+template <typename C, int NL, int NP>
+__device__ __forceinline__ void zfwd_fs_run(const View &v, C *A, const C *TW, const C *gx, int ni, int S, int nxp,
+                                            int nxh, int i0, double2 *o0, double2 *o1, double2 *o2, size_t orow) {
+  constexpr int Ng2 = NL, N2 = 2 * NL / 3, TM = fs_ti_max(NL);
+  constexpr int LU = (TM * Ng2 + 255) / 256, OU = (TM * N2 + 255) / 256;  // loads, outputs per thread
+  constexpr int half = N2 / 2, dz = Ng2 - N2;
+  const int tid = threadIdx.x;
+  // this thread's points of a field, z << 6 | ii for t = z ni + ii =
+  // tid + u 256 (ii < 32); points past the field's end take point 0 (same
+  // value, same LDS place), so every load and store is unconditional
+  int zi[LU];
+#pragma unroll
+  for (int u = 0; u < LU; u++) {
+    const int t = tid + u * 256;
+    const int z = t / ni, ii = t - z * ni;
+    zi[u] = t < ni * Ng2 ? (z << 6 | ii) : 0;
+  }
+  const double inv = 1.0 / Ng2;
+  double2 sq[OU];  // partial sums ikx X0 (u-planes then ikx X1)
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    const C *gp = gx + (size_t)p * Ng2 * nxp;
+    C g[LU];
+#pragma unroll
+    for (int u = 0; u < LU; u++) g[u] = gp[(zi[u] >> 6) * nxp + (zi[u] & 63)];
+#pragma unroll
+    for (int u = 0; u < LU; u++) A[(zi[u] & 63) * S + (zi[u] >> 6)] = g[u];
+    __syncthreads();
+    bf_stages_ip<C, NL, -1, false, 0, plan_len(NL)>(A, ni, S, TW);  // ends with a barrier
+#pragma unroll
+    for (int q = 0; q < OU; q++) {
+      const int t = tid + q * 256;
+      if (t < ni * N2) {
+        const int k = t / ni, ii = t - k * ni;
+        const double2 X = cscale(inv, to_d(A[ii * S + (k < half ? k : k + dz)]));
+        const size_t o = orow + (size_t)k * nxh + ii;
+        const bool live = k != half;
+        const double kxi = v.kx[i0 + ii], kzk = v.kz[k];
+        if (NP == 4) {  // Nu1 = ikx X0 + ikz X1, Nu3 = ikx X1 + ikz X3, fvv = X2
+          if (p == 0) sq[q] = cimul(kxi, X);
+          if (p == 1) {
+            o0[o] = live ? cadd(sq[q], cimul(kzk, X)) : make_double2(0, 0);
+            sq[q] = cimul(kxi, X);
+          }
+          if (p == 2) o2[o] = X;
+          if (p == 3) o1[o] = live ? cadd(sq[q], cimul(kzk, X)) : make_double2(0, 0);
+        } else {  // Nu2 = ikx X0 + ikz X1, fuv = X0, fwv = X1
+          if (p == 0) sq[q] = cimul(kxi, X);
+          if (p == 1) o0[o] = live ? cadd(sq[q], cimul(kzk, X)) : make_double2(0, 0);
+          (p == 0 ? o1 : o2)[o] = X;
+        }
+      }
+    }
+    if (p + 1 < NP) __syncthreads();  // every output read before the next field's stores
+  }
+}
+template <typename C, int NL>
+__global__ void __launch_bounds__(256) k_zfwd_fs(char *arena, const Chan *chans, int E, int ntile, int nslot,
+                                                 int nzfwd, int ndt, int ntiles_xprod, int e_xprod) {
+  extern __shared__ double2 smem_d[];
+  if ((int)blockIdx.x < ndt) {  // the step's time-step update, as in k_zfwd
+    dt_update_block(arena, chans[blockIdx.z], ntiles_xprod, e_xprod, reinterpret_cast<double *>(smem_d));
+    return;
+  }
+  const Chan ch = chans[blockIdx.z];
+  if (ch.tri) return;
+  const View v(arena, ch);
+  int tile, slot;
+  if (!xcd_tile(ntile, nslot, tile, slot, nzfwd, ndt)) return;
+  if (slot >= ch.nslot) return;
+  const int nxh = ch.nxh, nxp = ch.nxp;
+  const int TI = tile_of8(E, NL, nxh);
+  const int i0 = tile * TI;
+  if (i0 >= nxh) return;
+  const int ni = min(TI, nxh - i0);
+  const int S = seq_stride(NL, TI, sizeof(C));
+  C *A = reinterpret_cast<C *>(smem_d);
+  C *TW = A + TI * S;
+  stage_twiddles<C>(TW, v.twz, NL);  // published by the first field's barrier
+  const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * NL * nxp + i0;
+  if (slot < ch.nplu)
+    zfwd_fs_run<C, NL, 4>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl1, v.nl3, v.fvv,
+                              (size_t)(slot + 1) * ch.ldc + i0);
+  else
+    zfwd_fs_run<C, NL, 2>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl2, v.fuv, v.fwv,
+                              (size_t)(slot - ch.nplu + 1) * ch.ldc + i0);
+}
+
+// ------------------------------------------------------------------------
 // K4: right-hand sides.  rhs_point is the right-hand side of one point (row j,
 // column c): r1, r3 on rows 0..nyu-1 and r2 on rows 0..nyv-1 (zero on the
 // wall rows), and, when `put`, the new explicit terms N1o, N3o, N2o of the
@@ -1841,6 +1957,14 @@ static size_t lds_zfwd(int E, int Ng2, int nxh, bool ip) {
   const int ti = tile_of8(E, Ng2, nxh);
   return (ip && zfwd_inplace(Ng2, ti) ? 1 : 2) * seq_buf(Ng2, 4 * ti) + Ng2;
 }
+// the field-by-field z-forward stage runs when it is asked for and every
+// channel shares one compiled length whose tile it fits
+bool zfwd_fs_on(const ShapeMax &m) {
+  if (!m.zfwd_fs || m.dims.empty()) return false;
+  for (const ChanDims &d : m.dims)
+    if (d.Ng2 != m.dims[0].Ng2 || !zfwd_fs_fits(d.Ng2, d.N2, tile_of8(m.e_zfwd, d.Ng2, d.N1 / 2 + 1))) return false;
+  return true;
+}
 
 // tile counts and LDS sizes for the current targets, over every channel
 void shape_tiles(ShapeMax &m) {
@@ -1856,6 +1980,11 @@ void shape_tiles(ShapeMax &m) {
     m.lds_zinv = std::max(m.lds_zinv, lds_zinv(m.e_zinv, d.Ng2, nxh, m.zinv_ip != 0));
     m.lds_xprod = std::max(m.lds_xprod, lds_xprod(m.e_xprod, d.Ng1, d.Ng2, m.xprod_ip != 0, m.xprod_threads));
     m.lds_zfwd = std::max(m.lds_zfwd, lds_zfwd(m.e_zfwd, d.Ng2, nxh, m.zfwd_ip != 0));
+  }
+  if (zfwd_fs_on(m)) {
+    m.lds_zfwd = 0;
+    for (const ChanDims &d : m.dims)
+      m.lds_zfwd = std::max(m.lds_zfwd, seq_buf(d.Ng2, tile_of8(m.e_zfwd, d.Ng2, d.N1 / 2 + 1)) + d.Ng2);
   }
 }
 
@@ -1889,6 +2018,12 @@ template <typename T, bool F, bool I, int NL>
 static void xprod_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans,
                          const ShapeMax &m) {
   hipLaunchKernelGGL((k_xprod<T, F, I, NL>), g, b, lds, s, arena, d_chans, m.e_xprod);
+}
+template <typename T, int NL>
+static void zfwd_fs_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans,
+                           const ShapeMax &m, int nz, int dtblk) {
+  hipLaunchKernelGGL((k_zfwd_fs<T, NL>), g, b, lds, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot, nz,
+                     dtblk ? 1 : 0, m.xprod_tiles, m.e_xprod);
 }
 template <typename T, bool I, int NL>
 static void zfwd_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans, const ShapeMax &m,
@@ -1956,6 +2091,23 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
     const int nz = (int)xcd_grid(m.zfwd_tiles, m.nslot);
     dim3 g(nz + (dtblk ? 1 : 0), 1, m.nchan);
     const size_t lds = std::max(m.lds_zfwd * es, (size_t)48 * sizeof(double));
+    if (zfwd_fs_on(m)) {
+#define LH_ZFS(n)                                                                               \
+  case n:                                                                                       \
+    if (mixed) zfwd_fs_launch<float2, n>(g, b, lds, s, arena, d_chans, m, nz, dtblk);           \
+    else zfwd_fs_launch<double2, n>(g, b, lds, s, arena, d_chans, m, nz, dtblk);                \
+    break;
+      switch (m.dims[0].Ng2) {
+        LH_ZFS(24)
+        LH_ZFS(48)
+        LH_ZFS(96)
+        LH_ZFS(192)
+        LH_ZFS(384)
+        default: return hipErrorInvalidValue;
+      }
+#undef LH_ZFS
+      return hipGetLastError();
+    }
 #define LH_ZF(n)                                                                                  \
   if (mixed) {                                                                                    \
     if (m.zfwd_ip) zfwd_launch<float2, true, n>(g, b, lds, s, arena, d_chans, m, nz, dtblk);       \

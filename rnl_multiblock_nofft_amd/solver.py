@@ -198,6 +198,7 @@ class Solver:
                 "xprod_inplace": self.get_tuning(_lib.TUNE_XPROD_INPLACE),
                 "zfwd_inplace": self.get_tuning(_lib.TUNE_ZFWD_INPLACE),
                 "zinv_inplace": self.get_tuning(_lib.TUNE_ZINV_INPLACE),
+                "zfwd_fieldwise": self.get_tuning(_lib.TUNE_ZFWD_FIELDWISE),
                 "xprod_threads": self.get_tuning(_lib.TUNE_XPROD_THREADS),
                 "rhs_fused": self.get_tuning(_lib.TUNE_RHS_FUSED),
                 "ghost_in_zinv": self.get_tuning(_lib.TUNE_GHOST_IN_ZINV)}

@@ -212,6 +212,42 @@ def test_stage_inplace(stage, points, case):
         assert e <= TOL, (nm, e)
 
 
+@pytest.mark.parametrize("points,case", [(48, "N32_ny153"), (240, "N32_ny153"), (768, "N32_ny153"),
+                                         (72, "N16_ny33_poly_ctP"), (768, "N16_ny33_poly_ctP"), (384, "N8_ny17")])
+def test_zfwd_fieldwise(points, case):
+    """The z-forward stage transforming a plane's fields one after another
+    in tiles of 1, 3, 5, 9 and 16 x-modes (ragged last tiles of 2 and 1
+    modes) gives the oracle's step; at Ngal2 = 12 (no compiled plan) the
+    switch leaves the stage as it was."""
+    cfg = CASES[case]
+    say("zfwd fieldwise", points, case)
+    s, o, f = make_pair(cfg, seed=17)
+    s.set_tuning(_lib.TUNE_ZFWD_POINTS, points)
+    s.set_tuning(_lib.TUNE_ZFWD_FIELDWISE, 1)
+    assert s.get_tuning(_lib.TUNE_ZFWD_FIELDWISE) == 1
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("fieldwise", points, case, nm, e)
+        assert e <= TOL, (nm, e)
+
+
+def test_zfwd_fieldwise_mixed():
+    """The same in mixed precision, against the FP64 oracle at the mixed
+    bar."""
+    cfg = CASES["N32_ny153"]
+    s, o, f = make_pair(cfg, seed=18, precision="mixed")
+    s.set_tuning(_lib.TUNE_ZFWD_POINTS, 240)
+    s.set_tuning(_lib.TUNE_ZFWD_FIELDWISE, 1)
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("fieldwise mixed", nm, e)
+        assert e <= 1e-5, (nm, e)
+
+
 @pytest.mark.parametrize("threads,ip,points", [(128, 0, 768), (128, 1, 384), (128, 1, 768), (128, 0, 2048)])
 def test_xprod_threads(threads, ip, points):
     """The x-product stage in workgroups of 128 threads (two-buffer and
