@@ -270,13 +270,16 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
  * launch less per substep; the step's results are the same).  Channels with
  * N2 of 32 or 64 only; 0 keeps the separate launch. */
 #define LH_TUNE_GHOST_IN_ZINV 12
-/* LH_TUNE_ZINV_INPLACE (default 0): the same for the z-inverse stage. */
+/* LH_TUNE_ZINV_INPLACE (default 0): the same for the z-inverse stage; 2
+ * runs it in place in workgroups of 512 threads (room for tiles of 8
+ * x-modes at 384 points). */
 #define LH_TUNE_ZINV_INPLACE 13
-/* LH_TUNE_ZFWD_FIELDWISE (default 0): the z-forward stage transforms a
- * plane's fields one after another in one LDS buffer, which leaves room for
- * tiles several x-modes wide (contiguous loads and stores); used when every
- * channel shares one of the lengths 24, 48, 96, 192, 384 with N2 = 2 Ngal2 / 3
- * and the LH_TUNE_ZFWD_POINTS tile fits, else the stage runs as before. */
+/* LH_TUNE_ZFWD_FIELDWISE (default 0): 1 or 2, the z-forward stage
+ * transforms a plane's fields one after another in one LDS buffer, which
+ * leaves room for tiles several x-modes wide (contiguous loads and stores),
+ * in workgroups of 256 (1) or 512 (2) threads; used when every channel
+ * shares one of the lengths 24, 48, 96, 192, 384 with N2 = 2 Ngal2 / 3 and
+ * the LH_TUNE_ZFWD_POINTS tile fits, else the stage runs as before. */
 #define LH_TUNE_ZFWD_FIELDWISE 14
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);

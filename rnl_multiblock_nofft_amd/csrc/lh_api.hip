@@ -1433,7 +1433,9 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
   }
   if (key == LH_TUNE_XPROD_INPLACE || key == LH_TUNE_ZFWD_INPLACE || key == LH_TUNE_ZINV_INPLACE ||
       key == LH_TUNE_ZFWD_FIELDWISE) {
-    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: in-place and field-wise switches are 0 or 1");
+    if (value != 0 && value != 1 && !(value == 2 && (key == LH_TUNE_ZFWD_FIELDWISE || key == LH_TUNE_ZINV_INPLACE)))
+      return set_err(LH_EINVAL, "lh_set_tuning: in-place switches are 0 or 1 (LH_TUNE_ZINV_INPLACE and "
+                                "LH_TUNE_ZFWD_FIELDWISE also 2)");
     ShapeMax m = x->shape;
     (key == LH_TUNE_XPROD_INPLACE  ? m.xprod_ip
      : key == LH_TUNE_ZFWD_INPLACE ? m.zfwd_ip
@@ -1597,10 +1599,12 @@ int lh_autotune(lh_ctx *x, int reps) {
   for (int st = 0; st < 3; st++) {
     float best_ms = 1e30f;
     for (int E : pts)
-      for (int v2 = 0; v2 < (st == 1 ? 4 : st == 2 ? 3 : 2); v2++) {
-        // x-product: in place x 256/128 threads; z-forward: in place, field-wise
-        const int ip = (v2 & 1) || (st == 2 && v2 == 2), nt = (st == 1 && (v2 >> 1)) ? 128 : 256;
-        const int fs = st == 2 && v2 == 2;
+      for (int v2 = 0; v2 < (st == 0 ? 3 : 4); v2++) {
+        // z-inverse: in place in workgroups of 256 or 512 threads; x-product:
+        // in place x 256/128 threads; z-forward: in place, or field-wise in
+        // workgroups of 256 or 512 threads
+        const int ip = st == 0 ? v2 : (v2 & 1) || (st == 2 && v2 >= 2), nt = (st == 1 && (v2 >> 1)) ? 128 : 256;
+        const int fs = st == 2 && v2 >= 2 ? v2 - 1 : 0;
         ShapeMax m = best;
         *stage_knob(m, keys[st]) = E;
         if (st == 1) {

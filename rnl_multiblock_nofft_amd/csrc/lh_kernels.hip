@@ -110,10 +110,10 @@ __device__ __forceinline__ void zinv_tail_ip(C *A, const C *TW, int S, int ni, i
   }
 }
 // K1 in place (one LDS buffer) for the compiled lengths when every stage has
-// at most one item per thread: ti plan_max_items(N) <= 256
-__host__ __device__ inline bool zinv_inplace(int N, int ti) {
+// at most one item per thread: ti plan_max_items(N) <= nt threads
+__host__ __device__ inline bool zinv_inplace(int N, int ti, int nt = 256) {
   switch (N) {
-    case 24: case 48: case 96: case 384: return ti * plan_max_items(N) <= 256;
+    case 24: case 48: case 96: case 384: return ti * plan_max_items(N) <= nt;
     default: return false;
   }
 }
@@ -123,9 +123,11 @@ __host__ __device__ inline bool zinv_inplace(int N, int ti) {
 // NL: the z length Ngal2 of every channel of the launch when the host
 // specialises it (the stage then holds only that length's path, fewer
 // registers), 0 for a launch whose channels differ; IP: the stages run in
-// place where zinv_inplace allows (with NL the host has checked that they do)
-template <typename C, bool IP, int NL>
-__global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, int E, int ntile, int nslot, int nzinv,
+// place where zinv_inplace allows (with NL the host has checked that they do);
+// NT threads per workgroup (512 lets tiles of 8 x-modes, whole 128-byte
+// lines of the spectral and hz rows, run in place at 384 points)
+template <typename C, bool IP, int NL, int NT = 256>
+__global__ void __launch_bounds__(NT) k_zinv(char *arena, const Chan *chans, int E, int ntile, int nslot, int nzinv,
                                               int nghost) {
   if ((int)blockIdx.x < nghost) {
     // workgroups before the transform grid (dispatched first, so that they
@@ -160,7 +162,7 @@ __global__ void __launch_bounds__(256) k_zinv(char *arena, const Chan *chans, in
   extern __shared__ double2 smem_d[];
   // ni sequences of Ng2 points, S apart (padded: conflict-free stages)
   const int S = seq_stride(Ng2, TI, sizeof(C));
-  const bool inplace = IP && (NL ? true : zinv_inplace(Ng2, TI));
+  const bool inplace = IP && (NL ? true : zinv_inplace(Ng2, TI, NT));
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + TI * S;
   C *TW = inplace ? B : B + TI * S;
@@ -717,31 +719,33 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
 // (Issuing the next field's loads before the current transform was tried:
 // the compiler keeps that register buffer in scratch memory, 0.62 -> 0.86 ms
 // at 256 x 256 x 257.)
-__host__ __device__ constexpr int fs_ti_max(int n) {
-  return 256 / plan_max_items(n) < 6 * 256 / n ? 256 / plan_max_items(n) : 6 * 256 / n;
+// widest tile for nt threads: one item per thread in every stage, at most
+// six points of a field per thread
+__host__ __device__ constexpr int fs_ti_max(int n, int nt) {
+  return nt / plan_max_items(n) < 6 * nt / n ? nt / plan_max_items(n) : 6 * nt / n;
 }
-__host__ __device__ inline bool zfwd_fs_fits(int N, int N2, int ti) {
+__host__ __device__ inline bool zfwd_fs_fits(int N, int N2, int ti, int nt) {
   switch (N) {
-    case 24: case 48: case 96: case 192: case 384: return ti <= fs_ti_max(N) && 3 * N2 == 2 * N;
+    case 24: case 48: case 96: case 192: case 384: return ti <= fs_ti_max(N, nt) && 3 * N2 == 2 * N;
     default: return false;
   }
 }
 // NP fields (4: u-plane X0..X3, 2: v-plane X0, X1) of one tile; o0..o2 as in
-// ZOut (u: nl1, nl3, fvv; v: nl2, fuv, fwv), orow = j ldc + i0
-template <typename C, int NL, int NP>
+// ZOut (u: nl1, nl3, fvv; v: nl2, fuv, fwv), orow = j ldc + i0; NT threads
+template <typename C, int NL, int NP, int NT>
 __device__ __forceinline__ void zfwd_fs_run(const View &v, C *A, const C *TW, const C *gx, int ni, int S, int nxp,
                                             int nxh, int i0, double2 *o0, double2 *o1, double2 *o2, size_t orow) {
-  constexpr int Ng2 = NL, N2 = 2 * NL / 3, TM = fs_ti_max(NL);
-  constexpr int LU = (TM * Ng2 + 255) / 256, OU = (TM * N2 + 255) / 256;  // loads, outputs per thread
+  constexpr int Ng2 = NL, N2 = 2 * NL / 3, TM = fs_ti_max(NL, NT);
+  constexpr int LU = (TM * Ng2 + NT - 1) / NT, OU = (TM * N2 + NT - 1) / NT;  // loads, outputs per thread
   constexpr int half = N2 / 2, dz = Ng2 - N2;
   const int tid = threadIdx.x;
   // this thread's points of a field, z << 6 | ii for t = z ni + ii =
-  // tid + u 256 (ii < 32); points past the field's end take point 0 (same
+  // tid + u NT (ii < 64); points past the field's end take point 0 (same
   // value, same LDS place), so every load and store is unconditional
   int zi[LU];
 #pragma unroll
   for (int u = 0; u < LU; u++) {
-    const int t = tid + u * 256;
+    const int t = tid + u * NT;
     const int z = t / ni, ii = t - z * ni;
     zi[u] = t < ni * Ng2 ? (z << 6 | ii) : 0;
   }
@@ -759,7 +763,7 @@ __device__ __forceinline__ void zfwd_fs_run(const View &v, C *A, const C *TW, co
     bf_stages_ip<C, NL, -1, false, 0, plan_len(NL)>(A, ni, S, TW);  // ends with a barrier
 #pragma unroll
     for (int q = 0; q < OU; q++) {
-      const int t = tid + q * 256;
+      const int t = tid + q * NT;
       if (t < ni * N2) {
         const int k = t / ni, ii = t - k * ni;
         const double2 X = cscale(inv, to_d(A[ii * S + (k < half ? k : k + dz)]));
@@ -784,8 +788,8 @@ __device__ __forceinline__ void zfwd_fs_run(const View &v, C *A, const C *TW, co
     if (p + 1 < NP) __syncthreads();  // every output read before the next field's stores
   }
 }
-template <typename C, int NL>
-__global__ void __launch_bounds__(256) k_zfwd_fs(char *arena, const Chan *chans, int E, int ntile, int nslot,
+template <typename C, int NL, int NT>
+__global__ void __launch_bounds__(NT) k_zfwd_fs(char *arena, const Chan *chans, int E, int ntile, int nslot,
                                                  int nzfwd, int ndt, int ntiles_xprod, int e_xprod) {
   extern __shared__ double2 smem_d[];
   if ((int)blockIdx.x < ndt) {  // the step's time-step update, as in k_zfwd
@@ -809,10 +813,10 @@ __global__ void __launch_bounds__(256) k_zfwd_fs(char *arena, const Chan *chans,
   stage_twiddles<C>(TW, v.twz, NL);  // published by the first field's barrier
   const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * NL * nxp + i0;
   if (slot < ch.nplu)
-    zfwd_fs_run<C, NL, 4>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl1, v.nl3, v.fvv,
+    zfwd_fs_run<C, NL, 4, NT>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl1, v.nl3, v.fvv,
                               (size_t)(slot + 1) * ch.ldc + i0);
   else
-    zfwd_fs_run<C, NL, 2>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl2, v.fuv, v.fwv,
+    zfwd_fs_run<C, NL, 2, NT>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl2, v.fuv, v.fwv,
                               (size_t)(slot - ch.nplu + 1) * ch.ldc + i0);
 }
 
@@ -1945,9 +1949,9 @@ hipError_t launch_u11(char *arena, const Chan *d_chans, const ShapeMax &m, hipSt
 static size_t seq_buf(int n, int nb) {
   return (size_t)nb * std::max(seq_stride(n, nb, sizeof(double2)), seq_stride(n, nb, sizeof(float2)));
 }
-static size_t lds_zinv(int E, int Ng2, int nxh, bool ip) {
+static size_t lds_zinv(int E, int Ng2, int nxh, int ip) {
   const int ti = tile_of8(E, Ng2, nxh);
-  return (ip && zinv_inplace(Ng2, ti) ? 1 : 2) * seq_buf(Ng2, ti) + Ng2;
+  return (ip && zinv_inplace(Ng2, ti, ip == 2 ? 512 : 256) ? 1 : 2) * seq_buf(Ng2, ti) + Ng2;
 }
 static size_t lds_xprod(int E, int Ng1, int Ng2, bool ip, int nt) {
   const int tz = tile_of(E, Ng1, Ng2);
@@ -1961,8 +1965,10 @@ static size_t lds_zfwd(int E, int Ng2, int nxh, bool ip) {
 // channel shares one compiled length whose tile it fits
 bool zfwd_fs_on(const ShapeMax &m) {
   if (!m.zfwd_fs || m.dims.empty()) return false;
+  const int nt = m.zfwd_fs == 2 ? 512 : 256;
   for (const ChanDims &d : m.dims)
-    if (d.Ng2 != m.dims[0].Ng2 || !zfwd_fs_fits(d.Ng2, d.N2, tile_of8(m.e_zfwd, d.Ng2, d.N1 / 2 + 1))) return false;
+    if (d.Ng2 != m.dims[0].Ng2 || !zfwd_fs_fits(d.Ng2, d.N2, tile_of8(m.e_zfwd, d.Ng2, d.N1 / 2 + 1), nt))
+      return false;
   return true;
 }
 
@@ -1977,7 +1983,7 @@ void shape_tiles(ShapeMax &m) {
     m.zinv_tiles = max(m.zinv_tiles, (nxh + ti - 1) / ti);
     m.xprod_tiles = max(m.xprod_tiles, (d.Ng2 + tx - 1) / tx);
     m.zfwd_tiles = max(m.zfwd_tiles, (nxh + tf - 1) / tf);
-    m.lds_zinv = std::max(m.lds_zinv, lds_zinv(m.e_zinv, d.Ng2, nxh, m.zinv_ip != 0));
+    m.lds_zinv = std::max(m.lds_zinv, lds_zinv(m.e_zinv, d.Ng2, nxh, m.zinv_ip));
     m.lds_xprod = std::max(m.lds_xprod, lds_xprod(m.e_xprod, d.Ng1, d.Ng2, m.xprod_ip != 0, m.xprod_threads));
     m.lds_zfwd = std::max(m.lds_zfwd, lds_zfwd(m.e_zfwd, d.Ng2, nxh, m.zfwd_ip != 0));
   }
@@ -1997,7 +2003,8 @@ static int nl_length(const ShapeMax &m, int stage) {
     const int nxh = d.N1 / 2 + 1;
     int len = stage == 1 ? d.Ng1 : d.Ng2;
     if (stage == 1 && m.xprod_ip && !xprod_inplace(len, tile_of(m.e_xprod, d.Ng1, d.Ng2), m.xprod_threads)) len = 0;
-    if (stage == 0 && m.zinv_ip && !zinv_inplace(len, tile_of8(m.e_zinv, d.Ng2, nxh))) len = 0;
+    if (stage == 0 && m.zinv_ip && !zinv_inplace(len, tile_of8(m.e_zinv, d.Ng2, nxh), m.zinv_ip == 2 ? 512 : 256))
+      len = 0;
     if (stage == 2 && m.zfwd_ip && !zfwd_inplace(len, tile_of8(m.e_zfwd, d.Ng2, nxh))) len = 0;
     if (n >= 0 && len != n) return 0;
     n = len;
@@ -2008,7 +2015,10 @@ static int nl_length(const ShapeMax &m, int stage) {
 template <typename T, int NL>
 static void zinv_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans, const ShapeMax &m,
                         int nz, int extra) {
-  if (m.zinv_ip)
+  if (m.zinv_ip == 2)
+    hipLaunchKernelGGL((k_zinv<T, true, NL, 512>), g, dim3(512), lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles,
+                       m.nslot, nz, extra);
+  else if (m.zinv_ip)
     hipLaunchKernelGGL((k_zinv<T, true, NL>), g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz, extra);
   else
     hipLaunchKernelGGL((k_zinv<T, false, NL>), g, b, lds, s, arena, d_chans, m.e_zinv, m.zinv_tiles, m.nslot, nz,
@@ -2022,8 +2032,12 @@ static void xprod_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena,
 template <typename T, int NL>
 static void zfwd_fs_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans,
                            const ShapeMax &m, int nz, int dtblk) {
-  hipLaunchKernelGGL((k_zfwd_fs<T, NL>), g, b, lds, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot, nz,
-                     dtblk ? 1 : 0, m.xprod_tiles, m.e_xprod);
+  if (m.zfwd_fs == 2)
+    hipLaunchKernelGGL((k_zfwd_fs<T, NL, 512>), g, dim3(512), lds, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot,
+                       nz, dtblk ? 1 : 0, m.xprod_tiles, m.e_xprod);
+  else
+    hipLaunchKernelGGL((k_zfwd_fs<T, NL, 256>), g, dim3(256), lds, s, arena, d_chans, m.e_zfwd, m.zfwd_tiles, m.nslot,
+                       nz, dtblk ? 1 : 0, m.xprod_tiles, m.e_xprod);
 }
 template <typename T, bool I, int NL>
 static void zfwd_launch(dim3 g, dim3 b, size_t lds, hipStream_t s, char *arena, const Chan *d_chans, const ShapeMax &m,

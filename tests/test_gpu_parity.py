@@ -212,34 +212,38 @@ def test_stage_inplace(stage, points, case):
         assert e <= TOL, (nm, e)
 
 
-@pytest.mark.parametrize("points,case", [(48, "N32_ny153"), (240, "N32_ny153"), (768, "N32_ny153"),
-                                         (72, "N16_ny33_poly_ctP"), (768, "N16_ny33_poly_ctP"), (384, "N8_ny17")])
-def test_zfwd_fieldwise(points, case):
-    """The z-forward stage transforming a plane's fields one after another
-    in tiles of 1, 3, 5, 9 and 16 x-modes (ragged last tiles of 2 and 1
-    modes) gives the oracle's step; at Ngal2 = 12 (no compiled plan) the
-    switch leaves the stage as it was."""
+@pytest.mark.parametrize("mode,points,case", [(1, 48, "N32_ny153"), (1, 240, "N32_ny153"), (1, 768, "N32_ny153"),
+                                              (2, 240, "N32_ny153"), (2, 1536, "N32_ny153"),
+                                              (1, 72, "N16_ny33_poly_ctP"), (1, 768, "N16_ny33_poly_ctP"),
+                                              (2, 768, "N16_ny33_poly_ctP"), (1, 384, "N8_ny17")])
+def test_zfwd_fieldwise(mode, points, case):
+    """The z-forward stage transforming a plane's fields one after another,
+    in workgroups of 256 (mode 1) or 512 (mode 2) threads, in tiles of 1, 3,
+    5, 9 and 16 x-modes (ragged last tiles of 2 and 1 modes; at mode 2 and
+    1536 points a 17-mode tile), gives the oracle's step; at Ngal2 = 12 (no
+    compiled plan) the switch leaves the stage as it was."""
     cfg = CASES[case]
-    say("zfwd fieldwise", points, case)
+    say("zfwd fieldwise", mode, points, case)
     s, o, f = make_pair(cfg, seed=17)
+    s.set_tuning(_lib.TUNE_ZFWD_FIELDWISE, mode)
     s.set_tuning(_lib.TUNE_ZFWD_POINTS, points)
-    s.set_tuning(_lib.TUNE_ZFWD_FIELDWISE, 1)
-    assert s.get_tuning(_lib.TUNE_ZFWD_FIELDWISE) == 1
+    assert s.get_tuning(_lib.TUNE_ZFWD_FIELDWISE) == mode
     s.step(1)
     o.step(1)
     for nm in ("u1", "u2", "u3", "p"):
         e = relerr(s.download(0, nm), o.get(nm))
-        say("fieldwise", points, case, nm, e)
+        say("fieldwise", mode, points, case, nm, e)
         assert e <= TOL, (nm, e)
 
 
-def test_zfwd_fieldwise_mixed():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_zfwd_fieldwise_mixed(mode):
     """The same in mixed precision, against the FP64 oracle at the mixed
     bar."""
     cfg = CASES["N32_ny153"]
     s, o, f = make_pair(cfg, seed=18, precision="mixed")
+    s.set_tuning(_lib.TUNE_ZFWD_FIELDWISE, mode)
     s.set_tuning(_lib.TUNE_ZFWD_POINTS, 240)
-    s.set_tuning(_lib.TUNE_ZFWD_FIELDWISE, 1)
     s.step(1)
     o.step(1)
     for nm in ("u1", "u2", "u3"):
@@ -368,20 +372,43 @@ def test_ridges_wall_paths(sep):
         assert e <= 1e-11, (nm, e)
 
 
-@pytest.mark.parametrize("n,precision", [(32, "fp64"), (64, "fp64"), (32, "mixed")])
-def test_ghost_pass_in_zinv(n, precision):
+@pytest.mark.parametrize("points,case", [(768, "N32_ny153"), (4096, "N32_ny153"), (768, "N16_ny33_poly_ctP"),
+                                         (256, "N8_ny17")])
+def test_zinv_512_threads(points, case):
+    """The z-inverse stage in place in workgroups of 512 threads (tiles of
+    16 and 17 x-modes at 48 points, which 256 threads cannot run in place;
+    9 modes at 24; the two-buffer path at 12) gives the oracle's step."""
+    cfg = CASES[case]
+    say("zinv 512", points, case)
+    s, o, f = make_pair(cfg, seed=21)
+    s.set_tuning(_lib.TUNE_ZINV_INPLACE, 2)
+    s.set_tuning(_lib.TUNE_ZINV_POINTS, points)
+    assert s.get_tuning(_lib.TUNE_ZINV_INPLACE) == 2
+    s.step(1)
+    o.step(1)
+    for nm in ("u1", "u2", "u3", "p"):
+        e = relerr(s.download(0, nm), o.get(nm))
+        say("zinv 512", points, case, nm, e)
+        assert e <= TOL, (nm, e)
+
+
+@pytest.mark.parametrize("n,precision,zinv_ip", [(32, "fp64", 0), (64, "fp64", 0), (32, "mixed", 0),
+                                                 (64, "fp64", 2)])
+def test_ghost_pass_in_zinv(n, precision, zinv_ip):
     """Captured steps that leave the ridges' ghost-row pass to extra
     workgroups of the next z-inverse launch (within a step and across the
     steps of one call) give the oracle's steps and the separate launch's
-    steps (the same arithmetic).  In mixed precision the z-inverse stage is
-    FP32 and the ghost pass FP64; the oracle bound there is the mixed mode's
-    1e-5 (1e-4 for p, which carries ~10x the velocity round-off)."""
+    steps (the same arithmetic), also with the z-inverse stage in 512-thread
+    workgroups.  In mixed precision the z-inverse stage is FP32 and the ghost
+    pass FP64; the oracle bound there is the mixed mode's 1e-5 (1e-4 for p,
+    which carries ~10x the velocity round-off)."""
     cfg = small_cfg(n, 33)
     m = textures.stripes(n, n, 4, 0.5)
     tol = 1e-11 if precision == "fp64" else 1e-5
     out = {}
     for g in (1, 0):
         s, o = tex_pair(cfg, (m, m), seed=19, precision=precision)
+        s.set_tuning(_lib.TUNE_ZINV_INPLACE, zinv_ip)
         s.set_tuning(_lib.TUNE_GHOST_IN_ZINV, g)
         assert s.get_tuning(_lib.TUNE_GHOST_IN_ZINV) == g
         s.step(3)
