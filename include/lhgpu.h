@@ -277,9 +277,10 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
 /* LH_TUNE_ZFWD_FIELDWISE (default 0): 1 or 2, the z-forward stage
  * transforms a plane's fields one after another in one LDS buffer, which
  * leaves room for tiles several x-modes wide (contiguous loads and stores),
- * in workgroups of 256 (1) or 512 (2) threads; used when every channel
- * shares one of the lengths 24, 48, 96, 192, 384 with N2 = 2 Ngal2 / 3 and
- * the LH_TUNE_ZFWD_POINTS tile fits, else the stage runs as before. */
+ * in workgroups of 256 (1) or 512 (2) threads; used when every channel has
+ * N2 = 2 Ngal2 / 3, the LH_TUNE_ZFWD_POINTS tile fits, and the channels
+ * share one of the lengths 24, 48, 96, 192, 384 or all have lengths 24, 48
+ * or 96; else the stage runs as before. */
 #define LH_TUNE_ZFWD_FIELDWISE 14
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);

@@ -32,6 +32,7 @@ struct ShapeMax {
 };
 void shape_tiles(ShapeMax &m);
 bool zfwd_fs_on(const ShapeMax &m);  // the field-by-field z-forward stage is in use
+int zfwd_fs_len(const ShapeMax &m);  // its specialised length (0: mixed 24/48/96), -1 when off
 int solve_cols_default(int nyu);
 __host__ __device__ size_t solve_lds_bytes(int nyu, int C);
 

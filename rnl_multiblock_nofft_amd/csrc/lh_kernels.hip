@@ -122,7 +122,8 @@ __host__ __device__ inline bool zinv_inplace(int N, int ti, int nt = 256) {
 // K1: interpolation + inverse z transform
 // NL: the z length Ngal2 of every channel of the launch when the host
 // specialises it (the stage then holds only that length's path, fewer
-// registers), 0 for a launch whose channels differ; IP: the stages run in
+// registers), -1 for a launch whose channels mix the lengths 24, 48 and 96
+// (the paths of those three only), 0 for any other mix; IP: the stages run in
 // place where zinv_inplace allows (with NL the host has checked that they do);
 // NT threads per workgroup (512 lets tiles of 8 x-modes, whole 128-byte
 // lines of the spectral and hz rows, run in place at 384 points)
@@ -154,7 +155,7 @@ __global__ void __launch_bounds__(NT) k_zinv(char *arena, const Chan *chans, int
     const int nplv = ch.nslot - ch.nplu;
     slot = slot < 2 * nplv ? ((slot & 1) ? ch.nplu + (slot >> 1) : (slot >> 1)) : slot - nplv;
   }
-  const int Ng2 = NL ? NL : ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
+  const int Ng2 = NL > 0 ? NL : ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
   const int TI = tile_of8(E, Ng2, nxh);
   const int i0 = tile * TI;
   if (i0 >= nxh) return;
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(NT) k_zinv(char *arena, const Chan *chans, int
   extern __shared__ double2 smem_d[];
   // ni sequences of Ng2 points, S apart (padded: conflict-free stages)
   const int S = seq_stride(Ng2, TI, sizeof(C));
-  const bool inplace = IP && (NL ? true : zinv_inplace(Ng2, TI, NT));
+  const bool inplace = IP && (NL > 0 ? true : zinv_inplace(Ng2, TI, NT));
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + TI * S;
   C *TW = inplace ? B : B + TI * S;
@@ -230,25 +231,36 @@ __global__ void __launch_bounds__(NT) k_zinv(char *arena, const Chan *chans, int
   }
   __syncthreads();
   C *hz = Inter<C>::hz(v) + ((size_t)slot * 3 + f) * Ng2 * nxp + i0;
-  switch (inplace ? Ng2 : -Ng2) {
 #define LH_IP(n) \
   if constexpr (IP) zinv_tail_ip<C, n>(A, TW, S, ni, nxp, hz)
-    case 24: LH_IP(24); break;
-    case 48: LH_IP(48); break;
-    case 96: LH_IP(96); break;
-    case 384: LH_IP(384); break;
-#undef LH_IP
-    case -24: zinv_tail<C, 24>(A, B, TW, S, ni, nxp, hz); break;
-    case -48: zinv_tail<C, 48>(A, B, TW, S, ni, nxp, hz); break;
-    case -96: zinv_tail<C, 96>(A, B, TW, S, ni, nxp, hz); break;
-    case -384: zinv_tail<C, 384>(A, B, TW, S, ni, nxp, hz); break;
-    default: {
-      C *R = fft_lds_bf<C>(A, B, Ng2, ni, S, +1, TW);
-      Walk2 ws(threadIdx.x, blockDim.x, ni);  // t = z ni + ii
-      for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x, ws.next())
-        hz[(size_t)ws.hi * nxp + ws.lo] = R[ws.lo * S + ws.hi];
+  if constexpr (NL == -1) {
+    switch (inplace ? Ng2 : -Ng2) {
+      case 24: LH_IP(24); break;
+      case 48: LH_IP(48); break;
+      case 96: LH_IP(96); break;
+      case -24: zinv_tail<C, 24>(A, B, TW, S, ni, nxp, hz); break;
+      case -48: zinv_tail<C, 48>(A, B, TW, S, ni, nxp, hz); break;
+      default: zinv_tail<C, 96>(A, B, TW, S, ni, nxp, hz); break;
+    }
+  } else {
+    switch (inplace ? Ng2 : -Ng2) {
+      case 24: LH_IP(24); break;
+      case 48: LH_IP(48); break;
+      case 96: LH_IP(96); break;
+      case 384: LH_IP(384); break;
+      case -24: zinv_tail<C, 24>(A, B, TW, S, ni, nxp, hz); break;
+      case -48: zinv_tail<C, 48>(A, B, TW, S, ni, nxp, hz); break;
+      case -96: zinv_tail<C, 96>(A, B, TW, S, ni, nxp, hz); break;
+      case -384: zinv_tail<C, 384>(A, B, TW, S, ni, nxp, hz); break;
+      default: {
+        C *R = fft_lds_bf<C>(A, B, Ng2, ni, S, +1, TW);
+        Walk2 ws(threadIdx.x, blockDim.x, ni);  // t = z ni + ii
+        for (int t = threadIdx.x; t < ni * Ng2; t += blockDim.x, ws.next())
+          hz[(size_t)ws.hi * nxp + ws.lo] = R[ws.lo * S + ws.hi];
+      }
     }
   }
+#undef LH_IP
 }
 
 // ------------------------------------------------------------------------
@@ -722,6 +734,7 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
 // widest tile for nt threads: one item per thread in every stage, at most
 // six points of a field per thread
 __host__ __device__ constexpr int fs_ti_max(int n, int nt) {
+  if (n <= 0) return 1;
   return nt / plan_max_items(n) < 6 * nt / n ? nt / plan_max_items(n) : 6 * nt / n;
 }
 __host__ __device__ inline bool zfwd_fs_fits(int N, int N2, int ti, int nt) {
@@ -788,6 +801,30 @@ __device__ __forceinline__ void zfwd_fs_run(const View &v, C *A, const C *TW, co
     if (p + 1 < NP) __syncthreads();  // every output read before the next field's stores
   }
 }
+// one (tile, slot) of a channel whose z length is NL
+template <typename C, int NL, int NT>
+__device__ __forceinline__ void zfwd_fs_tile(const View &v, const Chan &ch, int E, int tile, int slot) {
+  const int nxh = ch.nxh, nxp = ch.nxp;
+  const int TI = tile_of8(E, NL, nxh);
+  const int i0 = tile * TI;
+  if (i0 >= nxh) return;
+  const int ni = min(TI, nxh - i0);
+  const int S = seq_stride(NL, TI, sizeof(C));
+  extern __shared__ double2 smem_d[];
+  C *A = reinterpret_cast<C *>(smem_d);
+  C *TW = A + TI * S;
+  stage_twiddles<C>(TW, v.twz, NL);  // published by the first field's barrier
+  const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * NL * nxp + i0;
+  if (slot < ch.nplu)
+    zfwd_fs_run<C, NL, 4, NT>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl1, v.nl3, v.fvv,
+                              (size_t)(slot + 1) * ch.ldc + i0);
+  else
+    zfwd_fs_run<C, NL, 2, NT>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl2, v.fuv, v.fwv,
+                              (size_t)(slot - ch.nplu + 1) * ch.ldc + i0);
+}
+// NL: the length every channel of the launch shares, or 0 for a launch
+// mixing the lengths 24, 48 and 96 (each channel takes its own length's
+// path; the registers are those of the largest of the three)
 template <typename C, int NL, int NT>
 __global__ void __launch_bounds__(NT) k_zfwd_fs(char *arena, const Chan *chans, int E, int ntile, int nslot,
                                                  int nzfwd, int ndt, int ntiles_xprod, int e_xprod) {
@@ -802,22 +839,15 @@ __global__ void __launch_bounds__(NT) k_zfwd_fs(char *arena, const Chan *chans, 
   int tile, slot;
   if (!xcd_tile(ntile, nslot, tile, slot, nzfwd, ndt)) return;
   if (slot >= ch.nslot) return;
-  const int nxh = ch.nxh, nxp = ch.nxp;
-  const int TI = tile_of8(E, NL, nxh);
-  const int i0 = tile * TI;
-  if (i0 >= nxh) return;
-  const int ni = min(TI, nxh - i0);
-  const int S = seq_stride(NL, TI, sizeof(C));
-  C *A = reinterpret_cast<C *>(smem_d);
-  C *TW = A + TI * S;
-  stage_twiddles<C>(TW, v.twz, NL);  // published by the first field's barrier
-  const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * NL * nxp + i0;
-  if (slot < ch.nplu)
-    zfwd_fs_run<C, NL, 4, NT>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl1, v.nl3, v.fvv,
-                              (size_t)(slot + 1) * ch.ldc + i0);
-  else
-    zfwd_fs_run<C, NL, 2, NT>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl2, v.fuv, v.fwv,
-                              (size_t)(slot - ch.nplu + 1) * ch.ldc + i0);
+  if constexpr (NL != 0) {
+    zfwd_fs_tile<C, NL, NT>(v, ch, E, tile, slot);
+  } else {
+    switch (ch.Ng2) {
+      case 24: zfwd_fs_tile<C, 24, NT>(v, ch, E, tile, slot); break;
+      case 48: zfwd_fs_tile<C, 48, NT>(v, ch, E, tile, slot); break;
+      default: zfwd_fs_tile<C, 96, NT>(v, ch, E, tile, slot); break;
+    }
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -1962,15 +1992,21 @@ static size_t lds_zfwd(int E, int Ng2, int nxh, bool ip) {
   return (ip && zfwd_inplace(Ng2, ti) ? 1 : 2) * seq_buf(Ng2, 4 * ti) + Ng2;
 }
 // the field-by-field z-forward stage runs when it is asked for and every
-// channel shares one compiled length whose tile it fits
-bool zfwd_fs_on(const ShapeMax &m) {
-  if (!m.zfwd_fs || m.dims.empty()) return false;
+// channel's tile fits, all channels sharing one compiled length or all of
+// them of length 24, 48 or 96; returns the length to specialise for (0: the
+// mixed form), or -1 when it does not run
+int zfwd_fs_len(const ShapeMax &m) {
+  if (!m.zfwd_fs || m.dims.empty()) return -1;
   const int nt = m.zfwd_fs == 2 ? 512 : 256;
-  for (const ChanDims &d : m.dims)
-    if (d.Ng2 != m.dims[0].Ng2 || !zfwd_fs_fits(d.Ng2, d.N2, tile_of8(m.e_zfwd, d.Ng2, d.N1 / 2 + 1), nt))
-      return false;
-  return true;
+  bool same = true, small = true;
+  for (const ChanDims &d : m.dims) {
+    if (!zfwd_fs_fits(d.Ng2, d.N2, tile_of8(m.e_zfwd, d.Ng2, d.N1 / 2 + 1), nt)) return -1;
+    same = same && d.Ng2 == m.dims[0].Ng2;
+    small = small && d.Ng2 <= 96;
+  }
+  return same ? m.dims[0].Ng2 : small ? 0 : -1;
 }
+bool zfwd_fs_on(const ShapeMax &m) { return zfwd_fs_len(m) >= 0; }
 
 // tile counts and LDS sizes for the current targets, over every channel
 void shape_tiles(ShapeMax &m) {
@@ -2073,7 +2109,13 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
 #define LH_ZI(n)                                                                      \
   if (mixed) zinv_launch<float2, n>(g, b, lds, s, arena, d_chans, m, nz, extra);     \
   else zinv_launch<double2, n>(g, b, lds, s, arena, d_chans, m, nz, extra)
-    LH_BY_LEN(nl, LH_ZI)
+    bool small = nl == 0;  // a mix of the lengths 24, 48, 96 only
+    for (const ChanDims &d : m.dims) small = small && (d.Ng2 == 24 || d.Ng2 == 48 || d.Ng2 == 96);
+    if (small) {
+      LH_ZI(-1);
+    } else {
+      LH_BY_LEN(nl, LH_ZI)
+    }
 #undef LH_ZI
   } else if (stage == 1) {
     dim3 g(m.xprod_tiles, m.nslot, m.nchan);
@@ -2111,7 +2153,8 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
     if (mixed) zfwd_fs_launch<float2, n>(g, b, lds, s, arena, d_chans, m, nz, dtblk);           \
     else zfwd_fs_launch<double2, n>(g, b, lds, s, arena, d_chans, m, nz, dtblk);                \
     break;
-      switch (m.dims[0].Ng2) {
+      switch (zfwd_fs_len(m)) {
+        LH_ZFS(0)
         LH_ZFS(24)
         LH_ZFS(48)
         LH_ZFS(96)

@@ -547,6 +547,37 @@ def test_ensemble_uneven():
             assert e <= 1e-11, (q, nm, e)
 
 
+@pytest.mark.parametrize("mode,points,zinv_ip", [(1, 768, 0), (2, 768, 1), (1, 96, 2)])
+def test_zfwd_fieldwise_mixed_lengths(mode, points, zinv_ip):
+    """One launch with channels of z lengths 24, 48 and 24 (each channel
+    takes its own length's field-wise z-forward path, and its own length's
+    z-inverse path, in place or not; 768 points: tiles of 9 and 16 x-modes,
+    96: 4 and 2) gives each channel the oracle's step."""
+    cfgs = [small_cfg(16, 33), configs.smooth_32()[0], small_cfg(16, 25, grid_kind=0)]
+    say("fieldwise mixed lengths", mode, points, zinv_ip)
+    s = Solver(cfgs)
+    s.set_tuning(_lib.TUNE_ZFWD_FIELDWISE, mode)
+    s.set_tuning(_lib.TUNE_ZFWD_POINTS, points)
+    s.set_tuning(_lib.TUNE_ZINV_INPLACE, zinv_ip)
+    oracles = []
+    for q, c in enumerate(cfgs):
+        o = oracle_for(c)
+        f = init_fields(c, s.grid(q), seed=300 + q)
+        for name in ("u1", "u2", "u3", "p"):
+            s.upload(q, name, f[name])
+            o.set(name, f[name])
+        o.init_flowrate()
+        oracles.append(o)
+    s.init_flowrate()
+    s.step(1)
+    for q, o in enumerate(oracles):
+        o.step(1)
+        for nm in ("u1", "u2", "u3", "p"):
+            e = relerr(s.download(q, nm), o.get(nm))
+            say("fieldwise mixed lengths chan", q, nm, e)
+            assert e <= TOL, (q, nm, e)
+
+
 def test_mixed_precision_error():
     cfg = configs.smooth_32()[0]
     say("mixed")
