@@ -269,6 +269,8 @@ def main():
     kname = dom
     if dom == "k_solve" and tuning.get("solve_cols", 0) > 0:
         kname = "k_solve_lds"  # the LDS-staged form the autotuner chose
+    if dom == "k_zfwd" and tuning.get("zfwd_fieldwise", 0) > 0:
+        kname = "k_zfwd_fs"  # the field-by-field form
     traffic = measured_traffic(args.workload, args.precision, kname)
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic, kernel=kname,
