@@ -269,6 +269,8 @@ def main():
     kname = dom
     if dom == "k_solve" and tuning.get("solve_cols", 0) > 0:
         kname = "k_solve_lds"  # the LDS-staged form the autotuner chose
+    if dom == "k_solve" and tuning.get("solve_cols", 0) < 0:
+        kname = "k_solve_wave"  # one column per wave
     if dom == "k_zfwd" and tuning.get("zfwd_fieldwise", 0) > 0:
         kname = "k_zfwd_fs"  # the field-by-field form
     traffic = measured_traffic(args.workload, args.precision, kname)

@@ -223,7 +223,9 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
 /* Tuning knobs of the launch shapes (no counterpart in the reference).
  * LH_TUNE_SOLVE_COLS: spectral columns per workgroup of the column solve;
  * 0 selects one lane per column streaming rows from HBM, 1..10 the form that
- * stages a workgroup's columns in LDS (must fit 160 KiB).
+ * stages a workgroup's columns in LDS (must fit 160 KiB), -1 one column per
+ * wave with its rows in the lanes' registers (ny < 384; every Thomas
+ * recurrence a chunked scan over the wave, results agree to rounding).
  * LH_TUNE_{ZINV,XPROD,ZFWD}_POINTS: complex points per field that one
  * workgroup of the z-inverse, x-product and z-forward stages transforms
  * (whole sequences; the LDS they need must fit 160 KiB).
@@ -245,7 +247,8 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes);
 #define LH_TUNE_SOLVE_SCAN 6
 /* LH_TUNE_SOLVE_WAVES (default 1): waves per workgroup of the LDS column
  * solve (1, 2 or 4); more waves give each staged column more lanes for its
- * recurrences and more loads in flight. */
+ * recurrences and more loads in flight.  (The one-column-per-wave form
+ * always runs eight consecutive columns per workgroup.) */
 #define LH_TUNE_SOLVE_WAVES 7
 /* LH_TUNE_XPROD_INPLACE (default 0): 1 runs the x-product stage's transforms
  * in place in one LDS buffer where every stage has at most one item per

@@ -1378,6 +1378,7 @@ int lh_copy_to_host(void *h, const void *d, size_t bytes) {
 // tuning
 static int solve_cols_ok(const lh_ctx *x, int C) {
   if (C == 0) return 1;
+  if (C == -1) return solve_wave_rows(x->shape.nyu) > 0;
   if (C < 1 || C > 10) return 0;
   return solve_lds_bytes(x->shape.nyu, C) <= 160 * 1024;
 }
@@ -1480,7 +1481,9 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
   if (key == LH_TUNE_SOLVE_COLS) {
     if (!solve_cols_ok(x, value))
       return set_err(LH_EINVAL,
-                     "lh_set_tuning: %d columns per workgroup do not fit (0, or 1..10 within 160 KiB of LDS)", value);
+                     "lh_set_tuning: %d columns per workgroup do not fit (0, -1 for ny < %d, or 1..10 "
+                     "within 160 KiB of LDS)",
+                     value, 64 * SOLVE_WAVE_KMAX);
     HIPCHK(hipStreamSynchronize(x->stream));
     x->shape.solve_cols = value;
     drop_graph(x);
@@ -1643,7 +1646,7 @@ int lh_autotune(lh_ctx *x, int reps) {
   // solve (a win only where the columns are many: at a few hundred columns
   // the separate kernel hides the loads' latency better)
   {
-    const int cand[] = {0, 1, 2, 3, 4, 5, 6, 8, 10};
+    const int cand[] = {0, -1, 1, 2, 3, 4, 5, 6, 8, 10};
     float best_ms = 1e30f;
     for (int C : cand) {
       if (!solve_cols_ok(x, C)) continue;

@@ -18,9 +18,9 @@ struct ShapeMax {
   int e_zinv = 1024, e_xprod = 512, e_zfwd = 768;
   int zinv_tiles = 0, xprod_tiles = 0, zfwd_tiles = 0;
   size_t lds_zinv = 0, lds_xprod = 0, lds_zfwd = 0;  // in complex elements
-  int solve_cols = 0;  // columns per workgroup of the LDS column solve (0: one lane per column)
+  int solve_cols = 0;  // columns per workgroup of the LDS column solve (0: one lane per column, -1: one column per wave)
   int solve_scan = 1;  // LDS column solve: recurrences split over lane groups (0: one lane per sequence)
-  int solve_waves = 1;  // LDS column solve: waves per workgroup (1, 2 or 4)
+  int solve_waves = 1;  // LDS and wave column solves: waves per workgroup (1, 2 or 4)
   int xprod_ip = 0;     // x-product stage: in-place single-buffer stages where they fit
   int zfwd_ip = 0;      // z-forward stage: the same
   int zinv_ip = 0;      // z-inverse stage: the same
@@ -34,6 +34,18 @@ void shape_tiles(ShapeMax &m);
 bool zfwd_fs_on(const ShapeMax &m);  // the field-by-field z-forward stage is in use
 int zfwd_fs_len(const ShapeMax &m);  // its specialised length (0: mixed 24/48/96), -1 when off
 int solve_cols_default(int nyu);
+// rows per lane of the one-column-per-wave solve (solve_cols = -1): K with
+// 64 K >= nyu, at most SOLVE_WAVE_KMAX; 0 when the column is too tall
+constexpr int SOLVE_WAVE_KMAX = 6;
+int solve_wave_rows(int nyu);
+// waves (consecutive columns) per workgroup of that form, and its LDS: a
+// staging tile of nw columns x S complex rows (S = nyu made odd, so that
+// neighbouring columns' rows fall on different banks) and 9 row-coefficient
+// arrays
+__host__ __device__ inline int solve_wave_stride(int nyu) { return nyu | 1; }
+__host__ __device__ inline size_t solve_wave_lds(int nyu, int nw) {
+  return (size_t)nw * solve_wave_stride(nyu) * 16 + (size_t)9 * nyu * 8;
+}
 __host__ __device__ size_t solve_lds_bytes(int nyu, int C);
 
 void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny);

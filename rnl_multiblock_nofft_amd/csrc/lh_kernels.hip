@@ -1817,6 +1817,477 @@ __global__ void __launch_bounds__(256) k_solve_lds(char *arena, const Chan *chan
 #undef SB2
 #undef LB
 
+// ------------------------------------------------------------------------
+// K5 (wave form): one spectral column per wavefront, NW consecutive
+// columns per workgroup.  Lane l of a wave owns the contiguous rows
+// l K .. l K + K - 1 of its column and keeps every value of them in
+// registers between the loads and the stores.  Each recurrence of the solve
+// (U/W and V Helmholtz forward and back substitution, Poisson elimination
+// from the top and back substitution) runs as a chunked scan: every lane
+// composes its chunk's affine map z -> A z + B, the wave scans the maps in
+// six shuffle steps, and every lane reruns its chunk from its true start
+// value.  Every stored row therefore comes from the serial sweep's
+// operations (LUsolU_W / LUsolV / LUsolP, tridLU_3D.f90:55-226, with
+// LU_dec's factors); only the chunk start values carry the scan's
+// reassociation.  Each input row is read once and each output row written
+// once, with no intermediate in HBM.
+// HBM is reached through an LDS tile: the workgroup moves one field at a
+// time, a row of its NW columns per NW * 16 contiguous bytes (64 / NW rows per
+// wave instruction), and each wave reads or writes its own column in the
+// tile.  (Gathering a column straight from HBM would touch 64 rows per
+// instruction, 528 KB apart at 256 x 256: a separate cache line and page
+// translation per lane.)
+// Rows outside a recurrence's range are identity maps (A = 1, B = 0), and
+// every scan starts from z = 0: the first row of each forward range ignores
+// its input (a1 = 0 on row 0; the top or pinned Poisson row), and the last
+// row of each backward range has c = 0.
+template <int S>
+__device__ __forceinline__ void wave_scan(double &A, double (&B)[S], int lane, bool up) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double Ap = up ? __shfl_up(A, o, 64) : __shfl_down(A, o, 64);
+    double Bp[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) Bp[s] = up ? __shfl_up(B[s], o, 64) : __shfl_down(B[s], o, 64);
+    if (up ? lane >= o : lane + o < 64) {
+#pragma unroll
+      for (int s = 0; s < S; s++) B[s] = A * Bp[s] + B[s];
+      A = A * Ap;
+    }
+  }
+  // exclusive: the value entering the lane's chunk (0 for the first lane)
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const double b = up ? __shfl_up(B[s], 1, 64) : __shfl_down(B[s], 1, 64);
+    B[s] = (up ? lane == 0 : lane == 63) ? 0.0 : b;
+  }
+}
+
+// staging helpers of k_solve_wave: thread tid of a workgroup of NW waves
+// moves row (u 64 NW + tid) / NW of column c0 + tid % NW, u < K
+template <int K, int NW>
+__device__ __forceinline__ void sw_fetch(double2 (&t)[K], const double2 *F, int n, int ldc, int c0, int ncol,
+                                         int tid) {
+#pragma unroll
+  for (int u = 0; u < K; u++) {
+    const int idx = u * 64 * NW + tid, r = min(idx / NW, n - 1), cc = min(c0 + idx % NW, ncol - 1);
+    t[u] = F[(size_t)r * ldc + cc];
+  }
+}
+template <int K, int NW>
+__device__ __forceinline__ void sw_fetch2(double2 (&t)[K], const double *Fa, int na, const double *Fb, int nb, int ldc,
+                                          int c0, int ncol, int tid) {
+#pragma unroll
+  for (int u = 0; u < K; u++) {
+    const int idx = u * 64 * NW + tid, r = idx / NW, cc = min(c0 + idx % NW, ncol - 1);
+    t[u] = make_double2(Fa[(size_t)min(r, na - 1) * ldc + cc], Fb[(size_t)min(r, nb - 1) * ldc + cc]);
+  }
+}
+template <int K, int NW>
+__device__ __forceinline__ void sw_put(double2 *T, const double2 (&t)[K], int S, int nyu, int tid) {
+#pragma unroll
+  for (int u = 0; u < K; u++) {
+    const int idx = u * 64 * NW + tid, r = min(idx / NW, nyu - 1);
+    T[(idx % NW) * S + r] = t[u];  // rows past nyu - 1 rewrite row nyu - 1 with its own value
+  }
+}
+template <int K>
+__device__ __forceinline__ void sw_get(double2 (&X)[K], const double2 *col, int j0, int nyu) {
+#pragma unroll
+  for (int q = 0; q < K; q++) X[q] = col[min(j0 + q, nyu - 1)];
+}
+
+template <int K>
+__device__ __forceinline__ void sw_putcol(double2 *col, const double2 (&X)[K], int j0, int nyu) {
+#pragma unroll
+  for (int q = 0; q < K; q++)
+    if (j0 + q < nyu) col[j0 + q] = X[q];
+}
+// rows jlo..jhi of the live columns from the tile to F; with ADDP, p (its
+// values in tp, fetched by sw_fetch) gets the tile added
+template <int K, int NW, bool ADDP>
+__device__ __forceinline__ void sw_store(const double2 *T, double2 *F, double2 *p, const double2 (&tp)[K], int S,
+                                         int jlo, int jhi, int ldc, int c0, int ncol, int tid) {
+#pragma unroll
+  for (int u = 0; u < K; u++) {
+    const int idx = u * 64 * NW + tid, r = idx / NW, cc = c0 + idx % NW;
+    if (r >= jlo && r <= jhi && cc < ncol) {
+      const double2 x = T[(idx % NW) * S + r];
+      const size_t o = (size_t)r * ldc + cc;
+      if (ADDP) p[o] = make_double2(tp[u].x + x.x, tp[u].y + x.y);
+      F[o] = x;
+    }
+  }
+}
+
+template <int K, int NW>
+__global__ void __launch_bounds__(64 * NW) k_solve_wave(char *arena, const Chan *chans, int kRK) {
+  const Chan ch = chans[blockIdx.z];
+  const int per = gridDim.x / 8;
+  const int grp = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int c0 = grp * NW;
+  if (c0 >= ch.ncol) return;  // the whole workgroup, before any barrier
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int c = c0 + w;  // waves past the last column compute a copy of it and store nothing
+  const View v(arena, ch);
+  const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu, nyv = ch.nyv, ne = nyu - 2, ncol = ch.ncol;
+  const int S = solve_wave_stride(nyu);
+  const size_t plane = (size_t)(nyu + PADROWS) * ldc;
+  const double dt = v.scal[SC_DT];
+  const double beta = rk_b(kRK) * dt / ch.Re;
+  const double dtirk = (1.0 / dt) / (rk_a(kRK) + rk_b(kRK));
+  const double dtrk = dt * (rk_a(kRK) + rk_b(kRK));
+  const double *fU = v.fac + (size_t)((kRK - 1) * 4) * plane, *fV = fU + 2 * plane, *fP = v.fac + (size_t)12 * plane;
+  const int cl = min(c, ncol - 1);
+  const double kxi = v.kx[cl % nxh], kzk = v.kz[cl / nxh];
+  const bool col0 = c == 0;
+  const int j0 = lane * K;
+
+  extern __shared__ double2 smem_w[];
+  double2 *T = smem_w;  // staging tile [NW][S]
+  double *A1U = reinterpret_cast<double *>(T + NW * S), *A3U = A1U + nyu, *A1V = A3U + nyu, *A3V = A1V + nyu;
+  double *A3P = A3V + nyu, *A1P = A3P + nyu, *DUY = A1P + nyu, *DVY = DUY + nyu, *DYV = DVY + nyu;
+
+  // ---- staged loads, two fields in flight at a time (issued before the
+  //      barrier that frees the tile); p is loaded with the last pair, for
+  //      the final p += psi.  (Three in flight make the compiler keep one
+  //      staging array in scratch memory.)
+  double2 t[K], t2[K], tp[K];
+  sw_fetch<K, NW>(t, v.r1, nyu, ldc, c0, ncol, tid);
+  sw_fetch<K, NW>(t2, v.r3, nyu, ldc, c0, ncol, tid);
+
+  // ---- row coefficients, shared by the workgroup's columns: sub- and
+  //      super-diagonals of the Helmholtz operators (wall rows included:
+  //      a1 = 0 and a3 = gw1 on row 0, a1 = gw2 on row nyu-1), of the
+  //      Poisson operator, and the y metric terms
+  for (int j = tid; j < nyu; j += 64 * NW) {
+    const int jv = min(j, nyv - 1), jm = max(min(j - 1, nyv - 1), 0);
+    const bool iu = j >= 1 && j <= nyu - 2, iv = j >= 1 && j <= nyv - 2;
+    const double huj = v.dthdyu[j], hvj = v.dthdyv[jv], hvm = v.dthdyv[jm];
+    A1U[j] = iu ? -beta * v.dyu2i[3 * j] : (j == 0 ? 0.0 : ch.gw2);
+    A3U[j] = iu ? -beta * v.dyu2i[3 * j + 2] : (j == 0 ? ch.gw1 : 0.0);
+    A1V[j] = iv ? -beta * v.dyv2i[3 * jv] : 0.0;
+    A3V[j] = iv ? -beta * v.dyv2i[3 * jv + 2] : 0.0;
+    A3P[j] = (j >= 1 && j < ne) ? huj * hvj : 0.0;
+    A1P[j] = (j >= 2 && j <= ne) ? (-huj) * (-hvm) : 0.0;
+    DUY[j] = huj;
+    DVY[j] = j < nyv ? hvj : 0.0;
+    DYV[j] = (j >= 1 && j < nyv) ? v.yv[jv] - v.yv[jm] : 0.0;
+  }
+
+  double2 X1[K], X3[K], X2[K], DUV[K];
+  double DP[K];
+  sw_put<K, NW>(T, t, S, nyu, tid);
+  __syncthreads();
+  sw_get<K>(X1, T + w * S, j0, nyu);
+  __syncthreads();
+  sw_put<K, NW>(T, t2, S, nyu, tid);
+  __syncthreads();
+  sw_get<K>(X3, T + w * S, j0, nyu);
+  sw_fetch<K, NW>(t, v.r2, nyv, ldc, c0, ncol, tid);
+  sw_fetch2<K, NW>(t2, fU, nyu, fV, nyv, ldc, c0, ncol, tid);
+  __syncthreads();
+  sw_put<K, NW>(T, t, S, nyu, tid);
+  __syncthreads();
+  sw_get<K>(X2, T + w * S, j0, nyu);
+  __syncthreads();
+  sw_put<K, NW>(T, t2, S, nyu, tid);
+  __syncthreads();
+  sw_get<K>(DUV, T + w * S, j0, nyu);
+  sw_fetch2<K, NW>(t, fP, nyu, fP, nyu, ldc, c0, ncol, tid);
+  sw_fetch<K, NW>(tp, v.p, nyu, ldc, c0, ncol, tid);
+  __syncthreads();
+  sw_put<K, NW>(T, t, S, nyu, tid);
+  __syncthreads();
+  {
+    double2 dp[K];
+    sw_get<K>(dp, T + w * S, j0, nyu);
+#pragma unroll
+    for (int q = 0; q < K; q++) DP[q] = dp[q].x;
+  }
+
+  // ---- Helmholtz forward substitution, U/W rows 0..nyu-1 and V rows
+  //      0..nyv-1: y_j = (r_j - a1_j y_(j-1)) d_j
+  {
+    double AU = 1.0, AV = 1.0, BU[4] = {0.0, 0.0, 0.0, 0.0}, BV[2] = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      if (j < nyu) {
+        const double a = A1U[jc], d = DUV[q].x;
+        BU[0] = (X1[q].x - a * BU[0]) * d;
+        BU[1] = (X1[q].y - a * BU[1]) * d;
+        BU[2] = (X3[q].x - a * BU[2]) * d;
+        BU[3] = (X3[q].y - a * BU[3]) * d;
+        AU = (0.0 - a * AU) * d;
+      }
+      if (j < nyv) {
+        const double a = A1V[jc], d = DUV[q].y;
+        BV[0] = (X2[q].x - a * BV[0]) * d;
+        BV[1] = (X2[q].y - a * BV[1]) * d;
+        AV = (0.0 - a * AV) * d;
+      }
+    }
+    wave_scan<4>(AU, BU, lane, true);
+    wave_scan<2>(AV, BV, lane, true);
+    asm volatile("" ::: "memory");  // reread the coefficients rather than hold them across the scan
+    double2 y1 = make_double2(BU[0], BU[1]), y3 = make_double2(BU[2], BU[3]), y2 = make_double2(BV[0], BV[1]);
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      if (j < nyu) {
+        const double a = A1U[jc], d = DUV[q].x;
+        y1 = make_double2((X1[q].x - a * y1.x) * d, (X1[q].y - a * y1.y) * d);
+        y3 = make_double2((X3[q].x - a * y3.x) * d, (X3[q].y - a * y3.y) * d);
+        X1[q] = y1;
+        X3[q] = y3;
+      }
+      if (j < nyv) {
+        const double a = A1V[jc], d = DUV[q].y;
+        y2 = make_double2((X2[q].x - a * y2.x) * d, (X2[q].y - a * y2.y) * d);
+        X2[q] = y2;
+      }
+    }
+  }
+  // ---- back substitution: x_j = y_j - (a3_j d_j) x_(j+1); c = 0 on the
+  //      last row of each system, c_0 = gw1 (U/W) or 0 (V)
+  {
+    double AU = 1.0, AV = 1.0, BU[4] = {0.0, 0.0, 0.0, 0.0}, BV[2] = {0.0, 0.0};
+#pragma unroll
+    for (int q = K - 1; q >= 0; q--) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      if (j < nyu) {
+        const double e = A3U[jc] * DUV[q].x;
+        BU[0] = X1[q].x - e * BU[0];
+        BU[1] = X1[q].y - e * BU[1];
+        BU[2] = X3[q].x - e * BU[2];
+        BU[3] = X3[q].y - e * BU[3];
+        AU = 0.0 - e * AU;
+      }
+      if (j < nyv) {
+        const double e = A3V[jc] * DUV[q].y;
+        BV[0] = X2[q].x - e * BV[0];
+        BV[1] = X2[q].y - e * BV[1];
+        AV = 0.0 - e * AV;
+      }
+    }
+    wave_scan<4>(AU, BU, lane, false);
+    wave_scan<2>(AV, BV, lane, false);
+    asm volatile("" ::: "memory");
+    double2 x1 = make_double2(BU[0], BU[1]), x3 = make_double2(BU[2], BU[3]), x2 = make_double2(BV[0], BV[1]);
+#pragma unroll
+    for (int q = K - 1; q >= 0; q--) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      if (j < nyu) {
+        const double e = A3U[jc] * DUV[q].x;
+        x1 = make_double2(X1[q].x - e * x1.x, X1[q].y - e * x1.y);
+        x3 = make_double2(X3[q].x - e * x3.x, X3[q].y - e * x3.y);
+        X1[q] = x1;
+        X3[q] = x3;
+      }
+      if (j < nyv) {
+        const double e = A3V[jc] * DUV[q].y;
+        x2 = make_double2(X2[q].x - e * x2.x, X2[q].y - e * x2.y);
+        X2[q] = x2;
+      }
+    }
+  }
+
+  // ---- flow rate of the mean mode and flowrate_corr's g
+  //      (littleharsh.f90:476-509): Qx = sum (yv_j - yv_(j-1)) Re u1_j
+  double g = 0.0;
+  if (col0) {
+    double part = 0.0;
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      const int j = j0 + q;
+      if (j >= 1 && j <= ne) part = part + DYV[j] * X1[q].x;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
+    if (ch.flag_ctpress == 0) {
+      g = (v.scal[SC_QXT] - part) / v.scal[SC_QX1 + kRK - 1];
+      if (lane == 0) {
+        v.scal[SC_MPGX] = v.scal[SC_MPGX] + g;
+        v.scal[SC_DGX] = g;
+      }
+    }
+  }
+
+  // ---- Poisson right-hand side on pressure rows 1..ne: div / (dt (a+b))
+  //      (littleharsh.f90:237-272,581-606); the mean mode's row 1 is pinned
+  //      (a3 = 0, rhs = 0, tridLU_3D.f90:384-390)
+  double2 PS[K];
+  {
+    const double2 vlast = X2[K - 1];
+    const double2 vin = make_double2(__shfl_up(vlast.x, 1, 64), __shfl_up(vlast.y, 1, 64));
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      const double2 vm = q ? X2[q - 1] : vin;
+      const double2 a = cimul(kxi, X1[q]), b = cimul(kzk, X3[q]);
+      const double wy = DUY[jc];
+      const double2 dv = make_double2((a.x + b.x) + wy * (X2[q].x - vm.x), (a.y + b.y) + wy * (X2[q].y - vm.y));
+      PS[q] = (j >= 1 && j <= ne && !(col0 && j == 1)) ? make_double2(dtirk * dv.x, dtirk * dv.y)
+                                                       : make_double2(0, 0);
+    }
+  }
+  // ---- Poisson elimination from the top, rows ne..1
+  {
+    double A = 1.0, B[2] = {0.0, 0.0};
+#pragma unroll
+    for (int q = K - 1; q >= 0; q--) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      if (j >= 1 && j <= ne) {
+        const double a = (col0 && j == 1) ? 0.0 : A3P[jc], d = DP[q];
+        B[0] = (PS[q].x - a * B[0]) * d;
+        B[1] = (PS[q].y - a * B[1]) * d;
+        A = (0.0 - a * A) * d;
+      }
+    }
+    wave_scan<2>(A, B, lane, false);
+    asm volatile("" ::: "memory");
+    double2 z = make_double2(B[0], B[1]);
+#pragma unroll
+    for (int q = K - 1; q >= 0; q--) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      if (j >= 1 && j <= ne) {
+        const double a = (col0 && j == 1) ? 0.0 : A3P[jc], d = DP[q];
+        z = make_double2((PS[q].x - a * z.x) * d, (PS[q].y - a * z.y) * d);
+        PS[q] = z;
+      }
+    }
+  }
+  // ---- back substitution from psi(1): psi_j = y_j - (a1_j d_j) psi_(j-1)
+  {
+    double A = 1.0, B[2] = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      if (j >= 1 && j <= ne) {
+        const double e = A1P[jc] * DP[q];
+        B[0] = PS[q].x - e * B[0];
+        B[1] = PS[q].y - e * B[1];
+        A = 0.0 - e * A;
+      }
+    }
+    wave_scan<2>(A, B, lane, true);
+    asm volatile("" ::: "memory");
+    double2 z = make_double2(B[0], B[1]);
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      if (j >= 1 && j <= ne) {
+        const double e = A1P[jc] * DP[q];
+        z = make_double2(PS[q].x - e * z.x, PS[q].y - e * z.y);
+        PS[q] = z;
+      }
+    }
+  }
+
+  // ---- u1, u3 -= dt(a+b) grad_xz psi (+ the flow-rate correction of the
+  //      mean mode); v -= dt(a+b) D_y psi (littleharsh.f90:956-990)
+  const double dkx = dtrk * kxi, dkz = dtrk * kzk;
+  {
+    const double2 pfirst = PS[0];
+    const double2 pnext = make_double2(__shfl_down(pfirst.x, 1, 64), __shfl_down(pfirst.y, 1, 64));
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      if (j >= 1 && j <= ne) {
+        const double2 ps = PS[q];
+        const double2 a = cimul(dkx, ps), b = cimul(dkz, ps);
+        X1[q] = make_double2(X1[q].x - a.x, X1[q].y - a.y);
+        if (col0) X1[q].x = X1[q].x + g * v.u11[(size_t)(kRK - 1) * nyu + j];
+        X3[q] = make_double2(X3[q].x - b.x, X3[q].y - b.y);
+        if (j <= ne - 1) {
+          const double2 pn = q < K - 1 ? PS[q + 1] : pnext;
+          const double sv = dtrk * DVY[jc];
+          X2[q] = make_double2(X2[q].x - sv * (pn.x - ps.x), X2[q].y - sv * (pn.y - ps.y));
+        }
+      }
+    }
+  }
+
+  // ---- ghost and wall rows (littleharsh.f90:992-1044)
+  double2 u1a = make_double2(0, 0), u1b = u1a, u3a = u1a, u3b = u1a;
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const int j = j0 + q;
+    if (j == 1) {
+      u1a = X1[q];
+      u3a = X3[q];
+    }
+    if (j == ne) {
+      u1b = X1[q];
+      u3b = X3[q];
+    }
+  }
+  const int la = 1 / K, lb = ne / K;
+  u1a = make_double2(__shfl(u1a.x, la, 64), __shfl(u1a.y, la, 64));
+  u3a = make_double2(__shfl(u3a.x, la, 64), __shfl(u3a.y, la, 64));
+  u1b = make_double2(__shfl(u1b.x, lb, 64), __shfl(u1b.y, lb, 64));
+  u3b = make_double2(__shfl(u3b.x, lb, 64), __shfl(u3b.y, lb, 64));
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const int j = j0 + q;
+    if (j == 0) {
+      X1[q] = cscale(ch.gwbc1, u1a);
+      X3[q] = cscale(ch.gwbc3, u3a);
+    }
+    if (j == nyu - 1) {
+      X1[q] = cscale(ch.gwbc1, u1b);
+      X3[q] = cscale(ch.gwbc3, u3b);
+    }
+    if (j == 0 || j == nyv - 1) X2[q] = make_double2(0, 0);
+  }
+  // divergence of the corrected field on pressure rows 1..ne (v(0) and
+  // v(ne) are the walls)
+  double2 DV[K];
+  {
+    const double2 vlast = X2[K - 1];
+    const double2 vin = make_double2(__shfl_up(vlast.x, 1, 64), __shfl_up(vlast.y, 1, 64));
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      const int j = j0 + q, jc = min(j, nyu - 1);
+      const double2 da = cimul(kxi, X1[q]), db = cimul(kzk, X3[q]);
+      const double2 vq = X2[q], vm = q ? X2[q - 1] : vin;
+      const double wy = DUY[jc];
+      DV[q] = make_double2((da.x + db.x) + wy * (vq.x - vm.x), (da.y + db.y) + wy * (vq.y - vm.y));
+    }
+  }
+
+  // ---- staged stores: each wave writes its column into the tile, the
+  //      workgroup stores rows jlo..jhi of its live columns (p gets psi
+  //      added on the way)
+  __syncthreads();
+  sw_putcol<K>(T + w * S, X1, j0, nyu);
+  __syncthreads();
+  sw_store<K, NW, false>(T, v.u1, v.p, tp, S, 0, nyu - 1, ldc, c0, ncol, tid);
+  __syncthreads();
+  sw_putcol<K>(T + w * S, X3, j0, nyu);
+  __syncthreads();
+  sw_store<K, NW, false>(T, v.u3, v.p, tp, S, 0, nyu - 1, ldc, c0, ncol, tid);
+  __syncthreads();
+  sw_putcol<K>(T + w * S, X2, j0, nyu);
+  __syncthreads();
+  sw_store<K, NW, false>(T, v.u2, v.p, tp, S, 0, nyv - 1, ldc, c0, ncol, tid);
+  __syncthreads();
+  sw_putcol<K>(T + w * S, PS, j0, nyu);
+  __syncthreads();
+  sw_store<K, NW, true>(T, v.psi, v.p, tp, S, 1, ne, ldc, c0, ncol, tid);
+  __syncthreads();
+  sw_putcol<K>(T + w * S, DV, j0, nyu);
+  __syncthreads();
+  sw_store<K, NW, false>(T, v.div, v.p, tp, S, 1, ne, ldc, c0, ncol, tid);
+}
+
+// rows per lane of the wave-form solve (0: the column is too tall for it)
+int solve_wave_rows(int nyu) {
+  const int K = (nyu + 63) / 64;
+  return K <= SOLVE_WAVE_KMAX ? K : 0;
+}
+
 hipError_t launch_factors(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s) {
   dim3 g((m.ncol + 63) / 64, 1, m.nchan);
   hipLaunchKernelGGL(k_factors, g, dim3(64), 0, s, arena, d_chans);
@@ -2211,7 +2682,29 @@ hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int k
 
 hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse) {
   const int C = m.solve_cols;
-  if (C <= 0) {
+  if (C < 0) {  // one column per wave, eight columns per workgroup
+    if (fuse) {  // the wave form reads r: compute it first
+      hipError_t e = launch_rhs(arena, d_chans, m, kRK, s);
+      if (e != hipSuccess) return e;
+    }
+    constexpr int nw = 8;
+    const int K = solve_wave_rows(m.nyu);
+    const int ngrp = (m.ncol + nw - 1) / nw;
+    dim3 g((ngrp + 7) / 8 * 8, 1, m.nchan), b(64 * nw);
+    const size_t lds = solve_wave_lds(m.nyu, nw);
+#define LH_SW(k) \
+  case k: hipLaunchKernelGGL((k_solve_wave<k, nw>), g, b, lds, s, arena, d_chans, kRK); break;
+    switch (K) {
+      LH_SW(1)
+      LH_SW(2)
+      LH_SW(3)
+      LH_SW(4)
+      LH_SW(5)
+      LH_SW(6)
+      default: return hipErrorInvalidValue;
+    }
+#undef LH_SW
+  } else if (C == 0) {
     if (fuse) {  // the one-column-per-lane solve reads r: compute it first
       hipError_t e = launch_rhs(arena, d_chans, m, kRK, s);
       if (e != hipSuccess) return e;

@@ -43,6 +43,9 @@ CASES = {
 }
 
 
+TALL = {"N16_ny257": small_cfg(16, 257)}
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_phases_kRK(name):
     cfg = CASES[name]
@@ -116,13 +119,18 @@ def test_one_step_graph(name):
                                                   (3, 0, 1, "N32_ny153"), (3, 1, 1, "N32_ny153"),
                                                   (3, 1, 4, "N32_ny153"), (4, 0, 2, "N32_ny153"),
                                                   (1, 1, 1, "N8_ny17"), (2, 1, 4, "N8_ny17"),
-                                                  (10, 0, 1, "N16_ny33_poly_ctP"), (10, 1, 4, "N16_ny33_poly_ctP")])
+                                                  (10, 0, 1, "N16_ny33_poly_ctP"), (10, 1, 4, "N16_ny33_poly_ctP"),
+                                                  (-1, 0, 1, "N32_ny153"), (-1, 0, 1, "N8_ny17"),
+                                                  (-1, 0, 1, "N16_ny33_poly_ctP"), (-1, 0, 1, "N16_ny257"),
+                                                  (0, 0, 1, "N16_ny257")])
 def test_column_solve_layouts(cols, scan, waves, case):
-    """Both column-solve forms (one lane per column from HBM, or C columns
-    per workgroup staged in LDS, with each recurrence on one lane or split
-    over a lane group) give the oracle's step.  At 8 x 8 x 17 the Poisson
-    sweeps have fewer rows than lanes (empty chunks)."""
-    cfg = CASES[case]
+    """Every column-solve form (one lane per column from HBM; C columns per
+    workgroup staged in LDS, with each recurrence on one lane or split over a
+    lane group; one column per wave, its rows in registers) gives the
+    oracle's step.  At 8 x 8 x 17 the Poisson sweeps have fewer rows than
+    lanes (empty chunks); 16 x 16 x 257 puts five rows on each lane of the
+    wave form."""
+    cfg = CASES[case] if case in CASES else TALL[case]
     say("solve layout", cols, scan, waves, case)
     s, o, f = make_pair(cfg, seed=7)
     s.set_tuning(_lib.TUNE_SOLVE_COLS, cols)
@@ -140,7 +148,8 @@ def test_column_solve_layouts(cols, scan, waves, case):
 
 
 @pytest.mark.parametrize("cols,scan,waves,case", [(0, 0, 1, "N32_ny153"), (3, 1, 1, "N32_ny153"),
-                                                  (2, 1, 4, "N8_ny17"), (10, 1, 4, "N16_ny33_poly_ctP")])
+                                                  (2, 1, 4, "N8_ny17"), (10, 1, 4, "N16_ny33_poly_ctP"),
+                                                  (-1, 0, 1, "N32_ny153"), (-1, 0, 1, "N16_ny33_poly_ctP")])
 def test_rhs_fused_into_solve(cols, scan, waves, case):
     """Smooth-walled steps with the right-hand sides computed inside the
     column solve give the oracle's steps and, to round-off, the steps of the
