@@ -1419,16 +1419,17 @@ void lho_v_corr(lho_state *s, int kRK) {
     cplx *u1 = s->u1 + (size_t)c * nyu, *u3 = s->u3 + (size_t)c * nyu;
     cplx *u2 = s->u2 + (size_t)c * nyv;
     const cplx *ps = s->psi + (size_t)c * nyu;
+    const int elo = s->c.band_edge & 1, ehi = (s->c.band_edge >> 1) & 1;
     for (int j = 1; j <= nyu - 2; j++) u1[j] = u1[j] - kx * ps[j];
-    u1[0] = s->gwbc1 * u1[1];
-    u1[nyu - 1] = s->gwbc1 * u1[nyu - 2];
+    if (!elo) u1[0] = s->gwbc1 * u1[1];
+    if (!ehi) u1[nyu - 1] = s->gwbc1 * u1[nyu - 2];
     for (int j = 1; j <= nyv - 2; j++)
       u2[j] = u2[j] - dtrk_u2 * s->dthdyv[j] * (ps[j + 1] - ps[j]);
     u2[0] = 0;
     u2[nyv - 1] = 0;
     for (int j = 1; j <= nyu - 2; j++) u3[j] = u3[j] - kz * ps[j];
-    u3[0] = s->gwbc3 * u3[1];
-    u3[nyu - 1] = s->gwbc3 * u3[nyu - 2];
+    if (!elo) u3[0] = s->gwbc3 * u3[1];
+    if (!ehi) u3[nyu - 1] = s->gwbc3 * u3[nyu - 2];
   }
   if (s->tex[0] || s->tex[1]) tex_ghosts(s);
   lho_divergence(s);
@@ -1590,5 +1591,263 @@ void lho_slstats(lho_state *s, double *maps) {
         m[1] = m[1] + cimag(v[q]);
       }
     }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multi-resolution bands.                                                   */
+struct lhob_state {
+  lho_state *b[3];  /* bottom (wall resolution), middle, top (wall resolution) */
+  int row0[3];      /* global row of each band's local row 0 */
+  int nb;           /* N(3,1): last v row of the bottom band */
+  int N1m, N2m;     /* middle-band modes */
+};
+
+/* signed z wavenumber index of column c of s (k1F_z, def_k start.f90:
+ * 1107-1130: indices N2/2..N2-1 are the negative modes) */
+static int zmode(const lho_state *s, int c) {
+  int k = c / s->nxh;
+  return k < s->c.N2 / 2 ? k : k - s->c.N2;
+}
+/* a mode of the middle band's resolution: the long pencils of
+ * proc_lims_columns (start.f90:1624-1648): i <= N1m/2, -N2m/2 <= kz < N2m/2 */
+static int is_long(const lhob_state *B, int i, int m) { return i <= B->N1m / 2 && m >= -B->N2m / 2 && m < B->N2m / 2; }
+
+int lhob_owns(const lhob_state *B, int q, int c) {
+  const lho_state *s = B->b[q];
+  int l = is_long(B, c % s->nxh, zmode(s, c));
+  return q == 1 ? 1 : !l;
+}
+lho_state *lhob_band(lhob_state *B, int q) { return B->b[q]; }
+int lhob_row0(const lhob_state *B, int q) { return B->row0[q]; }
+
+lhob_state *lhob_create(const lho_cfg *wall, int N1m, int N2m, int Ngal1m, int Ngal2m, int nb) {
+  int ny = wall->ny;
+  if (nb < 1 || 2 * (nb + 2) + 2 > ny || N1m > wall->N1 || N2m > wall->N2) return NULL;
+  lhob_state *B = calloc(1, sizeof(lhob_state));
+  B->nb = nb;
+  B->N1m = N1m;
+  B->N2m = N2m;
+  /* the middle band: the whole channel at the middle resolution */
+  lho_cfg cm = *wall;
+  cm.N1 = N1m;
+  cm.N2 = N2m;
+  cm.Ngal1 = Ngal1m;
+  cm.Ngal2 = Ngal2m;
+  cm.band_edge = 0;
+  B->b[1] = lho_create(&cm);
+  /* the wall bands: v rows 0..nb+1 and ny-2-nb..ny-1 (jlim, start.f90:
+   * 1677-1711), so nb+2 v rows and nb+3 u rows each, at the wall
+   * resolution; no mean-flow work (the mean mode is a long pencil) */
+  lho_cfg cw = *wall;
+  cw.ny = nb + 2;
+  cw.flag_ctpress = 1;
+  cw.mpgx = 0.0;
+  B->row0[0] = 0;
+  B->row0[1] = 0;
+  B->row0[2] = ny - 2 - nb;
+  const lho_state *m = B->b[1];
+  for (int q = 0; q < 3; q += 2) {
+    cw.band_edge = q == 0 ? 2 : 1;
+    lho_state *s = lho_create(&cw);
+    int r0 = B->row0[q];
+    lho_set_grid(s, m->yu + r0, m->yv + r0, m->dthdyu + r0, m->dthdyv + r0);
+    B->b[q] = s;
+  }
+  return B;
+}
+
+void lhob_destroy(lhob_state *B) {
+  if (!B) return;
+  for (int q = 0; q < 3; q++) lho_destroy(B->b[q]);
+  free(B);
+}
+
+void lhob_init_flowrate(lhob_state *B) { lho_init_flowrate(B->b[1]); }
+
+/* band of global u row j (planes N(4,q-1)+1..N(4,q), proc_lims_planes
+ * start.f90:1940-1965; N(4,1) = nb+1, N(4,2) = ny-2-nb) and v row j
+ * (N(3,q-1)+1..N(3,q); N(3,1) = nb, N(3,2) = ny-2-nb) */
+static int uband(const lhob_state *B, int j) {
+  int ny = B->b[1]->c.ny;
+  return j <= B->nb + 1 ? 0 : (j <= ny - 2 - B->nb ? 1 : 2);
+}
+static int vband(const lhob_state *B, int j) {
+  int ny = B->b[1]->c.ny;
+  return j <= B->nb ? 0 : (j <= ny - 2 - B->nb ? 1 : 2);
+}
+
+/* modes_to_planes_UVP for one plane of band q (FOU3D.f90:1000-1098): the
+ * long pencils from the middle band's row j and, in a wall band, that
+ * band's short pencils from its local row; column (i, kz) to P(2i+1, row)
+ * with negative kz at the top of the plane band's Ngal2 rows (dk,
+ * start.f90:1720-1730) */
+static void band_gather(const lhob_state *B, int q, int vgrid, const cplx *fm, const cplx *fw, int j, double *P) {
+  const lho_state *S = B->b[q], *M = B->b[1];
+  int igal = S->igal, kgal = S->kgal;
+  memset(P, 0, sizeof(double) * (size_t)igal * kgal);
+  int rm = vgrid ? M->nyv : M->nyu;
+  for (int c = 0; c < M->ncol; c++) {
+    int i = c % M->nxh, m = zmode(M, c), kr = m >= 0 ? m : m + kgal;
+    cplx v = fm[(size_t)c * rm + j];
+    P[(size_t)kr * igal + 2 * i] = creal(v);
+    P[(size_t)kr * igal + 2 * i + 1] = cimag(v);
+  }
+  if (q == 1) return;
+  int rw = vgrid ? S->nyv : S->nyu, jl = j - B->row0[q];
+  for (int c = 0; c < S->ncol; c++) {
+    int i = c % S->nxh, m = zmode(S, c);
+    if (is_long(B, i, m)) continue;
+    int kr = m >= 0 ? m : m + kgal;
+    cplx v = fw[(size_t)c * rw + jl];
+    P[(size_t)kr * igal + 2 * i] = creal(v);
+    P[(size_t)kr * igal + 2 * i + 1] = cimag(v);
+  }
+}
+/* planes_to_modes_UVP / _NUVP (FOU3D.f90:2295-2442,2772-2854): the inverse */
+static void band_scatter(const lhob_state *B, int q, int vgrid, const double *P, cplx *fm, cplx *fw, int j) {
+  const lho_state *S = B->b[q], *M = B->b[1];
+  int igal = S->igal, kgal = S->kgal;
+  int rm = vgrid ? M->nyv : M->nyu;
+  for (int c = 0; c < M->ncol; c++) {
+    int i = c % M->nxh, m = zmode(M, c), kr = m >= 0 ? m : m + kgal;
+    fm[(size_t)c * rm + j] = P[(size_t)kr * igal + 2 * i] + I * P[(size_t)kr * igal + 2 * i + 1];
+  }
+  if (q == 1) return;
+  int rw = vgrid ? S->nyv : S->nyu, jl = j - B->row0[q];
+  for (int c = 0; c < S->ncol; c++) {
+    int i = c % S->nxh, m = zmode(S, c);
+    if (is_long(B, i, m)) continue;
+    int kr = m >= 0 ? m : m + kgal;
+    fw[(size_t)c * rw + jl] = P[(size_t)kr * igal + 2 * i] + I * P[(size_t)kr * igal + 2 * i + 1];
+  }
+}
+
+/* nonlinear (FOU3D.f90:95-283) with ops_in_planes2 (FOU3D.f90:2090-2251)
+ * over the bands: every plane at its band's resolution */
+void lhob_nonlinear(lhob_state *B, int kRK) {
+  lho_state *M = B->b[1];
+  int ny = M->c.ny;
+  for (int q = 0; q < 3; q++) interp_all(B->b[q]);
+  /* u-planes: uu, uw, vv, ww -> Nu1, Nu3 (plane part) and vv */
+#pragma omp parallel for schedule(dynamic)
+  for (int j = 1; j <= ny - 1; j++) {
+    int q = uband(B, j);
+    lho_state *S = B->b[q];
+    size_t psz = (size_t)S->igal * S->kgal;
+    double *P1 = malloc(sizeof(double) * psz), *P2 = malloc(sizeof(double) * psz);
+    double *P3 = malloc(sizeof(double) * psz), *Q = malloc(sizeof(double) * psz);
+    double *R = malloc(sizeof(double) * psz);
+    band_gather(B, q, 0, M->u1, S->u1, j, P1);
+    band_gather(B, q, 0, M->u2i, S->u2i, j, P2);
+    band_gather(B, q, 0, M->u3, S->u3, j, P3);
+    lho_four_to_phys_plane(S, P1);
+    lho_four_to_phys_plane(S, P2);
+    lho_four_to_phys_plane(S, P3);
+    for (int k = 0; k < S->c.Ngal2; k++)
+      for (int i = 0; i < S->c.Ngal1; i++) {
+        size_t o = (size_t)k * S->igal + i;
+        double a = P1[o], b = P2[o], c = P3[o];
+        Q[o] = a * c;
+        P1[o] = a * a;
+        P2[o] = b * b;
+        P3[o] = c * c;
+      }
+    lho_phys_to_four_plane(S, P1);
+    lho_phys_to_four_plane(S, Q);
+    lho_phys_to_four_plane(S, P2);
+    lho_phys_to_four_plane(S, P3);
+    derxz_sum(S, P1, Q, R);
+    band_scatter(B, q, 0, R, M->Nu1, S->Nu1, j);
+    derxz_sum(S, Q, P3, R);
+    band_scatter(B, q, 0, R, M->Nu3, S->Nu3, j);
+    band_scatter(B, q, 0, P2, M->vvc, S->vvc, j);
+    free(P1); free(P2); free(P3); free(Q); free(R);
+  }
+  /* v-planes: uv, wv -> Nu2 (plane part), uv, wv */
+#pragma omp parallel for schedule(dynamic)
+  for (int j = 1; j <= ny - 2; j++) {
+    int q = vband(B, j);
+    lho_state *S = B->b[q];
+    size_t psz = (size_t)S->igal * S->kgal;
+    double *P1 = malloc(sizeof(double) * psz), *P2 = malloc(sizeof(double) * psz);
+    double *P3 = malloc(sizeof(double) * psz), *R = malloc(sizeof(double) * psz);
+    band_gather(B, q, 1, M->u1i, S->u1i, j, P1);
+    band_gather(B, q, 1, M->u2, S->u2, j, P2);
+    band_gather(B, q, 1, M->u3i, S->u3i, j, P3);
+    lho_four_to_phys_plane(S, P1);
+    lho_four_to_phys_plane(S, P2);
+    lho_four_to_phys_plane(S, P3);
+    for (int k = 0; k < S->c.Ngal2; k++)
+      for (int i = 0; i < S->c.Ngal1; i++) {
+        size_t o = (size_t)k * S->igal + i;
+        double v = P2[o];
+        P1[o] = P1[o] * v;
+        P3[o] = P3[o] * v;
+      }
+    lho_phys_to_four_plane(S, P1);
+    lho_phys_to_four_plane(S, P3);
+    derxz_sum(S, P1, P3, R);
+    band_scatter(B, q, 1, R, M->Nu2, S->Nu2, j);
+    band_scatter(B, q, 1, P1, M->uvf, S->uvf, j);
+    band_scatter(B, q, 1, P3, M->wvf, S->wvf, j);
+    free(P1); free(P2); free(P3); free(R);
+  }
+  /* per band: the flux rows no plane fills are zero (the wall faces and
+   * ghost centres, and a wall band's edge rows, which planes_to_modes skips,
+   * FOU3D.f90:2295-2442), then der_yu_h / der_yv_h over the band's rows
+   * (FOU3D.f90:266-283) and du += -gRK N (FOU3D.f90:398-410) */
+  for (int q = 0; q < 3; q++) {
+    lho_state *s = B->b[q];
+    int nyu = s->nyu, nyv = s->nyv;
+    double ddv = s->ddthetavi, C1 = -s->gRK[kRK - 1];
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < s->ncol; c++) {
+      cplx *uv = s->uvf + (size_t)c * nyv, *wv = s->wvf + (size_t)c * nyv, *vv = s->vvc + (size_t)c * nyu;
+      uv[0] = uv[nyv - 1] = wv[0] = wv[nyv - 1] = 0;
+      vv[0] = vv[nyu - 1] = 0;
+      if (!lhob_owns(B, q, c)) continue;
+      cplx *N1 = s->Nu1 + (size_t)c * nyu, *N3 = s->Nu3 + (size_t)c * nyu, *N2 = s->Nu2 + (size_t)c * nyv;
+      for (int j = 1; j <= nyu - 2; j++) {
+        N1[j] = N1[j] + (uv[j] - uv[j - 1]) * s->dthdyu[j] * ddv;
+        N3[j] = N3[j] + (wv[j] - wv[j - 1]) * s->dthdyu[j] * ddv;
+      }
+      for (int j = 1; j <= nyv - 2; j++) N2[j] = N2[j] + (vv[j + 1] - vv[j]) * s->dthdyv[j] * ddv;
+      for (int j = 1; j <= nyu - 2; j++) {
+        s->du1[(size_t)c * nyu + j] += C1 * N1[j];
+        s->du3[(size_t)c * nyu + j] += C1 * N3[j];
+      }
+      for (int j = 1; j <= nyv - 2; j++) s->du2[(size_t)c * nyv + j] += C1 * N2[j];
+    }
+  }
+  /* fixed dt: the time advances at the first substep */
+  if (kRK == 1)
+    for (int q = 0; q < 3; q++) B->b[q]->t = B->b[q]->t + B->b[q]->dt;
+}
+
+/* one RK3 substep over the bands (littleharsh.f90:176-215): each band's
+ * column phases run over its own rows and edge rows (RHS0, solveU_W,
+ * solveV, solveP, v_corr loop iband = sband..eband); the flow-rate
+ * correction acts on the mean mode, a long pencil */
+void lhob_substep(lhob_state *B, int kRK) {
+  for (int q = 0; q < 3; q++) lho_rhs0(B->b[q], kRK);
+  lhob_nonlinear(B, kRK);
+  for (int q = 0; q < 3; q++) {
+    lho_solve_uw(B->b[q], kRK);
+    lho_solve_v(B->b[q], kRK);
+  }
+  if (B->b[1]->c.flag_ctpress == 0) lho_meanflow_ctU(B->b[1], kRK);
+  for (int q = 0; q < 3; q++) {
+    lho_solve_p(B->b[q], kRK);
+    lho_v_corr(B->b[q], kRK);
+  }
+}
+
+void lhob_step(lhob_state *B, int nsteps) {
+  for (int n = 0; n < nsteps; n++) {
+    for (int kRK = 1; kRK <= 3; kRK++) lhob_substep(B, kRK);
+    lho_state *M = B->b[1];
+    if (M->c.flag_ctpress == 1) M->Qx = flowrate_c(M, M->u1);
+    for (int q = 0; q < 3; q++) B->b[q]->iter = B->b[q]->iter + 1;
   }
 }

@@ -43,6 +43,12 @@ typedef struct lho_cfg {
   int fixed_dt;        /* 1: dt fixed; 0: dt = min(2.5 dtv, CFL dtc) at kRK=1 */
   double dt;
   double CFL;
+  /* band edges (multi-resolution bands, start.f90:985-1060): bit 0 when the
+   * first row of this column set is a band edge (the top short band's
+   * bottom), bit 1 when the last row is (the bottom short band's top).  A
+   * band edge keeps its solved ghost value: v_corr resets only the wall
+   * ghosts (littleharsh.f90:992-1044).  0 for a whole channel. */
+  int band_edge;
 } lho_cfg;
 
 typedef struct lho_state lho_state;
@@ -135,6 +141,36 @@ void lho_stats(lho_state *s, double *prof, double *sp);
  * gradient with the sign flipped as there). */
 void lho_slstats(lho_state *s, double *maps);
 
+/* ---- Multi-resolution bands (the reference's nband = 3 blocks,
+ * declaration.f90:8-13).  A banded channel has long pencils, the modes of
+ * the middle band's resolution (N1m x N2m), over the whole height, and short
+ * pencils, the remaining modes of the wall bands' resolution (N1 x N2 of
+ * the wall configuration), over the bottom band (v rows 0..nb+1) and the top
+ * band (v rows ny-2-nb..ny-1), each with its own edge rows
+ * (proc_lims_columns start.f90:1490-1774, jlim :1677-1711).  Planes of a
+ * band transform at that band's resolution and hold every mode present at
+ * their row (modes_to_planes_UVP FOU3D.f90:1000-1098 with dk,
+ * proc_lims_planes start.f90:1836-2142).  Band q (0 bottom, 1 middle, 2
+ * top) is an lho_state whose rows are the band's rows: the middle band the
+ * whole channel at N1m x N2m, the wall bands sub-grids at N1 x N2 whose
+ * long-pencil columns stay zero (those modes live in the middle band). */
+typedef struct lhob_state lhob_state;
+/* nb: the last v row of the bottom band, N(3,1) of ygrid (start.f90:
+ * 999-1011); 1 <= nb and the two wall bands must not meet */
+lhob_state *lhob_create(const lho_cfg *wall, int N1m, int N2m, int Ngal1m, int Ngal2m, int nb);
+void lhob_destroy(lhob_state *b);
+lho_state *lhob_band(lhob_state *b, int q);
+/* global row of the band's local row 0 */
+int lhob_row0(const lhob_state *b, int q);
+/* 1 when column c of band q is a pencil of that band (a long pencil of the
+ * middle band, a short pencil of a wall band) */
+int lhob_owns(const lhob_state *b, int q, int c);
+void lhob_init_flowrate(lhob_state *b);
+void lhob_substep(lhob_state *b, int kRK);
+void lhob_step(lhob_state *b, int nsteps);
+/* the banded nonlinear term alone (interp, planes, products, fluxes, N),
+ * as lho_nonlinear for one channel */
+void lhob_nonlinear(lhob_state *b, int kRK);
 #ifdef __cplusplus
 }
 #endif

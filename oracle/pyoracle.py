@@ -34,7 +34,7 @@ class Cfg(ctypes.Structure):
                 ("bslpu1", ctypes.c_double), ("bslpu3", ctypes.c_double),
                 ("flag_ctpress", ctypes.c_int), ("mpgx", ctypes.c_double),
                 ("fixed_dt", ctypes.c_int), ("dt", ctypes.c_double),
-                ("CFL", ctypes.c_double)]
+                ("CFL", ctypes.c_double), ("band_edge", ctypes.c_int)]
 
 
 def build():
@@ -93,6 +93,17 @@ def lib():
         L.lho_set_triads.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.lho_set_triads.restype = ctypes.c_int
+        L.lhob_create.restype = vp
+        L.lhob_create.argtypes = [ctypes.POINTER(Cfg), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int]
+        L.lhob_destroy.argtypes = [vp]
+        L.lhob_band.restype = vp
+        L.lhob_band.argtypes = [vp, ctypes.c_int]
+        L.lhob_row0.argtypes = [vp, ctypes.c_int]
+        L.lhob_owns.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.lhob_init_flowrate.argtypes = [vp]
+        for name in ("lhob_substep", "lhob_step", "lhob_nonlinear"):
+            getattr(L, name).argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -308,3 +319,99 @@ class Oracle:
         assert maps.shape == (2, 4, N2, N1 + 2) and maps.flags.c_contiguous
         lib().lho_slstats(self.h, _dp(maps))
         return maps
+
+
+def band_split(yu, h):
+    """N(3,1) of ygrid (start.f90:999-1011): the first row j with
+    yu(j) >= yu(0) + h, the last v row of the bottom band."""
+    j = 0
+    while yu[j] < yu[0] + h:
+        j += 1
+    return j
+
+
+class _BandView(Oracle):
+    """One band of a BandOracle: an Oracle over that band's rows."""
+
+    def __init__(self, parent, q, h, cfg):
+        self.parent = parent
+        self.q = q
+        self.h = h
+        self.cfg = cfg
+        self.ny = cfg["ny"]
+        self.nyu = self.ny + 1
+        self.ncol = (cfg["N1"] // 2 + 1) * cfg["N2"]
+
+    def __del__(self):
+        pass
+
+
+class BandOracle:
+    """A channel with multi-resolution bands on the CPU (the reference's
+    three blocks): middle-band modes N1m x N2m over the whole height, the
+    wall configuration's modes in the bottom and top bands (v rows 0..nb+1
+    and ny-2-nb..ny-1).  band(q) is an Oracle view of band q (0 bottom,
+    1 middle, 2 top) over its own rows; owns(q) flags its pencils (the wall
+    bands' long-mode columns are not theirs and stay zero)."""
+
+    def __init__(self, N1m, N2m, nb=None, band_h=0.25, **cfg):
+        c = Cfg()
+        defaults = dict(grid_kind=0, dyq=10.0, ppp=5, gamma=2.0, bslpu1=0.0,
+                        bslpu3=0.0, flag_ctpress=0, mpgx=0.0, fixed_dt=1,
+                        dt=1e-3, CFL=0.7, alp=1.0, bet=2.0, band_edge=0)
+        defaults.update(cfg)
+        if "Ngal1" not in defaults:
+            defaults["Ngal1"] = 3 * defaults["N1"] // 2
+        if "Ngal2" not in defaults:
+            defaults["Ngal2"] = 3 * defaults["N2"] // 2
+        for k, v in defaults.items():
+            setattr(c, k, v)
+        self.cfg = defaults
+        if nb is None:
+            tmp = Oracle(**defaults)
+            nb = band_split(tmp.grid()["yu"], band_h)
+            del tmp
+        self.nb = nb
+        self.N1m, self.N2m = N1m, N2m
+        Ng1m, Ng2m = 3 * N1m // 2, 3 * N2m // 2
+        self.h = lib().lhob_create(ctypes.byref(c), N1m, N2m, Ng1m, Ng2m, nb)
+        if not self.h:
+            raise ValueError("bands do not fit: nb=%d ny=%d" % (nb, defaults["ny"]))
+        self.bands = []
+        for q in range(3):
+            bc = dict(defaults)
+            if q == 1:
+                bc.update(N1=N1m, N2=N2m, Ngal1=Ng1m, Ngal2=Ng2m)
+            else:
+                bc.update(ny=nb + 2, flag_ctpress=1, mpgx=0.0, band_edge=2 if q == 0 else 1)
+            self.bands.append(_BandView(self, q, lib().lhob_band(self.h, q), bc))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().lhob_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def band(self, q):
+        return self.bands[q]
+
+    def row0(self, q):
+        return lib().lhob_row0(self.h, q)
+
+    def owns(self, q):
+        b = self.bands[q]
+        return np.array([bool(lib().lhob_owns(self.h, q, c)) for c in range(b.ncol)])
+
+    def init_flowrate(self):
+        lib().lhob_init_flowrate(self.h)
+
+    def substep(self, kRK):
+        lib().lhob_substep(self.h, kRK)
+
+    def step(self, n=1):
+        lib().lhob_step(self.h, n)
+
+    def nonlinear(self, kRK):
+        lib().lhob_nonlinear(self.h, kRK)

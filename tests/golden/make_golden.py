@@ -2,7 +2,8 @@
 
 * FFTPACK (cft_buff.f, rft_buff.f) and the tridiagonal solvers
   (tridLU_3D.f90) compiled unmodified from the reference sources into
-  oracle/_ref/libref.so by oracle/build_ref.sh, run on seeded inputs;
+  oracle/_ref/libref.so by oracle/build_ref.sh, run on seeded inputs (one
+  channel, and a banded channel whose short pencils live in the wall bands);
 * the y-grid, flow rate and dtv of the reference's 32x32x153 case, read from
   its input fixture (input/u1_0032x0032x0153_t00000.dat, record 3) and its
   history file (output/hist_0032x0032x0153.dat, iter 0/1 records).
@@ -122,6 +123,103 @@ def make_tridlu():
     np.savez_compressed(os.path.join(HERE, "tridlu.npz"), **out)
 
 
+def band_limits(ny, nb):
+    """N(3,q), N(4,q) of ygrid after the band split (start.f90:990-1060;
+    dny = 0, N(3,nband) = ny - 2) and the row limits jlim(1:2, grid, band)
+    of proc_lims_columns (start.f90:1677-1711), grid = vgrid 1, ugrid 2,
+    pgrid 3 (declaration.f90:15-17)."""
+    N3 = [0, nb, ny - 2 - nb, ny - 2]
+    N4 = [0, N3[1] + 1, N3[2], N3[3] + 1]
+    lims = np.zeros((3, 3, 2), np.int32)  # [band][grid][lo/hi] (C order of jlim(2,3,3))
+    lims[0][1] = (N4[0], N4[1] + 1)
+    lims[1][1] = (N4[0], N4[3] + 1)
+    lims[2][1] = (N4[2], N4[3] + 1)
+    lims[0][0] = (N3[0], N3[1] + 1)
+    lims[1][0] = (N3[0], N3[3] + 1)
+    lims[2][0] = (N3[2], N3[3] + 1)
+    lims[0][2] = (N4[0] + 1, N4[1])
+    lims[1][2] = (N4[0] + 1, N4[3])
+    lims[2][2] = (N4[2] + 1, N4[3])
+    return N3, N4, lims
+
+
+def make_tridlu_bands():
+    """The reference's LUsolU_W / LUsolV / LUsolP on a banded channel: the
+    short pencils of the bottom and top bands over their own rows with the
+    band-edge weights of start.f90:539-544, the long pencils over the whole
+    channel."""
+    out = {}
+    cfg = dict(N1=8, N2=8, ny=33, Re=3250.0, grid_kind=0, dyq=10.0, ppp=5, dt=2e-3)
+    N1m, N2m = 4, 4
+    B = po.BandOracle(N1m, N2m, band_h=0.25, **cfg)
+    nb = B.nb
+    ny = cfg["ny"]
+    g = B.band(1).grid()
+    yu, yv = g["yu"], g["yv"]
+    N3, N4, lims = band_limits(ny, nb)
+    gw = np.zeros((3, 2))
+    gw[0, 0] = -(yv[N3[0]] - yu[N4[0]]) / (yv[N3[0]] - yu[N4[0] + 1])
+    gw[0, 1] = (yu[N4[1] + 1] - yv[N3[1] + 1]) / (yv[N3[1] + 1] - yu[N4[1]])
+    gw[1, 0] = -(yv[N3[0]] - yu[N4[0]]) / (yv[N3[0]] - yu[N4[0] + 1])
+    gw[1, 1] = (yu[N4[3] + 1] - yv[N3[3] + 1]) / (yv[N3[3] + 1] - yu[N4[3]])
+    gw[2, 0] = -(yv[N3[2]] - yu[N4[2]]) / (yv[N3[2]] - yu[N4[2] + 1])
+    gw[2, 1] = (yu[N4[3] + 1] - yv[N3[3] + 1]) / (yv[N3[3] + 1] - yu[N4[3]])
+    # columns: the middle band's long pencils (fine-grid k index), the wall
+    # bands' short pencils, in the oracle's band column order
+    ci, ck, ncb = [], [], []
+    for q in range(3):
+        b = B.band(q)
+        nxh, N2 = b.cfg["N1"] // 2 + 1, b.cfg["N2"]
+        own = B.owns(q)
+        cols = [c for c in range(b.ncol) if own[c]]
+        ncb.append(len(cols))
+        for c in cols:
+            i, k = c % nxh, c // nxh
+            m = k if k < N2 // 2 else k - N2
+            ci.append(i)
+            ck.append((m % cfg["N2"]) + 1)
+        out["b%d_cols" % q] = np.array(cols, np.int32)
+    ci = np.array(ci, np.int32)
+    ck = np.array(ck, np.int32)
+    ncb = np.array(ncb, np.int32)
+    dyu2i = np.ascontiguousarray(g["dyu2i"])
+    dyv2i = np.ascontiguousarray(g["dyv2i"])
+    gwf = np.ascontiguousarray(gw)  # C (band, side) = Fortran gw(side, band)
+    R.ref_setup_bands.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double] + [ctypes.c_void_p] * 11
+    R.ref_setup_bands(cfg["N1"], cfg["N2"], ny, 1.0, 2.0, cfg["Re"], cfg["dt"],
+                      *[a.ctypes.data for a in (yu, yv, g["dthdyu"], g["dthdyv"], dyu2i, dyv2i, gwf,
+                                                np.ascontiguousarray(lims), ncb, ci, ck)])
+    out["nb"] = np.array(nb)
+    out["lims"] = lims
+    out["gw"] = gw
+    rng = np.random.default_rng(11)
+    for q in range(3):
+        n = int(ncb[q])
+        lu, lv, lp = lims[q][1], lims[q][0], lims[q][2]
+        nu, nv, npr = lu[1] - lu[0] + 1, lv[1] - lv[0] + 1, lp[1] - lp[0] + 1
+        for kRK in (1, 3):
+            ru = rng.standard_normal((n, nu)) + 1j * rng.standard_normal((n, nu))
+            rw = rng.standard_normal((n, nu)) + 1j * rng.standard_normal((n, nu))
+            rv = rng.standard_normal((n, nv)) + 1j * rng.standard_normal((n, nv))
+            u = np.zeros_like(ru)
+            w = np.zeros_like(rw)
+            v = np.zeros_like(rv)
+            R.ref_lusol_uw_b(ctypes.c_int(q + 1), ctypes.c_int(kRK), ctypes.c_void_p(u.ctypes.data),
+                             ctypes.c_void_p(ru.ctypes.data), ctypes.c_void_p(w.ctypes.data),
+                             ctypes.c_void_p(rw.ctypes.data))
+            R.ref_lusol_v_b(ctypes.c_int(q + 1), ctypes.c_int(kRK), ctypes.c_void_p(v.ctypes.data),
+                            ctypes.c_void_p(rv.ctypes.data))
+            for nm, a in (("ru", ru), ("rw", rw), ("rv", rv), ("u", u), ("w", w), ("v", v)):
+                out["b%d_k%d_%s" % (q, kRK, nm)] = a
+        rp = rng.standard_normal((n, npr)) + 1j * rng.standard_normal((n, npr))
+        xp = rp.copy()
+        R.ref_lusol_p_b(ctypes.c_int(q + 1), ctypes.c_void_p(xp.ctypes.data))
+        out["b%d_rp" % q] = rp
+        out["b%d_xp" % q] = xp
+    np.savez_compressed(os.path.join(HERE, "tridlu_bands.npz"), **out)
+
+
 def make_reference_case():
     d1 = fieldio.read_field(os.path.join(REF, "input", "u1_0032x0032x0153_t00000.dat"))
     d2 = fieldio.read_field(os.path.join(REF, "input", "u2_0032x0032x0153_t00000.dat"))
@@ -139,6 +237,7 @@ def make_reference_case():
 if __name__ == "__main__":
     make_fft()
     make_tridlu()
+    make_tridlu_bands()
     make_reference_case()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
