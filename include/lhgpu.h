@@ -220,6 +220,30 @@ int lh_dev_free(void *p);
 int lh_copy_to_device(void *d, const void *h, size_t bytes);
 int lh_copy_to_host(void *h, const void *d, size_t bytes);
 
+/* Multi-resolution bands (the reference's nband = 3 blocks, declaration.f90:
+ * 8-13; its MULTIBLOCK layout).  A banded channel is three channels of one
+ * context: its middle band (the long pencils: that channel's N1 x N2 modes
+ * over the whole height, ny v points) and its bottom and top wall bands (the
+ * short pencils: the wall resolution's remaining modes, each over its own
+ * rows, v rows 0..nb+1 and ny-2-nb..ny-1, created with ny = nb + 2,
+ * flag_ctpress = 1, mpgx = 0, more or as many modes in x and z, the same
+ * alp, bet, Re, dt and slip lengths).  lh_set_bands links them: the wall
+ * bands take the middle band's grid rows (their band-edge weights follow,
+ * start.f90:539-559), each band's column phases run over its own rows with
+ * its own edge rows (RHS0, solveU_W, solveV, solveP, v_corr loop over
+ * iband; LUsolU_W / LUsolV / LUsolP tridLU_3D.f90:55-226 with jlim,
+ * start.f90:1677-1711), and every plane transforms at its band's resolution
+ * holding every mode present at its row (modes_to_planes_UVP with dk,
+ * FOU3D.f90:1000-1098; proc_lims_planes start.f90:1836-2142).  In a wall
+ * band's arrays the long pencils' columns are not the band's: read the
+ * middle band for those modes.  Fixed dt; no textures, triad lists or
+ * closure on banded channels.  Replaces the band set-up of
+ * proc_lims_columns/proc_lims_planes (start.f90:1490,1836).
+ * lh_band_rows: nb = N(3,1) of ygrid's band split (start.f90:999-1011), the
+ * first row j with yu(j) >= yu(0) + band_h on the grid of `mid`. */
+int lh_band_rows(const lh_config *mid, double band_h, int *nb);
+int lh_set_bands(lh_ctx *ctx, int ch_bottom, int ch_middle, int ch_top, int nb);
+
 /* Tuning knobs of the launch shapes (no counterpart in the reference).
  * LH_TUNE_SOLVE_COLS: spectral columns per workgroup of the column solve;
  * 0 selects one lane per column streaming rows from HBM, 1..10 the form that

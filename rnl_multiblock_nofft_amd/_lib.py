@@ -45,7 +45,7 @@ EXPORTS = ["lh_last_error", "lh_ctx_create", "lh_ctx_destroy", "lh_set_grid", "l
            "lh_closure_set", "lh_closure_enable", "lh_closure_eval", "lh_closure_sample", "lh_closure_time",
            "lh_mlp_create", "lh_mlp_destroy", "lh_mlp_set_params", "lh_mlp_get_params", "lh_mlp_set_batch",
            "lh_mlp_sample", "lh_mlp_set_points", "lh_mlp_forward", "lh_mlp_grad", "lh_mlp_get_grad",
-           "lh_mlp_adam", "lh_mlp_sync"]
+           "lh_mlp_adam", "lh_mlp_sync", "lh_band_rows", "lh_set_bands"]
 CL_NIN, CL_NHID, CL_NOUT, CL_NPARAM = 135, 128, 3, 50819
 ST_N = 13
 SP_IDS = {"U": 0, "V": 1, "W": 2, "UV": 3, "P": 4}
@@ -83,6 +83,8 @@ def load():
     L.lh_ctx_create.argtypes = [ctypes.POINTER(LhConfig), ip, ctypes.POINTER(vp)]
     L.lh_ctx_destroy.argtypes = [vp]
     L.lh_set_grid.argtypes = [vp, ip, dp, dp, dp, dp]
+    L.lh_band_rows.argtypes = [ctypes.POINTER(LhConfig), ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    L.lh_set_bands.argtypes = [vp, ip, ip, ip, ip]
     L.lh_get_grid.argtypes = [vp, ip, dp, dp, dp, dp]
     L.lh_field_upload.argtypes = [vp, ip, ip, dp]
     L.lh_field_download.argtypes = [vp, ip, ip, dp]

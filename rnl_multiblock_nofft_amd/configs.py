@@ -68,6 +68,21 @@ def fine_128():
     return [channel(128, 128, 153, re_code(2800.0))]
 
 
+def banded(wall, N1m, N2m, nb):
+    """The three channel configurations of a banded channel (lh_set_bands):
+    the bottom wall band, the middle band and the top wall band.  `wall` is
+    the whole channel at the wall bands' resolution; the middle band takes
+    N1m x N2m modes over the same ny, the wall bands the wall's modes over
+    nb + 2 v rows each (v rows 0..nb+1 and ny-2-nb..ny-1; nb from
+    solver.band_rows, ygrid's split of start.f90:999-1011) and no mean mode.
+    Returns [bottom, middle, top]."""
+    mid = dict(wall, N1=N1m, N2=N2m, Ngal1=3 * N1m // 2, Ngal2=3 * N2m // 2)
+    w = dict(wall, ny=nb + 2, flag_ctpress=1, mpgx=0.0)
+    for c in (mid, w):
+        c.pop("texture", None)
+    return [dict(w), mid, dict(w)]
+
+
 def smooth(cfgs):
     """The same channels with smooth walls."""
     out = [dict(c) for c in cfgs]

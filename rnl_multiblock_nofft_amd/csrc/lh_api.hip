@@ -52,6 +52,7 @@ struct HostChan {
   Chan dev;  // offsets into the context arena
   int iter = 0;
   int xinv = 0;  // every textured wall's mask is independent of x
+  int band = -1;  // in a banded channel: 0 bottom wall band, 1 middle band, 2 top wall band
 };
 
 double ipow(double x, int n) {
@@ -242,6 +243,11 @@ struct lh_ctx {
   std::vector<char *> cl_mem;
   std::vector<int> cl_on;
   int ncl = 0;  // channels whose step adds the correction
+  // banded channels (lh_set_bands): their links on the host and device,
+  // and the largest middle band's column count
+  std::vector<BandLink> bands;
+  BandLink *d_bands = nullptr;
+  int band_ncol = 0;
 };
 
 namespace {
@@ -371,6 +377,8 @@ size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
   d.nplu = h.nyu - 2;
   d.nplv = h.nyv - 2;
   d.nslot = h.nslot;
+  d.pu0 = d.pv0 = 1;
+  d.band_edge = 0;
   d.flag_ctpress = c.flag_ctpress;
   d.adaptive = c.adaptive_dt != 0;
   d.CFL = c.CFL;
@@ -489,8 +497,11 @@ bool ghost_deferrable(const lh_ctx *x) {
 // the nonlinear products of every channel: the FFT stages for channels
 // without triad lists, the list kernels for those with
 int raw_nonlinear(lh_ctx *x, int kRK, int ghost = 0, int dtblk = 0) {
+  const int nb = (int)x->bands.size();
+  HIPCHK(launch_band_copy(x->arena, x->d_chans, x->d_bands, nb, x->band_ncol, 0, x->stream));
   if (x->ntri < (int)x->ch.size())
     HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream, kRK, ghost, dtblk));
+  HIPCHK(launch_band_copy(x->arena, x->d_chans, x->d_bands, nb, x->band_ncol, 1, x->stream));
   if (x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
   return LH_OK;
 }
@@ -739,6 +750,7 @@ int lh_ctx_destroy(lh_ctx *x) {
   if (x->graph2) (void)hipGraphExecDestroy(x->graph2);
   if (x->arena) (void)hipFree(x->arena);
   if (x->d_chans) (void)hipFree(x->d_chans);
+  if (x->d_bands) (void)hipFree(x->d_bands);
   if (x->staging) (void)hipFree(x->staging);
   if (x->stats) (void)hipFree(x->stats);
   if (x->hist) (void)hipFree(x->hist);
@@ -770,6 +782,93 @@ int lh_set_grid(lh_ctx *x, int ch, const double *yu, const double *yv, const dou
   rc = push_descriptors(x);
   if (rc) return rc;
   return refresh_tables(x);
+}
+
+static void drop_graph(lh_ctx *x);
+
+int lh_band_rows(const lh_config *mid, double band_h, int *nb) {
+  if (!mid || !nb || !(band_h > 0.0)) return set_err(LH_EINVAL, "lh_band_rows: bad arguments");
+  HostChan h;
+  int rc = setup_host(h, *mid);
+  if (rc) return rc;
+  // ygrid's band split (start.f90:999-1011): the first row with
+  // yu(j) >= yu(0) + h
+  int j = 0;
+  while (j < h.nyu - 1 && h.yu[j] < h.yu[0] + band_h) j++;
+  *nb = j;
+  return LH_OK;
+}
+
+int lh_set_bands(lh_ctx *x, int cb, int cm, int ct, int nb) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  const int n = (int)x->ch.size();
+  if (cb < 0 || cm < 0 || ct < 0 || cb >= n || cm >= n || ct >= n || cb == cm || cb == ct || cm == ct)
+    return set_err(LH_EINVAL, "lh_set_bands: bad channels %d, %d, %d", cb, cm, ct);
+  HostChan &B = x->ch[cb], &M = x->ch[cm], &T = x->ch[ct];
+  if (B.band >= 0 || M.band >= 0 || T.band >= 0)
+    return set_err(LH_EINVAL, "lh_set_bands: a channel is a band already");
+  const int ny = M.cfg.ny;
+  if (nb < 1 || 2 * (nb + 2) + 2 > ny)
+    return set_err(LH_EINVAL, "lh_set_bands: the wall bands (v rows 0..%d and %d..%d) leave no middle band", nb + 1,
+                   ny - 2 - nb, ny - 1);
+  for (HostChan *w : {&B, &T}) {
+    const lh_config &c = w->cfg;
+    if (c.ny != nb + 2)
+      return set_err(LH_EINVAL, "lh_set_bands: a wall band has ny = %d, not nb + 2 = %d", c.ny, nb + 2);
+    if (c.N1 < M.cfg.N1 || c.N2 < M.cfg.N2)
+      return set_err(LH_EINVAL, "lh_set_bands: a wall band has fewer modes than the middle band");
+    if (c.alp != M.cfg.alp || c.bet != M.cfg.bet || c.Re != M.cfg.Re || c.dt != M.cfg.dt ||
+        c.bslpu1 != M.cfg.bslpu1 || c.bslpu3 != M.cfg.bslpu3)
+      return set_err(LH_EINVAL, "lh_set_bands: the bands differ in alp, bet, Re, dt or slip lengths");
+    if (c.flag_ctpress != 1 || c.mpgx != 0.0)
+      return set_err(LH_EINVAL, "lh_set_bands: a wall band holds no mean mode (flag_ctpress 1, mpgx 0)");
+    if (c.adaptive_dt || M.cfg.adaptive_dt) return set_err(LH_EINVAL, "lh_set_bands: banded channels take a fixed dt");
+    if (w->dev.tex || (x->tri_mem.size() > (size_t)(w - x->ch.data()) && x->tri_mem[w - x->ch.data()]))
+      return set_err(LH_EINVAL, "lh_set_bands: a wall band has textured walls or triad lists");
+  }
+  if (M.dev.tex || (x->tri_mem.size() > (size_t)cm && x->tri_mem[cm]))
+    return set_err(LH_EINVAL, "lh_set_bands: the middle band has textured walls or triad lists");
+  for (int c : {cb, cm, ct})
+    if (x->cl_on.size() > (size_t)c && x->cl_on[c])
+      return set_err(LH_EINVAL, "lh_set_bands: channel %d runs the closure", c);
+  HIPCHK(hipStreamSynchronize(x->stream));
+  // the wall bands' grids are the middle band's rows (jlim, start.f90:
+  // 1677-1711): v rows 0..nb+1 and ny-2-nb..ny-1, u rows one more; their
+  // edge weights then follow from their own end rows (start.f90:539-559)
+  const int r0t = ny - 2 - nb;
+  for (int q = 0; q < 2; q++) {
+    HostChan &w = q == 0 ? B : T;
+    const int r0 = q == 0 ? 0 : r0t;
+    w.yu.assign(M.yu.begin() + r0, M.yu.begin() + r0 + w.nyu);
+    w.yv.assign(M.yv.begin() + r0, M.yv.begin() + r0 + w.nyv);
+    w.dthdyu.assign(M.dthdyu.begin() + r0, M.dthdyu.begin() + r0 + w.nyu);
+    w.dthdyv.assign(M.dthdyv.begin() + r0, M.dthdyv.begin() + r0 + w.nyv);
+    derive_grid(w);
+    if ((rc = upload_chan_constants(x, w))) return rc;
+    w.dev.band_edge = q == 0 ? 2 : 1;
+  }
+  // the middle band's planes: u rows nb+2..ny-2-nb, v rows nb+1..ny-2-nb
+  // (proc_lims_planes, start.f90:1940-1965)
+  M.dev.pu0 = nb + 2;
+  M.dev.nplu = ny - 3 - 2 * nb;
+  M.dev.pv0 = nb + 1;
+  M.dev.nplv = ny - 2 - 2 * nb;
+  M.dev.nslot = M.dev.nplu + M.dev.nplv;
+  B.band = 0;
+  M.band = 1;
+  T.band = 2;
+  x->bands.push_back(BandLink{cb, cm, ct, r0t});
+  x->band_ncol = std::max(x->band_ncol, M.ncol);
+  if (x->d_bands) (void)hipFree(x->d_bands);
+  x->d_bands = nullptr;
+  hipError_t e = hipMalloc((void **)&x->d_bands, x->bands.size() * sizeof(BandLink));
+  if (e != hipSuccess) return set_err(LH_ENOMEM, "hipMalloc(band links): %s", hipGetErrorString(e));
+  HIPCHK(hipMemcpy(x->d_bands, x->bands.data(), x->bands.size() * sizeof(BandLink), hipMemcpyHostToDevice));
+  if ((rc = push_descriptors(x))) return rc;
+  if ((rc = refresh_tables(x))) return rc;
+  drop_graph(x);
+  return LH_OK;
 }
 
 int lh_get_grid(lh_ctx *x, int ch, double *yu, double *yv, double *dthdyu, double *dthdyv) {
@@ -951,8 +1050,13 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
   for (int n = 0; n < nsub && rc == LH_OK; n++) {
     HIPCHK(hipEventRecord(ev[0], x->stream));
     for (int st = 0; st < 3; st++) {
+      // the band copies are counted with the first and the last stage
+      if (st == 0)
+        HIPCHK(launch_band_copy(x->arena, x->d_chans, x->d_bands, (int)x->bands.size(), x->band_ncol, 0, x->stream));
       if (x->ntri < (int)x->ch.size())
         HIPCHK(launch_nl_stage(x->arena, x->d_chans, x->shape, x->mixed, st, x->stream, kRK));
+      if (st == 2)
+        HIPCHK(launch_band_copy(x->arena, x->d_chans, x->d_bands, (int)x->bands.size(), x->band_ncol, 1, x->stream));
       // the triad-list kernels are counted with the x-product stage, the
       // closure with the z-forward one
       if (st == 1 && x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
@@ -1693,6 +1797,8 @@ int lh_set_triads(lh_ctx *x, int ch, int nfile, const int *lens, const signed ch
   if (nfile < 0 || (nfile > 0 && (!lens || !rows || !map_u || !map_v)))
     return set_err(LH_EINVAL, "lh_set_triads: bad arguments");
   HostChan &h = x->ch[ch];
+  if (nfile > 0 && h.band >= 0)
+    return set_err(LH_EINVAL, "lh_set_triads: channel %d is a band of a banded channel", ch);
   if (x->tri_mem.size() != x->ch.size()) x->tri_mem.assign(x->ch.size(), nullptr);
   HIPCHK(hipStreamSynchronize(x->stream));
   if (nfile == 0) {
@@ -1795,6 +1901,8 @@ int lh_set_texture(lh_ctx *x, int ch, const int *bottom, const int *top) {
   if (rc) return rc;
   if (ch < 0 || ch >= (int)x->ch.size()) return set_err(LH_EINVAL, "lh_set_texture: bad channel %d", ch);
   HostChan &h = x->ch[ch];
+  if ((bottom || top) && h.band >= 0)
+    return set_err(LH_EINVAL, "lh_set_texture: channel %d is a band of a banded channel", ch);
   // the wall passes transform at lengths N1 and N2 (the N grid of
   // four_to_phys_N), which the LDS transforms support for 2^a 3^b only
   if ((bottom || top) && (!valid_len(h.cfg.N1) || !valid_len(h.cfg.N2)))
@@ -1923,6 +2031,8 @@ int lh_closure_enable(lh_ctx *x, int ch, int on) {
   if (rc) return rc;
   if ((rc = cl_chan_ok(x, ch, "lh_closure_enable"))) return rc;
   if (on < 0 || on > 2) return set_err(LH_EINVAL, "lh_closure_enable: on is 0, 1 or 2");
+  if (on && x->ch[ch].band >= 0)
+    return set_err(LH_EINVAL, "lh_closure_enable: channel %d is a band of a banded channel", ch);
   if (on && !x->cl_net) return set_err(LH_ESTATE, "lh_closure_enable: no network set (lh_closure_set)");
   if (on) {
     float *ph, *co;

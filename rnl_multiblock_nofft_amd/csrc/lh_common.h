@@ -51,7 +51,14 @@ struct Chan {
   int N1, N2, Ng1, Ng2;
   int ny, nyu, nyv, nxh, ncol, ldc;
   int nxp;  // row length of the hz/gx intermediates: nxh rounded up to 16
-  int nplu, nplv, nslot;
+  // plane slots: u-planes j = pu0 .. pu0+nplu-1, then v-planes
+  // j = pv0 .. pv0+nplv-1 (pu0 = pv0 = 1 for a whole channel; the middle
+  // band of a banded channel transforms only its own band's planes)
+  int nplu, nplv, nslot, pu0, pv0;
+  // band edges of a wall band of a banded channel: bit 0 when the first row
+  // is a band edge, bit 1 when the last row is; a band edge keeps its solved
+  // ghost value (v_corr resets only wall ghosts, littleharsh.f90:992-1044)
+  int band_edge;
   int flag_ctpress;
   int adaptive;  // dt = min(2.5 dtv, CFL dtc) at every kRK = 1 (else fixed)
   double CFL;

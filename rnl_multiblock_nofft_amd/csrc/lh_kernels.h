@@ -68,6 +68,16 @@ hipError_t launch_history(char *arena, const Chan *d_chans, const ShapeMax &m, d
 // r1, r2, r3 that launch_rhs stores (smooth-walled channels only)
 hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse = 0);
 hipError_t launch_flowrate(char *arena, const Chan *d_chans, int nchan, int set_target, hipStream_t s);
+// a banded channel: the channels of its bottom wall band, middle band and
+// top wall band, and the top band's first row in the middle band's rows
+struct BandLink {
+  int cb, cm, ct, r0t;
+};
+// out = 0: the long pencils' rows u1, u2, u3, p of the wall bands from the
+// middle band; out = 1: the wall bands' nonlinear terms and fluxes of those
+// pencils back to the middle band
+hipError_t launch_band_copy(char *arena, const Chan *d_chans, const BandLink *d_links, int nlinks, int ncol_mid,
+                            int out, hipStream_t s);
 hipError_t launch_to_rows(const double2 *src, double2 *dst, int ncol, int nrows, int jlo, int ldc,
                           hipStream_t s);
 hipError_t launch_from_rows(const double2 *src, double2 *dst, int ncol, int nrows, int jlo, int ldc,

@@ -152,8 +152,9 @@ __global__ void __launch_bounds__(NT) k_zinv(char *arena, const Chan *chans, int
   // logical order u0 v0 u1 v1 ... (then the last u-plane): u-plane j and
   // v-planes j-1, j interpolate from the same spectral rows
   {
-    const int nplv = ch.nslot - ch.nplu;
-    slot = slot < 2 * nplv ? ((slot & 1) ? ch.nplu + (slot >> 1) : (slot >> 1)) : slot - nplv;
+    const int nplv = ch.nslot - ch.nplu, m = min(ch.nplu, nplv);
+    slot = slot < 2 * m ? ((slot & 1) ? ch.nplu + (slot >> 1) : (slot >> 1))
+                        : (ch.nplu > nplv ? slot - m : ch.nplu + slot - m);
   }
   const int Ng2 = NL > 0 ? NL : ch.Ng2, nxh = ch.nxh, N2 = ch.N2, ldc = ch.ldc, nxp = ch.nxp;
   const int TI = tile_of8(E, Ng2, nxh);
@@ -168,7 +169,7 @@ __global__ void __launch_bounds__(NT) k_zinv(char *arena, const Chan *chans, int
   C *B = A + TI * S;
   C *TW = inplace ? B : B + TI * S;
   const bool isU = slot < ch.nplu;
-  const int j = isU ? slot + 1 : slot - ch.nplu + 1;
+  const int j = isU ? ch.pu0 + slot : ch.pv0 + slot - ch.nplu;
 
   stage_twiddles<C>(TW, v.twz, Ng2);
   // only the zero-padding rows z = N2/2 .. Ngal2-N2/2-1 need clearing
@@ -648,7 +649,7 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
   const int ni = min(TI, nxh - i0);
   const bool isU = slot < ch.nplu;
   const int np = isU ? 4 : 2;
-  const int j = isU ? slot + 1 : slot - ch.nplu + 1;
+  const int j = isU ? ch.pu0 + slot : ch.pv0 + slot - ch.nplu;
   // np ni sequences of Ng2 points, S apart (padded: conflict-free stages);
   // one buffer when the stages run in place
   const int S = seq_stride(Ng2, 4 * TI, sizeof(C));
@@ -817,10 +818,10 @@ __device__ __forceinline__ void zfwd_fs_tile(const View &v, const Chan &ch, int 
   const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * NL * nxp + i0;
   if (slot < ch.nplu)
     zfwd_fs_run<C, NL, 4, NT>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl1, v.nl3, v.fvv,
-                              (size_t)(slot + 1) * ch.ldc + i0);
+                              (size_t)(ch.pu0 + slot) * ch.ldc + i0);
   else
     zfwd_fs_run<C, NL, 2, NT>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl2, v.fuv, v.fwv,
-                              (size_t)(slot - ch.nplu + 1) * ch.ldc + i0);
+                              (size_t)(ch.pv0 + slot - ch.nplu) * ch.ldc + i0);
 }
 // NL: the length every channel of the launch shares, or 0 for a launch
 // mixing the lengths 24, 48 and 96 (each channel takes its own length's
@@ -1305,11 +1306,16 @@ __global__ void __launch_bounds__(64) k_solve(char *arena, const Chan *chans, in
     double2 da = cimul(kxi, u1p), db = cimul(kzk, u3p);
     double w = du[ne];
     STC(dvg, ne, make_double2((da.x + db.x) + w * (0.0 - vpp.x), (da.y + db.y) + w * (0.0 - vpp.y)));
-    // ghost cells and walls (v_corr, littleharsh.f90:992-1044)
-    STC(u1, 0, cscale(ch.gwbc1, u1first));
-    STC(u1, nyu - 1, cscale(ch.gwbc1, u1p));
-    STC(u3, 0, cscale(ch.gwbc3, u3first));
-    STC(u3, nyu - 1, cscale(ch.gwbc3, u3p));
+    // ghost cells and walls (v_corr, littleharsh.f90:992-1044); a band
+    // edge keeps its solved ghost value
+    if (!(ch.band_edge & 1)) {
+      STC(u1, 0, cscale(ch.gwbc1, u1first));
+      STC(u3, 0, cscale(ch.gwbc3, u3first));
+    }
+    if (!(ch.band_edge & 2)) {
+      STC(u1, nyu - 1, cscale(ch.gwbc1, u1p));
+      STC(u3, nyu - 1, cscale(ch.gwbc3, u3p));
+    }
     STC(u2, 0, make_double2(0, 0));
     STC(u2, nyv - 1, make_double2(0, 0));
   }
@@ -1794,12 +1800,11 @@ __global__ void __launch_bounds__(256) k_solve_lds(char *arena, const Chan *chan
     if (c >= ncol) continue;
     const size_t o = (size_t)j * ldc + c;
     double2 n1, n3;
-    if (j == 0) {
-      n1 = cscale(ch.gwbc1, X1[1 * C + cl]);
-      n3 = cscale(ch.gwbc3, X3[1 * C + cl]);
-    } else if (j == nyu - 1) {
-      n1 = cscale(ch.gwbc1, X1[ne * C + cl]);
-      n3 = cscale(ch.gwbc3, X3[ne * C + cl]);
+    if (j == 0 || j == nyu - 1) {  // a band edge keeps its solved ghost value
+      const bool edge = (ch.band_edge >> (j == 0 ? 0 : 1)) & 1;
+      const int jn = j == 0 ? 1 : ne;
+      n1 = edge ? X1[j * C + cl] : cscale(ch.gwbc1, X1[jn * C + cl]);
+      n3 = edge ? X3[j * C + cl] : cscale(ch.gwbc3, X3[jn * C + cl]);
     } else {
       n1 = X1[j * C + cl];
       n3 = X3[j * C + cl];
@@ -2231,11 +2236,11 @@ __global__ void __launch_bounds__(64 * NW) k_solve_wave(char *arena, const Chan 
 #pragma unroll
   for (int q = 0; q < K; q++) {
     const int j = j0 + q;
-    if (j == 0) {
+    if (j == 0 && !(ch.band_edge & 1)) {  // a band edge keeps its solved ghost value
       X1[q] = cscale(ch.gwbc1, u1a);
       X3[q] = cscale(ch.gwbc3, u3a);
     }
-    if (j == nyu - 1) {
+    if (j == nyu - 1 && !(ch.band_edge & 2)) {
       X1[q] = cscale(ch.gwbc1, u1b);
       X3[q] = cscale(ch.gwbc3, u3b);
     }
@@ -2728,6 +2733,60 @@ int solve_cols_default(int nyu) {
   if (C > 10) C = 10;
   if (C < 1 && 1 * b <= 160 * 1024) C = 1;
   return C;
+}
+
+// ------------------------------------------------------------------------
+// Multi-resolution bands.  A banded channel is three channels of one
+// context: the middle band (the long pencils, its N1 x N2 modes over the
+// whole channel, transforming only its own band's planes) and the bottom and
+// top wall bands (their finer N1 x N2 modes over their own rows, with a band
+// edge on one side).  A wall band's planes hold every mode present at their
+// rows (modes_to_planes_UVP with dk, FOU3D.f90:1000-1098): before the
+// nonlinear stage the long pencils' rows are copied into the wall bands'
+// columns of the same (kx, kz) (out = 0), and after it the nonlinear terms
+// and fluxes of those columns' plane rows go back to the middle band
+// (out = 1; planes_to_modes, FOU3D.f90:2295-2442,2772-2854).  In a wall band
+// those columns are no pencils of the band: they carry the copies, and
+// nothing reads the values its column phases leave in them.  p is copied
+// too, so that they stay bounded.
+__global__ void __launch_bounds__(256) k_band_copy(char *arena, const Chan *chans, const BandLink *links, int out) {
+  const BandLink L = links[blockIdx.z >> 1];
+  const bool top = blockIdx.z & 1;
+  const Chan M = chans[L.cm], W = chans[top ? L.ct : L.cb];
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= M.ncol) return;
+  const int r0 = top ? L.r0t : 0;
+  const int i = c % M.nxh, km = c / M.nxh, m = km < M.N2 / 2 ? km : km - M.N2;
+  const int cw = i + W.nxh * (m < 0 ? m + W.N2 : m);
+  const View vm(arena, M), vw(arena, W);
+  for (int r = blockIdx.y; r < W.nyu; r += gridDim.y) {
+    const size_t om = (size_t)(r0 + r) * M.ldc + c, ow = (size_t)r * W.ldc + cw;
+    if (!out) {
+      vw.u1[ow] = vm.u1[om];
+      vw.u3[ow] = vm.u3[om];
+      vw.p[ow] = vm.p[om];
+      if (r < W.nyv) vw.u2[ow] = vm.u2[om];
+    } else {
+      if (r >= 1 && r <= W.nyu - 2) {
+        vm.nl1[om] = vw.nl1[ow];
+        vm.nl3[om] = vw.nl3[ow];
+        vm.fvv[om] = vw.fvv[ow];
+      }
+      if (r >= 1 && r <= W.nyv - 2) {
+        vm.nl2[om] = vw.nl2[ow];
+        vm.fuv[om] = vw.fuv[ow];
+        vm.fwv[om] = vw.fwv[ow];
+      }
+    }
+  }
+}
+
+hipError_t launch_band_copy(char *arena, const Chan *d_chans, const BandLink *d_links, int nlinks, int ncol_mid,
+                            int out, hipStream_t s) {
+  if (nlinks <= 0) return hipSuccess;
+  dim3 g((ncol_mid + 255) / 256, 32, 2 * nlinks);
+  hipLaunchKernelGGL(k_band_copy, g, dim3(256), 0, s, arena, d_chans, d_links, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_flowrate(char *arena, const Chan *d_chans, int nchan, int set_target, hipStream_t s) {

@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) k_triad(char *arena, const Chan *chans) {
   if (!isU && ti >= 4) return;
   const View v(arena, ch);
   const int type = isU ? 2 * ti + 1 : 2 * ti + 2;
-  const int j = isU ? slot + 1 : slot - ch.nplu + 1;
+  const int j = isU ? ch.pu0 + slot : ch.pv0 + slot - ch.nplu;
   const int ncol = ch.ncol, nxh = ch.nxh, N1 = ch.N1, N2 = ch.N2;
   // passive / active fields of each type (FOU3D.f90:1821-1825, 1953-1956)
   //   u: 1 uu (u1,u1) 3 uw (u3,u1) 5 vv (v,v) 7 wu (u1,u3) 9 ww (u3,u3)
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) k_triad_fin(char *arena, const Chan *chan
   if (c >= ch.ncol) return;
   const View v(arena, ch);
   const bool isU = slot < ch.nplu;
-  const int j = isU ? slot + 1 : slot - ch.nplu + 1;
+  const int j = isU ? ch.pu0 + slot : ch.pv0 + slot - ch.nplu;
   const int ncol = ch.ncol, i = c % ch.nxh, k = c / ch.nxh;
   const bool live = k != ch.N2 / 2;
   const double kx = v.kx[i], kz = v.kz[k];

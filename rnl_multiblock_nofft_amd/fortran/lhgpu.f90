@@ -69,6 +69,21 @@ module lhgpu
       real(c_double), intent(in) :: yu(*), yv(*), dthdyu(*), dthdyv(*)
       integer(c_int) :: rc
     end function
+    ! multi-resolution bands: nb of ygrid's band split, and the link of a
+    ! banded channel's three channels (bottom wall band, middle, top)
+    function lh_band_rows(mid, band_h, nb) bind(C, name='lh_band_rows') result(rc)
+      import :: lh_config, c_int, c_double
+      type(lh_config), intent(in) :: mid
+      real(c_double), value :: band_h
+      integer(c_int), intent(out) :: nb
+      integer(c_int) :: rc
+    end function
+    function lh_set_bands(ctx, ch_bottom, ch_middle, ch_top, nb) bind(C, name='lh_set_bands') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ch_bottom, ch_middle, ch_top, nb
+      integer(c_int) :: rc
+    end function
     function lh_get_grid(ctx, ch, yu, yv, dthdyu, dthdyv) bind(C, name='lh_get_grid') result(rc)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

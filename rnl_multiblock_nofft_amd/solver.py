@@ -24,6 +24,25 @@ def _ncol(cfg):
     return (cfg["N1"] // 2 + 1) * cfg["N2"]
 
 
+def _config(c, precision=_lib.PREC_FP64):
+    s = LhConfig()
+    for f, _ in LhConfig._fields_:
+        if f == "precision":
+            s.precision = precision
+        elif f in c:
+            setattr(s, f, c[f])
+    return s
+
+
+def band_rows(cfg, band_h):
+    """nb = N(3,1) of ygrid's band split (start.f90:999-1011) on the grid of
+    the channel configuration `cfg`: the last v row of the bottom band."""
+    L = _lib.load()
+    nb = ctypes.c_int()
+    check(L.lh_band_rows(ctypes.byref(_config(cfg)), float(band_h), ctypes.byref(nb)))
+    return nb.value
+
+
 class Solver:
     """`cfgs`: list of channel dicts (see configs.channel).  precision:
     "fp64" (plain) or "mixed"."""
@@ -75,6 +94,11 @@ class Solver:
         dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         check(self.L.lh_get_grid(self.h, ch, dp(yu), dp(yv), dp(du), dp(dv)))
         return dict(yu=yu, yv=yv, dthdyu=du, dthdyv=dv)
+
+    def set_bands(self, ch_bottom, ch_middle, ch_top, nb):
+        """Link three channels into one banded channel (lh_set_bands)."""
+        check(self.L.lh_set_bands(self.h, ch_bottom, ch_middle, ch_top, nb))
+        self.bands = getattr(self, "bands", []) + [(ch_bottom, ch_middle, ch_top, nb)]
 
     def set_grid(self, ch, yu, yv, dthdyu, dthdyv):
         arrs = [np.ascontiguousarray(a, np.float64) for a in (yu, yv, dthdyu, dthdyv)]
