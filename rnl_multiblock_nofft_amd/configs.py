@@ -68,6 +68,16 @@ def fine_128():
     return [channel(128, 128, 153, re_code(2800.0))]
 
 
+def banded_256():
+    """The 256x256x257 channel (Re_b = 10000, smooth walls) with
+    multi-resolution bands: 256x256 modes in wall bands of height 0.25
+    (ygrid's split, start.f90:999-1011) and 128x128 modes in the middle
+    band."""
+    from .solver import band_rows
+    wall = channel(256, 256, 257, re_code(10000.0))
+    return banded(wall, 128, 128, band_rows(wall, 0.25))
+
+
 def banded(wall, N1m, N2m, nb):
     """The three channel configurations of a banded channel (lh_set_bands):
     the bottom wall band, the middle band and the top wall band.  `wall` is
@@ -80,7 +90,10 @@ def banded(wall, N1m, N2m, nb):
     w = dict(wall, ny=nb + 2, flag_ctpress=1, mpgx=0.0)
     for c in (mid, w):
         c.pop("texture", None)
-    return [dict(w), mid, dict(w)]
+    out = [dict(w), mid, dict(w)]
+    for q, c in enumerate(out):
+        c["band"] = (q, nb)  # Solver links the three (lh_set_bands)
+    return out
 
 
 def smooth(cfgs):
@@ -93,7 +106,7 @@ def smooth(cfgs):
 
 WORKLOADS = {"smooth_32x32x153": smooth_32, "ridges_64x64x153": ridges_64,
              "ensemble_48x32_16x64": ensemble_64, "large_256x256x257": large_256,
-             "fine_128x128x153": fine_128}
+             "fine_128x128x153": fine_128, "banded_256x256x257_mid128": banded_256}
 
 
 def reference_input_case():

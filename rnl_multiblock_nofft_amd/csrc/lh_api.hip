@@ -209,6 +209,14 @@ template <typename T> size_t al(size_t n) { return (n * sizeof(T) + 255) / 256 *
 
 }  // namespace
 
+// channels whose nonlinear stages launch together: one plane shape, so
+// that every stage runs its length's specialised code
+struct NlGroup {
+  ShapeMax m;  // the members' dimensions (the context's knobs are applied per launch)
+  std::vector<int> members;
+  Chan *d = nullptr;  // the members' descriptors
+};
+
 struct lh_ctx {
   std::vector<HostChan> ch;
   bool mixed = false;
@@ -248,6 +256,10 @@ struct lh_ctx {
   std::vector<BandLink> bands;
   BandLink *d_bands = nullptr;
   int band_ncol = 0;
+  // launch groups of the nonlinear stages (empty: one launch covers every
+  // channel); a context with banded channels mixes two plane shapes, and
+  // its groups keep each launch specialised
+  std::vector<NlGroup> nlg;
 };
 
 namespace {
@@ -404,7 +416,102 @@ int push_descriptors(lh_ctx *x) {
   std::vector<Chan> v;
   for (auto &h : x->ch) v.push_back(h.dev);
   HIPCHK(hipMemcpy(x->d_chans, v.data(), v.size() * sizeof(Chan), hipMemcpyHostToDevice));
+  for (auto &g : x->nlg) {
+    std::vector<Chan> gv;
+    for (int c : g.members) gv.push_back(x->ch[c].dev);
+    HIPCHK(hipMemcpy(g.d, gv.data(), gv.size() * sizeof(Chan), hipMemcpyHostToDevice));
+  }
   return LH_OK;
+}
+
+// a group's launch shape with the knobs of `k`
+ShapeMax group_shape(const NlGroup &g, const ShapeMax &k) {
+  ShapeMax m = g.m;
+  m.e_zinv = k.e_zinv;
+  m.e_xprod = k.e_xprod;
+  m.e_zfwd = k.e_zfwd;
+  m.xprod_ip = k.xprod_ip;
+  m.zfwd_ip = k.zfwd_ip;
+  m.zinv_ip = k.zinv_ip;
+  m.zfwd_fs = k.zfwd_fs;
+  m.xprod_threads = k.xprod_threads;
+  m.ghost_zinv = k.ghost_zinv;
+  shape_tiles(m);
+  return m;
+}
+
+// one nonlinear stage over every channel, with the knobs of `k`
+hipError_t nl_stage(lh_ctx *x, const ShapeMax &k, int st, int kRK, int ghost = 0, int dtblk = 0) {
+  if (x->nlg.empty()) return launch_nl_stage(x->arena, x->d_chans, k, x->mixed, st, x->stream, kRK, ghost, dtblk);
+  for (auto &g : x->nlg) {
+    hipError_t e = launch_nl_stage(x->arena, g.d, group_shape(g, k), x->mixed, st, x->stream, kRK, ghost, dtblk);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// the right-hand sides and the column solve over every channel, with the
+// knobs of `k` (per group where there are groups: their grids then fit
+// each group's columns and rows)
+ShapeMax col_shape(const NlGroup &g, const ShapeMax &k) {
+  ShapeMax m = group_shape(g, k);
+  m.solve_cols = k.solve_cols;
+  m.solve_scan = k.solve_scan;
+  m.solve_waves = k.solve_waves;
+  m.rhs_fused = k.rhs_fused;
+  return m;
+}
+hipError_t rhs_stage(lh_ctx *x, const ShapeMax &k, int kRK) {
+  if (x->nlg.empty()) return launch_rhs(x->arena, x->d_chans, k, kRK, x->stream);
+  for (auto &g : x->nlg) {
+    hipError_t e = launch_rhs(x->arena, g.d, col_shape(g, k), kRK, x->stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+hipError_t solve_stage(lh_ctx *x, const ShapeMax &k, int kRK, int fuse = 0) {
+  if (x->nlg.empty()) return launch_solve(x->arena, x->d_chans, k, kRK, x->stream, fuse);
+  for (auto &g : x->nlg) {
+    hipError_t e = launch_solve(x->arena, g.d, col_shape(g, k), kRK, x->stream, fuse);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// the field-by-field z-forward form is on for some launch of these knobs
+bool nl_fs_on(lh_ctx *x, const ShapeMax &k) {
+  if (x->nlg.empty()) return zfwd_fs_on(k);
+  for (auto &g : x->nlg)
+    if (zfwd_fs_on(group_shape(g, k))) return true;
+  return false;
+}
+
+// groups channels of equal plane shape (contexts with banded channels)
+int build_nl_groups(lh_ctx *x) {
+  for (auto &g : x->nlg)
+    if (g.d) (void)hipFree(g.d);
+  x->nlg.clear();
+  if (x->bands.empty()) return LH_OK;
+  for (int c = 0; c < (int)x->ch.size(); c++) {
+    const lh_config &cf = x->ch[c].cfg;
+    NlGroup *g = nullptr;
+    for (auto &q : x->nlg)
+      if (q.m.dims[0].Ng1 == cf.Ngal1 && q.m.dims[0].Ng2 == cf.Ngal2) g = &q;
+    if (!g) {
+      x->nlg.emplace_back();
+      g = &x->nlg.back();
+    }
+    g->members.push_back(c);
+    shape_add(g->m, cf.N1, cf.N2, cf.Ngal1, cf.Ngal2, cf.ny);
+  }
+  for (auto &g : x->nlg) {
+    hipError_t e = hipMalloc((void **)&g.d, g.members.size() * sizeof(Chan));
+    if (e != hipSuccess) {
+      g.d = nullptr;
+      return set_err(LH_ENOMEM, "hipMalloc(group descriptors): %s", hipGetErrorString(e));
+    }
+  }
+  return push_descriptors(x);
 }
 
 // rows of a field in the reference layout, device array, jlo
@@ -477,7 +584,7 @@ void tex_classify(lh_ctx *x) {
 int solve_phase(lh_ctx *x, int kRK, int fuse = 0, bool defer_ghost = false) {
   int rc = tex_part(x, kRK, 0);
   if (rc) return rc;
-  HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream, fuse));
+  HIPCHK(solve_stage(x, x->shape, kRK, fuse));
   return tex_part(x, kRK, 1, defer_ghost);
 }
 
@@ -500,7 +607,7 @@ int raw_nonlinear(lh_ctx *x, int kRK, int ghost = 0, int dtblk = 0) {
   const int nb = (int)x->bands.size();
   HIPCHK(launch_band_copy(x->arena, x->d_chans, x->d_bands, nb, x->band_ncol, 0, x->stream));
   if (x->ntri < (int)x->ch.size())
-    HIPCHK(launch_nonlinear(x->arena, x->d_chans, x->shape, x->mixed, x->stream, kRK, ghost, dtblk));
+    for (int st = 0; st < 3; st++) HIPCHK(nl_stage(x, x->shape, st, kRK, st == 0 ? ghost : 0, st == 2 ? dtblk : 0));
   HIPCHK(launch_band_copy(x->arena, x->d_chans, x->d_bands, nb, x->band_ncol, 1, x->stream));
   if (x->ntri) HIPCHK(launch_triads(x->arena, x->d_chans, x->shape, x->stream));
   return LH_OK;
@@ -592,7 +699,7 @@ int run_substep(lh_ctx *x, int kRK, bool in_step = false, bool prev_ghost = fals
   // the textured-wall sources read r1 and r3 before the solve; without
   // textured walls the solve computes the right-hand sides itself
   const int fuse = x->shape.rhs_fused && !x->ntex;
-  if (!fuse) HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
+  if (!fuse) HIPCHK(rhs_stage(x, x->shape, kRK));
   return solve_phase(x, kRK, fuse, defer && (kRK < 3 || keep_last));
 }
 
@@ -751,6 +858,8 @@ int lh_ctx_destroy(lh_ctx *x) {
   if (x->arena) (void)hipFree(x->arena);
   if (x->d_chans) (void)hipFree(x->d_chans);
   if (x->d_bands) (void)hipFree(x->d_bands);
+  for (auto &g : x->nlg)
+    if (g.d) (void)hipFree(g.d);
   if (x->staging) (void)hipFree(x->staging);
   if (x->stats) (void)hipFree(x->stats);
   if (x->hist) (void)hipFree(x->hist);
@@ -865,7 +974,7 @@ int lh_set_bands(lh_ctx *x, int cb, int cm, int ct, int nb) {
   hipError_t e = hipMalloc((void **)&x->d_bands, x->bands.size() * sizeof(BandLink));
   if (e != hipSuccess) return set_err(LH_ENOMEM, "hipMalloc(band links): %s", hipGetErrorString(e));
   HIPCHK(hipMemcpy(x->d_bands, x->bands.data(), x->bands.size() * sizeof(BandLink), hipMemcpyHostToDevice));
-  if ((rc = push_descriptors(x))) return rc;
+  if ((rc = build_nl_groups(x))) return rc;
   if ((rc = refresh_tables(x))) return rc;
   drop_graph(x);
   return LH_OK;
@@ -998,7 +1107,7 @@ int lh_run_phase(lh_ctx *x, int phase, int kRK) {
       if (r) return r;
       break;
     }
-    case LH_PHASE_RHS: HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream)); break;
+    case LH_PHASE_RHS: HIPCHK(rhs_stage(x, x->shape, kRK)); break;
     case LH_PHASE_SOLVE: {
       int r = solve_phase(x, kRK);
       if (r) return r;
@@ -1053,8 +1162,7 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
       // the band copies are counted with the first and the last stage
       if (st == 0)
         HIPCHK(launch_band_copy(x->arena, x->d_chans, x->d_bands, (int)x->bands.size(), x->band_ncol, 0, x->stream));
-      if (x->ntri < (int)x->ch.size())
-        HIPCHK(launch_nl_stage(x->arena, x->d_chans, x->shape, x->mixed, st, x->stream, kRK));
+      if (x->ntri < (int)x->ch.size()) HIPCHK(nl_stage(x, x->shape, st, kRK));
       if (st == 2)
         HIPCHK(launch_band_copy(x->arena, x->d_chans, x->d_bands, (int)x->bands.size(), x->band_ncol, 1, x->stream));
       // the triad-list kernels are counted with the x-product stage, the
@@ -1065,11 +1173,11 @@ int lh_time_kernels(lh_ctx *x, int nsub, double *ms) {
     }
     if (rc) break;
     if ((rc = time_step(x, kRK))) break;
-    HIPCHK(launch_rhs(x->arena, x->d_chans, x->shape, kRK, x->stream));
+    HIPCHK(rhs_stage(x, x->shape, kRK));
     HIPCHK(hipEventRecord(ev[4], x->stream));
     if ((rc = tex_part(x, kRK, 0))) break;
     HIPCHK(hipEventRecord(ev[5], x->stream));
-    HIPCHK(launch_solve(x->arena, x->d_chans, x->shape, kRK, x->stream));
+    HIPCHK(solve_stage(x, x->shape, kRK));
     HIPCHK(hipEventRecord(ev[6], x->stream));
     if ((rc = tex_part(x, kRK, 1))) break;
     HIPCHK(hipEventRecord(ev[7], x->stream));
@@ -1724,10 +1832,10 @@ int lh_autotune(lh_ctx *x, int reps) {
         }
         if (st == 0) m.zinv_ip = ip;
         shape_tiles(m);
-        if (fs && !zfwd_fs_on(m)) continue;
+        if (fs && !nl_fs_on(x, m)) continue;
         if (stage_lds_bytes(x, m) > 160 * 1024) continue;
         float ms = time_launches(
-            x, e0, e1, reps, [&](int) { return launch_nl_stage(x->arena, x->d_chans, m, x->mixed, st, x->stream); });
+            x, e0, e1, reps, [&](int) { return nl_stage(x, m, st, 1); });
         if (ms >= 0.f && ms < best_ms) {
           best_ms = ms;
           *stage_knob(best, keys[st]) = E;
@@ -1764,10 +1872,10 @@ int lh_autotune(lh_ctx *x, int reps) {
             float ms = time_launches(x, e0, e1, reps, [&](int r) {
               const int kRK = 1 + r % 3;
               if (!fu) {
-                hipError_t e = launch_rhs(x->arena, x->d_chans, m, kRK, x->stream);
+                hipError_t e = rhs_stage(x, m, kRK);
                 if (e != hipSuccess) return e;
               }
-              return launch_solve(x->arena, x->d_chans, m, kRK, x->stream, fu);
+              return solve_stage(x, m, kRK, fu);
             });
             if (ms >= 0.f && ms < best_ms) {
               best_ms = ms;

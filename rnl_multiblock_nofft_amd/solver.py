@@ -70,6 +70,16 @@ class Solver:
         for q, c in enumerate(self.cfgs):
             if c.get("texture") is not None:
                 self.set_texture(q, *c["texture"])
+        # banded channels from configs.banded: three consecutive channels
+        # marked (0, nb), (1, nb), (2, nb)
+        q = 0
+        while q + 2 < n:
+            b = [self.cfgs[q + r].get("band") for r in range(3)]
+            if b[0] is not None and b[0][0] == 0 and b[1] == (1, b[0][1]) and b[2] == (2, b[0][1]):
+                self.set_bands(q, q + 1, q + 2, b[0][1])
+                q += 3
+            else:
+                q += 1
 
     def close(self):
         if getattr(self, "h", None):

@@ -32,8 +32,8 @@ def banded_pair(wall, N1m, N2m, seed=1234, extra=()):
     short pencils."""
     nb = band_rows(wall, 0.25)
     cfgs = configs.banded(wall, N1m, N2m, nb) + list(extra)
-    s = Solver(cfgs)
-    s.set_bands(0, 1, 2, nb)
+    s = Solver(cfgs)  # links the three bands (configs.banded marks them)
+    assert s.bands == [(0, 1, 2, nb)]
     B = po.BandOracle(N1m, N2m, nb=nb, **ocfg(wall))
     M = B.band(1)
     f = initial.perturbed_poiseuille(cfgs[1], s.grid(1), seed, kmax=min(3, N1m // 2 - 1))
