@@ -51,14 +51,20 @@ def algorithmic_bytes(cfg, precision):
     ncol = nxh * N2
     nyu, nyv = ny + 1, ny
     nplu, nplv = nyu - 2, nyv - 2
+    npu, npv = nplu, nplv  # planes the nonlinear stages transform
+    if cfg.get("band", (None,))[0] == 1:
+        # the middle band of a banded channel transforms its own planes only
+        # (u rows nb+2 .. ny-nb-2, v rows nb+1 .. ny-nb-2)
+        nb = cfg["band"][1]
+        npu, npv = ny - 3 - 2 * nb, ny - 2 - 2 * nb
     row = ncol * 16
     es = 8 if precision == "mixed" else 16
     plane = Ng2 * nxh * es
-    hz = (nplu + nplv) * 3 * plane
-    gx = (nplu * 4 + nplv * 2) * plane
+    hz = (npu + npv) * 3 * plane
+    gx = (npu * 4 + npv * 2) * plane
     zinv = (2 * nyu + nyv) * row + hz
     xprod = hz + gx
-    zfwd = gx + (3 * nplu + 3 * nplv) * row
+    zfwd = gx + (3 * npu + 3 * npv) * row
     rhs = ((2 * nyu + nyv + (nyu - 2)) + (2 * nplu + nplv) * 2 + (2 * nyv + nplu)) * row \
         + (2 * nyu + nyv + 2 * nplu + nplv) * row
     # reads r1, r3, r2, p and the pivots of the U/W, V and Poisson operators
