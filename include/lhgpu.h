@@ -282,7 +282,7 @@ int lh_set_bands(lh_ctx *ctx, int ch_bottom, int ch_middle, int ch_top, int nb);
 /* LH_TUNE_ZFWD_INPLACE (default 0): the same for the z-forward stage. */
 #define LH_TUNE_ZFWD_INPLACE 9
 /* LH_TUNE_XPROD_THREADS (default 256): threads per workgroup of the
- * x-product stage, 128 or 256. */
+ * x-product stage, 128, 192 or 256. */
 #define LH_TUNE_XPROD_THREADS 10
 /* LH_TUNE_RHS_FUSED (default 1): for contexts without textured walls, the
  * column solve of lh_step/lh_substep computes the right-hand sides itself

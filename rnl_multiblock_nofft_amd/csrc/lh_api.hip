@@ -1634,7 +1634,8 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     return LH_OK;
   }
   if (key == LH_TUNE_XPROD_THREADS) {
-    if (value != 128 && value != 256) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_XPROD_THREADS is 128 or 256");
+    if (value != 128 && value != 192 && value != 256)
+      return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_XPROD_THREADS is 128, 192 or 256");
     ShapeMax m = x->shape;
     m.xprod_threads = value;
     shape_tiles(m);
@@ -1814,11 +1815,12 @@ int lh_autotune(lh_ctx *x, int reps) {
   for (int st = 0; st < 3; st++) {
     float best_ms = 1e30f;
     for (int E : pts)
-      for (int v2 = 0; v2 < (st == 0 ? 3 : 4); v2++) {
+      for (int v2 = 0; v2 < (st == 0 ? 3 : st == 1 ? 6 : 4); v2++) {
         // z-inverse: in place in workgroups of 256 or 512 threads; x-product:
-        // in place x 256/128 threads; z-forward: in place, or field-wise in
-        // workgroups of 256 or 512 threads
-        const int ip = st == 0 ? v2 : (v2 & 1) || (st == 2 && v2 >= 2), nt = (st == 1 && (v2 >> 1)) ? 128 : 256;
+        // in place x 256/128/192 threads; z-forward: in place, or field-wise
+        // in workgroups of 256 or 512 threads
+        const int ip = st == 0 ? v2 : (v2 & 1) || (st == 2 && v2 >= 2);
+        const int nt = st == 1 ? (v2 >> 1 == 0 ? 256 : v2 >> 1 == 1 ? 128 : 192) : 256;
         const int fs = st == 2 && v2 >= 2 ? v2 - 1 : 0;
         ShapeMax m = best;
         *stage_knob(m, keys[st]) = E;

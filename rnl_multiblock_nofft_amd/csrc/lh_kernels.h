@@ -25,7 +25,7 @@ struct ShapeMax {
   int zfwd_ip = 0;      // z-forward stage: the same
   int zinv_ip = 0;      // z-inverse stage: the same
   int zfwd_fs = 0;      // z-forward stage: fields one after another in one LDS buffer, wider tiles
-  int xprod_threads = 256;  // x-product stage: threads per workgroup (128 or 256)
+  int xprod_threads = 256;  // x-product stage: threads per workgroup (128, 192 or 256)
   int rhs_fused = 1;   // smooth-walled ensembles: right-hand sides computed inside the column solve
   int ghost_zinv = 1;  // in a captured step, ghost rows of x-independent textures done by k_zinv's extra workgroups
   int nphys = 0;       // N1*N2, largest over channels (texture mask kernels)

@@ -331,7 +331,7 @@ __device__ __forceinline__ C *xprod_fused_ip(C *A, const C *TW, int S, int nz, b
   typedef typename CT<C>::R R_t;
   constexpr int L = plan_len(N), R = plan_at(N, L - 1), NB = N / R;
   bf_stages_ip<C, N, +1, false, 0, L - 1>(A, 2 * nz, S, TW);
-  const int t = threadIdx.x, jj = t / nz, zz = t - jj * nz;
+  const int t = threadIdx.x, jj = div_small(t, nz), zz = t - jj * nz;
   const bool has = t < nz * NB;
   C pp[R], qq[R];
   if (has) {

@@ -44,6 +44,8 @@ CASES = {
 
 
 TALL = {"N16_ny257": small_cfg(16, 257)}
+# the 384-point transforms of a 256x256 channel on a short y grid
+SHORT = {"N256_ny9": small_cfg(256, 9)}
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -261,11 +263,16 @@ def test_zfwd_fieldwise_mixed(mode):
         assert e <= 1e-5, (nm, e)
 
 
-@pytest.mark.parametrize("threads,ip,points", [(128, 0, 768), (128, 1, 384), (128, 1, 768), (128, 0, 2048)])
-def test_xprod_threads(threads, ip, points):
-    """The x-product stage in workgroups of 128 threads (two-buffer and
-    in-place forms) gives the oracle's step."""
-    cfg = CASES["N32_ny153"]
+@pytest.mark.parametrize("threads,ip,points,case", [(128, 0, 768, "N32_ny153"), (128, 1, 384, "N32_ny153"),
+                                                    (128, 1, 768, "N32_ny153"), (128, 0, 2048, "N32_ny153"),
+                                                    (192, 1, 576, "N32_ny153"), (192, 0, 768, "N32_ny153"),
+                                                    (192, 1, 768, "N256_ny9"), (192, 0, 1536, "N256_ny9"),
+                                                    (256, 1, 768, "N256_ny9")])
+def test_xprod_threads(threads, ip, points, case):
+    """The x-product stage in workgroups of 128, 192 or 256 threads
+    (two-buffer and in-place forms; 48- and 384-point lengths) gives the
+    oracle's step."""
+    cfg = CASES[case] if case in CASES else SHORT[case]
     s, o, f = make_pair(cfg, seed=12)
     s.set_tuning(_lib.TUNE_XPROD_POINTS, points)
     s.set_tuning(_lib.TUNE_XPROD_THREADS, threads)
