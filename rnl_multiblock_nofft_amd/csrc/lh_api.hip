@@ -364,6 +364,7 @@ size_t layout_chan(lh_ctx *x, HostChan &h, size_t base) {
   int ng2max = 0;
   for (const ChanDims &q : x->shape.dims) ng2max = std::max(ng2max, q.Ng2);
   d.o_cfl = take(al<double>(2 * (size_t)h.nslot * ng2max));
+  d.o_pmean = take(al<double>(2 * (size_t)h.nslot));
   if (x->mixed) {
     d.o_hzf = take(al<float2>(3 * ninter));
     d.o_gxf = take(al<float2>(4 * ninter));

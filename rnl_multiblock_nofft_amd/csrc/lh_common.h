@@ -89,6 +89,9 @@ struct Chan {
   size_t o_mask, o_wtab, o_texu, o_texw, o_iscal;
   size_t o_twzN, o_twxN;  // N2- and N1-point twiddles of the N-grid
   size_t o_cfl;           // [slot][tile][2] CFL maxima of k_xprod's workgroups
+  // [slot][2] the plane means of u1 and u3 (the (0,0) modes, interpolated on
+  // v-planes) that the FP32 stages of mixed precision leave out
+  size_t o_pmean;
   // statistics accumulators, byte offsets into the context's separate
   // statistics buffer (allocated on first use): prof[ST_*][nyu] row profiles,
   // spec[SP_*][nyu][ldc] per-mode spectra (doubles) and the wall maps of
@@ -122,7 +125,7 @@ struct View {
   float2 *hzf, *gxf;
   double *fac;
   int *mask, *iscal;
-  double *wtab, *cfl;
+  double *wtab, *cfl, *pmean;
   double2 *texu, *texw;
   __host__ __device__ View(char *A, const Chan &c)
       : yu((const double *)(A + c.o_yu)), yv((const double *)(A + c.o_yv)),
@@ -140,7 +143,7 @@ struct View {
         scal((double *)(A + c.o_scal)), hz((double2 *)(A + c.o_hz)), gx((double2 *)(A + c.o_gx)),
         hzf((float2 *)(A + c.o_hzf)), gxf((float2 *)(A + c.o_gxf)), fac((double *)(A + c.o_fac)),
         mask((int *)(A + c.o_mask)), iscal((int *)(A + c.o_iscal)), wtab((double *)(A + c.o_wtab)),
-        cfl((double *)(A + c.o_cfl)),
+        cfl((double *)(A + c.o_cfl)), pmean((double *)(A + c.o_pmean)),
         texu((double2 *)(A + c.o_texu)), texw((double2 *)(A + c.o_texw)) {}
 };
 

@@ -204,6 +204,10 @@ __global__ void __launch_bounds__(NT) k_zinv(char *arena, const Chan *chans, int
     }
   }
   const int half = N2 / 2, dz = Ng2 - N2;
+  // mixed precision: u1 and u3 enter the FP32 transforms without their plane
+  // means (at[u] == 0 is the point z = 0 of the x-mode i0 + 0; with i0 == 0
+  // that is the (0,0) mode)
+  const bool split = sizeof(C) == 8 && f != 1 && i0 == 0;
   {
     constexpr int LU = 4;  // loads in flight per thread before the LDS stores
     const int total = N2 * ni;
@@ -224,9 +228,15 @@ __global__ void __launch_bounds__(NT) k_zinv(char *arena, const Chan *chans, int
 #pragma unroll
       for (int u = 0; u < LU; u++) {
         if (at[u] < 0) continue;
-        double2 v = va[u];
-        if (s1) v = make_double2((wa * v.x + wb * vb[u].x) / wd, (wa * v.y + wb * vb[u].y) / wd);
-        A[at[u]] = from_d<C>(v);
+        double2 vv = va[u];
+        if (s1) vv = make_double2((wa * vv.x + wb * vb[u].x) / wd, (wa * vv.y + wb * vb[u].y) / wd);
+        if (split && at[u] == 0) {
+          // the (0,0) mode of u1 or u3: the plane mean, kept in FP64 for the
+          // z-forward stage; the FP32 stages carry the fluctuation
+          v.pmean[2 * slot + (f >> 1)] = vv.x;
+          vv = make_double2(0, 0);
+        }
+        A[at[u]] = from_d<C>(vv);
       }
     }
   }
@@ -273,9 +283,11 @@ __global__ void __launch_bounds__(NT) k_zinv(char *arena, const Chan *chans, int
 // the points the reversed plan's first item reads); the remaining forward
 // stages run through LDS.  Output sequence zz nseq + q.  Returns the buffer
 // with the forward result.
+// (CFL maxima: m1, m3 are the plane means of u1 and u3 that the mixed
+// precision's FP32 points leave out, zero in FP64)
 template <typename C, bool CFL, int N>
 __device__ __forceinline__ C *xprod_fused(C *A, C *B, const C *TW, int S, int nz, bool isU, typename CT<C>::R &ma,
-                                          typename CT<C>::R &mb) {
+                                          typename CT<C>::R &mb, typename CT<C>::R m1, typename CT<C>::R m3) {
   typedef typename CT<C>::R R_t;
   constexpr int L = plan_len(N), R = plan_at(N, L - 1), NB = N / R;
   C *X = bf_stages<C, N, +1, false, 0, L - 1>(A, B, 2 * nz, S, TW);
@@ -293,8 +305,8 @@ __device__ __forceinline__ C *xprod_fused(C *A, C *B, const C *TW, int S, int nz
       for (int r = 0; r < R; r++) {
         const R_t a1 = a[r].x, a3 = a[r].y, a2 = b[r].x;
         if (CFL) {
-          ma = fmax(ma, fabs(a1));
-          mb = fmax(mb, fabs(a3));
+          ma = fmax(ma, fabs(a1 + m1));
+          mb = fmax(mb, fabs(a3 + m3));
         }
         pp[r] = CT<C>::mk(a1 * a1, a1 * a3);  // uu + i uw
         qq[r] = CT<C>::mk(a2 * a2, a3 * a3);  // vv + i ww
@@ -327,7 +339,7 @@ __host__ __device__ inline bool xprod_inplace(int N, int tz, int nt) {
 }
 template <typename C, bool CFL, int N>
 __device__ __forceinline__ C *xprod_fused_ip(C *A, const C *TW, int S, int nz, bool isU, typename CT<C>::R &ma,
-                                             typename CT<C>::R &mb) {
+                                             typename CT<C>::R &mb, typename CT<C>::R m1, typename CT<C>::R m3) {
   typedef typename CT<C>::R R_t;
   constexpr int L = plan_len(N), R = plan_at(N, L - 1), NB = N / R;
   bf_stages_ip<C, N, +1, false, 0, L - 1>(A, 2 * nz, S, TW);
@@ -343,8 +355,8 @@ __device__ __forceinline__ C *xprod_fused_ip(C *A, const C *TW, int S, int nz, b
       const R_t a1 = a[r].x, a3 = a[r].y, a2 = b[r].x;
       if (isU) {
         if (CFL) {
-          ma = fmax(ma, fabs(a1));
-          mb = fmax(mb, fabs(a3));
+          ma = fmax(ma, fabs(a1 + m1));
+          mb = fmax(mb, fabs(a3 + m3));
         }
         pp[r] = CT<C>::mk(a1 * a1, a1 * a3);  // uu + i uw
         qq[r] = CT<C>::mk(a2 * a2, a3 * a3);  // vv + i ww
@@ -448,20 +460,22 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
   // CFL maxima of dtc_calc (FOU3D.f90:699-736), at the first substep only:
   // |u1|, |u3| on u-planes, |u2| on v-planes (times dthdyv(j) in k_dt_update)
   R_t ma = 0, mb = 0;
+  const R_t m1 = sizeof(C) == 8 && isU ? (R_t)v.pmean[2 * slot] : 0,
+            m3 = sizeof(C) == 8 && isU ? (R_t)v.pmean[2 * slot + 1] : 0;
   C *Y;
   // (the in-place path is its own instantiation: its registers would lower
   // the occupancy of the two-buffer path)
   switch (inplace ? Ng1 : -Ng1) {
-#define LH_IP(n) Y = IP ? xprod_fused_ip<C, CFL, n>(A, TW, S, nz, isU, ma, mb) : nullptr
+#define LH_IP(n) Y = IP ? xprod_fused_ip<C, CFL, n>(A, TW, S, nz, isU, ma, mb, m1, m3) : nullptr
     case 24: LH_IP(24); break;
     case 48: LH_IP(48); break;
     case 96: LH_IP(96); break;
     case 384: LH_IP(384); break;
 #undef LH_IP
-    case -24: Y = xprod_fused<C, CFL, 24>(A, B, TW, S, nz, isU, ma, mb); break;
-    case -48: Y = xprod_fused<C, CFL, 48>(A, B, TW, S, nz, isU, ma, mb); break;
-    case -96: Y = xprod_fused<C, CFL, 96>(A, B, TW, S, nz, isU, ma, mb); break;
-    case -384: Y = xprod_fused<C, CFL, 384>(A, B, TW, S, nz, isU, ma, mb); break;
+    case -24: Y = xprod_fused<C, CFL, 24>(A, B, TW, S, nz, isU, ma, mb, m1, m3); break;
+    case -48: Y = xprod_fused<C, CFL, 48>(A, B, TW, S, nz, isU, ma, mb, m1, m3); break;
+    case -96: Y = xprod_fused<C, CFL, 96>(A, B, TW, S, nz, isU, ma, mb, m1, m3); break;
+    case -384: Y = xprod_fused<C, CFL, 384>(A, B, TW, S, nz, isU, ma, mb, m1, m3); break;
     default: {
       C *X = fft_lds_bf<C>(A, B, Ng1, 2 * nz, S, +1, TW);
       C *Yp = X == A ? B : A;
@@ -472,8 +486,8 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
         const R_t a1 = s0.x, a3 = s0.y, a2 = s1.x;
         if (CFL) {
           if (isU) {
-            ma = fmax(ma, fabs(a1));
-            mb = fmax(mb, fabs(a3));
+            ma = fmax(ma, fabs(a1 + m1));
+            mb = fmax(mb, fabs(a3 + m3));
           } else {
             ma = fmax(ma, fabs(a2));
           }
@@ -537,6 +551,45 @@ __global__ void __launch_bounds__(256) k_xprod(char *arena, const Chan *chans, i
 }
 
 // ------------------------------------------------------------------------
+// Mixed precision: the FP32 stages transform u1 - U1 and u3 - U3, U1 and U3
+// the plane means (k_zinv), so the products they form are the fluctuation
+// products.  The rest of each product is linear in the fields' modes and is
+// added here in FP64, per output mode (k, i) of the plane:
+//   u-planes  uu: + 2 U1 u1'   uw: + U1 u3' + U3 u1'   ww: + 2 U3 u3'
+//   v-planes  uv: + U1 u2      wv: + U3 u2
+// (u' = the modes without (0,0); the constant U1^2, U1 U3, U3^2 sit in the
+// (0,0) mode, which Nu1 and Nu3 multiply by kx = kz = 0).  The FP32 part is
+// then small next to the products' leading terms, and the digits the FP32
+// stages would lose on the mean flow are carried in FP64.
+struct MeanLin {
+  const double2 *a, *b;  // u-planes: u1, u3 at the output's offsets; v-planes: u2
+  double U1, U3;
+  bool on, isU;
+  __device__ static MeanLin make(const View &v, const Chan &ch, int slot, bool on) {
+    MeanLin m;
+    m.on = on;
+    m.isU = slot < ch.nplu;
+    m.a = m.isU ? v.u1 : v.u2;
+    m.b = v.u3;
+    m.U1 = on ? v.pmean[2 * slot] : 0.0;
+    m.U3 = on ? v.pmean[2 * slot + 1] : 0.0;
+    return m;
+  }
+  // the linear term of product p (u: X0..X3, v: X0, X1) at field offset o;
+  // c0: the output is the (0,0) mode
+  __device__ __forceinline__ double2 term(int p, size_t o, bool c0) const {
+    if (isU) {
+      if (p == 2) return make_double2(0, 0);
+      const double2 a1 = c0 ? make_double2(0, 0) : a[o], a3 = c0 ? make_double2(0, 0) : b[o];
+      if (p == 0) return cscale(2 * U1, a1);
+      if (p == 1) return cadd(cscale(U1, a3), cscale(U3, a1));
+      return cscale(2 * U3, a3);
+    }
+    return cscale(p == 0 ? U1 : U3, a[o]);
+  }
+};
+
+// ------------------------------------------------------------------------
 // K3 outputs of one (kz index k, x-mode i0 + ii): u-planes Nu1 = ikx X0 +
 // ikz X1, Nu3 = ikx X1 + ikz X3, fvv = X2; v-planes Nu2 = ikx X0 + ikz X1,
 // fuv = X0, fwv = X1 (der_x/der_z FOU3D.f90:476-593; kz = -N2/2 zeroed)
@@ -546,9 +599,16 @@ struct ZOut {
   size_t row;             // j * ldc
   int nxh, i0, N2, Ng2;
   bool isU;
+  MeanLin ml;
   __device__ __forceinline__ void put(int k, int ii, double2 X0, double2 X1, double2 X2, double2 X3) const {
     const int i = i0 + ii;
     const size_t o = row + (size_t)k * nxh + i;
+    if (ml.on) {
+      const bool c0 = k == 0 && i == 0;
+      X0 = cadd(X0, ml.term(0, o, c0));
+      X1 = cadd(X1, ml.term(1, o, c0));
+      if (isU) X3 = cadd(X3, ml.term(3, o, c0));
+    }
     const bool live = k != N2 / 2;
     const double kxi = kx[i], kzk = kz[k];
     if (isU) {
@@ -683,7 +743,7 @@ __global__ void __launch_bounds__(256) k_zfwd(char *arena, const Chan *chans, in
   }
   __syncthreads();
   ZOut zo{v.kx, v.kz, isU ? v.nl1 : v.nl2, isU ? v.nl3 : v.fuv, isU ? v.fvv : v.fwv, (size_t)j * ldc, nxh, i0,
-          N2, Ng2, isU};
+          N2, Ng2, isU, MeanLin::make(v, ch, slot, sizeof(C) == 8)};
   // (a register-fused last stage, as in k_zinv, needs two sequences' points
   // live per item; at FP64 that cost occupancy more than it saved)
   C *R = A;
@@ -748,7 +808,8 @@ __host__ __device__ inline bool zfwd_fs_fits(int N, int N2, int ti, int nt) {
 // ZOut (u: nl1, nl3, fvv; v: nl2, fuv, fwv), orow = j ldc + i0; NT threads
 template <typename C, int NL, int NP, int NT>
 __device__ __forceinline__ void zfwd_fs_run(const View &v, C *A, const C *TW, const C *gx, int ni, int S, int nxp,
-                                            int nxh, int i0, double2 *o0, double2 *o1, double2 *o2, size_t orow) {
+                                            int nxh, int i0, double2 *o0, double2 *o1, double2 *o2, size_t orow,
+                                            const MeanLin &ml) {
   constexpr int Ng2 = NL, N2 = 2 * NL / 3, TM = fs_ti_max(NL, NT);
   constexpr int LU = (TM * Ng2 + NT - 1) / NT, OU = (TM * N2 + NT - 1) / NT;  // loads, outputs per thread
   constexpr int half = N2 / 2, dz = Ng2 - N2;
@@ -765,6 +826,7 @@ __device__ __forceinline__ void zfwd_fs_run(const View &v, C *A, const C *TW, co
   }
   const double inv = 1.0 / Ng2;
   double2 sq[OU];  // partial sums ikx X0 (u-planes then ikx X1)
+  double2 lt[OU];  // mixed precision: the field mode of the next linear mean term
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     const C *gp = gx + (size_t)p * Ng2 * nxp;
@@ -780,8 +842,28 @@ __device__ __forceinline__ void zfwd_fs_run(const View &v, C *A, const C *TW, co
       const int t = tid + q * NT;
       if (t < ni * N2) {
         const int k = t / ni, ii = t - k * ni;
-        const double2 X = cscale(inv, to_d(A[ii * S + (k < half ? k : k + dz)]));
         const size_t o = orow + (size_t)k * nxh + ii;
+        double2 X = cscale(inv, to_d(A[ii * S + (k < half ? k : k + dz)]));
+        if (ml.on) {
+          // MeanLin::term with each field mode read once: lt holds u1 (u-planes,
+          // fields 0-1) then u3 (fields 1-3), or u2 (v-planes)
+          const bool c0 = NP == 4 && k == 0 && i0 + ii == 0;
+          if (NP == 4) {
+            if (p == 0) {
+              lt[q] = c0 ? make_double2(0, 0) : ml.a[o];
+              X = cadd(X, cscale(2 * ml.U1, lt[q]));
+            } else if (p == 1) {
+              const double2 a3 = c0 ? make_double2(0, 0) : ml.b[o];
+              X = cadd(X, cadd(cscale(ml.U1, a3), cscale(ml.U3, lt[q])));
+              lt[q] = a3;
+            } else if (p == 3) {
+              X = cadd(X, cscale(2 * ml.U3, lt[q]));
+            }
+          } else {
+            if (p == 0) lt[q] = ml.a[o];
+            X = cadd(X, cscale(p == 0 ? ml.U1 : ml.U3, lt[q]));
+          }
+        }
         const bool live = k != half;
         const double kxi = v.kx[i0 + ii], kzk = v.kz[k];
         if (NP == 4) {  // Nu1 = ikx X0 + ikz X1, Nu3 = ikx X1 + ikz X3, fvv = X2
@@ -816,12 +898,13 @@ __device__ __forceinline__ void zfwd_fs_tile(const View &v, const Chan &ch, int 
   C *TW = A + TI * S;
   stage_twiddles<C>(TW, v.twz, NL);  // published by the first field's barrier
   const C *gx = Inter<C>::gx(v) + (size_t)slot * 4 * NL * nxp + i0;
+  const MeanLin ml = MeanLin::make(v, ch, slot, sizeof(C) == 8);
   if (slot < ch.nplu)
     zfwd_fs_run<C, NL, 4, NT>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl1, v.nl3, v.fvv,
-                              (size_t)(ch.pu0 + slot) * ch.ldc + i0);
+                              (size_t)(ch.pu0 + slot) * ch.ldc + i0, ml);
   else
     zfwd_fs_run<C, NL, 2, NT>(v, A, TW, gx, ni, S, nxp, nxh, i0, v.nl2, v.fuv, v.fwv,
-                              (size_t)(ch.pv0 + slot - ch.nplu) * ch.ldc + i0);
+                              (size_t)(ch.pv0 + slot - ch.nplu) * ch.ldc + i0, ml);
 }
 // NL: the length every channel of the launch shares, or 0 for a launch
 // mixing the lengths 24, 48 and 96 (each channel takes its own length's

@@ -416,8 +416,7 @@ def test_ghost_pass_in_zinv(n, precision, zinv_ip):
     steps of one call) give the oracle's steps and the separate launch's
     steps (the same arithmetic), also with the z-inverse stage in 512-thread
     workgroups.  In mixed precision the z-inverse stage is FP32 and the ghost
-    pass FP64; the oracle bound there is the mixed mode's 1e-5 (1e-4 for p,
-    which carries ~10x the velocity round-off)."""
+    pass FP64; the oracle bound there is the mixed mode's 1e-5."""
     cfg = small_cfg(n, 33)
     m = textures.stripes(n, n, 4, 0.5)
     tol = 1e-11 if precision == "fp64" else 1e-5
@@ -434,7 +433,7 @@ def test_ghost_pass_in_zinv(n, precision, zinv_ip):
             for nm in out[g]:
                 e = relerr(out[g][nm], o.get(nm))
                 say("ghost in zinv", n, precision, nm, e)
-                assert e <= (10 * tol if nm == "p" and precision == "mixed" else tol), (nm, e)
+                assert e <= tol, (nm, e)
     for nm in out[1]:
         e = relerr(out[1][nm], out[0][nm])
         say("ghost in zinv vs separate", n, nm, e)
@@ -595,6 +594,10 @@ def test_zfwd_fieldwise_mixed_lengths(mode, points, zinv_ip):
 
 
 def test_mixed_precision_error():
+    """Mixed precision on the smooth 32x32x153 channel: <= 1e-5 per field
+    against the FP64 oracle after one step, and the bars the FP64 plane-mean
+    terms make possible (velocities <= 1e-8, p <= 1e-6; with the means in the
+    FP32 transforms the step gives u3 4.5e-9, p 6.0e-6)."""
     cfg = configs.smooth_32()[0]
     say("mixed")
     s, o, f = make_pair(cfg, precision="mixed")
@@ -604,6 +607,7 @@ def test_mixed_precision_error():
         e = relerr(s.download(0, nm), o.get(nm))
         say("mixed", nm, e)
         assert e <= 1e-5, (nm, e)
+        assert e <= (1e-6 if nm == "p" else 1e-8), (nm, e)
 
 
 @pytest.mark.parametrize("knobs", [dict(xprod_ip=1, xprod_threads=256, zfwd_ip=0, xpts=768, zpts=384),
@@ -613,7 +617,7 @@ def test_mixed_precision_launch_forms(knobs):
     """Mixed precision with the in-place and 128-thread stage forms the
     autotuner can pick: same bound as the default forms against the FP64
     oracle, and the default forms' result to FP32 rounding (velocities 1e-7;
-    p, whose mixed-precision error is itself ~6e-6, 1e-5)."""
+    p, whose mixed-precision error is itself ~2e-7, 1e-6)."""
     cfg = configs.smooth_32()[0]
     s, o, f = make_pair(cfg, precision="mixed")
     ref = Solver([cfg], precision="mixed")
@@ -633,7 +637,7 @@ def test_mixed_precision_launch_forms(knobs):
         d = relerr(s.download(0, nm), ref.download(0, nm))
         say("mixed forms", knobs, nm, e, d)
         assert e <= 1e-5, (nm, e)
-        assert d <= (1e-5 if nm == "p" else 1e-7), (nm, d)
+        assert d <= (1e-6 if nm == "p" else 1e-7), (nm, d)
     ref.close()
 
 
