@@ -65,6 +65,9 @@ def algorithmic_bytes(cfg, precision):
     zinv = (2 * nyu + nyv) * row + hz
     xprod = hz + gx
     zfwd = gx + (3 * npu + 3 * npv) * row
+    if precision == "mixed":
+        # the FP64 modes of the plane-mean terms: u1, u3 on u-planes, u2 on v-planes
+        zfwd += (2 * npu + npv) * row
     rhs = ((2 * nyu + nyv + (nyu - 2)) + (2 * nplu + nplv) * 2 + (2 * nyv + nplu)) * row \
         + (2 * nyu + nyv + 2 * nplu + nplv) * row
     # reads r1, r3, r2, p and the pivots of the U/W, V and Poisson operators
@@ -227,15 +230,21 @@ def main():
     cfgs = configs.WORKLOADS[args.workload]()
     if args.smooth:
         cfgs = configs.smooth(cfgs)
-    s = Solver(cfgs, precision=args.precision)
-    for q, c in enumerate(cfgs):
-        if c.get("band", (1,))[0] != 1:
-            continue  # a wall band of a banded channel starts from zero short pencils
-        kmax = min(8, c["N1"] // 2 - 1, c["N2"] // 2 - 1)
-        f = initial.perturbed_poiseuille(c, s.grid(q), 1234 + q, kmax=kmax)
-        for name in ("u1", "u2", "u3", "p"):
-            s.upload(q, name, f[name])
-    s.init_flowrate()
+
+    def start(precision):
+        """a solver of the workload holding the seeded initial fields"""
+        s = Solver(cfgs, precision=precision)
+        for q, c in enumerate(cfgs):
+            if c.get("band", (1,))[0] != 1:
+                continue  # a wall band of a banded channel starts from zero short pencils
+            kmax = min(8, c["N1"] // 2 - 1, c["N2"] // 2 - 1)
+            f = initial.perturbed_poiseuille(c, s.grid(q), 1234 + q, kmax=kmax)
+            for name in ("u1", "u2", "u3", "p"):
+                s.upload(q, name, f[name])
+        s.init_flowrate()
+        return s
+
+    s = start(args.precision)
     tri_files = {}
     if args.triads:
         for q, c in enumerate(cfgs):
@@ -306,6 +315,28 @@ def main():
     step_alg = sum(alg[k] for k in kernels)
     hbm_pct_alg = 100.0 * step_alg * sub_per_s / (HBM_PEAK_GBS * 1e9)
 
+    mixed_err = None
+    if args.precision == "mixed":
+        # one step of the same seeded fields in mixed precision and in FP64
+        # (default launch shapes): relative L2 per field, the largest over
+        # the channels
+        import numpy as np
+        pair = [start("mixed"), start("fp64")]
+        for x in pair:
+            x.step(1)
+            x.sync()
+        mixed_err = {}
+        for name in ("u1", "u2", "u3", "p"):
+            e = 0.0
+            for q in range(len(cfgs)):
+                a, b = pair[0].download(q, name), pair[1].download(q, name)
+                nb = float(np.linalg.norm(b))
+                if nb > 0:
+                    e = max(e, float(np.linalg.norm(a - b)) / nb)
+            mixed_err[name] = float("%.3g" % e)
+        for x in pair:
+            x.close()
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(max(cfgs, key=lambda c: c["N1"] * c["N2"] * c["ny"]), triads_mode=args.triads or None)
@@ -324,6 +355,8 @@ def main():
                            hbm_pct_of_peak_alg_bytes=round(hbm_pct_alg, 3),
                            tuning=tuning),
                roofline=roofline, cpu_baseline=cpu)
+    if mixed_err is not None:
+        rec["config"]["mixed_error_vs_fp64_one_step"] = mixed_err
     print(json.dumps(rec), flush=True)
     s.close()
 
