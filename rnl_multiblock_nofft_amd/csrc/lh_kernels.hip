@@ -1008,11 +1008,18 @@ __global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int
   // XCD-aware order: tiles (column block, row j) go round-robin to the 8
   // XCDs; hand each XCD a contiguous run so that rows j-1, j, j+1 of a column
   // block are read through one L2.  gridDim.x is gx*gy rounded up to 8.
+  // Within a run the tiles go in groups of RG column blocks, rows outer:
+  // tile (b, j+1) follows tile (b, j) RG tiles later, after ~RG x 52 KB of
+  // other reads, so the rows it shares with (b, j) are still in the 4 MB L2
+  // (row-major over all column blocks, that distance is a whole row of every
+  // field, megabytes at 256 x 256, and the shared rows came from HBM again).
+  constexpr int RG = 16;
   const int per = gridDim.x / 8;
   const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (tile >= gx * gy) return;
-  const int j = tile / gx;
-  const int c = (tile - j * gx) * blockDim.x + threadIdx.x;
+  const int cg = tile / (RG * gy), rem = tile - cg * RG * gy, w = min(RG, gx - cg * RG);
+  const int j = rem / w;
+  const int c = (cg * RG + rem - j * w) * blockDim.x + threadIdx.x;
   if (j >= ch.nyu || c >= ch.ncol) return;
   double2 r1, r3, r2;
   rhs_point(v, ch, kRK, j, c, true, r1, r3, r2);
