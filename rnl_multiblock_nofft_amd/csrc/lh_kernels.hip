@@ -647,10 +647,12 @@ __device__ __forceinline__ void dt_update_block(char *arena, const Chan &ch, int
   const int TZ = tile_of(e_xprod, ch.Ng1, ch.Ng2), nt = (ch.Ng2 + TZ - 1) / TZ;
   const double2 *cm = reinterpret_cast<const double2 *>(v.cfl);
   const int nu = ch.nplu * nt;
-#pragma unroll 4
-  for (int q = threadIdx.x; q < nu; q += blockDim.x) {
-    const int slot = q / nt, tz = q - slot * nt;
-    const double2 c = cm[(size_t)slot * ntiles_max + tz];
+  Walk2 w(threadIdx.x, blockDim.x, nt);  // q = slot nt + tz
+  // one workgroup reads every tile's pair: keep 16 loads in flight per
+  // thread (the loop is bound by memory latency, not bandwidth)
+#pragma unroll 16
+  for (int q = threadIdx.x; q < nu; q += blockDim.x, w.next()) {
+    const double2 c = cm[(size_t)w.hi * ntiles_max + w.lo];
     m1 = fmax(m1, c.x);
     m3 = fmax(m3, c.y);
   }
