@@ -309,6 +309,13 @@ int lh_set_bands(lh_ctx *ctx, int ch_bottom, int ch_middle, int ch_top, int nb);
  * share one of the lengths 24, 48, 96, 192, 384 or all have lengths 24, 48
  * or 96; else the stage runs as before. */
 #define LH_TUNE_ZFWD_FIELDWISE 14
+/* LH_TUNE_TEX_FUSED (default 1): textured walls with x-varying masks run
+ * each field's wall-source iteration in one workgroup with the wall planes
+ * resident in LDS when they fit (160 KiB); 0 always runs the wall
+ * transforms as separate launches (z transforms, then the x transforms and
+ * the mask in one launch), as channels too large for LDS do.  Results agree
+ * to rounding. */
+#define LH_TUNE_TEX_FUSED 15
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

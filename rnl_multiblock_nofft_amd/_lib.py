@@ -28,6 +28,7 @@ TUNE_RHS_FUSED = 11
 TUNE_GHOST_IN_ZINV = 12
 TUNE_ZINV_INPLACE = 13
 TUNE_ZFWD_FIELDWISE = 14
+TUNE_TEX_FUSED = 15
 PREC_FP64, PREC_MIXED = 0, 1
 
 # every symbol include/lhgpu.h declares

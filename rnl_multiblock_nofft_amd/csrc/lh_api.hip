@@ -1625,6 +1625,13 @@ static void drop_graph(lh_ctx *x) {
 int lh_set_tuning(lh_ctx *x, int key, int value) {
   int rc = check_ctx(x);
   if (rc) return rc;
+  if (key == LH_TUNE_TEX_FUSED) {
+    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_TEX_FUSED is 0 or 1");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->tex_fused = value;
+    drop_graph(x);
+    return LH_OK;
+  }
   if (key == LH_TUNE_TEX_SEPARABLE) {
     if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_TEX_SEPARABLE is 0 or 1");
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1721,6 +1728,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
   int rc = check_ctx(x);
   if (rc) return rc;
   if (!value) return set_err(LH_EINVAL, "lh_get_tuning: null value");
+  if (key == LH_TUNE_TEX_FUSED) {
+    *value = x->tex_fused;
+    return LH_OK;
+  }
   if (key == LH_TUNE_TEX_SEPARABLE) {
     *value = x->tex_sep;
     return LH_OK;

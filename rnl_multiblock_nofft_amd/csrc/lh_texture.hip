@@ -162,9 +162,34 @@ __global__ void __launch_bounds__(256) k_tex_pre(char *arena, const Chan *chans,
   }
 }
 
-// pointwise mask on the physical N-grid planes of texw; grid (points, 4, ch)
+// the pointwise mask of field f, wall w at one physical point (fs: the point
+// is free-shear)
 // mode 0: v <- P v / (1 - gbar P)   (wall sources)
 // mode 1: v <- (1 - gwbc) v on free-shear points, 0 elsewhere (ghost rows)
+struct TexMask {
+  double g0, gw, gwbc;
+  int mode;
+  __device__ TexMask(const View &v, const Chan &ch, int kRK, int mode_, int f, int w) : mode(mode_) {
+    g0 = 0.0;
+    if (mode == 0) {
+      const int n = ch.nyu, ldc = ch.ldc;
+      const size_t tab = (size_t)n * ldc;
+      const double *Wb = v.wtab + (size_t)((kRK - 1) * 2) * tab, *Wt = Wb + tab;
+      g0 = w == 0 ? Wb[0] : Wt[(size_t)(n - 1) * ldc];
+    }
+    gw = w == 0 ? ch.gw1 : ch.gw2;
+    gwbc = f == 0 ? ch.gwbc1 : ch.gwbc3;
+  }
+  __device__ __forceinline__ double operator()(double val, bool fs) const {
+    if (mode == 0) {
+      const double Pw = fs ? 1.0 + gw : 0.0;
+      return Pw * val / (1.0 - g0 * Pw);
+    }
+    return fs ? (1.0 - gwbc) * val : 0.0;
+  }
+};
+
+// pointwise mask on the physical N-grid planes of texw; grid (points, 4, ch)
 __global__ void __launch_bounds__(256) k_tex_mask(char *arena, const Chan *chans, int kRK, int mode) {
   const Chan ch = chans[blockIdx.z];
   if (!ch.tex || ch.tex1d) return;
@@ -175,17 +200,81 @@ __global__ void __launch_bounds__(256) k_tex_mask(char *arena, const Chan *chans
   const int p = blockIdx.y, f = p >> 1, w = p & 1;
   const int z = t / N1, x = t - z * N1;
   double *P = reinterpret_cast<double *>(v.texw + (size_t)p * ch.ncol);
-  const bool fs = v.mask[w * N1 * N2 + t] == 2;
+  const TexMask mk(v, ch, kRK, mode, f, w);
   double &val = P[(size_t)z * (N1 + 2) + x];
-  if (mode == 0) {
-    const int n = ch.nyu, ldc = ch.ldc;
-    const size_t tab = (size_t)n * ldc;
-    const double *Wb = v.wtab + (size_t)((kRK - 1) * 2) * tab, *Wt = Wb + tab;
-    const double g0 = w == 0 ? Wb[0] : Wt[(size_t)(n - 1) * ldc];
-    const double Pw = fs ? 1.0 + (w == 0 ? ch.gw1 : ch.gw2) : 0.0;
-    val = Pw * val / (1.0 - g0 * Pw);
-  } else {
-    val = fs ? (1.0 - (f == 0 ? ch.gwbc1 : ch.gwbc3)) * val : 0.0;
+  val = mk(val, v.mask[w * N1 * N2 + t] == 2);
+}
+
+// The x transforms of a mask pass and the mask in one launch: per pair of
+// z rows of a wall plane, the inverse real x transform (rft packing two
+// rows in one complex FFT, exactly as k_plane_x), the pointwise mask of
+// k_tex_mask and the forward real x transform, the physical rows staying in
+// LDS; the same arithmetic as the three launches, one HBM round trip and
+// two launches less.  Grid (row-pair tiles, 4 planes, channels).
+__host__ __device__ inline int tex_xmask_tile(int N1) {
+  const int t = 1024 / N1;
+  return t < 1 ? 1 : t;
+}
+__global__ void __launch_bounds__(256) k_tex_xmask(char *arena, const Chan *chans, int kRK, int mode) {
+  const Chan ch = chans[blockIdx.z];
+  if (!ch.tex || ch.tex1d) return;
+  const View v(arena, ch);
+  const int Ng1 = ch.N1, Ng2 = ch.N2, igal = Ng1 + 2, h = Ng1 / 2;
+  const int TZ = tex_xmask_tile(Ng1);  // pairs of rows per workgroup
+  const int z0 = blockIdx.x * 2 * TZ;
+  if (z0 >= Ng2) return;
+  const int p = blockIdx.y, f = p >> 1, w = p & 1;
+  double *P = reinterpret_cast<double *>(v.texw + (size_t)p * ch.ncol);
+  const int nzr = min(2 * TZ, Ng2 - z0), npair = (nzr + 1) / 2;
+  extern __shared__ double2 smem_d[];
+  double2 *A = smem_d, *B = A + TZ * Ng1, *TW = B + TZ * Ng1;
+  stage_twiddles<double2>(TW, reinterpret_cast<const double2 *>(arena + ch.o_twxN), Ng1);
+  for (int t = threadIdx.x; t < npair * Ng1; t += blockDim.x) {
+    const int q = t / Ng1, m = t - q * Ng1;
+    const int za = z0 + 2 * q, zb = za + 1;
+    const int mm = m <= h ? m : Ng1 - m;
+    const double sg = m <= h ? 1.0 : -1.0;
+    const double *ra = P + (size_t)za * igal;
+    double2 a = make_double2(ra[2 * mm], sg * ra[2 * mm + 1]), b = make_double2(0, 0);
+    if (mm == 0 || mm == h) a.y = 0.0;
+    if (zb < z0 + nzr) {
+      const double *rb = P + (size_t)zb * igal;
+      b = make_double2(rb[2 * mm], sg * rb[2 * mm + 1]);
+      if (mm == 0 || mm == h) b.y = 0.0;
+    }
+    A[q * Ng1 + m] = make_double2(a.x - b.y, a.y + b.x);
+  }
+  __syncthreads();
+  double2 *R = fft_lds<double2>(A, B, Ng1, npair, Ng1, +1, TW);
+  const TexMask mk(v, ch, kRK, mode, f, w);
+  const int *fsm = v.mask + w * Ng1 * Ng2;
+  for (int t = threadIdx.x; t < npair * Ng1; t += blockDim.x) {
+    const int q = t / Ng1, n = t - q * Ng1;
+    const int za = z0 + 2 * q, zb = za + 1;
+    const double2 x = R[q * Ng1 + n];
+    R[q * Ng1 + n] = make_double2(mk(x.x, fsm[za * Ng1 + n] == 2), zb < z0 + nzr ? mk(x.y, fsm[zb * Ng1 + n] == 2) : 0.0);
+  }
+  __syncthreads();
+  double2 *Y = fft_lds<double2>(R, R == A ? B : A, Ng1, npair, Ng1, -1, TW);
+  const double inv = 1.0 / Ng1;
+  for (int t = threadIdx.x; t < npair * (h + 1); t += blockDim.x) {
+    const int q = t / (h + 1), m = t - q * (h + 1);
+    const int za = z0 + 2 * q, zb = za + 1;
+    const double2 Ym = Y[q * Ng1 + m], Yc = Y[q * Ng1 + (m == 0 ? 0 : Ng1 - m)];
+    double2 Pa, Pb;
+    if (m == 0 || m == h) {
+      Pa = make_double2(0.5 * (Ym.x + Yc.x) * inv, 0.0);
+      Pb = make_double2(0.5 * (Ym.y + Yc.y) * inv, 0.0);
+    } else {
+      Pa = make_double2(0.5 * (Ym.x + Yc.x) * inv, 0.5 * (Ym.y - Yc.y) * inv);
+      Pb = make_double2(0.5 * (Ym.y + Yc.y) * inv, -0.5 * (Ym.x - Yc.x) * inv);
+    }
+    P[(size_t)za * igal + 2 * m] = Pa.x;
+    P[(size_t)za * igal + 2 * m + 1] = Pa.y;
+    if (zb < z0 + nzr) {
+      P[(size_t)zb * igal + 2 * m] = Pb.x;
+      P[(size_t)zb * igal + 2 * m + 1] = Pb.y;
+    }
   }
 }
 
@@ -239,16 +328,29 @@ hipError_t launch_tex_tables(char *arena, const Chan *d_chans, const ShapeMax &m
   return hipGetLastError();
 }
 
-static hipError_t tex_transforms(char *arena, const std::vector<TexGroup> &groups, int dir, hipStream_t s) {
+// the z transforms of every group's wall planes (dir +1: to physical z)
+static hipError_t tex_ztransforms(char *arena, const std::vector<TexGroup> &groups, int dir, hipStream_t s) {
   for (const TexGroup &g : groups) {
     double *planes = reinterpret_cast<double *>(arena + g.o_texw);
     const double2 *tz = reinterpret_cast<const double2 *>(arena + g.o_twz);
-    const double2 *tx = reinterpret_cast<const double2 *>(arena + g.o_twx);
-    hipError_t e = dir > 0 ? launch_plane_f2p(g.N1, g.N1, g.N2, planes, g.nplanes, tz, tx, s, false)
-                           : launch_plane_p2f(g.N1, g.N1, g.N2, planes, g.nplanes, tz, tx, s, false);
+    hipError_t e = launch_plane_z(g.N1, g.N1, g.N2, planes, g.nplanes, tz, dir, s);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+// x transforms and mask of every textured channel's four wall planes
+static hipError_t tex_xmask(char *arena, const Chan *d_chans, const ShapeMax &m, const std::vector<TexGroup> &groups,
+                            int kRK, int mode, hipStream_t s) {
+  int gx = 1;
+  size_t lds = 0;
+  for (const TexGroup &g : groups) {
+    const int TZ = tex_xmask_tile(g.N1);
+    gx = std::max(gx, (g.N2 + 2 * TZ - 1) / (2 * TZ));
+    lds = std::max(lds, (size_t)(2 * TZ + 1) * g.N1 * sizeof(double2));
+  }
+  hipLaunchKernelGGL(k_tex_xmask, dim3(gx, 4, m.nchan), dim3(256), lds, s, arena, d_chans, kRK, mode);
+  return hipGetLastError();
 }
 
 hipError_t launch_tex_sources(char *arena, const Chan *d_chans, const ShapeMax &m,
@@ -256,10 +358,9 @@ hipError_t launch_tex_sources(char *arena, const Chan *d_chans, const ShapeMax &
   const dim3 b(256), gc((m.ncol + 255) / 256, 1, m.nchan);
   for (int it = 0; it < nit; it++) {
     hipLaunchKernelGGL(k_tex_pre, gc, b, 0, s, arena, d_chans, kRK, it);
-    hipError_t e = tex_transforms(arena, groups, +1, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_tex_mask, dim3((m.nphys + 255) / 256, 4, m.nchan), b, 0, s, arena, d_chans, kRK, 0);
-    e = tex_transforms(arena, groups, -1, s);
+    hipError_t e = tex_ztransforms(arena, groups, +1, s);
+    if (e == hipSuccess) e = tex_xmask(arena, d_chans, m, groups, kRK, 0, s);
+    if (e == hipSuccess) e = tex_ztransforms(arena, groups, -1, s);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_tex_put, gc, b, 0, s, arena, d_chans, 0);
@@ -270,10 +371,9 @@ hipError_t launch_tex_ghosts(char *arena, const Chan *d_chans, const ShapeMax &m
                              const std::vector<TexGroup> &groups, hipStream_t s) {
   const dim3 b(256), gc((m.ncol + 255) / 256, 1, m.nchan);
   hipLaunchKernelGGL(k_tex_get, gc, b, 0, s, arena, d_chans);
-  hipError_t e = tex_transforms(arena, groups, +1, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_tex_mask, dim3((m.nphys + 255) / 256, 4, m.nchan), b, 0, s, arena, d_chans, 1, 1);
-  e = tex_transforms(arena, groups, -1, s);
+  hipError_t e = tex_ztransforms(arena, groups, +1, s);
+  if (e == hipSuccess) e = tex_xmask(arena, d_chans, m, groups, 1, 1, s);
+  if (e == hipSuccess) e = tex_ztransforms(arena, groups, -1, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_tex_put, gc, b, 0, s, arena, d_chans, 1);
   return hipGetLastError();

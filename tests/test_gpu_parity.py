@@ -371,6 +371,26 @@ def test_textured_step(name):
         assert e <= 1e-11, (nm, e)
 
 
+@pytest.mark.parametrize("fused", [1, 0])
+def test_posts_wall_launch_forms(fused):
+    """x-varying masks (posts on both walls, one wall stripes): the wall
+    iteration with the planes resident in one workgroup's LDS (fused = 1)
+    and as separate launches (z transforms, then the x transforms and the
+    mask in one launch, the form of channels too large for LDS) both match
+    the oracle, one and three steps."""
+    cfg = small_cfg(16, 33)
+    s, o = tex_pair(cfg, (textures.posts(16, 16, 8, 4), textures.stripes(16, 16, 4, 0.5)), seed=23)
+    s.set_tuning(_lib.TUNE_TEX_FUSED, fused)
+    assert s.get_tuning(_lib.TUNE_TEX_FUSED) == fused
+    for n, tol in ((1, TOL), (2, 1e-11)):
+        s.step(n)
+        o.step(n)
+        for nm in ("u1", "u2", "u3", "p"):
+            e = relerr(s.download(0, nm), o.get(nm))
+            say("posts wall launches", fused, n, nm, e)
+            assert e <= tol, (nm, e)
+
+
 @pytest.mark.parametrize("sep", [1, 0])
 def test_ridges_wall_paths(sep):
     """x-independent masks: the per-x-mode wall iteration (sep = 1) and the
