@@ -25,11 +25,12 @@ static __host__ __device__ int plane_tile(int n) {
 // it for the retained x-modes i = 0..N1/2 (ise = Ngal1+2, isa = 2).
 // dir = +1: inverse (with zero_nyq: zero row Ngal2/2 first); dir = -1:
 // forward, scaled, then (zero_nyq) zero row Ngal2/2 of the whole plane.
+// TI: vectors per workgroup (at most plane_tile(Ng2), which the LDS size
+// the launches give assumes)
 __global__ void __launch_bounds__(256) k_plane_z(double *planes, int nv, int Ng1, int Ng2, int ise, int isa,
-                                                 size_t pstride, int dir, const double2 *twz, int zero_nyq) {
+                                                 size_t pstride, int dir, const double2 *twz, int zero_nyq, int TI) {
   const int igal = Ng1 + 2;
   double *P = planes + (size_t)blockIdx.y * pstride;
-  const int TI = plane_tile(Ng2);
   const int i0 = blockIdx.x * TI;
   extern __shared__ double2 smem_d[];
   if (i0 < nv) {
@@ -145,7 +146,7 @@ hipError_t launch_plane_f2p(int N1, int Ng1, int Ng2, double *planes, int nplane
   int nv = N1 / 2 + 1, TI = plane_tile(Ng2), TZ = plane_tile(Ng1);
   dim3 gz((nv + TI - 1) / TI, nplanes), gx((Ng2 + 2 * TZ - 1) / (2 * TZ), nplanes);
   hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, nv, Ng1, Ng2, igal,
-                     2, ps, +1, twz, (int)zero_nyq);
+                     2, ps, +1, twz, (int)zero_nyq, TI);
   hipLaunchKernelGGL(k_plane_x, gx, dim3(256), (2 * TZ + 1) * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, igal, ps,
                      +1, twx);
   return hipGetLastError();
@@ -155,10 +156,13 @@ hipError_t launch_plane_z(int N1, int Ng1, int Ng2, double *planes, int nplanes,
                           hipStream_t s) {
   const int igal = Ng1 + 2;
   const size_t ps = (size_t)igal * Ng2;
-  int nv = N1 / 2 + 1, TI = plane_tile(Ng2);
+  // (narrower tiles, one or two vectors per workgroup of 64 or 128
+  // threads, fill more CUs for the few wall planes but run slower: 256 x
+  // 256 wall work 169 -> 182 us per substep)
+  const int nv = N1 / 2 + 1, TI = plane_tile(Ng2);
   dim3 gz((nv + TI - 1) / TI, nplanes);
   hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, nv, Ng1, Ng2, igal,
-                     2, ps, dir, twz, 0);
+                     2, ps, dir, twz, 0, TI);
   return hipGetLastError();
 }
 
@@ -171,7 +175,7 @@ hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplane
   hipLaunchKernelGGL(k_plane_x, gx, dim3(256), (2 * TZ + 1) * Ng1 * sizeof(double2), s, planes, Ng1, Ng2, igal, ps,
                      -1, twx);
   hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, nv, Ng1, Ng2, igal,
-                     2, ps, -1, twz, (int)zero_nyq);
+                     2, ps, -1, twz, (int)zero_nyq, TI);
   return hipGetLastError();
 }
 
@@ -180,7 +184,7 @@ hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplane
 hipError_t launch_cft(int n, int ise, int isa, int m, int iopt, double *c, const double2 *tw, hipStream_t s) {
   const int TI = plane_tile(n);
   hipLaunchKernelGGL(k_plane_z, dim3((m + TI - 1) / TI, 1), dim3(256), (2 * TI + 1) * n * sizeof(double2), s, c, m,
-                     0, n, ise, isa, (size_t)0, iopt < 0 ? -1 : +1, tw, 0);
+                     0, n, ise, isa, (size_t)0, iopt < 0 ? -1 : +1, tw, 0, TI);
   return hipGetLastError();
 }
 
