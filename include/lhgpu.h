@@ -284,12 +284,15 @@ int lh_set_bands(lh_ctx *ctx, int ch_bottom, int ch_middle, int ch_top, int nb);
 /* LH_TUNE_XPROD_THREADS (default 256): threads per workgroup of the
  * x-product stage, 128, 192 or 256. */
 #define LH_TUNE_XPROD_THREADS 10
-/* LH_TUNE_RHS_FUSED (default 1): for contexts without textured walls, the
- * column solve of lh_step/lh_substep computes the right-hand sides itself
- * and r1, r2, r3 are not stored (the LH_PHASE_RHS / LH_PHASE_SOLVE phases
- * and lh_time_kernels keep the separate right-hand-side kernel); 0 always
- * runs that kernel.  lh_autotune sets it with the solve's shape, from the
- * timings of both forms. */
+/* LH_TUNE_RHS_FUSED (default 1): the column solve of lh_step/lh_substep
+ * computes the right-hand sides itself and r1, r2, r3 are not stored --
+ * for contexts without textured walls in every solve form, with textured
+ * walls in the one-column-per-wave form (LH_TUNE_SOLVE_COLS = -1), after a
+ * pass that stores the near-wall rows of r1 and r3 the wall sources read.
+ * (The LH_PHASE_RHS / LH_PHASE_SOLVE phases and lh_time_kernels keep the
+ * separate right-hand-side kernel); 0 always runs that kernel.
+ * lh_autotune sets it with the solve's shape, from the timings of both
+ * forms. */
 #define LH_TUNE_RHS_FUSED 11
 /* LH_TUNE_GHOST_IN_ZINV (default 1): in lh_step's captured step, the
  * ghost-row pass of x-independent textured walls after substeps 1 and 2

@@ -462,10 +462,10 @@ ShapeMax col_shape(const NlGroup &g, const ShapeMax &k) {
   m.rhs_fused = k.rhs_fused;
   return m;
 }
-hipError_t rhs_stage(lh_ctx *x, const ShapeMax &k, int kRK) {
-  if (x->nlg.empty()) return launch_rhs(x->arena, x->d_chans, k, kRK, x->stream);
+hipError_t rhs_stage(lh_ctx *x, const ShapeMax &k, int kRK, int band = 0) {
+  if (x->nlg.empty()) return launch_rhs(x->arena, x->d_chans, k, kRK, x->stream, band);
   for (auto &g : x->nlg) {
-    hipError_t e = launch_rhs(x->arena, g.d, col_shape(g, k), kRK, x->stream);
+    hipError_t e = launch_rhs(x->arena, g.d, col_shape(g, k), kRK, x->stream, band);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -687,6 +687,11 @@ int time_step(lh_ctx *x, int kRK, bool dt_done = false) {
 // with chained steps, the third substep's pass goes to the next step's first
 // substep as well (prev_ghost: this step's first substep takes it; keep_last:
 // leave this step's third pass to the next step)
+// whether the column solve computes the right-hand sides (LH_TUNE_RHS_FUSED):
+// any solve form without textured walls, the wave form with them
+static int fused_rhs(const lh_ctx *x, const ShapeMax &m) {
+  return m.rhs_fused && (!x->ntex || m.solve_cols < 0);
+}
 int run_substep(lh_ctx *x, int kRK, bool in_step = false, bool prev_ghost = false, bool keep_last = false) {
   const bool defer = in_step && ghost_deferrable(x);
   // the time-step update rides on the z-forward launch when there is one
@@ -697,10 +702,12 @@ int run_substep(lh_ctx *x, int kRK, bool in_step = false, bool prev_ghost = fals
   if (rc) return rc;
   rc = time_step(x, kRK, dt_in_zfwd);
   if (rc) return rc;
-  // the textured-wall sources read r1 and r3 before the solve; without
-  // textured walls the solve computes the right-hand sides itself
-  const int fuse = x->shape.rhs_fused && !x->ntex;
+  // the textured-wall sources read r1 and r3 near the walls before the
+  // solve: with textured walls only the one-column-per-wave solve computes
+  // the right-hand sides itself, after a pass over the rows the sources read
+  const int fuse = fused_rhs(x, x->shape);
   if (!fuse) HIPCHK(rhs_stage(x, x->shape, kRK));
+  else if (x->ntex) HIPCHK(rhs_stage(x, x->shape, kRK, 1));
   return solve_phase(x, kRK, fuse, defer && (kRK < 3 || keep_last));
 }
 
@@ -1878,15 +1885,15 @@ int lh_autotune(lh_ctx *x, int reps) {
       if (!solve_cols_ok(x, C)) continue;
       for (int sc = 0; sc <= (C > 0 ? 1 : 0); sc++)
         for (int nw = 1; nw <= (C > 0 ? 4 : 1); nw *= 2)
-          for (int fu = 0; fu <= (C > 0 && !x->ntex ? 1 : 0); fu++) {
+          for (int fu = 0; fu <= ((C > 0 && !x->ntex) || C < 0 ? 1 : 0); fu++) {
             ShapeMax m = best;
             m.solve_cols = C;
             m.solve_scan = sc;
             m.solve_waves = nw;
             float ms = time_launches(x, e0, e1, reps, [&](int r) {
               const int kRK = 1 + r % 3;
-              if (!fu) {
-                hipError_t e = rhs_stage(x, m, kRK);
+              if (!fu || x->ntex) {
+                hipError_t e = rhs_stage(x, m, kRK, fu);
                 if (e != hipSuccess) return e;
               }
               return solve_stage(x, m, kRK, fu);
@@ -1896,7 +1903,7 @@ int lh_autotune(lh_ctx *x, int reps) {
               best.solve_cols = C;
               best.solve_scan = sc;
               best.solve_waves = nw;
-              if (!x->ntex) best.rhs_fused = fu;
+              best.rhs_fused = fu;
             }
           }
     }

@@ -58,14 +58,18 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
                            int kRK = 1, int ghost = 0, int dtblk = 0);
 hipError_t launch_nonlinear(char *arena, const Chan *d_chans, const ShapeMax &m, bool mixed, hipStream_t s,
                             int kRK = 1, int ghost = 0, int dtblk = 0);
-hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s);
+// band = 1: only the near-wall rows of r1 and r3 that textured walls'
+// source passes read, for a solve that computes the right-hand sides itself
+hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int band = 0);
 hipError_t launch_factors(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 hipError_t launch_dt_update(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 hipError_t launch_u11(char *arena, const Chan *d_chans, const ShapeMax &m, hipStream_t s);
 // err, Qx, Umax, Uslip of every channel into out[ch*8 + 0..3] (device)
 hipError_t launch_history(char *arena, const Chan *d_chans, const ShapeMax &m, double *out, hipStream_t s);
 // fuse: compute the right-hand sides inside the solve instead of reading the
-// r1, r2, r3 that launch_rhs stores (smooth-walled channels only)
+// r1, r2, r3 that launch_rhs stores (smooth-walled channels; with textured
+// walls the one-column-per-wave form, C < 0, which reads the wall rows'
+// sources, after launch_rhs with band = 1)
 hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse = 0);
 hipError_t launch_flowrate(char *arena, const Chan *d_chans, int nchan, int set_target, hipStream_t s);
 // a banded channel: the channels of its bottom wall band, middle band and

@@ -942,15 +942,16 @@ __global__ void __launch_bounds__(NT) k_zfwd_fs(char *arena, const Chan *chans, 
 // wall rows), and, when `put`, the new explicit terms N1o, N3o, N2o of the
 // interior rows.  k_rhs stores r; the column solve of smooth-walled channels
 // calls it in its staging phase instead (the values go straight to LDS).
-__device__ __forceinline__ void rhs_point(const View &v, const Chan &ch, int kRK, int j, int c, bool put, double2 &r1,
-                                          double2 &r3, double2 &r2) {
-  const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu, nyv = ch.nyv;
+// (rhs_u: r1 and r3 with N1o, N3o; rhs_v: r2 with N2o; rhs_point: both)
+__device__ __forceinline__ void rhs_u(const View &v, const Chan &ch, int kRK, int j, int c, bool put, double2 &r1,
+                                      double2 &r3) {
+  const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu;
   const int i = c % nxh, k = c / nxh;
   const double dt = v.scal[SC_DT];
   const double C1 = rk_a(kRK) / ch.Re, C2 = -rk_c(kRK), Cg = -rk_g(kRK), dR = rk_d(kRK);
   const size_t o = (size_t)j * ldc + c;
   const double k2x = v.k2x[i], k2z = v.k2z[k];
-  r1 = r3 = r2 = make_double2(0, 0);
+  r1 = r3 = make_double2(0, 0);
   if (j >= 1 && j <= nyu - 2) {
     const double a0 = v.dyu2i[3 * j], a1 = v.dyu2i[3 * j + 1], a2 = v.dyu2i[3 * j + 2];
     const double dd = (k2x + k2z) + a1;
@@ -984,6 +985,15 @@ __device__ __forceinline__ void rhs_point(const View &v, const Chan &ch, int kRK
       if (put) v.N3o[o] = Nn;
     }
   }
+}
+__device__ __forceinline__ void rhs_v(const View &v, const Chan &ch, int kRK, int j, int c, bool put, double2 &r2) {
+  const int nxh = ch.nxh, ldc = ch.ldc, nyv = ch.nyv;
+  const int i = c % nxh, k = c / nxh;
+  const double dt = v.scal[SC_DT];
+  const double C1 = rk_a(kRK) / ch.Re, C2 = -rk_c(kRK), Cg = -rk_g(kRK), dR = rk_d(kRK);
+  const size_t o = (size_t)j * ldc + c;
+  const double k2x = v.k2x[i], k2z = v.k2z[k];
+  r2 = make_double2(0, 0);
   if (j >= 1 && j <= nyv - 2) {
     const double a0 = v.dyv2i[3 * j], a1 = v.dyv2i[3 * j + 1], a2 = v.dyv2i[3 * j + 2];
     const double dd = (k2x + k2z) + a1;
@@ -1003,9 +1013,19 @@ __device__ __forceinline__ void rhs_point(const View &v, const Chan &ch, int kRK
     if (put) v.N2o[o] = Nn;
   }
 }
+__device__ __forceinline__ void rhs_point(const View &v, const Chan &ch, int kRK, int j, int c, bool put, double2 &r1,
+                                          double2 &r3, double2 &r2) {
+  rhs_u(v, ch, kRK, j, c, put, r1, r3);
+  rhs_v(v, ch, kRK, j, c, put, r2);
+}
 
-__global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int kRK, int gx, int gy) {
+// band = 1: only the rows of textured channels that the wall-source passes
+// read (within the truncation depths iscal of either wall, wall rows
+// included), r1 and r3 only, N*o untouched: the column solve that follows
+// computes every right-hand side itself (rhs_fused with the wave form)
+__global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int kRK, int gx, int gy, int band) {
   const Chan ch = chans[blockIdx.z];
+  if (band && !ch.tex) return;
   const View v(arena, ch);
   // XCD-aware order: tiles (column block, row j) go round-robin to the 8
   // XCDs; hand each XCD a contiguous run so that rows j-1, j, j+1 of a column
@@ -1023,9 +1043,17 @@ __global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int
   const int j = rem / w;
   const int c = (cg * RG + rem - j * w) * blockDim.x + threadIdx.x;
   if (j >= ch.nyu || c >= ch.ncol) return;
+  const size_t o = (size_t)j * ch.ldc + c;
+  if (band) {
+    if (j >= v.iscal[0] && j < ch.nyu - v.iscal[1]) return;
+    double2 r1, r3;
+    rhs_u(v, ch, kRK, j, c, false, r1, r3);
+    v.r1[o] = r1;
+    v.r3[o] = r3;
+    return;
+  }
   double2 r1, r3, r2;
   rhs_point(v, ch, kRK, j, c, true, r1, r3, r2);
-  const size_t o = (size_t)j * ch.ldc + c;
   v.r1[o] = r1;
   v.r3[o] = r3;
   if (j < ch.nyv) v.r2[o] = r2;
@@ -1971,6 +1999,34 @@ __device__ __forceinline__ void sw_fetch(double2 (&t)[K], const double2 *F, int 
     t[u] = F[(size_t)r * ldc + cc];
   }
 }
+// the same elements' right-hand sides computed in place of the loads of r1
+// and r3 (rhs_u) or r2 (rhs_v); each live element puts its new explicit
+// terms once (clamped copies do not); the wall rows of textured walls hold
+// the wall sources, which are read
+template <int K, int NW>
+__device__ __forceinline__ void sw_rhs_u(double2 (&t)[K], double2 (&t2)[K], const View &v, const Chan &ch, int kRK,
+                                         int c0, int tid) {
+  const int nyu = ch.nyu, ncol = ch.ncol, ldc = ch.ldc;
+#pragma unroll
+  for (int u = 0; u < K; u++) {
+    const int idx = u * 64 * NW + tid, r = min(idx / NW, nyu - 1), cr = c0 + idx % NW, cc = min(cr, ncol - 1);
+    if ((r == 0 && (ch.tex & 1)) || (r == nyu - 1 && (ch.tex & 2))) {
+      t[u] = v.r1[(size_t)r * ldc + cc];
+      t2[u] = v.r3[(size_t)r * ldc + cc];
+    } else {
+      rhs_u(v, ch, kRK, r, cc, cr < ncol && idx / NW < nyu, t[u], t2[u]);
+    }
+  }
+}
+template <int K, int NW>
+__device__ __forceinline__ void sw_rhs_v(double2 (&t)[K], const View &v, const Chan &ch, int kRK, int c0, int tid) {
+  const int nyv = ch.nyv, ncol = ch.ncol;
+#pragma unroll
+  for (int u = 0; u < K; u++) {
+    const int idx = u * 64 * NW + tid, r = min(idx / NW, nyv - 1), cr = c0 + idx % NW, cc = min(cr, ncol - 1);
+    rhs_v(v, ch, kRK, r, cc, cr < ncol && idx / NW < nyv, t[u]);
+  }
+}
 template <int K, int NW>
 __device__ __forceinline__ void sw_fetch2(double2 (&t)[K], const double *Fa, int na, const double *Fb, int nb, int ldc,
                                           int c0, int ncol, int tid) {
@@ -2017,7 +2073,9 @@ __device__ __forceinline__ void sw_store(const double2 *T, double2 *F, double2 *
   }
 }
 
-template <int K, int NW>
+// FUSE: the right-hand sides are computed in the staging (sw_rhs_u/_v)
+// instead of read from r1, r3, r2
+template <int K, int NW, bool FUSE>
 __global__ void __launch_bounds__(64 * NW) k_solve_wave(char *arena, const Chan *chans, int kRK) {
   const Chan ch = chans[blockIdx.z];
   const int per = gridDim.x / 8;
@@ -2050,8 +2108,12 @@ __global__ void __launch_bounds__(64 * NW) k_solve_wave(char *arena, const Chan 
   //      the final p += psi.  (Three in flight make the compiler keep one
   //      staging array in scratch memory.)
   double2 t[K], t2[K], tp[K];
-  sw_fetch<K, NW>(t, v.r1, nyu, ldc, c0, ncol, tid);
-  sw_fetch<K, NW>(t2, v.r3, nyu, ldc, c0, ncol, tid);
+  if constexpr (FUSE) {
+    sw_rhs_u<K, NW>(t, t2, v, ch, kRK, c0, tid);
+  } else {
+    sw_fetch<K, NW>(t, v.r1, nyu, ldc, c0, ncol, tid);
+    sw_fetch<K, NW>(t2, v.r3, nyu, ldc, c0, ncol, tid);
+  }
 
   // ---- row coefficients, shared by the workgroup's columns: sub- and
   //      super-diagonals of the Helmholtz operators (wall rows included:
@@ -2081,7 +2143,10 @@ __global__ void __launch_bounds__(64 * NW) k_solve_wave(char *arena, const Chan 
   sw_put<K, NW>(T, t2, S, nyu, tid);
   __syncthreads();
   sw_get<K>(X3, T + w * S, j0, nyu);
-  sw_fetch<K, NW>(t, v.r2, nyv, ldc, c0, ncol, tid);
+  if constexpr (FUSE)
+    sw_rhs_v<K, NW>(t, v, ch, kRK, c0, tid);
+  else
+    sw_fetch<K, NW>(t, v.r2, nyv, ldc, c0, ncol, tid);
   sw_fetch2<K, NW>(t2, fU, nyu, fV, nyv, ldc, c0, ncol, tid);
   __syncthreads();
   sw_put<K, NW>(T, t, S, nyu, tid);
@@ -2770,27 +2835,29 @@ void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny) {
   m.nchan++;
 }
 
-hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s) {
+hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int band) {
   const int gx = (m.ncol + 255) / 256, gy = m.nyu;
   dim3 g((gx * gy + 7) / 8 * 8, 1, m.nchan);
-  hipLaunchKernelGGL(k_rhs, g, dim3(256), 0, s, arena, d_chans, kRK, gx, gy);
+  hipLaunchKernelGGL(k_rhs, g, dim3(256), 0, s, arena, d_chans, kRK, gx, gy, band);
   return hipGetLastError();
 }
 
 hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse) {
   const int C = m.solve_cols;
-  if (C < 0) {  // one column per wave, eight columns per workgroup
-    if (fuse) {  // the wave form reads r: compute it first
-      hipError_t e = launch_rhs(arena, d_chans, m, kRK, s);
-      if (e != hipSuccess) return e;
-    }
+  if (C < 0) {  // one column per wave, eight columns per workgroup; with
+                // fuse the right-hand sides are computed in its staging
     constexpr int nw = 8;
     const int K = solve_wave_rows(m.nyu);
     const int ngrp = (m.ncol + nw - 1) / nw;
     dim3 g((ngrp + 7) / 8 * 8, 1, m.nchan), b(64 * nw);
     const size_t lds = solve_wave_lds(m.nyu, nw);
-#define LH_SW(k) \
-  case k: hipLaunchKernelGGL((k_solve_wave<k, nw>), g, b, lds, s, arena, d_chans, kRK); break;
+#define LH_SW(k)                                                                            \
+  case k:                                                                                  \
+    if (fuse)                                                                              \
+      hipLaunchKernelGGL((k_solve_wave<k, nw, true>), g, b, lds, s, arena, d_chans, kRK);  \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_solve_wave<k, nw, false>), g, b, lds, s, arena, d_chans, kRK); \
+    break;
     switch (K) {
       LH_SW(1)
       LH_SW(2)

@@ -179,6 +179,38 @@ def test_rhs_fused_into_solve(cols, scan, waves, case):
         assert e <= 1e-13, (nm, e)
 
 
+@pytest.mark.parametrize("walls", ["posts_stripes", "ridges", "posts_bottom"])
+def test_rhs_fused_textured_wave_solve(walls):
+    """Textured walls with the right-hand sides computed inside the
+    one-column-per-wave solve: the near-wall rows the wall sources read are
+    computed first, the solve reads the wall rows' sources and computes every
+    other row itself.  Steps equal the oracle's and, to round-off, the
+    separate right-hand-side kernel's (x-varying masks on both walls,
+    x-independent masks, one textured wall)."""
+    cfg = small_cfg(16, 33)
+    masks = {"posts_stripes": (textures.posts(16, 16, 8, 4), textures.stripes(16, 16, 4, 0.5)),
+             "ridges": (textures.stripes(16, 16, 4, 0.5), textures.stripes(16, 16, 4, 0.5)),
+             "posts_bottom": (textures.posts(16, 16, 4, 2), None)}[walls]
+    out = {}
+    for fused in (1, 0):
+        s, o = tex_pair(cfg, masks, seed=29)
+        s.set_tuning(_lib.TUNE_SOLVE_COLS, -1)
+        s.set_tuning(_lib.TUNE_RHS_FUSED, fused)
+        assert s.get_tuning(_lib.TUNE_RHS_FUSED) == fused
+        s.step(2)
+        out[fused] = {nm: s.download(0, nm) for nm in ("u1", "u2", "u3", "p", "Nu1", "Nu2", "Nu3")}
+        if fused:
+            o.step(2)
+            for nm in ("u1", "u2", "u3", "p"):
+                e = relerr(out[1][nm], o.get(nm))
+                say("fused textured", walls, nm, e)
+                assert e <= 1e-11, (nm, e)
+    for nm in out[1]:
+        e = relerr(out[1][nm], out[0][nm])
+        say("fused textured vs separate", walls, nm, e)
+        assert e <= 1e-13, (nm, e)
+
+
 @pytest.mark.parametrize("key,points", [(_lib.TUNE_ZINV_POINTS, 256), (_lib.TUNE_ZINV_POINTS, 4096),
                                         (_lib.TUNE_XPROD_POINTS, 128), (_lib.TUNE_XPROD_POINTS, 2048),
                                         (_lib.TUNE_ZFWD_POINTS, 100), (_lib.TUNE_ZFWD_POINTS, 3000)])
