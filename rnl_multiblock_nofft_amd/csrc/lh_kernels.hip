@@ -942,9 +942,13 @@ __global__ void __launch_bounds__(NT) k_zfwd_fs(char *arena, const Chan *chans, 
 // wall rows), and, when `put`, the new explicit terms N1o, N3o, N2o of the
 // interior rows.  k_rhs stores r; the column solve of smooth-walled channels
 // calls it in its staging phase instead (the values go straight to LDS).
-// (rhs_u: r1 and r3 with N1o, N3o; rhs_v: r2 with N2o; rhs_point: both)
-__device__ __forceinline__ void rhs_u(const View &v, const Chan &ch, int kRK, int j, int c, bool put, double2 &r1,
-                                      double2 &r3) {
+// (rhs_u: r1 and r3 with N1o, N3o; rhs_v: r2 with N2o; rhs_point: both.
+// With n1, n3 given, rhs_u returns the new N1o, N3o there instead of
+// storing them, and whether the row is interior: a caller that computes
+// several points defers the stores past all its loads, which the compiler
+// may not move across a store to the same arena.)
+__device__ __forceinline__ bool rhs_u(const View &v, const Chan &ch, int kRK, int j, int c, bool put, double2 &r1,
+                                      double2 &r3, double2 *n1 = nullptr, double2 *n3 = nullptr) {
   const int nxh = ch.nxh, ldc = ch.ldc, nyu = ch.nyu;
   const int i = c % nxh, k = c / nxh;
   const double dt = v.scal[SC_DT];
@@ -969,7 +973,8 @@ __device__ __forceinline__ void rhs_u(const View &v, const Chan &ch, int kRK, in
       if (c == 0) du.x = du.x + (-dR * v.scal[SC_MPGX]);
       du = make_double2(du.x + Cg * Nn.x, du.y + Cg * Nn.y);
       r1 = make_double2(u0.x + dt * du.x, u0.y + dt * du.y);
-      if (put) v.N1o[o] = Nn;
+      if (n1) *n1 = Nn;
+      else if (put) v.N1o[o] = Nn;
     }
     {
       double2 um = v.u3[o - ldc], u0 = v.u3[o], up = v.u3[o + ldc];
@@ -982,11 +987,15 @@ __device__ __forceinline__ void rhs_u(const View &v, const Chan &ch, int kRK, in
       double2 du = make_double2(C1 * L.x + C2 * No.x + pc.x, C1 * L.y + C2 * No.y + pc.y);
       du = make_double2(du.x + Cg * Nn.x, du.y + Cg * Nn.y);
       r3 = make_double2(u0.x + dt * du.x, u0.y + dt * du.y);
-      if (put) v.N3o[o] = Nn;
+      if (n3) *n3 = Nn;
+      else if (put) v.N3o[o] = Nn;
     }
+    return true;
   }
+  return false;
 }
-__device__ __forceinline__ void rhs_v(const View &v, const Chan &ch, int kRK, int j, int c, bool put, double2 &r2) {
+__device__ __forceinline__ bool rhs_v(const View &v, const Chan &ch, int kRK, int j, int c, bool put, double2 &r2,
+                                      double2 *n2 = nullptr) {
   const int nxh = ch.nxh, ldc = ch.ldc, nyv = ch.nyv;
   const int i = c % nxh, k = c / nxh;
   const double dt = v.scal[SC_DT];
@@ -1010,13 +1019,25 @@ __device__ __forceinline__ void rhs_v(const View &v, const Chan &ch, int kRK, in
                               C1 * L.y + C2 * No.y + C3 * (pp.y - p0.y));
     du = make_double2(du.x + Cg * Nn.x, du.y + Cg * Nn.y);
     r2 = make_double2(u0.x + dt * du.x, u0.y + dt * du.y);
-    if (put) v.N2o[o] = Nn;
+    if (n2) *n2 = Nn;
+    else if (put) v.N2o[o] = Nn;
+    return true;
   }
+  return false;
 }
 __device__ __forceinline__ void rhs_point(const View &v, const Chan &ch, int kRK, int j, int c, bool put, double2 &r1,
                                           double2 &r3, double2 &r2) {
-  rhs_u(v, ch, kRK, j, c, put, r1, r3);
-  rhs_v(v, ch, kRK, j, c, put, r2);
+  double2 n1, n3, n2;
+  const bool iu = rhs_u(v, ch, kRK, j, c, false, r1, r3, &n1, &n3);
+  const bool iv = rhs_v(v, ch, kRK, j, c, false, r2, &n2);
+  if (put) {  // every load above is issued before these stores
+    const size_t o = (size_t)j * ch.ldc + c;
+    if (iu) {
+      v.N1o[o] = n1;
+      v.N3o[o] = n3;
+    }
+    if (iv) v.N2o[o] = n2;
+  }
 }
 
 // band = 1: only the rows of textured channels that the wall-source passes
@@ -2007,24 +2028,43 @@ template <int K, int NW>
 __device__ __forceinline__ void sw_rhs_u(double2 (&t)[K], double2 (&t2)[K], const View &v, const Chan &ch, int kRK,
                                          int c0, int tid) {
   const int nyu = ch.nyu, ncol = ch.ncol, ldc = ch.ldc;
+  double2 n1[K], n3[K];
+  bool put[K];
 #pragma unroll
   for (int u = 0; u < K; u++) {
     const int idx = u * 64 * NW + tid, r = min(idx / NW, nyu - 1), cr = c0 + idx % NW, cc = min(cr, ncol - 1);
+    put[u] = false;
     if ((r == 0 && (ch.tex & 1)) || (r == nyu - 1 && (ch.tex & 2))) {
       t[u] = v.r1[(size_t)r * ldc + cc];
       t2[u] = v.r3[(size_t)r * ldc + cc];
     } else {
-      rhs_u(v, ch, kRK, r, cc, cr < ncol && idx / NW < nyu, t[u], t2[u]);
+      put[u] = rhs_u(v, ch, kRK, r, cc, false, t[u], t2[u], &n1[u], &n3[u]) && cr < ncol && idx / NW < nyu;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < K; u++) {
+    const int idx = u * 64 * NW + tid;
+    if (put[u]) {
+      const size_t o = (size_t)(idx / NW) * ldc + c0 + idx % NW;
+      v.N1o[o] = n1[u];
+      v.N3o[o] = n3[u];
     }
   }
 }
 template <int K, int NW>
 __device__ __forceinline__ void sw_rhs_v(double2 (&t)[K], const View &v, const Chan &ch, int kRK, int c0, int tid) {
-  const int nyv = ch.nyv, ncol = ch.ncol;
+  const int nyv = ch.nyv, ncol = ch.ncol, ldc = ch.ldc;
+  double2 n2[K];
+  bool put[K];
 #pragma unroll
   for (int u = 0; u < K; u++) {
     const int idx = u * 64 * NW + tid, r = min(idx / NW, nyv - 1), cr = c0 + idx % NW, cc = min(cr, ncol - 1);
-    rhs_v(v, ch, kRK, r, cc, cr < ncol && idx / NW < nyv, t[u]);
+    put[u] = rhs_v(v, ch, kRK, r, cc, false, t[u], &n2[u]) && cr < ncol && idx / NW < nyv;
+  }
+#pragma unroll
+  for (int u = 0; u < K; u++) {
+    const int idx = u * 64 * NW + tid;
+    if (put[u]) v.N2o[(size_t)(idx / NW) * ldc + c0 + idx % NW] = n2[u];
   }
 }
 template <int K, int NW>
