@@ -1044,6 +1044,7 @@ __device__ __forceinline__ void rhs_point(const View &v, const Chan &ch, int kRK
 // read (within the truncation depths iscal of either wall, wall rows
 // included), r1 and r3 only, N*o untouched: the column solve that follows
 // computes every right-hand side itself (rhs_fused with the wave form)
+constexpr int RB_BAND = 16;
 __global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int kRK, int gx, int gy, int band) {
   const Chan ch = chans[blockIdx.z];
   if (band && !ch.tex) return;
@@ -1063,16 +1064,25 @@ __global__ void __launch_bounds__(256) k_rhs(char *arena, const Chan *chans, int
   const int cg = tile / (RG * gy), rem = tile - cg * RG * gy, w = min(RG, gx - cg * RG);
   const int j = rem / w;
   const int c = (cg * RG + rem - j * w) * blockDim.x + threadIdx.x;
-  if (j >= ch.nyu || c >= ch.ncol) return;
-  const size_t o = (size_t)j * ch.ldc + c;
   if (band) {
-    if (j >= v.iscal[0] && j < ch.nyu - v.iscal[1]) return;
-    double2 r1, r3;
-    rhs_u(v, ch, kRK, j, c, false, r1, r3);
-    v.r1[o] = r1;
-    v.r3[o] = r3;
+    // tile rows are slots: slot s < RB takes the rows s, s + RB, ... below
+    // the bottom depth iscal[0]; slot RB + s the rows nyu-1-s, ... above
+    // the top depth (gy = 2 RB: a few thousand workgroups at 256 x 256,
+    // whatever the depths)
+    if (c >= ch.ncol) return;
+    const int n = ch.nyu, top = j >= RB_BAND, s0 = top ? j - RB_BAND : j, J = v.iscal[top];
+    for (int q = s0; q < J && q < n; q += RB_BAND) {
+      const int jj = top ? n - 1 - q : q;
+      double2 r1, r3;
+      rhs_u(v, ch, kRK, jj, c, false, r1, r3);
+      const size_t o = (size_t)jj * ch.ldc + c;
+      v.r1[o] = r1;
+      v.r3[o] = r3;
+    }
     return;
   }
+  if (j >= ch.nyu || c >= ch.ncol) return;
+  const size_t o = (size_t)j * ch.ldc + c;
   double2 r1, r3, r2;
   rhs_point(v, ch, kRK, j, c, true, r1, r3, r2);
   v.r1[o] = r1;
@@ -2876,7 +2886,7 @@ void shape_add(ShapeMax &m, int N1, int N2, int Ng1, int Ng2, int ny) {
 }
 
 hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int band) {
-  const int gx = (m.ncol + 255) / 256, gy = m.nyu;
+  const int gx = (m.ncol + 255) / 256, gy = band ? 2 * RB_BAND : m.nyu;
   dim3 g((gx * gy + 7) / 8 * 8, 1, m.nchan);
   hipLaunchKernelGGL(k_rhs, g, dim3(256), 0, s, arena, d_chans, kRK, gx, gy, band);
   return hipGetLastError();
