@@ -470,10 +470,10 @@ hipError_t rhs_stage(lh_ctx *x, const ShapeMax &k, int kRK, int band = 0) {
   }
   return hipSuccess;
 }
-hipError_t solve_stage(lh_ctx *x, const ShapeMax &k, int kRK, int fuse = 0) {
-  if (x->nlg.empty()) return launch_solve(x->arena, x->d_chans, k, kRK, x->stream, fuse);
+hipError_t solve_stage(lh_ctx *x, const ShapeMax &k, int kRK, int fuse = 0, int keep = 1) {
+  if (x->nlg.empty()) return launch_solve(x->arena, x->d_chans, k, kRK, x->stream, fuse, keep);
   for (auto &g : x->nlg) {
-    hipError_t e = launch_solve(x->arena, g.d, col_shape(g, k), kRK, x->stream, fuse);
+    hipError_t e = launch_solve(x->arena, g.d, col_shape(g, k), kRK, x->stream, fuse, keep);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -582,10 +582,10 @@ void tex_classify(lh_ctx *x) {
 }
 
 // column solves of a substep (with the texture parts around them)
-int solve_phase(lh_ctx *x, int kRK, int fuse = 0, bool defer_ghost = false) {
+int solve_phase(lh_ctx *x, int kRK, int fuse = 0, bool defer_ghost = false, int keep = 1) {
   int rc = tex_part(x, kRK, 0);
   if (rc) return rc;
-  HIPCHK(solve_stage(x, x->shape, kRK, fuse));
+  HIPCHK(solve_stage(x, x->shape, kRK, fuse, keep));
   return tex_part(x, kRK, 1, defer_ghost);
 }
 
@@ -708,7 +708,9 @@ int run_substep(lh_ctx *x, int kRK, bool in_step = false, bool prev_ghost = fals
   const int fuse = fused_rhs(x, x->shape);
   if (!fuse) HIPCHK(rhs_stage(x, x->shape, kRK));
   else if (x->ntex) HIPCHK(rhs_stage(x, x->shape, kRK, 1));
-  return solve_phase(x, kRK, fuse, defer && (kRK < 3 || keep_last));
+  // psi and div of substeps 1 and 2 of a step are read by nothing (lh_history
+  // and downloads see the last substep's)
+  return solve_phase(x, kRK, fuse, defer && (kRK < 3 || keep_last), !in_step || kRK == 3);
 }
 
 // capture one step into *out

@@ -70,7 +70,9 @@ hipError_t launch_history(char *arena, const Chan *d_chans, const ShapeMax &m, d
 // r1, r2, r3 that launch_rhs stores (smooth-walled channels; with textured
 // walls the one-column-per-wave form, C < 0, which reads the wall rows'
 // sources, after launch_rhs with band = 1)
-hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse = 0);
+// keep = 0 lets the wave form skip storing psi and div (p still gets psi)
+hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse = 0,
+                        int keep = 1);
 hipError_t launch_flowrate(char *arena, const Chan *d_chans, int nchan, int set_target, hipStream_t s);
 // a banded channel: the channels of its bottom wall band, middle band and
 // top wall band, and the top band's first row in the middle band's rows

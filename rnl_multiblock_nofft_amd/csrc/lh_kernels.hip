@@ -2106,8 +2106,9 @@ __device__ __forceinline__ void sw_putcol(double2 *col, const double2 (&X)[K], i
   for (int q = 0; q < K; q++)
     if (j0 + q < nyu) col[j0 + q] = X[q];
 }
-// rows jlo..jhi of the live columns from the tile to F; with ADDP, p (its
-// values in tp, fetched by sw_fetch) gets the tile added
+// rows jlo..jhi of the live columns from the tile to F (none when F is
+// null); with ADDP, p (its values in tp, fetched by sw_fetch) gets the tile
+// added
 template <int K, int NW, bool ADDP>
 __device__ __forceinline__ void sw_store(const double2 *T, double2 *F, double2 *p, const double2 (&tp)[K], int S,
                                          int jlo, int jhi, int ldc, int c0, int ncol, int tid) {
@@ -2118,15 +2119,17 @@ __device__ __forceinline__ void sw_store(const double2 *T, double2 *F, double2 *
       const double2 x = T[(idx % NW) * S + r];
       const size_t o = (size_t)r * ldc + cc;
       if (ADDP) p[o] = make_double2(tp[u].x + x.x, tp[u].y + x.y);
-      F[o] = x;
+      if (F) F[o] = x;
     }
   }
 }
 
 // FUSE: the right-hand sides are computed in the staging (sw_rhs_u/_v)
 // instead of read from r1, r3, r2
+// keep = 0: psi and div are not stored (p still gets psi added): the first
+// two substeps of a captured step, whose psi and div nothing reads
 template <int K, int NW, bool FUSE>
-__global__ void __launch_bounds__(64 * NW) k_solve_wave(char *arena, const Chan *chans, int kRK) {
+__global__ void __launch_bounds__(64 * NW) k_solve_wave(char *arena, const Chan *chans, int kRK, int keep) {
   const Chan ch = chans[blockIdx.z];
   const int per = gridDim.x / 8;
   const int grp = (blockIdx.x % 8) * per + blockIdx.x / 8;
@@ -2487,7 +2490,8 @@ __global__ void __launch_bounds__(64 * NW) k_solve_wave(char *arena, const Chan 
   __syncthreads();
   sw_putcol<K>(T + w * S, PS, j0, nyu);
   __syncthreads();
-  sw_store<K, NW, true>(T, v.psi, v.p, tp, S, 1, ne, ldc, c0, ncol, tid);
+  sw_store<K, NW, true>(T, keep ? v.psi : nullptr, v.p, tp, S, 1, ne, ldc, c0, ncol, tid);
+  if (!keep) return;
   __syncthreads();
   sw_putcol<K>(T + w * S, DV, j0, nyu);
   __syncthreads();
@@ -2892,7 +2896,8 @@ hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int k
   return hipGetLastError();
 }
 
-hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse) {
+hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse,
+                        int keep) {
   const int C = m.solve_cols;
   if (C < 0) {  // one column per wave, eight columns per workgroup; with
                 // fuse the right-hand sides are computed in its staging
@@ -2903,10 +2908,10 @@ hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int
     const size_t lds = solve_wave_lds(m.nyu, nw);
 #define LH_SW(k)                                                                            \
   case k:                                                                                  \
-    if (fuse)                                                                              \
-      hipLaunchKernelGGL((k_solve_wave<k, nw, true>), g, b, lds, s, arena, d_chans, kRK);  \
-    else                                                                                   \
-      hipLaunchKernelGGL((k_solve_wave<k, nw, false>), g, b, lds, s, arena, d_chans, kRK); \
+    if (fuse)                                                                                    \
+      hipLaunchKernelGGL((k_solve_wave<k, nw, true>), g, b, lds, s, arena, d_chans, kRK, keep);  \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_solve_wave<k, nw, false>), g, b, lds, s, arena, d_chans, kRK, keep); \
     break;
     switch (K) {
       LH_SW(1)

@@ -116,6 +116,25 @@ def test_one_step_graph(name):
         assert e <= 1e-11, (nm, e)
 
 
+@pytest.mark.parametrize("cols", [-1, 3])
+def test_step_psi_div_are_the_last_substeps(cols):
+    """A captured step stores psi and div at its last substep only (the wave
+    form skips them at substeps 1 and 2): after two steps they are the
+    oracle's last-substep psi, a divergence at round-off, and the history's
+    error is that divergence."""
+    cfg = CASES["N32_ny153"]
+    s, o, f = make_pair(cfg, seed=41)
+    s.set_tuning(_lib.TUNE_SOLVE_COLS, cols)
+    s.step(2)
+    o.step(2)
+    e = np.linalg.norm(s.download(0, "psi") - o.get("psi")) / np.linalg.norm(o.get("p"))
+    say("step psi/|p|", cols, e)
+    assert e <= 5e-11, e
+    d = s.download(0, "div")
+    assert np.abs(d).max() < 1e-10 * max(1.0, np.abs(s.download(0, "u1")).max())
+    assert s.history(0)["err"] == pytest.approx(np.abs(d).max(), rel=1e-14)
+
+
 @pytest.mark.parametrize("cols,scan,waves,case", [(0, 0, 1, "N32_ny153"), (1, 0, 1, "N32_ny153"),
                                                   (1, 1, 1, "N32_ny153"), (2, 1, 1, "N32_ny153"),
                                                   (3, 0, 1, "N32_ny153"), (3, 1, 1, "N32_ny153"),
