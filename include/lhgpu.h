@@ -271,8 +271,9 @@ int lh_set_bands(lh_ctx *ctx, int ch_bottom, int ch_middle, int ch_top, int nb);
 #define LH_TUNE_SOLVE_SCAN 6
 /* LH_TUNE_SOLVE_WAVES (default 1): waves per workgroup of the LDS column
  * solve (1, 2 or 4); more waves give each staged column more lanes for its
- * recurrences and more loads in flight.  (The one-column-per-wave form
- * always runs eight consecutive columns per workgroup.) */
+ * recurrences and more loads in flight.  The one-column-per-wave form runs
+ * eight consecutive columns per workgroup, or two when this is 4 and the
+ * columns have at most 192 rows (more workgroups for small channels). */
 #define LH_TUNE_SOLVE_WAVES 7
 /* LH_TUNE_XPROD_INPLACE (default 0): 1 runs the x-product stage's transforms
  * in place in one LDS buffer where every stage has at most one item per

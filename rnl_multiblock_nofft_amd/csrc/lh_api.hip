@@ -1886,7 +1886,7 @@ int lh_autotune(lh_ctx *x, int reps) {
     for (int C : cand) {
       if (!solve_cols_ok(x, C)) continue;
       for (int sc = 0; sc <= (C > 0 ? 1 : 0); sc++)
-        for (int nw = 1; nw <= (C > 0 ? 4 : 1); nw *= 2)
+        for (int nw = 1; nw <= (C > 0 || solve_wave_rows(x->shape.nyu) <= 3 ? 4 : 1); nw *= (C > 0 ? 2 : 4))
           for (int fu = 0; fu <= ((C > 0 && !x->ntex) || C < 0 ? 1 : 0); fu++) {
             ShapeMax m = best;
             m.solve_cols = C;

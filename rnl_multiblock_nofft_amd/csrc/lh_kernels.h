@@ -38,6 +38,8 @@ int solve_cols_default(int nyu);
 // 64 K >= nyu, at most SOLVE_WAVE_KMAX; 0 when the column is too tall
 constexpr int SOLVE_WAVE_KMAX = 6;
 int solve_wave_rows(int nyu);
+// its columns per workgroup for LH_TUNE_SOLVE_WAVES `waves` (8, or 2)
+int solve_wave_cols(int nyu, int waves);
 // waves (consecutive columns) per workgroup of that form, and its LDS: a
 // staging tile of nw columns x S complex rows (S = nyu made odd, so that
 // neighbouring columns' rows fall on different banks) and 9 row-coefficient

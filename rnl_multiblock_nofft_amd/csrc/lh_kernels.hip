@@ -2498,6 +2498,11 @@ __global__ void __launch_bounds__(64 * NW) k_solve_wave(char *arena, const Chan 
   sw_store<K, NW, false>(T, v.div, v.p, tp, S, 1, ne, ldc, c0, ncol, tid);
 }
 
+// columns (waves) per workgroup of the wave-form solve: 8 (128-byte rows of
+// the staging tile), or 2 for columns of at most 192 rows when
+// LH_TUNE_SOLVE_WAVES asks for it (4): small channels then spread over four
+// times the workgroups
+int solve_wave_cols(int nyu, int waves) { return waves == 4 && solve_wave_rows(nyu) <= 3 ? 2 : 8; }
 // rows per lane of the wave-form solve (0: the column is too tall for it)
 int solve_wave_rows(int nyu) {
   const int K = (nyu + 63) / 64;
@@ -2899,27 +2904,31 @@ hipError_t launch_rhs(char *arena, const Chan *d_chans, const ShapeMax &m, int k
 hipError_t launch_solve(char *arena, const Chan *d_chans, const ShapeMax &m, int kRK, hipStream_t s, int fuse,
                         int keep) {
   const int C = m.solve_cols;
-  if (C < 0) {  // one column per wave, eight columns per workgroup; with
-                // fuse the right-hand sides are computed in its staging
-    constexpr int nw = 8;
+  if (C < 0) {  // one column per wave, 8 (or, for short columns, 2)
+                // columns per workgroup; with fuse the right-hand sides are
+                // computed in its staging
+    const int nw = solve_wave_cols(m.nyu, m.solve_waves);
     const int K = solve_wave_rows(m.nyu);
     const int ngrp = (m.ncol + nw - 1) / nw;
     dim3 g((ngrp + 7) / 8 * 8, 1, m.nchan), b(64 * nw);
     const size_t lds = solve_wave_lds(m.nyu, nw);
-#define LH_SW(k)                                                                            \
-  case k:                                                                                  \
+#define LH_SW(k, n)                                                                              \
+  case k * 16 + n:                                                                               \
     if (fuse)                                                                                    \
-      hipLaunchKernelGGL((k_solve_wave<k, nw, true>), g, b, lds, s, arena, d_chans, kRK, keep);  \
+      hipLaunchKernelGGL((k_solve_wave<k, n, true>), g, b, lds, s, arena, d_chans, kRK, keep);   \
     else                                                                                         \
-      hipLaunchKernelGGL((k_solve_wave<k, nw, false>), g, b, lds, s, arena, d_chans, kRK, keep); \
+      hipLaunchKernelGGL((k_solve_wave<k, n, false>), g, b, lds, s, arena, d_chans, kRK, keep);  \
     break;
-    switch (K) {
-      LH_SW(1)
-      LH_SW(2)
-      LH_SW(3)
-      LH_SW(4)
-      LH_SW(5)
-      LH_SW(6)
+    switch (K * 16 + nw) {
+      LH_SW(1, 8)
+      LH_SW(2, 8)
+      LH_SW(3, 8)
+      LH_SW(4, 8)
+      LH_SW(5, 8)
+      LH_SW(6, 8)
+      LH_SW(1, 2)
+      LH_SW(2, 2)
+      LH_SW(3, 2)
       default: return hipErrorInvalidValue;
     }
 #undef LH_SW

@@ -142,6 +142,7 @@ def test_step_psi_div_are_the_last_substeps(cols):
                                                   (1, 1, 1, "N8_ny17"), (2, 1, 4, "N8_ny17"),
                                                   (10, 0, 1, "N16_ny33_poly_ctP"), (10, 1, 4, "N16_ny33_poly_ctP"),
                                                   (-1, 0, 1, "N32_ny153"), (-1, 0, 1, "N8_ny17"),
+                                                  (-1, 0, 4, "N32_ny153"), (-1, 0, 4, "N8_ny17"),
                                                   (-1, 0, 1, "N16_ny33_poly_ctP"), (-1, 0, 1, "N16_ny257"),
                                                   (0, 0, 1, "N16_ny257")])
 def test_column_solve_layouts(cols, scan, waves, case):
@@ -170,7 +171,8 @@ def test_column_solve_layouts(cols, scan, waves, case):
 
 @pytest.mark.parametrize("cols,scan,waves,case", [(0, 0, 1, "N32_ny153"), (3, 1, 1, "N32_ny153"),
                                                   (2, 1, 4, "N8_ny17"), (10, 1, 4, "N16_ny33_poly_ctP"),
-                                                  (-1, 0, 1, "N32_ny153"), (-1, 0, 1, "N16_ny33_poly_ctP")])
+                                                  (-1, 0, 1, "N32_ny153"), (-1, 0, 1, "N16_ny33_poly_ctP"),
+                                                  (-1, 0, 4, "N32_ny153"), (-1, 0, 4, "N16_ny33_poly_ctP")])
 def test_rhs_fused_into_solve(cols, scan, waves, case):
     """Smooth-walled steps with the right-hand sides computed inside the
     column solve give the oracle's steps and, to round-off, the steps of the
