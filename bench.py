@@ -258,7 +258,8 @@ def main():
                 "zinv_inplace": _lib.TUNE_ZINV_INPLACE,
                 "zfwd_fieldwise": _lib.TUNE_ZFWD_FIELDWISE,
                 "xprod_threads": _lib.TUNE_XPROD_THREADS, "rhs_fused": _lib.TUNE_RHS_FUSED,
-                "ghost_in_zinv": _lib.TUNE_GHOST_IN_ZINV}
+                "ghost_in_zinv": _lib.TUNE_GHOST_IN_ZINV, "tex_fused": _lib.TUNE_TEX_FUSED,
+                "tex_separable": _lib.TUNE_TEX_SEPARABLE}
         for kv in args.tune.split(","):
             k, v = kv.split("=")
             s.set_tuning(keys[k], int(v))
