@@ -259,7 +259,7 @@ def main():
                 "zfwd_fieldwise": _lib.TUNE_ZFWD_FIELDWISE,
                 "xprod_threads": _lib.TUNE_XPROD_THREADS, "rhs_fused": _lib.TUNE_RHS_FUSED,
                 "ghost_in_zinv": _lib.TUNE_GHOST_IN_ZINV, "tex_fused": _lib.TUNE_TEX_FUSED,
-                "tex_separable": _lib.TUNE_TEX_SEPARABLE}
+                "tex_separable": _lib.TUNE_TEX_SEPARABLE, "shape_groups": _lib.TUNE_SHAPE_GROUPS}
         for kv in args.tune.split(","):
             k, v = kv.split("=")
             s.set_tuning(keys[k], int(v))

@@ -320,6 +320,12 @@ int lh_set_bands(lh_ctx *ctx, int ch_bottom, int ch_middle, int ch_top, int nb);
  * the mask in one launch), as channels too large for LDS do.  Results agree
  * to rounding. */
 #define LH_TUNE_TEX_FUSED 15
+/* LH_TUNE_SHAPE_GROUPS (default 1): a context whose channels have several
+ * plane shapes launches the nonlinear stages, the right-hand sides and the
+ * solve once per group of channels of one shape (as banded contexts always
+ * do): each launch runs its length's specialised code; 0 launches every
+ * stage once over all channels.  Results are identical. */
+#define LH_TUNE_SHAPE_GROUPS 16
 int lh_set_tuning(lh_ctx *ctx, int key, int value);
 int lh_get_tuning(lh_ctx *ctx, int key, int *value);
 /* Autotuner: times each feasible setting of every knob on the context's own

@@ -29,6 +29,7 @@ TUNE_GHOST_IN_ZINV = 12
 TUNE_ZINV_INPLACE = 13
 TUNE_ZFWD_FIELDWISE = 14
 TUNE_TEX_FUSED = 15
+TUNE_SHAPE_GROUPS = 16
 PREC_FP64, PREC_MIXED = 0, 1
 
 # every symbol include/lhgpu.h declares

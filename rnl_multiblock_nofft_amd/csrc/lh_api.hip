@@ -234,6 +234,7 @@ struct lh_ctx {
   int ntex = 0;                      // channels with a textured wall
   int tex_nit = 3;                   // wall-source iterations per substep
   int tex_fused = 1;                 // LDS-resident wall transforms when they fit
+  int shape_groups = 1;              // launches per plane shape without bands too
   int tex_sep = 1;                   // per-x-mode wall iteration for x-independent masks
   int ntex1d = 0;                    // channels on that path (the rest: ntex - ntex1d)
   bool adaptive = false;             // a channel with an adaptive time step
@@ -492,7 +493,13 @@ int build_nl_groups(lh_ctx *x) {
   for (auto &g : x->nlg)
     if (g.d) (void)hipFree(g.d);
   x->nlg.clear();
-  if (x->bands.empty()) return LH_OK;
+  if (x->bands.empty()) {
+    // without bands: only when asked and the channels have several shapes
+    bool several = false;
+    for (auto &h : x->ch)
+      several = several || h.cfg.Ngal1 != x->ch[0].cfg.Ngal1 || h.cfg.Ngal2 != x->ch[0].cfg.Ngal2;
+    if (!x->shape_groups || !several) return LH_OK;
+  }
   for (int c = 0; c < (int)x->ch.size(); c++) {
     const lh_config &cf = x->ch[c].cfg;
     NlGroup *g = nullptr;
@@ -848,6 +855,7 @@ int lh_ctx_create(const lh_config *cfgs, int nchan, lh_ctx **out) {
     x->staging_elems = stage;
   }
   if (rc == LH_OK) rc = push_descriptors(x);
+  if (rc == LH_OK) rc = build_nl_groups(x);  // channels of several plane shapes
   if (rc == LH_OK) rc = refresh_tables(x);
   if (rc != LH_OK) {
     std::string keep = g_err;
@@ -1641,6 +1649,14 @@ int lh_set_tuning(lh_ctx *x, int key, int value) {
     drop_graph(x);
     return LH_OK;
   }
+  if (key == LH_TUNE_SHAPE_GROUPS) {
+    if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_SHAPE_GROUPS is 0 or 1");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->shape_groups = value;
+    if ((rc = build_nl_groups(x))) return rc;
+    drop_graph(x);
+    return LH_OK;
+  }
   if (key == LH_TUNE_TEX_SEPARABLE) {
     if (value != 0 && value != 1) return set_err(LH_EINVAL, "lh_set_tuning: LH_TUNE_TEX_SEPARABLE is 0 or 1");
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -1739,6 +1755,10 @@ int lh_get_tuning(lh_ctx *x, int key, int *value) {
   if (!value) return set_err(LH_EINVAL, "lh_get_tuning: null value");
   if (key == LH_TUNE_TEX_FUSED) {
     *value = x->tex_fused;
+    return LH_OK;
+  }
+  if (key == LH_TUNE_SHAPE_GROUPS) {
+    *value = x->shape_groups;
     return LH_OK;
   }
   if (key == LH_TUNE_TEX_SEPARABLE) {

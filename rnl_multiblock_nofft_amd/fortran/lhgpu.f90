@@ -20,7 +20,8 @@ module lhgpu
                                LH_PHASE_SOLVE = 3
   integer(c_int), parameter :: LH_TUNE_SOLVE_COLS = 1, LH_TUNE_ZINV_POINTS = 2, &
                                LH_TUNE_XPROD_POINTS = 3, LH_TUNE_ZFWD_POINTS = 4, &
-                               LH_TUNE_TEX_SEPARABLE = 5, LH_TUNE_TEX_FUSED = 15
+                               LH_TUNE_TEX_SEPARABLE = 5, LH_TUNE_TEX_FUSED = 15, &
+                               LH_TUNE_SHAPE_GROUPS = 16
   ! statistics profiles in write_stats' order (stats.f90:356-368), spectra
   integer(c_int), parameter :: LH_ST_UM = 0, LH_ST_U2M = 1, LH_ST_VM = 2, LH_ST_V2M = 3, &
                                LH_ST_WM = 4, LH_ST_W2M = 5, LH_ST_PM = 6, LH_ST_P2M = 7, &

@@ -612,10 +612,16 @@ def test_adaptive_time_step(name):
             assert e <= 1e-11, (nm, e)
 
 
-def test_ensemble_uneven():
+@pytest.mark.parametrize("groups", [0, 1])
+def test_ensemble_uneven(groups):
+    """Channels of different sizes in one launch per stage (groups = 0) or
+    one launch per plane shape (LH_TUNE_SHAPE_GROUPS = 1) each equal their
+    own oracle over two steps."""
     cfgs = [small_cfg(8, 17), small_cfg(16, 33), small_cfg(8, 17, grid_kind=0), small_cfg(16, 25)]
-    say("ensemble", len(cfgs))
+    say("ensemble", len(cfgs), "groups", groups)
     s = Solver(cfgs)
+    s.set_tuning(_lib.TUNE_SHAPE_GROUPS, groups)
+    assert s.get_tuning(_lib.TUNE_SHAPE_GROUPS) == groups
     oracles = []
     for q, c in enumerate(cfgs):
         o = oracle_for(c)
