@@ -23,6 +23,7 @@
 // The nonlinear stages run in FP64 (C = double2) or, in mixed mode, FP32
 // (C = float2) with FP64 twiddles; k_rhs and k_solve are always FP64.
 #include <algorithm>
+#include <type_traits>
 
 #include "lh_common.h"
 #include "lh_fft.h"
@@ -167,11 +168,17 @@ __global__ void __launch_bounds__(NT) k_zinv(char *arena, const Chan *chans, int
   const bool inplace = IP && (NL > 0 ? true : zinv_inplace(Ng2, TI, NT));
   C *A = reinterpret_cast<C *>(smem_d);
   C *B = A + TI * S;
-  C *TW = inplace ? B : B + TI * S;
+  C *TWs = inplace ? B : B + TI * S;
   const bool isU = slot < ch.nplu;
   const int j = isU ? ch.pu0 + slot : ch.pv0 + slot - ch.nplu;
 
-  stage_twiddles<C>(TW, v.twz, Ng2);
+  // FP64 in place at a specialised length: the twiddles are read from the
+  // table in HBM (L1-resident, 6 KiB at 384 points) rather than staged, so
+  // that the LDS of three 8-x-mode workgroups fits a CU (the host leaves
+  // them out of the launch's LDS)
+  constexpr bool tw_hbm = std::is_same<C, double2>::value && IP && NL > 0;
+  const C *TW = tw_hbm ? reinterpret_cast<const C *>(v.twz) : TWs;
+  if (!tw_hbm) stage_twiddles<C>(TWs, v.twz, Ng2);
   // only the zero-padding rows z = N2/2 .. Ngal2-N2/2-1 need clearing
   {
     const int half0 = ch.N2 / 2, npad = Ng2 - ch.N2;
@@ -2798,9 +2805,13 @@ hipError_t launch_nl_stage(char *arena, const Chan *d_chans, const ShapeMax &m, 
       }
     dim3 g(nz + extra, 1, m.nchan * 3);
     const size_t lds = std::max(m.lds_zinv * es, ghost ? tex_1d_lds(n2) : (size_t)0);
+    // FP64 in place at a specialised length reads its twiddles from HBM:
+    // every channel has Ng2 = nl, and lds_zinv holds nl elements for them
+    const size_t lds64 =
+        std::max((m.lds_zinv - (m.zinv_ip && nl > 0 ? nl : 0)) * es, ghost ? tex_1d_lds(n2) : (size_t)0);
 #define LH_ZI(n)                                                                      \
   if (mixed) zinv_launch<float2, n>(g, b, lds, s, arena, d_chans, m, nz, extra);     \
-  else zinv_launch<double2, n>(g, b, lds, s, arena, d_chans, m, nz, extra)
+  else zinv_launch<double2, n>(g, b, lds64, s, arena, d_chans, m, nz, extra)
     bool small = nl == 0;  // a mix of the lengths 24, 48, 96 only
     for (const ChanDims &d : m.dims) small = small && (d.Ng2 == 24 || d.Ng2 == 48 || d.Ng2 == 96);
     if (small) {
