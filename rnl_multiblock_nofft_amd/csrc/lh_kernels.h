@@ -95,9 +95,6 @@ hipError_t launch_from_rows(const double2 *src, double2 *dst, int ncol, int nrow
 // four_to_phys_N / phys_to_four_N do not)
 hipError_t launch_plane_f2p(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
                             const double2 *twx, hipStream_t s, bool zero_nyq = true);
-// the z transforms of launch_plane_f2p (dir +1) or launch_plane_p2f (dir -1) only
-hipError_t launch_plane_z(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz, int dir,
-                          hipStream_t s);
 hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
                             const double2 *twx, hipStream_t s, bool zero_nyq = true);
 hipError_t launch_cft(int n, int ise, int isa, int m, int iopt, double *c, const double2 *tw, hipStream_t s);

@@ -152,20 +152,6 @@ hipError_t launch_plane_f2p(int N1, int Ng1, int Ng2, double *planes, int nplane
   return hipGetLastError();
 }
 
-hipError_t launch_plane_z(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz, int dir,
-                          hipStream_t s) {
-  const int igal = Ng1 + 2;
-  const size_t ps = (size_t)igal * Ng2;
-  // (narrower tiles, one or two vectors per workgroup of 64 or 128
-  // threads, fill more CUs for the few wall planes but run slower: 256 x
-  // 256 wall work 169 -> 182 us per substep)
-  const int nv = N1 / 2 + 1, TI = plane_tile(Ng2);
-  dim3 gz((nv + TI - 1) / TI, nplanes);
-  hipLaunchKernelGGL(k_plane_z, gz, dim3(256), (2 * TI + 1) * Ng2 * sizeof(double2), s, planes, nv, Ng1, Ng2, igal,
-                     2, ps, dir, twz, 0, TI);
-  return hipGetLastError();
-}
-
 hipError_t launch_plane_p2f(int N1, int Ng1, int Ng2, double *planes, int nplanes, const double2 *twz,
                             const double2 *twx, hipStream_t s, bool zero_nyq) {
   const int igal = Ng1 + 2;

@@ -328,15 +328,102 @@ hipError_t launch_tex_tables(char *arena, const Chan *d_chans, const ShapeMax &m
   return hipGetLastError();
 }
 
-// the z transforms of every group's wall planes (dir +1: to physical z)
-static hipError_t tex_ztransforms(char *arena, const std::vector<TexGroup> &groups, int dir, hipStream_t s) {
-  for (const TexGroup &g : groups) {
-    double *planes = reinterpret_cast<double *>(arena + g.o_texw);
-    const double2 *tz = reinterpret_cast<const double2 *>(arena + g.o_twz);
-    hipError_t e = launch_plane_z(g.N1, g.N1, g.N2, planes, g.nplanes, tz, dir, s);
-    if (e != hipSuccess) return e;
+// The z transforms of a field's two wall planes with the Fourier-space work
+// between passes fused in: one workgroup per (tile of x-modes, field,
+// channel) holds both walls' sequences, since the source update couples the
+// walls.  mode 0: texw = U (the first pass's update), inverse z transform;
+// mode 1: forward z transform, the update of k_tex_pre for pass `it`,
+// inverse z transform; mode 2: forward z transform, the sources into the
+// wall rows of r (k_tex_put 0); mode 3: texw = rows 1 and ny-1 of u
+// (k_tex_get), inverse z transform; mode 4: forward z transform, the ghost
+// rows of u += texw (k_tex_put 1).  The arithmetic per value is that of the
+// separate kernels, in the same order.
+__host__ __device__ inline int tex_z_tile(int N2) {
+  const int t = 512 / N2;
+  return t < 1 ? 1 : t;
+}
+__global__ void __launch_bounds__(256) k_tex_z(char *arena, const Chan *chans, int kRK, int mode, int it) {
+  const Chan ch = chans[blockIdx.z];
+  if (!ch.tex || ch.tex1d) return;
+  const View v(arena, ch);
+  const int f = blockIdx.y, N2 = ch.N2, nxh = ch.nxh, nc = ch.ncol, igal = ch.N1 + 2;
+  const int n = ch.nyu, ldc = ch.ldc;
+  const int TI = tex_z_tile(N2), i0 = blockIdx.x * TI;
+  if (i0 >= nxh) return;
+  const int ni = min(TI, nxh - i0), nseq = 2 * ni;  // sequence q = w ni + ii
+  extern __shared__ double2 smem_d[];
+  double2 *A = smem_d, *B = A + 2 * TI * N2, *TW = B + 2 * TI * N2;
+  stage_twiddles<double2>(TW, reinterpret_cast<const double2 *>(arena + ch.o_twzN), N2);
+  double *P[2] = {reinterpret_cast<double *>(v.texw + (size_t)(f * 2) * nc),
+                  reinterpret_cast<double *>(v.texw + (size_t)(f * 2 + 1) * nc)};
+  double2 *R = A;
+  if (mode == 0 || mode == 3) {  // Fourier-space values, then the inverse transform
+    const double2 *src = f == 0 ? v.u1 : v.u3;
+    for (int t = threadIdx.x; t < nseq * N2; t += blockDim.x) {
+      const int q = t / N2, k = t - q * N2, w = q / ni, ii = q - w * ni;
+      const int c = k * nxh + i0 + ii;
+      A[q * N2 + k] = mode == 0 ? v.texu[(size_t)(f * 2 + w) * nc + c] : src[(size_t)(w == 0 ? 1 : n - 2) * ldc + c];
+    }
+    __syncthreads();
+  } else {  // physical-z values, the forward transform
+    for (int t = threadIdx.x; t < nseq * N2; t += blockDim.x) {
+      const int q = t / N2, z = t - q * N2, w = q / ni, ii = q - w * ni;
+      const double *pz = P[w] + (size_t)z * igal + 2 * (i0 + ii);
+      A[q * N2 + z] = make_double2(pz[0], pz[1]);
+    }
+    __syncthreads();
+    R = fft_lds<double2>(A, B, N2, nseq, N2, -1, TW);
+    const double sc = 1.0 / N2;
+    const size_t tab = (size_t)n * ldc;
+    const double *Wb = v.wtab + (size_t)((kRK - 1) * 2) * tab, *Wt = Wb + tab;
+    for (int t = threadIdx.x; t < ni * N2; t += blockDim.x) {
+      const int ii = t / N2, k = t - ii * N2, c = k * nxh + i0 + ii;
+      const double2 yb = R[ii * N2 + k], yt = R[(ni + ii) * N2 + k];
+      const double2 sb = make_double2(yb.x * sc, yb.y * sc), st = make_double2(yt.x * sc, yt.y * sc);
+      if (mode == 1) {
+        const double gbb = Wb[c], gbt = Wb[(size_t)(n - 1) * ldc + c];
+        const double gtb = Wt[c], gtt = Wt[(size_t)(n - 1) * ldc + c];
+        const double gbb0 = Wb[0], gtt0 = Wt[(size_t)(n - 1) * ldc];
+        const double2 Ub = v.texu[(size_t)(f * 2) * nc + c], Ut = v.texu[(size_t)(f * 2 + 1) * nc + c];
+        const double db = gbb - gbb0, dt_ = gtt - gtt0;
+        R[ii * N2 + k] = make_double2(Ub.x + db * sb.x + gbt * st.x, Ub.y + db * sb.y + gbt * st.y);
+        R[(ni + ii) * N2 + k] = make_double2(Ut.x + gtb * sb.x + dt_ * st.x, Ut.y + gtb * sb.y + dt_ * st.y);
+      } else {
+        double2 *dst = mode == 2 ? (f == 0 ? v.r1 : v.r3) : (f == 0 ? v.u1 : v.u3);
+        const size_t ot = (size_t)(n - 1) * ldc + c;
+        if (mode == 2) {
+          if (ch.tex & 1) dst[c] = sb;
+          if (ch.tex & 2) dst[ot] = st;
+        } else {
+          if (ch.tex & 1) dst[c] = cadd(dst[c], sb);
+          if (ch.tex & 2) dst[ot] = cadd(dst[ot], st);
+        }
+      }
+    }
+    if (mode != 1) return;
+    __syncthreads();
   }
-  return hipSuccess;
+  // inverse z transform to the physical-z planes
+  double2 *X = fft_lds<double2>(R, R == A ? B : A, N2, nseq, N2, +1, TW);
+  for (int t = threadIdx.x; t < nseq * N2; t += blockDim.x) {
+    const int q = t / N2, z = t - q * N2, w = q / ni, ii = q - w * ni;
+    double *pz = P[w] + (size_t)z * igal + 2 * (i0 + ii);
+    const double2 x = X[q * N2 + z];
+    pz[0] = x.x;
+    pz[1] = x.y;
+  }
+}
+static hipError_t tex_z(char *arena, const Chan *d_chans, const ShapeMax &m, const std::vector<TexGroup> &groups,
+                        int kRK, int mode, int it, hipStream_t s) {
+  int gx = 1;
+  size_t lds = 0;
+  for (const TexGroup &g : groups) {
+    const int TI = tex_z_tile(g.N2);
+    gx = std::max(gx, (g.N1 / 2 + 1 + TI - 1) / TI);
+    lds = std::max(lds, (size_t)(4 * TI + 1) * g.N2 * sizeof(double2));
+  }
+  hipLaunchKernelGGL(k_tex_z, dim3(gx, 2, m.nchan), dim3(256), lds, s, arena, d_chans, kRK, mode, it);
+  return hipGetLastError();
 }
 
 // x transforms and mask of every textured channel's four wall planes
@@ -353,30 +440,32 @@ static hipError_t tex_xmask(char *arena, const Chan *d_chans, const ShapeMax &m,
   return hipGetLastError();
 }
 
+// nit passes: z inverse with the first update (k_tex_z 0), then per pass
+// the x transforms and mask (k_tex_xmask) and either the next pass's update
+// between the z transforms (k_tex_z 1) or, after the last, the sources into
+// r (k_tex_z 2)
 hipError_t launch_tex_sources(char *arena, const Chan *d_chans, const ShapeMax &m,
                               const std::vector<TexGroup> &groups, int kRK, int nit, hipStream_t s) {
-  const dim3 b(256), gc((m.ncol + 255) / 256, 1, m.nchan);
-  for (int it = 0; it < nit; it++) {
-    hipLaunchKernelGGL(k_tex_pre, gc, b, 0, s, arena, d_chans, kRK, it);
-    hipError_t e = tex_ztransforms(arena, groups, +1, s);
-    if (e == hipSuccess) e = tex_xmask(arena, d_chans, m, groups, kRK, 0, s);
-    if (e == hipSuccess) e = tex_ztransforms(arena, groups, -1, s);
-    if (e != hipSuccess) return e;
+  if (nit < 1) {  // no pass: the sources are U (as k_tex_pre 0 then k_tex_put 0)
+    const dim3 b(256), gc((m.ncol + 255) / 256, 1, m.nchan);
+    hipLaunchKernelGGL(k_tex_pre, gc, b, 0, s, arena, d_chans, kRK, 0);
+    hipLaunchKernelGGL(k_tex_put, gc, b, 0, s, arena, d_chans, 0);
+    return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_tex_put, gc, b, 0, s, arena, d_chans, 0);
-  return hipGetLastError();
+  hipError_t e = tex_z(arena, d_chans, m, groups, kRK, 0, 0, s);
+  for (int it = 0; it < nit && e == hipSuccess; it++) {
+    e = tex_xmask(arena, d_chans, m, groups, kRK, 0, s);
+    if (e == hipSuccess) e = tex_z(arena, d_chans, m, groups, kRK, it + 1 < nit ? 1 : 2, it + 1, s);
+  }
+  return e;
 }
 
 hipError_t launch_tex_ghosts(char *arena, const Chan *d_chans, const ShapeMax &m,
                              const std::vector<TexGroup> &groups, hipStream_t s) {
-  const dim3 b(256), gc((m.ncol + 255) / 256, 1, m.nchan);
-  hipLaunchKernelGGL(k_tex_get, gc, b, 0, s, arena, d_chans);
-  hipError_t e = tex_ztransforms(arena, groups, +1, s);
+  hipError_t e = tex_z(arena, d_chans, m, groups, 1, 3, 0, s);
   if (e == hipSuccess) e = tex_xmask(arena, d_chans, m, groups, 1, 1, s);
-  if (e == hipSuccess) e = tex_ztransforms(arena, groups, -1, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_tex_put, gc, b, 0, s, arena, d_chans, 1);
-  return hipGetLastError();
+  if (e == hipSuccess) e = tex_z(arena, d_chans, m, groups, 1, 4, 0, s);
+  return e;
 }
 
 // ------------------------------------------------------------------------
