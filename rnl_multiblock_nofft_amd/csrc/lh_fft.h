@@ -341,20 +341,22 @@ __device__ __forceinline__ C *fft_ct_run(C *x, C *y, int nb, int sstr, const C *
 // twiddle reads are broadcasts (one address per lane group).
 
 // LDS stride (elements) of nb sequences of length n with elements of es
-// bytes: the smallest S >= n that is an odd multiple of rho = G / p, p the
-// largest power of two <= min(nb, G); then b S mod G, b = 0 .. p-1, are
-// distinct multiples of rho and p consecutive sequences fill the G lanes'
-// banks without overlap.  Exception, measured: four sequences of 16-byte
-// elements take rho = 2, not 4 (the x-product's tiles of two z rows at 384
-// points: its stages' writes, whose lanes run 8 or more elements apart
-// within a sequence, conflict less; k_xprod 0.93 -> 0.90 ms at 256 x 256 x
-// 257, where rho = 1 gave 0.94 ms and rho = 1 at eight sequences nothing)
+// bytes: the smallest S >= n that is an odd multiple of rho, p the largest
+// power of two <= min(nb, G).  rho = G / p would put the p sequences' same
+// item on p lane groups G / p banks apart, filling the G lanes' banks; but
+// when nb < G a group of G lanes also holds neighbouring items of the same
+// sequences, and the measured best is half that spacing, rho = G / (2 p)
+// (1 once p = G): then b S mod G, b = 0 .. p-1, are distinct multiples of
+// rho, every other one of the 2 p such slots.  At 256 x 256 x 257: k_xprod 0.93 -> 0.89 ms (FP64, four
+// sequences) and 0.57 -> 0.48 ms (FP32, four), z stages of eight FP32
+// sequences 4% faster, FP64 eight-sequence stages unchanged; rho = G / (4 p)
+// for four FP64 sequences was slower (0.94 ms).
 __host__ __device__ inline int seq_stride(int n, int nb, int es) {
   const int G = 256 / es;
   int p = 1;
   while (2 * p <= nb && 2 * p <= G) p *= 2;
   if (p == 1) return n;
-  const int rho = nb == 4 && es == 16 ? 2 : G / p;
+  const int rho = G / p > 1 ? G / p / 2 : 1;
   int S = (n + rho - 1) / rho * rho;
   if ((S / rho) % 2 == 0) S += rho;
   return S;
